@@ -1,0 +1,155 @@
+/*
+ * tbg.h -- C ABI of the MI355X king-slice retrograde engine (libtbg.so).
+ *
+ * Plain C types only.  The library owns all device memory; host buffers
+ * passed to the get/put calls belong to the caller.  Every call returns an
+ * int status: TBG_OK (0) or a negative TBG_E_* code (tbg_strerror).
+ *
+ * The reference (syzygy1/tb8gen) has no library boundary: its hot path sits
+ * behind three internal interfaces that these entry points replace:
+ *   - the worker interface run_threaded(func, work) over [begin, end)
+ *     index units (src/threads.h:28-53, src/threads.c:266-289), used by
+ *     predecessors / predecessors_sub / check_successors / calc_sub /
+ *     calc_illegal / calc_mate (src/tb8gen.c:280-768);
+ *   - the slice store src/kslice.h:75-98 (slot-addressed bitmaps, clear /
+ *     and_not / or / count, file read/write);
+ *   - capture resolution int probe_wdl(Position*, int, int)
+ *     (src/probe.h:170, src/probe.c:1677-1684).
+ * Bitmaps use the reference's bit order: bit idx&63 of little-endian
+ * 64-bit word idx>>6 (= bit idx&7 of byte idx>>3, src/kslice.h:30-55),
+ * each slice padded to 64 bytes (src/kslice.c:92-96).
+ */
+#ifndef TBG_H
+#define TBG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TBG_OK 0
+#define TBG_E_INVAL (-1)   /* bad argument (material string, side, slice, size) */
+#define TBG_E_NOMEM (-2)   /* device or host allocation failed            */
+#define TBG_E_HIP (-3)     /* a HIP runtime call or kernel launch failed  */
+#define TBG_E_NOSUB (-4)   /* a smaller material's table is not supplied  */
+#define TBG_E_STATE (-5)   /* call out of order (e.g. step before init)   */
+#define TBG_E_BUDGET (-6)  /* job exceeds the device-memory budget        */
+#define TBG_E_NODEV (-7)   /* no HIP device                               */
+
+#define TBG_NSLICES 462
+#define TBG_DRAW_RULE 100 /* src/tb8gen.c:36 */
+
+/* Position codes of a finished table (one uint16 per side/slice/index):
+ * 0 draw, 3 illegal, (n << 2) | 1 in W_n, (n << 2) | 2 in L_n. */
+#define TBG_CODE_DRAW 0
+#define TBG_CODE_ILLEGAL 3
+
+/* which bitmap plane tbg_get_bitmap returns */
+#define TBG_PLANE_WINS 0   /* illegal + W_1..W_n so far   (0/wins/<c>)      */
+#define TBG_PLANE_W 1      /* W_n of the last completed ply (n/W/<c>)      */
+#define TBG_PLANE_L 2      /* L_n of the last completed ply (n/L/<c>)      */
+
+/* sub (capture exit) kinds, src/tb8gen.c:321-325 */
+#define TBG_SUB_LOSS 0
+#define TBG_SUB_BLOSS 1
+#define TBG_SUB_CWIN 2
+#define TBG_SUB_WIN 3
+
+/* generation forms (tbg_generate) */
+#define TBG_FORM_PLAIN 0   /* one slice per launch, reference manager order */
+#define TBG_FORM_BATCHED 1 /* one launch per phase over all 462 slices      */
+#define TBG_FORM_GRAPH 2   /* batched, each ply replayed from a hipGraph with
+                              device-side step flags                        */
+
+typedef struct tbg_ctx tbg_ctx;
+
+typedef struct tbg_info {
+  int npieces;
+  uint8_t pt[8];            /* piece codes in slot order, src/types.h:19-22 */
+  int numsets;
+  int mult[6];
+  uint64_t slice_size;      /* positions per king slice (IdxInfo.size)      */
+  uint64_t slice_words;     /* 64-bit words per slice bitmap (64 B aligned) */
+  uint64_t capt_size[6];    /* positions per slice with one piece of set k removed */
+  uint64_t sub_words[2];    /* words per slice of a colour's sub region     */
+  uint64_t sub_offset[6];   /* word offset of set k inside its colour region */
+  uint64_t device_bytes;    /* device memory held by the context            */
+  int synth_slice;          /* -1, or the single slice of a synthetic job   */
+} tbg_info;
+
+/* Per-step counts, in the order the reference prints them. */
+typedef struct tbg_counts {
+  uint64_t broken[2];       /* broken_w / broken_b  (tb8gen.c:764-765) */
+  uint64_t mates[2];        /* l0_w / l0_b          (tb8gen.c:766-767) */
+  int nplies;               /* last ply executed                       */
+} tbg_counts;
+
+/* Callback receiving each count line as the reference formats it, e.g.
+ * "l17_w = 1234" (tb8gen.c:1024, 1082). */
+typedef void (*tbg_line_fn)(const char *line, void *user);
+
+/* Library setup: selects the HIP device.  Replaces init_threads /
+ * init_movegen / init_tables (tb8gen.c:1279-1282). */
+int tbg_init(int device);
+const char *tbg_strerror(int err);
+
+/* Lays out a material (tb8gen.c:1284-1385, kslice.c:98-127) and allocates
+ * its resident device state.  material e.g. "KQRvKR". */
+int tbg_create(tbg_ctx **out, const char *material);
+/* Synthetic single-slice job: only king slice `slice` (0 <= slice < 441) of
+ * `material`; capture and king-move exits come from a seeded splitmix64
+ * hash (win below 0.03, loss in [0.03, 0.06)). */
+int tbg_create_synth(tbg_ctx **out, const char *material, int slice, uint64_t seed);
+void tbg_destroy(tbg_ctx *ctx);
+int tbg_get_info(const tbg_ctx *ctx, tbg_info *info);
+
+/* Capture exits for both colours: calc_sub_kslices (tb8gen.c:317-342)
+ * with probe_wdl answered from the finished device tables of the smaller
+ * materials in subs[] (either colour orientation).  KvK needs no table. */
+int tbg_resolve_captures(tbg_ctx *ctx, tbg_ctx *const *subs, int nsubs);
+/* Illegal positions and mates: calc_illegal_and_mate (tb8gen.c:730-768). */
+int tbg_init_illegal_mate(tbg_ctx *ctx, tbg_counts *counts);
+
+/* One half-step over all slices of the material (batched form):
+ * calc_L_n (tb8gen.c:943-1026) and calc_W_n (tb8gen.c:1028-1084).
+ * `more` is the reference's more_l / more_w flag.  *count = |L_n| / |W_n|. */
+int tbg_step_L(tbg_ctx *ctx, int stm, int n, int more, uint64_t *count);
+int tbg_step_W(tbg_ctx *ctx, int stm, int n, int more, uint64_t *count);
+/* Plain form of the same half-steps: one launch per slice and phase, in the
+ * reference's manager order (kslice.c:41-89). */
+int tbg_step_L_plain(tbg_ctx *ctx, int stm, int n, int more, uint64_t *count);
+int tbg_step_W_plain(tbg_ctx *ctx, int stm, int n, int more, uint64_t *count);
+/* Ends ply n: this ply's W_n / L_n become the next ply's frontiers. */
+int tbg_end_ply(tbg_ctx *ctx);
+
+/* The whole generator for one material after tbg_resolve_captures:
+ * illegal/mate, then the ply loop of main() (tb8gen.c:1406-1444) with its
+ * more_* flags, emitting the reference's count lines through `fn`. */
+int tbg_generate(tbg_ctx *ctx, int form, tbg_line_fn fn, void *user, tbg_counts *counts);
+
+/* Data plane (device -> host, synchronous). */
+int tbg_get_codes(tbg_ctx *ctx, int stm, int slice, uint16_t *dst, uint64_t n);
+int tbg_get_bitmap(tbg_ctx *ctx, int plane, int stm, int slice, uint64_t *dst, uint64_t nwords);
+int tbg_get_sub_bitmap(tbg_ctx *ctx, int kind, int colour, int slice, uint64_t *dst,
+                       uint64_t nwords);
+
+/* Last kernel timings of the context (HIP events on its stream). */
+typedef struct tbg_timing {
+  double ms_total;          /* device time of the last tbg_generate / steps */
+  double ms_pred, ms_verify, ms_sweep, ms_uncap, ms_init, ms_resolve;
+  uint64_t launches;
+  uint64_t half_steps;      /* half-steps executed (L or W, one side)       */
+} tbg_timing;
+int tbg_get_timing(const tbg_ctx *ctx, tbg_timing *t);
+int tbg_set_profiling(tbg_ctx *ctx, int on);
+
+/* Self-test of the index, symmetry and attack primitives compiled for the
+ * host (no device needed).  0 on success; msg receives a description. */
+int tbg_host_selftest(char *msg, int len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

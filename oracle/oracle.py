@@ -1,0 +1,174 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+Test infrastructure only: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the product package.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+
+DRAW, ILLEGAL = 0, 3
+
+
+def build():
+    """Compile the oracle library and CLI (gcc + OpenMP)."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = ctypes.CDLL(LIB)
+        u8p = ctypes.POINTER(ctypes.c_uint8)
+        L.tbo_generate.argtypes = [ctypes.c_char_p, ctypes.c_char_p]
+        L.tbo_find.argtypes = [ctypes.c_char_p]
+        L.tbo_slice_size.restype = ctypes.c_uint64
+        L.tbo_codes.restype = ctypes.POINTER(ctypes.c_uint16)
+        L.tbo_codes.argtypes = [ctypes.c_int, ctypes.c_int]
+        L.tbo_count.restype = ctypes.c_int64
+        L.tbo_broken.restype = ctypes.c_int64
+        L.tbo_mates.restype = ctypes.c_int64
+        L.tbo_sq_to_idx.restype = ctypes.c_uint64
+        L.tbo_sq_to_idx.argtypes = [ctypes.c_int, u8p]
+        L.tbo_idx_to_sq.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_uint64, u8p]
+        L.tbo_capt_idx_to_sq.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, u8p]
+        L.tbo_capt_size.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64)]
+        L.tbo_sub_words.restype = ctypes.c_uint64
+        L.tbo_sub_offset_words.restype = ctypes.c_uint64
+        L.tbo_sub_bitmap.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                     ctypes.POINTER(ctypes.c_uint64)]
+        L.tbo_lookup_bulk.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_uint64, u8p,
+                                      ctypes.POINTER(ctypes.c_uint16)]
+        L.tbo_synth_slice.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_uint64,
+                                      ctypes.c_char_p, ctypes.c_int]
+        L.tbo_synth_hash.restype = ctypes.c_uint64
+        L.tbo_synth_hash.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_uint64]
+        _lib = L
+    return _lib
+
+
+def set_threads(n):
+    lib().tbo_set_threads(int(n))
+
+
+def set_ref_king_capture_index(on):
+    lib().tbo_set_ref_king_capture_index(1 if on else 0)
+
+
+def free_all():
+    lib().tbo_free_all()
+
+
+class Table:
+    """One generated material (and, implicitly, its smaller materials)."""
+
+    def __init__(self, material, log_path=None, synth_slice=None, seed=0, max_plies=0):
+        L = lib()
+        if synth_slice is None:
+            h = L.tbo_generate(material.encode(), log_path.encode() if log_path else None)
+        else:
+            h = L.tbo_synth_slice(material.encode(), int(synth_slice), int(seed),
+                                  log_path.encode() if log_path else None, int(max_plies))
+        if h < 0:
+            raise RuntimeError(f"oracle generation of {material} failed ({h})")
+        self.h = h
+        self.material = material
+        self.synth_slice = synth_slice
+        self.size = L.tbo_slice_size(h)
+        self.num = L.tbo_num_pieces(h)
+        pt = (ctypes.c_uint8 * 8)()
+        L.tbo_piece_types(h, pt)
+        self.pt = list(pt)[: self.num]
+        self.numsets = L.tbo_num_sets(h)
+
+    def codes(self, stm):
+        """uint16 codes [462, size] (synthetic mode: [1, size])."""
+        n = self.size * (1 if self.synth_slice is not None else 462)
+        p = lib().tbo_codes(self.h, stm)
+        a = np.ctypeslib.as_array(p, shape=(n,)).copy()
+        return a.reshape(-1, self.size)
+
+    def count(self, kind, stm, n):
+        """kind 'L' or 'W'; None if the step did not run."""
+        c = lib().tbo_count(self.h, 0 if kind == "L" else 1, stm, n)
+        return None if c < 0 else int(c)
+
+    @property
+    def nplies(self):
+        return lib().tbo_num_plies(self.h)
+
+    def broken(self, stm):
+        return int(lib().tbo_broken(self.h, stm))
+
+    def mates(self, stm):
+        return int(lib().tbo_mates(self.h, stm))
+
+    def capt_size(self, k):
+        v = ctypes.c_uint64()
+        lib().tbo_capt_size(self.h, k, ctypes.byref(v))
+        return v.value
+
+    def idx_to_sq(self, s, idx):
+        out = (ctypes.c_uint8 * 8)()
+        if lib().tbo_idx_to_sq(self.h, s, idx, out):
+            raise ValueError("index out of range")
+        return list(out)[: self.num]
+
+    def capt_idx_to_sq(self, k, s, idx):
+        out = (ctypes.c_uint8 * 8)()
+        if lib().tbo_capt_idx_to_sq(self.h, k, s, idx, out):
+            raise ValueError("index out of range")
+        return list(out)[: self.num]
+
+    def sq_to_idx(self, sq):
+        a = (ctypes.c_uint8 * 8)(*sq)
+        return int(lib().tbo_sq_to_idx(self.h, a))
+
+    def sub_words(self, colour):
+        return int(lib().tbo_sub_words(self.h, colour))
+
+    def sub_bitmap(self, kind, colour, s):
+        w = self.sub_words(colour)
+        out = np.zeros(w, dtype=np.uint64)
+        lib().tbo_sub_bitmap(self.h, kind, colour, s,
+                             out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)))
+        return out
+
+    def lookup_bulk(self, stm, squares):
+        """squares: uint8 array [n, num] in slot order -> uint16 codes
+        (0xffff for placements with two pieces on one square)."""
+        sq = np.ascontiguousarray(squares, dtype=np.uint8)
+        n = sq.shape[0]
+        out = np.zeros(n, dtype=np.uint16)
+        lib().tbo_lookup_bulk(self.h, stm, n, sq.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                              out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)))
+        return out
+
+
+def kkidx(tri, bk):
+    return lib().tbo_kkidx(tri, bk)
+
+
+def kkmap(wk, bk):
+    return lib().tbo_kkmap(wk, bk)
+
+
+def kksquare(s):
+    out = (ctypes.c_uint8 * 2)()
+    lib().tbo_kksquare(s, out)
+    return out[0], out[1]
+
+
+def synth_hash(seed, stm, kind, region, idx):
+    return int(lib().tbo_synth_hash(seed, stm, kind, region, idx))

@@ -1,0 +1,258 @@
+"""tb8gen_amd -- MI355X king-slice retrograde engine (Python side of the C ABI).
+
+The engine is libtbg.so (HIP kernels + the C ABI declared in include/tbg.h);
+this module binds it with ctypes and mirrors the reference generator's
+driver (syzygy1/tb8gen src/tb8gen.c:1245-1453): a material is generated
+after every smaller material its captures reach, and the per-ply count
+lines come out in the reference's format.  There is no CPU fallback: if the
+library or a HIP device is missing, every call raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+LIB_PATH = os.path.join(PKG, "libtbg.so")
+DRIVER_PATH = os.path.join(PKG, "tb8gen")
+
+FORM_PLAIN, FORM_BATCHED, FORM_GRAPH = 0, 1, 2
+PLANE_WINS, PLANE_W, PLANE_L = 0, 1, 2
+SUB_LOSS, SUB_BLOSS, SUB_CWIN, SUB_WIN = 0, 1, 2, 3
+NSLICES = 462
+DRAW_RULE = 100
+
+# the exported entry points of include/tbg.h
+ABI_SYMBOLS = [
+    "tbg_init", "tbg_strerror", "tbg_create", "tbg_create_synth", "tbg_destroy",
+    "tbg_get_info", "tbg_resolve_captures", "tbg_init_illegal_mate", "tbg_step_L",
+    "tbg_step_W", "tbg_step_L_plain", "tbg_step_W_plain", "tbg_end_ply", "tbg_generate",
+    "tbg_get_codes", "tbg_get_bitmap", "tbg_get_sub_bitmap", "tbg_get_timing",
+    "tbg_set_profiling", "tbg_host_selftest",
+]
+
+
+class TbgError(RuntimeError):
+    pass
+
+
+class Info(ctypes.Structure):
+    _fields_ = [("npieces", ctypes.c_int), ("pt", ctypes.c_uint8 * 8), ("numsets", ctypes.c_int),
+                ("mult", ctypes.c_int * 6), ("slice_size", ctypes.c_uint64),
+                ("slice_words", ctypes.c_uint64), ("capt_size", ctypes.c_uint64 * 6),
+                ("sub_words", ctypes.c_uint64 * 2), ("sub_offset", ctypes.c_uint64 * 6),
+                ("device_bytes", ctypes.c_uint64), ("synth_slice", ctypes.c_int)]
+
+
+class Counts(ctypes.Structure):
+    _fields_ = [("broken", ctypes.c_uint64 * 2), ("mates", ctypes.c_uint64 * 2),
+                ("nplies", ctypes.c_int)]
+
+
+class Timing(ctypes.Structure):
+    _fields_ = [("ms_total", ctypes.c_double), ("ms_pred", ctypes.c_double),
+                ("ms_verify", ctypes.c_double), ("ms_sweep", ctypes.c_double),
+                ("ms_uncap", ctypes.c_double), ("ms_init", ctypes.c_double),
+                ("ms_resolve", ctypes.c_double), ("launches", ctypes.c_uint64),
+                ("half_steps", ctypes.c_uint64)]
+
+
+LINE_FN = ctypes.CFUNCTYPE(None, ctypes.c_char_p, ctypes.c_void_p)
+
+_lib = None
+_inited = False
+
+
+def lib():
+    """Load libtbg.so (built in-tree by tb8gen_amd.build); raises if absent."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise TbgError(f"{LIB_PATH} is not built (run python -m tb8gen_amd.build)")
+        L = ctypes.CDLL(LIB_PATH)
+        vp = ctypes.c_void_p
+        L.tbg_strerror.restype = ctypes.c_char_p
+        L.tbg_create.argtypes = [ctypes.POINTER(vp), ctypes.c_char_p]
+        L.tbg_create_synth.argtypes = [ctypes.POINTER(vp), ctypes.c_char_p, ctypes.c_int,
+                                       ctypes.c_uint64]
+        L.tbg_destroy.argtypes = [vp]
+        L.tbg_destroy.restype = None
+        L.tbg_get_info.argtypes = [vp, ctypes.POINTER(Info)]
+        L.tbg_resolve_captures.argtypes = [vp, ctypes.POINTER(vp), ctypes.c_int]
+        L.tbg_init_illegal_mate.argtypes = [vp, ctypes.POINTER(Counts)]
+        for f in ("tbg_step_L", "tbg_step_W", "tbg_step_L_plain", "tbg_step_W_plain"):
+            getattr(L, f).argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                      ctypes.POINTER(ctypes.c_uint64)]
+        L.tbg_end_ply.argtypes = [vp]
+        L.tbg_generate.argtypes = [vp, ctypes.c_int, LINE_FN, vp, ctypes.POINTER(Counts)]
+        L.tbg_get_codes.argtypes = [vp, ctypes.c_int, ctypes.c_int,
+                                    ctypes.POINTER(ctypes.c_uint16), ctypes.c_uint64]
+        L.tbg_get_bitmap.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                     ctypes.POINTER(ctypes.c_uint64), ctypes.c_uint64]
+        L.tbg_get_sub_bitmap.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                         ctypes.POINTER(ctypes.c_uint64), ctypes.c_uint64]
+        L.tbg_get_timing.argtypes = [vp, ctypes.POINTER(Timing)]
+        L.tbg_set_profiling.argtypes = [vp, ctypes.c_int]
+        L.tbg_host_selftest.argtypes = [ctypes.c_char_p, ctypes.c_int]
+        _lib = L
+    return _lib
+
+
+def check(r, what=""):
+    if r != 0:
+        msg = lib().tbg_strerror(r).decode()
+        raise TbgError(f"{what}: {msg} ({r})")
+
+
+def init(device=0):
+    global _inited
+    if not _inited:
+        check(lib().tbg_init(device), "tbg_init")
+        _inited = True
+
+
+def host_selftest():
+    buf = ctypes.create_string_buffer(256)
+    r = lib().tbg_host_selftest(buf, 256)
+    return r, buf.value.decode()
+
+
+class Table:
+    """Device-resident state of one material (a tbg_ctx)."""
+
+    def __init__(self, material, synth_slice=None, seed=0):
+        init()
+        self.material = material
+        self.ptr = ctypes.c_void_p()
+        if synth_slice is None:
+            check(lib().tbg_create(ctypes.byref(self.ptr), material.encode()), material)
+        else:
+            check(lib().tbg_create_synth(ctypes.byref(self.ptr), material.encode(),
+                                         int(synth_slice), int(seed)), material)
+        self.synth_slice = synth_slice
+        self.lines = []
+        self.counts = Counts()
+        info = Info()
+        check(lib().tbg_get_info(self.ptr, ctypes.byref(info)), "info")
+        self.info = info
+        self.size = info.slice_size
+        self.words = info.slice_words
+
+    def close(self):
+        if self.ptr:
+            lib().tbg_destroy(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def resolve(self, subs=()):
+        arr = (ctypes.c_void_p * max(1, len(subs)))(*[s.ptr.value for s in subs])
+        check(lib().tbg_resolve_captures(self.ptr, arr, len(subs)), f"resolve {self.material}")
+
+    def generate(self, form=FORM_BATCHED):
+        self.lines = []
+
+        def cb(line, _user):
+            self.lines.append(line.decode())
+
+        fn = LINE_FN(cb)
+        check(lib().tbg_generate(self.ptr, form, fn, None, ctypes.byref(self.counts)),
+              f"generate {self.material}")
+        return self.lines
+
+    def codes(self, stm, slice_=None):
+        """uint16 codes [462, size] (or one slice)."""
+        slices = [slice_] if slice_ is not None else (
+            [self.synth_slice] if self.synth_slice is not None else range(NSLICES))
+        out = np.zeros((len(slices), self.size), dtype=np.uint16)
+        for i, s in enumerate(slices):
+            check(lib().tbg_get_codes(self.ptr, stm, s,
+                                      out[i].ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)),
+                                      self.size), "codes")
+        return out
+
+    def bitmap(self, plane, stm, s):
+        out = np.zeros(self.words, dtype=np.uint64)
+        check(lib().tbg_get_bitmap(self.ptr, plane, stm, s,
+                                   out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                   self.words), "bitmap")
+        return out
+
+    def sub_bitmap(self, kind, colour, s):
+        w = self.info.sub_words[colour]
+        out = np.zeros(w, dtype=np.uint64)
+        check(lib().tbg_get_sub_bitmap(self.ptr, kind, colour, s,
+                                       out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), w),
+              "sub bitmap")
+        return out
+
+    def set_profiling(self, on):
+        check(lib().tbg_set_profiling(self.ptr, 1 if on else 0), "profiling")
+
+    def timing(self):
+        t = Timing()
+        check(lib().tbg_get_timing(self.ptr, ctypes.byref(t)), "timing")
+        return t
+
+
+_ORDER = "KQRBN"
+
+
+def _side_key(side):
+    return (len(side), [-_ORDER.index(ch) for ch in side])
+
+
+def canonical(material):
+    """Colour orientation used for generated smaller materials: the side with
+    more pieces, then the stronger piece string, is white."""
+    w, b = material.split("v")
+    w = "".join(sorted(w, key=_ORDER.index))
+    b = "".join(sorted(b, key=_ORDER.index))
+    return f"{w}v{b}" if _side_key(w) >= _side_key(b) else f"{b}v{w}"
+
+
+def sub_materials(material):
+    """Materials one capture away (excluding a bare pair of kings)."""
+    w, b = material.split("v")
+    out = []
+    for side, other, white in ((w, b, True), (b, w, False)):
+        for i, ch in enumerate(side):
+            if ch == "K":
+                continue
+            rest = side[:i] + side[i + 1:]
+            m = f"{rest}v{other}" if white else f"{other}v{rest}"
+            if len(m) - 1 > 2:
+                c = canonical(m)
+                if c not in out:
+                    out.append(c)
+    return out
+
+
+class Registry:
+    """Generates materials in dependency order and keeps their tables
+    resident (the engine's replacement for Syzygy files)."""
+
+    def __init__(self, form=FORM_BATCHED):
+        self.form = form
+        self.tables = {}
+
+    def get(self, material, form=None):
+        if material in self.tables:
+            return self.tables[material]
+        for sub in sub_materials(material):
+            self.get(sub, form)
+        t = Table(material)
+        t.resolve(list(self.tables.values()))
+        t.generate(self.form if form is None else form)
+        self.tables[material] = t
+        return t
+
+    def close(self):
+        for t in self.tables.values():
+            t.close()
+        self.tables = {}

@@ -1,0 +1,321 @@
+// tbg_dev.h -- position, index and move primitives of the king-slice engine.
+//
+// Everything here is __host__ __device__ so the host build of the library
+// can self-test it against brute force (tbg_host_selftest) before a kernel
+// ever runs.  A position keeps its (up to) 8 piece squares packed one per
+// byte in a uint64_t ("sqp", slot i in byte i, slot order of the reference:
+// WK, BK, then the sets, src/tb8gen.c:1331-1367).  Packed squares keep every
+// slot access a shift, so per-lane state never spills to scratch memory,
+// and the board symmetries become byte-parallel bit operations.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define TBG_HD __host__ __device__ __forceinline__
+
+namespace tbg {
+
+constexpr int MAXP = 8;      // pieces incl. kings (src/defs.h:11)
+constexpr int MAXS = 6;      // sets of like non-king pieces (src/defs.h:12)
+constexpr int NSL = 462;     // king slices
+constexpr int DRAW_RULE = 100;
+
+enum { WHITE = 0, BLACK = 1 };
+enum { KNIGHT = 2, BISHOP = 3, ROOK = 4, QUEEN = 5, KING = 6 };
+
+// Index layout of one material or capture space (src/tb8gen.h:4-13).
+struct Idx {
+  int numsets;
+  int first[MAXS], mult[MAXS], last[MAXS];
+  uint64_t size;
+  uint64_t factor[MAXS];
+  double inv[MAXS];          // 1.0 / factor[k], for digit extraction
+};
+
+// Everything a kernel needs to know about a material.
+struct Mat {
+  int num, numsets;
+  int pt[MAXP];              // piece code per slot (colour bit 8)
+  int nk[2];                 // number of non-king pieces per colour
+  int pcs[2][MAXP];          // their slots
+  int setcol[MAXS];          // colour of each set
+  uint64_t set_of;           // packed: set index of slot i in byte i (0xff for kings)
+  uint64_t setmask80[MAXS];  // 0x80 in the byte of each slot of set k
+  Idx ii, capt[MAXS];
+  uint64_t words;            // 64-bit words per slice bitmap
+  uint64_t cwords[MAXS], coff[MAXS], ctotal[2];
+  uint64_t nslices_phys;     // 462, or 1 for a synthetic single-slice job
+  int synth, synth_slice;
+  uint64_t seed;
+};
+
+// Tables shared by all kernels (copied to LDS per work-group).
+struct Tabs {
+  uint64_t king[64];
+  uint64_t knight[64];
+  uint32_t binom[8 * 64];    // binom[k * 64 + n] = C(n, k)
+  int16_t kkidx[10 * 64];    // slice numbering (src/probe.c:441-522)
+  int8_t tri[64];            // white-king triangle square -> row of kkidx
+  uint8_t kksq[NSL][2];      // slice -> (wk, bk)
+};
+
+// ---------------------------------------------------------------- bits
+TBG_HD uint64_t bit(int sq) { return 1ull << sq; }
+TBG_HD int ctz64(uint64_t x) { return __builtin_ctzll(x); }
+TBG_HD int popc64(uint64_t x) { return __builtin_popcountll(x); }
+TBG_HD int popc32(uint32_t x) { return __builtin_popcount(x); }
+TBG_HD uint64_t brev64(uint64_t x) { return __builtin_bitreverse64(x); }
+
+TBG_HD int sq_get(uint64_t p, int i) { return (int)((p >> (i << 3)) & 0xff); }
+TBG_HD uint64_t sq_set(uint64_t p, int i, int v)
+{
+  int sh = i << 3;
+  return (p & ~(0xffull << sh)) | ((uint64_t)v << sh);
+}
+TBG_HD uint64_t bytes_mask(int n) { return n >= 8 ? ~0ull : (1ull << (8 * n)) - 1; }
+
+// Slot holding square `to` among the slots selected by byte mask `valid`
+// (0x80 in each candidate byte); -1 if none.
+TBG_HD int slot_of(uint64_t p, int to, uint64_t valid80)
+{
+  const uint64_t lo7 = 0x7f7f7f7f7f7f7f7full;
+  uint64_t x = p ^ (0x0101010101010101ull * (uint64_t)to);
+  // per-byte "x != 0" without carries between bytes
+  uint64_t nz = ((x & lo7) + lo7) | x;
+  uint64_t z = ~nz & valid80;
+  return z ? (ctz64(z) >> 3) : -1;
+}
+
+// ------------------------------------------------------------- attacks
+// Line attacks by the o^(o-2r) identity in both directions (bit reversal
+// turns the negative direction into a positive one); mask excludes the
+// slider's own square.
+TBG_HD uint64_t line_att(uint64_t occ, uint64_t r, uint64_t mask)
+{
+  uint64_t o = occ & mask;
+  uint64_t f = o - r;
+  uint64_t b = brev64(o) - brev64(r);
+  return (f ^ brev64(b)) & mask;
+}
+TBG_HD uint64_t rook_att(int sq, uint64_t occ)
+{
+  uint64_t r = bit(sq);
+  uint64_t file = (0x0101010101010101ull << (sq & 7)) ^ r;
+  uint64_t rank = (0xffull << (sq & 56)) ^ r;
+  return line_att(occ, r, file) | line_att(occ, r, rank);
+}
+TBG_HD uint64_t bishop_att(int sq, uint64_t occ)
+{
+  uint64_t r = bit(sq);
+  int d = (sq & 7) - (sq >> 3);
+  uint64_t diag = d >= 0 ? 0x8040201008040201ull >> (8 * d) : 0x8040201008040201ull << (-8 * d);
+  int e = (sq & 7) + (sq >> 3) - 7;
+  uint64_t anti = e >= 0 ? 0x0102040810204080ull << (8 * e) : 0x0102040810204080ull >> (-8 * e);
+  return line_att(occ, r, diag ^ r) | line_att(occ, r, anti ^ r);
+}
+// piece_attacks / non_king_piece_attacks (src/movegen.h:79-117)
+TBG_HD uint64_t piece_att(const Tabs &T, int pt, int sq, uint64_t occ)
+{
+  switch (pt & 7) {
+  case KNIGHT: return T.knight[sq];
+  case BISHOP: return bishop_att(sq, occ);
+  case ROOK: return rook_att(sq, occ);
+  case QUEEN: return rook_att(sq, occ) | bishop_att(sq, occ);
+  default: return T.king[sq];
+  }
+}
+
+// ------------------------------------------------------------- symmetry
+// MirrorMask (src/tb8gen.c:75-76) and FlipTest (src/tb8gen.c:78-86)
+TBG_HD int mirror_mask(int wk) { return ((wk & 4) ? 7 : 0) | ((wk & 32) ? 0x38 : 0); }
+TBG_HD bool flip_test(int wk, int bk)
+{
+  const uint64_t A1D1D4 = 0x080c0e0full, A1D4 = 0x08040201ull, LOWER = 0x80c0e0f0f8fcfeffull;
+  int m = mirror_mask(wk);
+  int s1 = wk ^ m, s2 = bk ^ m;
+  return !((A1D1D4 >> s1) & 1) || (((A1D4 >> s1) & 1) && !((LOWER >> s2) & 1));
+}
+// FlipDiag of every byte
+TBG_HD uint64_t flip_diag_bytes(uint64_t p)
+{
+  return ((p & 0x0707070707070707ull) << 3) | ((p >> 3) & 0x0707070707070707ull);
+}
+// normalize (src/tb8gen.c:230-238) of packed squares; returns the slice
+// KKMap[wk][bk] (-1 for kings that touch).
+TBG_HD int normalize(const Tabs &T, uint64_t p, uint64_t *out)
+{
+  int wk = sq_get(p, 0), bk = sq_get(p, 1);
+  int m = mirror_mask(wk);
+  uint64_t q = p ^ (0x0101010101010101ull * (uint64_t)m);
+  if (flip_test(wk, bk)) q = flip_diag_bytes(q);
+  *out = q;
+  return T.kkidx[T.tri[sq_get(q, 0)] * 64 + sq_get(q, 1)];
+}
+// mirror_diagonal of the non-king slots (src/tb8gen.c:240-244)
+TBG_HD uint64_t mirror_nonkings(uint64_t p)
+{
+  uint64_t f = flip_diag_bytes(p);
+  return (p & 0xffffull) | (f & ~0xffffull);
+}
+
+// ---------------------------------------------------------------- index
+// r-th (0-based) zero bit of occ (the _pdep_u64 of src/tb8gen.c:117,121)
+TBG_HD int select_zero(uint64_t occ, int r)
+{
+  uint64_t f = ~occ;
+  uint32_t x = (uint32_t)f;
+  int base = 0, c = popc32(x);
+  if (r >= c) { r -= c; x = (uint32_t)(f >> 32); base = 32; }
+  c = popc32(x & 0xffffu); if (r >= c) { r -= c; x >>= 16; base += 16; }
+  c = popc32(x & 0xffu);   if (r >= c) { r -= c; x >>= 8;  base += 8; }
+  c = popc32(x & 0xfu);    if (r >= c) { r -= c; x >>= 4;  base += 4; }
+  c = popc32(x & 0x3u);    if (r >= c) { r -= c; x >>= 2;  base += 2; }
+  c = (int)(x & 1u);       if (r >= c) { base += 1; }
+  return base;
+}
+
+// floor(n / d) for n < 2^53 via one double multiply and a +-1 fix-up
+TBG_HD uint64_t div_fix(uint64_t n, uint64_t d, double inv, uint64_t *rem)
+{
+  uint64_t q = (uint64_t)((double)n * inv);
+  int64_t r = (int64_t)(n - q * d);
+  if (r < 0) { q--; r += (int64_t)d; }
+  else if ((uint64_t)r >= d) { q++; r -= (int64_t)d; }
+  *rem = (uint64_t)r;
+  return q;
+}
+
+// sq_to_idx_helper (src/tb8gen.c:129-149).  Within a set the members need
+// not be sorted: member a's place in sorted order is the number of members
+// on lower squares.
+TBG_HD uint64_t sq_to_idx(const Tabs &T, const Idx &ii, uint64_t p)
+{
+  uint64_t occ = bit(sq_get(p, 0)) | bit(sq_get(p, 1));
+  uint64_t idx = 0;
+  for (int k = 0; k < MAXS; k++) {
+    if (k >= ii.numsets) break;
+    int f = ii.first[k], m = ii.mult[k];
+    if (m == 1) {
+      int s = sq_get(p, f);
+      uint32_t rank = (uint32_t)(s - popc64(occ & (bit(s) - 1)));
+      idx += (uint64_t)rank * ii.factor[k];
+      occ |= bit(s);
+    } else {
+      uint64_t setb = 0;
+      for (int a = 0; a < MAXS; a++)
+        if (a < m) setb |= bit(sq_get(p, f + a));
+      uint32_t d = 0;
+      for (int a = 0; a < MAXS; a++) {
+        if (a >= m) break;
+        int s = sq_get(p, f + a);
+        int rank = s - popc64(occ & (bit(s) - 1));
+        int pos = popc64(setb & (bit(s) - 1));
+        d += T.binom[(pos + 1) * 64 + rank];
+      }
+      idx += (uint64_t)d * ii.factor[k];
+      occ |= setb;
+    }
+  }
+  return idx;
+}
+
+// idx_to_sq_init + idx_to_sq_unpack + unrank_binomial (src/tb8gen.c:105-175).
+// p holds the kings in slots 0 and 1 on entry; returns the occupancy.
+TBG_HD uint64_t idx_to_sq(const Tabs &T, const Idx &ii, uint64_t idx, uint64_t *pp)
+{
+  uint64_t p = *pp;
+  uint64_t occ = bit(sq_get(p, 0)) | bit(sq_get(p, 1));
+  for (int k = 0; k < MAXS; k++) {
+    if (k >= ii.numsets) break;
+    uint64_t rem;
+    uint64_t d = ii.factor[k] == 1 ? idx : div_fix(idx, ii.factor[k], ii.inv[k], &rem);
+    idx = ii.factor[k] == 1 ? 0 : rem;
+    int f = ii.first[k], m = ii.mult[k];
+    if (m == 0) continue;  // set emptied by a capture (unrank_binomial n == 0)
+    if (m == 1) {
+      int s = select_zero(occ, (int)d);
+      p = sq_set(p, f, s);
+      occ |= bit(s);
+    } else {
+      uint64_t occ0 = occ;
+      uint32_t dd = (uint32_t)d;
+      for (int i = MAXS - 1; i > 0; i--) {
+        if (i >= m) continue;
+        // largest r with C(r, i+1) <= dd
+        int lo = i, hi = 62;
+        while (hi - lo > 1) {
+          int mid = (lo + hi) >> 1;
+          if (T.binom[(i + 1) * 64 + mid] <= dd) lo = mid; else hi = mid;
+        }
+        dd -= T.binom[(i + 1) * 64 + lo];
+        int s = select_zero(occ0, lo);
+        p = sq_set(p, f + i, s);
+        occ |= bit(s);
+      }
+      int s = select_zero(occ0, (int)dd);
+      p = sq_set(p, f, s);
+      occ |= bit(s);
+    }
+  }
+  *pp = p;
+  return occ;
+}
+
+// Occupancy of each set (bitboard of its members), for set-wise capture
+// enumeration.
+TBG_HD void set_occ(const Idx &ii, uint64_t p, uint64_t *so)
+{
+  for (int k = 0; k < MAXS; k++) {
+    uint64_t b = 0;
+    if (k < ii.numsets)
+      for (int a = 0; a < MAXS; a++)
+        if (a < ii.mult[k]) b |= bit(sq_get(p, ii.first[k] + a));
+    so[k] = b;
+  }
+}
+
+// is `target` attacked by the non-king pieces of colour c (plus the king of
+// c); slot `skip` (captured piece) ignored.  Mirrors is_attacked_by
+// (src/movegen.h:180-188) over the packed position.
+TBG_HD bool attacked_by(const Tabs &T, const Mat &M, uint64_t p, uint64_t occ, int target, int c,
+                        int skip)
+{
+  uint64_t tb = bit(target);
+  if (T.king[sq_get(p, c)] & tb) return true;
+  for (int t = 0; t < MAXP - 1; t++) {
+    if (t >= M.nk[c]) break;
+    int j = M.pcs[c][t];
+    if (j == skip) continue;
+    if (piece_att(T, M.pt[j], sq_get(p, j), occ) & tb) return true;
+  }
+  return false;
+}
+
+// splitmix64 and the seeded exit hash of synthetic jobs
+TBG_HD uint64_t splitmix64(uint64_t x)
+{
+  x += 0x9e3779b97f4a7c15ull;
+  x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
+  x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
+  return x ^ (x >> 31);
+}
+TBG_HD uint64_t synth_hash(uint64_t seed, int stm, int kind, int region, uint64_t idx)
+{
+  uint64_t key = (idx & 0xfffffffffull) | ((uint64_t)(region & 0xfff) << 36) |
+                 ((uint64_t)(kind & 3) << 48) | ((uint64_t)(stm & 1) << 50);
+  return splitmix64(seed ^ splitmix64(key));
+}
+constexpr uint64_t SYN_T_WIN = 553402322211286548ull;    // 0.03 * 2^64
+constexpr uint64_t SYN_T_LOSS = 1106804644422573097ull;  // 0.06 * 2^64
+TBG_HD int synth_value(uint64_t h) { return h < SYN_T_WIN ? 2 : h < SYN_T_LOSS ? -2 : 0; }
+
+// code <-> WDL (tb8gen.c:300-312 classes; W_n with n <= 100 is a win)
+TBG_HD int code_wdl(uint16_t c)
+{
+  int tag = c & 3, n = c >> 2;
+  if (tag == 1) return n <= DRAW_RULE ? 2 : 1;
+  if (tag == 2) return n <= DRAW_RULE ? -2 : -1;
+  return 0;
+}
+
+}  // namespace tbg

@@ -1,0 +1,955 @@
+// tbg_host.hip -- host side of the C ABI (include/tbg.h): material layout,
+// resident device planes, kernel launches for the plain / batched / graph
+// forms, and the generator's ply loop.
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <vector>
+
+#include "../../include/tbg.h"
+#include "tbg_kern.h"
+
+using namespace tbg;
+
+namespace {
+
+thread_local char g_errbuf[256];
+int g_device = -1;
+int g_cus = 256;
+Tabs g_htabs;
+
+int fail(int code, const char *fmt, ...)
+{
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_errbuf, sizeof g_errbuf, fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+#define HIPCHK(x)                                                                \
+  do {                                                                           \
+    hipError_t e_ = (x);                                                         \
+    if (e_ != hipSuccess)                                                        \
+      return fail(TBG_E_HIP, "%s: %s (%s:%d)", #x, hipGetErrorString(e_), __FILE__, \
+                  __LINE__);                                                     \
+  } while (0)
+
+// ------------------------------------------------------------ tables
+const uint8_t kInvTri[10] = { 1, 2, 3, 10, 11, 19, 0, 9, 18, 27 };  // src/probe.c:327-329
+
+bool on_diag(int s) { return (s & 7) == (s >> 3); }
+bool touching(int a, int b)
+{
+  int df = (a & 7) - (b & 7), dr = (a >> 3) - (b >> 3);
+  return df >= -1 && df <= 1 && dr >= -1 && dr <= 1;
+}
+
+void build_tabs(Tabs &T)
+{
+  memset(&T, 0, sizeof T);
+  for (int sq = 0; sq < 64; sq++) {
+    int f = sq & 7, r = sq >> 3;
+    for (int df = -2; df <= 2; df++)
+      for (int dr = -2; dr <= 2; dr++) {
+        int f2 = f + df, r2 = r + dr;
+        if (f2 < 0 || f2 > 7 || r2 < 0 || r2 > 7 || (!df && !dr)) continue;
+        int ad = df < 0 ? -df : df, ar = dr < 0 ? -dr : dr;
+        if (ad <= 1 && ar <= 1) T.king[sq] |= 1ull << (r2 * 8 + f2);
+        if ((ad == 1 && ar == 2) || (ad == 2 && ar == 1)) T.knight[sq] |= 1ull << (r2 * 8 + f2);
+      }
+  }
+  for (int n = 0; n < 64; n++) T.binom[n] = 1;
+  for (int k = 1; k < 8; k++) {
+    T.binom[k * 64] = 0;
+    for (int n = 1; n < 64; n++) {
+      uint64_t v = (uint64_t)T.binom[(k - 1) * 64 + n - 1] + T.binom[k * 64 + n - 1];
+      T.binom[k * 64 + n] = v > 0xffffffffull ? 0xffffffffu : (uint32_t)v;
+    }
+  }
+  // slice numbering: valid (wk, bk) pairs in triangle order, pairs with both
+  // kings on a1-h8 numbered last (src/probe.c:441-522)
+  int n = 0, nd = 441;
+  for (int i = 0; i < 10; i++) {
+    int wk = kInvTri[i];
+    for (int bk = 0; bk < 64; bk++) {
+      int16_t v = -1;
+      if (bk != wk && !touching(wk, bk) && !(on_diag(wk) && (bk & 7) < (bk >> 3)))
+        v = (int16_t)((on_diag(wk) && on_diag(bk)) ? nd++ : n++);
+      T.kkidx[i * 64 + bk] = v;
+      if (v >= 0) {
+        T.kksq[v][0] = (uint8_t)wk;
+        T.kksq[v][1] = (uint8_t)bk;
+      }
+    }
+  }
+  for (int i = 0; i < 64; i++) T.tri[i] = 0;
+  for (int i = 0; i < 10; i++) T.tri[kInvTri[i]] = (int8_t)i;
+}
+
+int kkmap(int wk, int bk)
+{
+  uint64_t q;
+  uint64_t p = (uint64_t)wk | ((uint64_t)bk << 8);
+  if (wk == bk || touching(wk, bk)) return -1;
+  return normalize(g_htabs, p, &q);
+}
+
+// manager order (src/kslice.c:41-84): the slices of the black-king walk;
+// white's order is its colour flip
+void manager_order(int stm, int *out)
+{
+  int s = 0;
+  for (int i = 0; i < 10; i++)
+    for (int bk = 0; bk < 64; bk++) {
+      int v = g_htabs.kkidx[i * 64 + bk];
+      if (v < 0) continue;
+      out[s++] = stm == BLACK ? v : kkmap(g_htabs.kksq[v][1], g_htabs.kksq[v][0]);
+    }
+}
+
+// ------------------------------------------------------------ material
+const char kPieceChar[] = " PNBRQK";
+const int kPieceOrder[16] = { 0, 0, 3, 5, 7, 9, 1, 0, 0, 0, 4, 6, 8, 10, 2, 0 };  // tb8gen.c:1331
+
+void calc_idx(Idx &ii)
+{
+  int n = 62;
+  uint64_t sub[MAXS];
+  for (int i = 0; i < ii.numsets; i++) {
+    sub[i] = g_htabs.binom[ii.mult[i] * 64 + n];
+    n -= ii.mult[i];
+  }
+  uint64_t f = 1;
+  for (int i = MAXS - 1; i >= 0; i--) {
+    if (i >= ii.numsets) { ii.factor[i] = 0; ii.inv[i] = 0; continue; }
+    ii.factor[i] = f;
+    ii.inv[i] = 1.0 / (double)f;
+    f *= sub[i];
+  }
+  ii.size = f;
+}
+
+int setup_mat(Mat &M, const char *name)
+{
+  memset(&M, 0, sizeof M);
+  int pt[16], k = 0, color = 0, cnt[16] = { 0 };
+  for (const char *s = name; *s; s++) {
+    if (*s == 'v' && !color) { color = 8; continue; }
+    int found = 0;
+    for (int i = KNIGHT; i <= KING; i++)
+      if (*s == kPieceChar[i]) {
+        if (k >= MAXP) return fail(TBG_E_INVAL, "too many pieces in %s", name);
+        cnt[i | color]++;
+        pt[k++] = i | color;
+        found = 1;
+      }
+    if (!found) return fail(TBG_E_INVAL, "bad piece '%c' in %s (pawnless only)", *s, name);
+  }
+  if (!color) return fail(TBG_E_INVAL, "material %s has no 'v'", name);
+  if (k < 3) return fail(TBG_E_INVAL, "need at least 3 pieces");
+  if (cnt[KING] != 1 || cnt[KING | 8] != 1) return fail(TBG_E_INVAL, "need one king per side");
+  for (int i = 0; i < k; i++)
+    for (int j = i + 1; j < k; j++)
+      if (kPieceOrder[pt[i]] > kPieceOrder[pt[j]]) { int x = pt[i]; pt[i] = pt[j]; pt[j] = x; }
+  M.num = k;
+  for (int i = 0; i < MAXP; i++) M.pt[i] = i < k ? pt[i] : 0;
+  for (int c = 0; c < 2; c++) {
+    M.nk[c] = 0;
+    for (int i = 2; i < k; i++)
+      if ((pt[i] >> 3) == c) M.pcs[c][M.nk[c]++] = i;
+  }
+  M.set_of = ~0ull;
+  int ns = 0;
+  for (int i = 2; i < k;) {
+    int j = i;
+    for (; i < k && pt[i] == pt[j]; i++) M.set_of = sq_set(M.set_of, i, ns);
+    M.ii.first[ns] = j;
+    M.ii.mult[ns] = i - j;
+    M.ii.last[ns] = i - 1;
+    M.setcol[ns] = pt[j] >> 3;
+    M.setmask80[ns] = 0;
+    for (int x = j; x < i; x++) M.setmask80[ns] |= 0x80ull << (8 * x);
+    ns++;
+  }
+  M.numsets = M.ii.numsets = ns;
+  calc_idx(M.ii);
+  M.words = ((M.ii.size + 511) / 512) * 8;
+  for (int s = 0; s < ns; s++) {
+    M.capt[s] = M.ii;
+    M.capt[s].mult[s]--;
+    calc_idx(M.capt[s]);
+    int c = M.setcol[s];
+    M.cwords[s] = ((M.capt[s].size + 511) / 512) * 8;
+    M.coff[s] = M.ctotal[c];
+    M.ctotal[c] += M.cwords[s];
+  }
+  return TBG_OK;
+}
+
+// create_material_string (src/probe.c:1480-1499)
+void material_string(const int *pt, int num, bool flip, char *out)
+{
+  int cnt[16] = { 0 };
+  for (int i = 0; i < num; i++) cnt[pt[i] ^ (flip ? 8 : 0)]++;
+  int j = 0;
+  for (int i = KING; i >= KNIGHT; i--)
+    while (cnt[i]--) out[j++] = kPieceChar[i];
+  out[j++] = 'v';
+  for (int i = KING; i >= KNIGHT; i--)
+    while (cnt[i + 8]--) out[j++] = kPieceChar[i];
+  out[j] = 0;
+}
+
+enum { CAT_PRED, CAT_VERIFY, CAT_SWEEP, CAT_UNCAP, CAT_INIT, CAT_RESOLVE, CAT_N };
+
+}  // namespace
+
+struct tbg_ctx {
+  char name[16];
+  Mat h;
+  Mat *dM = nullptr;
+  SubRef hR[MAXS];
+  SubRef *dR = nullptr;
+  uint64_t nsl = 0, pw = 0;
+  uint64_t *arena = nullptr, *subarena = nullptr;
+  uint64_t *wins[2], *prevW[2], *prevL[2], *newW[2], *newL[2], *cand[2];
+  uint64_t *sub[2][4];
+  uint16_t *codes[2] = { nullptr, nullptr };
+  unsigned long long *dcnt = nullptr;
+  hipStream_t st = nullptr;
+  int resolved = 0, inited = 0;
+  uint64_t bytes = 0;
+  int grid = 0;
+  int mgr[2][NSL];
+  bool prof = false;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> evs;
+  double ms[CAT_N];
+  uint64_t launches = 0, half_steps = 0;
+};
+
+namespace {
+
+int ctx_alloc(tbg_ctx *c, bool with_codes)
+{
+  Mat &M = c->h;
+  c->nsl = M.synth ? 1 : NSL;
+  M.nslices_phys = c->nsl;
+  c->pw = c->nsl * M.words;
+  HIPCHK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+  size_t pb = c->pw * 8;
+  HIPCHK(hipMalloc(&c->arena, pb * 12));
+  c->bytes += pb * 12;
+  uint64_t *p = c->arena;
+  for (int s = 0; s < 2; s++) {
+    c->wins[s] = p; p += c->pw;
+    c->prevW[s] = p; p += c->pw;
+    c->prevL[s] = p; p += c->pw;
+    c->newW[s] = p; p += c->pw;
+    c->newL[s] = p; p += c->pw;
+    c->cand[s] = p; p += c->pw;
+  }
+  HIPCHK(hipMemsetAsync(c->arena, 0, pb * 12, c->st));
+  uint64_t sw = 4 * c->nsl * (M.ctotal[0] + M.ctotal[1]);
+  HIPCHK(hipMalloc(&c->subarena, sw * 8 + 64));
+  c->bytes += sw * 8;
+  HIPCHK(hipMemsetAsync(c->subarena, 0, sw * 8 + 64, c->st));
+  p = c->subarena;
+  for (int col = 0; col < 2; col++)
+    for (int k = 0; k < 4; k++) {
+      c->sub[col][k] = p;
+      p += c->nsl * M.ctotal[col];
+    }
+  if (with_codes)
+    for (int s = 0; s < 2; s++) {
+      size_t cb = c->nsl * M.ii.size * 2;
+      HIPCHK(hipMalloc(&c->codes[s], cb));
+      c->bytes += cb;
+    }
+  HIPCHK(hipMalloc(&c->dM, sizeof(Mat)));
+  HIPCHK(hipMemcpyAsync(c->dM, &c->h, sizeof(Mat), hipMemcpyHostToDevice, c->st));
+  HIPCHK(hipMalloc(&c->dR, sizeof(SubRef) * MAXS));
+  HIPCHK(hipMemsetAsync(c->dR, 0, sizeof(SubRef) * MAXS, c->st));
+  HIPCHK(hipMalloc(&c->dcnt, 16 * sizeof(unsigned long long)));
+  HIPCHK(hipMemsetAsync(c->dcnt, 0, 16 * sizeof(unsigned long long), c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  c->grid = g_cus * 16;
+  manager_order(WHITE, c->mgr[WHITE]);
+  manager_order(BLACK, c->mgr[BLACK]);
+  for (int i = 0; i < CAT_N; i++) c->ms[i] = 0;
+  return TBG_OK;
+}
+
+// optional per-launch timing
+struct Timed {
+  tbg_ctx *c;
+  int cat;
+  hipEvent_t a = nullptr, b = nullptr;
+  Timed(tbg_ctx *c_, int cat_) : c(c_), cat(cat_)
+  {
+    c->launches++;
+    if (c->prof) {
+      (void)hipEventCreate(&a);
+      (void)hipEventCreate(&b);
+      (void)hipEventRecord(a, c->st);
+    }
+  }
+  ~Timed()
+  {
+    if (c->prof) {
+      (void)hipEventRecord(b, c->st);
+      c->evs.push_back({ cat, { a, b } });
+    }
+  }
+};
+
+void flush_timing(tbg_ctx *c)
+{
+  if (c->evs.empty()) return;
+  (void)hipStreamSynchronize(c->st);
+  for (auto &e : c->evs) {
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e.second.first, e.second.second);
+    c->ms[e.first] += ms;
+    (void)hipEventDestroy(e.second.first);
+    (void)hipEventDestroy(e.second.second);
+  }
+  c->evs.clear();
+}
+
+unsigned blocks_for(tbg_ctx *c, uint64_t waves)
+{
+  uint64_t b = (waves + 3) / 4;
+  if (b > (uint64_t)c->grid) b = (uint64_t)c->grid;
+  return (unsigned)(b ? b : 1);
+}
+
+int launch_pred(tbg_ctx *c, int stm, const uint64_t *src, int s0, int ns)
+{
+  uint64_t cps = (c->h.words + 63) / 64;
+  Timed t(c, CAT_PRED);
+  hipLaunchKernelGGL(k_pred, dim3(blocks_for(c, ns * cps)), dim3(256), 0, c->st, c->dM, stm, src,
+                     c->cand[stm], s0, ns, cps);
+  HIPCHK(hipGetLastError());
+  return TBG_OK;
+}
+
+int launch_uncap(tbg_ctx *c, int stm, int kind, int kind2, int legality, int s0, int ns)
+{
+  int o = stm ^ 1;
+  for (int k = 0; k < c->h.numsets; k++) {
+    if (c->h.setcol[k] != o) continue;
+    uint64_t cps = (c->h.cwords[k] + 63) / 64;
+    Timed t(c, CAT_UNCAP);
+    hipLaunchKernelGGL(k_uncap, dim3(blocks_for(c, ns * cps)), dim3(256), 0, c->st, c->dM, stm, k,
+                       c->sub[o][kind], kind2 >= 0 ? c->sub[o][kind2] : nullptr, legality,
+                       c->cand[stm], s0, ns, cps);
+    HIPCHK(hipGetLastError());
+  }
+  return TBG_OK;
+}
+
+int launch_verify(tbg_ctx *c, int stm, int n, int s0, int ns)
+{
+  int o = stm ^ 1;
+  uint64_t cps = (c->h.words + 63) / 64;
+  Timed t(c, CAT_VERIFY);
+  hipLaunchKernelGGL(k_verify, dim3(blocks_for(c, ns * cps)), dim3(256), 0, c->st, c->dM, stm, n,
+                     c->cand[stm], c->wins[o], c->sub[o][n <= DRAW_RULE ? TBG_SUB_WIN : TBG_SUB_CWIN],
+                     c->newL[stm], c->codes[stm], s0, ns, cps, c->dcnt);
+  HIPCHK(hipGetLastError());
+  return TBG_OK;
+}
+
+int launch_wsweep(tbg_ctx *c, int stm, int n, uint64_t w0, uint64_t nw)
+{
+  Timed t(c, CAT_SWEEP);
+  uint64_t threads = (nw + 1) / 2;
+  uint64_t b = (threads + 255) / 256;
+  if (b > (uint64_t)g_cus * 8) b = (uint64_t)g_cus * 8;
+  hipLaunchKernelGGL(k_wsweep, dim3((unsigned)(b ? b : 1)), dim3(256), 0, c->st, c->cand[stm],
+                     c->wins[stm], c->newW[stm], c->codes[stm], nw, c->h.words, c->h.ii.size, w0, n,
+                     c->dcnt);
+  HIPCHK(hipGetLastError());
+  return TBG_OK;
+}
+
+int launch_exits(tbg_ctx *c, int stm, int which)
+{
+  Timed t(c, CAT_UNCAP);
+  hipLaunchKernelGGL(k_synth_exits, dim3(blocks_for(c, c->h.words)), dim3(256), 0, c->st, c->dM,
+                     stm, which, c->cand[stm]);
+  HIPCHK(hipGetLastError());
+  return TBG_OK;
+}
+
+int read_count(tbg_ctx *c, uint64_t *out)
+{
+  unsigned long long v = 0;
+  HIPCHK(hipMemcpyAsync(&v, c->dcnt, sizeof v, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipMemsetAsync(c->dcnt, 0, sizeof v, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  if (out) *out = v;
+  return TBG_OK;
+}
+
+#define TRY(x)                  \
+  do {                          \
+    int r_ = (x);               \
+    if (r_ != TBG_OK) return r_; \
+  } while (0)
+
+int check_stm(tbg_ctx *c, int stm)
+{
+  if (!c) return fail(TBG_E_INVAL, "null context");
+  if (stm != 0 && stm != 1) return fail(TBG_E_INVAL, "side to move must be 0 or 1");
+  if (!c->resolved || !c->inited) return fail(TBG_E_STATE, "resolve captures and init first");
+  return TBG_OK;
+}
+
+int step_L(tbg_ctx *c, int stm, int n, int more, uint64_t *count, bool plain)
+{
+  TRY(check_stm(c, stm));
+  if (n < 1) return fail(TBG_E_INVAL, "ply must be >= 1");
+  int o = stm ^ 1;
+  c->half_steps++;
+  int slices = (int)c->nsl;
+  // candidates (tb8gen.c:951-976)
+  for (int i = 0; i < (plain ? slices : 1); i++) {
+    int s0 = plain ? (c->h.synth ? c->h.synth_slice : c->mgr[stm][i]) : (c->h.synth ? c->h.synth_slice : 0);
+    int ns = plain ? 1 : (c->h.synth ? 1 : NSL);
+    if (n == 1) TRY(launch_uncap(c, stm, TBG_SUB_WIN, -1, 1, s0, ns));
+    else if (n == DRAW_RULE + 1) TRY(launch_uncap(c, stm, TBG_SUB_CWIN, TBG_SUB_WIN, 0, s0, ns));
+    if (more && n > 1) TRY(launch_pred(c, stm, c->prevW[o], s0, ns));
+  }
+  if (n == 1 && c->h.synth) TRY(launch_exits(c, stm, 0));
+  // verification (tb8gen.c:999-1022)
+  for (int i = 0; i < (plain ? slices : 1); i++) {
+    int s0 = plain ? (c->h.synth ? c->h.synth_slice : c->mgr[stm][i]) : (c->h.synth ? c->h.synth_slice : 0);
+    int ns = plain ? 1 : (c->h.synth ? 1 : NSL);
+    TRY(launch_verify(c, stm, n, s0, ns));
+  }
+  return read_count(c, count);
+}
+
+int step_W(tbg_ctx *c, int stm, int n, int more, uint64_t *count, bool plain)
+{
+  TRY(check_stm(c, stm));
+  if (n < 1) return fail(TBG_E_INVAL, "ply must be >= 1");
+  int o = stm ^ 1;
+  c->half_steps++;
+  int slices = (int)c->nsl;
+  for (int i = 0; i < (plain ? slices : 1); i++) {
+    int s0 = plain ? (c->h.synth ? c->h.synth_slice : c->mgr[stm][i]) : (c->h.synth ? c->h.synth_slice : 0);
+    int ns = plain ? 1 : (c->h.synth ? 1 : NSL);
+    if (n == 1) TRY(launch_uncap(c, stm, TBG_SUB_LOSS, -1, 0, s0, ns));
+    else if (n == DRAW_RULE + 1) TRY(launch_uncap(c, stm, TBG_SUB_BLOSS, -1, 0, s0, ns));
+    if (more) TRY(launch_pred(c, stm, c->prevL[o], s0, ns));
+  }
+  if (n == 1 && c->h.synth) TRY(launch_exits(c, stm, 1));
+  // W = C & ~wins; wins |= W (tb8gen.c:1072-1077)
+  if (plain && !c->h.synth) {
+    for (int i = 0; i < slices; i++) {
+      uint64_t s = (uint64_t)c->mgr[stm][i];
+      TRY(launch_wsweep(c, stm, n, s * c->h.words, c->h.words));
+    }
+  } else {
+    TRY(launch_wsweep(c, stm, n, 0, c->pw));
+  }
+  return read_count(c, count);
+}
+
+int zero_plane(tbg_ctx *c, uint64_t *p)
+{
+  HIPCHK(hipMemsetAsync(p, 0, c->pw * 8, c->st));
+  return TBG_OK;
+}
+
+}  // namespace
+
+// ====================================================================
+// C ABI
+// ====================================================================
+
+extern "C" {
+
+int tbg_init(int device)
+{
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return fail(TBG_E_NODEV, "no HIP device");
+  if (device < 0 || device >= n) return fail(TBG_E_INVAL, "device %d out of range", device);
+  HIPCHK(hipSetDevice(device));
+  hipDeviceProp_t prop;
+  HIPCHK(hipGetDeviceProperties(&prop, device));
+  g_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+  build_tabs(g_htabs);
+  HIPCHK(upload_tabs(&g_htabs));
+  HIPCHK(hipDeviceSynchronize());
+  g_device = device;
+  return TBG_OK;
+}
+
+const char *tbg_strerror(int err)
+{
+  switch (err) {
+  case TBG_OK: return "ok";
+  case TBG_E_INVAL: return g_errbuf[0] ? g_errbuf : "invalid argument";
+  case TBG_E_NOMEM: return "out of memory";
+  case TBG_E_HIP: return g_errbuf[0] ? g_errbuf : "HIP error";
+  case TBG_E_NOSUB: return g_errbuf[0] ? g_errbuf : "missing smaller table";
+  case TBG_E_STATE: return g_errbuf[0] ? g_errbuf : "call out of order";
+  case TBG_E_BUDGET: return "device memory budget exceeded";
+  case TBG_E_NODEV: return "no HIP device";
+  default: return "unknown error";
+  }
+}
+
+static int create_common(tbg_ctx **out, const char *material, int synth, int slice, uint64_t seed)
+{
+  if (!out || !material) return fail(TBG_E_INVAL, "null argument");
+  *out = nullptr;
+  if (g_device < 0) return fail(TBG_E_STATE, "tbg_init not called");
+  tbg_ctx *c = new tbg_ctx();
+  snprintf(c->name, sizeof c->name, "%s", material);
+  int r = setup_mat(c->h, material);
+  if (r != TBG_OK) { delete c; return r; }
+  c->h.synth = synth;
+  c->h.synth_slice = slice;
+  c->h.seed = seed;
+  r = ctx_alloc(c, !synth);
+  if (r != TBG_OK) {
+    tbg_destroy(c);
+    return r == TBG_E_HIP ? fail(TBG_E_NOMEM, "device allocation failed") : r;
+  }
+  *out = c;
+  return TBG_OK;
+}
+
+int tbg_create(tbg_ctx **out, const char *material) { return create_common(out, material, 0, -1, 0); }
+
+int tbg_create_synth(tbg_ctx **out, const char *material, int slice, uint64_t seed)
+{
+  if (slice < 0 || slice >= 441) return fail(TBG_E_INVAL, "synthetic slice must be in [0, 441)");
+  return create_common(out, material, 1, slice, seed);
+}
+
+void tbg_destroy(tbg_ctx *c)
+{
+  if (!c) return;
+  if (c->st) (void)hipStreamSynchronize(c->st);
+  flush_timing(c);
+  (void)hipFree(c->arena);
+  (void)hipFree(c->subarena);
+  (void)hipFree(c->codes[0]);
+  (void)hipFree(c->codes[1]);
+  (void)hipFree(c->dM);
+  (void)hipFree(c->dR);
+  (void)hipFree(c->dcnt);
+  if (c->st) (void)hipStreamDestroy(c->st);
+  delete c;
+}
+
+int tbg_get_info(const tbg_ctx *c, tbg_info *info)
+{
+  if (!c || !info) return fail(TBG_E_INVAL, "null argument");
+  memset(info, 0, sizeof *info);
+  info->npieces = c->h.num;
+  for (int i = 0; i < 8; i++) info->pt[i] = (uint8_t)c->h.pt[i];
+  info->numsets = c->h.numsets;
+  for (int k = 0; k < c->h.numsets; k++) {
+    info->mult[k] = c->h.ii.mult[k];
+    info->capt_size[k] = c->h.capt[k].size;
+    info->sub_offset[k] = c->h.coff[k];
+  }
+  info->slice_size = c->h.ii.size;
+  info->slice_words = c->h.words;
+  info->sub_words[0] = c->h.ctotal[0];
+  info->sub_words[1] = c->h.ctotal[1];
+  info->device_bytes = c->bytes;
+  info->synth_slice = c->h.synth ? c->h.synth_slice : -1;
+  return TBG_OK;
+}
+
+int tbg_resolve_captures(tbg_ctx *c, tbg_ctx *const *subs, int nsubs)
+{
+  if (!c) return fail(TBG_E_INVAL, "null context");
+  Mat &M = c->h;
+  for (int k = 0; k < M.numsets; k++) {
+    SubRef &R = c->hR[k];
+    memset(&R, 0, sizeof R);
+    int spt[MAXP], slots[MAXP], n = 0;
+    for (int i = 0; i < M.num; i++)
+      if (i != M.ii.last[k]) { spt[n] = M.pt[i]; slots[n] = i; n++; }
+    if (M.synth || n == 2) continue;  // hashed exits / KvK
+    char nm[16], fl[16];
+    material_string(spt, n, false, nm);
+    material_string(spt, n, true, fl);
+    tbg_ctx *hit = nullptr;
+    int flip = 0;
+    for (int i = 0; i < nsubs && !hit; i++) {
+      if (!subs[i]) continue;
+      if (!strcmp(subs[i]->name, nm)) hit = subs[i];
+      else if (!strcmp(subs[i]->name, fl)) { hit = subs[i]; flip = 1; }
+    }
+    if (!hit) return fail(TBG_E_NOSUB, "table %s (or %s) not supplied", nm, fl);
+    if (!hit->codes[0] || !hit->inited) return fail(TBG_E_STATE, "table %s not generated", hit->name);
+    R.present = 1;
+    R.flip = flip;
+    R.num = hit->h.num;
+    R.ii = hit->h.ii;
+    R.codes[0] = hit->codes[0];
+    R.codes[1] = hit->codes[1];
+    bool used[MAXP] = { false };
+    R.map = 0;
+    for (int j = 0; j < R.num; j++) {
+      int want = hit->h.pt[j] ^ (flip ? 8 : 0);
+      int src = -1;
+      for (int i = 0; i < n; i++)
+        if (!used[i] && spt[i] == want) { used[i] = true; src = slots[i]; break; }
+      if (src < 0) return fail(TBG_E_NOSUB, "table %s does not match", hit->name);
+      R.map = sq_set(R.map, j, src);
+    }
+  }
+  HIPCHK(hipMemcpyAsync(c->dR, c->hR, sizeof(SubRef) * MAXS, hipMemcpyHostToDevice, c->st));
+  int s0 = M.synth ? M.synth_slice : 0, ns = M.synth ? 1 : NSL;
+  for (int k = 0; k < M.numsets; k++) {
+    int col = M.setcol[k];
+    uint64_t waves = (uint64_t)ns * M.cwords[k];
+    Timed t(c, CAT_RESOLVE);
+    hipLaunchKernelGGL(k_resolve, dim3(blocks_for(c, waves)), dim3(256), 0, c->st, c->dM, c->dR + k,
+                       col, k, c->sub[col][0], c->sub[col][1], c->sub[col][2], c->sub[col][3], s0, ns);
+    HIPCHK(hipGetLastError());
+  }
+  HIPCHK(hipStreamSynchronize(c->st));
+  c->resolved = 1;
+  return TBG_OK;
+}
+
+int tbg_init_illegal_mate(tbg_ctx *c, tbg_counts *counts)
+{
+  if (!c) return fail(TBG_E_INVAL, "null context");
+  Mat &M = c->h;
+  int s0 = M.synth ? M.synth_slice : 0, ns = M.synth ? 1 : NSL;
+  HIPCHK(hipMemsetAsync(c->dcnt, 0, 4 * sizeof(unsigned long long), c->st));
+  {
+    Timed t(c, CAT_INIT);
+    hipLaunchKernelGGL(k_init, dim3(blocks_for(c, (uint64_t)ns * M.words)), dim3(256), 0, c->st,
+                       c->dM, c->wins[0], c->wins[1], c->prevL[0], c->prevL[1], c->codes[0],
+                       c->codes[1], s0, ns, c->dcnt);
+    HIPCHK(hipGetLastError());
+  }
+  unsigned long long v[4];
+  HIPCHK(hipMemcpyAsync(v, c->dcnt, sizeof v, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipMemsetAsync(c->dcnt, 0, 4 * sizeof(unsigned long long), c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  if (counts) {
+    counts->broken[0] = v[0];
+    counts->broken[1] = v[1];
+    counts->mates[0] = v[2];
+    counts->mates[1] = v[3];
+    counts->nplies = 0;
+  }
+  c->inited = 1;
+  return TBG_OK;
+}
+
+int tbg_step_L(tbg_ctx *c, int stm, int n, int more, uint64_t *count)
+{
+  return step_L(c, stm, n, more, count, false);
+}
+int tbg_step_W(tbg_ctx *c, int stm, int n, int more, uint64_t *count)
+{
+  return step_W(c, stm, n, more, count, false);
+}
+int tbg_step_L_plain(tbg_ctx *c, int stm, int n, int more, uint64_t *count)
+{
+  return step_L(c, stm, n, more, count, true);
+}
+int tbg_step_W_plain(tbg_ctx *c, int stm, int n, int more, uint64_t *count)
+{
+  return step_W(c, stm, n, more, count, true);
+}
+
+int tbg_end_ply(tbg_ctx *c)
+{
+  if (!c) return fail(TBG_E_INVAL, "null context");
+  for (int s = 0; s < 2; s++) {
+    uint64_t *x = c->prevW[s];
+    c->prevW[s] = c->newW[s];
+    c->newW[s] = x;
+    x = c->prevL[s];
+    c->prevL[s] = c->newL[s];
+    c->newL[s] = x;
+  }
+  return TBG_OK;
+}
+
+int tbg_generate(tbg_ctx *c, int form, tbg_line_fn fn, void *user, tbg_counts *counts)
+{
+  if (!c) return fail(TBG_E_INVAL, "null context");
+  if (form < TBG_FORM_PLAIN || form > TBG_FORM_GRAPH) return fail(TBG_E_INVAL, "bad form");
+  if (!c->resolved) return fail(TBG_E_STATE, "resolve captures first");
+  bool plain = form == TBG_FORM_PLAIN;
+  char line[64];
+  tbg_counts cnt;
+  TRY(tbg_init_illegal_mate(c, &cnt));
+  if (fn) {
+    const char *nm[4] = { "broken_w", "broken_b", "l0_w", "l0_b" };
+    uint64_t v[4] = { cnt.broken[0], cnt.broken[1], cnt.mates[0], cnt.mates[1] };
+    for (int i = 0; i < 4; i++) {
+      snprintf(line, sizeof line, "%s = %llu", nm[i], (unsigned long long)v[i]);
+      fn(line, user);
+    }
+  }
+  // ply loop of main() (tb8gen.c:1406-1444)
+  bool more_ww = true, more_wb = true, more_lw = true, more_lb = true;
+  int n;
+  for (n = 1;; n++) {
+    bool forced = n == DRAW_RULE + 1;
+    if (n > DRAW_RULE + 1 && !(more_ww || more_wb || more_lw || more_lb)) break;
+    bool ml[2] = { more_lw, more_lb }, mw[2] = { more_ww, more_wb };
+    bool lres[2], wres[2];
+    for (int stm = 0; stm < 2; stm++) {
+      if (!(forced || ml[stm])) { lres[stm] = false; TRY(zero_plane(c, c->newL[stm])); continue; }
+      uint64_t k = 0;
+      TRY(step_L(c, stm, n, ml[stm], &k, plain));
+      lres[stm] = k != 0;
+      if (fn) {
+        snprintf(line, sizeof line, "l%d_%c = %llu", n, "wb"[stm], (unsigned long long)k);
+        fn(line, user);
+      }
+    }
+    for (int stm = 0; stm < 2; stm++) {
+      if (!(forced || mw[stm])) { wres[stm] = false; TRY(zero_plane(c, c->newW[stm])); continue; }
+      uint64_t k = 0;
+      TRY(step_W(c, stm, n, mw[stm], &k, plain));
+      wres[stm] = k != 0;
+      if (fn) {
+        snprintf(line, sizeof line, "w%d_%c = %llu", n, "wb"[stm], (unsigned long long)k);
+        fn(line, user);
+      }
+    }
+    more_wb = lres[WHITE];
+    more_ww = lres[BLACK];
+    more_lb = wres[WHITE];
+    more_lw = wres[BLACK];
+    TRY(tbg_end_ply(c));
+  }
+  if (counts) {
+    *counts = cnt;
+    counts->nplies = n - 1;
+  }
+  return TBG_OK;
+}
+
+int tbg_get_codes(tbg_ctx *c, int stm, int slice, uint16_t *dst, uint64_t n)
+{
+  if (!c || !dst || (stm & ~1)) return fail(TBG_E_INVAL, "bad argument");
+  if (!c->codes[stm]) return fail(TBG_E_STATE, "context keeps no codes");
+  uint64_t ps = c->h.synth ? 0 : (uint64_t)slice;
+  if (c->h.synth ? slice != c->h.synth_slice : (slice < 0 || slice >= NSL))
+    return fail(TBG_E_INVAL, "slice out of range");
+  if (n > c->h.ii.size) return fail(TBG_E_INVAL, "too many codes");
+  HIPCHK(hipMemcpyAsync(dst, c->codes[stm] + ps * c->h.ii.size, n * 2, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return TBG_OK;
+}
+
+int tbg_get_bitmap(tbg_ctx *c, int plane, int stm, int slice, uint64_t *dst, uint64_t nwords)
+{
+  if (!c || !dst || (stm & ~1)) return fail(TBG_E_INVAL, "bad argument");
+  if (c->h.synth ? slice != c->h.synth_slice : (slice < 0 || slice >= NSL))
+    return fail(TBG_E_INVAL, "slice out of range");
+  if (nwords > c->h.words) return fail(TBG_E_INVAL, "too many words");
+  uint64_t *p = plane == TBG_PLANE_WINS ? c->wins[stm]
+              : plane == TBG_PLANE_W    ? c->prevW[stm]
+              : plane == TBG_PLANE_L    ? c->prevL[stm]
+                                        : nullptr;
+  if (!p) return fail(TBG_E_INVAL, "bad plane");
+  uint64_t ps = c->h.synth ? 0 : (uint64_t)slice;
+  HIPCHK(hipMemcpyAsync(dst, p + ps * c->h.words, nwords * 8, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return TBG_OK;
+}
+
+int tbg_get_sub_bitmap(tbg_ctx *c, int kind, int colour, int slice, uint64_t *dst, uint64_t nwords)
+{
+  if (!c || !dst || (colour & ~1) || kind < 0 || kind > 3) return fail(TBG_E_INVAL, "bad argument");
+  if (c->h.synth ? slice != c->h.synth_slice : (slice < 0 || slice >= NSL))
+    return fail(TBG_E_INVAL, "slice out of range");
+  if (nwords > c->h.ctotal[colour]) return fail(TBG_E_INVAL, "too many words");
+  uint64_t ps = c->h.synth ? 0 : (uint64_t)slice;
+  HIPCHK(hipMemcpyAsync(dst, c->sub[colour][kind] + ps * c->h.ctotal[colour], nwords * 8,
+                        hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return TBG_OK;
+}
+
+int tbg_set_profiling(tbg_ctx *c, int on)
+{
+  if (!c) return fail(TBG_E_INVAL, "null context");
+  flush_timing(c);
+  c->prof = on != 0;
+  for (int i = 0; i < CAT_N; i++) c->ms[i] = 0;
+  c->launches = c->half_steps = 0;
+  return TBG_OK;
+}
+
+int tbg_get_timing(const tbg_ctx *cc, tbg_timing *t)
+{
+  tbg_ctx *c = const_cast<tbg_ctx *>(cc);
+  if (!c || !t) return fail(TBG_E_INVAL, "null argument");
+  flush_timing(c);
+  t->ms_pred = c->ms[CAT_PRED];
+  t->ms_verify = c->ms[CAT_VERIFY];
+  t->ms_sweep = c->ms[CAT_SWEEP];
+  t->ms_uncap = c->ms[CAT_UNCAP];
+  t->ms_init = c->ms[CAT_INIT];
+  t->ms_resolve = c->ms[CAT_RESOLVE];
+  t->ms_total = 0;
+  for (int i = 0; i < CAT_N; i++) t->ms_total += c->ms[i];
+  t->launches = c->launches;
+  t->half_steps = c->half_steps;
+  return TBG_OK;
+}
+
+}  // extern "C"
+
+// ====================================================================
+// Host self-test of the __host__ __device__ primitives (no GPU needed):
+// attacks against ray walking, select_zero and div_fix against brute
+// force, index round trips and bijection, slice symmetry invariants.
+// ====================================================================
+namespace {
+
+uint64_t walk_att(int pt, int sq, uint64_t occ)
+{
+  static const int df[8] = { 1, -1, 0, 0, 1, 1, -1, -1 }, dr[8] = { 0, 0, 1, -1, 1, -1, 1, -1 };
+  int d0 = (pt & 7) == BISHOP ? 4 : 0, d1 = (pt & 7) == ROOK ? 4 : 8;
+  uint64_t a = 0;
+  for (int d = d0; d < d1; d++)
+    for (int f = (sq & 7) + df[d], r = (sq >> 3) + dr[d]; f >= 0 && f < 8 && r >= 0 && r < 8;
+         f += df[d], r += dr[d]) {
+      a |= 1ull << (r * 8 + f);
+      if (occ & (1ull << (r * 8 + f))) break;
+    }
+  return a;
+}
+
+uint64_t rng_state = 0x2545f4914f6cdd1dull;
+uint64_t rnd()
+{
+  rng_state ^= rng_state << 13;
+  rng_state ^= rng_state >> 7;
+  rng_state ^= rng_state << 17;
+  return rng_state;
+}
+
+}  // namespace
+
+extern "C" int tbg_host_selftest(char *msg, int len)
+{
+#define ST_FAIL(...)                       \
+  do {                                     \
+    if (msg) snprintf(msg, len, __VA_ARGS__); \
+    return 1;                              \
+  } while (0)
+  build_tabs(g_htabs);
+  const Tabs &T = g_htabs;
+  for (int it = 0; it < 4000; it++) {
+    uint64_t occ = rnd() & rnd();
+    for (int sq = 0; sq < 64; sq++) {
+      uint64_t o = occ | (1ull << sq);
+      if (rook_att(sq, o) != walk_att(ROOK, sq, o)) ST_FAIL("rook attacks sq %d occ %llx", sq, (unsigned long long)o);
+      if (bishop_att(sq, o) != walk_att(BISHOP, sq, o)) ST_FAIL("bishop attacks sq %d", sq);
+    }
+  }
+  for (int it = 0; it < 20000; it++) {
+    uint64_t occ = rnd() & rnd() & rnd();
+    int nz = 64 - popc64(occ);
+    int r = (int)(rnd() % (uint64_t)nz);
+    int s = -1, seen = 0;
+    for (int q = 0; q < 64; q++)
+      if (!(occ >> q & 1) && seen++ == r) { s = q; break; }
+    if (select_zero(occ, r) != s) ST_FAIL("select_zero");
+  }
+  for (int it = 0; it < 200000; it++) {
+    uint64_t d = 1 + rnd() % 0xffffffffull, n = rnd() % (1ull << 40), rem;
+    uint64_t q = div_fix(n, d, 1.0 / (double)d, &rem);
+    if (q != n / d || rem != n % d) ST_FAIL("div_fix %llu / %llu", (unsigned long long)n, (unsigned long long)d);
+  }
+  // 462 slices, KKMap invariant under the 8 board symmetries
+  int cnt[NSL] = { 0 };
+  for (int wk = 0; wk < 64; wk++)
+    for (int bk = 0; bk < 64; bk++) {
+      int s = kkmap(wk, bk);
+      if (wk == bk || touching(wk, bk)) { if (s != -1) ST_FAIL("kkmap touching"); continue; }
+      if (s < 0 || s >= NSL) ST_FAIL("kkmap range %d %d", wk, bk);
+      cnt[s]++;
+      for (int sym = 0; sym < 8; sym++) {
+        auto tr = [&](int x) {
+          int f = x & 7, r = x >> 3;
+          if (sym & 1) f = 7 - f;
+          if (sym & 2) r = 7 - r;
+          if (sym & 4) { int t = f; f = r; r = t; }
+          return r * 8 + f;
+        };
+        if (kkmap(tr(wk), tr(bk)) != s) ST_FAIL("kkmap symmetry");
+      }
+    }
+  for (int s = 0; s < NSL; s++) {
+    if (!cnt[s]) ST_FAIL("slice %d unused", s);
+    int wk = T.kksq[s][0], bk = T.kksq[s][1];
+    if (kkmap(wk, bk) != s) ST_FAIL("kksq/kkmap disagree at %d", s);
+    if ((s >= 441) != (on_diag(wk) && on_diag(bk))) ST_FAIL("diagonal slice numbering");
+  }
+  // index bijection (exhaustive on small materials) and round trips
+  const char *mats[] = { "KRvK", "KRRvK", "KBNvK", "KQRvKR", "KNNNvKB", "KQRBvKQRN", "KRRRvKRR" };
+  for (const char *nm : mats) {
+    Mat M;
+    if (setup_mat(M, nm) != TBG_OK) ST_FAIL("setup %s", nm);
+    std::vector<Idx> spaces(1, M.ii);
+    for (int k = 0; k < M.numsets; k++) spaces.push_back(M.capt[k]);
+    for (size_t sp = 0; sp < spaces.size(); sp++) {
+      const Idx &ii = spaces[sp];
+      bool exhaustive = ii.size <= 300000;
+      for (int s = 0; s < NSL; s += exhaustive ? 37 : 1) {
+        uint64_t kings = (uint64_t)T.kksq[s][0] | ((uint64_t)T.kksq[s][1] << 8);
+        uint64_t nsamp = exhaustive ? ii.size : 200;
+        for (uint64_t t = 0; t < nsamp; t++) {
+          uint64_t idx = exhaustive ? t : rnd() % ii.size;
+          uint64_t p = kings;
+          uint64_t occ = idx_to_sq(T, ii, idx, &p);
+          uint64_t chk = 0;
+          int npl = 2;
+          for (int k = 0; k < ii.numsets; k++) npl += ii.mult[k];
+          for (int k = 0; k < ii.numsets; k++)
+            for (int a = 0; a < ii.mult[k]; a++) {
+              int q = sq_get(p, ii.first[k] + a);
+              if (q > 63) ST_FAIL("%s: square out of range", nm);
+              chk |= 1ull << q;
+            }
+          chk |= (1ull << T.kksq[s][0]) | (1ull << T.kksq[s][1]);
+          if (chk != occ || popc64(occ) != npl) ST_FAIL("%s space %zu: occupancy idx %llu", nm, sp, (unsigned long long)idx);
+          // members of a set ascend (unrank_binomial order)
+          for (int k = 0; k < ii.numsets; k++)
+            for (int a = 1; a < ii.mult[k]; a++)
+              if (sq_get(p, ii.first[k] + a) <= sq_get(p, ii.first[k] + a - 1)) ST_FAIL("%s: set order", nm);
+          if (sq_to_idx(T, ii, p) != idx) ST_FAIL("%s space %zu: round trip idx %llu slice %d", nm, sp, (unsigned long long)idx, s);
+          // permuting members within a set does not change the index
+          for (int k = 0; k < ii.numsets; k++)
+            if (ii.mult[k] >= 2) {
+              int f = ii.first[k];
+              uint64_t p2 = sq_set(sq_set(p, f, sq_get(p, f + 1)), f + 1, sq_get(p, f));
+              if (sq_to_idx(T, ii, p2) != idx) ST_FAIL("%s: set permutation", nm);
+            }
+        }
+      }
+    }
+  }
+  if (msg) snprintf(msg, len, "ok");
+  return 0;
+#undef ST_FAIL
+}
