@@ -1,0 +1,183 @@
+#!/usr/bin/env python3
+"""bench.py -- retrograde throughput of the MI355X king-slice engine.
+
+Workload (default, one GPU): the KQRvKR table -- 462 king slices x 226,920
+positions per side (62*61*60), exhaustive, no input data -- generated
+end to end on the device: capture exits resolved against the resident
+KQvKR / KRvKR / KQRvK tables (generated beforehand, untimed), illegal and
+mate init, then the ply loop of the reference's main() to its fixpoint.
+One step = one such generation.
+
+value = position-side-plies per second over the whole step: every executed
+half-step (L or W, one side) covers 462 * 226,920 positions for half a ply,
+so a full ply of both sides is 2 * 104,837,040 position-side-plies.
+roofline: the kernel with the largest device time; algorithmic bytes per
+launch = its bits per position (pred 2, verify 3, W sweep 4; 11 per
+position-side-ply in all) * positions it covers / 8, over its average
+launch time measured with HIP events on the engine's stream.
+cpu_baseline: the CPU oracle (a plain-C restatement of the reference's
+algorithm) on the host cores, timed over plies 1..2 of the same material.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--form batched|plain]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+BITS = {"pred": 2, "verify": 3, "sweep": 4}
+METRIC = "retro positions/s per GPU + HBM GB/s fraction; p50/p99 slice latency vs plain"
+
+
+def cpu_threads():
+    n = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    return max(1, min(n, 16))
+
+
+def run_steps(table, subs, form, steps, tbg):
+    """Time `steps` full generations; returns (seconds list, timing, half_steps, lines)."""
+    secs = []
+    table.set_profiling(True)
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        table.resolve(subs)
+        lines = table.generate(form)
+        secs.append(time.perf_counter() - t0)
+    timing = table.timing()
+    table.set_profiling(False)
+    return secs, timing, lines
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--material", default="KQRvKR")
+    ap.add_argument("--form", default="batched", choices=["batched", "plain"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain-form comparison")
+    ap.add_argument("--cpu-plies", type=int, default=2)
+    ap.add_argument("--scatter", default="block", choices=["block", "global"])
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB (calibrates the "
+                         "FETCH_SIZE / WRITE_SIZE counters against known bytes)")
+    args = ap.parse_args()
+
+    if args.copy_probe:
+        import torch
+        n = args.copy_probe << 20
+        a = torch.ones(n, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": n, "copies": 3}))
+        return
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py: this engine is measured on one GPU; multi-GPU slice sharding is "
+              "designed (DESIGN.md) but not built, so --gpus must be 1", file=sys.stderr)
+        sys.exit(2)
+
+    import tb8gen_amd as tbg
+
+    tbg.init(0)
+    form = tbg.FORM_BATCHED if args.form == "batched" else tbg.FORM_PLAIN
+    reg = tbg.Registry(tbg.FORM_BATCHED)
+    for sub in tbg.sub_materials(args.material):
+        reg.get(sub)
+    subs = list(reg.tables.values())
+    table = tbg.Table(args.material)
+    table.set_scatter(tbg.SCATTER_BLOCK if args.scatter == "block" else tbg.SCATTER_GLOBAL)
+    n_total = tbg.NSLICES * table.size
+
+    for _ in range(args.warmup):
+        table.resolve(subs)
+        table.generate(form)
+    secs, tm, lines = run_steps(table, subs, form, args.steps, tbg)
+    ms_step = 1000.0 * max(secs)
+    half_steps = tm.half_steps / max(1, args.steps)
+    psp = half_steps * n_total / 2.0  # position-side-plies per step
+    value = psp / (sum(secs) / len(secs))
+
+    cats = {"pred": (tm.ms_pred, tm.n_pred), "verify": (tm.ms_verify, tm.n_verify),
+            "sweep": (tm.ms_sweep, tm.n_sweep)}
+    dom = max(cats, key=lambda k: cats[k][0])
+    dom_ms, dom_n = cats[dom]
+    avg_ms = dom_ms / max(1, dom_n)
+    covered = n_total if form == tbg.FORM_BATCHED else table.size
+    alg_bytes = BITS[dom] * covered / 8.0
+    achieved = alg_bytes / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"traffic_{args.material}_{args.form}.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get(dom)
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "position-side-plies/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{args.material} table generation, {args.form} form: 462 king slices x "
+                        f"{table.size} positions per side, capture exits from resident smaller "
+                        "tables, init + ply loop to the fixpoint",
+            "material": args.material, "form": args.form, "scatter": args.scatter, "slices": 462,
+            "positions_per_side": n_total, "half_steps_per_step": half_steps,
+            "plies": table.counts.nplies,
+        },
+        "roofline": {
+            "kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+            "alg_bytes_per_launch": alg_bytes, "avg_launch_ms": avg_ms, "launches": dom_n / max(1, args.steps),
+        },
+        "kernels_ms_per_step": {
+            "pred": tm.ms_pred / args.steps, "verify": tm.ms_verify / args.steps,
+            "sweep": tm.ms_sweep / args.steps, "uncap": tm.ms_uncap / args.steps,
+            "init": tm.ms_init / args.steps, "resolve": tm.ms_resolve / args.steps,
+            "launches": tm.launches / args.steps,
+        },
+    }
+
+    if not args.no_plain and form == tbg.FORM_BATCHED:
+        t0 = time.perf_counter()
+        table.resolve(subs)
+        plines = table.generate(tbg.FORM_PLAIN)
+        pt = time.perf_counter() - t0
+        out["plain"] = {"ms_per_step": 1000 * pt, "value": psp / pt,
+                        "batched_over_plain": (psp / (sum(secs) / len(secs))) / (psp / pt),
+                        "same_count_lines": plines == lines}
+
+    if not args.no_cpu:
+        from oracle import oracle as O
+        threads = cpu_threads()
+        O.set_threads(threads)
+        sec, hs = O.time_plies(args.material, args.cpu_plies)
+        out["cpu_baseline"] = {
+            "value": hs * n_total / 2.0 / sec, "unit": "position-side-plies/s", "cores": threads,
+            "kind": "port",
+            "sample": f"CPU oracle (C restatement of the reference algorithm, OpenMP) on plies "
+                      f"1..{args.cpu_plies} of {args.material}: {hs} half-steps over all 462 "
+                      f"slices in {sec:.2f} s; smaller tables generated untimed",
+        }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -135,12 +135,22 @@ int tbg_get_bitmap(tbg_ctx *ctx, int plane, int stm, int slice, uint64_t *dst, u
 int tbg_get_sub_bitmap(tbg_ctx *ctx, int kind, int colour, int slice, uint64_t *dst,
                        uint64_t nwords);
 
+/* Predecessor scatter strategy (the reference scatters with byte-wide
+ * relaxed atomic ORs, src/kslice.h:40-45):
+ *   TBG_SCATTER_GLOBAL: one 64-bit device atomic OR per unmove;
+ *   TBG_SCATTER_BLOCK (default): a work-group owns a block of the index and
+ *     ORs same-block unmoves into an LDS bitmap, merged once per word. */
+#define TBG_SCATTER_GLOBAL 0
+#define TBG_SCATTER_BLOCK 1
+int tbg_set_scatter(tbg_ctx *ctx, int mode);
+
 /* Last kernel timings of the context (HIP events on its stream). */
 typedef struct tbg_timing {
-  double ms_total;          /* device time of the last tbg_generate / steps */
+  double ms_total;          /* device time since tbg_set_profiling(ctx, 1)  */
   double ms_pred, ms_verify, ms_sweep, ms_uncap, ms_init, ms_resolve;
   uint64_t launches;
   uint64_t half_steps;      /* half-steps executed (L or W, one side)       */
+  uint64_t n_pred, n_verify, n_sweep, n_uncap, n_init, n_resolve; /* launches */
 } tbg_timing;
 int tbg_get_timing(const tbg_ctx *ctx, tbg_timing *t);
 int tbg_set_profiling(tbg_ctx *ctx, int on);

@@ -54,6 +54,8 @@ def lib():
         L.tbo_synth_hash.restype = ctypes.c_uint64
         L.tbo_synth_hash.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                      ctypes.c_uint64]
+        L.tbo_time_plies.argtypes = [ctypes.c_char_p, ctypes.c_int,
+                                     ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)]
         _lib = L
     return _lib
 
@@ -172,3 +174,14 @@ def kksquare(s):
 
 def synth_hash(seed, stm, kind, region, idx):
     return int(lib().tbo_synth_hash(seed, stm, kind, region, idx))
+
+
+def time_plies(material, max_plies):
+    """Seconds spent in the first max_plies plies of the ply loop and the
+    half-steps executed (the smaller materials are generated untimed)."""
+    sec = ctypes.c_double()
+    hs = ctypes.c_int64()
+    r = lib().tbo_time_plies(material.encode(), int(max_plies), ctypes.byref(sec), ctypes.byref(hs))
+    if r:
+        raise RuntimeError(f"oracle timing of {material} failed ({r})")
+    return sec.value, hs.value

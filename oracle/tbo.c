@@ -986,6 +986,9 @@ static int64_t count_bm(const Table *t, const uint64_t *bm)
 
 /* The generator proper: the stage order and ply loop of main()
  * (tb8gen.c:1390-1444) with calc_L_n / calc_W_n (tb8gen.c:943-1084). */
+static double g_loop_seconds;
+static int64_t g_half_steps;
+
 static void run_generation(Table *t, FILE *log, int max_plies)
 {
   uint64_t nw = t->synth ? t->words : t->words * NSLICES;
@@ -1023,6 +1026,8 @@ static void run_generation(Table *t, FILE *log, int max_plies)
 
   bool more_ww = true, more_wb = true, more_lw = true, more_lb = true;
   int n;
+  double t_loop0 = omp_get_wtime();
+  g_half_steps = 0;
   for (n = 1;; n++) {
     if (n > DRAW_RULE + 1 && !(more_ww || more_wb || more_lw || more_lb)) break;
     if (max_plies > 0 && n > max_plies) break;
@@ -1036,6 +1041,7 @@ static void run_generation(Table *t, FILE *log, int max_plies)
       memset(newL[stm], 0, nw * 8);
       if (!(forced || more_l[stm])) { next_w_other[stm] = false; continue; }
       memset(cand[stm], 0, nw * 8);
+      g_half_steps++;
       if (n == 1) {
         predecessors_sub(t, stm, t->sub[o][3], NULL, true, cand[stm]);
         if (t->synth)
@@ -1056,6 +1062,7 @@ static void run_generation(Table *t, FILE *log, int max_plies)
       memset(newW[stm], 0, nw * 8);
       if (!(forced || more_w[stm])) { next_l_other[stm] = false; continue; }
       memset(cand[stm], 0, nw * 8);
+      g_half_steps++;
       if (n == 1) {
         predecessors_sub(t, stm, t->sub[o][0], NULL, false, cand[stm]);
         if (t->synth)
@@ -1085,6 +1092,7 @@ static void run_generation(Table *t, FILE *log, int max_plies)
     }
   }
   t->nplies = n - 1;
+  g_loop_seconds = omp_get_wtime() - t_loop0;
   for (int c = 0; c < 2; c++) {
     free(wins[c]); free(prevW[c]); free(prevL[c]); free(newW[c]); free(newL[c]); free(cand[c]);
     free(kexW[c]); free(kexL[c]);
@@ -1255,3 +1263,37 @@ void tbo_lookup_bulk(int h, int stm, uint64_t n, const uint8_t *sq, uint16_t *ou
     out[i] = bad ? 0xffff : table_lookup(t, stm, p);
   }
 }
+
+/* CPU baseline sample: generate the smaller materials (untimed), then run
+ * the material's sub stage, init and at most max_plies plies; reports the
+ * seconds spent in the ply loop and the half-steps it executed.  The table
+ * is discarded. */
+int tbo_time_plies(const char *material, int max_plies, double *loop_seconds, int64_t *half_steps)
+{
+  init_tables();
+  Table *t = calloc(1, sizeof(Table));
+  if (!t) return -2;
+  if (setup_material(t, material)) { free(t); return -1; }
+  for (int k = 0; k < t->ii.numsets; k++) {
+    uint8_t spt[MAX_PIECES];
+    int n = 0;
+    for (int i = 0; i < t->num; i++)
+      if (i != t->ii.last[k]) spt[n++] = t->pt[i];
+    if (n == 2) continue;
+    char nm[16], fl[16], cn[16];
+    material_string(spt, n, false, nm);
+    material_string(spt, n, true, fl);
+    if (tbo_find(nm) >= 0 || tbo_find(fl) >= 0) continue;
+    canonical_name(nm, cn);
+    if (generate_rec(cn, NULL) < 0) { free(t); return -3; }
+  }
+  run_generation(t, NULL, max_plies);
+  if (loop_seconds) *loop_seconds = g_loop_seconds;
+  if (half_steps) *half_steps = g_half_steps;
+  for (int c = 0; c < 2; c++)
+    for (int q = 0; q < 4; q++) free(t->sub[c][q]);
+  free(t);
+  return 0;
+}
+
+int tbo_get_threads(void) { return g_threads; }

@@ -88,6 +88,10 @@ int tbo_synth_slice(const char *material, int s, uint64_t seed,
                     const char *log_path, int max_plies);
 uint64_t tbo_synth_hash(uint64_t seed, int stm, int kind, int region, uint64_t idx);
 
+/* CPU baseline sample (see tbo.c). */
+int tbo_time_plies(const char *material, int max_plies, double *loop_seconds, int64_t *half_steps);
+int tbo_get_threads(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -29,8 +29,9 @@ ABI_SYMBOLS = [
     "tbg_get_info", "tbg_resolve_captures", "tbg_init_illegal_mate", "tbg_step_L",
     "tbg_step_W", "tbg_step_L_plain", "tbg_step_W_plain", "tbg_end_ply", "tbg_generate",
     "tbg_get_codes", "tbg_get_bitmap", "tbg_get_sub_bitmap", "tbg_get_timing",
-    "tbg_set_profiling", "tbg_host_selftest",
+    "tbg_set_profiling", "tbg_host_selftest", "tbg_set_scatter",
 ]
+SCATTER_GLOBAL, SCATTER_BLOCK = 0, 1
 
 
 class TbgError(RuntimeError):
@@ -55,7 +56,10 @@ class Timing(ctypes.Structure):
                 ("ms_verify", ctypes.c_double), ("ms_sweep", ctypes.c_double),
                 ("ms_uncap", ctypes.c_double), ("ms_init", ctypes.c_double),
                 ("ms_resolve", ctypes.c_double), ("launches", ctypes.c_uint64),
-                ("half_steps", ctypes.c_uint64)]
+                ("half_steps", ctypes.c_uint64), ("n_pred", ctypes.c_uint64),
+                ("n_verify", ctypes.c_uint64), ("n_sweep", ctypes.c_uint64),
+                ("n_uncap", ctypes.c_uint64), ("n_init", ctypes.c_uint64),
+                ("n_resolve", ctypes.c_uint64)]
 
 
 LINE_FN = ctypes.CFUNCTYPE(None, ctypes.c_char_p, ctypes.c_void_p)
@@ -94,6 +98,7 @@ def lib():
                                          ctypes.POINTER(ctypes.c_uint64), ctypes.c_uint64]
         L.tbg_get_timing.argtypes = [vp, ctypes.POINTER(Timing)]
         L.tbg_set_profiling.argtypes = [vp, ctypes.c_int]
+        L.tbg_set_scatter.argtypes = [vp, ctypes.c_int]
         L.tbg_host_selftest.argtypes = [ctypes.c_char_p, ctypes.c_int]
         _lib = L
     return _lib
@@ -190,6 +195,9 @@ class Table:
                                        out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), w),
               "sub bitmap")
         return out
+
+    def set_scatter(self, mode):
+        check(lib().tbg_set_scatter(self.ptr, int(mode)), "scatter")
 
     def set_profiling(self, on):
         check(lib().tbg_set_profiling(self.ptr, 1 if on else 0), "profiling")

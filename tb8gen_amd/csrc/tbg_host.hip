@@ -227,7 +227,9 @@ struct tbg_ctx {
   bool prof = false;
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> evs;
   double ms[CAT_N];
+  uint64_t nl[CAT_N];
   uint64_t launches = 0, half_steps = 0;
+  int scatter = TBG_SCATTER_BLOCK;
 };
 
 namespace {
@@ -278,7 +280,7 @@ int ctx_alloc(tbg_ctx *c, bool with_codes)
   c->grid = g_cus * 16;
   manager_order(WHITE, c->mgr[WHITE]);
   manager_order(BLACK, c->mgr[BLACK]);
-  for (int i = 0; i < CAT_N; i++) c->ms[i] = 0;
+  for (int i = 0; i < CAT_N; i++) c->ms[i] = 0, c->nl[i] = 0;
   return TBG_OK;
 }
 
@@ -290,6 +292,7 @@ struct Timed {
   Timed(tbg_ctx *c_, int cat_) : c(c_), cat(cat_)
   {
     c->launches++;
+    c->nl[cat]++;
     if (c->prof) {
       (void)hipEventCreate(&a);
       (void)hipEventCreate(&b);
@@ -326,12 +329,41 @@ unsigned blocks_for(tbg_ctx *c, uint64_t waves)
   return (unsigned)(b ? b : 1);
 }
 
+// Block of the index a k_pred_blk work-group privatises in LDS: the whole
+// slice if it fits, else the run of positions sharing the high digits
+// 0..d-1 (factor[d-1] positions) for the smallest d that fits.
+constexpr uint64_t kBlkMaxPos = 40 * 1024 * 8 - 128;
+
+void choose_block(const Mat &M, uint64_t *blk_pos, uint64_t *nblk)
+{
+  if (M.ii.size <= kBlkMaxPos) { *blk_pos = M.ii.size; *nblk = 1; return; }
+  for (int d = 1; d <= M.ii.numsets; d++)
+    if (M.ii.factor[d - 1] <= kBlkMaxPos) {
+      *blk_pos = M.ii.factor[d - 1];
+      *nblk = M.ii.size / M.ii.factor[d - 1];
+      return;
+    }
+  *blk_pos = 1;
+  *nblk = M.ii.size;
+}
+
 int launch_pred(tbg_ctx *c, int stm, const uint64_t *src, int s0, int ns)
 {
-  uint64_t cps = (c->h.words + 63) / 64;
   Timed t(c, CAT_PRED);
-  hipLaunchKernelGGL(k_pred, dim3(blocks_for(c, ns * cps)), dim3(256), 0, c->st, c->dM, stm, src,
-                     c->cand[stm], s0, ns, cps);
+  if (c->scatter == TBG_SCATTER_GLOBAL) {
+    uint64_t cps = (c->h.words + 63) / 64;
+    hipLaunchKernelGGL(k_pred, dim3(blocks_for(c, ns * cps)), dim3(256), 0, c->st, c->dM, stm, src,
+                       c->cand[stm], s0, ns, cps);
+  } else {
+    uint64_t blk_pos, nblk;
+    choose_block(c->h, &blk_pos, &nblk);
+    uint64_t blk_words = (blk_pos + 63) / 64 + 1;
+    size_t lds = blk_words * 8 + PB_QUEUE_BYTES;
+    uint64_t nb = (uint64_t)ns * nblk;
+    uint64_t g = nb < (uint64_t)g_cus * 3 ? nb : (uint64_t)g_cus * 3;
+    hipLaunchKernelGGL(k_pred_blk, dim3((unsigned)g), dim3(PB_THREADS), lds, c->st, c->dM, stm,
+                       src, c->cand[stm], s0, ns, blk_pos, nblk, blk_words);
+  }
   HIPCHK(hipGetLastError());
   return TBG_OK;
 }
@@ -787,12 +819,20 @@ int tbg_get_sub_bitmap(tbg_ctx *c, int kind, int colour, int slice, uint64_t *ds
   return TBG_OK;
 }
 
+int tbg_set_scatter(tbg_ctx *c, int mode)
+{
+  if (!c) return fail(TBG_E_INVAL, "null context");
+  if (mode != TBG_SCATTER_GLOBAL && mode != TBG_SCATTER_BLOCK) return fail(TBG_E_INVAL, "bad scatter mode");
+  c->scatter = mode;
+  return TBG_OK;
+}
+
 int tbg_set_profiling(tbg_ctx *c, int on)
 {
   if (!c) return fail(TBG_E_INVAL, "null context");
   flush_timing(c);
   c->prof = on != 0;
-  for (int i = 0; i < CAT_N; i++) c->ms[i] = 0;
+  for (int i = 0; i < CAT_N; i++) c->ms[i] = 0, c->nl[i] = 0;
   c->launches = c->half_steps = 0;
   return TBG_OK;
 }
@@ -812,6 +852,12 @@ int tbg_get_timing(const tbg_ctx *cc, tbg_timing *t)
   for (int i = 0; i < CAT_N; i++) t->ms_total += c->ms[i];
   t->launches = c->launches;
   t->half_steps = c->half_steps;
+  t->n_pred = c->nl[CAT_PRED];
+  t->n_verify = c->nl[CAT_VERIFY];
+  t->n_sweep = c->nl[CAT_SWEEP];
+  t->n_uncap = c->nl[CAT_UNCAP];
+  t->n_init = c->nl[CAT_INIT];
+  t->n_resolve = c->nl[CAT_RESOLVE];
   return TBG_OK;
 }
 

@@ -17,6 +17,10 @@ struct SubRef {
 
 __global__ void k_pred(const Mat *Mp, int stm, const uint64_t *src, uint64_t *cand, int s0, int ns,
                        uint64_t cps);
+__global__ void k_pred_blk(const Mat *Mp, int stm, const uint64_t *src, uint64_t *cand, int s0,
+                           int ns, uint64_t blk_pos, uint64_t nblk, uint64_t blk_words);
+constexpr int PB_THREADS = 512;
+constexpr int PB_QUEUE_BYTES = 8 * 1024 * 2;
 __global__ void k_uncap(const Mat *Mp, int stm, int k, const uint64_t *src0, const uint64_t *src1,
                         int legality, uint64_t *cand, int s0, int ns, uint64_t cps);
 __global__ void k_verify(const Mat *Mp, int stm, int n, uint64_t *cand, const uint64_t *wins_o,

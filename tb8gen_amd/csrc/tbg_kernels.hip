@@ -156,6 +156,101 @@ __global__ __launch_bounds__(256) void k_pred(const Mat *__restrict__ Mp, int st
   }
 }
 
+// Block-privatised form of k_pred.  One work-group owns one block of the
+// index (positions [lo, hi) of slice s, a run of whole low-order digits, so
+// every unmove of a piece in the low sets stays inside it): those targets
+// are OR-ed into an LDS bitmap with LDS atomics, and only king unmoves and
+// unmoves of high-set pieces go to the global bitmap.  The LDS bitmap is
+// merged with one global atomic per nonzero word.  Device-scope atomics on
+// gfx950 travel to the memory side, so this cuts the scatter's HBM write
+// traffic by the ratio of same-block to all unmoves.
+constexpr int PB_WAVES = 8;       // waves per work-group
+constexpr int PB_CAP = 1024;      // queue entries per wave per round
+
+__global__ __launch_bounds__(512) void k_pred_blk(const Mat *__restrict__ Mp, int stm,
+                                                   const uint64_t *__restrict__ src, uint64_t *cand,
+                                                   int s0, int ns, uint64_t blk_pos,
+                                                   uint64_t nblk, uint64_t blk_words)
+{
+  __shared__ Tabs T;
+  __shared__ int rmax[PB_WAVES];
+  extern __shared__ uint64_t dyn[];
+  load_tabs(T);
+  const Mat &M = *Mp;
+  uint64_t *B = dyn;
+  uint16_t *Qall = reinterpret_cast<uint16_t *>(dyn + blk_words);
+  int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  uint16_t *q = Qall + wave * PB_CAP;
+  uint64_t nb = (uint64_t)ns * nblk;
+  for (uint64_t b = blockIdx.x; b < nb; b += gridDim.x) {
+    int s = s0 + (int)(b / nblk);
+    uint64_t lo = (b % nblk) * blk_pos;
+    uint64_t hi = lo + blk_pos < M.ii.size ? lo + blk_pos : M.ii.size;
+    uint64_t wlo = lo >> 6, nbw = ((hi + 63) >> 6) - wlo;
+    for (uint64_t w = threadIdx.x; w < nbw; w += blockDim.x) B[w] = 0;
+    __syncthreads();
+    const uint64_t *srcs = src + phys(M, s) * M.words;
+    uint64_t *own = cand + phys(M, s) * M.words;
+    for (uint64_t c0 = 0; c0 < nbw; c0 += (uint64_t)PB_WAVES * 64) {
+      uint64_t wi = c0 + (uint64_t)wave * 64 + lane;
+      uint64_t x = wi < nbw ? srcs[wlo + wi] : 0;
+      if (wi == 0) x &= ~0ull << (lo & 63);                      // not below the block
+      if (wi + 1 == nbw && (hi & 63)) x &= (1ull << (hi & 63)) - 1;  // nor above it
+      int total;
+      int off = wave_excl_scan(popc64(x), lane, &total);
+      // every wave runs the same number of rounds (they share barriers)
+      if (lane == 0) rmax[wave] = (total + PB_CAP - 1) / PB_CAP;
+      __syncthreads();
+      int allr = 0;
+      for (int v = 0; v < PB_WAVES; v++) allr = rmax[v] > allr ? rmax[v] : allr;
+      __syncthreads();
+      for (int r = 0; r < allr; r++) {
+        int base = r * PB_CAP;
+        uint64_t y = x;
+        int k = off;
+        while (y && k < base + PB_CAP) {
+          int bt = ctz64(y);
+          y &= y - 1;
+          if (k >= base) q[k - base] = (uint16_t)((lane << 6) | bt);
+          k++;
+        }
+        __syncthreads();
+        int cnt = total - base;
+        cnt = cnt < 0 ? 0 : cnt > PB_CAP ? PB_CAP : cnt;
+        for (int e = lane; e < cnt; e += 64) {
+          uint16_t v = q[e];
+          uint64_t idx = (wlo + c0 + (uint64_t)wave * 64 + (v >> 6)) * 64 + (v & 63);
+          uint64_t p = kings_of(T, s);
+          uint64_t occ = idx_to_sq(T, M.ii, idx, &p);
+          mark_king_unmoves(T, M, stm, occ, p, s, cand);
+          for (int t = 0; t < MAXP - 1; t++) {
+            if (t >= M.nk[stm]) break;
+            int j = M.pcs[stm][t];
+            uint64_t bb = piece_att(T, M.pt[j], sq_get(p, j), occ) & ~occ;
+            while (bb) {
+              int tq = ctz64(bb);
+              bb &= bb - 1;
+              uint64_t i2 = sq_to_idx(T, M.ii, sq_set(p, j, tq));
+              if (i2 >= lo && i2 < hi)
+                __hip_atomic_fetch_or(reinterpret_cast<unsigned long long *>(&B[(i2 >> 6) - wlo]),
+                                      (unsigned long long)bit(i2 & 63), __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_WORKGROUP);
+              else
+                atomic_or64(own + (i2 >> 6), bit(i2 & 63));
+            }
+          }
+        }
+        __syncthreads();
+      }
+    }
+    for (uint64_t w = threadIdx.x; w < nbw; w += blockDim.x) {
+      uint64_t v = B[w];
+      if (v) atomic_or64(own + wlo + w, v);
+    }
+    __syncthreads();
+  }
+}
+
 // predecessors_sub_worker (src/tb8gen.c:346-394): uncapture from the
 // stm^1 capture-exit bitmap of set k (src0 & ~src1 when src1 is given).
 __global__ __launch_bounds__(256) void k_uncap(const Mat *__restrict__ Mp, int stm, int k,

@@ -1,0 +1,74 @@
+"""The C ABI library: it loads, exports every entry point of include/tbg.h,
+its host-compiled primitives pass their self-test, and its error paths are
+reachable from the host without a GPU."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+import tb8gen_amd as tbg
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "tbg.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(tbg_[A-Za-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = tbg.lib()
+    names = declared_functions()
+    assert len(names) >= 19
+    for n in names:
+        assert hasattr(L, n), n
+    assert sorted(tbg.ABI_SYMBOLS) == names
+
+
+def test_host_selftest_of_device_primitives():
+    r, msg = tbg.host_selftest()
+    assert r == 0, msg
+
+
+def test_error_codes_have_messages():
+    L = tbg.lib()
+    for code in (0, -1, -2, -3, -4, -5, -6, -7):
+        assert L.tbg_strerror(code)
+
+
+def test_calls_before_init_are_rejected():
+    L = tbg.lib()
+    if tbg._inited:
+        pytest.skip("library already initialised in this process")
+    p = ctypes.c_void_p()
+    assert L.tbg_create(ctypes.byref(p), b"KRvK") == -5  # TBG_E_STATE
+    assert L.tbg_get_info(None, None) == -1
+
+
+def test_init_without_device_fails_loudly():
+    r = subprocess.run(["rocminfo"], capture_output=True, text=True) if os.path.exists(
+        "/opt/rocm/bin/rocminfo") else None
+    has_gpu = r is not None and "gfx9" in (r.stdout or "")
+    if has_gpu:
+        pytest.skip("a GPU is present")
+    assert tbg.lib().tbg_init(0) == -7  # TBG_E_NODEV
+    with pytest.raises(tbg.TbgError):
+        tbg.Table("KRvK")
+
+
+def test_driver_binary_usage():
+    exe = tbg.DRIVER_PATH
+    assert os.path.exists(exe)
+    r = subprocess.run([exe], capture_output=True, text=True)
+    assert r.returncode != 0 and "No tablebase specified" in r.stdout
+
+
+def test_material_helpers():
+    assert tbg.canonical("KvKR") == "KRvK"
+    assert tbg.canonical("KRvKQ") == "KQvKR"
+    assert tbg.canonical("KNvKBN") == "KBNvKN"
+    assert tbg.sub_materials("KQRvKR") == ["KRvKR", "KQvKR", "KQRvK"]
+    assert tbg.sub_materials("KRvK") == []

@@ -1,0 +1,145 @@
+"""Closed forms, invariants and known answers that pin the oracle."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+O.set_threads(min(8, os.cpu_count() or 1))
+
+# Entries of the reference's KKIdx table (src/probe.c:441-522): the slice
+# numbering, including the 21 slices with both kings on a1-h8 (441..461).
+KKIDX_VECTORS = [
+    (0, 3, 0), (0, 63, 57), (1, 0, 58), (1, 4, 59), (2, 0, 116), (2, 60, 170),
+    (3, 0, 174), (4, 0, 229), (5, 0, 284), (5, 63, 338), (6, 2, 339), (6, 18, 441),
+    (6, 63, 446), (7, 3, 366), (7, 27, 447), (8, 0, 452), (8, 1, 391), (8, 36, 453),
+    (9, 0, 457), (9, 9, 458), (9, 63, 461), (9, 55, 440), (9, 62, -1), (0, 0, -1), (6, 8, -1),
+]
+
+
+@pytest.mark.parametrize("tri,bk,want", KKIDX_VECTORS)
+def test_kkidx_vectors(tri, bk, want):
+    assert O.kkidx(tri, bk) == want
+
+
+def test_slices_partition_king_pairs():
+    """462 classes of legal king pairs; 8-fold symmetric; diagonal slices."""
+    seen = np.zeros(462, dtype=np.int64)
+    for wk in range(64):
+        for bk in range(64):
+            s = O.kkmap(wk, bk)
+            touching = max(abs((wk & 7) - (bk & 7)), abs((wk >> 3) - (bk >> 3))) <= 1
+            if touching:
+                assert s == -1
+                continue
+            seen[s] += 1
+            f = lambda x: (x >> 3) * 8 + (7 - (x & 7))          # file mirror
+            r = lambda x: ((7 - (x >> 3)) * 8) + (x & 7)        # rank mirror
+            d = lambda x: ((x & 7) << 3) | (x >> 3)              # a1-h8 flip
+            for g in (f, r, d, lambda x: f(r(x)), lambda x: d(f(x))):
+                assert O.kkmap(g(wk), g(bk)) == s
+    assert (seen > 0).all()
+    assert seen.sum() == 64 * 63 - 420  # 3612 legal king pairs
+    for s in range(462):
+        wk, bk = O.kksquare(s)
+        both_diag = (wk & 7) == (wk >> 3) and (bk & 7) == (bk >> 3)
+        assert both_diag == (s >= 441)
+        assert O.kkmap(wk, bk) == s
+
+
+def test_index_is_a_bijection():
+    t = O.Table("KRvKN")
+    assert t.size == 62 * 61
+    for s in (0, 170, 441):
+        seen = set()
+        for idx in range(t.size):
+            sq = t.idx_to_sq(s, idx)
+            assert len(set(sq)) == len(sq)
+            assert t.sq_to_idx(sq) == idx
+            seen.add(tuple(sq))
+        assert len(seen) == t.size
+
+
+def test_index_sizes_are_binomial_products():
+    for material, size in [("KRRvK", math.comb(62, 2)), ("KQRvKR", 62 * 61 * 60),
+                           ("KBNvKQ", 62 * 61 * 60)]:
+        t = O.Table(material)
+        assert t.size == size
+
+
+def _max_n(table, stm, tag):
+    c = table.codes(stm).ravel()
+    sel = c[(c & 3) == tag]
+    return int(sel.max() >> 2) if sel.size else 0
+
+
+def test_known_answers_three_piece():
+    """KQvK mate in 10 and KRvK mate in 16 (DTZ = DTM in plies here);
+    KBvK and KNvK hold no win at all."""
+    q, r = O.Table("KQvK"), O.Table("KRvK")
+    assert _max_n(q, 0, 1) == 19 and _max_n(q, 1, 2) == 20
+    assert _max_n(r, 0, 1) == 31 and _max_n(r, 1, 2) == 32
+    for m in ("KBvK", "KNvK"):
+        t = O.Table(m)
+        for stm in range(2):
+            c = t.codes(stm)
+            assert not ((c & 3) == 1).any() and not ((c & 3) == 2).any()
+
+
+def test_known_answer_kbnk():
+    """KBNvK: the longest win is mate in 33 (65 plies)."""
+    t = O.Table("KBNvK")
+    assert _max_n(t, 0, 1) == 65
+
+
+def test_count_lines_format_and_totals(tmp_path):
+    log = tmp_path / "krvk.log"
+    t = O.Table("KRvK", log_path=str(log)) if O.lib().tbo_find(b"KRvK") < 0 else None
+    if t is None:
+        O.free_all()
+        t = O.Table("KRvK", log_path=str(log))
+    lines = log.read_text().split("\n")
+    assert lines[0].startswith("broken_w = ") and lines[3].startswith("l0_b = ")
+    # every W_n / L_n line equals the number of codes with that value
+    for stm, c in ((0, "w"), (1, "b")):
+        codes = t.codes(stm).ravel()
+        for line in lines[4:]:
+            if not line:
+                continue
+            key, val = line.split(" = ")
+            kind, rest = key[0], key[1:]
+            n, side = rest.split("_")
+            if side != c:
+                continue
+            tag = 2 if kind == "l" else 1
+            assert int(val) == int(((codes >> 2) == int(n))[(codes & 3) == tag].sum())
+
+
+def test_reference_king_capture_switch_is_inert_on_distinct_pieces():
+    """The king-capture index variant only matters for duplicated sets."""
+    O.free_all()
+    base = [O.Table("KQvKR").codes(s).copy() for s in range(2)]
+    O.free_all()
+    O.set_ref_king_capture_index(True)
+    try:
+        alt = [O.Table("KQvKR").codes(s).copy() for s in range(2)]
+    finally:
+        O.set_ref_king_capture_index(False)
+        O.free_all()
+    for s in range(2):
+        assert np.array_equal(base[s], alt[s])
+
+
+def test_synthetic_slice_deterministic_and_seeded():
+    a = O.Table("KQvKR", synth_slice=170, seed=7)
+    b = O.Table("KQvKR", synth_slice=170, seed=7)
+    c = O.Table("KQvKR", synth_slice=170, seed=8)
+    for stm in range(2):
+        assert np.array_equal(a.codes(stm), b.codes(stm))
+    assert not np.array_equal(a.codes(0), c.codes(0))
+    # the exits are Bernoulli(0.03) wins and losses
+    h = np.array([O.synth_hash(7, 0, 0, 0, i) for i in range(20000)], dtype=np.uint64)
+    win = (h < np.uint64(553402322211286548)).mean()
+    assert 0.02 < win < 0.04
