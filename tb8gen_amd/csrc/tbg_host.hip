@@ -353,7 +353,7 @@ int launch_pred(tbg_ctx *c, int stm, const uint64_t *src, int s0, int ns)
   if (c->scatter == TBG_SCATTER_GLOBAL) {
     uint64_t cps = (c->h.words + 63) / 64;
     hipLaunchKernelGGL(k_pred, dim3(blocks_for(c, ns * cps)), dim3(256), 0, c->st, c->dM, stm, src,
-                       c->cand[stm], s0, ns, cps);
+                       c->cand[stm], c->wins[stm], s0, ns, cps);
   } else {
     uint64_t blk_pos, nblk;
     choose_block(c->h, &blk_pos, &nblk);
@@ -362,7 +362,7 @@ int launch_pred(tbg_ctx *c, int stm, const uint64_t *src, int s0, int ns)
     uint64_t nb = (uint64_t)ns * nblk;
     uint64_t g = nb < (uint64_t)g_cus * 3 ? nb : (uint64_t)g_cus * 3;
     hipLaunchKernelGGL(k_pred_blk, dim3((unsigned)g), dim3(PB_THREADS), lds, c->st, c->dM, stm,
-                       src, c->cand[stm], s0, ns, blk_pos, nblk, blk_words);
+                       src, c->cand[stm], c->wins[stm], s0, ns, blk_pos, nblk, blk_words);
   }
   HIPCHK(hipGetLastError());
   return TBG_OK;
@@ -377,7 +377,7 @@ int launch_uncap(tbg_ctx *c, int stm, int kind, int kind2, int legality, int s0,
     Timed t(c, CAT_UNCAP);
     hipLaunchKernelGGL(k_uncap, dim3(blocks_for(c, ns * cps)), dim3(256), 0, c->st, c->dM, stm, k,
                        c->sub[o][kind], kind2 >= 0 ? c->sub[o][kind2] : nullptr, legality,
-                       c->cand[stm], s0, ns, cps);
+                       c->cand[stm], c->wins[stm], s0, ns, cps);
     HIPCHK(hipGetLastError());
   }
   return TBG_OK;
@@ -390,7 +390,7 @@ int launch_verify(tbg_ctx *c, int stm, int n, int s0, int ns)
   Timed t(c, CAT_VERIFY);
   hipLaunchKernelGGL(k_verify, dim3(blocks_for(c, ns * cps)), dim3(256), 0, c->st, c->dM, stm, n,
                      c->cand[stm], c->wins[o], c->sub[o][n <= DRAW_RULE ? TBG_SUB_WIN : TBG_SUB_CWIN],
-                     c->newL[stm], c->codes[stm], s0, ns, cps, c->dcnt);
+                     c->wins[stm], c->newL[stm], c->codes[stm], s0, ns, cps, c->dcnt);
   HIPCHK(hipGetLastError());
   return TBG_OK;
 }

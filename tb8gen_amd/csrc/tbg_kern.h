@@ -15,17 +15,19 @@ struct SubRef {
   int present;               // 0: a bare pair of kings (always a draw)
 };
 
-__global__ void k_pred(const Mat *Mp, int stm, const uint64_t *src, uint64_t *cand, int s0, int ns,
-                       uint64_t cps);
-__global__ void k_pred_blk(const Mat *Mp, int stm, const uint64_t *src, uint64_t *cand, int s0,
-                           int ns, uint64_t blk_pos, uint64_t nblk, uint64_t blk_words);
+__global__ void k_pred(const Mat *Mp, int stm, const uint64_t *src, uint64_t *cand,
+                       const uint64_t *won, int s0, int ns, uint64_t cps);
+__global__ void k_pred_blk(const Mat *Mp, int stm, const uint64_t *src, uint64_t *cand,
+                           const uint64_t *won, int s0, int ns, uint64_t blk_pos, uint64_t nblk,
+                           uint64_t blk_words);
 constexpr int PB_THREADS = 512;
 constexpr int PB_QUEUE_BYTES = 8 * 1024 * 2;
 __global__ void k_uncap(const Mat *Mp, int stm, int k, const uint64_t *src0, const uint64_t *src1,
-                        int legality, uint64_t *cand, int s0, int ns, uint64_t cps);
+                        int legality, uint64_t *cand, const uint64_t *won, int s0, int ns,
+                        uint64_t cps);
 __global__ void k_verify(const Mat *Mp, int stm, int n, uint64_t *cand, const uint64_t *wins_o,
-                         const uint64_t *sub_o, uint64_t *newL, uint16_t *codes, int s0, int ns,
-                         uint64_t cps, unsigned long long *cnt);
+                         const uint64_t *sub_o, const uint64_t *won, uint64_t *newL,
+                         uint16_t *codes, int s0, int ns, uint64_t cps, unsigned long long *cnt);
 __global__ void k_wsweep(uint64_t *cand, uint64_t *wins, uint64_t *newW, uint16_t *codes,
                          uint64_t nw, uint64_t words, uint64_t size, uint64_t w0, int n,
                          unsigned long long *cnt);

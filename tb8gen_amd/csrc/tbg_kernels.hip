@@ -65,8 +65,11 @@ __device__ __forceinline__ uint64_t phys(const Mat &M, int s) { return M.synth ?
 
 // ------------------------------------------------------------- unmoves
 // mark_king_unmoves (src/tb8gen.c:246-262)
+// Targets already in wins(stm) (illegal or won faster) can never become new
+// W_n or L_n positions (W_n = C & ~wins; a won position fails verification),
+// so they are tested with a cached read before the memory-side atomic.
 __device__ void mark_king_unmoves(const Tabs &T, const Mat &M, int stm, uint64_t occ, uint64_t p,
-                                  int s, uint64_t *cand)
+                                  int s, uint64_t *cand, const uint64_t *__restrict__ won)
 {
   uint64_t b = T.king[sq_get(p, stm)] & ~(occ | T.king[sq_get(p, stm ^ 1)]);
   while (b) {
@@ -76,11 +79,12 @@ __device__ void mark_king_unmoves(const Tabs &T, const Mat &M, int stm, uint64_t
     int s2 = normalize(T, sq_set(p, stm, t), &q);
     if (M.synth && s2 != M.synth_slice) continue;  // leaves a single-slice job
     uint64_t *base = cand + phys(M, s2) * M.words;
+    const uint64_t *wb = won + phys(M, s2) * M.words;
     uint64_t i2 = sq_to_idx(T, M.ii, q);
-    atomic_or64(base + (i2 >> 6), bit(i2 & 63));
+    if (!bm_test(wb, i2)) atomic_or64(base + (i2 >> 6), bit(i2 & 63));
     if (s < 441 && s2 >= 441) {
       i2 = sq_to_idx(T, M.ii, mirror_nonkings(q));
-      atomic_or64(base + (i2 >> 6), bit(i2 & 63));
+      if (!bm_test(wb, i2)) atomic_or64(base + (i2 >> 6), bit(i2 & 63));
     }
   }
 }
@@ -88,14 +92,15 @@ __device__ void mark_king_unmoves(const Tabs &T, const Mat &M, int stm, uint64_t
 // mark_unmoves (src/tb8gen.c:264-276) of the non-king piece in slot j; the
 // unmoved position is q with slot j replaced.
 __device__ __forceinline__ void mark_unmoves(const Tabs &T, const Mat &M, int j, uint64_t occ,
-                                             uint64_t q, uint64_t *base)
+                                             uint64_t q, uint64_t *base,
+                                             const uint64_t *__restrict__ wb)
 {
   uint64_t b = piece_att(T, M.pt[j], sq_get(q, j), occ) & ~occ;
   while (b) {
     int t = ctz64(b);
     b &= b - 1;
     uint64_t i2 = sq_to_idx(T, M.ii, sq_set(q, j, t));
-    atomic_or64(base + (i2 >> 6), bit(i2 & 63));
+    if (!bm_test(wb, i2)) atomic_or64(base + (i2 >> 6), bit(i2 & 63));
   }
 }
 
@@ -122,7 +127,8 @@ __device__ __forceinline__ int fill_queue(uint16_t *q, uint64_t x, int lane)
 // frontier `src` -> stm unmoves OR-ed into `cand`.
 __global__ __launch_bounds__(256) void k_pred(const Mat *__restrict__ Mp, int stm,
                                                const uint64_t *__restrict__ src, uint64_t *cand,
-                                               int s0, int ns, uint64_t cps)
+                                               const uint64_t *__restrict__ won, int s0, int ns,
+                                               uint64_t cps)
 {
   __shared__ Tabs T;
   __shared__ Queue Q;
@@ -139,6 +145,7 @@ __global__ __launch_bounds__(256) void k_pred(const Mat *__restrict__ Mp, int st
     int total = fill_queue(Q.e[wave], x, lane);
     __syncthreads();
     uint64_t *own = cand + phys(M, s) * M.words;
+    const uint64_t *ownw = won + phys(M, s) * M.words;
     uint64_t wbase = (g % cps) * 64;
     for (int e = lane; e < total; e += 64) {
       uint16_t v = Q.e[wave][e];
@@ -146,10 +153,10 @@ __global__ __launch_bounds__(256) void k_pred(const Mat *__restrict__ Mp, int st
       if (idx >= M.ii.size) continue;
       uint64_t p = kings_of(T, s);
       uint64_t occ = idx_to_sq(T, M.ii, idx, &p);
-      mark_king_unmoves(T, M, stm, occ, p, s, cand);
+      mark_king_unmoves(T, M, stm, occ, p, s, cand, won);
       for (int t = 0; t < MAXP - 1; t++) {
         if (t >= M.nk[stm]) break;
-        mark_unmoves(T, M, M.pcs[stm][t], occ, p, own);
+        mark_unmoves(T, M, M.pcs[stm][t], occ, p, own, ownw);
       }
     }
     __syncthreads();
@@ -169,8 +176,9 @@ constexpr int PB_CAP = 1024;      // queue entries per wave per round
 
 __global__ __launch_bounds__(512) void k_pred_blk(const Mat *__restrict__ Mp, int stm,
                                                    const uint64_t *__restrict__ src, uint64_t *cand,
-                                                   int s0, int ns, uint64_t blk_pos,
-                                                   uint64_t nblk, uint64_t blk_words)
+                                                   const uint64_t *__restrict__ won, int s0, int ns,
+                                                   uint64_t blk_pos, uint64_t nblk,
+                                                   uint64_t blk_words)
 {
   __shared__ Tabs T;
   __shared__ int rmax[PB_WAVES];
@@ -191,6 +199,7 @@ __global__ __launch_bounds__(512) void k_pred_blk(const Mat *__restrict__ Mp, in
     __syncthreads();
     const uint64_t *srcs = src + phys(M, s) * M.words;
     uint64_t *own = cand + phys(M, s) * M.words;
+    const uint64_t *ownw = won + phys(M, s) * M.words;
     for (uint64_t c0 = 0; c0 < nbw; c0 += (uint64_t)PB_WAVES * 64) {
       uint64_t wi = c0 + (uint64_t)wave * 64 + lane;
       uint64_t x = wi < nbw ? srcs[wlo + wi] : 0;
@@ -222,7 +231,7 @@ __global__ __launch_bounds__(512) void k_pred_blk(const Mat *__restrict__ Mp, in
           uint64_t idx = (wlo + c0 + (uint64_t)wave * 64 + (v >> 6)) * 64 + (v & 63);
           uint64_t p = kings_of(T, s);
           uint64_t occ = idx_to_sq(T, M.ii, idx, &p);
-          mark_king_unmoves(T, M, stm, occ, p, s, cand);
+          mark_king_unmoves(T, M, stm, occ, p, s, cand, won);
           for (int t = 0; t < MAXP - 1; t++) {
             if (t >= M.nk[stm]) break;
             int j = M.pcs[stm][t];
@@ -235,7 +244,7 @@ __global__ __launch_bounds__(512) void k_pred_blk(const Mat *__restrict__ Mp, in
                 __hip_atomic_fetch_or(reinterpret_cast<unsigned long long *>(&B[(i2 >> 6) - wlo]),
                                       (unsigned long long)bit(i2 & 63), __ATOMIC_RELAXED,
                                       __HIP_MEMORY_SCOPE_WORKGROUP);
-              else
+              else if (!bm_test(ownw, i2))
                 atomic_or64(own + (i2 >> 6), bit(i2 & 63));
             }
           }
@@ -256,7 +265,8 @@ __global__ __launch_bounds__(512) void k_pred_blk(const Mat *__restrict__ Mp, in
 __global__ __launch_bounds__(256) void k_uncap(const Mat *__restrict__ Mp, int stm, int k,
                                                 const uint64_t *__restrict__ src0,
                                                 const uint64_t *__restrict__ src1, int legality,
-                                                uint64_t *cand, int s0, int ns, uint64_t cps)
+                                                uint64_t *cand, const uint64_t *__restrict__ won,
+                                                int s0, int ns, uint64_t cps)
 {
   __shared__ Tabs T;
   __shared__ Queue Q;
@@ -279,6 +289,7 @@ __global__ __launch_bounds__(256) void k_uncap(const Mat *__restrict__ Mp, int s
     int total = fill_queue(Q.e[wave], x, lane);
     __syncthreads();
     uint64_t *own = cand + phys(M, s) * M.words;
+    const uint64_t *ownw = won + phys(M, s) * M.words;
     uint64_t wbase = (g % cps) * 64;
     for (int e = lane; e < total; e += 64) {
       uint16_t v = Q.e[wave][e];
@@ -288,12 +299,12 @@ __global__ __launch_bounds__(256) void k_uncap(const Mat *__restrict__ Mp, int s
       uint64_t occ = idx_to_sq(T, ci, idx, &p);
       if (legality && attacked_by(T, M, p, occ, sq_get(p, stm), o, m)) continue;
       // uncapture by the king
-      mark_king_unmoves(T, M, stm, occ, sq_set(p, m, sq_get(p, stm)), s, cand);
+      mark_king_unmoves(T, M, stm, occ, sq_set(p, m, sq_get(p, stm)), s, cand, won);
       // uncapture by the non-king pieces
       for (int t = 0; t < MAXP - 1; t++) {
         if (t >= M.nk[stm]) break;
         int j = M.pcs[stm][t];
-        mark_unmoves(T, M, j, occ, sq_set(p, m, sq_get(p, j)), own);
+        mark_unmoves(T, M, j, occ, sq_set(p, m, sq_get(p, j)), own, ownw);
       }
     }
     __syncthreads();
@@ -408,7 +419,8 @@ __device__ bool verify_pos(const Tabs &T, const Mat &M, int stm, int s, uint64_t
 // with the count and, if codes != null, the L_n codes.
 __global__ __launch_bounds__(256) void k_verify(const Mat *__restrict__ Mp, int stm, int n,
                                                  uint64_t *cand, const uint64_t *__restrict__ wins_o,
-                                                 const uint64_t *__restrict__ sub_o, uint64_t *newL,
+                                                 const uint64_t *__restrict__ sub_o,
+                                                 const uint64_t *__restrict__ won, uint64_t *newL,
                                                  uint16_t *codes, int s0, int ns, uint64_t cps,
                                                  unsigned long long *cnt)
 {
@@ -425,7 +437,10 @@ __global__ __launch_bounds__(256) void k_verify(const Mat *__restrict__ Mp, int 
     uint64_t w = (g % cps) * 64 + lane;
     bool live = g < nchunks && w < M.words;
     uint64_t a = phys(M, s) * M.words + w;
-    uint64_t x = live ? cand[a] : 0;
+    uint64_t x0 = live ? cand[a] : 0;
+    // a position already in wins(stm) can not be lost (and illegal ones are
+    // not positions): only the rest is verified
+    uint64_t x = x0 ? x0 & ~won[a] : 0;
     Q.res[wave][lane] = x;
     int total = fill_queue(Q.e[wave], x, lane);
     __syncthreads();
@@ -443,7 +458,7 @@ __global__ __launch_bounds__(256) void k_verify(const Mat *__restrict__ Mp, int 
     if (live) {
       uint64_t r = Q.res[wave][lane];
       newL[a] = r;
-      if (x) cand[a] = 0;
+      if (x0) cand[a] = 0;
       mycnt += popc64(r);
       if (codes && r) {
         uint16_t *c = codes + phys(M, s) * M.ii.size + w * 64;
