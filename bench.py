@@ -53,6 +53,70 @@ def run_steps(table, subs, form, steps, tbg):
     return secs, timing, lines
 
 
+def run_synth(args, tbg):
+    """One king slice at real shape with seeded Bernoulli(0.03) win / loss
+    exits for captures and king moves; one step = capture exits + init + the
+    ply loop to the fixpoint."""
+    material = args.material if args.material != "KQRvKR" else "KQRBvKQRN"
+    t0 = time.perf_counter()
+    table = tbg.Table(material, synth_slice=args.slice, seed=args.seed)
+    table.set_scatter(tbg.SCATTER_BLOCK if args.scatter == "block" else tbg.SCATTER_GLOBAL)
+    alloc_s = time.perf_counter() - t0
+    n = table.size
+    print(f"synth {material} slice {args.slice}: {n} positions per side, "
+          f"{table.info.device_bytes / 2**30:.1f} GiB resident, alloc {alloc_s:.1f} s",
+          file=sys.stderr, flush=True)
+    secs = []
+    table.set_profiling(True)
+    for step in range(args.warmup + args.steps):
+        t0 = time.perf_counter()
+
+        def progress(line, _u, t0=t0):
+            print(f"  [{time.perf_counter() - t0:8.2f} s] {line.decode()}", file=sys.stderr,
+                  flush=True)
+
+        fn = tbg.LINE_FN(progress)
+        table.resolve([])
+        tbg.check(tbg.lib().tbg_generate(table.ptr, tbg.FORM_BATCHED, fn, None,
+                                         tbg.ctypes.byref(table.counts)), "generate")
+        dt = time.perf_counter() - t0
+        if step == args.warmup - 1:
+            table.set_profiling(False)
+            table.set_profiling(True)
+        if step >= args.warmup:
+            secs.append(dt)
+    tm = table.timing()
+    steps = max(1, args.steps)
+    half_steps = tm.half_steps / steps
+    psp = half_steps * n / 2.0
+    cats = {"pred": (tm.ms_pred, tm.n_pred), "verify": (tm.ms_verify, tm.n_verify),
+            "sweep": (tm.ms_sweep, tm.n_sweep)}
+    dom = max(cats, key=lambda k: cats[k][0])
+    avg_ms = cats[dom][0] / max(1, cats[dom][1])
+    alg = BITS[dom] * n / 8.0
+    achieved = alg / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    sweep_ms = tm.ms_sweep / max(1, tm.n_sweep)
+    sweep_gbs = BITS["sweep"] * n / 8.0 / (sweep_ms * 1e-3) / 1e9 if sweep_ms > 0 else 0.0
+    out = {
+        "metric": METRIC, "value": psp / (sum(secs) / len(secs)),
+        "unit": "position-side-plies/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1000 * max(secs), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u64", "data": "synthetic",
+        "config": {"workload": f"{material} single king slice {args.slice} at real shape, "
+                               f"seeded hash exits (seed {args.seed}), full fixpoint",
+                   "material": material, "slice": args.slice, "positions_per_side": n,
+                   "half_steps_per_step": half_steps, "plies": table.counts.nplies},
+        "roofline": {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "alg_bytes_per_launch": alg, "avg_launch_ms": avg_ms},
+        "sweep_roofline": {"achieved": sweep_gbs, "frac": sweep_gbs / HBM_PEAK_GBS,
+                           "avg_launch_ms": sweep_ms},
+        "kernels_ms_per_step": {k: getattr(tm, "ms_" + k) / steps for k in
+                                ("pred", "verify", "sweep", "uncap", "init", "resolve")},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -64,6 +128,11 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-form comparison")
     ap.add_argument("--cpu-plies", type=int, default=2)
     ap.add_argument("--scatter", default="block", choices=["block", "global"])
+    ap.add_argument("--workload", default="table", choices=["table", "synth"],
+                    help="table: a whole material (default KQRvKR); synth: one king slice of "
+                         "--material with seeded hash exits (default KQRBvKQRN slice 170)")
+    ap.add_argument("--slice", type=int, default=170)
+    ap.add_argument("--seed", type=int, default=20261203)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters against known bytes)")
@@ -89,6 +158,8 @@ def main():
     import tb8gen_amd as tbg
 
     tbg.init(0)
+    if args.workload == "synth":
+        return run_synth(args, tbg)
     form = tbg.FORM_BATCHED if args.form == "batched" else tbg.FORM_PLAIN
     reg = tbg.Registry(tbg.FORM_BATCHED)
     for sub in tbg.sub_materials(args.material):

@@ -408,11 +408,17 @@ int launch_wsweep(tbg_ctx *c, int stm, int n, uint64_t w0, uint64_t nw)
   return TBG_OK;
 }
 
+// Synthetic jobs: the ply-1 king-move exit candidates were computed by
+// k_init into planes that ply 1 does not otherwise read: L_1 candidates of
+// stm in newW[stm], W_1 candidates in prevW[stm].  which 0: L, 1: W.
 int launch_exits(tbg_ctx *c, int stm, int which)
 {
   Timed t(c, CAT_UNCAP);
-  hipLaunchKernelGGL(k_synth_exits, dim3(blocks_for(c, c->h.words)), dim3(256), 0, c->st, c->dM,
-                     stm, which, c->cand[stm]);
+  const uint64_t *src = which == 0 ? c->newW[stm] : c->prevW[stm];
+  uint64_t b = (c->pw / 2 + 255) / 256;
+  if (b > (uint64_t)g_cus * 8) b = (uint64_t)g_cus * 8;
+  hipLaunchKernelGGL(k_or, dim3((unsigned)(b ? b : 1)), dim3(256), 0, c->st, c->cand[stm], src,
+                     c->pw);
   HIPCHK(hipGetLastError());
   return TBG_OK;
 }
@@ -669,7 +675,8 @@ int tbg_init_illegal_mate(tbg_ctx *c, tbg_counts *counts)
     Timed t(c, CAT_INIT);
     hipLaunchKernelGGL(k_init, dim3(blocks_for(c, (uint64_t)ns * M.words)), dim3(256), 0, c->st,
                        c->dM, c->wins[0], c->wins[1], c->prevL[0], c->prevL[1], c->codes[0],
-                       c->codes[1], s0, ns, c->dcnt);
+                       c->codes[1], c->prevW[0], c->prevW[1], c->newW[0], c->newW[1], s0, ns,
+                       c->dcnt);
     HIPCHK(hipGetLastError());
   }
   unsigned long long v[4];

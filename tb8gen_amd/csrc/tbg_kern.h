@@ -32,11 +32,12 @@ __global__ void k_wsweep(uint64_t *cand, uint64_t *wins, uint64_t *newW, uint16_
                          uint64_t nw, uint64_t words, uint64_t size, uint64_t w0, int n,
                          unsigned long long *cnt);
 __global__ void k_init(const Mat *Mp, uint64_t *winsW, uint64_t *winsB, uint64_t *mateW,
-                       uint64_t *mateB, uint16_t *codesW, uint16_t *codesB, int s0, int ns,
+                       uint64_t *mateB, uint16_t *codesW, uint16_t *codesB, uint64_t *kexW0,
+                       uint64_t *kexW1, uint64_t *kexL0, uint64_t *kexL1, int s0, int ns,
                        unsigned long long *cnt);
 __global__ void k_resolve(const Mat *Mp, const SubRef *Rp, int c, int k, uint64_t *loss,
                           uint64_t *bloss, uint64_t *cwin, uint64_t *win, int s0, int ns);
-__global__ void k_synth_exits(const Mat *Mp, int stm, int which, uint64_t *cand);
+__global__ void k_or(uint64_t *dst, const uint64_t *src, uint64_t nw);
 
 // copies the shared tables to the device (defined with the kernels)
 hipError_t upload_tabs(const Tabs *h);

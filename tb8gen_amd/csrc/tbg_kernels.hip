@@ -520,6 +520,62 @@ __global__ __launch_bounds__(256) void k_wsweep(uint64_t *cand, uint64_t *wins, 
   if (lane == 0 && mycnt) atomicAdd(cnt, (unsigned long long)mycnt);
 }
 
+// Synthetic job, ply 1: does stm's king have a legal move or capture into
+// an exit (a position of another slice, valued by the seeded hash) that is
+// a loss (*w: W_1 candidate) or a win (*l: L_1 candidate) for the opponent?
+// The opponent's attack sets are computed once with stm's king lifted, so
+// each king move's legality is a bit test.
+__device__ void synth_exits_pos(const Tabs &T, const Mat &M, uint64_t p, uint64_t occ, int stm,
+                                bool *w, bool *l)
+{
+  int o = stm ^ 1, s = M.synth_slice, from = sq_get(p, stm), oks = sq_get(p, o);
+  uint64_t occk = occ ^ bit(from);
+  uint64_t att[MAXS];
+  uint64_t own = bit(from);
+#pragma unroll
+  for (int t = 0; t < MAXS; t++) {
+    att[t] = 0;
+    if (t < M.nk[o]) {
+      int j = M.pcs[o][t];
+      att[t] = piece_att(T, M.pt[j], sq_get(p, j), occk);
+    }
+    if (t < M.nk[stm]) own |= bit(sq_get(p, M.pcs[stm][t]));
+  }
+  uint64_t so[MAXS];
+  set_occ(M.ii, p, so);
+  uint64_t b = T.king[from] & ~T.king[oks] & ~own;
+  bool ww = false, ll = false;
+  while (b) {
+    int to = ctz64(b);
+    b &= b - 1;
+    int jj = -1, l = 0;
+    if (occ & bit(to)) {
+      for (int x = 0; x < MAXS; x++)
+        if (x < M.numsets && (so[x] & bit(to))) l = x;
+      jj = slot_of(p, to, M.setmask80[l]);
+    }
+    uint64_t q0 = sq_set(p, stm, to), q;
+    int s2 = normalize(T, q0, &q);
+    if (s2 == s) continue;
+    uint64_t a = T.king[oks];
+#pragma unroll
+    for (int t = 0; t < MAXS; t++)
+      if (t < M.nk[o] && M.pcs[o][t] != jj) a |= att[t];
+    if (a & bit(to)) continue;  // the king would stand in check
+    int v;
+    if (jj >= 0) {
+      uint64_t c = sq_set(q, jj, sq_get(q, M.ii.last[l]));
+      v = synth_value(synth_hash(M.seed, o, 2, s2 * 8 + l, sq_to_idx(T, M.capt[l], c)));
+    } else {
+      v = synth_value(synth_hash(M.seed, o, 1, s2, sq_to_idx(T, M.ii, q)));
+    }
+    ww |= v == -2;
+    ll |= v == 2;
+  }
+  *w = ww;
+  *l = ll;
+}
+
 // has_legal_moves || has_legal_caps (src/movegen.c:75-111) for colour c.
 __device__ bool has_legal(const Tabs &T, const Mat &M, uint64_t p, uint64_t occ, int c)
 {
@@ -558,7 +614,9 @@ __device__ bool has_legal(const Tabs &T, const Mat &M, uint64_t p, uint64_t occ,
 // and every code.
 __global__ __launch_bounds__(256) void k_init(const Mat *__restrict__ Mp, uint64_t *winsW,
                                                uint64_t *winsB, uint64_t *mateW, uint64_t *mateB,
-                                               uint16_t *codesW, uint16_t *codesB, int s0, int ns,
+                                               uint16_t *codesW, uint16_t *codesB,
+                                               uint64_t *kexW0, uint64_t *kexW1, uint64_t *kexL0,
+                                               uint64_t *kexL1, int s0, int ns,
                                                unsigned long long *cnt)
 {
   __shared__ Tabs T;
@@ -572,9 +630,14 @@ __global__ __launch_bounds__(256) void k_init(const Mat *__restrict__ Mp, uint64
     uint64_t w = g % M.words;
     uint64_t idx = w * 64 + lane;
     bool wi = false, bi = false, wm = false, bm = false;
+    bool xw0 = false, xw1 = false, xl0 = false, xl1 = false;
     if (idx < M.ii.size) {
       uint64_t p = kings_of(T, s);
       uint64_t occ = idx_to_sq(T, M.ii, idx, &p);
+      if (M.synth) {
+        synth_exits_pos(T, M, p, occ, WHITE, &xw0, &xl0);
+        synth_exits_pos(T, M, p, occ, BLACK, &xw1, &xl1);
+      }
       wi = attacked_by(T, M, p, occ, sq_get(p, BLACK), WHITE, -1);
       bi = attacked_by(T, M, p, occ, sq_get(p, WHITE), BLACK, -1);
       if (wi != bi) {
@@ -589,6 +652,15 @@ __global__ __launch_bounds__(256) void k_init(const Mat *__restrict__ Mp, uint64
     }
     uint64_t a = phys(M, s) * M.words + w;
     uint64_t bwi = __ballot(wi), bbi = __ballot(bi), bwm = __ballot(wm), bbm = __ballot(bm);
+    if (M.synth) {
+      uint64_t e0 = __ballot(xw0), e1 = __ballot(xw1), e2 = __ballot(xl0), e3 = __ballot(xl1);
+      if (lane == 0) {
+        kexW0[a] = e0;
+        kexW1[a] = e1;
+        kexL0[a] = e2;
+        kexL1[a] = e3;
+      }
+    }
     if (lane == 0) {
       winsW[a] = bwi;
       winsB[a] = bbi;
@@ -663,49 +735,18 @@ __global__ __launch_bounds__(256) void k_resolve(const Mat *__restrict__ Mp,
   }
 }
 
-// Synthetic job, ply 1: positions whose king can move or capture into an
-// exit that is a win (which == 0: L_1 candidates) or a loss (which == 1:
-// W_1 candidates) for the opponent.
-__global__ __launch_bounds__(256) void k_synth_exits(const Mat *__restrict__ Mp, int stm, int which,
-                                                      uint64_t *cand)
+// dst |= src over nw words (2 per lane)
+__global__ __launch_bounds__(256) void k_or(uint64_t *dst, const uint64_t *__restrict__ src, uint64_t nw)
 {
-  __shared__ Tabs T;
-  load_tabs(T);
-  const Mat &M = *Mp;
-  int o = stm ^ 1, s = M.synth_slice;
-  int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  for (uint64_t g = (uint64_t)blockIdx.x * 4 + wave; g < M.words; g += (uint64_t)gridDim.x * 4) {
-    uint64_t idx = g * 64 + lane;
-    bool hit = false;
-    if (idx < M.ii.size) {
-      uint64_t p = kings_of(T, s);
-      uint64_t occ = idx_to_sq(T, M.ii, idx, &p);
-      int from = sq_get(p, stm);
-      uint64_t so[MAXS];
-      set_occ(M.ii, p, so);
-      uint64_t b = T.king[from] & ~T.king[sq_get(p, o)];
-      uint64_t own = occ;
-      for (int l = 0; l < MAXS; l++)
-        if (l < M.numsets && M.setcol[l] == o) own &= ~so[l];
-      b &= ~own;
-      while (b && !hit) {
-        int to = ctz64(b);
-        b &= b - 1;
-        int jj = -1, l = 0;
-        if (occ & bit(to)) {
-          for (int x = 0; x < MAXS; x++)
-            if (x < M.numsets && (so[x] & bit(to))) l = x;
-          jj = slot_of(p, to, M.setmask80[l]);
-        }
-        uint64_t q0 = sq_set(p, stm, to), q;
-        int s2 = normalize(T, q0, &q);
-        if (s2 == s) continue;
-        int v = synth_exit(T, M, stm, q0, q, occ, from, jj, l, s2);
-        hit = which == 0 ? v == 2 : v == -2;
-      }
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x * 2;
+  for (uint64_t i = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * 2; i < nw; i += stride) {
+    ulonglong2 a = *reinterpret_cast<const ulonglong2 *>(src + i);
+    if (a.x | a.y) {
+      ulonglong2 d = *reinterpret_cast<const ulonglong2 *>(dst + i);
+      d.x |= a.x;
+      d.y |= a.y;
+      *reinterpret_cast<ulonglong2 *>(dst + i) = d;
     }
-    uint64_t bw = __ballot(hit);
-    if (lane == 0 && bw) atomic_or64(cand + g, bw);
   }
 }
 
