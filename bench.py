@@ -77,7 +77,9 @@ def run_synth(args, tbg):
 
         fn = tbg.LINE_FN(progress)
         table.resolve([])
-        tbg.check(tbg.lib().tbg_generate(table.ptr, tbg.FORM_BATCHED, fn, None,
+        form = {"batched": tbg.FORM_BATCHED, "graph": tbg.FORM_GRAPH,
+                "plain": tbg.FORM_PLAIN}[args.form]
+        tbg.check(tbg.lib().tbg_generate(table.ptr, form, fn, None,
                                          tbg.ctypes.byref(table.counts)), "generate")
         dt = time.perf_counter() - t0
         if step == args.warmup - 1:
@@ -123,7 +125,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--material", default="KQRvKR")
-    ap.add_argument("--form", default="batched", choices=["batched", "plain"])
+    ap.add_argument("--form", default="batched", choices=["batched", "graph", "plain"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-form comparison")
     ap.add_argument("--cpu-plies", type=int, default=2)
@@ -160,7 +162,7 @@ def main():
     tbg.init(0)
     if args.workload == "synth":
         return run_synth(args, tbg)
-    form = tbg.FORM_BATCHED if args.form == "batched" else tbg.FORM_PLAIN
+    form = {"batched": tbg.FORM_BATCHED, "graph": tbg.FORM_GRAPH, "plain": tbg.FORM_PLAIN}[args.form]
     reg = tbg.Registry(tbg.FORM_BATCHED)
     for sub in tbg.sub_materials(args.material):
         reg.get(sub)
@@ -183,7 +185,7 @@ def main():
     dom = max(cats, key=lambda k: cats[k][0])
     dom_ms, dom_n = cats[dom]
     avg_ms = dom_ms / max(1, dom_n)
-    covered = n_total if form == tbg.FORM_BATCHED else table.size
+    covered = table.size if form == tbg.FORM_PLAIN else n_total
     alg_bytes = BITS[dom] * covered / 8.0
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
     traffic = None
@@ -226,7 +228,7 @@ def main():
         },
     }
 
-    if not args.no_plain and form == tbg.FORM_BATCHED:
+    if not args.no_plain and form != tbg.FORM_PLAIN:
         t0 = time.perf_counter()
         table.resolve(subs)
         plines = table.generate(tbg.FORM_PLAIN)

@@ -129,6 +129,14 @@ int tbg_end_ply(tbg_ctx *ctx);
  * more_* flags, emitting the reference's count lines through `fn`. */
 int tbg_generate(tbg_ctx *ctx, int form, tbg_line_fn fn, void *user, tbg_counts *counts);
 
+/* Several materials generated together (batched and graph forms: every
+ * launch covers all of them; plain form: one after the other).  Each
+ * context must have its capture exits resolved; they must not depend on
+ * each other.  counts receives nc entries (or is null). */
+int tbg_generate_batch(tbg_ctx *const *ctxs, int nc, int form, tbg_counts *counts);
+/* The count lines of a generated context, in the reference's order. */
+int tbg_emit_lines(const tbg_ctx *ctx, tbg_line_fn fn, void *user);
+
 /* Data plane (device -> host, synchronous). */
 int tbg_get_codes(tbg_ctx *ctx, int stm, int slice, uint16_t *dst, uint64_t n);
 int tbg_get_bitmap(tbg_ctx *ctx, int plane, int stm, int slice, uint64_t *dst, uint64_t nwords);

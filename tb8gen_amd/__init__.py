@@ -29,7 +29,8 @@ ABI_SYMBOLS = [
     "tbg_get_info", "tbg_resolve_captures", "tbg_init_illegal_mate", "tbg_step_L",
     "tbg_step_W", "tbg_step_L_plain", "tbg_step_W_plain", "tbg_end_ply", "tbg_generate",
     "tbg_get_codes", "tbg_get_bitmap", "tbg_get_sub_bitmap", "tbg_get_timing",
-    "tbg_set_profiling", "tbg_host_selftest", "tbg_set_scatter",
+    "tbg_set_profiling", "tbg_host_selftest", "tbg_set_scatter", "tbg_generate_batch",
+    "tbg_emit_lines",
 ]
 SCATTER_GLOBAL, SCATTER_BLOCK = 0, 1
 
@@ -99,6 +100,9 @@ def lib():
         L.tbg_get_timing.argtypes = [vp, ctypes.POINTER(Timing)]
         L.tbg_set_profiling.argtypes = [vp, ctypes.c_int]
         L.tbg_set_scatter.argtypes = [vp, ctypes.c_int]
+        L.tbg_generate_batch.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.c_int,
+                                         ctypes.POINTER(Counts)]
+        L.tbg_emit_lines.argtypes = [vp, LINE_FN, vp]
         L.tbg_host_selftest.argtypes = [ctypes.c_char_p, ctypes.c_int]
         _lib = L
     return _lib
@@ -170,6 +174,32 @@ class Table:
               f"generate {self.material}")
         return self.lines
 
+    def emit_lines(self):
+        self.lines = []
+
+        def cb(line, _user):
+            self.lines.append(line.decode())
+
+        fn = LINE_FN(cb)
+        check(lib().tbg_emit_lines(self.ptr, fn, None), f"lines {self.material}")
+        return self.lines
+
+    def step(self, kind, stm, n, more, plain=False):
+        """One host-driven half-step (kind 'L' or 'W'); returns the count."""
+        cnt = ctypes.c_uint64()
+        f = {("L", False): "tbg_step_L", ("W", False): "tbg_step_W",
+             ("L", True): "tbg_step_L_plain", ("W", True): "tbg_step_W_plain"}[(kind, plain)]
+        check(getattr(lib(), f)(self.ptr, stm, n, 1 if more else 0, ctypes.byref(cnt)), f)
+        return cnt.value
+
+    def init_illegal_mate(self):
+        c = Counts()
+        check(lib().tbg_init_illegal_mate(self.ptr, ctypes.byref(c)), "init")
+        return c
+
+    def end_ply(self):
+        check(lib().tbg_end_ply(self.ptr), "end_ply")
+
     def codes(self, stm, slice_=None):
         """uint16 codes [462, size] (or one slice)."""
         slices = [slice_] if slice_ is not None else (
@@ -206,6 +236,18 @@ class Table:
         t = Timing()
         check(lib().tbg_get_timing(self.ptr, ctypes.byref(t)), "timing")
         return t
+
+
+def generate_batch(tables, form=FORM_GRAPH):
+    """Generate several independent materials together (one launch per phase
+    covers all of them); returns each table's count lines."""
+    arr = (ctypes.c_void_p * len(tables))(*[t.ptr.value for t in tables])
+    counts = (Counts * len(tables))()
+    check(lib().tbg_generate_batch(arr, len(tables), form, counts), "generate_batch")
+    for t, c in zip(tables, counts):
+        t.counts = c
+        t.emit_lines()
+    return [t.lines for t in tables]
 
 
 _ORDER = "KQRBN"
@@ -259,6 +301,33 @@ class Registry:
         t.generate(self.form if form is None else form)
         self.tables[material] = t
         return t
+
+    def get_many(self, materials, form=FORM_GRAPH):
+        """Generate materials level by level (every smaller material first),
+        each level as one batch."""
+        need = []
+
+        def visit(m):
+            if m in self.tables or m in need:
+                return
+            for sub in sub_materials(m):
+                visit(sub)
+            need.append(m)
+
+        for m in materials:
+            visit(m)
+        levels = {}
+        for m in need:
+            levels.setdefault(len(m) - 1, []).append(m)
+        for k in sorted(levels):
+            batch = [Table(m) for m in levels[k]]
+            subs = list(self.tables.values())
+            for t in batch:
+                t.resolve(subs)
+            generate_batch(batch, form)
+            for m, t in zip(levels[k], batch):
+                self.tables[m] = t
+        return [self.tables[m] for m in materials]
 
     def close(self):
         for t in self.tables.values():

@@ -2,6 +2,7 @@
 // resident device planes, kernel launches for the plain / batched / graph
 // forms, and the generator's ply loop.
 #include <stdarg.h>
+#include <stddef.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -35,6 +36,12 @@ int fail(int code, const char *fmt, ...)
     if (e_ != hipSuccess)                                                        \
       return fail(TBG_E_HIP, "%s: %s (%s:%d)", #x, hipGetErrorString(e_), __FILE__, \
                   __LINE__);                                                     \
+  } while (0)
+
+#define TRY(x)                   \
+  do {                           \
+    int r_ = (x);                \
+    if (r_ != TBG_OK) return r_; \
   } while (0)
 
 // ------------------------------------------------------------ tables
@@ -207,6 +214,23 @@ enum { CAT_PRED, CAT_VERIFY, CAT_SWEEP, CAT_UNCAP, CAT_INIT, CAT_RESOLVE, CAT_N 
 
 }  // namespace
 
+// Device arrays of one launch configuration: the (job, side) pairs and, per
+// kernel type, the prefix sums of their work units.
+struct LaunchSet {
+  Pair *pairs = nullptr;
+  uint64_t *P[4] = { nullptr, nullptr, nullptr, nullptr };
+  uint64_t total[4] = { 0, 0, 0, 0 };
+  int np = 0;
+  size_t pred_lds = 0;
+  void release()
+  {
+    (void)hipFree(pairs);
+    for (auto &x : P) (void)hipFree(x);
+    pairs = nullptr;
+    for (auto &x : P) x = nullptr;
+  }
+};
+
 struct tbg_ctx {
   char name[16];
   Mat h;
@@ -215,24 +239,121 @@ struct tbg_ctx {
   SubRef *dR = nullptr;
   uint64_t nsl = 0, pw = 0;
   uint64_t *arena = nullptr, *subarena = nullptr;
-  uint64_t *wins[2], *prevW[2], *prevL[2], *newW[2], *newL[2], *cand[2];
+  uint64_t *wins[2], *prevW[2], *prevL[2], *newW[2], *newL[2], *cand[2];  // canonical planes
   uint64_t *sub[2][4];
   uint16_t *codes[2] = { nullptr, nullptr };
-  unsigned long long *dcnt = nullptr;
+  unsigned long long *dcnt = nullptr;  // init counts
+  unsigned *dflags = nullptr;          // [0] jobs done, [1] uncapture ply pending
+  JobDev hj;                           // host mirror of the job
+  JobDev *dj = nullptr;                // the job on the device
+  long long *dhist = nullptr;
+  std::vector<long long> hist;
+  int max_ply = 4096;
   hipStream_t st = nullptr;
-  int resolved = 0, inited = 0;
+  int resolved = 0, inited = 0, generated = 0;
   uint64_t bytes = 0;
-  int grid = 0;
   int mgr[2][NSL];
+  LaunchSet both, side[2], plain[2];
   bool prof = false;
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> evs;
   double ms[CAT_N];
   uint64_t nl[CAT_N];
   uint64_t launches = 0, half_steps = 0;
   int scatter = TBG_SCATTER_BLOCK;
+  tbg_counts counts;
+  double graph_ms = 0;
+  uint64_t hs_generated = 0;  // half-steps executed by tbg_generate*
 };
 
 namespace {
+
+// Block of the index a k_pred_j work-group owns: the whole slice if its
+// bitmap fits the LDS budget, else the run of positions sharing the high
+// digits 0..d-1 (factor[d-1] positions) for the smallest d that fits.
+constexpr uint64_t kBlkMaxPos = 40 * 1024 * 8 - 128;
+
+void choose_block(const Mat &M, uint64_t *blk_pos, uint64_t *nblk)
+{
+  if (M.ii.size <= kBlkMaxPos) { *blk_pos = M.ii.size; *nblk = 1; return; }
+  for (int d = 1; d <= M.ii.numsets; d++)
+    if (M.ii.factor[d - 1] <= kBlkMaxPos) {
+      *blk_pos = M.ii.factor[d - 1];
+      *nblk = M.ii.size / M.ii.factor[d - 1];
+      return;
+    }
+  *blk_pos = 1;
+  *nblk = M.ii.size;
+}
+
+// window_slices 0: each job's own slices (462, or 1 for a synthetic job)
+int build_launchset(LaunchSet &L, const std::vector<const JobDev *> &hjobs,
+                    const std::vector<const Mat *> &mats, const std::vector<Pair> &pairs,
+                    uint64_t window_slices, hipStream_t st)
+{
+  L.release();
+  L.np = (int)pairs.size();
+  uint64_t maxbw = 1;
+  std::vector<uint64_t> P[4];
+  for (int kt = 0; kt < 4; kt++) {
+    P[kt].assign(pairs.size() + 1, 0);
+    for (size_t i = 0; i < pairs.size(); i++) {
+      const Mat &M = *mats[pairs[i].job];
+      uint64_t ns = window_slices ? window_slices : (M.synth ? 1 : NSL);
+      P[kt][i + 1] = P[kt][i] + ns * units_per_slice(M, hjobs[pairs[i].job]->nblk, kt, pairs[i].side);
+    }
+    L.total[kt] = P[kt][pairs.size()];
+  }
+  for (const JobDev *j : hjobs) maxbw = j->blk_words > maxbw ? j->blk_words : maxbw;
+  L.pred_lds = maxbw * 8 + PB_QUEUE_BYTES;
+  HIPCHK(hipMalloc(&L.pairs, sizeof(Pair) * pairs.size()));
+  HIPCHK(hipMemcpyAsync(L.pairs, pairs.data(), sizeof(Pair) * pairs.size(), hipMemcpyHostToDevice, st));
+  for (int kt = 0; kt < 4; kt++) {
+    HIPCHK(hipMalloc(&L.P[kt], sizeof(uint64_t) * P[kt].size()));
+    HIPCHK(hipMemcpyAsync(L.P[kt], P[kt].data(), sizeof(uint64_t) * P[kt].size(),
+                          hipMemcpyHostToDevice, st));
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  return TBG_OK;
+}
+
+// canonical job state: planes in their allocation order, ply 1, all steps
+void reset_job(tbg_ctx *c)
+{
+  JobDev &J = c->hj;
+  memset(&J, 0, sizeof J);
+  J.M = c->dM;
+  for (int s = 0; s < 2; s++) {
+    J.wins[s] = c->wins[s];
+    J.prevW[s] = c->prevW[s];
+    J.prevL[s] = c->prevL[s];
+    J.newW[s] = c->newW[s];
+    J.newL[s] = c->newL[s];
+    J.cand[s] = c->cand[s];
+    J.codes[s] = c->codes[s];
+    for (int k = 0; k < 4; k++) J.sub[s][k] = c->sub[s][k];
+  }
+  J.hist = c->dhist;
+  J.n = 1;
+  J.mask = 0xf;
+  J.more = 0xf;
+  J.scatter = c->scatter;
+  J.max_ply = c->max_ply;
+  choose_block(c->h, &J.blk_pos, &J.nblk);
+  J.blk_words = (J.blk_pos + 63) / 64 + 1;
+}
+
+int upload_job(tbg_ctx *c)
+{
+  HIPCHK(hipMemcpyAsync(c->dj, &c->hj, sizeof(JobDev), hipMemcpyHostToDevice, c->st));
+  return TBG_OK;
+}
+
+int download_job(tbg_ctx *c)
+{
+  HIPCHK(hipMemcpyAsync(&c->hj, c->dj, sizeof(JobDev), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return TBG_OK;
+}
 
 int ctx_alloc(tbg_ctx *c, bool with_codes)
 {
@@ -276,21 +397,37 @@ int ctx_alloc(tbg_ctx *c, bool with_codes)
   HIPCHK(hipMemsetAsync(c->dR, 0, sizeof(SubRef) * MAXS, c->st));
   HIPCHK(hipMalloc(&c->dcnt, 16 * sizeof(unsigned long long)));
   HIPCHK(hipMemsetAsync(c->dcnt, 0, 16 * sizeof(unsigned long long), c->st));
-  HIPCHK(hipStreamSynchronize(c->st));
-  c->grid = g_cus * 16;
+  HIPCHK(hipMalloc(&c->dflags, 4 * sizeof(unsigned)));
+  HIPCHK(hipMalloc(&c->dj, sizeof(JobDev)));
+  HIPCHK(hipMalloc(&c->dhist, sizeof(long long) * 4 * c->max_ply));
   manager_order(WHITE, c->mgr[WHITE]);
   manager_order(BLACK, c->mgr[BLACK]);
   for (int i = 0; i < CAT_N; i++) c->ms[i] = 0, c->nl[i] = 0;
+  reset_job(c);
+  {
+    std::vector<const JobDev *> hj{ &c->hj };
+    std::vector<const Mat *> mats{ &c->h };
+    int r = build_launchset(c->both, hj, mats, { { 0, 0 }, { 0, 1 } }, 0, c->st);
+    if (r != TBG_OK) return r;
+    for (int s = 0; s < 2; s++) {
+      r = build_launchset(c->side[s], hj, mats, { { 0, s } }, 0, c->st);
+      if (r != TBG_OK) return r;
+      r = build_launchset(c->plain[s], hj, mats, { { 0, s } }, 1, c->st);
+      if (r != TBG_OK) return r;
+    }
+  }
+  HIPCHK(hipStreamSynchronize(c->st));
   return TBG_OK;
 }
 
-// optional per-launch timing
+// optional per-launch timing (not inside graph capture)
 struct Timed {
   tbg_ctx *c;
   int cat;
   hipEvent_t a = nullptr, b = nullptr;
   Timed(tbg_ctx *c_, int cat_) : c(c_), cat(cat_)
   {
+    if (!c) return;
     c->launches++;
     c->nl[cat]++;
     if (c->prof) {
@@ -301,7 +438,7 @@ struct Timed {
   }
   ~Timed()
   {
-    if (c->prof) {
+    if (c && c->prof) {
       (void)hipEventRecord(b, c->st);
       c->evs.push_back({ cat, { a, b } });
     }
@@ -322,122 +459,61 @@ void flush_timing(tbg_ctx *c)
   c->evs.clear();
 }
 
-unsigned blocks_for(tbg_ctx *c, uint64_t waves)
-{
-  uint64_t b = (waves + 3) / 4;
-  if (b > (uint64_t)c->grid) b = (uint64_t)c->grid;
-  return (unsigned)(b ? b : 1);
-}
+unsigned cap_grid(uint64_t want, uint64_t cap) { return (unsigned)(want < 1 ? 1 : want < cap ? want : cap); }
 
-// Block of the index a k_pred_blk work-group privatises in LDS: the whole
-// slice if it fits, else the run of positions sharing the high digits
-// 0..d-1 (factor[d-1] positions) for the smallest d that fits.
-constexpr uint64_t kBlkMaxPos = 40 * 1024 * 8 - 128;
-
-void choose_block(const Mat &M, uint64_t *blk_pos, uint64_t *nblk)
+// One launch of a job kernel.  kt: KT_*; kind: 0 L step, 1 W step.
+// prof: context receiving timings (null inside graph capture).
+int launch(tbg_ctx *prof, JobDev *jobs, const LaunchSet &L, int kt, int kind, int sw0,
+           const unsigned *flags, hipStream_t st)
 {
-  if (M.ii.size <= kBlkMaxPos) { *blk_pos = M.ii.size; *nblk = 1; return; }
-  for (int d = 1; d <= M.ii.numsets; d++)
-    if (M.ii.factor[d - 1] <= kBlkMaxPos) {
-      *blk_pos = M.ii.factor[d - 1];
-      *nblk = M.ii.size / M.ii.factor[d - 1];
-      return;
-    }
-  *blk_pos = 1;
-  *nblk = M.ii.size;
-}
-
-int launch_pred(tbg_ctx *c, int stm, const uint64_t *src, int s0, int ns)
-{
-  Timed t(c, CAT_PRED);
-  if (c->scatter == TBG_SCATTER_GLOBAL) {
-    uint64_t cps = (c->h.words + 63) / 64;
-    hipLaunchKernelGGL(k_pred, dim3(blocks_for(c, ns * cps)), dim3(256), 0, c->st, c->dM, stm, src,
-                       c->cand[stm], c->wins[stm], s0, ns, cps);
-  } else {
-    uint64_t blk_pos, nblk;
-    choose_block(c->h, &blk_pos, &nblk);
-    uint64_t blk_words = (blk_pos + 63) / 64 + 1;
-    size_t lds = blk_words * 8 + PB_QUEUE_BYTES;
-    uint64_t nb = (uint64_t)ns * nblk;
-    uint64_t g = nb < (uint64_t)g_cus * 3 ? nb : (uint64_t)g_cus * 3;
-    hipLaunchKernelGGL(k_pred_blk, dim3((unsigned)g), dim3(PB_THREADS), lds, c->st, c->dM, stm,
-                       src, c->cand[stm], c->wins[stm], s0, ns, blk_pos, nblk, blk_words);
+  uint64_t total = L.total[kt];
+  if (!total) return TBG_OK;
+  switch (kt) {
+  case KT_PRED: {
+    Timed t(prof, CAT_PRED);
+    hipLaunchKernelGGL(k_pred_j, dim3(cap_grid(total, (uint64_t)g_cus * 3)), dim3(PB_THREADS),
+                       L.pred_lds, st, jobs, L.pairs, L.P[kt], L.np, sw0, kind);
+    break;
+  }
+  case KT_UNCAP: {
+    Timed t(prof, CAT_UNCAP);
+    hipLaunchKernelGGL(k_uncap_j, dim3(cap_grid(total, (uint64_t)g_cus * 16)), dim3(256), 0, st,
+                       jobs, L.pairs, L.P[kt], L.np, sw0, kind, flags);
+    break;
+  }
+  case KT_VERIFY: {
+    Timed t(prof, CAT_VERIFY);
+    hipLaunchKernelGGL(k_verify_j, dim3(cap_grid(total, (uint64_t)g_cus * 16)), dim3(256), 0, st,
+                       jobs, L.pairs, L.P[kt], L.np, sw0);
+    break;
+  }
+  default: {
+    Timed t(prof, CAT_SWEEP);
+    hipLaunchKernelGGL(k_sweep_j, dim3(cap_grid(total, (uint64_t)g_cus * 16)), dim3(256), 0, st, jobs,
+                       L.pairs, L.P[kt], L.np, sw0);
+    break;
+  }
   }
   HIPCHK(hipGetLastError());
   return TBG_OK;
 }
 
-int launch_uncap(tbg_ctx *c, int stm, int kind, int kind2, int legality, int s0, int ns)
+// One whole ply of every job of a launch set (batched form; also what a
+// graph captures): L candidates and verification for both sides, then W
+// candidates and the sweep, then the control kernel.
+int launch_ply(tbg_ctx *prof, JobDev *jobs, int nj, const LaunchSet &L, unsigned *flags,
+               hipStream_t st, bool edge = true)
 {
-  int o = stm ^ 1;
-  for (int k = 0; k < c->h.numsets; k++) {
-    if (c->h.setcol[k] != o) continue;
-    uint64_t cps = (c->h.cwords[k] + 63) / 64;
-    Timed t(c, CAT_UNCAP);
-    hipLaunchKernelGGL(k_uncap, dim3(blocks_for(c, ns * cps)), dim3(256), 0, c->st, c->dM, stm, k,
-                       c->sub[o][kind], kind2 >= 0 ? c->sub[o][kind2] : nullptr, legality,
-                       c->cand[stm], c->wins[stm], s0, ns, cps);
-    HIPCHK(hipGetLastError());
-  }
-  return TBG_OK;
-}
-
-int launch_verify(tbg_ctx *c, int stm, int n, int s0, int ns)
-{
-  int o = stm ^ 1;
-  uint64_t cps = (c->h.words + 63) / 64;
-  Timed t(c, CAT_VERIFY);
-  hipLaunchKernelGGL(k_verify, dim3(blocks_for(c, ns * cps)), dim3(256), 0, c->st, c->dM, stm, n,
-                     c->cand[stm], c->wins[o], c->sub[o][n <= DRAW_RULE ? TBG_SUB_WIN : TBG_SUB_CWIN],
-                     c->wins[stm], c->newL[stm], c->codes[stm], s0, ns, cps, c->dcnt);
+  if (edge) TRY(launch(prof, jobs, L, KT_UNCAP, 0, 0, flags, st));
+  TRY(launch(prof, jobs, L, KT_PRED, 0, 0, flags, st));
+  TRY(launch(prof, jobs, L, KT_VERIFY, 0, 0, flags, st));
+  if (edge) TRY(launch(prof, jobs, L, KT_UNCAP, 1, 0, flags, st));
+  TRY(launch(prof, jobs, L, KT_PRED, 1, 0, flags, st));
+  TRY(launch(prof, jobs, L, KT_SWEEP, 1, 0, flags, st));
+  hipLaunchKernelGGL(k_ctrl, dim3(1), dim3(1024), 0, st, jobs, nj, flags);
   HIPCHK(hipGetLastError());
   return TBG_OK;
 }
-
-int launch_wsweep(tbg_ctx *c, int stm, int n, uint64_t w0, uint64_t nw)
-{
-  Timed t(c, CAT_SWEEP);
-  uint64_t threads = (nw + 1) / 2;
-  uint64_t b = (threads + 255) / 256;
-  if (b > (uint64_t)g_cus * 8) b = (uint64_t)g_cus * 8;
-  hipLaunchKernelGGL(k_wsweep, dim3((unsigned)(b ? b : 1)), dim3(256), 0, c->st, c->cand[stm],
-                     c->wins[stm], c->newW[stm], c->codes[stm], nw, c->h.words, c->h.ii.size, w0, n,
-                     c->dcnt);
-  HIPCHK(hipGetLastError());
-  return TBG_OK;
-}
-
-// Synthetic jobs: the ply-1 king-move exit candidates were computed by
-// k_init into planes that ply 1 does not otherwise read: L_1 candidates of
-// stm in newW[stm], W_1 candidates in prevW[stm].  which 0: L, 1: W.
-int launch_exits(tbg_ctx *c, int stm, int which)
-{
-  Timed t(c, CAT_UNCAP);
-  const uint64_t *src = which == 0 ? c->newW[stm] : c->prevW[stm];
-  uint64_t b = (c->pw / 2 + 255) / 256;
-  if (b > (uint64_t)g_cus * 8) b = (uint64_t)g_cus * 8;
-  hipLaunchKernelGGL(k_or, dim3((unsigned)(b ? b : 1)), dim3(256), 0, c->st, c->cand[stm], src,
-                     c->pw);
-  HIPCHK(hipGetLastError());
-  return TBG_OK;
-}
-
-int read_count(tbg_ctx *c, uint64_t *out)
-{
-  unsigned long long v = 0;
-  HIPCHK(hipMemcpyAsync(&v, c->dcnt, sizeof v, hipMemcpyDeviceToHost, c->st));
-  HIPCHK(hipMemsetAsync(c->dcnt, 0, sizeof v, c->st));
-  HIPCHK(hipStreamSynchronize(c->st));
-  if (out) *out = v;
-  return TBG_OK;
-}
-
-#define TRY(x)                  \
-  do {                          \
-    int r_ = (x);               \
-    if (r_ != TBG_OK) return r_; \
-  } while (0)
 
 int check_stm(tbg_ctx *c, int stm)
 {
@@ -447,62 +523,213 @@ int check_stm(tbg_ctx *c, int stm)
   return TBG_OK;
 }
 
-int step_L(tbg_ctx *c, int stm, int n, int more, uint64_t *count, bool plain)
+// Host-driven single half-step (kind 0: L, 1: W) of ctx for stm at ply n.
+int manual_step(tbg_ctx *c, int kind, int stm, int n, int more, uint64_t *count, bool plain)
 {
   TRY(check_stm(c, stm));
-  if (n < 1) return fail(TBG_E_INVAL, "ply must be >= 1");
-  int o = stm ^ 1;
+  if (n < 1 || n >= c->max_ply) return fail(TBG_E_INVAL, "ply out of range");
+  JobDev &J = c->hj;
+  J.n = n;
+  J.done = 0;
+  J.mask = 1u << (kind * 2 + stm);
+  J.more = more ? (kind == 0 ? 1u << (2 + stm) : 1u << stm) : 0u;
+  for (int i = 0; i < 4; i++) J.cnt[i] = 0;
+  TRY(upload_job(c));
   c->half_steps++;
-  int slices = (int)c->nsl;
-  // candidates (tb8gen.c:951-976)
-  for (int i = 0; i < (plain ? slices : 1); i++) {
-    int s0 = plain ? (c->h.synth ? c->h.synth_slice : c->mgr[stm][i]) : (c->h.synth ? c->h.synth_slice : 0);
-    int ns = plain ? 1 : (c->h.synth ? 1 : NSL);
-    if (n == 1) TRY(launch_uncap(c, stm, TBG_SUB_WIN, -1, 1, s0, ns));
-    else if (n == DRAW_RULE + 1) TRY(launch_uncap(c, stm, TBG_SUB_CWIN, TBG_SUB_WIN, 0, s0, ns));
-    if (more && n > 1) TRY(launch_pred(c, stm, c->prevW[o], s0, ns));
+  bool edge = n == 1 || n == DRAW_RULE + 1;
+  if (!plain) {
+    if (edge) TRY(launch(c, c->dj, c->side[stm], KT_UNCAP, kind, 0, nullptr, c->st));
+    TRY(launch(c, c->dj, c->side[stm], KT_PRED, kind, 0, nullptr, c->st));
+    TRY(launch(c, c->dj, c->side[stm], kind == 0 ? KT_VERIFY : KT_SWEEP, kind, 0, nullptr, c->st));
+  } else {
+    // one slice per launch in the reference's manager order (kslice.c:41-89)
+    int ns = (int)c->nsl;
+    for (int i = 0; i < ns; i++) {
+      int s = c->h.synth ? c->h.synth_slice : c->mgr[stm][i];
+      if (edge) TRY(launch(c, c->dj, c->plain[stm], KT_UNCAP, kind, s, nullptr, c->st));
+      TRY(launch(c, c->dj, c->plain[stm], KT_PRED, kind, s, nullptr, c->st));
+    }
+    for (int i = 0; i < ns; i++) {
+      int s = c->h.synth ? c->h.synth_slice : c->mgr[stm][i];
+      TRY(launch(c, c->dj, c->plain[stm], kind == 0 ? KT_VERIFY : KT_SWEEP, kind, s, nullptr, c->st));
+    }
   }
-  if (n == 1 && c->h.synth) TRY(launch_exits(c, stm, 0));
-  // verification (tb8gen.c:999-1022)
-  for (int i = 0; i < (plain ? slices : 1); i++) {
-    int s0 = plain ? (c->h.synth ? c->h.synth_slice : c->mgr[stm][i]) : (c->h.synth ? c->h.synth_slice : 0);
-    int ns = plain ? 1 : (c->h.synth ? 1 : NSL);
-    TRY(launch_verify(c, stm, n, s0, ns));
-  }
-  return read_count(c, count);
+  unsigned long long v = 0;
+  HIPCHK(hipMemcpyAsync(&v, reinterpret_cast<char *>(c->dj) + offsetof(JobDev, cnt) +
+                                sizeof(unsigned long long) * (kind * 2 + stm),
+                        sizeof v, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  if (count) *count = v;
+  return TBG_OK;
 }
 
-int step_W(tbg_ctx *c, int stm, int n, int more, uint64_t *count, bool plain)
+int init_job_state(tbg_ctx *c)
 {
-  TRY(check_stm(c, stm));
-  if (n < 1) return fail(TBG_E_INVAL, "ply must be >= 1");
-  int o = stm ^ 1;
-  c->half_steps++;
-  int slices = (int)c->nsl;
-  for (int i = 0; i < (plain ? slices : 1); i++) {
-    int s0 = plain ? (c->h.synth ? c->h.synth_slice : c->mgr[stm][i]) : (c->h.synth ? c->h.synth_slice : 0);
-    int ns = plain ? 1 : (c->h.synth ? 1 : NSL);
-    if (n == 1) TRY(launch_uncap(c, stm, TBG_SUB_LOSS, -1, 0, s0, ns));
-    else if (n == DRAW_RULE + 1) TRY(launch_uncap(c, stm, TBG_SUB_BLOSS, -1, 0, s0, ns));
-    if (more) TRY(launch_pred(c, stm, c->prevL[o], s0, ns));
-  }
-  if (n == 1 && c->h.synth) TRY(launch_exits(c, stm, 1));
-  // W = C & ~wins; wins |= W (tb8gen.c:1072-1077)
-  if (plain && !c->h.synth) {
-    for (int i = 0; i < slices; i++) {
-      uint64_t s = (uint64_t)c->mgr[stm][i];
-      TRY(launch_wsweep(c, stm, n, s * c->h.words, c->h.words));
-    }
-  } else {
-    TRY(launch_wsweep(c, stm, n, 0, c->pw));
-  }
-  return read_count(c, count);
+  reset_job(c);
+  TRY(upload_job(c));
+  HIPCHK(hipMemsetAsync(c->dhist, 0xff, sizeof(long long) * 4 * c->max_ply, c->st));
+  unsigned f[4] = { 0, 1, 0, 0 };
+  HIPCHK(hipMemcpyAsync(c->dflags, f, sizeof f, hipMemcpyHostToDevice, c->st));
+  return TBG_OK;
 }
 
 int zero_plane(tbg_ctx *c, uint64_t *p)
 {
   HIPCHK(hipMemsetAsync(p, 0, c->pw * 8, c->st));
   return TBG_OK;
+}
+
+int run_init(tbg_ctx *c)
+{
+  Mat &M = c->h;
+  int s0 = M.synth ? M.synth_slice : 0, ns = M.synth ? 1 : NSL;
+  HIPCHK(hipMemsetAsync(c->dcnt, 0, 4 * sizeof(unsigned long long), c->st));
+  {
+    Timed t(c, CAT_INIT);
+    hipLaunchKernelGGL(k_init, dim3(cap_grid(((uint64_t)ns * M.words + 3) / 4, (uint64_t)g_cus * 16)),
+                       dim3(256), 0, c->st, c->dM, c->wins[0], c->wins[1], c->prevL[0], c->prevL[1],
+                       c->codes[0], c->codes[1], c->prevW[0], c->prevW[1], c->newW[0], c->newW[1],
+                       s0, ns, c->dcnt);
+    HIPCHK(hipGetLastError());
+  }
+  unsigned long long v[4];
+  HIPCHK(hipMemcpyAsync(v, c->dcnt, sizeof v, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  c->counts.broken[0] = v[0];
+  c->counts.broken[1] = v[1];
+  c->counts.mates[0] = v[2];
+  c->counts.mates[1] = v[3];
+  c->counts.nplies = 0;
+  c->inited = 1;
+  c->generated = 0;
+  return init_job_state(c);
+}
+
+// plain form: the host reads the job's flags each ply and issues one launch
+// per slice and phase for the steps that run
+int run_plain(tbg_ctx *c)
+{
+  for (;;) {
+    TRY(download_job(c));
+    JobDev &J = c->hj;
+    if (J.done) break;
+    bool edge = J.n == 1 || J.n == DRAW_RULE + 1;
+    int ns = (int)c->nsl;
+    for (int kind = 0; kind < 2; kind++)
+      for (int stm = 0; stm < 2; stm++) {
+        if (!((J.mask >> (kind * 2 + stm)) & 1)) {
+          TRY(zero_plane(c, kind == 0 ? J.newL[stm] : J.newW[stm]));
+          continue;
+        }
+        for (int i = 0; i < ns; i++) {
+          int s = c->h.synth ? c->h.synth_slice : c->mgr[stm][i];
+          if (edge) TRY(launch(c, c->dj, c->plain[stm], KT_UNCAP, kind, s, nullptr, c->st));
+          TRY(launch(c, c->dj, c->plain[stm], KT_PRED, kind, s, nullptr, c->st));
+        }
+        for (int i = 0; i < ns; i++) {
+          int s = c->h.synth ? c->h.synth_slice : c->mgr[stm][i];
+          TRY(launch(c, c->dj, c->plain[stm], kind == 0 ? KT_VERIFY : KT_SWEEP, kind, s, nullptr,
+                     c->st));
+        }
+      }
+    hipLaunchKernelGGL(k_ctrl, dim3(1), dim3(1024), 0, c->st, c->dj, 1, c->dflags);
+    HIPCHK(hipGetLastError());
+  }
+  return TBG_OK;
+}
+
+// Batched and graph forms over any set of jobs.  jobs: device array of nj
+// JobDev; L: their launch set.  Counts executed half-steps into *hs.
+int run_batch(tbg_ctx *prof, JobDev *jobs, int nj, const LaunchSet &L, unsigned *flags,
+              hipStream_t st, bool graph, double *graph_ms)
+{
+  unsigned f[2] = { 0, 1 };
+  if (!graph) {
+    for (;;) {
+      // the host reads the flags after every ply, so it knows whether any
+      // job is at ply 1 or 101 and needs the uncapture launches
+      TRY(launch_ply(prof, jobs, nj, L, flags, st, f[1] != 0));
+      HIPCHK(hipMemcpyAsync(f, flags, sizeof f, hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      if ((int)f[0] >= nj) break;
+    }
+    return TBG_OK;
+  }
+  // capture one ply once, replay it; check for completion every 4 plies
+  hipGraph_t g = nullptr;
+  hipGraphExec_t ge = nullptr;
+  HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  int r = launch_ply(nullptr, jobs, nj, L, flags, st);
+  hipError_t e = hipStreamEndCapture(st, &g);
+  if (r != TBG_OK) return r;
+  HIPCHK(e);
+  HIPCHK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+  hipEvent_t a, b;
+  HIPCHK(hipEventCreate(&a));
+  HIPCHK(hipEventCreate(&b));
+  HIPCHK(hipEventRecord(a, st));
+  for (;;) {
+    for (int i = 0; i < 4; i++) HIPCHK(hipGraphLaunch(ge, st));
+    HIPCHK(hipMemcpyAsync(f, flags, sizeof f, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if ((int)f[0] >= nj) break;
+  }
+  HIPCHK(hipEventRecord(b, st));
+  HIPCHK(hipEventSynchronize(b));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, a, b));
+  if (graph_ms) *graph_ms += ms;
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  (void)hipGraphExecDestroy(ge);
+  (void)hipGraphDestroy(g);
+  return TBG_OK;
+}
+
+// after a generation: pull the final job state and the per-ply counts
+int finish_job(tbg_ctx *c)
+{
+  TRY(download_job(c));
+  c->hist.assign((size_t)4 * c->max_ply, -1);
+  HIPCHK(hipMemcpyAsync(c->hist.data(), c->dhist, sizeof(long long) * 4 * c->max_ply,
+                        hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  int last = 0;
+  for (int n = 1; n < c->max_ply; n++)
+    for (int i = 0; i < 4; i++)
+      if (c->hist[(size_t)n * 4 + i] >= 0) last = n;
+  c->counts.nplies = last;
+  uint64_t hs = 0;
+  for (int n = 1; n <= last; n++)
+    for (int i = 0; i < 4; i++) hs += c->hist[(size_t)n * 4 + i] >= 0;
+  c->hs_generated += hs;
+  c->generated = 1;
+  return TBG_OK;
+}
+
+uint64_t executed_half_steps(const tbg_ctx *c)
+{
+  uint64_t hs = 0;
+  for (size_t i = 4; i < c->hist.size(); i++) hs += c->hist[i] >= 0;
+  return hs;
+}
+
+void emit_lines(const tbg_ctx *c, tbg_line_fn fn, void *user)
+{
+  char line[64];
+  const char *nm[4] = { "broken_w", "broken_b", "l0_w", "l0_b" };
+  uint64_t v[4] = { c->counts.broken[0], c->counts.broken[1], c->counts.mates[0], c->counts.mates[1] };
+  for (int i = 0; i < 4; i++) {
+    snprintf(line, sizeof line, "%s = %llu", nm[i], (unsigned long long)v[i]);
+    fn(line, user);
+  }
+  for (int n = 1; n <= c->counts.nplies; n++)
+    for (int i = 0; i < 4; i++) {
+      long long x = c->hist[(size_t)n * 4 + i];
+      if (x < 0) continue;
+      snprintf(line, sizeof line, "%c%d_%c = %lld", i < 2 ? 'l' : 'w', n, "wb"[i & 1], x);
+      fn(line, user);
+    }
 }
 
 }  // namespace
@@ -538,7 +765,7 @@ const char *tbg_strerror(int err)
   case TBG_E_HIP: return g_errbuf[0] ? g_errbuf : "HIP error";
   case TBG_E_NOSUB: return g_errbuf[0] ? g_errbuf : "missing smaller table";
   case TBG_E_STATE: return g_errbuf[0] ? g_errbuf : "call out of order";
-  case TBG_E_BUDGET: return "device memory budget exceeded";
+  case TBG_E_BUDGET: return g_errbuf[0] ? g_errbuf : "device memory budget exceeded";
   case TBG_E_NODEV: return "no HIP device";
   default: return "unknown error";
   }
@@ -578,6 +805,8 @@ void tbg_destroy(tbg_ctx *c)
   if (!c) return;
   if (c->st) (void)hipStreamSynchronize(c->st);
   flush_timing(c);
+  c->both.release();
+  for (int s = 0; s < 2; s++) c->side[s].release(), c->plain[s].release();
   (void)hipFree(c->arena);
   (void)hipFree(c->subarena);
   (void)hipFree(c->codes[0]);
@@ -585,6 +814,9 @@ void tbg_destroy(tbg_ctx *c)
   (void)hipFree(c->dM);
   (void)hipFree(c->dR);
   (void)hipFree(c->dcnt);
+  (void)hipFree(c->dflags);
+  (void)hipFree(c->dj);
+  (void)hipFree(c->dhist);
   if (c->st) (void)hipStreamDestroy(c->st);
   delete c;
 }
@@ -632,7 +864,7 @@ int tbg_resolve_captures(tbg_ctx *c, tbg_ctx *const *subs, int nsubs)
       else if (!strcmp(subs[i]->name, fl)) { hit = subs[i]; flip = 1; }
     }
     if (!hit) return fail(TBG_E_NOSUB, "table %s (or %s) not supplied", nm, fl);
-    if (!hit->codes[0] || !hit->inited) return fail(TBG_E_STATE, "table %s not generated", hit->name);
+    if (!hit->codes[0] || !hit->generated) return fail(TBG_E_STATE, "table %s not generated", hit->name);
     R.present = 1;
     R.flip = flip;
     R.num = hit->h.num;
@@ -656,8 +888,9 @@ int tbg_resolve_captures(tbg_ctx *c, tbg_ctx *const *subs, int nsubs)
     int col = M.setcol[k];
     uint64_t waves = (uint64_t)ns * M.cwords[k];
     Timed t(c, CAT_RESOLVE);
-    hipLaunchKernelGGL(k_resolve, dim3(blocks_for(c, waves)), dim3(256), 0, c->st, c->dM, c->dR + k,
-                       col, k, c->sub[col][0], c->sub[col][1], c->sub[col][2], c->sub[col][3], s0, ns);
+    hipLaunchKernelGGL(k_resolve, dim3(cap_grid((waves + 3) / 4, (uint64_t)g_cus * 16)), dim3(256), 0,
+                       c->st, c->dM, c->dR + k, col, k, c->sub[col][0], c->sub[col][1],
+                       c->sub[col][2], c->sub[col][3], s0, ns);
     HIPCHK(hipGetLastError());
   }
   HIPCHK(hipStreamSynchronize(c->st));
@@ -668,118 +901,117 @@ int tbg_resolve_captures(tbg_ctx *c, tbg_ctx *const *subs, int nsubs)
 int tbg_init_illegal_mate(tbg_ctx *c, tbg_counts *counts)
 {
   if (!c) return fail(TBG_E_INVAL, "null context");
-  Mat &M = c->h;
-  int s0 = M.synth ? M.synth_slice : 0, ns = M.synth ? 1 : NSL;
-  HIPCHK(hipMemsetAsync(c->dcnt, 0, 4 * sizeof(unsigned long long), c->st));
-  {
-    Timed t(c, CAT_INIT);
-    hipLaunchKernelGGL(k_init, dim3(blocks_for(c, (uint64_t)ns * M.words)), dim3(256), 0, c->st,
-                       c->dM, c->wins[0], c->wins[1], c->prevL[0], c->prevL[1], c->codes[0],
-                       c->codes[1], c->prevW[0], c->prevW[1], c->newW[0], c->newW[1], s0, ns,
-                       c->dcnt);
-    HIPCHK(hipGetLastError());
-  }
-  unsigned long long v[4];
-  HIPCHK(hipMemcpyAsync(v, c->dcnt, sizeof v, hipMemcpyDeviceToHost, c->st));
-  HIPCHK(hipMemsetAsync(c->dcnt, 0, 4 * sizeof(unsigned long long), c->st));
-  HIPCHK(hipStreamSynchronize(c->st));
-  if (counts) {
-    counts->broken[0] = v[0];
-    counts->broken[1] = v[1];
-    counts->mates[0] = v[2];
-    counts->mates[1] = v[3];
-    counts->nplies = 0;
-  }
-  c->inited = 1;
+  TRY(run_init(c));
+  if (counts) *counts = c->counts;
   return TBG_OK;
 }
 
 int tbg_step_L(tbg_ctx *c, int stm, int n, int more, uint64_t *count)
 {
-  return step_L(c, stm, n, more, count, false);
+  return manual_step(c, 0, stm, n, more, count, false);
 }
 int tbg_step_W(tbg_ctx *c, int stm, int n, int more, uint64_t *count)
 {
-  return step_W(c, stm, n, more, count, false);
+  return manual_step(c, 1, stm, n, more, count, false);
 }
 int tbg_step_L_plain(tbg_ctx *c, int stm, int n, int more, uint64_t *count)
 {
-  return step_L(c, stm, n, more, count, true);
+  return manual_step(c, 0, stm, n, more, count, true);
 }
 int tbg_step_W_plain(tbg_ctx *c, int stm, int n, int more, uint64_t *count)
 {
-  return step_W(c, stm, n, more, count, true);
+  return manual_step(c, 1, stm, n, more, count, true);
 }
 
 int tbg_end_ply(tbg_ctx *c)
 {
   if (!c) return fail(TBG_E_INVAL, "null context");
+  JobDev &J = c->hj;
   for (int s = 0; s < 2; s++) {
-    uint64_t *x = c->prevW[s];
-    c->prevW[s] = c->newW[s];
-    c->newW[s] = x;
-    x = c->prevL[s];
-    c->prevL[s] = c->newL[s];
-    c->newL[s] = x;
+    uint64_t *x = J.prevW[s];
+    J.prevW[s] = J.newW[s];
+    J.newW[s] = x;
+    x = J.prevL[s];
+    J.prevL[s] = J.newL[s];
+    J.newL[s] = x;
   }
+  return upload_job(c);
+}
+
+int tbg_generate_batch(tbg_ctx *const *cs, int nc, int form, tbg_counts *counts)
+{
+  if (!cs || nc < 1 || nc > 1024) return fail(TBG_E_INVAL, "bad batch");
+  if (form < TBG_FORM_PLAIN || form > TBG_FORM_GRAPH) return fail(TBG_E_INVAL, "bad form");
+  for (int i = 0; i < nc; i++) {
+    if (!cs[i]) return fail(TBG_E_INVAL, "null context in batch");
+    if (!cs[i]->resolved) return fail(TBG_E_STATE, "resolve captures of %s first", cs[i]->name);
+    for (int j = 0; j < i; j++)
+      if (cs[j] == cs[i]) return fail(TBG_E_INVAL, "context twice in a batch");
+  }
+  for (int i = 0; i < nc; i++) TRY(run_init(cs[i]));
+  tbg_ctx *c0 = cs[0];
+  if (form == TBG_FORM_PLAIN) {
+    for (int i = 0; i < nc; i++) {
+      TRY(run_plain(cs[i]));
+      TRY(finish_job(cs[i]));
+    }
+  } else if (nc == 1) {
+    TRY(run_batch(c0, c0->dj, 1, c0->both, c0->dflags, c0->st, form == TBG_FORM_GRAPH, &c0->graph_ms));
+    TRY(finish_job(c0));
+  } else {
+    // the jobs of all contexts in one device array, one launch set
+    JobDev *dj = nullptr;
+    unsigned *df = nullptr;
+    HIPCHK(hipMalloc(&dj, sizeof(JobDev) * nc));
+    HIPCHK(hipMalloc(&df, 4 * sizeof(unsigned)));
+    std::vector<const JobDev *> hj;
+    std::vector<const Mat *> mats;
+    std::vector<Pair> pairs;
+    for (int i = 0; i < nc; i++) {
+      HIPCHK(hipStreamSynchronize(cs[i]->st));
+      HIPCHK(hipMemcpyAsync(dj + i, &cs[i]->hj, sizeof(JobDev), hipMemcpyHostToDevice, c0->st));
+      hj.push_back(&cs[i]->hj);
+      mats.push_back(&cs[i]->h);
+      pairs.push_back({ i, 0 });
+      pairs.push_back({ i, 1 });
+    }
+    unsigned f[4] = { 0, 1, 0, 0 };
+    HIPCHK(hipMemcpyAsync(df, f, sizeof f, hipMemcpyHostToDevice, c0->st));
+    LaunchSet L;
+    int r = build_launchset(L, hj, mats, pairs, 0, c0->st);
+    if (r == TBG_OK)
+      r = run_batch(c0, dj, nc, L, df, c0->st, form == TBG_FORM_GRAPH, &c0->graph_ms);
+    if (r == TBG_OK)
+      for (int i = 0; i < nc; i++) {
+        hipError_t e = hipMemcpyAsync(cs[i]->dj, dj + i, sizeof(JobDev), hipMemcpyDeviceToDevice, c0->st);
+        if (e != hipSuccess) { r = fail(TBG_E_HIP, "copy back job: %s", hipGetErrorString(e)); break; }
+      }
+    (void)hipStreamSynchronize(c0->st);
+    L.release();
+    (void)hipFree(dj);
+    (void)hipFree(df);
+    if (r != TBG_OK) return r;
+    for (int i = 0; i < nc; i++) TRY(finish_job(cs[i]));
+  }
+  if (counts)
+    for (int i = 0; i < nc; i++) counts[i] = cs[i]->counts;
+  return TBG_OK;
+}
+
+int tbg_emit_lines(const tbg_ctx *c, tbg_line_fn fn, void *user)
+{
+  if (!c || !fn) return fail(TBG_E_INVAL, "null argument");
+  if (!c->generated) return fail(TBG_E_STATE, "%s is not generated", c->name);
+  emit_lines(c, fn, user);
   return TBG_OK;
 }
 
 int tbg_generate(tbg_ctx *c, int form, tbg_line_fn fn, void *user, tbg_counts *counts)
 {
   if (!c) return fail(TBG_E_INVAL, "null context");
-  if (form < TBG_FORM_PLAIN || form > TBG_FORM_GRAPH) return fail(TBG_E_INVAL, "bad form");
-  if (!c->resolved) return fail(TBG_E_STATE, "resolve captures first");
-  bool plain = form == TBG_FORM_PLAIN;
-  char line[64];
-  tbg_counts cnt;
-  TRY(tbg_init_illegal_mate(c, &cnt));
-  if (fn) {
-    const char *nm[4] = { "broken_w", "broken_b", "l0_w", "l0_b" };
-    uint64_t v[4] = { cnt.broken[0], cnt.broken[1], cnt.mates[0], cnt.mates[1] };
-    for (int i = 0; i < 4; i++) {
-      snprintf(line, sizeof line, "%s = %llu", nm[i], (unsigned long long)v[i]);
-      fn(line, user);
-    }
-  }
-  // ply loop of main() (tb8gen.c:1406-1444)
-  bool more_ww = true, more_wb = true, more_lw = true, more_lb = true;
-  int n;
-  for (n = 1;; n++) {
-    bool forced = n == DRAW_RULE + 1;
-    if (n > DRAW_RULE + 1 && !(more_ww || more_wb || more_lw || more_lb)) break;
-    bool ml[2] = { more_lw, more_lb }, mw[2] = { more_ww, more_wb };
-    bool lres[2], wres[2];
-    for (int stm = 0; stm < 2; stm++) {
-      if (!(forced || ml[stm])) { lres[stm] = false; TRY(zero_plane(c, c->newL[stm])); continue; }
-      uint64_t k = 0;
-      TRY(step_L(c, stm, n, ml[stm], &k, plain));
-      lres[stm] = k != 0;
-      if (fn) {
-        snprintf(line, sizeof line, "l%d_%c = %llu", n, "wb"[stm], (unsigned long long)k);
-        fn(line, user);
-      }
-    }
-    for (int stm = 0; stm < 2; stm++) {
-      if (!(forced || mw[stm])) { wres[stm] = false; TRY(zero_plane(c, c->newW[stm])); continue; }
-      uint64_t k = 0;
-      TRY(step_W(c, stm, n, mw[stm], &k, plain));
-      wres[stm] = k != 0;
-      if (fn) {
-        snprintf(line, sizeof line, "w%d_%c = %llu", n, "wb"[stm], (unsigned long long)k);
-        fn(line, user);
-      }
-    }
-    more_wb = lres[WHITE];
-    more_ww = lres[BLACK];
-    more_lb = wres[WHITE];
-    more_lw = wres[BLACK];
-    TRY(tbg_end_ply(c));
-  }
-  if (counts) {
-    *counts = cnt;
-    counts->nplies = n - 1;
-  }
+  tbg_ctx *one[1] = { c };
+  TRY(tbg_generate_batch(one, 1, form, counts));
+  if (fn) emit_lines(c, fn, user);
   return TBG_OK;
 }
 
@@ -802,9 +1034,9 @@ int tbg_get_bitmap(tbg_ctx *c, int plane, int stm, int slice, uint64_t *dst, uin
   if (c->h.synth ? slice != c->h.synth_slice : (slice < 0 || slice >= NSL))
     return fail(TBG_E_INVAL, "slice out of range");
   if (nwords > c->h.words) return fail(TBG_E_INVAL, "too many words");
-  uint64_t *p = plane == TBG_PLANE_WINS ? c->wins[stm]
-              : plane == TBG_PLANE_W    ? c->prevW[stm]
-              : plane == TBG_PLANE_L    ? c->prevL[stm]
+  uint64_t *p = plane == TBG_PLANE_WINS ? c->hj.wins[stm]
+              : plane == TBG_PLANE_W    ? c->hj.prevW[stm]
+              : plane == TBG_PLANE_L    ? c->hj.prevL[stm]
                                         : nullptr;
   if (!p) return fail(TBG_E_INVAL, "bad plane");
   uint64_t ps = c->h.synth ? 0 : (uint64_t)slice;
@@ -840,7 +1072,8 @@ int tbg_set_profiling(tbg_ctx *c, int on)
   flush_timing(c);
   c->prof = on != 0;
   for (int i = 0; i < CAT_N; i++) c->ms[i] = 0, c->nl[i] = 0;
-  c->launches = c->half_steps = 0;
+  c->launches = c->half_steps = c->hs_generated = 0;
+  c->graph_ms = 0;
   return TBG_OK;
 }
 
@@ -858,7 +1091,7 @@ int tbg_get_timing(const tbg_ctx *cc, tbg_timing *t)
   t->ms_total = 0;
   for (int i = 0; i < CAT_N; i++) t->ms_total += c->ms[i];
   t->launches = c->launches;
-  t->half_steps = c->half_steps;
+  t->half_steps = c->half_steps + c->hs_generated;
   t->n_pred = c->nl[CAT_PRED];
   t->n_verify = c->nl[CAT_VERIFY];
   t->n_sweep = c->nl[CAT_SWEEP];

@@ -1,6 +1,7 @@
 // tbg_kern.h -- kernel entry points and the capture-table reference shared
 // by the kernel and host translation units.
 #pragma once
+#include "tbg.h"
 #include "tbg_dev.h"
 
 namespace tbg {
@@ -15,29 +16,68 @@ struct SubRef {
   int present;               // 0: a bare pair of kings (always a draw)
 };
 
-__global__ void k_pred(const Mat *Mp, int stm, const uint64_t *src, uint64_t *cand,
-                       const uint64_t *won, int s0, int ns, uint64_t cps);
-__global__ void k_pred_blk(const Mat *Mp, int stm, const uint64_t *src, uint64_t *cand,
-                           const uint64_t *won, int s0, int ns, uint64_t blk_pos, uint64_t nblk,
-                           uint64_t blk_words);
+// Device-resident state of one generation job (a material, or one king
+// slice of one).  The kernels read the ply, the step mask and the plane
+// pointers from here, so a whole ply can be captured in a hipGraph and
+// replayed; k_ctrl advances it (the more_* logic of src/tb8gen.c:1406-1444).
+struct JobDev {
+  const Mat *M;
+  uint64_t *wins[2], *prevW[2], *prevL[2], *newW[2], *newL[2], *cand[2];
+  uint64_t *sub[2][4];
+  uint16_t *codes[2];
+  long long *hist;            // [max_ply][4] counts per step, -1: step did not run
+  unsigned long long cnt[4];  // this ply: L(W), L(B), W(W), W(B)
+  int n, done;
+  unsigned mask;              // steps of this ply: bit 0 L(W), 1 L(B), 2 W(W), 3 W(B)
+  unsigned more;              // bit 0 more_ww, 1 more_wb, 2 more_lw, 3 more_lb
+  int scatter;                // TBG_SCATTER_*
+  int max_ply;
+  uint64_t blk_pos, nblk, blk_words;
+};
+
+// One (job, side to move) pair of a launch.
+struct Pair {
+  int job, side;
+};
+
+// Work units per slice of each launch type, one unit per work-group at a
+// time: pred = one index block; uncap / verify = 256 bitmap words (64 per
+// wave); sweep = 4096 words.
+enum { KT_PRED = 0, KT_UNCAP = 1, KT_VERIFY = 2, KT_SWEEP = 3 };
+constexpr uint64_t UNIT_WORDS = 256, SWEEP_UNIT_WORDS = 4096;
+
+TBG_HD uint64_t units_per_slice(const Mat &M, uint64_t nblk, int kt, int side)
+{
+  switch (kt) {
+  case KT_PRED: return nblk;
+  case KT_UNCAP: {
+    uint64_t u = 0;
+    for (int k = 0; k < M.numsets; k++)
+      if (M.setcol[k] != side) u += (M.cwords[k] + UNIT_WORDS - 1) / UNIT_WORDS;
+    return u;
+  }
+  case KT_VERIFY: return (M.words + UNIT_WORDS - 1) / UNIT_WORDS;
+  default: return (M.words + SWEEP_UNIT_WORDS - 1) / SWEEP_UNIT_WORDS;
+  }
+}
+
 constexpr int PB_THREADS = 512;
 constexpr int PB_QUEUE_BYTES = 8 * 1024 * 2;
-__global__ void k_uncap(const Mat *Mp, int stm, int k, const uint64_t *src0, const uint64_t *src1,
-                        int legality, uint64_t *cand, const uint64_t *won, int s0, int ns,
-                        uint64_t cps);
-__global__ void k_verify(const Mat *Mp, int stm, int n, uint64_t *cand, const uint64_t *wins_o,
-                         const uint64_t *sub_o, const uint64_t *won, uint64_t *newL,
-                         uint16_t *codes, int s0, int ns, uint64_t cps, unsigned long long *cnt);
-__global__ void k_wsweep(uint64_t *cand, uint64_t *wins, uint64_t *newW, uint16_t *codes,
-                         uint64_t nw, uint64_t words, uint64_t size, uint64_t w0, int n,
-                         unsigned long long *cnt);
+
+__global__ void k_pred_j(const JobDev *jobs, const Pair *pairs, const uint64_t *P, int np, int sw0,
+                         int kind);
+__global__ void k_uncap_j(const JobDev *jobs, const Pair *pairs, const uint64_t *P, int np, int sw0,
+                          int kind, const unsigned *flags);
+__global__ void k_verify_j(JobDev *jobs, const Pair *pairs, const uint64_t *P, int np, int sw0);
+__global__ void k_sweep_j(JobDev *jobs, const Pair *pairs, const uint64_t *P, int np, int sw0);
+// flags[0]: jobs done; flags[1]: some job is at ply 1 or 101 (uncapture)
+__global__ void k_ctrl(JobDev *jobs, int nj, unsigned *flags);
 __global__ void k_init(const Mat *Mp, uint64_t *winsW, uint64_t *winsB, uint64_t *mateW,
                        uint64_t *mateB, uint16_t *codesW, uint16_t *codesB, uint64_t *kexW0,
                        uint64_t *kexW1, uint64_t *kexL0, uint64_t *kexL1, int s0, int ns,
                        unsigned long long *cnt);
 __global__ void k_resolve(const Mat *Mp, const SubRef *Rp, int c, int k, uint64_t *loss,
                           uint64_t *bloss, uint64_t *cwin, uint64_t *win, int s0, int ns);
-__global__ void k_or(uint64_t *dst, const uint64_t *src, uint64_t nw);
 
 // copies the shared tables to the device (defined with the kernels)
 hipError_t upload_tabs(const Tabs *h);

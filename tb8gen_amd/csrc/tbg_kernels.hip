@@ -105,97 +105,120 @@ __device__ __forceinline__ void mark_unmoves(const Tabs &T, const Mat &M, int j,
 }
 
 // ----------------------------------------------------------- frontier
+constexpr int QCAP = 1024;  // queue entries per wave per round
+
 struct Queue {
-  uint16_t e[4][4096];
+  uint16_t e[4][QCAP];
   uint64_t res[4][64];
 };
 
-// Fills the wave's queue from its chunk; returns the entry count.
-__device__ __forceinline__ int fill_queue(uint16_t *q, uint64_t x, int lane)
+// LDS written by some lanes of a wave becomes visible to the other lanes
+__device__ __forceinline__ void wave_sync()
 {
-  int total;
-  int off = wave_excl_scan(popc64(x), lane, &total);
-  while (x) {
-    int b = ctz64(x);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Writes the wave's set bits of rank [base, base + QCAP) (rank = position
+// in the wave's bit order: lane, then bit) to its queue.
+__device__ __forceinline__ void fill_round(uint16_t *q, uint64_t x, int off, int base, int lane)
+{
+  int k = off;
+  while (x && k < base + QCAP) {
+    int bt = ctz64(x);
     x &= x - 1;
-    q[off++] = (uint16_t)((lane << 6) | b);
-  }
-  return total;
-}
-
-// predecessors_worker (src/tb8gen.c:484-516): every set bit of the stm^1
-// frontier `src` -> stm unmoves OR-ed into `cand`.
-__global__ __launch_bounds__(256) void k_pred(const Mat *__restrict__ Mp, int stm,
-                                               const uint64_t *__restrict__ src, uint64_t *cand,
-                                               const uint64_t *__restrict__ won, int s0, int ns,
-                                               uint64_t cps)
-{
-  __shared__ Tabs T;
-  __shared__ Queue Q;
-  load_tabs(T);
-  const Mat &M = *Mp;
-  int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  uint64_t nchunks = (uint64_t)ns * cps;
-  for (uint64_t g0 = (uint64_t)blockIdx.x * 4; g0 < nchunks; g0 += (uint64_t)gridDim.x * 4) {
-    uint64_t g = g0 + wave;
-    int s = s0 + (int)(g / cps);
-    uint64_t w = (g % cps) * 64 + lane;
-    uint64_t x = 0;
-    if (g < nchunks && w < M.words) x = src[phys(M, s) * M.words + w];
-    int total = fill_queue(Q.e[wave], x, lane);
-    __syncthreads();
-    uint64_t *own = cand + phys(M, s) * M.words;
-    const uint64_t *ownw = won + phys(M, s) * M.words;
-    uint64_t wbase = (g % cps) * 64;
-    for (int e = lane; e < total; e += 64) {
-      uint16_t v = Q.e[wave][e];
-      uint64_t idx = (wbase + (v >> 6)) * 64 + (v & 63);
-      if (idx >= M.ii.size) continue;
-      uint64_t p = kings_of(T, s);
-      uint64_t occ = idx_to_sq(T, M.ii, idx, &p);
-      mark_king_unmoves(T, M, stm, occ, p, s, cand, won);
-      for (int t = 0; t < MAXP - 1; t++) {
-        if (t >= M.nk[stm]) break;
-        mark_unmoves(T, M, M.pcs[stm][t], occ, p, own, ownw);
-      }
-    }
-    __syncthreads();
+    if (k >= base) q[k - base] = (uint16_t)((lane << 6) | bt);
+    k++;
   }
 }
 
-// Block-privatised form of k_pred.  One work-group owns one block of the
-// index (positions [lo, hi) of slice s, a run of whole low-order digits, so
-// every unmove of a piece in the low sets stays inside it): those targets
-// are OR-ed into an LDS bitmap with LDS atomics, and only king unmoves and
-// unmoves of high-set pieces go to the global bitmap.  The LDS bitmap is
-// merged with one global atomic per nonzero word.  Device-scope atomics on
-// gfx950 travel to the memory side, so this cuts the scatter's HBM write
-// traffic by the ratio of same-block to all unmoves.
-constexpr int PB_WAVES = 8;       // waves per work-group
-constexpr int PB_CAP = 1024;      // queue entries per wave per round
+__device__ __forceinline__ int find_pair(const uint64_t *__restrict__ P, int np, uint64_t u)
+{
+  int lo = 0, hi = np;
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (P[mid] <= u) lo = mid; else hi = mid;
+  }
+  return lo;
+}
 
-__global__ __launch_bounds__(512) void k_pred_blk(const Mat *__restrict__ Mp, int stm,
-                                                   const uint64_t *__restrict__ src, uint64_t *cand,
-                                                   const uint64_t *__restrict__ won, int s0, int ns,
-                                                   uint64_t blk_pos, uint64_t nblk,
-                                                   uint64_t blk_words)
+// Copies the material of job j into LDS when the work-group's unit moves to
+// another job (j is uniform across the work-group).  Kernels then read the
+// material from LDS: no reloads after their global atomics and stores.
+__device__ __forceinline__ void cache_mat(Mat &Ms, const Mat *src, int j, int *cached)
+{
+  if (j == *cached) return;
+  __syncthreads();
+  const uint32_t *a = reinterpret_cast<const uint32_t *>(src);
+  uint32_t *b = reinterpret_cast<uint32_t *>(&Ms);
+  for (int i = threadIdx.x; i < (int)(sizeof(Mat) / 4); i += blockDim.x) b[i] = a[i];
+  __syncthreads();
+  *cached = j;
+}
+
+// Slice of a job-local work unit: a synthetic job has its one slice, a
+// material covers slices sw0, sw0+1, ... of the launch window.
+__device__ __forceinline__ int unit_slice(const Mat &M, int sw0, uint64_t local, uint64_t ups,
+                                          uint64_t *within)
+{
+  *within = local % ups;
+  return M.synth ? M.synth_slice : sw0 + (int)(local / ups);
+}
+
+// predecessors_worker (src/tb8gen.c:484-516) over every (job, side) of a
+// launch: each set bit of the stm^1 frontier (W_{n-1} for an L step, kind
+// 0; L_{n-1} for a W step, kind 1) -> stm unmoves OR-ed into cand[stm].
+//
+// One work-group owns one block of the index (positions [lo, hi) of slice
+// s, a run of whole low-order digits, so every unmove of a piece in the low
+// sets stays inside it): with the block scatter those targets are OR-ed
+// into an LDS bitmap with LDS atomics and merged with one global atomic per
+// nonzero word; king unmoves and unmoves of high-set pieces (all targets,
+// with the global scatter) use 64-bit device atomics.  Device-scope atomics
+// on gfx950 travel to the memory side, so privatising cuts the scatter's HBM
+// write traffic by the share of same-block unmoves.
+constexpr int PB_WAVES = 8;   // waves per work-group
+constexpr int PB_CAP = 1024;  // queue entries per wave per round
+
+__global__ __launch_bounds__(512) void k_pred_j(const JobDev *__restrict__ jobs,
+                                                 const Pair *__restrict__ pairs,
+                                                 const uint64_t *__restrict__ P, int np, int sw0,
+                                                 int kind)
 {
   __shared__ Tabs T;
+  __shared__ Mat Ms;
   __shared__ int rmax[PB_WAVES];
   extern __shared__ uint64_t dyn[];
   load_tabs(T);
-  const Mat &M = *Mp;
-  uint64_t *B = dyn;
-  uint16_t *Qall = reinterpret_cast<uint16_t *>(dyn + blk_words);
+  int cached = -1;
   int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  uint16_t *q = Qall + wave * PB_CAP;
-  uint64_t nb = (uint64_t)ns * nblk;
-  for (uint64_t b = blockIdx.x; b < nb; b += gridDim.x) {
-    int s = s0 + (int)(b / nblk);
-    uint64_t lo = (b % nblk) * blk_pos;
-    uint64_t hi = lo + blk_pos < M.ii.size ? lo + blk_pos : M.ii.size;
+  uint64_t total = P[np];
+  for (uint64_t u = blockIdx.x; u < total; u += gridDim.x) {
+    int pi = find_pair(P, np, u);
+    Pair pr = pairs[pi];
+    const JobDev &J = jobs[pr.job];
+    int stm = pr.side, o = stm ^ 1;
+    bool en = !J.done && ((J.mask >> (kind * 2 + stm)) & 1);
+    if (kind == 0) en = en && J.n > 1 && ((J.more >> (2 + stm)) & 1);  // more_lw / more_lb
+    else en = en && ((J.more >> stm) & 1);                            // more_ww / more_wb
+    if (!en) continue;  // uniform across the work-group
+    cache_mat(Ms, J.M, pr.job, &cached);
+    const Mat &M = Ms;
+    uint64_t bi;
+    int s = unit_slice(M, sw0, u - P[pi], J.nblk, &bi);
+    uint64_t blk_words = J.blk_words;
+    uint64_t *B = dyn;
+    uint16_t *q = reinterpret_cast<uint16_t *>(dyn + blk_words) + wave * PB_CAP;
+    bool lds = J.scatter == TBG_SCATTER_BLOCK;
+    const uint64_t *src = kind == 0 ? J.prevW[o] : J.prevL[o];
+    uint64_t *cand = J.cand[stm];
+    const uint64_t *won = J.wins[stm];
+    uint64_t lo = bi * J.blk_pos;
+    uint64_t hi = lo + J.blk_pos < M.ii.size ? lo + J.blk_pos : M.ii.size;
     uint64_t wlo = lo >> 6, nbw = ((hi + 63) >> 6) - wlo;
-    for (uint64_t w = threadIdx.x; w < nbw; w += blockDim.x) B[w] = 0;
+    if (lds)
+      for (uint64_t w = threadIdx.x; w < nbw; w += blockDim.x) B[w] = 0;
     __syncthreads();
     const uint64_t *srcs = src + phys(M, s) * M.words;
     uint64_t *own = cand + phys(M, s) * M.words;
@@ -203,12 +226,12 @@ __global__ __launch_bounds__(512) void k_pred_blk(const Mat *__restrict__ Mp, in
     for (uint64_t c0 = 0; c0 < nbw; c0 += (uint64_t)PB_WAVES * 64) {
       uint64_t wi = c0 + (uint64_t)wave * 64 + lane;
       uint64_t x = wi < nbw ? srcs[wlo + wi] : 0;
-      if (wi == 0) x &= ~0ull << (lo & 63);                      // not below the block
+      if (wi == 0) x &= ~0ull << (lo & 63);                          // not below the block
       if (wi + 1 == nbw && (hi & 63)) x &= (1ull << (hi & 63)) - 1;  // nor above it
-      int total;
-      int off = wave_excl_scan(popc64(x), lane, &total);
+      int tot;
+      int off = wave_excl_scan(popc64(x), lane, &tot);
       // every wave runs the same number of rounds (they share barriers)
-      if (lane == 0) rmax[wave] = (total + PB_CAP - 1) / PB_CAP;
+      if (lane == 0) rmax[wave] = (tot + PB_CAP - 1) / PB_CAP;
       __syncthreads();
       int allr = 0;
       for (int v = 0; v < PB_WAVES; v++) allr = rmax[v] > allr ? rmax[v] : allr;
@@ -224,7 +247,7 @@ __global__ __launch_bounds__(512) void k_pred_blk(const Mat *__restrict__ Mp, in
           k++;
         }
         __syncthreads();
-        int cnt = total - base;
+        int cnt = tot - base;
         cnt = cnt < 0 ? 0 : cnt > PB_CAP ? PB_CAP : cnt;
         for (int e = lane; e < cnt; e += 64) {
           uint16_t v = q[e];
@@ -240,7 +263,7 @@ __global__ __launch_bounds__(512) void k_pred_blk(const Mat *__restrict__ Mp, in
               int tq = ctz64(bb);
               bb &= bb - 1;
               uint64_t i2 = sq_to_idx(T, M.ii, sq_set(p, j, tq));
-              if (i2 >= lo && i2 < hi)
+              if (lds && i2 >= lo && i2 < hi)
                 __hip_atomic_fetch_or(reinterpret_cast<unsigned long long *>(&B[(i2 >> 6) - wlo]),
                                       (unsigned long long)bit(i2 & 63), __ATOMIC_RELAXED,
                                       __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -252,62 +275,97 @@ __global__ __launch_bounds__(512) void k_pred_blk(const Mat *__restrict__ Mp, in
         __syncthreads();
       }
     }
-    for (uint64_t w = threadIdx.x; w < nbw; w += blockDim.x) {
-      uint64_t v = B[w];
-      if (v) atomic_or64(own + wlo + w, v);
-    }
+    if (lds)
+      for (uint64_t w = threadIdx.x; w < nbw; w += blockDim.x) {
+        uint64_t v = B[w];
+        if (v) atomic_or64(own + wlo + w, v);
+      }
     __syncthreads();
   }
 }
 
-// predecessors_sub_worker (src/tb8gen.c:346-394): uncapture from the
-// stm^1 capture-exit bitmap of set k (src0 & ~src1 when src1 is given).
-__global__ __launch_bounds__(256) void k_uncap(const Mat *__restrict__ Mp, int stm, int k,
-                                                const uint64_t *__restrict__ src0,
-                                                const uint64_t *__restrict__ src1, int legality,
-                                                uint64_t *cand, const uint64_t *__restrict__ won,
-                                                int s0, int ns, uint64_t cps)
+// predecessors_sub_worker (src/tb8gen.c:346-414) over every (job, side):
+// uncapture from the stm^1 capture-exit bitmaps, at n == 1 (L: sub/win with
+// the legality filter; W: sub/loss) and n == 101 (L: sub/cwin & ~sub/win;
+// W: sub/bloss).
+__global__ __launch_bounds__(256) void k_uncap_j(const JobDev *__restrict__ jobs,
+                                                  const Pair *__restrict__ pairs,
+                                                  const uint64_t *__restrict__ P, int np, int sw0,
+                                                  int kind, const unsigned *__restrict__ flags)
 {
   __shared__ Tabs T;
   __shared__ Queue Q;
+  __shared__ Mat Ms;
+  if (flags && !flags[1]) return;  // no job is at ply 1 or 101 (uniform)
   load_tabs(T);
-  const Mat &M = *Mp;
-  const Idx &ci = M.capt[k];
-  int o = stm ^ 1, m = M.ii.last[k];
+  int cached = -1;
   int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  uint64_t nchunks = (uint64_t)ns * cps;
-  for (uint64_t g0 = (uint64_t)blockIdx.x * 4; g0 < nchunks; g0 += (uint64_t)gridDim.x * 4) {
-    uint64_t g = g0 + wave;
-    int s = s0 + (int)(g / cps);
-    uint64_t w = (g % cps) * 64 + lane;
+  uint64_t total = P[np];
+  for (uint64_t u = blockIdx.x; u < total; u += gridDim.x) {
+    int pi = find_pair(P, np, u);
+    Pair pr = pairs[pi];
+    const JobDev &J = jobs[pr.job];
+    int stm = pr.side, o = stm ^ 1, n = J.n;
+    bool en = !J.done && ((J.mask >> (kind * 2 + stm)) & 1) && (n == 1 || n == DRAW_RULE + 1);
+    if (!en) continue;  // uniform across the work-group
+    cache_mat(Ms, J.M, pr.job, &cached);
+    const Mat &M = Ms;
+    uint64_t within;
+    int s = unit_slice(M, sw0, u - P[pi], units_per_slice(M, J.nblk, KT_UNCAP, stm), &within);
+    int k;
+    for (k = 0; k < M.numsets; k++) {
+      if (M.setcol[k] == stm) continue;
+      uint64_t cpb = (M.cwords[k] + UNIT_WORDS - 1) / UNIT_WORDS;
+      if (within < cpb) break;
+      within -= cpb;
+    }
+    const uint64_t *src0, *src1 = nullptr;
+    int legality = 0;
+    if (kind == 0) {
+      src0 = J.sub[o][n == 1 ? TBG_SUB_WIN : TBG_SUB_CWIN];
+      if (n != 1) src1 = J.sub[o][TBG_SUB_WIN];
+      legality = n == 1;
+    } else {
+      src0 = J.sub[o][n == 1 ? TBG_SUB_LOSS : TBG_SUB_BLOSS];
+    }
+    uint64_t wbase = within * UNIT_WORDS + (uint64_t)wave * 64;
+    uint64_t w = wbase + lane;
     uint64_t x = 0;
-    if (g < nchunks && w < M.cwords[k]) {
+    if (w < M.cwords[k]) {
       uint64_t a = phys(M, s) * M.ctotal[o] + M.coff[k] + w;
       x = src0[a];
       if (src1) x &= ~src1[a];
     }
-    int total = fill_queue(Q.e[wave], x, lane);
-    __syncthreads();
+    int tot;
+    int off = wave_excl_scan(popc64(x), lane, &tot);
+    const Idx &ci = M.capt[k];
+    int m = M.ii.last[k];
+    uint64_t *cand = J.cand[stm];
+    const uint64_t *won = J.wins[stm];
     uint64_t *own = cand + phys(M, s) * M.words;
     const uint64_t *ownw = won + phys(M, s) * M.words;
-    uint64_t wbase = (g % cps) * 64;
-    for (int e = lane; e < total; e += 64) {
-      uint16_t v = Q.e[wave][e];
-      uint64_t idx = (wbase + (v >> 6)) * 64 + (v & 63);
-      if (idx >= ci.size) continue;
-      uint64_t p = kings_of(T, s);
-      uint64_t occ = idx_to_sq(T, ci, idx, &p);
-      if (legality && attacked_by(T, M, p, occ, sq_get(p, stm), o, m)) continue;
-      // uncapture by the king
-      mark_king_unmoves(T, M, stm, occ, sq_set(p, m, sq_get(p, stm)), s, cand, won);
-      // uncapture by the non-king pieces
-      for (int t = 0; t < MAXP - 1; t++) {
-        if (t >= M.nk[stm]) break;
-        int j = M.pcs[stm][t];
-        mark_unmoves(T, M, j, occ, sq_set(p, m, sq_get(p, j)), own, ownw);
+    for (int base = 0; base < tot; base += QCAP) {
+      fill_round(Q.e[wave], x, off, base, lane);
+      wave_sync();
+      int cnt = tot - base < QCAP ? tot - base : QCAP;
+      for (int e = lane; e < cnt; e += 64) {
+        uint16_t v = Q.e[wave][e];
+        uint64_t idx = (wbase + (v >> 6)) * 64 + (v & 63);
+        if (idx >= ci.size) continue;
+        uint64_t p = kings_of(T, s);
+        uint64_t occ = idx_to_sq(T, ci, idx, &p);
+        if (legality && attacked_by(T, M, p, occ, sq_get(p, stm), o, m)) continue;
+        // uncapture by the king
+        mark_king_unmoves(T, M, stm, occ, sq_set(p, m, sq_get(p, stm)), s, cand, won);
+        // uncapture by the non-king pieces
+        for (int t = 0; t < MAXP - 1; t++) {
+          if (t >= M.nk[stm]) break;
+          int j = M.pcs[stm][t];
+          mark_unmoves(T, M, j, occ, sq_set(p, m, sq_get(p, j)), own, ownw);
+        }
       }
+      wave_sync();
     }
-    __syncthreads();
   }
 }
 
@@ -414,110 +472,218 @@ __device__ bool verify_pos(const Tabs &T, const Mat &M, int stm, int s, uint64_t
   return true;
 }
 
-// check_successors (src/tb8gen.c:607-661): prune the candidates of stm in
-// `cand` to L_n, written to newL (cand is cleared for the next half-step),
-// with the count and, if codes != null, the L_n codes.
-__global__ __launch_bounds__(256) void k_verify(const Mat *__restrict__ Mp, int stm, int n,
-                                                 uint64_t *cand, const uint64_t *__restrict__ wins_o,
-                                                 const uint64_t *__restrict__ sub_o,
-                                                 const uint64_t *__restrict__ won, uint64_t *newL,
-                                                 uint16_t *codes, int s0, int ns, uint64_t cps,
-                                                 unsigned long long *cnt)
+// check_successors (src/tb8gen.c:607-661) over every (job, side): prune
+// the candidates in cand[stm] to L_n, written to newL[stm] (cand is cleared
+// for the next half-step), counting into the job and writing L_n codes.  A
+// candidate already in wins(stm) can not be lost, so it is dropped without
+// a decode.  A step that does not run this ply writes an empty newL.  For a
+// synthetic job at n == 1 the king-move exit candidates (computed by k_init
+// into newW[stm]) join the candidates.
+__global__ __launch_bounds__(256) void k_verify_j(JobDev *__restrict__ jobs,
+                                                   const Pair *__restrict__ pairs,
+                                                   const uint64_t *__restrict__ P, int np, int sw0)
 {
   __shared__ Tabs T;
   __shared__ Queue Q;
+  __shared__ Mat Ms;
   load_tabs(T);
-  const Mat &M = *Mp;
+  int cached = -1;
   int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  uint64_t nchunks = (uint64_t)ns * cps;
-  uint64_t mycnt = 0;
-  for (uint64_t g0 = (uint64_t)blockIdx.x * 4; g0 < nchunks; g0 += (uint64_t)gridDim.x * 4) {
-    uint64_t g = g0 + wave;
-    int s = s0 + (int)(g / cps);
-    uint64_t w = (g % cps) * 64 + lane;
-    bool live = g < nchunks && w < M.words;
+  uint64_t total = P[np];
+  for (uint64_t u = blockIdx.x; u < total; u += gridDim.x) {
+    int pi = find_pair(P, np, u);
+    Pair pr = pairs[pi];
+    JobDev &J = jobs[pr.job];
+    if (J.done) continue;  // uniform across the work-group
+    cache_mat(Ms, J.M, pr.job, &cached);
+    const Mat &M = Ms;
+    int stm = pr.side, o = stm ^ 1, n = J.n;
+    bool run = (J.mask >> stm) & 1;
+    uint64_t within;
+    int s = unit_slice(M, sw0, u - P[pi], (M.words + UNIT_WORDS - 1) / UNIT_WORDS, &within);
+    uint64_t wbase = within * UNIT_WORDS + (uint64_t)wave * 64;
+    uint64_t w = wbase + lane;
+    bool live = w < M.words;
     uint64_t a = phys(M, s) * M.words + w;
-    uint64_t x0 = live ? cand[a] : 0;
-    // a position already in wins(stm) can not be lost (and illegal ones are
-    // not positions): only the rest is verified
-    uint64_t x = x0 ? x0 & ~won[a] : 0;
-    Q.res[wave][lane] = x;
-    int total = fill_queue(Q.e[wave], x, lane);
-    __syncthreads();
-    uint64_t wbase = (g % cps) * 64;
-    for (int e = lane; e < total; e += 64) {
-      uint16_t v = Q.e[wave][e];
-      uint64_t idx = (wbase + (v >> 6)) * 64 + (v & 63);
-      bool ok = idx < M.ii.size && verify_pos(T, M, stm, s, idx, wins_o, sub_o);
-      if (!ok)
-        __hip_atomic_fetch_and(reinterpret_cast<unsigned long long *>(&Q.res[wave][v >> 6]),
-                               ~(1ull << (v & 63)), __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_WORKGROUP);
-    }
-    __syncthreads();
+    uint64_t *cand = J.cand[stm];
+    uint64_t x0 = 0, x = 0;
     if (live) {
-      uint64_t r = Q.res[wave][lane];
-      newL[a] = r;
+      x0 = cand[a];
+      uint64_t c = x0;
+      if (run && M.synth && n == 1) c |= J.newW[stm][a];
+      x = run && c ? c & ~J.wins[stm][a] : 0;
+    }
+    Q.res[wave][lane] = x;
+    int tot;
+    int off = wave_excl_scan(popc64(x), lane, &tot);
+    const uint64_t *wins_o = J.wins[o];
+    const uint64_t *sub_o = J.sub[o][n <= DRAW_RULE ? TBG_SUB_WIN : TBG_SUB_CWIN];
+    for (int base = 0; base < tot; base += QCAP) {
+      fill_round(Q.e[wave], x, off, base, lane);
+      wave_sync();
+      int cnt = tot - base < QCAP ? tot - base : QCAP;
+      for (int e = lane; e < cnt; e += 64) {
+        uint16_t v = Q.e[wave][e];
+        uint64_t idx = (wbase + (v >> 6)) * 64 + (v & 63);
+        bool ok = idx < M.ii.size && verify_pos(T, M, stm, s, idx, wins_o, sub_o);
+        if (!ok)
+          __hip_atomic_fetch_and(reinterpret_cast<unsigned long long *>(&Q.res[wave][v >> 6]),
+                                 ~(1ull << (v & 63)), __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_WORKGROUP);
+      }
+      wave_sync();
+    }
+    wave_sync();
+    uint64_t r = 0;
+    if (live) {
+      r = Q.res[wave][lane];
+      J.newL[stm][a] = r;
       if (x0) cand[a] = 0;
-      mycnt += popc64(r);
+      uint16_t *codes = J.codes[stm];
       if (codes && r) {
         uint16_t *c = codes + phys(M, s) * M.ii.size + w * 64;
         uint16_t code = (uint16_t)((n << 2) | 2);
-        while (r) {
-          int b = ctz64(r);
-          r &= r - 1;
+        uint64_t rr = r;
+        while (rr) {
+          int b = ctz64(rr);
+          rr &= rr - 1;
           c[b] = code;
         }
       }
     }
-    __syncthreads();
+    uint64_t sum = wave_sum64(popc64(r));
+    if (lane == 0 && sum) atomicAdd(&J.cnt[stm], (unsigned long long)sum);
   }
-  mycnt = wave_sum64(mycnt);
-  if (lane == 0 && mycnt) atomicAdd(cnt, (unsigned long long)mycnt);
 }
 
-// calc_W_n tail (src/tb8gen.c:1072-1077) fused: W = C & ~wins; wins |= W;
-// count; codes; C cleared.  One plane region of nw words (slices s0..).
-__global__ __launch_bounds__(256) void k_wsweep(uint64_t *cand, uint64_t *wins, uint64_t *newW,
-                                                 uint16_t *codes, uint64_t nw, uint64_t words,
-                                                 uint64_t size, uint64_t w0, int n,
-                                                 unsigned long long *cnt)
+// calc_W_n tail (src/tb8gen.c:1072-1077) fused, over every (job, side):
+// W = C & ~wins; wins |= W; count; codes; C cleared.  4096 words per unit,
+// two per lane per pass with 16-byte accesses.  A step that does not run
+// writes an empty newW.  A synthetic job at n == 1 adds the king-move exit
+// candidates computed by k_init into prevW[stm].
+__global__ __launch_bounds__(256) void k_sweep_j(JobDev *__restrict__ jobs,
+                                                  const Pair *__restrict__ pairs,
+                                                  const uint64_t *__restrict__ P, int np, int sw0)
 {
-  uint64_t mycnt = 0;
-  uint64_t stride = (uint64_t)gridDim.x * blockDim.x * 2;
-  for (uint64_t i = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * 2; i < nw; i += stride) {
-    uint64_t a = w0 + i;
-    ulonglong2 c = *reinterpret_cast<const ulonglong2 *>(cand + a);
-    ulonglong2 wi = *reinterpret_cast<const ulonglong2 *>(wins + a);
-    ulonglong2 wn;
-    wn.x = c.x & ~wi.x;
-    wn.y = c.y & ~wi.y;
-    *reinterpret_cast<ulonglong2 *>(newW + a) = wn;
-    if (wn.x | wn.y) {
-      ulonglong2 o2;
-      o2.x = wi.x | wn.x;
-      o2.y = wi.y | wn.y;
-      *reinterpret_cast<ulonglong2 *>(wins + a) = o2;
-    }
-    if (c.x | c.y) *reinterpret_cast<ulonglong2 *>(cand + a) = make_ulonglong2(0, 0);
-    mycnt += popc64(wn.x) + popc64(wn.y);
-    if (codes && (wn.x | wn.y)) {
-      uint16_t code = (uint16_t)((n << 2) | 1);
-      for (int h = 0; h < 2; h++) {
-        uint64_t r = h ? wn.y : wn.x;
-        uint64_t wa = a + h;
-        uint16_t *cp = codes + (wa / words) * size + (wa % words) * 64;
-        while (r) {
-          int b = ctz64(r);
-          r &= r - 1;
-          cp[b] = code;
+  __shared__ Mat Ms;
+  int cached = -1;
+  int lane = threadIdx.x & 63;
+  uint64_t total = P[np];
+  for (uint64_t u = blockIdx.x; u < total; u += gridDim.x) {
+    int pi = find_pair(P, np, u);
+    Pair pr = pairs[pi];
+    JobDev &J = jobs[pr.job];
+    if (J.done) continue;  // uniform across the work-group
+    cache_mat(Ms, J.M, pr.job, &cached);
+    const Mat &M = Ms;
+    int stm = pr.side, n = J.n;
+    bool run = (J.mask >> (2 + stm)) & 1;
+    bool exits = run && M.synth && n == 1;
+    uint64_t within;
+    int s = unit_slice(M, sw0, u - P[pi], (M.words + SWEEP_UNIT_WORDS - 1) / SWEEP_UNIT_WORDS,
+                       &within);
+    uint64_t *cand = J.cand[stm], *wins = J.wins[stm], *newW = J.newW[stm];
+    const uint64_t *kex = J.prevW[stm];
+    uint16_t *codes = J.codes[stm];
+    uint64_t base = phys(M, s) * M.words;
+    uint64_t wend = (within + 1) * SWEEP_UNIT_WORDS < M.words ? (within + 1) * SWEEP_UNIT_WORDS : M.words;
+    uint64_t mycnt = 0;
+    for (uint64_t w = within * SWEEP_UNIT_WORDS + threadIdx.x * 2; w < wend; w += 512) {
+      uint64_t a = base + w;
+      ulonglong2 c = *reinterpret_cast<const ulonglong2 *>(cand + a);
+      ulonglong2 cc = c;
+      if (exits) {
+        ulonglong2 e = *reinterpret_cast<const ulonglong2 *>(kex + a);
+        cc.x |= e.x;
+        cc.y |= e.y;
+      }
+      if (!run) cc = make_ulonglong2(0, 0);
+      ulonglong2 wn = make_ulonglong2(0, 0);
+      if (cc.x | cc.y) {
+        ulonglong2 wi = *reinterpret_cast<const ulonglong2 *>(wins + a);
+        wn.x = cc.x & ~wi.x;
+        wn.y = cc.y & ~wi.y;
+        if (wn.x | wn.y) {
+          ulonglong2 o2;
+          o2.x = wi.x | wn.x;
+          o2.y = wi.y | wn.y;
+          *reinterpret_cast<ulonglong2 *>(wins + a) = o2;
+        }
+      }
+      *reinterpret_cast<ulonglong2 *>(newW + a) = wn;
+      if (c.x | c.y) *reinterpret_cast<ulonglong2 *>(cand + a) = make_ulonglong2(0, 0);
+      mycnt += popc64(wn.x) + popc64(wn.y);
+      if (codes && (wn.x | wn.y)) {
+        uint16_t code = (uint16_t)((n << 2) | 1);
+        for (int h = 0; h < 2; h++) {
+          uint64_t r = h ? wn.y : wn.x;
+          uint16_t *cp = codes + phys(M, s) * M.ii.size + (w + h) * 64;
+          while (r) {
+            int b = ctz64(r);
+            r &= r - 1;
+            cp[b] = code;
+          }
         }
       }
     }
+    mycnt = wave_sum64(mycnt);
+    if (lane == 0 && mycnt) atomicAdd(&J.cnt[2 + stm], (unsigned long long)mycnt);
   }
-  int lane = threadIdx.x & 63;
-  mycnt = wave_sum64(mycnt);
-  if (lane == 0 && mycnt) atomicAdd(cnt, (unsigned long long)mycnt);
+}
+
+// End of a ply for every job (one lane per job): record the counts, derive
+// the next ply's flags exactly as main() does (src/tb8gen.c:1416-1423: an
+// L(stm) result enables W(stm^1), a W(stm) result enables L(stm^1)), swap
+// the W/L planes, and advance n.  Plies 2..100 in which nothing can run are
+// skipped to ply 101, which always runs (src/tb8gen.c:1426-1433); after it
+// the job ends once every flag is false (src/tb8gen.c:1435).
+__device__ void ctrl_one(JobDev &J, unsigned *flags, unsigned *any_edge);
+
+__global__ __launch_bounds__(1024) void k_ctrl(JobDev *jobs, int nj, unsigned *flags)
+{
+  __shared__ unsigned any_edge;
+  if (threadIdx.x == 0) any_edge = 0;
+  __syncthreads();
+  for (int j = threadIdx.x; j < nj; j += blockDim.x)
+    if (!jobs[j].done) ctrl_one(jobs[j], flags, &any_edge);
+  __syncthreads();
+  if (threadIdx.x == 0) flags[1] = any_edge;
+}
+
+__device__ void ctrl_one(JobDev &J, unsigned *flags, unsigned *any_edge)
+{
+  int n = J.n;
+  if (J.hist)
+    for (int i = 0; i < 4; i++)
+      J.hist[(uint64_t)n * 4 + i] = ((J.mask >> i) & 1) ? (long long)J.cnt[i] : -1;
+  bool lw = (J.mask & 1) && J.cnt[0], lb = (J.mask & 2) && J.cnt[1];
+  bool ww = (J.mask & 4) && J.cnt[2], wb = (J.mask & 8) && J.cnt[3];
+  // more_ww <- L_n(B), more_wb <- L_n(W), more_lw <- W_n(B), more_lb <- W_n(W)
+  unsigned more = (lb ? 1u : 0u) | (lw ? 2u : 0u) | (wb ? 4u : 0u) | (ww ? 8u : 0u);
+  for (int c = 0; c < 2; c++) {
+    uint64_t *x = J.prevW[c];
+    J.prevW[c] = J.newW[c];
+    J.newW[c] = x;
+    x = J.prevL[c];
+    J.prevL[c] = J.newL[c];
+    J.newL[c] = x;
+  }
+  for (int i = 0; i < 4; i++) J.cnt[i] = 0;
+  n++;
+  J.more = more;
+  if ((n > DRAW_RULE + 1 && !more) || n >= J.max_ply) {
+    J.n = n;
+    J.mask = 0;
+    J.done = 1;
+    atomicAdd(&flags[0], 1u);
+    return;
+  }
+  if (n < DRAW_RULE + 1 && !more) n = DRAW_RULE + 1;
+  bool forced = n == DRAW_RULE + 1;
+  J.mask = ((forced || (more & 4)) ? 1u : 0u) | ((forced || (more & 8)) ? 2u : 0u) |
+           ((forced || (more & 1)) ? 4u : 0u) | ((forced || (more & 2)) ? 8u : 0u);
+  J.n = n;
+  if (forced) atomicOr(any_edge, 1u);
 }
 
 // Synthetic job, ply 1: does stm's king have a legal move or capture into
@@ -731,21 +897,6 @@ __global__ __launch_bounds__(256) void k_resolve(const Mat *__restrict__ Mp,
       bloss[a] = b1;
       cwin[a] = b2;
       win[a] = b3;
-    }
-  }
-}
-
-// dst |= src over nw words (2 per lane)
-__global__ __launch_bounds__(256) void k_or(uint64_t *dst, const uint64_t *__restrict__ src, uint64_t nw)
-{
-  uint64_t stride = (uint64_t)gridDim.x * blockDim.x * 2;
-  for (uint64_t i = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * 2; i < nw; i += stride) {
-    ulonglong2 a = *reinterpret_cast<const ulonglong2 *>(src + i);
-    if (a.x | a.y) {
-      ulonglong2 d = *reinterpret_cast<const ulonglong2 *>(dst + i);
-      d.x |= a.x;
-      d.y |= a.y;
-      *reinterpret_cast<ulonglong2 *>(dst + i) = d;
     }
   }
 }

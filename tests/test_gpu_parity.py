@@ -160,3 +160,78 @@ def test_synthetic_block_scatter_multi_block():
         # the oracle was cut at ply 12; compare the common prefix
         assert lines[:len(olines)] == olines
         t.close()
+
+
+def test_graph_form(registry):
+    """Each ply captured once in a hipGraph and replayed, with the step flags
+    and the ply counter kept on the device."""
+    material = "KQvKR"
+    registry.get(material)
+    print("case graph KQvKR", flush=True)
+    t = tbg.Table(material)
+    t.resolve(list(registry.tables.values()))
+    lines = t.generate(tbg.FORM_GRAPH)
+    assert lines == oracle_lines(material)
+    assert_same_table(material, t)
+    t.close()
+
+
+@pytest.mark.parametrize("form", [tbg.FORM_BATCHED, tbg.FORM_GRAPH])
+def test_batch_of_materials_equals_each_alone(registry, form):
+    """Several 4-piece materials in one batch (every launch covers all of
+    them): each table and count line equals the oracle's, as alone."""
+    mats = ["KQvKR", "KRvKN", "KBNvK", "KRRvK", "KBBvK", "KQvKQ"]
+    for m in mats:
+        registry.get(m)
+    base = [registry.tables[m] for m in ("KQvK", "KRvK", "KBvK", "KNvK")]
+    print(f"case batch form {form}", flush=True)
+    batch = [tbg.Table(m) for m in mats]
+    for t in batch:
+        t.resolve(base)
+    all_lines = tbg.generate_batch(batch, form)
+    for m, t, lines in zip(mats, batch, all_lines):
+        assert lines == oracle_lines(m), m
+        assert_same_table(m, t)
+        t.close()
+
+
+def test_host_driven_steps_follow_main_loop(registry):
+    """tbg_step_L / tbg_step_W / tbg_end_ply driven by the reference's own
+    ply loop (tb8gen.c:1414-1444) give the same table and lines."""
+    material = "KRvKN"
+    registry.get(material)
+    print("case manual KRvKN", flush=True)
+    t = tbg.Table(material)
+    t.resolve(list(registry.tables.values()))
+    c = t.init_illegal_mate()
+    lines = [f"broken_w = {c.broken[0]}", f"broken_b = {c.broken[1]}",
+             f"l0_w = {c.mates[0]}", f"l0_b = {c.mates[1]}"]
+
+    def L(stm, n, more):
+        k = t.step("L", stm, n, more)
+        lines.append(f"l{n}_{'wb'[stm]} = {k}")
+        return k != 0
+
+    def W(stm, n, more):
+        k = t.step("W", stm, n, more)
+        lines.append(f"w{n}_{'wb'[stm]} = {k}")
+        return k != 0
+
+    more_ww = more_wb = more_lw = more_lb = True
+    n = 1
+    while True:
+        forced = n == 101
+        if n > 101 and not (more_ww or more_wb or more_lw or more_lb):
+            break
+        wb_next = L(0, n, more_lw) if (forced or more_lw) else False
+        ww_next = L(1, n, more_lb) if (forced or more_lb) else False
+        more_lb = W(0, n, more_ww) if (forced or more_ww) else False
+        more_lw = W(1, n, more_wb) if (forced or more_wb) else False
+        more_wb, more_ww = wb_next, ww_next
+        t.end_ply()
+        n += 1
+    assert lines == oracle_lines(material)
+    # manual steps do not record codes for plies through tbg_generate's
+    # history, but the codes array is written by the kernels themselves
+    assert_same_table(material, t)
+    t.close()
