@@ -53,6 +53,143 @@ def run_steps(table, subs, form, steps, tbg):
     return secs, timing, lines
 
 
+def pawnless_materials(extra):
+    """Every pawnless material with `extra` non-king pieces, in the colour
+    orientation the engine generates (stronger side white)."""
+    import itertools
+
+    import tb8gen_amd as tbg
+    out = []
+    for w in range(extra, -1, -1):
+        b = extra - w
+        if b > w:
+            continue
+        for ws in itertools.combinations_with_replacement("QRBN", w):
+            for bs in itertools.combinations_with_replacement("QRBN", b):
+                m = tbg.canonical("K" + "".join(ws) + "vK" + "".join(bs))
+                if m not in out:
+                    out.append(m)
+    return out
+
+
+def run_stream(args, tbg):
+    """Config 3: every pawnless 3-, 4- and 5-piece material (4 + 20 + 60)
+    as a stream of king-slice jobs, generated level by level in dependency
+    order, each level as one batch (one launch per phase covers all of its
+    materials x 462 slices x both sides).  Optionally the 4-piece level again
+    in the plain form (one slice per launch) for the ratio."""
+    form = {"batched": tbg.FORM_BATCHED, "graph": tbg.FORM_GRAPH, "plain": tbg.FORM_PLAIN}[args.form]
+    levels = [pawnless_materials(1), pawnless_materials(2), pawnless_materials(3)]
+    tables = {}
+    for lev in levels:  # allocation is not part of the timed region
+        for m in lev:
+            tables[m] = tbg.Table(m)
+    done = []
+    per_level = []
+    total_s, total_psp = 0.0, 0.0
+    for lev in levels:
+        t0 = time.perf_counter()
+        subs = [tables[m] for m in done]
+        for m in lev:
+            tables[m].resolve(subs)
+        batch = [tables[m] for m in lev]
+        lines = tbg.generate_batch(batch, form)
+        dt = time.perf_counter() - t0
+        psp = sum((len(l) - 4) * tbg.NSLICES * t.size / 2.0 for l, t in zip(lines, batch))
+        per_level.append({"pieces": len(lev[0]) - 1, "materials": len(lev), "s": dt,
+                          "position_side_plies": psp, "value": psp / dt})
+        print(f"level {len(lev[0]) - 1}: {len(lev)} materials, {dt:.2f} s, "
+              f"{psp / dt:.3e} position-side-plies/s", file=sys.stderr, flush=True)
+        total_s += dt
+        total_psp += psp
+        done += lev
+    out = {
+        "metric": METRIC, "value": total_psp / total_s, "unit": "position-side-plies/s",
+        "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": 1000 * total_s,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
+        "data": "synthetic",
+        "config": {"workload": f"stream of all {sum(len(l) for l in levels)} pawnless 3/4/5-piece "
+                               f"materials, {args.form} form, level by level",
+                   "form": args.form, "levels": per_level},
+    }
+    if args.stream_plain:
+        lev = levels[1]
+        base = [tables[m] for m in levels[0]]
+        fresh = [tbg.Table(m) for m in lev]
+        t0 = time.perf_counter()
+        for t in fresh:
+            t.resolve(base)
+        plines = tbg.generate_batch(fresh, tbg.FORM_PLAIN)
+        pt = time.perf_counter() - t0
+        psp = sum((len(l) - 4) * tbg.NSLICES * t.size / 2.0 for l, t in zip(plines, fresh))
+        same = all(pl == tables[m].lines for pl, m in zip(plines, lev))
+        lvl4 = per_level[1]
+        out["plain_4piece_level"] = {"s": pt, "value": psp / pt,
+                                     "batched_over_plain": lvl4["value"] / (psp / pt),
+                                     "same_count_lines": same}
+        for t in fresh:
+            t.close()
+    print(json.dumps(out), flush=True)
+    for t in tables.values():
+        t.close()
+
+
+def run_served(args, tbg):
+    """Config 4: seeded Poisson arrivals of king-slice jobs (one king slice
+    of a 6-piece pawnless material each, seeded hash exits), served with
+    continuous batching under an HBM budget, against the plain server (one
+    job at a time, FIFO) on the same arrival stream; p50 / p99 latency
+    (arrival -> codes on the host) and throughput."""
+    import numpy as np
+    mats = pawnless_materials(4)
+    rng = np.random.default_rng(args.seed)
+    jobs = [(mats[int(rng.integers(len(mats)))], int(rng.integers(0, 441)),
+             int(rng.integers(1, 1 << 62))) for _ in range(args.jobs)]
+    budget = int(args.budget_gb * (1 << 30))
+
+    def serve(arrivals, form, slots):
+        srv = tbg.Server(budget, slots)
+        ids = [srv.submit(m, s, seed, a) for (m, s, seed), a in zip(jobs, arrivals)]
+        st = srv.run(form)
+        res = [srv.result(i) for i in ids]
+        srv.close()
+        lat = np.array([r.done_ms - r.arrival_ms for r in res if r.status == 0])
+        return st, res, lat
+
+    # capacity of the batched server: the whole job list arriving at once
+    st0, _, _ = serve([0.0] * len(jobs), tbg.FORM_BATCHED, args.max_active)
+    cap = len(jobs) / (st0.wall_ms / 1000.0)
+    lam = 0.7 * cap  # offered load: 70 % of the batched capacity
+    gaps = rng.exponential(1.0 / lam, len(jobs))
+    arrivals = list(np.cumsum(gaps) * 1000.0)
+    out_modes = {}
+    for name, form, slots in (("batched", tbg.FORM_BATCHED, args.max_active),
+                              ("plain", tbg.FORM_PLAIN, 1)):
+        st, res, lat = serve(arrivals, form, slots)
+        out_modes[name] = {
+            "wall_ms": st.wall_ms, "value": st.position_side_plies / (st.wall_ms / 1000.0),
+            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "mean_resident_gb": st.mean_resident_bytes / 2**30,
+            "peak_resident_gb": st.peak_resident_bytes / 2**30, "peak_active": st.peak_active,
+            "plies": st.plies, "jobs_done": st.jobs_done}
+        print(f"{name}: {out_modes[name]}", file=sys.stderr, flush=True)
+    b, p = out_modes["batched"], out_modes["plain"]
+    out = {
+        "metric": METRIC, "value": b["value"], "unit": "position-side-plies/s", "n_gpus": 1,
+        "steps": 1, "warmup": 0, "ms_per_step": b["wall_ms"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u64", "data": "synthetic",
+        "config": {"workload": f"served: {len(jobs)} king-slice jobs of 6-piece pawnless materials, "
+                               f"seeded Poisson arrivals at 70% of batched capacity "
+                               f"({lam:.1f} jobs/s), {args.budget_gb:g} GB budget, "
+                               f"{args.max_active} slots", "seed": args.seed,
+                   "lambda_jobs_per_s": lam, "budget_gb": args.budget_gb},
+        "latency_ms": {"batched_p50": b["p50_ms"], "batched_p99": b["p99_ms"],
+                       "plain_p50": p["p50_ms"], "plain_p99": p["p99_ms"]},
+        "served": out_modes, "batched_over_plain": b["value"] / p["value"],
+    }
+    print(json.dumps(out), flush=True)
+
+
 def run_synth(args, tbg):
     """One king slice at real shape with seeded Bernoulli(0.03) win / loss
     exits for captures and king moves; one step = capture exits + init + the
@@ -130,10 +267,15 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-form comparison")
     ap.add_argument("--cpu-plies", type=int, default=2)
     ap.add_argument("--scatter", default="block", choices=["block", "global"])
-    ap.add_argument("--workload", default="table", choices=["table", "synth"],
+    ap.add_argument("--workload", default="table", choices=["table", "synth", "stream", "served"],
                     help="table: a whole material (default KQRvKR); synth: one king slice of "
                          "--material with seeded hash exits (default KQRBvKQRN slice 170)")
     ap.add_argument("--slice", type=int, default=170)
+    ap.add_argument("--jobs", type=int, default=96, help="served: number of jobs")
+    ap.add_argument("--budget-gb", type=float, default=64.0, help="served: HBM admission budget")
+    ap.add_argument("--max-active", type=int, default=64, help="served: job slots")
+    ap.add_argument("--stream-plain", action="store_true",
+                    help="stream workload: also time the 4-piece level in the plain form")
     ap.add_argument("--seed", type=int, default=20261203)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (calibrates the "
@@ -162,6 +304,10 @@ def main():
     tbg.init(0)
     if args.workload == "synth":
         return run_synth(args, tbg)
+    if args.workload == "stream":
+        return run_stream(args, tbg)
+    if args.workload == "served":
+        return run_served(args, tbg)
     form = {"batched": tbg.FORM_BATCHED, "graph": tbg.FORM_GRAPH, "plain": tbg.FORM_PLAIN}[args.form]
     reg = tbg.Registry(tbg.FORM_BATCHED)
     for sub in tbg.sub_materials(args.material):

@@ -137,6 +137,48 @@ int tbg_generate_batch(tbg_ctx *const *ctxs, int nc, int form, tbg_counts *count
 /* The count lines of a generated context, in the reference's order. */
 int tbg_emit_lines(const tbg_ctx *ctx, tbg_line_fn fn, void *user);
 
+/* ---------------------------------------------------------------------
+ * Served form: a continuous-batching job server.  Jobs are synthetic
+ * single-king-slice jobs (tbg_create_synth) with their codes kept.  run()
+ * admits arrived jobs in FIFO order while the device-memory budget and the
+ * slot limit allow, runs one ply of every admitted job per launch sequence
+ * (jobs join and leave the running batch at ply boundaries), and spills
+ * each finished job's codes to pinned host memory on a side stream.  With
+ * TBG_FORM_PLAIN it serves one job at a time instead (the comparison).
+ * No counterpart in the reference, which generates one material per run.
+ * --------------------------------------------------------------------- */
+typedef struct tbg_server tbg_server;
+typedef struct tbg_job_spec {
+  const char *material;  /* e.g. "KQRvKQR"                                  */
+  int slice;             /* king slice, 0 <= slice < 441                    */
+  uint64_t seed;         /* exit hash seed                                  */
+  double arrival_ms;     /* arrival time after the start of tbg_server_run  */
+} tbg_job_spec;
+typedef struct tbg_job_result {
+  int status;            /* TBG_OK, TBG_E_BUDGET (never fits), ...          */
+  int nplies;
+  double arrival_ms, admit_ms, done_ms; /* done = codes on the host        */
+  uint64_t positions;    /* per side                                        */
+  uint64_t device_bytes;
+  uint64_t half_steps;
+} tbg_job_result;
+typedef struct tbg_server_stats {
+  double wall_ms;
+  uint64_t jobs_done, jobs_rejected;
+  double mean_resident_bytes;  /* time-averaged device bytes of admitted jobs */
+  uint64_t peak_resident_bytes;
+  int peak_active;
+  uint64_t plies;              /* launch sequences run                        */
+  double position_side_plies;  /* sum over jobs of half-steps x positions / 2 */
+} tbg_server_stats;
+int tbg_server_create(tbg_server **out, uint64_t budget_bytes, int max_active);
+void tbg_server_destroy(tbg_server *srv);
+int tbg_server_submit(tbg_server *srv, const tbg_job_spec *spec, uint64_t *job_id);
+int tbg_server_run(tbg_server *srv, int form, tbg_server_stats *stats);
+int tbg_server_result(tbg_server *srv, uint64_t job_id, tbg_job_result *out);
+int tbg_server_codes(tbg_server *srv, uint64_t job_id, int stm, uint16_t *dst, uint64_t n);
+int tbg_server_lines(tbg_server *srv, uint64_t job_id, tbg_line_fn fn, void *user);
+
 /* Data plane (device -> host, synchronous). */
 int tbg_get_codes(tbg_ctx *ctx, int stm, int slice, uint16_t *dst, uint64_t n);
 int tbg_get_bitmap(tbg_ctx *ctx, int plane, int stm, int slice, uint64_t *dst, uint64_t nwords);

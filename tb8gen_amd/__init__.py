@@ -30,7 +30,8 @@ ABI_SYMBOLS = [
     "tbg_step_W", "tbg_step_L_plain", "tbg_step_W_plain", "tbg_end_ply", "tbg_generate",
     "tbg_get_codes", "tbg_get_bitmap", "tbg_get_sub_bitmap", "tbg_get_timing",
     "tbg_set_profiling", "tbg_host_selftest", "tbg_set_scatter", "tbg_generate_batch",
-    "tbg_emit_lines",
+    "tbg_emit_lines", "tbg_server_create", "tbg_server_destroy", "tbg_server_submit",
+    "tbg_server_run", "tbg_server_result", "tbg_server_codes", "tbg_server_lines",
 ]
 SCATTER_GLOBAL, SCATTER_BLOCK = 0, 1
 
@@ -61,6 +62,25 @@ class Timing(ctypes.Structure):
                 ("n_verify", ctypes.c_uint64), ("n_sweep", ctypes.c_uint64),
                 ("n_uncap", ctypes.c_uint64), ("n_init", ctypes.c_uint64),
                 ("n_resolve", ctypes.c_uint64)]
+
+
+class JobSpec(ctypes.Structure):
+    _fields_ = [("material", ctypes.c_char_p), ("slice", ctypes.c_int), ("seed", ctypes.c_uint64),
+                ("arrival_ms", ctypes.c_double)]
+
+
+class JobResult(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int), ("nplies", ctypes.c_int), ("arrival_ms", ctypes.c_double),
+                ("admit_ms", ctypes.c_double), ("done_ms", ctypes.c_double),
+                ("positions", ctypes.c_uint64), ("device_bytes", ctypes.c_uint64),
+                ("half_steps", ctypes.c_uint64)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("wall_ms", ctypes.c_double), ("jobs_done", ctypes.c_uint64),
+                ("jobs_rejected", ctypes.c_uint64), ("mean_resident_bytes", ctypes.c_double),
+                ("peak_resident_bytes", ctypes.c_uint64), ("peak_active", ctypes.c_int),
+                ("plies", ctypes.c_uint64), ("position_side_plies", ctypes.c_double)]
 
 
 LINE_FN = ctypes.CFUNCTYPE(None, ctypes.c_char_p, ctypes.c_void_p)
@@ -103,6 +123,15 @@ def lib():
         L.tbg_generate_batch.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.c_int,
                                          ctypes.POINTER(Counts)]
         L.tbg_emit_lines.argtypes = [vp, LINE_FN, vp]
+        L.tbg_server_create.argtypes = [ctypes.POINTER(vp), ctypes.c_uint64, ctypes.c_int]
+        L.tbg_server_destroy.argtypes = [vp]
+        L.tbg_server_destroy.restype = None
+        L.tbg_server_submit.argtypes = [vp, ctypes.POINTER(JobSpec), ctypes.POINTER(ctypes.c_uint64)]
+        L.tbg_server_run.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ServerStats)]
+        L.tbg_server_result.argtypes = [vp, ctypes.c_uint64, ctypes.POINTER(JobResult)]
+        L.tbg_server_codes.argtypes = [vp, ctypes.c_uint64, ctypes.c_int,
+                                       ctypes.POINTER(ctypes.c_uint16), ctypes.c_uint64]
+        L.tbg_server_lines.argtypes = [vp, ctypes.c_uint64, LINE_FN, vp]
         L.tbg_host_selftest.argtypes = [ctypes.c_char_p, ctypes.c_int]
         _lib = L
     return _lib
@@ -248,6 +277,63 @@ def generate_batch(tables, form=FORM_GRAPH):
         t.counts = c
         t.emit_lines()
     return [t.lines for t in tables]
+
+
+class Server:
+    """Continuous-batching server of synthetic king-slice jobs (libtbg)."""
+
+    def __init__(self, budget_bytes, max_active=64):
+        init()
+        self.ptr = ctypes.c_void_p()
+        check(lib().tbg_server_create(ctypes.byref(self.ptr), int(budget_bytes), int(max_active)),
+              "server")
+        self.ids = []
+
+    def submit(self, material, slice_, seed, arrival_ms=0.0):
+        spec = JobSpec(material.encode(), int(slice_), int(seed), float(arrival_ms))
+        jid = ctypes.c_uint64()
+        check(lib().tbg_server_submit(self.ptr, ctypes.byref(spec), ctypes.byref(jid)), "submit")
+        self.ids.append(jid.value)
+        return jid.value
+
+    def run(self, form=FORM_BATCHED):
+        st = ServerStats()
+        check(lib().tbg_server_run(self.ptr, form, ctypes.byref(st)), "server run")
+        return st
+
+    def result(self, jid):
+        r = JobResult()
+        check(lib().tbg_server_result(self.ptr, jid, ctypes.byref(r)), "result")
+        return r
+
+    def codes(self, jid, stm):
+        r = self.result(jid)
+        out = np.zeros(r.positions, dtype=np.uint16)
+        check(lib().tbg_server_codes(self.ptr, jid, stm,
+                                     out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)),
+                                     r.positions), "codes")
+        return out
+
+    def lines(self, jid):
+        out = []
+
+        def cb(line, _u):
+            out.append(line.decode())
+
+        fn = LINE_FN(cb)
+        check(lib().tbg_server_lines(self.ptr, jid, fn, None), "lines")
+        return out
+
+    def close(self):
+        if self.ptr:
+            lib().tbg_server_destroy(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 _ORDER = "KQRBN"

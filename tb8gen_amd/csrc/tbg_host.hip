@@ -7,6 +7,10 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
+#include <chrono>
+#include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/tbg.h"
@@ -246,6 +250,7 @@ struct tbg_ctx {
   unsigned *dflags = nullptr;          // [0] jobs done, [1] uncapture ply pending
   JobDev hj;                           // host mirror of the job
   JobDev *dj = nullptr;                // the job on the device
+  JobDev **djp = nullptr;              // device array holding dj (a batch of one)
   long long *dhist = nullptr;
   std::vector<long long> hist;
   int max_ply = 4096;
@@ -399,6 +404,8 @@ int ctx_alloc(tbg_ctx *c, bool with_codes)
   HIPCHK(hipMemsetAsync(c->dcnt, 0, 16 * sizeof(unsigned long long), c->st));
   HIPCHK(hipMalloc(&c->dflags, 4 * sizeof(unsigned)));
   HIPCHK(hipMalloc(&c->dj, sizeof(JobDev)));
+  HIPCHK(hipMalloc(&c->djp, sizeof(JobDev *)));
+  HIPCHK(hipMemcpyAsync(c->djp, &c->dj, sizeof(JobDev *), hipMemcpyHostToDevice, c->st));
   HIPCHK(hipMalloc(&c->dhist, sizeof(long long) * 4 * c->max_ply));
   manager_order(WHITE, c->mgr[WHITE]);
   manager_order(BLACK, c->mgr[BLACK]);
@@ -463,7 +470,7 @@ unsigned cap_grid(uint64_t want, uint64_t cap) { return (unsigned)(want < 1 ? 1 
 
 // One launch of a job kernel.  kt: KT_*; kind: 0 L step, 1 W step.
 // prof: context receiving timings (null inside graph capture).
-int launch(tbg_ctx *prof, JobDev *jobs, const LaunchSet &L, int kt, int kind, int sw0,
+int launch(tbg_ctx *prof, JobDev *const *jobs, const LaunchSet &L, int kt, int kind, int sw0,
            const unsigned *flags, hipStream_t st)
 {
   uint64_t total = L.total[kt];
@@ -501,8 +508,8 @@ int launch(tbg_ctx *prof, JobDev *jobs, const LaunchSet &L, int kt, int kind, in
 // One whole ply of every job of a launch set (batched form; also what a
 // graph captures): L candidates and verification for both sides, then W
 // candidates and the sweep, then the control kernel.
-int launch_ply(tbg_ctx *prof, JobDev *jobs, int nj, const LaunchSet &L, unsigned *flags,
-               hipStream_t st, bool edge = true)
+int launch_ply(tbg_ctx *prof, JobDev *const *jobs, int nj, const LaunchSet &L, unsigned *flags,
+               hipStream_t st, bool edge = true, unsigned *done_out = nullptr)
 {
   if (edge) TRY(launch(prof, jobs, L, KT_UNCAP, 0, 0, flags, st));
   TRY(launch(prof, jobs, L, KT_PRED, 0, 0, flags, st));
@@ -510,7 +517,7 @@ int launch_ply(tbg_ctx *prof, JobDev *jobs, int nj, const LaunchSet &L, unsigned
   if (edge) TRY(launch(prof, jobs, L, KT_UNCAP, 1, 0, flags, st));
   TRY(launch(prof, jobs, L, KT_PRED, 1, 0, flags, st));
   TRY(launch(prof, jobs, L, KT_SWEEP, 1, 0, flags, st));
-  hipLaunchKernelGGL(k_ctrl, dim3(1), dim3(1024), 0, st, jobs, nj, flags);
+  hipLaunchKernelGGL(k_ctrl, dim3(1), dim3(1024), 0, st, jobs, nj, flags, done_out);
   HIPCHK(hipGetLastError());
   return TBG_OK;
 }
@@ -538,20 +545,20 @@ int manual_step(tbg_ctx *c, int kind, int stm, int n, int more, uint64_t *count,
   c->half_steps++;
   bool edge = n == 1 || n == DRAW_RULE + 1;
   if (!plain) {
-    if (edge) TRY(launch(c, c->dj, c->side[stm], KT_UNCAP, kind, 0, nullptr, c->st));
-    TRY(launch(c, c->dj, c->side[stm], KT_PRED, kind, 0, nullptr, c->st));
-    TRY(launch(c, c->dj, c->side[stm], kind == 0 ? KT_VERIFY : KT_SWEEP, kind, 0, nullptr, c->st));
+    if (edge) TRY(launch(c, c->djp, c->side[stm], KT_UNCAP, kind, 0, nullptr, c->st));
+    TRY(launch(c, c->djp, c->side[stm], KT_PRED, kind, 0, nullptr, c->st));
+    TRY(launch(c, c->djp, c->side[stm], kind == 0 ? KT_VERIFY : KT_SWEEP, kind, 0, nullptr, c->st));
   } else {
     // one slice per launch in the reference's manager order (kslice.c:41-89)
     int ns = (int)c->nsl;
     for (int i = 0; i < ns; i++) {
       int s = c->h.synth ? c->h.synth_slice : c->mgr[stm][i];
-      if (edge) TRY(launch(c, c->dj, c->plain[stm], KT_UNCAP, kind, s, nullptr, c->st));
-      TRY(launch(c, c->dj, c->plain[stm], KT_PRED, kind, s, nullptr, c->st));
+      if (edge) TRY(launch(c, c->djp, c->plain[stm], KT_UNCAP, kind, s, nullptr, c->st));
+      TRY(launch(c, c->djp, c->plain[stm], KT_PRED, kind, s, nullptr, c->st));
     }
     for (int i = 0; i < ns; i++) {
       int s = c->h.synth ? c->h.synth_slice : c->mgr[stm][i];
-      TRY(launch(c, c->dj, c->plain[stm], kind == 0 ? KT_VERIFY : KT_SWEEP, kind, s, nullptr, c->st));
+      TRY(launch(c, c->djp, c->plain[stm], kind == 0 ? KT_VERIFY : KT_SWEEP, kind, s, nullptr, c->st));
     }
   }
   unsigned long long v = 0;
@@ -623,16 +630,16 @@ int run_plain(tbg_ctx *c)
         }
         for (int i = 0; i < ns; i++) {
           int s = c->h.synth ? c->h.synth_slice : c->mgr[stm][i];
-          if (edge) TRY(launch(c, c->dj, c->plain[stm], KT_UNCAP, kind, s, nullptr, c->st));
-          TRY(launch(c, c->dj, c->plain[stm], KT_PRED, kind, s, nullptr, c->st));
+          if (edge) TRY(launch(c, c->djp, c->plain[stm], KT_UNCAP, kind, s, nullptr, c->st));
+          TRY(launch(c, c->djp, c->plain[stm], KT_PRED, kind, s, nullptr, c->st));
         }
         for (int i = 0; i < ns; i++) {
           int s = c->h.synth ? c->h.synth_slice : c->mgr[stm][i];
-          TRY(launch(c, c->dj, c->plain[stm], kind == 0 ? KT_VERIFY : KT_SWEEP, kind, s, nullptr,
+          TRY(launch(c, c->djp, c->plain[stm], kind == 0 ? KT_VERIFY : KT_SWEEP, kind, s, nullptr,
                      c->st));
         }
       }
-    hipLaunchKernelGGL(k_ctrl, dim3(1), dim3(1024), 0, c->st, c->dj, 1, c->dflags);
+    hipLaunchKernelGGL(k_ctrl, dim3(1), dim3(1024), 0, c->st, c->djp, 1, c->dflags, nullptr);
     HIPCHK(hipGetLastError());
   }
   return TBG_OK;
@@ -640,7 +647,7 @@ int run_plain(tbg_ctx *c)
 
 // Batched and graph forms over any set of jobs.  jobs: device array of nj
 // JobDev; L: their launch set.  Counts executed half-steps into *hs.
-int run_batch(tbg_ctx *prof, JobDev *jobs, int nj, const LaunchSet &L, unsigned *flags,
+int run_batch(tbg_ctx *prof, JobDev *const *jobs, int nj, const LaunchSet &L, unsigned *flags,
               hipStream_t st, bool graph, double *graph_ms)
 {
   unsigned f[2] = { 0, 1 };
@@ -771,7 +778,8 @@ const char *tbg_strerror(int err)
   }
 }
 
-static int create_common(tbg_ctx **out, const char *material, int synth, int slice, uint64_t seed)
+static int create_common(tbg_ctx **out, const char *material, int synth, int slice, uint64_t seed,
+                         bool keep_codes = false)
 {
   if (!out || !material) return fail(TBG_E_INVAL, "null argument");
   *out = nullptr;
@@ -783,7 +791,7 @@ static int create_common(tbg_ctx **out, const char *material, int synth, int sli
   c->h.synth = synth;
   c->h.synth_slice = slice;
   c->h.seed = seed;
-  r = ctx_alloc(c, !synth);
+  r = ctx_alloc(c, !synth || keep_codes);
   if (r != TBG_OK) {
     tbg_destroy(c);
     return r == TBG_E_HIP ? fail(TBG_E_NOMEM, "device allocation failed") : r;
@@ -816,6 +824,7 @@ void tbg_destroy(tbg_ctx *c)
   (void)hipFree(c->dcnt);
   (void)hipFree(c->dflags);
   (void)hipFree(c->dj);
+  (void)hipFree(c->djp);
   (void)hipFree(c->dhist);
   if (c->st) (void)hipStreamDestroy(c->st);
   delete c;
@@ -956,36 +965,33 @@ int tbg_generate_batch(tbg_ctx *const *cs, int nc, int form, tbg_counts *counts)
       TRY(finish_job(cs[i]));
     }
   } else if (nc == 1) {
-    TRY(run_batch(c0, c0->dj, 1, c0->both, c0->dflags, c0->st, form == TBG_FORM_GRAPH, &c0->graph_ms));
+    TRY(run_batch(c0, c0->djp, 1, c0->both, c0->dflags, c0->st, form == TBG_FORM_GRAPH, &c0->graph_ms));
     TRY(finish_job(c0));
   } else {
-    // the jobs of all contexts in one device array, one launch set
-    JobDev *dj = nullptr;
+    // the jobs of all contexts behind one pointer array, one launch set
+    JobDev **dj = nullptr;
     unsigned *df = nullptr;
-    HIPCHK(hipMalloc(&dj, sizeof(JobDev) * nc));
+    HIPCHK(hipMalloc(&dj, sizeof(JobDev *) * nc));
     HIPCHK(hipMalloc(&df, 4 * sizeof(unsigned)));
     std::vector<const JobDev *> hj;
     std::vector<const Mat *> mats;
     std::vector<Pair> pairs;
+    std::vector<JobDev *> ptrs;
     for (int i = 0; i < nc; i++) {
       HIPCHK(hipStreamSynchronize(cs[i]->st));
-      HIPCHK(hipMemcpyAsync(dj + i, &cs[i]->hj, sizeof(JobDev), hipMemcpyHostToDevice, c0->st));
+      ptrs.push_back(cs[i]->dj);
       hj.push_back(&cs[i]->hj);
       mats.push_back(&cs[i]->h);
       pairs.push_back({ i, 0 });
       pairs.push_back({ i, 1 });
     }
+    HIPCHK(hipMemcpyAsync(dj, ptrs.data(), sizeof(JobDev *) * nc, hipMemcpyHostToDevice, c0->st));
     unsigned f[4] = { 0, 1, 0, 0 };
     HIPCHK(hipMemcpyAsync(df, f, sizeof f, hipMemcpyHostToDevice, c0->st));
     LaunchSet L;
     int r = build_launchset(L, hj, mats, pairs, 0, c0->st);
     if (r == TBG_OK)
       r = run_batch(c0, dj, nc, L, df, c0->st, form == TBG_FORM_GRAPH, &c0->graph_ms);
-    if (r == TBG_OK)
-      for (int i = 0; i < nc; i++) {
-        hipError_t e = hipMemcpyAsync(cs[i]->dj, dj + i, sizeof(JobDev), hipMemcpyDeviceToDevice, c0->st);
-        if (e != hipSuccess) { r = fail(TBG_E_HIP, "copy back job: %s", hipGetErrorString(e)); break; }
-      }
     (void)hipStreamSynchronize(c0->st);
     L.release();
     (void)hipFree(dj);
@@ -1239,3 +1245,293 @@ extern "C" int tbg_host_selftest(char *msg, int len)
   return 0;
 #undef ST_FAIL
 }
+
+// ====================================================================
+// Served form: continuous-batching job server
+// ====================================================================
+namespace {
+
+struct SrvJob {
+  std::string material;
+  tbg_job_spec spec;
+  tbg_job_result res;
+  int state = 0;  // 0 queued, 1 active, 2 spilling, 3 done
+  tbg_ctx *ctx = nullptr;
+  uint16_t *pinned[2] = { nullptr, nullptr };
+  hipEvent_t ev = nullptr;
+  std::vector<long long> hist;
+  tbg_counts counts;
+};
+
+uint64_t estimate_bytes(const char *material, int *err)
+{
+  Mat M;
+  *err = setup_mat(M, material);
+  if (*err) return 0;
+  uint64_t pw = M.words, sw = 4 * (M.ctotal[0] + M.ctotal[1]);
+  return pw * 8 * 12 + sw * 8 + 2 * M.ii.size * 2 + 65536;
+}
+
+}  // namespace
+
+struct tbg_server {
+  uint64_t budget = 0;
+  int max_active = 1;
+  std::vector<SrvJob> jobs;
+  hipStream_t st = nullptr, side = nullptr;
+};
+
+extern "C" {
+
+int tbg_server_create(tbg_server **out, uint64_t budget_bytes, int max_active)
+{
+  if (!out || max_active < 1 || max_active > 1024) return fail(TBG_E_INVAL, "bad server arguments");
+  if (g_device < 0) return fail(TBG_E_STATE, "tbg_init not called");
+  tbg_server *s = new tbg_server();
+  s->budget = budget_bytes;
+  s->max_active = max_active;
+  hipError_t e1 = hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking);
+  hipError_t e2 = hipStreamCreateWithFlags(&s->side, hipStreamNonBlocking);
+  if (e1 != hipSuccess || e2 != hipSuccess) {
+    tbg_server_destroy(s);
+    return fail(TBG_E_HIP, "server streams");
+  }
+  *out = s;
+  return TBG_OK;
+}
+
+void tbg_server_destroy(tbg_server *s)
+{
+  if (!s) return;
+  if (s->st) (void)hipStreamSynchronize(s->st);
+  if (s->side) (void)hipStreamSynchronize(s->side);
+  for (auto &j : s->jobs) {
+    if (j.ctx) tbg_destroy(j.ctx);
+    for (auto &p : j.pinned) (void)hipHostFree(p);
+    if (j.ev) (void)hipEventDestroy(j.ev);
+  }
+  if (s->st) (void)hipStreamDestroy(s->st);
+  if (s->side) (void)hipStreamDestroy(s->side);
+  delete s;
+}
+
+int tbg_server_submit(tbg_server *s, const tbg_job_spec *spec, uint64_t *job_id)
+{
+  if (!s || !spec || !spec->material) return fail(TBG_E_INVAL, "null argument");
+  if (spec->slice < 0 || spec->slice >= 441) return fail(TBG_E_INVAL, "slice must be in [0, 441)");
+  int err;
+  estimate_bytes(spec->material, &err);
+  if (err) return err;
+  SrvJob j;
+  j.material = spec->material;
+  j.spec = *spec;
+  j.spec.material = nullptr;
+  memset(&j.res, 0, sizeof j.res);
+  j.res.arrival_ms = spec->arrival_ms;
+  s->jobs.push_back(std::move(j));
+  if (job_id) *job_id = s->jobs.size() - 1;
+  return TBG_OK;
+}
+
+int tbg_server_run(tbg_server *s, int form, tbg_server_stats *stats)
+{
+  if (!s) return fail(TBG_E_INVAL, "null server");
+  using clk = std::chrono::steady_clock;
+  auto t0 = clk::now();
+  auto now_ms = [&]() { return std::chrono::duration<double, std::milli>(clk::now() - t0).count(); };
+  int max_active = form == TBG_FORM_PLAIN ? 1 : s->max_active;
+  std::vector<int> pending;
+  for (size_t i = 0; i < s->jobs.size(); i++)
+    if (s->jobs[i].state == 0) pending.push_back((int)i);
+  std::stable_sort(pending.begin(), pending.end(), [&](int a, int b) {
+    return s->jobs[a].spec.arrival_ms < s->jobs[b].spec.arrival_ms;
+  });
+  size_t next = 0;
+  std::vector<int> active, spilling;
+  uint64_t used = 0, peak = 0;
+  int peak_active = 0;
+  double occ_integral = 0, last_t = 0;
+  uint64_t plies = 0, done = 0, rejected = 0;
+  double psp = 0;
+  // batch state
+  JobDev **dptr = nullptr;
+  unsigned *dflags = nullptr, *ddone = nullptr;
+  HIPCHK(hipMalloc(&dptr, sizeof(JobDev *) * max_active));
+  HIPCHK(hipMalloc(&dflags, 4 * sizeof(unsigned)));
+  HIPCHK(hipMalloc(&ddone, sizeof(unsigned) * max_active));
+  LaunchSet L;
+  bool changed = true;
+  unsigned hflags[2] = { 0, 1 };
+  std::vector<unsigned> hdone(max_active);
+  int r = TBG_OK;
+  while (r == TBG_OK && (next < pending.size() || !active.empty() || !spilling.empty())) {
+    double t = now_ms();
+    occ_integral += (double)used * (t - last_t);
+    last_t = t;
+    // spills that reached the host end their jobs
+    for (size_t i = 0; i < spilling.size();) {
+      SrvJob &j = s->jobs[spilling[i]];
+      if (hipEventQuery(j.ev) == hipSuccess) {
+        j.res.done_ms = now_ms();
+        used -= j.res.device_bytes;
+        tbg_destroy(j.ctx);
+        j.ctx = nullptr;
+        j.state = 3;
+        done++;
+        spilling.erase(spilling.begin() + i);
+      } else {
+        i++;
+      }
+    }
+    // admission: arrived jobs in FIFO order while slots and budget allow
+    while (next < pending.size() && (int)active.size() < max_active) {
+      SrvJob &j = s->jobs[pending[next]];
+      if (j.spec.arrival_ms > now_ms()) break;
+      int err;
+      uint64_t need = estimate_bytes(j.material.c_str(), &err);
+      if (need > s->budget) {
+        j.res.status = TBG_E_BUDGET;
+        j.res.admit_ms = j.res.done_ms = now_ms();
+        j.state = 3;
+        rejected++;
+        next++;
+        continue;
+      }
+      if (used + need > s->budget) break;  // wait for memory
+      tbg_ctx *c = nullptr;
+      r = create_common(&c, j.material.c_str(), 1, j.spec.slice, j.spec.seed, true);
+      if (r != TBG_OK) break;
+      j.ctx = c;
+      r = tbg_resolve_captures(c, nullptr, 0);
+      if (r == TBG_OK) r = run_init(c);
+      if (r != TBG_OK) break;
+      j.res.device_bytes = c->bytes;
+      j.res.positions = c->h.ii.size;
+      j.res.admit_ms = now_ms();
+      used += c->bytes;
+      peak = used > peak ? used : peak;
+      j.state = 1;
+      active.push_back(pending[next]);
+      next++;
+      changed = true;
+    }
+    if (r != TBG_OK) break;
+    peak_active = (int)active.size() > peak_active ? (int)active.size() : peak_active;
+    if (active.empty()) {
+      if (!spilling.empty()) {
+        (void)hipStreamSynchronize(s->side);
+      } else if (next < pending.size()) {
+        double wait = s->jobs[pending[next]].spec.arrival_ms - now_ms();
+        if (wait > 0) std::this_thread::sleep_for(std::chrono::microseconds((int64_t)(wait * 1000)));
+      }
+      continue;
+    }
+    if (changed) {
+      std::vector<JobDev *> ptrs;
+      std::vector<const JobDev *> hj;
+      std::vector<const Mat *> mats;
+      std::vector<Pair> pairs;
+      for (size_t i = 0; i < active.size(); i++) {
+        tbg_ctx *c = s->jobs[active[i]].ctx;
+        HIPCHK(hipStreamSynchronize(c->st));
+        ptrs.push_back(c->dj);
+        hj.push_back(&c->hj);
+        mats.push_back(&c->h);
+        pairs.push_back({ (int)i, 0 });
+        pairs.push_back({ (int)i, 1 });
+      }
+      HIPCHK(hipMemcpyAsync(dptr, ptrs.data(), sizeof(JobDev *) * ptrs.size(), hipMemcpyHostToDevice,
+                            s->st));
+      hflags[0] = 0;
+      hflags[1] = 1;  // a new job may be at ply 1
+      HIPCHK(hipMemcpyAsync(dflags, hflags, sizeof hflags, hipMemcpyHostToDevice, s->st));
+      r = build_launchset(L, hj, mats, pairs, 0, s->st);
+      if (r != TBG_OK) break;
+      changed = false;
+    }
+    // one ply of every active job
+    int na = (int)active.size();
+    r = launch_ply(nullptr, dptr, na, L, dflags, s->st, hflags[1] != 0, ddone);
+    if (r != TBG_OK) break;
+    plies++;
+    HIPCHK(hipMemcpyAsync(hflags, dflags, sizeof hflags, hipMemcpyDeviceToHost, s->st));
+    HIPCHK(hipMemcpyAsync(hdone.data(), ddone, sizeof(unsigned) * na, hipMemcpyDeviceToHost, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
+    if (!hflags[0]) continue;
+    // finished jobs leave the batch; their codes go to pinned host memory
+    std::vector<int> keep;
+    for (int i = 0; i < na; i++) {
+      SrvJob &j = s->jobs[active[i]];
+      if (!hdone[i]) { keep.push_back(active[i]); continue; }
+      tbg_ctx *c = j.ctx;
+      r = finish_job(c);
+      if (r != TBG_OK) break;
+      j.hist = c->hist;
+      j.counts = c->counts;
+      j.res.nplies = c->counts.nplies;
+      j.res.half_steps = executed_half_steps(c);
+      psp += (double)j.res.half_steps * (double)c->h.ii.size / 2.0;
+      uint64_t n = c->h.ii.size;
+      for (int stm = 0; stm < 2; stm++) {
+        if (!j.pinned[stm]) HIPCHK(hipHostMalloc((void **)&j.pinned[stm], n * 2, hipHostMallocDefault));
+        HIPCHK(hipMemcpyAsync(j.pinned[stm], c->codes[stm], n * 2, hipMemcpyDeviceToHost, s->side));
+      }
+      if (!j.ev) HIPCHK(hipEventCreateWithFlags(&j.ev, hipEventDisableTiming));
+      HIPCHK(hipEventRecord(j.ev, s->side));
+      j.state = 2;
+      spilling.push_back(active[i]);
+    }
+    if (r != TBG_OK) break;
+    active = keep;
+    changed = true;
+  }
+  (void)hipStreamSynchronize(s->st);
+  (void)hipStreamSynchronize(s->side);
+  L.release();
+  (void)hipFree(dptr);
+  (void)hipFree(dflags);
+  (void)hipFree(ddone);
+  if (r != TBG_OK) return r;
+  if (stats) {
+    stats->wall_ms = now_ms();
+    stats->jobs_done = done;
+    stats->jobs_rejected = rejected;
+    stats->mean_resident_bytes = last_t > 0 ? occ_integral / last_t : 0;
+    stats->peak_resident_bytes = peak;
+    stats->peak_active = peak_active;
+    stats->plies = plies;
+    stats->position_side_plies = psp;
+  }
+  return TBG_OK;
+}
+
+int tbg_server_result(tbg_server *s, uint64_t id, tbg_job_result *out)
+{
+  if (!s || !out || id >= s->jobs.size()) return fail(TBG_E_INVAL, "bad job id");
+  *out = s->jobs[id].res;
+  return TBG_OK;
+}
+
+int tbg_server_codes(tbg_server *s, uint64_t id, int stm, uint16_t *dst, uint64_t n)
+{
+  if (!s || !dst || id >= s->jobs.size() || (stm & ~1)) return fail(TBG_E_INVAL, "bad argument");
+  SrvJob &j = s->jobs[id];
+  if (j.state != 3 || j.res.status != TBG_OK) return fail(TBG_E_STATE, "job has no result");
+  if (n > j.res.positions) return fail(TBG_E_INVAL, "too many codes");
+  memcpy(dst, j.pinned[stm], n * 2);
+  return TBG_OK;
+}
+
+int tbg_server_lines(tbg_server *s, uint64_t id, tbg_line_fn fn, void *user)
+{
+  if (!s || !fn || id >= s->jobs.size()) return fail(TBG_E_INVAL, "bad argument");
+  SrvJob &j = s->jobs[id];
+  if (j.state != 3 || j.res.status != TBG_OK) return fail(TBG_E_STATE, "job has no result");
+  tbg_ctx tmp;
+  tmp.counts = j.counts;
+  tmp.hist = j.hist;
+  emit_lines(&tmp, fn, user);
+  return TBG_OK;
+}
+
+}  // extern "C"

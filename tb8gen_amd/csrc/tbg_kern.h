@@ -64,14 +64,15 @@ TBG_HD uint64_t units_per_slice(const Mat &M, uint64_t nblk, int kt, int side)
 constexpr int PB_THREADS = 512;
 constexpr int PB_QUEUE_BYTES = 8 * 1024 * 2;
 
-__global__ void k_pred_j(const JobDev *jobs, const Pair *pairs, const uint64_t *P, int np, int sw0,
+__global__ void k_pred_j(JobDev *const *jobs, const Pair *pairs, const uint64_t *P, int np, int sw0,
                          int kind);
-__global__ void k_uncap_j(const JobDev *jobs, const Pair *pairs, const uint64_t *P, int np, int sw0,
+__global__ void k_uncap_j(JobDev *const *jobs, const Pair *pairs, const uint64_t *P, int np, int sw0,
                           int kind, const unsigned *flags);
-__global__ void k_verify_j(JobDev *jobs, const Pair *pairs, const uint64_t *P, int np, int sw0);
-__global__ void k_sweep_j(JobDev *jobs, const Pair *pairs, const uint64_t *P, int np, int sw0);
+__global__ void k_verify_j(JobDev *const *jobs, const Pair *pairs, const uint64_t *P, int np, int sw0);
+__global__ void k_sweep_j(JobDev *const *jobs, const Pair *pairs, const uint64_t *P, int np, int sw0);
 // flags[0]: jobs done; flags[1]: some job is at ply 1 or 101 (uncapture)
-__global__ void k_ctrl(JobDev *jobs, int nj, unsigned *flags);
+// done_out (nullable): done_out[j] = job j finished
+__global__ void k_ctrl(JobDev *const *jobs, int nj, unsigned *flags, unsigned *done_out);
 __global__ void k_init(const Mat *Mp, uint64_t *winsW, uint64_t *winsB, uint64_t *mateW,
                        uint64_t *mateB, uint16_t *codesW, uint16_t *codesB, uint64_t *kexW0,
                        uint64_t *kexW1, uint64_t *kexL0, uint64_t *kexL1, int s0, int ns,

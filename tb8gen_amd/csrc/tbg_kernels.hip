@@ -181,7 +181,7 @@ __device__ __forceinline__ int unit_slice(const Mat &M, int sw0, uint64_t local,
 constexpr int PB_WAVES = 8;   // waves per work-group
 constexpr int PB_CAP = 1024;  // queue entries per wave per round
 
-__global__ __launch_bounds__(512) void k_pred_j(const JobDev *__restrict__ jobs,
+__global__ __launch_bounds__(512) void k_pred_j(JobDev *const *__restrict__ jobs,
                                                  const Pair *__restrict__ pairs,
                                                  const uint64_t *__restrict__ P, int np, int sw0,
                                                  int kind)
@@ -197,7 +197,7 @@ __global__ __launch_bounds__(512) void k_pred_j(const JobDev *__restrict__ jobs,
   for (uint64_t u = blockIdx.x; u < total; u += gridDim.x) {
     int pi = find_pair(P, np, u);
     Pair pr = pairs[pi];
-    const JobDev &J = jobs[pr.job];
+    const JobDev &J = *jobs[pr.job];
     int stm = pr.side, o = stm ^ 1;
     bool en = !J.done && ((J.mask >> (kind * 2 + stm)) & 1);
     if (kind == 0) en = en && J.n > 1 && ((J.more >> (2 + stm)) & 1);  // more_lw / more_lb
@@ -288,7 +288,7 @@ __global__ __launch_bounds__(512) void k_pred_j(const JobDev *__restrict__ jobs,
 // uncapture from the stm^1 capture-exit bitmaps, at n == 1 (L: sub/win with
 // the legality filter; W: sub/loss) and n == 101 (L: sub/cwin & ~sub/win;
 // W: sub/bloss).
-__global__ __launch_bounds__(256) void k_uncap_j(const JobDev *__restrict__ jobs,
+__global__ __launch_bounds__(256) void k_uncap_j(JobDev *const *__restrict__ jobs,
                                                   const Pair *__restrict__ pairs,
                                                   const uint64_t *__restrict__ P, int np, int sw0,
                                                   int kind, const unsigned *__restrict__ flags)
@@ -304,7 +304,7 @@ __global__ __launch_bounds__(256) void k_uncap_j(const JobDev *__restrict__ jobs
   for (uint64_t u = blockIdx.x; u < total; u += gridDim.x) {
     int pi = find_pair(P, np, u);
     Pair pr = pairs[pi];
-    const JobDev &J = jobs[pr.job];
+    const JobDev &J = *jobs[pr.job];
     int stm = pr.side, o = stm ^ 1, n = J.n;
     bool en = !J.done && ((J.mask >> (kind * 2 + stm)) & 1) && (n == 1 || n == DRAW_RULE + 1);
     if (!en) continue;  // uniform across the work-group
@@ -479,7 +479,7 @@ __device__ bool verify_pos(const Tabs &T, const Mat &M, int stm, int s, uint64_t
 // a decode.  A step that does not run this ply writes an empty newL.  For a
 // synthetic job at n == 1 the king-move exit candidates (computed by k_init
 // into newW[stm]) join the candidates.
-__global__ __launch_bounds__(256) void k_verify_j(JobDev *__restrict__ jobs,
+__global__ __launch_bounds__(256) void k_verify_j(JobDev *const *__restrict__ jobs,
                                                    const Pair *__restrict__ pairs,
                                                    const uint64_t *__restrict__ P, int np, int sw0)
 {
@@ -493,7 +493,7 @@ __global__ __launch_bounds__(256) void k_verify_j(JobDev *__restrict__ jobs,
   for (uint64_t u = blockIdx.x; u < total; u += gridDim.x) {
     int pi = find_pair(P, np, u);
     Pair pr = pairs[pi];
-    JobDev &J = jobs[pr.job];
+    JobDev &J = *jobs[pr.job];
     if (J.done) continue;  // uniform across the work-group
     cache_mat(Ms, J.M, pr.job, &cached);
     const Mat &M = Ms;
@@ -561,7 +561,7 @@ __global__ __launch_bounds__(256) void k_verify_j(JobDev *__restrict__ jobs,
 // two per lane per pass with 16-byte accesses.  A step that does not run
 // writes an empty newW.  A synthetic job at n == 1 adds the king-move exit
 // candidates computed by k_init into prevW[stm].
-__global__ __launch_bounds__(256) void k_sweep_j(JobDev *__restrict__ jobs,
+__global__ __launch_bounds__(256) void k_sweep_j(JobDev *const *__restrict__ jobs,
                                                   const Pair *__restrict__ pairs,
                                                   const uint64_t *__restrict__ P, int np, int sw0)
 {
@@ -572,7 +572,7 @@ __global__ __launch_bounds__(256) void k_sweep_j(JobDev *__restrict__ jobs,
   for (uint64_t u = blockIdx.x; u < total; u += gridDim.x) {
     int pi = find_pair(P, np, u);
     Pair pr = pairs[pi];
-    JobDev &J = jobs[pr.job];
+    JobDev &J = *jobs[pr.job];
     if (J.done) continue;  // uniform across the work-group
     cache_mat(Ms, J.M, pr.job, &cached);
     const Mat &M = Ms;
@@ -639,13 +639,16 @@ __global__ __launch_bounds__(256) void k_sweep_j(JobDev *__restrict__ jobs,
 // the job ends once every flag is false (src/tb8gen.c:1435).
 __device__ void ctrl_one(JobDev &J, unsigned *flags, unsigned *any_edge);
 
-__global__ __launch_bounds__(1024) void k_ctrl(JobDev *jobs, int nj, unsigned *flags)
+__global__ __launch_bounds__(1024) void k_ctrl(JobDev *const *jobs, int nj, unsigned *flags,
+                                                unsigned *done_out)
 {
   __shared__ unsigned any_edge;
   if (threadIdx.x == 0) any_edge = 0;
   __syncthreads();
-  for (int j = threadIdx.x; j < nj; j += blockDim.x)
-    if (!jobs[j].done) ctrl_one(jobs[j], flags, &any_edge);
+  for (int j = threadIdx.x; j < nj; j += blockDim.x) {
+    if (!jobs[j]->done) ctrl_one(*jobs[j], flags, &any_edge);
+    if (done_out) done_out[j] = jobs[j]->done;
+  }
   __syncthreads();
   if (threadIdx.x == 0) flags[1] = any_edge;
 }

@@ -5,6 +5,11 @@ reference's format must agree, for materials whose captures exercise the
 whole chain (3-piece tables probe KvK, 4-piece probe 3-piece, 5-piece probe
 4-piece), for the batched and the plain (one slice per launch) forms.
 """
+import glob
+import json
+import os
+import zlib
+
 import numpy as np
 import pytest
 
@@ -91,24 +96,27 @@ def test_plain_form_equals_batched(registry):
     t.close()
 
 
-def test_five_piece_kqrvkr_against_oracle_digests(registry):
-    """KQRvKR (462 x 226,920 per side): count lines and per-slice CRC32 of
-    the codes equal the oracle's (tests/golden/oracle_KQRvKR.json)."""
-    import json
-    import os
-    import zlib
-    path = os.path.join(os.path.dirname(__file__), "golden", "oracle_KQRvKR.json")
+_DIGESTS = sorted(glob.glob(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                                         "oracle_*.json")))
+
+
+@pytest.mark.parametrize("path", _DIGESTS, ids=[os.path.basename(p)[7:-5] for p in _DIGESTS])
+def test_five_piece_against_oracle_digests(registry, path):
+    """5-piece tables (KQRvKR 462 x 226,920 per side; KQQQvK with a triple
+    set; KRRvKB, KBNvKN): count lines and per-slice CRC32 of the codes equal
+    the oracle's (tests/golden/oracle_*.json)."""
     with open(path) as f:
         ref = json.load(f)
-    print("case KQRvKR", flush=True)
-    t = registry.get("KQRvKR")
+    material = ref["material"]
+    print(f"case {material}", flush=True)
+    t = registry.get(material)
     assert t.size == ref["slice_size"]
     assert t.lines == ref["lines"]
     for stm in range(2):
         c = t.codes(stm)
         crc = [zlib.crc32(c[s].tobytes()) for s in range(c.shape[0])]
         bad = [s for s in range(len(crc)) if crc[s] != ref["crc32"][str(stm)][s]]
-        assert not bad, f"stm {stm}: slices {bad[:10]} differ"
+        assert not bad, f"{material} stm {stm}: slices {bad[:10]} differ"
 
 
 @pytest.mark.parametrize("material,slice_,seed", [("KQRvKR", 170, 7), ("KRBvKN", 57, 11),
@@ -235,3 +243,54 @@ def test_host_driven_steps_follow_main_loop(registry):
     # history, but the codes array is written by the kernels themselves
     assert_same_table(material, t)
     t.close()
+
+
+SERVED_JOBS = [("KQRvKR", 170, 7), ("KRBvKN", 57, 11), ("KRRvKB", 300, 3), ("KQNvKB", 12, 5),
+               ("KQRvKR", 440, 9), ("KBNvKN", 200, 2), ("KRRvKB", 13, 4), ("KQvKRN", 99, 8)]
+
+
+def _oracle_synth(material, s, seed):
+    log = f"/tmp/oracle_srv_{material}_{s}_{seed}.log"
+    o = O.Table(material, synth_slice=s, seed=seed, log_path=log)
+    with open(log) as f:
+        return o, [l.strip() for l in f if l.strip()]
+
+
+def test_served_jobs_same_alone_and_in_the_mix():
+    """Jobs arriving while others run join the running batch; each job's
+    codes and count lines equal the oracle's for that job alone."""
+    print("case served mix", flush=True)
+    srv = tbg.Server(budget_bytes=8 << 30, max_active=6)
+    ids = [srv.submit(m, s, seed, arrival_ms=3.0 * i) for i, (m, s, seed) in enumerate(SERVED_JOBS)]
+    st = srv.run(tbg.FORM_BATCHED)
+    assert st.jobs_done == len(SERVED_JOBS) and st.jobs_rejected == 0
+    assert st.peak_active > 1  # jobs overlapped on the device
+    for jid, (m, s, seed) in zip(ids, SERVED_JOBS):
+        o, olines = _oracle_synth(m, s, seed)
+        assert srv.lines(jid) == olines, (m, s)
+        for stm in range(2):
+            assert np.array_equal(srv.codes(jid, stm), o.codes(stm)[0]), (m, s, stm)
+        r = srv.result(jid)
+        assert r.arrival_ms <= r.admit_ms <= r.done_ms
+    srv.close()
+
+
+def test_served_admission_under_small_budget():
+    """A budget that holds one job at a time serialises admission without
+    changing results; a job larger than the whole budget is rejected."""
+    print("case served budget", flush=True)
+    probe = tbg.Table("KQRvKR", synth_slice=170, seed=7)
+    one = probe.info.device_bytes
+    probe.close()
+    srv = tbg.Server(budget_bytes=int(one * 1.6) + (1 << 20), max_active=8)
+    jobs = SERVED_JOBS[:3] + [("KQRBvKQR", 170, 1)]  # the 7-piece slice never fits
+    ids = [srv.submit(m, s, seed, arrival_ms=0.0) for (m, s, seed) in jobs]
+    st = srv.run(tbg.FORM_BATCHED)
+    assert st.jobs_rejected == 1 and st.jobs_done == 3
+    assert srv.result(ids[3]).status == -6  # TBG_E_BUDGET
+    assert st.peak_active == 1
+    for jid, (m, s, seed) in zip(ids[:3], jobs[:3]):
+        o, olines = _oracle_synth(m, s, seed)
+        assert srv.lines(jid) == olines
+        assert np.array_equal(srv.codes(jid, 0), o.codes(0)[0])
+    srv.close()
