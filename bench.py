@@ -84,6 +84,22 @@ def run_stream(args, tbg):
     for lev in levels:  # allocation is not part of the timed region
         for m in lev:
             tables[m] = tbg.Table(m)
+    geoms = {}
+    if args.autotune:
+        # untimed pass: each level generated once, its launch geometry tuned
+        # on its largest material and given to all of its materials
+        done = []
+        for lev in levels:
+            subs = [tables[m] for m in done]
+            for m in lev:
+                tables[m].resolve(subs)
+            big = max(lev, key=lambda m: tables[m].size)
+            g, _ = tables[big].autotune()
+            for m in lev:
+                tables[m].set_geometry(g)
+            tbg.generate_batch([tables[m] for m in lev], form)
+            geoms[len(lev[0]) - 1] = g.as_dict()
+            done += lev
     done = []
     per_level = []
     total_s, total_psp = 0.0, 0.0
@@ -97,7 +113,8 @@ def run_stream(args, tbg):
         dt = time.perf_counter() - t0
         psp = sum((len(l) - 4) * tbg.NSLICES * t.size / 2.0 for l, t in zip(lines, batch))
         per_level.append({"pieces": len(lev[0]) - 1, "materials": len(lev), "s": dt,
-                          "position_side_plies": psp, "value": psp / dt})
+                          "position_side_plies": psp, "value": psp / dt,
+                          "geometry": geoms.get(len(lev[0]) - 1)})
         print(f"level {len(lev[0]) - 1}: {len(lev)} materials, {dt:.2f} s, "
               f"{psp / dt:.3e} position-side-plies/s", file=sys.stderr, flush=True)
         total_s += dt
@@ -266,7 +283,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-form comparison")
     ap.add_argument("--cpu-plies", type=int, default=2)
-    ap.add_argument("--scatter", default="block", choices=["block", "global"])
+    ap.add_argument("--scatter", default="block", choices=["block", "global"],
+                    help="scatter mode when not autotuned")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="keep the built-in launch geometry (table and stream workloads)")
     ap.add_argument("--workload", default="table", choices=["table", "synth", "stream", "served"],
                     help="table: a whole material (default KQRvKR); synth: one king slice of "
                          "--material with seeded hash exits (default KQRBvKQRN slice 170)")
@@ -317,6 +337,11 @@ def main():
     table.set_scatter(tbg.SCATTER_BLOCK if args.scatter == "block" else tbg.SCATTER_GLOBAL)
     n_total = tbg.NSLICES * table.size
 
+    tuned = None
+    if args.autotune:
+        table.resolve(subs)
+        g, tms = table.autotune()
+        tuned = dict(g.as_dict(), trial_ms=tms)
     for _ in range(args.warmup):
         table.resolve(subs)
         table.generate(form)
@@ -357,7 +382,8 @@ def main():
             "workload": f"{args.material} table generation, {args.form} form: 462 king slices x "
                         f"{table.size} positions per side, capture exits from resident smaller "
                         "tables, init + ply loop to the fixpoint",
-            "material": args.material, "form": args.form, "scatter": args.scatter, "slices": 462,
+            "material": args.material, "form": args.form, "slices": 462,
+            "geometry": table.geometry().as_dict(), "autotuned": tuned is not None,
             "positions_per_side": n_total, "half_steps_per_step": half_steps,
             "plies": table.counts.nplies,
         },

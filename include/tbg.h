@@ -194,6 +194,35 @@ int tbg_get_sub_bitmap(tbg_ctx *ctx, int kind, int colour, int slice, uint64_t *
 #define TBG_SCATTER_BLOCK 1
 int tbg_set_scatter(tbg_ctx *ctx, int mode);
 
+/* Launch geometry of the ply kernels.  Results never depend on it; only
+ * speed does.  tbg_autotune picks one by measurement; tbg_set_geometry with
+ * a NULL context sets the default that later contexts and servers start
+ * from (a batch runs with the geometry of its first context). */
+typedef struct tbg_geometry {
+  int verify_unit_words;  /* bitmap words per verification work unit: multiple of 256, <= 65536 */
+  int sweep_unit_words;   /* bitmap words per W-sweep work unit: multiple of 512, <= 65536 */
+  int pred_wg_per_cu;     /* grid caps, work-groups per CU, 1..64 */
+  int uncap_wg_per_cu;
+  int verify_wg_per_cu;
+  int sweep_wg_per_cu;
+  int scatter;            /* TBG_SCATTER_* */
+} tbg_geometry;
+int tbg_set_geometry(tbg_ctx *ctx, const tbg_geometry *g);
+int tbg_get_geometry(const tbg_ctx *ctx, tbg_geometry *g);
+
+/* Autotuner: times a fixed set of candidate geometries on ctx's own
+ * generation (captures resolved first; each trial regenerates the table,
+ * stopping after max_plies plies when max_plies > 0) and applies the
+ * fastest: each kernel's parameters from its own device time, the
+ * combination confirmed by one more trial against the best single
+ * candidate.  cache_path (nullable) is a text file of earlier choices keyed
+ * by material, slice, GPU architecture, CU count and max_plies: a hit
+ * applies the stored geometry without trials (*ms = -1), a miss appends the
+ * choice.  After a bounded run the table must be generated again.  No
+ * counterpart in the reference, which has one fixed thread pool. */
+int tbg_autotune(tbg_ctx *ctx, int max_plies, const char *cache_path, tbg_geometry *out,
+                 double *ms);
+
 /* Last kernel timings of the context (HIP events on its stream). */
 typedef struct tbg_timing {
   double ms_total;          /* device time since tbg_set_profiling(ctx, 1)  */

@@ -32,6 +32,7 @@ ABI_SYMBOLS = [
     "tbg_set_profiling", "tbg_host_selftest", "tbg_set_scatter", "tbg_generate_batch",
     "tbg_emit_lines", "tbg_server_create", "tbg_server_destroy", "tbg_server_submit",
     "tbg_server_run", "tbg_server_result", "tbg_server_codes", "tbg_server_lines",
+    "tbg_set_geometry", "tbg_get_geometry", "tbg_autotune",
 ]
 SCATTER_GLOBAL, SCATTER_BLOCK = 0, 1
 
@@ -62,6 +63,16 @@ class Timing(ctypes.Structure):
                 ("n_verify", ctypes.c_uint64), ("n_sweep", ctypes.c_uint64),
                 ("n_uncap", ctypes.c_uint64), ("n_init", ctypes.c_uint64),
                 ("n_resolve", ctypes.c_uint64)]
+
+
+class Geometry(ctypes.Structure):
+    _fields_ = [("verify_unit_words", ctypes.c_int), ("sweep_unit_words", ctypes.c_int),
+                ("pred_wg_per_cu", ctypes.c_int), ("uncap_wg_per_cu", ctypes.c_int),
+                ("verify_wg_per_cu", ctypes.c_int), ("sweep_wg_per_cu", ctypes.c_int),
+                ("scatter", ctypes.c_int)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
 
 
 class JobSpec(ctypes.Structure):
@@ -133,6 +144,10 @@ def lib():
                                        ctypes.POINTER(ctypes.c_uint16), ctypes.c_uint64]
         L.tbg_server_lines.argtypes = [vp, ctypes.c_uint64, LINE_FN, vp]
         L.tbg_host_selftest.argtypes = [ctypes.c_char_p, ctypes.c_int]
+        L.tbg_set_geometry.argtypes = [vp, ctypes.POINTER(Geometry)]
+        L.tbg_get_geometry.argtypes = [vp, ctypes.POINTER(Geometry)]
+        L.tbg_autotune.argtypes = [vp, ctypes.c_int, ctypes.c_char_p, ctypes.POINTER(Geometry),
+                                   ctypes.POINTER(ctypes.c_double)]
         _lib = L
     return _lib
 
@@ -261,10 +276,40 @@ class Table:
     def set_profiling(self, on):
         check(lib().tbg_set_profiling(self.ptr, 1 if on else 0), "profiling")
 
+    def geometry(self):
+        g = Geometry()
+        check(lib().tbg_get_geometry(self.ptr, ctypes.byref(g)), "geometry")
+        return g
+
+    def set_geometry(self, g):
+        check(lib().tbg_set_geometry(self.ptr, ctypes.byref(g)), "geometry")
+
+    def autotune(self, max_plies=0, cache_path=None):
+        """Pick the launch geometry by measurement (captures resolved first);
+        returns (Geometry, ms of the chosen trial, -1 on a cache hit).  The
+        table must be generated again afterwards."""
+        g = Geometry()
+        ms = ctypes.c_double()
+        check(lib().tbg_autotune(self.ptr, int(max_plies),
+                                 cache_path.encode() if cache_path else None,
+                                 ctypes.byref(g), ctypes.byref(ms)), "autotune")
+        return g, ms.value
+
     def timing(self):
         t = Timing()
         check(lib().tbg_get_timing(self.ptr, ctypes.byref(t)), "timing")
         return t
+
+
+def default_geometry():
+    g = Geometry()
+    check(lib().tbg_get_geometry(None, ctypes.byref(g)), "geometry")
+    return g
+
+
+def set_default_geometry(g):
+    """Geometry of contexts and servers created from now on."""
+    check(lib().tbg_set_geometry(None, ctypes.byref(g)), "geometry")
 
 
 def generate_batch(tables, form=FORM_GRAPH):

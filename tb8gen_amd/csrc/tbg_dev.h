@@ -28,7 +28,7 @@ struct Idx {
   int numsets;
   int first[MAXS], mult[MAXS], last[MAXS];
   uint64_t size;
-  uint64_t factor[MAXS];
+  uint32_t factor[MAXS];     // below 2^32 for up to 8 pieces (at most 61*60*59*58*57)
   double inv[MAXS];          // 1.0 / factor[k], for digit extraction
 };
 
@@ -41,12 +41,14 @@ struct Mat {
   int setcol[MAXS];          // colour of each set
   uint64_t set_of;           // packed: set index of slot i in byte i (0xff for kings)
   uint64_t setmask80[MAXS];  // 0x80 in the byte of each slot of set k
+  uint64_t col80[2];         // 0x80 in the byte of each non-king slot of colour c
   Idx ii, capt[MAXS];
   uint64_t words;            // 64-bit words per slice bitmap
   uint64_t cwords[MAXS], coff[MAXS], ctotal[2];
   uint64_t nslices_phys;     // 462, or 1 for a synthetic single-slice job
   int synth, synth_slice;
   uint64_t seed;
+  int all_single;            // every set holds one piece (set k in slot 2 + k)
 };
 
 // Tables shared by all kernels (copied to LDS per work-group).
@@ -259,6 +261,38 @@ TBG_HD uint64_t idx_to_sq(const Tabs &T, const Idx &ii, uint64_t idx, uint64_t *
   }
   *pp = p;
   return occ;
+}
+
+// Index after the piece of set kj (all sets single) moves from a to b, from
+// the index before: its own rank changes, and each later piece strictly
+// between a and b shifts its rank by one (its set of lower occupied squares
+// gains or loses a).  occb = kings + pieces of the sets before kj;
+// rank_a = rank of a among the squares free of occb.
+TBG_HD uint64_t moved_idx(const Idx &ii, uint64_t idx, uint64_t p, int kj, int a, int b,
+                          uint64_t occb, uint32_t rank_a)
+{
+  uint32_t f = ii.factor[kj];
+  uint32_t rank_b = (uint32_t)(b - popc64(occb & (bit(b) - 1)));
+  int lo = a < b ? a : b, hi = a < b ? b : a;
+  uint32_t acc = 0;  // sum of the later factors: below 2 * factor[kj + 1]
+  for (int k = kj + 1; k < MAXS; k++) {
+    if (k >= ii.numsets) break;
+    int s = sq_get(p, 2 + k);
+    acc += (s > lo && s < hi) ? ii.factor[k] : 0u;
+  }
+  uint64_t r = idx + (uint64_t)rank_b * f - (uint64_t)rank_a * f;  // modulo 2^64
+  return a < b ? r + acc : r - acc;
+}
+
+// kings + pieces of the sets before kj (all sets single)
+TBG_HD uint64_t occ_before(uint64_t p, int kj)
+{
+  uint64_t o = bit(sq_get(p, 0)) | bit(sq_get(p, 1));
+  for (int k = 0; k < MAXS; k++) {
+    if (k >= kj) break;
+    o |= bit(sq_get(p, 2 + k));
+  }
+  return o;
 }
 
 // Occupancy of each set (bitboard of its members), for set-wise capture

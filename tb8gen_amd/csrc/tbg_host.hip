@@ -3,6 +3,7 @@
 // forms, and the generator's ply loop.
 #include <stdarg.h>
 #include <stddef.h>
+#include <assert.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -24,6 +25,10 @@ thread_local char g_errbuf[256];
 int g_device = -1;
 int g_cus = 256;
 Tabs g_htabs;
+// geometry and scatter given to new contexts and servers (tbg_set_geometry(NULL, ..))
+Geom g_geom = { 256, 4096, { 3, 16, 16, 16 } };
+int g_scatter = TBG_SCATTER_BLOCK;
+char g_arch[64] = "none";
 
 int fail(int code, const char *fmt, ...)
 {
@@ -136,7 +141,8 @@ void calc_idx(Idx &ii)
   uint64_t f = 1;
   for (int i = MAXS - 1; i >= 0; i--) {
     if (i >= ii.numsets) { ii.factor[i] = 0; ii.inv[i] = 0; continue; }
-    ii.factor[i] = f;
+    assert(f < (1ull << 32));  // the sets after the first hold at most 5 of 8 pieces
+    ii.factor[i] = (uint32_t)f;
     ii.inv[i] = 1.0 / (double)f;
     f *= sub[i];
   }
@@ -169,8 +175,12 @@ int setup_mat(Mat &M, const char *name)
   for (int i = 0; i < MAXP; i++) M.pt[i] = i < k ? pt[i] : 0;
   for (int c = 0; c < 2; c++) {
     M.nk[c] = 0;
+    M.col80[c] = 0;
     for (int i = 2; i < k; i++)
-      if ((pt[i] >> 3) == c) M.pcs[c][M.nk[c]++] = i;
+      if ((pt[i] >> 3) == c) {
+        M.pcs[c][M.nk[c]++] = i;
+        M.col80[c] |= 0x80ull << (8 * i);
+      }
   }
   M.set_of = ~0ull;
   int ns = 0;
@@ -186,6 +196,8 @@ int setup_mat(Mat &M, const char *name)
     ns++;
   }
   M.numsets = M.ii.numsets = ns;
+  M.all_single = 1;
+  for (int x = 0; x < ns; x++) M.all_single &= M.ii.mult[x] == 1;
   calc_idx(M.ii);
   M.words = ((M.ii.size + 511) / 512) * 8;
   for (int s = 0; s < ns; s++) {
@@ -226,6 +238,7 @@ struct LaunchSet {
   uint64_t total[4] = { 0, 0, 0, 0 };
   int np = 0;
   size_t pred_lds = 0;
+  Geom g;
   void release()
   {
     (void)hipFree(pairs);
@@ -264,7 +277,8 @@ struct tbg_ctx {
   double ms[CAT_N];
   uint64_t nl[CAT_N];
   uint64_t launches = 0, half_steps = 0;
-  int scatter = TBG_SCATTER_BLOCK;
+  int scatter = g_scatter;
+  Geom geom = g_geom;
   tbg_counts counts;
   double graph_ms = 0;
   uint64_t hs_generated = 0;  // half-steps executed by tbg_generate*
@@ -293,10 +307,11 @@ void choose_block(const Mat &M, uint64_t *blk_pos, uint64_t *nblk)
 // window_slices 0: each job's own slices (462, or 1 for a synthetic job)
 int build_launchset(LaunchSet &L, const std::vector<const JobDev *> &hjobs,
                     const std::vector<const Mat *> &mats, const std::vector<Pair> &pairs,
-                    uint64_t window_slices, hipStream_t st)
+                    uint64_t window_slices, const Geom &g, hipStream_t st)
 {
   L.release();
   L.np = (int)pairs.size();
+  L.g = g;
   uint64_t maxbw = 1;
   std::vector<uint64_t> P[4];
   for (int kt = 0; kt < 4; kt++) {
@@ -304,12 +319,17 @@ int build_launchset(LaunchSet &L, const std::vector<const JobDev *> &hjobs,
     for (size_t i = 0; i < pairs.size(); i++) {
       const Mat &M = *mats[pairs[i].job];
       uint64_t ns = window_slices ? window_slices : (M.synth ? 1 : NSL);
-      P[kt][i + 1] = P[kt][i] + ns * units_per_slice(M, hjobs[pairs[i].job]->nblk, kt, pairs[i].side);
+      P[kt][i + 1] = P[kt][i] + ns * units_per_slice(M, hjobs[pairs[i].job]->nblk, kt, pairs[i].side, g);
     }
     L.total[kt] = P[kt][pairs.size()];
   }
-  for (const JobDev *j : hjobs) maxbw = j->blk_words > maxbw ? j->blk_words : maxbw;
-  L.pred_lds = maxbw * 8 + PB_QUEUE_BYTES;
+  bool block = false;
+  for (const JobDev *j : hjobs) {
+    maxbw = j->blk_words > maxbw ? j->blk_words : maxbw;
+    block |= j->scatter == TBG_SCATTER_BLOCK;
+  }
+  // the queues follow the block bitmap, which only the block scatter uses
+  L.pred_lds = (block ? maxbw * 8 : 0) + PB_QUEUE_BYTES;
   HIPCHK(hipMalloc(&L.pairs, sizeof(Pair) * pairs.size()));
   HIPCHK(hipMemcpyAsync(L.pairs, pairs.data(), sizeof(Pair) * pairs.size(), hipMemcpyHostToDevice, st));
   for (int kt = 0; kt < 4; kt++) {
@@ -357,6 +377,23 @@ int download_job(tbg_ctx *c)
 {
   HIPCHK(hipMemcpyAsync(&c->hj, c->dj, sizeof(JobDev), hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
+  return TBG_OK;
+}
+
+// the context's own launch sets (both sides, one side, one slice per
+// launch for the plain form), with its geometry
+int build_ctx_launchsets(tbg_ctx *c)
+{
+  std::vector<const JobDev *> hj{ &c->hj };
+  std::vector<const Mat *> mats{ &c->h };
+  int r = build_launchset(c->both, hj, mats, { { 0, 0 }, { 0, 1 } }, 0, c->geom, c->st);
+  if (r != TBG_OK) return r;
+  for (int s = 0; s < 2; s++) {
+    r = build_launchset(c->side[s], hj, mats, { { 0, s } }, 0, c->geom, c->st);
+    if (r != TBG_OK) return r;
+    r = build_launchset(c->plain[s], hj, mats, { { 0, s } }, 1, c->geom, c->st);
+    if (r != TBG_OK) return r;
+  }
   return TBG_OK;
 }
 
@@ -412,16 +449,8 @@ int ctx_alloc(tbg_ctx *c, bool with_codes)
   for (int i = 0; i < CAT_N; i++) c->ms[i] = 0, c->nl[i] = 0;
   reset_job(c);
   {
-    std::vector<const JobDev *> hj{ &c->hj };
-    std::vector<const Mat *> mats{ &c->h };
-    int r = build_launchset(c->both, hj, mats, { { 0, 0 }, { 0, 1 } }, 0, c->st);
+    int r = build_ctx_launchsets(c);
     if (r != TBG_OK) return r;
-    for (int s = 0; s < 2; s++) {
-      r = build_launchset(c->side[s], hj, mats, { { 0, s } }, 0, c->st);
-      if (r != TBG_OK) return r;
-      r = build_launchset(c->plain[s], hj, mats, { { 0, s } }, 1, c->st);
-      if (r != TBG_OK) return r;
-    }
   }
   HIPCHK(hipStreamSynchronize(c->st));
   return TBG_OK;
@@ -478,26 +507,26 @@ int launch(tbg_ctx *prof, JobDev *const *jobs, const LaunchSet &L, int kt, int k
   switch (kt) {
   case KT_PRED: {
     Timed t(prof, CAT_PRED);
-    hipLaunchKernelGGL(k_pred_j, dim3(cap_grid(total, (uint64_t)g_cus * 3)), dim3(PB_THREADS),
+    hipLaunchKernelGGL(k_pred_j, dim3(cap_grid(total, (uint64_t)g_cus * L.g.wg[kt])), dim3(PB_THREADS),
                        L.pred_lds, st, jobs, L.pairs, L.P[kt], L.np, sw0, kind);
     break;
   }
   case KT_UNCAP: {
     Timed t(prof, CAT_UNCAP);
-    hipLaunchKernelGGL(k_uncap_j, dim3(cap_grid(total, (uint64_t)g_cus * 16)), dim3(256), 0, st,
+    hipLaunchKernelGGL(k_uncap_j, dim3(cap_grid(total, (uint64_t)g_cus * L.g.wg[kt])), dim3(256), 0, st,
                        jobs, L.pairs, L.P[kt], L.np, sw0, kind, flags);
     break;
   }
   case KT_VERIFY: {
     Timed t(prof, CAT_VERIFY);
-    hipLaunchKernelGGL(k_verify_j, dim3(cap_grid(total, (uint64_t)g_cus * 16)), dim3(256), 0, st,
-                       jobs, L.pairs, L.P[kt], L.np, sw0);
+    hipLaunchKernelGGL(k_verify_j, dim3(cap_grid(total, (uint64_t)g_cus * L.g.wg[kt])), dim3(256), 0, st,
+                       jobs, L.pairs, L.P[kt], L.np, sw0, (int)L.g.verify_uw);
     break;
   }
   default: {
     Timed t(prof, CAT_SWEEP);
-    hipLaunchKernelGGL(k_sweep_j, dim3(cap_grid(total, (uint64_t)g_cus * 16)), dim3(256), 0, st, jobs,
-                       L.pairs, L.P[kt], L.np, sw0);
+    hipLaunchKernelGGL(k_sweep_j, dim3(cap_grid(total, (uint64_t)g_cus * L.g.wg[kt])), dim3(256), 0, st, jobs,
+                       L.pairs, L.P[kt], L.np, sw0, (int)L.g.sweep_uw);
     break;
   }
   }
@@ -739,6 +768,17 @@ void emit_lines(const tbg_ctx *c, tbg_line_fn fn, void *user)
     }
 }
 
+// The scatter mode of the job, host and device copies alike (only that
+// field: a generation may be under way), so the pred LDS size of the launch
+// sets rebuilt next matches what the kernel uses.
+int push_scatter(tbg_ctx *c)
+{
+  c->hj.scatter = c->scatter;
+  HIPCHK(hipMemcpyAsync(&c->dj->scatter, &c->hj.scatter, sizeof(int), hipMemcpyHostToDevice, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return TBG_OK;
+}
+
 }  // namespace
 
 // ====================================================================
@@ -756,6 +796,9 @@ int tbg_init(int device)
   hipDeviceProp_t prop;
   HIPCHK(hipGetDeviceProperties(&prop, device));
   g_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+  snprintf(g_arch, sizeof g_arch, "%s", prop.gcnArchName);
+  for (char *q = g_arch; *q; q++)
+    if (*q == ' ' || *q == ':') *q = '_';
   build_tabs(g_htabs);
   HIPCHK(upload_tabs(&g_htabs));
   HIPCHK(hipDeviceSynchronize());
@@ -989,7 +1032,7 @@ int tbg_generate_batch(tbg_ctx *const *cs, int nc, int form, tbg_counts *counts)
     unsigned f[4] = { 0, 1, 0, 0 };
     HIPCHK(hipMemcpyAsync(df, f, sizeof f, hipMemcpyHostToDevice, c0->st));
     LaunchSet L;
-    int r = build_launchset(L, hj, mats, pairs, 0, c0->st);
+    int r = build_launchset(L, hj, mats, pairs, 0, c0->geom, c0->st);
     if (r == TBG_OK)
       r = run_batch(c0, dj, nc, L, df, c0->st, form == TBG_FORM_GRAPH, &c0->graph_ms);
     (void)hipStreamSynchronize(c0->st);
@@ -1068,7 +1111,199 @@ int tbg_set_scatter(tbg_ctx *c, int mode)
 {
   if (!c) return fail(TBG_E_INVAL, "null context");
   if (mode != TBG_SCATTER_GLOBAL && mode != TBG_SCATTER_BLOCK) return fail(TBG_E_INVAL, "bad scatter mode");
+  HIPCHK(hipStreamSynchronize(c->st));
   c->scatter = mode;
+  TRY(push_scatter(c));
+  return build_ctx_launchsets(c);
+}
+
+namespace {
+int check_geometry(const tbg_geometry *g)
+{
+  if (!g) return fail(TBG_E_INVAL, "null geometry");
+  if (g->verify_unit_words < 256 || g->verify_unit_words > 65536 || g->verify_unit_words % 256)
+    return fail(TBG_E_INVAL, "verify_unit_words must be a multiple of 256 in [256, 65536]");
+  if (g->sweep_unit_words < 512 || g->sweep_unit_words > 65536 || g->sweep_unit_words % 512)
+    return fail(TBG_E_INVAL, "sweep_unit_words must be a multiple of 512 in [512, 65536]");
+  const int wg[4] = { g->pred_wg_per_cu, g->uncap_wg_per_cu, g->verify_wg_per_cu, g->sweep_wg_per_cu };
+  for (int x : wg)
+    if (x < 1 || x > 64) return fail(TBG_E_INVAL, "work-groups per CU must be in [1, 64]");
+  if (g->scatter != TBG_SCATTER_GLOBAL && g->scatter != TBG_SCATTER_BLOCK)
+    return fail(TBG_E_INVAL, "bad scatter mode");
+  return TBG_OK;
+}
+
+Geom to_geom(const tbg_geometry *g)
+{
+  Geom r;
+  r.verify_uw = (uint32_t)g->verify_unit_words;
+  r.sweep_uw = (uint32_t)g->sweep_unit_words;
+  r.wg[KT_PRED] = (uint32_t)g->pred_wg_per_cu;
+  r.wg[KT_UNCAP] = (uint32_t)g->uncap_wg_per_cu;
+  r.wg[KT_VERIFY] = (uint32_t)g->verify_wg_per_cu;
+  r.wg[KT_SWEEP] = (uint32_t)g->sweep_wg_per_cu;
+  return r;
+}
+
+void from_geom(const Geom &r, int scatter, tbg_geometry *g)
+{
+  g->verify_unit_words = (int)r.verify_uw;
+  g->sweep_unit_words = (int)r.sweep_uw;
+  g->pred_wg_per_cu = (int)r.wg[KT_PRED];
+  g->uncap_wg_per_cu = (int)r.wg[KT_UNCAP];
+  g->verify_wg_per_cu = (int)r.wg[KT_VERIFY];
+  g->sweep_wg_per_cu = (int)r.wg[KT_SWEEP];
+  g->scatter = scatter;
+}
+}  // namespace
+
+int tbg_set_geometry(tbg_ctx *c, const tbg_geometry *g)
+{
+  TRY(check_geometry(g));
+  if (!c) {
+    g_geom = to_geom(g);
+    g_scatter = g->scatter;
+    return TBG_OK;
+  }
+  HIPCHK(hipStreamSynchronize(c->st));
+  c->geom = to_geom(g);
+  c->scatter = g->scatter;
+  TRY(push_scatter(c));
+  return build_ctx_launchsets(c);
+}
+
+int tbg_get_geometry(const tbg_ctx *c, tbg_geometry *g)
+{
+  if (!g) return fail(TBG_E_INVAL, "null geometry");
+  if (c) from_geom(c->geom, c->scatter, g);
+  else from_geom(g_geom, g_scatter, g);
+  return TBG_OK;
+}
+
+namespace {
+// Candidate geometries of the autotuner.  Trial i runs candidate i of every
+// kernel at once: the kernels of a ply run one after another on one stream,
+// so each kernel's device time depends on its own parameters only, and
+// each kernel's best candidate is read from its own time.
+const tbg_geometry kTune[] = {
+  // verify uw, sweep uw, pred wg, uncap wg, verify wg, sweep wg, scatter
+  { 256, 4096, 3, 16, 16, 16, TBG_SCATTER_BLOCK },  // the built-in default
+  { 1024, 2048, 2, 8, 16, 16, TBG_SCATTER_BLOCK },
+  { 4096, 16384, 4, 32, 8, 8, TBG_SCATTER_BLOCK },
+  { 512, 8192, 8, 16, 32, 16, TBG_SCATTER_GLOBAL },
+  { 2048, 1024, 3, 4, 16, 32, TBG_SCATTER_GLOBAL },
+  { 8192, 32768, 6, 16, 8, 8, TBG_SCATTER_BLOCK },
+};
+constexpr int kNTune = (int)(sizeof kTune / sizeof kTune[0]);
+
+void tune_key(const tbg_ctx *c, int max_plies, char *k, size_t n)
+{
+  snprintf(k, n, "%s %d %s %d %d", c->name, c->h.synth ? c->h.synth_slice : -1, g_arch, g_cus,
+           max_plies);
+}
+
+bool tune_lookup(const char *path, const char *key, tbg_geometry *g)
+{
+  FILE *f = path ? fopen(path, "r") : nullptr;
+  if (!f) return false;
+  char line[512];
+  bool hit = false;
+  while (fgets(line, sizeof line, f)) {
+    char mat[64], arch[64], k[256];
+    int sl, cus, mp;
+    tbg_geometry t;
+    double ms;
+    if (sscanf(line, "tbg-geometry %63s %d %63s %d %d %d %d %d %d %d %d %d %lf", mat, &sl, arch, &cus,
+               &mp, &t.verify_unit_words, &t.sweep_unit_words, &t.pred_wg_per_cu,
+               &t.uncap_wg_per_cu, &t.verify_wg_per_cu, &t.sweep_wg_per_cu, &t.scatter, &ms) != 13)
+      continue;
+    snprintf(k, sizeof k, "%s %d %s %d %d", mat, sl, arch, cus, mp);
+    if (strcmp(k, key) == 0 && check_geometry(&t) == TBG_OK) {
+      *g = t;  // the last matching line wins
+      hit = true;
+    }
+  }
+  fclose(f);
+  return hit;
+}
+
+// One timed generation (bounded to max_plies plies when > 0) with geometry
+// g; per-category device times into ms[], their sum returned in *total.
+int tune_trial(tbg_ctx *c, const tbg_geometry &g, int max_plies, double *ms, double *total)
+{
+  TRY(tbg_set_geometry(c, &g));
+  TRY(tbg_set_profiling(c, 1));
+  int saved = c->max_ply;
+  if (max_plies > 0 && max_plies + 1 < saved) c->max_ply = max_plies + 1;
+  tbg_ctx *one[1] = { c };
+  int r = tbg_generate_batch(one, 1, TBG_FORM_BATCHED, nullptr);
+  c->max_ply = saved;
+  if (r != TBG_OK) return r;
+  flush_timing(c);
+  *total = 0;
+  for (int i = 0; i < CAT_N; i++) {
+    ms[i] = c->ms[i];
+    *total += c->ms[i];
+  }
+  return TBG_OK;
+}
+}  // namespace
+
+int tbg_autotune(tbg_ctx *c, int max_plies, const char *cache_path, tbg_geometry *out, double *out_ms)
+{
+  if (!c) return fail(TBG_E_INVAL, "null context");
+  if (!c->resolved) return fail(TBG_E_STATE, "resolve captures of %s first", c->name);
+  if (max_plies < 0) return fail(TBG_E_INVAL, "max_plies must be >= 0");
+  char key[256];
+  tune_key(c, max_plies, key, sizeof key);
+  tbg_geometry best;
+  if (tune_lookup(cache_path, key, &best)) {
+    TRY(tbg_set_geometry(c, &best));
+    if (out) *out = best;
+    if (out_ms) *out_ms = -1;
+    return TBG_OK;
+  }
+  bool prof = c->prof;
+  double ms[kNTune][CAT_N], tot[kNTune];
+  for (int i = 0; i < kNTune; i++) TRY(tune_trial(c, kTune[i], max_plies, ms[i], &tot[i]));
+  // each kernel's fastest candidate
+  auto argmin = [&](int cat) {
+    int b = 0;
+    for (int i = 1; i < kNTune; i++)
+      if (ms[i][cat] < ms[b][cat]) b = i;
+    return b;
+  };
+  int bp = argmin(CAT_PRED), bv = argmin(CAT_VERIFY), bs = argmin(CAT_SWEEP), bu = argmin(CAT_UNCAP);
+  tbg_geometry mix;
+  mix.verify_unit_words = kTune[bv].verify_unit_words;
+  mix.verify_wg_per_cu = kTune[bv].verify_wg_per_cu;
+  mix.sweep_unit_words = kTune[bs].sweep_unit_words;
+  mix.sweep_wg_per_cu = kTune[bs].sweep_wg_per_cu;
+  mix.pred_wg_per_cu = kTune[bp].pred_wg_per_cu;
+  mix.scatter = kTune[bp].scatter;
+  mix.uncap_wg_per_cu = kTune[bu].uncap_wg_per_cu;
+  double mms[CAT_N], mtot;
+  TRY(tune_trial(c, mix, max_plies, mms, &mtot));
+  // the combination is confirmed against the best whole trial
+  int bt = 0;
+  for (int i = 1; i < kNTune; i++)
+    if (tot[i] < tot[bt]) bt = i;
+  best = mtot <= tot[bt] ? mix : kTune[bt];
+  double best_ms = mtot <= tot[bt] ? mtot : tot[bt];
+  TRY(tbg_set_geometry(c, &best));
+  TRY(tbg_set_profiling(c, prof));
+  if (max_plies > 0) c->generated = 0;  // a bounded trial leaves a partial table
+  if (cache_path) {
+    FILE *f = fopen(cache_path, "a");
+    if (f) {
+      fprintf(f, "tbg-geometry %s %d %d %d %d %d %d %d %.4f\n", key, best.verify_unit_words,
+              best.sweep_unit_words, best.pred_wg_per_cu, best.uncap_wg_per_cu,
+              best.verify_wg_per_cu, best.sweep_wg_per_cu, best.scatter, best_ms);
+      fclose(f);
+    }
+  }
+  if (out) *out = best;
+  if (out_ms) *out_ms = best_ms;
   return TBG_OK;
 }
 
@@ -1241,6 +1476,29 @@ extern "C" int tbg_host_selftest(char *msg, int len)
       }
     }
   }
+  // incremental index of a moved piece equals the recomputed index
+  {
+    const char *dm[] = { "KQRvKR", "KQRBvKQRN", "KRvKN", "KQRBNvK" };
+    for (const char *nm : dm) {
+      Mat M;
+      if (setup_mat(M, nm) != TBG_OK || !M.all_single) ST_FAIL("setup %s", nm);
+      for (int it = 0; it < 20000; it++) {
+        int s = (int)(rnd() % NSL);
+        uint64_t idx = rnd() % M.ii.size;
+        uint64_t p = (uint64_t)T.kksq[s][0] | ((uint64_t)T.kksq[s][1] << 8);
+        uint64_t occ = idx_to_sq(T, M.ii, idx, &p);
+        int kj = (int)(rnd() % (uint64_t)M.ii.numsets);
+        int a = sq_get(p, 2 + kj);
+        uint64_t ob = occ_before(p, kj);
+        uint32_t ra = (uint32_t)(a - popc64(ob & (bit(a) - 1)));
+        for (int b = 0; b < 64; b++) {
+          if (occ & bit(b)) continue;
+          uint64_t want = sq_to_idx(T, M.ii, sq_set(p, 2 + kj, b));
+          if (moved_idx(M.ii, idx, p, kj, a, b, ob, ra) != want) ST_FAIL("%s: moved_idx", nm);
+        }
+      }
+    }
+  }
   if (msg) snprintf(msg, len, "ok");
   return 0;
 #undef ST_FAIL
@@ -1277,6 +1535,7 @@ uint64_t estimate_bytes(const char *material, int *err)
 struct tbg_server {
   uint64_t budget = 0;
   int max_active = 1;
+  Geom geom = g_geom;
   std::vector<SrvJob> jobs;
   hipStream_t st = nullptr, side = nullptr;
 };
@@ -1445,7 +1704,7 @@ int tbg_server_run(tbg_server *s, int form, tbg_server_stats *stats)
       hflags[0] = 0;
       hflags[1] = 1;  // a new job may be at ply 1
       HIPCHK(hipMemcpyAsync(dflags, hflags, sizeof hflags, hipMemcpyHostToDevice, s->st));
-      r = build_launchset(L, hj, mats, pairs, 0, s->st);
+      r = build_launchset(L, hj, mats, pairs, 0, s->geom, s->st);
       if (r != TBG_OK) break;
       changed = false;
     }

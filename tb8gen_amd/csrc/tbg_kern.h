@@ -41,23 +41,34 @@ struct Pair {
 };
 
 // Work units per slice of each launch type, one unit per work-group at a
-// time: pred = one index block; uncap / verify = 256 bitmap words (64 per
-// wave); sweep = 4096 words.
+// time: pred = one index block; uncap = 256 bitmap words (64 per wave);
+// verify and sweep = a number of words chosen by the launch geometry.
 enum { KT_PRED = 0, KT_UNCAP = 1, KT_VERIFY = 2, KT_SWEEP = 3 };
-constexpr uint64_t UNIT_WORDS = 256, SWEEP_UNIT_WORDS = 4096;
+constexpr uint64_t UNIT_WORDS = 256;
 
-TBG_HD uint64_t units_per_slice(const Mat &M, uint64_t nblk, int kt, int side)
+// Launch geometry of the ply kernels (tbg_geometry of the C-ABI).
+struct Geom {
+  uint32_t verify_uw;  // words per verify unit, multiple of 256
+  uint32_t sweep_uw;   // words per sweep unit, multiple of 512
+  uint32_t wg[4];      // grid cap per kernel type, work-groups per CU
+};
+
+// uncapture units: the capture spaces of the opponent's sets, back to back
+TBG_HD uint64_t uncap_units(const Mat &M, int side)
+{
+  uint64_t u = 0;
+  for (int k = 0; k < M.numsets; k++)
+    if (M.setcol[k] != side) u += (M.cwords[k] + UNIT_WORDS - 1) / UNIT_WORDS;
+  return u;
+}
+
+TBG_HD uint64_t units_per_slice(const Mat &M, uint64_t nblk, int kt, int side, const Geom &g)
 {
   switch (kt) {
   case KT_PRED: return nblk;
-  case KT_UNCAP: {
-    uint64_t u = 0;
-    for (int k = 0; k < M.numsets; k++)
-      if (M.setcol[k] != side) u += (M.cwords[k] + UNIT_WORDS - 1) / UNIT_WORDS;
-    return u;
-  }
-  case KT_VERIFY: return (M.words + UNIT_WORDS - 1) / UNIT_WORDS;
-  default: return (M.words + SWEEP_UNIT_WORDS - 1) / SWEEP_UNIT_WORDS;
+  case KT_UNCAP: return uncap_units(M, side);
+  case KT_VERIFY: return (M.words + g.verify_uw - 1) / g.verify_uw;
+  default: return (M.words + g.sweep_uw - 1) / g.sweep_uw;
   }
 }
 
@@ -68,8 +79,10 @@ __global__ void k_pred_j(JobDev *const *jobs, const Pair *pairs, const uint64_t 
                          int kind);
 __global__ void k_uncap_j(JobDev *const *jobs, const Pair *pairs, const uint64_t *P, int np, int sw0,
                           int kind, const unsigned *flags);
-__global__ void k_verify_j(JobDev *const *jobs, const Pair *pairs, const uint64_t *P, int np, int sw0);
-__global__ void k_sweep_j(JobDev *const *jobs, const Pair *pairs, const uint64_t *P, int np, int sw0);
+__global__ void k_verify_j(JobDev *const *jobs, const Pair *pairs, const uint64_t *P, int np, int sw0,
+                           int uw);
+__global__ void k_sweep_j(JobDev *const *jobs, const Pair *pairs, const uint64_t *P, int np, int sw0,
+                          int uw);
 // flags[0]: jobs done; flags[1]: some job is at ply 1 or 101 (uncapture)
 // done_out (nullable): done_out[j] = job j finished
 __global__ void k_ctrl(JobDev *const *jobs, int nj, unsigned *flags, unsigned *done_out);

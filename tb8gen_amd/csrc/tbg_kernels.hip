@@ -166,6 +166,17 @@ __device__ __forceinline__ int unit_slice(const Mat &M, int sw0, uint64_t local,
   return M.synth ? M.synth_slice : sw0 + (int)(local / ups);
 }
 
+// Word wi of a pred block's frontier: bits below lo and from hi on (the
+// neighbouring blocks' positions sharing its first and last words) masked.
+__device__ __forceinline__ uint64_t block_word(const uint64_t *srcs, uint64_t wlo, uint64_t wi,
+                                               uint64_t nbw, uint64_t lo, uint64_t hi)
+{
+  uint64_t x = srcs[wlo + wi];
+  if (wi == 0) x &= ~0ull << (lo & 63);
+  if (wi + 1 == nbw && (hi & 63)) x &= (1ull << (hi & 63)) - 1;
+  return x;
+}
+
 // predecessors_worker (src/tb8gen.c:484-516) over every (job, side) of a
 // launch: each set bit of the stm^1 frontier (W_{n-1} for an L step, kind
 // 0; L_{n-1} for a W step, kind 1) -> stm unmoves OR-ed into cand[stm].
@@ -179,7 +190,7 @@ __device__ __forceinline__ int unit_slice(const Mat &M, int sw0, uint64_t local,
 // on gfx950 travel to the memory side, so privatising cuts the scatter's HBM
 // write traffic by the share of same-block unmoves.
 constexpr int PB_WAVES = 8;   // waves per work-group
-constexpr int PB_CAP = 1024;  // queue entries per wave per round
+constexpr int PB_CAP = QCAP;  // queue entries per wave per round
 
 __global__ __launch_bounds__(512) void k_pred_j(JobDev *const *__restrict__ jobs,
                                                  const Pair *__restrict__ pairs,
@@ -188,7 +199,6 @@ __global__ __launch_bounds__(512) void k_pred_j(JobDev *const *__restrict__ jobs
 {
   __shared__ Tabs T;
   __shared__ Mat Ms;
-  __shared__ int rmax[PB_WAVES];
   extern __shared__ uint64_t dyn[];
   load_tabs(T);
   int cached = -1;
@@ -208,61 +218,62 @@ __global__ __launch_bounds__(512) void k_pred_j(JobDev *const *__restrict__ jobs
     uint64_t bi;
     int s = unit_slice(M, sw0, u - P[pi], J.nblk, &bi);
     uint64_t blk_words = J.blk_words;
-    uint64_t *B = dyn;
-    uint16_t *q = reinterpret_cast<uint16_t *>(dyn + blk_words) + wave * PB_CAP;
     bool lds = J.scatter == TBG_SCATTER_BLOCK;
+    // LDS: [block bitmap (block scatter only)][8 wave queues]
+    uint64_t *B = dyn;
+    uint16_t *q = reinterpret_cast<uint16_t *>(dyn + (lds ? blk_words : 0)) + wave * PB_CAP;
     const uint64_t *src = kind == 0 ? J.prevW[o] : J.prevL[o];
     uint64_t *cand = J.cand[stm];
     const uint64_t *won = J.wins[stm];
     uint64_t lo = bi * J.blk_pos;
     uint64_t hi = lo + J.blk_pos < M.ii.size ? lo + J.blk_pos : M.ii.size;
     uint64_t wlo = lo >> 6, nbw = ((hi + 63) >> 6) - wlo;
-    if (lds)
-      for (uint64_t w = threadIdx.x; w < nbw; w += blockDim.x) B[w] = 0;
-    __syncthreads();
     const uint64_t *srcs = src + phys(M, s) * M.words;
     uint64_t *own = cand + phys(M, s) * M.words;
     const uint64_t *ownw = won + phys(M, s) * M.words;
-    for (uint64_t c0 = 0; c0 < nbw; c0 += (uint64_t)PB_WAVES * 64) {
-      uint64_t wi = c0 + (uint64_t)wave * 64 + lane;
-      uint64_t x = wi < nbw ? srcs[wlo + wi] : 0;
-      if (wi == 0) x &= ~0ull << (lo & 63);                          // not below the block
-      if (wi + 1 == nbw && (hi & 63)) x &= (1ull << (hi & 63)) - 1;  // nor above it
+    // a block with no frontier bit is skipped whole (sparse plies); the
+    // words read here are found in cache by the per-wave pass
+    int nz = 0;
+    for (uint64_t wi = threadIdx.x; wi < nbw; wi += blockDim.x)
+      nz |= block_word(srcs, wlo, wi, nbw, lo, hi) != 0;
+    if (!__syncthreads_or(nz)) continue;
+    if (lds)
+      for (uint64_t w = threadIdx.x; w < nbw; w += blockDim.x) B[w] = 0;
+    __syncthreads();
+    // each wave walks its own 64-word chunks with a private queue: no
+    // work-group barrier until the merge
+    for (uint64_t c0 = (uint64_t)wave * 64; c0 < nbw; c0 += (uint64_t)PB_WAVES * 64) {
+      uint64_t wi = c0 + lane;
+      uint64_t x = wi < nbw ? block_word(srcs, wlo, wi, nbw, lo, hi) : 0;
+      if (!__ballot(x != 0)) continue;
       int tot;
       int off = wave_excl_scan(popc64(x), lane, &tot);
-      // every wave runs the same number of rounds (they share barriers)
-      if (lane == 0) rmax[wave] = (tot + PB_CAP - 1) / PB_CAP;
-      __syncthreads();
-      int allr = 0;
-      for (int v = 0; v < PB_WAVES; v++) allr = rmax[v] > allr ? rmax[v] : allr;
-      __syncthreads();
-      for (int r = 0; r < allr; r++) {
-        int base = r * PB_CAP;
-        uint64_t y = x;
-        int k = off;
-        while (y && k < base + PB_CAP) {
-          int bt = ctz64(y);
-          y &= y - 1;
-          if (k >= base) q[k - base] = (uint16_t)((lane << 6) | bt);
-          k++;
-        }
-        __syncthreads();
-        int cnt = tot - base;
-        cnt = cnt < 0 ? 0 : cnt > PB_CAP ? PB_CAP : cnt;
+      for (int base = 0; base < tot; base += QCAP) {
+        fill_round(q, x, off, base, lane);
+        wave_sync();
+        int cnt = tot - base < QCAP ? tot - base : QCAP;
         for (int e = lane; e < cnt; e += 64) {
           uint16_t v = q[e];
-          uint64_t idx = (wlo + c0 + (uint64_t)wave * 64 + (v >> 6)) * 64 + (v & 63);
+          uint64_t idx = (wlo + c0 + (v >> 6)) * 64 + (v & 63);
           uint64_t p = kings_of(T, s);
           uint64_t occ = idx_to_sq(T, M.ii, idx, &p);
           mark_king_unmoves(T, M, stm, occ, p, s, cand, won);
           for (int t = 0; t < MAXP - 1; t++) {
             if (t >= M.nk[stm]) break;
             int j = M.pcs[stm][t];
-            uint64_t bb = piece_att(T, M.pt[j], sq_get(p, j), occ) & ~occ;
+            int a = sq_get(p, j);
+            uint64_t bb = piece_att(T, M.pt[j], a, occ) & ~occ;
+            uint64_t ob = 0;
+            uint32_t ra = 0;
+            if (M.all_single) {
+              ob = occ_before(p, j - 2);
+              ra = (uint32_t)(a - popc64(ob & (bit(a) - 1)));
+            }
             while (bb) {
               int tq = ctz64(bb);
               bb &= bb - 1;
-              uint64_t i2 = sq_to_idx(T, M.ii, sq_set(p, j, tq));
+              uint64_t i2 = M.all_single ? moved_idx(M.ii, idx, p, j - 2, a, tq, ob, ra)
+                                         : sq_to_idx(T, M.ii, sq_set(p, j, tq));
               if (lds && i2 >= lo && i2 < hi)
                 __hip_atomic_fetch_or(reinterpret_cast<unsigned long long *>(&B[(i2 >> 6) - wlo]),
                                       (unsigned long long)bit(i2 & 63), __ATOMIC_RELAXED,
@@ -272,9 +283,10 @@ __global__ __launch_bounds__(512) void k_pred_j(JobDev *const *__restrict__ jobs
             }
           }
         }
-        __syncthreads();
+        wave_sync();
       }
     }
+    __syncthreads();
     if (lds)
       for (uint64_t w = threadIdx.x; w < nbw; w += blockDim.x) {
         uint64_t v = B[w];
@@ -311,7 +323,7 @@ __global__ __launch_bounds__(256) void k_uncap_j(JobDev *const *__restrict__ job
     cache_mat(Ms, J.M, pr.job, &cached);
     const Mat &M = Ms;
     uint64_t within;
-    int s = unit_slice(M, sw0, u - P[pi], units_per_slice(M, J.nblk, KT_UNCAP, stm), &within);
+    int s = unit_slice(M, sw0, u - P[pi], uncap_units(M, stm), &within);
     int k;
     for (k = 0; k < M.numsets; k++) {
       if (M.setcol[k] == stm) continue;
@@ -429,6 +441,16 @@ __device__ bool verify_pos(const Tabs &T, const Mat &M, int stm, int s, uint64_t
       }
     }
     uint64_t qm = a & ~occ;
+    if (M.all_single) {
+      int fr = sq_get(p, j);
+      uint64_t ob = occ_before(p, j - 2);
+      uint32_t ra = (uint32_t)(fr - popc64(ob & (bit(fr) - 1)));
+      while (qm) {
+        int to = ctz64(qm);
+        qm &= qm - 1;
+        if (!bm_test(wins_s, moved_idx(M.ii, idx, p, j - 2, fr, to, ob, ra))) return false;
+      }
+    }
     while (qm) {
       int to = ctz64(qm);
       qm &= qm - 1;
@@ -479,9 +501,13 @@ __device__ bool verify_pos(const Tabs &T, const Mat &M, int stm, int s, uint64_t
 // a decode.  A step that does not run this ply writes an empty newL.  For a
 // synthetic job at n == 1 the king-move exit candidates (computed by k_init
 // into newW[stm]) join the candidates.
-__global__ __launch_bounds__(256) void k_verify_j(JobDev *const *__restrict__ jobs,
+// Verification is latency-bound (a dependent chain of random bitmap tests
+// per candidate): 4 waves per SIMD (at most 128 VGPRs, a few spilled) beat
+// 3 waves without spills.
+__global__ __launch_bounds__(256, 4) void k_verify_j(JobDev *const *__restrict__ jobs,
                                                    const Pair *__restrict__ pairs,
-                                                   const uint64_t *__restrict__ P, int np, int sw0)
+                                                   const uint64_t *__restrict__ P, int np, int sw0,
+                                                   int uw)
 {
   __shared__ Tabs T;
   __shared__ Queue Q;
@@ -499,71 +525,84 @@ __global__ __launch_bounds__(256) void k_verify_j(JobDev *const *__restrict__ jo
     const Mat &M = Ms;
     int stm = pr.side, o = stm ^ 1, n = J.n;
     bool run = (J.mask >> stm) & 1;
+    bool exits = run && M.synth && n == 1;
     uint64_t within;
-    int s = unit_slice(M, sw0, u - P[pi], (M.words + UNIT_WORDS - 1) / UNIT_WORDS, &within);
-    uint64_t wbase = within * UNIT_WORDS + (uint64_t)wave * 64;
-    uint64_t w = wbase + lane;
-    bool live = w < M.words;
-    uint64_t a = phys(M, s) * M.words + w;
-    uint64_t *cand = J.cand[stm];
-    uint64_t x0 = 0, x = 0;
-    if (live) {
-      x0 = cand[a];
-      uint64_t c = x0;
-      if (run && M.synth && n == 1) c |= J.newW[stm][a];
-      x = run && c ? c & ~J.wins[stm][a] : 0;
-    }
-    Q.res[wave][lane] = x;
-    int tot;
-    int off = wave_excl_scan(popc64(x), lane, &tot);
-    const uint64_t *wins_o = J.wins[o];
-    const uint64_t *sub_o = J.sub[o][n <= DRAW_RULE ? TBG_SUB_WIN : TBG_SUB_CWIN];
-    for (int base = 0; base < tot; base += QCAP) {
-      fill_round(Q.e[wave], x, off, base, lane);
-      wave_sync();
-      int cnt = tot - base < QCAP ? tot - base : QCAP;
-      for (int e = lane; e < cnt; e += 64) {
-        uint16_t v = Q.e[wave][e];
-        uint64_t idx = (wbase + (v >> 6)) * 64 + (v & 63);
-        bool ok = idx < M.ii.size && verify_pos(T, M, stm, s, idx, wins_o, sub_o);
-        if (!ok)
-          __hip_atomic_fetch_and(reinterpret_cast<unsigned long long *>(&Q.res[wave][v >> 6]),
-                                 ~(1ull << (v & 63)), __ATOMIC_RELAXED,
-                                 __HIP_MEMORY_SCOPE_WORKGROUP);
+    int s = unit_slice(M, sw0, u - P[pi], (M.words + uw - 1) / uw, &within);
+    uint64_t u0 = within * (uint64_t)uw;
+    uint64_t uend = u0 + uw < M.words ? u0 + uw : M.words;
+    uint64_t mysum = 0;
+    // one 64-word chunk per wave per pass (the 4 waves cover 256 words); a
+    // chunk without candidates costs one load and one store per lane.  The
+    // plane pointers are read from the job where used rather than held
+    // across the verification loop, which keeps the register count (and so
+    // the occupancy) of this latency-bound kernel where it is.
+    for (uint64_t wbase = u0 + (uint64_t)wave * 64; wbase < uend; wbase += 256) {
+      uint64_t w = wbase + lane;
+      bool live = w < uend;
+      uint64_t a = phys(M, s) * M.words + w;
+      uint64_t x0 = 0, x = 0;
+      if (live) {
+        x0 = J.cand[stm][a];
+        uint64_t c = x0;
+        if (exits) c |= J.newW[stm][a];
+        x = run && c ? c & ~J.wins[stm][a] : 0;
       }
-      wave_sync();
-    }
-    wave_sync();
-    uint64_t r = 0;
-    if (live) {
-      r = Q.res[wave][lane];
-      J.newL[stm][a] = r;
-      if (x0) cand[a] = 0;
-      uint16_t *codes = J.codes[stm];
-      if (codes && r) {
-        uint16_t *c = codes + phys(M, s) * M.ii.size + w * 64;
-        uint16_t code = (uint16_t)((n << 2) | 2);
-        uint64_t rr = r;
-        while (rr) {
-          int b = ctz64(rr);
-          rr &= rr - 1;
-          c[b] = code;
+      uint64_t r = 0;
+      if (__ballot(x != 0)) {
+        Q.res[wave][lane] = x;
+        int tot;
+        int off = wave_excl_scan(popc64(x), lane, &tot);
+        const uint64_t *wins_o = J.wins[o];
+        const uint64_t *sub_o = J.sub[o][n <= DRAW_RULE ? TBG_SUB_WIN : TBG_SUB_CWIN];
+        for (int base = 0; base < tot; base += QCAP) {
+          fill_round(Q.e[wave], x, off, base, lane);
+          wave_sync();
+          int cnt = tot - base < QCAP ? tot - base : QCAP;
+          for (int e = lane; e < cnt; e += 64) {
+            uint16_t v = Q.e[wave][e];
+            uint64_t idx = (wbase + (v >> 6)) * 64 + (v & 63);
+            bool ok = idx < M.ii.size && verify_pos(T, M, stm, s, idx, wins_o, sub_o);
+            if (!ok)
+              __hip_atomic_fetch_and(reinterpret_cast<unsigned long long *>(&Q.res[wave][v >> 6]),
+                                     ~(1ull << (v & 63)), __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_WORKGROUP);
+          }
+          wave_sync();
+        }
+        r = Q.res[wave][lane];
+        wave_sync();  // the next chunk overwrites Q.res
+      }
+      if (live) {
+        J.newL[stm][a] = r;
+        if (x0) J.cand[stm][a] = 0;
+        uint16_t *codes = J.codes[stm];
+        if (codes && r) {
+          uint16_t *c = codes + phys(M, s) * M.ii.size + w * 64;
+          uint16_t code = (uint16_t)((n << 2) | 2);
+          uint64_t rr = r;
+          while (rr) {
+            int b = ctz64(rr);
+            rr &= rr - 1;
+            c[b] = code;
+          }
         }
       }
+      mysum += popc64(r);
     }
-    uint64_t sum = wave_sum64(popc64(r));
+    uint64_t sum = wave_sum64(mysum);
     if (lane == 0 && sum) atomicAdd(&J.cnt[stm], (unsigned long long)sum);
   }
 }
 
 // calc_W_n tail (src/tb8gen.c:1072-1077) fused, over every (job, side):
-// W = C & ~wins; wins |= W; count; codes; C cleared.  4096 words per unit,
-// two per lane per pass with 16-byte accesses.  A step that does not run
+// W = C & ~wins; wins |= W; count; codes; C cleared.  uw words per unit
+// (a multiple of 512), two per lane per pass with 16-byte accesses.  A step that does not run
 // writes an empty newW.  A synthetic job at n == 1 adds the king-move exit
 // candidates computed by k_init into prevW[stm].
 __global__ __launch_bounds__(256) void k_sweep_j(JobDev *const *__restrict__ jobs,
                                                   const Pair *__restrict__ pairs,
-                                                  const uint64_t *__restrict__ P, int np, int sw0)
+                                                  const uint64_t *__restrict__ P, int np, int sw0,
+                                                  int uw)
 {
   __shared__ Mat Ms;
   int cached = -1;
@@ -580,15 +619,14 @@ __global__ __launch_bounds__(256) void k_sweep_j(JobDev *const *__restrict__ job
     bool run = (J.mask >> (2 + stm)) & 1;
     bool exits = run && M.synth && n == 1;
     uint64_t within;
-    int s = unit_slice(M, sw0, u - P[pi], (M.words + SWEEP_UNIT_WORDS - 1) / SWEEP_UNIT_WORDS,
-                       &within);
+    int s = unit_slice(M, sw0, u - P[pi], (M.words + uw - 1) / uw, &within);
     uint64_t *cand = J.cand[stm], *wins = J.wins[stm], *newW = J.newW[stm];
     const uint64_t *kex = J.prevW[stm];
     uint16_t *codes = J.codes[stm];
     uint64_t base = phys(M, s) * M.words;
-    uint64_t wend = (within + 1) * SWEEP_UNIT_WORDS < M.words ? (within + 1) * SWEEP_UNIT_WORDS : M.words;
+    uint64_t wend = (within + 1) * uw < M.words ? (within + 1) * uw : M.words;
     uint64_t mycnt = 0;
-    for (uint64_t w = within * SWEEP_UNIT_WORDS + threadIdx.x * 2; w < wend; w += 512) {
+    for (uint64_t w = within * uw + threadIdx.x * 2; w < wend; w += 512) {
       uint64_t a = base + w;
       ulonglong2 c = *reinterpret_cast<const ulonglong2 *>(cand + a);
       ulonglong2 cc = c;
@@ -803,12 +841,14 @@ __global__ __launch_bounds__(256) void k_init(const Mat *__restrict__ Mp, uint64
     if (idx < M.ii.size) {
       uint64_t p = kings_of(T, s);
       uint64_t occ = idx_to_sq(T, M.ii, idx, &p);
-      if (M.synth) {
-        synth_exits_pos(T, M, p, occ, WHITE, &xw0, &xl0);
-        synth_exits_pos(T, M, p, occ, BLACK, &xw1, &xl1);
-      }
       wi = attacked_by(T, M, p, occ, sq_get(p, BLACK), WHITE, -1);
       bi = attacked_by(T, M, p, occ, sq_get(p, WHITE), BLACK, -1);
+      // an illegal position is in wins(stm) from the start, which masks its
+      // exit candidates in both the W sweep and verification
+      if (M.synth) {
+        if (!wi) synth_exits_pos(T, M, p, occ, WHITE, &xw0, &xl0);
+        if (!bi) synth_exits_pos(T, M, p, occ, BLACK, &xw1, &xl1);
+      }
       if (wi != bi) {
         int c = bi ? WHITE : BLACK;
         bool mate = !has_legal(T, M, p, occ, c);

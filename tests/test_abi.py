@@ -39,6 +39,27 @@ def test_error_codes_have_messages():
         assert L.tbg_strerror(code)
 
 
+def test_geometry_is_validated_on_the_host():
+    """Launch geometries are checked before anything reaches a device; the
+    process default round-trips."""
+    L = tbg.lib()
+    g0 = tbg.default_geometry()
+    assert g0.verify_unit_words % 256 == 0 and g0.sweep_unit_words % 512 == 0
+    for field, bad in [("verify_unit_words", 100), ("verify_unit_words", 0),
+                       ("sweep_unit_words", 256), ("sweep_unit_words", 1 << 20),
+                       ("pred_wg_per_cu", 0), ("verify_wg_per_cu", 65), ("scatter", 7)]:
+        g = tbg.Geometry(**g0.as_dict())
+        setattr(g, field, bad)
+        assert L.tbg_set_geometry(None, ctypes.byref(g)) == -1, field
+    g = tbg.Geometry(**g0.as_dict())
+    g.verify_unit_words, g.scatter = 1024, tbg.SCATTER_GLOBAL
+    tbg.set_default_geometry(g)
+    assert tbg.default_geometry().as_dict() == g.as_dict()
+    tbg.set_default_geometry(g0)
+    assert tbg.default_geometry().as_dict() == g0.as_dict()
+    assert L.tbg_autotune(None, 0, None, None, None) == -1
+
+
 def test_calls_before_init_are_rejected():
     L = tbg.lib()
     if tbg._inited:

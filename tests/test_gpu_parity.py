@@ -151,6 +151,85 @@ def test_scatter_modes_agree(registry, mode):
     t.close()
 
 
+def _tune_candidates():
+    # the autotuner's candidates span these unit sizes, grid caps and both
+    # scatter modes; results must not depend on any of them
+    return [
+        dict(verify_unit_words=256, sweep_unit_words=512, pred_wg_per_cu=1, uncap_wg_per_cu=1,
+             verify_wg_per_cu=1, sweep_wg_per_cu=1, scatter=tbg.SCATTER_BLOCK),
+        dict(verify_unit_words=4096, sweep_unit_words=16384, pred_wg_per_cu=4, uncap_wg_per_cu=32,
+             verify_wg_per_cu=8, sweep_wg_per_cu=8, scatter=tbg.SCATTER_BLOCK),
+        dict(verify_unit_words=65536, sweep_unit_words=65536, pred_wg_per_cu=8, uncap_wg_per_cu=4,
+             verify_wg_per_cu=64, sweep_wg_per_cu=64, scatter=tbg.SCATTER_GLOBAL),
+        dict(verify_unit_words=768, sweep_unit_words=1536, pred_wg_per_cu=3, uncap_wg_per_cu=16,
+             verify_wg_per_cu=16, sweep_wg_per_cu=16, scatter=tbg.SCATTER_GLOBAL),
+    ]
+
+
+@pytest.mark.parametrize("gi", range(4))
+def test_geometry_does_not_change_results(registry, gi):
+    """Unit sizes (including ones that do not divide a slice), grid caps and
+    the scatter mode change speed only."""
+    material = "KQvKR"
+    registry.get(material)
+    geo = _tune_candidates()[gi]
+    print(f"case geometry {gi} {geo}", flush=True)
+    t = tbg.Table(material)
+    t.set_geometry(tbg.Geometry(**geo))
+    assert t.geometry().as_dict() == geo
+    t.resolve(list(registry.tables.values()))
+    for form in (tbg.FORM_BATCHED, tbg.FORM_GRAPH):
+        lines = t.generate(form)
+        assert lines == oracle_lines(material)
+        assert_same_table(material, t)
+    t.close()
+
+
+def test_geometry_synthetic_multi_block():
+    """A multi-block 6-piece synthetic slice under two extreme geometries."""
+    material, s, seed = "KQRvKRN", 170, 5
+    print(f"case geometry synth {material}", flush=True)
+    ref = None
+    for geo in _tune_candidates()[:3]:
+        t = tbg.Table(material, synth_slice=s, seed=seed)
+        t.set_geometry(tbg.Geometry(**geo))
+        t.resolve([])
+        lines = t.generate(tbg.FORM_BATCHED)
+        # synthetic jobs keep bitmaps only: the final wins(stm) of the slice
+        wins = [t.bitmap(tbg.PLANE_WINS, stm, s) for stm in range(2)]
+        if ref is None:
+            ref = (lines, wins)
+        else:
+            assert lines == ref[0]
+            for stm in range(2):
+                assert np.array_equal(wins[stm], ref[1][stm])
+        t.close()
+
+
+def test_autotune_picks_and_caches(registry, tmp_path):
+    """The autotuner times its candidates, applies one, appends it to the
+    cache file, and a second call is a cache hit; the table generated
+    afterwards is exact."""
+    material = "KQvKR"
+    registry.get(material)
+    cache = str(tmp_path / "geometry.txt")
+    print("case autotune KQvKR", flush=True)
+    t = tbg.Table(material)
+    t.resolve(list(registry.tables.values()))
+    g, ms = t.autotune(max_plies=0, cache_path=cache)
+    assert ms > 0
+    assert t.geometry().as_dict() == g.as_dict()
+    with open(cache) as f:
+        rows = [l for l in f if l.startswith("tbg-geometry KQvKR ")]
+    assert len(rows) == 1
+    g2, ms2 = t.autotune(max_plies=0, cache_path=cache)
+    assert ms2 == -1 and g2.as_dict() == g.as_dict()
+    lines = t.generate(tbg.FORM_BATCHED)
+    assert lines == oracle_lines(material)
+    assert_same_table(material, t)
+    t.close()
+
+
 def test_synthetic_block_scatter_multi_block():
     """A 6-piece single slice is larger than one LDS block, so the block
     scatter splits it by its leading digit."""
