@@ -314,7 +314,7 @@ typedef struct {
   /* synthetic single-slice mode */
   int synth;
   int synth_slice;
-  uint64_t seed;
+  uint64_t seed, seedmix;
 } Table;
 
 #define MAXTABLES 512
@@ -523,11 +523,17 @@ static inline uint64_t splitmix64(uint64_t x)
   return x ^ (x >> 31);
 }
 
-uint64_t tbo_synth_hash(uint64_t seed, int stm, int kind, int region, uint64_t idx)
+/* one splitmix64 round over the packed key XOR splitmix64(seed) */
+static inline uint64_t synth_hash_mixed(uint64_t seedmix, int stm, int kind, int region, uint64_t idx)
 {
   uint64_t key = (idx & 0xfffffffffULL) | ((uint64_t)(region & 0xfff) << 36) |
                  ((uint64_t)(kind & 3) << 48) | ((uint64_t)(stm & 1) << 50);
-  return splitmix64(seed ^ splitmix64(key));
+  return splitmix64(key ^ seedmix);
+}
+
+uint64_t tbo_synth_hash(uint64_t seed, int stm, int kind, int region, uint64_t idx)
+{
+  return synth_hash_mixed(splitmix64(seed), stm, kind, region, idx);
 }
 /* win for the side to move below T_WIN, loss in [T_WIN, T_LOSS) */
 #define SYN_T_WIN 553402322211286548ULL   /* 0.03 * 2^64 */
@@ -578,7 +584,7 @@ static void calc_sub(Table *t, int stm)
         }
         int v;
         if (t->synth)
-          v = synth_value(tbo_synth_hash(t->seed, stm, 0, k, idx));
+          v = synth_value(synth_hash_mixed(t->seedmix, stm, 0, k, idx));
         else
           v = probe_wdl(spt, sq, n, stm);
         if (v == -2) bm_set(p[0], idx);
@@ -793,12 +799,12 @@ static int synth_king_exit(const Table *t, int stm, const uint8_t *sq, bb_t occ,
     int l = t->pc_to_set[j];
     memcpy(csq, sq2, MAX_PIECES);
     if (!g_ref_kcap) csq[j] = csq[t->ii.last[l]];
-    return synth_value(tbo_synth_hash(t->seed, o, 2, s2 * 8 + l, sq_to_idx(&t->capt[l], csq)));
+    return synth_value(synth_hash_mixed(t->seedmix, o, 2, s2 * 8 + l, sq_to_idx(&t->capt[l], csq)));
   }
   if (attacked_by(t->pt, sq, t->num, occ ^ bit(from) ^ bit(to), to, o)) return 3;
   uint8_t csq[MAX_PIECES];
   memcpy(csq, sq2, MAX_PIECES);
-  return synth_value(tbo_synth_hash(t->seed, o, 1, s2, sq_to_idx(&t->ii, csq)));
+  return synth_value(synth_hash_mixed(t->seedmix, o, 1, s2, sq_to_idx(&t->ii, csq)));
 }
 
 /* check_king_moves (tb8gen.c:537-569) */
@@ -1155,6 +1161,7 @@ int tbo_synth_slice(const char *material, int s, uint64_t seed, const char *log_
   t->synth = 1;
   t->synth_slice = s;
   t->seed = seed;
+  t->seedmix = splitmix64(seed);
   snprintf(t->name, sizeof t->name, "#%d", g_ntab);
   for (int c = 0; c < 2; c++) {
     t->code[c] = calloc(t->size, sizeof(uint16_t));

@@ -47,7 +47,7 @@ struct Mat {
   uint64_t cwords[MAXS], coff[MAXS], ctotal[2];
   uint64_t nslices_phys;     // 462, or 1 for a synthetic single-slice job
   int synth, synth_slice;
-  uint64_t seed;
+  uint64_t seed, seedmix;    // synthetic exits: seed and splitmix64(seed)
   int all_single;            // every set holds one piece (set k in slot 2 + k)
 };
 
@@ -333,11 +333,13 @@ TBG_HD uint64_t splitmix64(uint64_t x)
   x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
   return x ^ (x >> 31);
 }
-TBG_HD uint64_t synth_hash(uint64_t seed, int stm, int kind, int region, uint64_t idx)
+// One splitmix64 round over the packed key XOR the premixed seed
+// (seedmix = splitmix64(seed), kept in the material).
+TBG_HD uint64_t synth_hash(uint64_t seedmix, int stm, int kind, int region, uint64_t idx)
 {
   uint64_t key = (idx & 0xfffffffffull) | ((uint64_t)(region & 0xfff) << 36) |
                  ((uint64_t)(kind & 3) << 48) | ((uint64_t)(stm & 1) << 50);
-  return splitmix64(seed ^ splitmix64(key));
+  return splitmix64(key ^ seedmix);
 }
 constexpr uint64_t SYN_T_WIN = 553402322211286548ull;    // 0.03 * 2^64
 constexpr uint64_t SYN_T_LOSS = 1106804644422573097ull;  // 0.06 * 2^64

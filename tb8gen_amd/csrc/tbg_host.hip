@@ -24,6 +24,8 @@ namespace {
 thread_local char g_errbuf[256];
 int g_device = -1;
 int g_cus = 256;
+// bitmap planes per job: wins, prevW, prevL, newW, newL, cand per side
+constexpr uint64_t kPlanes = 12;
 Tabs g_htabs;
 // geometry and scatter given to new contexts and servers (tbg_set_geometry(NULL, ..))
 Geom g_geom = { 256, 4096, { 3, 16, 16, 16 } };
@@ -405,8 +407,8 @@ int ctx_alloc(tbg_ctx *c, bool with_codes)
   c->pw = c->nsl * M.words;
   HIPCHK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
   size_t pb = c->pw * 8;
-  HIPCHK(hipMalloc(&c->arena, pb * 12));
-  c->bytes += pb * 12;
+  HIPCHK(hipMalloc(&c->arena, pb * kPlanes));
+  c->bytes += pb * kPlanes;
   uint64_t *p = c->arena;
   for (int s = 0; s < 2; s++) {
     c->wins[s] = p; p += c->pw;
@@ -416,7 +418,7 @@ int ctx_alloc(tbg_ctx *c, bool with_codes)
     c->newL[s] = p; p += c->pw;
     c->cand[s] = p; p += c->pw;
   }
-  HIPCHK(hipMemsetAsync(c->arena, 0, pb * 12, c->st));
+  HIPCHK(hipMemsetAsync(c->arena, 0, pb * kPlanes, c->st));
   uint64_t sw = 4 * c->nsl * (M.ctotal[0] + M.ctotal[1]);
   HIPCHK(hipMalloc(&c->subarena, sw * 8 + 64));
   c->bytes += sw * 8;
@@ -834,6 +836,7 @@ static int create_common(tbg_ctx **out, const char *material, int synth, int sli
   c->h.synth = synth;
   c->h.synth_slice = slice;
   c->h.seed = seed;
+  c->h.seedmix = splitmix64(seed);
   r = ctx_alloc(c, !synth || keep_codes);
   if (r != TBG_OK) {
     tbg_destroy(c);
@@ -1527,7 +1530,7 @@ uint64_t estimate_bytes(const char *material, int *err)
   *err = setup_mat(M, material);
   if (*err) return 0;
   uint64_t pw = M.words, sw = 4 * (M.ctotal[0] + M.ctotal[1]);
-  return pw * 8 * 12 + sw * 8 + 2 * M.ii.size * 2 + 65536;
+  return pw * 8 * kPlanes + sw * 8 + 2 * M.ii.size * 2 + 65536;
 }
 
 }  // namespace

@@ -391,10 +391,10 @@ __device__ int synth_exit(const Tabs &T, const Mat &M, int stm, uint64_t q0, uin
   if (jj >= 0) {
     if (attacked_by(T, M, q0, occ ^ bit(from), to, o, jj)) return 3;
     uint64_t c = sq_set(q, jj, sq_get(q, M.ii.last[l]));
-    return synth_value(synth_hash(M.seed, o, 2, s2 * 8 + l, sq_to_idx(T, M.capt[l], c)));
+    return synth_value(synth_hash(M.seedmix, o, 2, s2 * 8 + l, sq_to_idx(T, M.capt[l], c)));
   }
   if (attacked_by(T, M, q0, occ ^ bit(from) ^ bit(to), to, o, -1)) return 3;
-  return synth_value(synth_hash(M.seed, o, 1, s2, sq_to_idx(T, M.ii, q)));
+  return synth_value(synth_hash(M.seedmix, o, 1, s2, sq_to_idx(T, M.ii, q)));
 }
 
 // check_successors_worker + check_moves + check_king_moves
@@ -772,9 +772,9 @@ __device__ void synth_exits_pos(const Tabs &T, const Mat &M, uint64_t p, uint64_
     int v;
     if (jj >= 0) {
       uint64_t c = sq_set(q, jj, sq_get(q, M.ii.last[l]));
-      v = synth_value(synth_hash(M.seed, o, 2, s2 * 8 + l, sq_to_idx(T, M.capt[l], c)));
+      v = synth_value(synth_hash(M.seedmix, o, 2, s2 * 8 + l, sq_to_idx(T, M.capt[l], c)));
     } else {
-      v = synth_value(synth_hash(M.seed, o, 1, s2, sq_to_idx(T, M.ii, q)));
+      v = synth_value(synth_hash(M.seedmix, o, 1, s2, sq_to_idx(T, M.ii, q)));
     }
     ww |= v == -2;
     ll |= v == 2;
@@ -783,34 +783,85 @@ __device__ void synth_exits_pos(const Tabs &T, const Mat &M, uint64_t p, uint64_
   *l = ll;
 }
 
-// has_legal_moves || has_legal_caps (src/movegen.c:75-111) for colour c.
+// has_legal_moves || has_legal_caps (src/movegen.c:75-111) for colour c:
+// the same answer, found with less work.  King moves are tested against the
+// opponent's attacks computed once with c's king lifted (so a slider's ray
+// passes through the square the king leaves); a capturing king move drops
+// the captured piece's attacks.  Only if no king move is legal are the other
+// pieces' moves tried, each with a full attack test (pins): under a double
+// check none can be legal, under a single check only moves onto the checker
+// or between it and the king can be.
 __device__ bool has_legal(const Tabs &T, const Mat &M, uint64_t p, uint64_t occ, int c)
 {
   int o = c ^ 1;
-  uint64_t own = bit(sq_get(p, c));
+  int k = sq_get(p, c), ok = sq_get(p, o);
+  uint64_t own = bit(k);
   for (int t = 0; t < MAXP - 1; t++) {
     if (t >= M.nk[c]) break;
     own |= bit(sq_get(p, M.pcs[c][t]));
   }
-  uint64_t enemy80 = 0;
-  for (int t = 0; t < MAXP - 1; t++) {
-    if (t >= M.nk[o]) break;
-    enemy80 |= 0x80ull << (8 * M.pcs[o][t]);
+  uint64_t occk = occ ^ bit(k);
+  uint64_t att[MAXS];
+  uint64_t all = T.king[ok];
+#pragma unroll
+  for (int t = 0; t < MAXS; t++) {
+    att[t] = 0;
+    if (t < M.nk[o]) {
+      int j = M.pcs[o][t];
+      att[t] = piece_att(T, M.pt[j], sq_get(p, j), occk);
+    }
+    all |= att[t];
   }
-  for (int t = -1; t < MAXP - 1; t++) {
+  uint64_t kb = T.king[k] & ~own & ~T.king[ok];
+  if (kb & ~occ & ~all) return true;  // a quiet king move to a safe square
+  uint64_t caps = kb & occ;           // king captures (the enemy king is never adjacent)
+  while (caps) {
+    int to = ctz64(caps);
+    caps &= caps - 1;
+    int cap = slot_of(p, to, M.col80[o]);
+    uint64_t a = T.king[ok];
+#pragma unroll
+    for (int t = 0; t < MAXS; t++)
+      if (t < M.nk[o] && M.pcs[o][t] != cap) a |= att[t];
+    if (!(a & bit(to))) return true;
+  }
+  // checkers (the lifted king does not block rays onto its own square)
+  uint64_t block = ~0ull;
+  int nchk = 0;
+#pragma unroll
+  for (int t = 0; t < MAXS; t++) {
+    if (t < M.nk[o] && (att[t] & bit(k))) {
+      int j = M.pcs[o][t];
+      int s = sq_get(p, j);
+      uint64_t b = bit(s);
+      int pt = M.pt[j] & 7;
+      if (pt == ROOK || pt == QUEEN) {
+        uint64_t r = rook_att(s, occ) & rook_att(k, occ);
+        if ((s >> 3) == (k >> 3) || (s & 7) == (k & 7)) b |= r;
+      }
+      if (pt == BISHOP || pt == QUEEN) {
+        uint64_t r = bishop_att(s, occ) & bishop_att(k, occ);
+        if ((s >> 3) - (k >> 3) == (s & 7) - (k & 7) || (s >> 3) + (s & 7) == (k >> 3) + (k & 7))
+          b |= r;
+      }
+      block = nchk ? 0 : b;
+      nchk++;
+    }
+  }
+  if (nchk > 1) return false;
+  block &= ~bit(ok);  // the enemy king is never captured
+  for (int t = 0; t < MAXP - 1; t++) {
     if (t >= M.nk[c]) break;
-    int j = t < 0 ? c : M.pcs[c][t];
+    int j = M.pcs[c][t];
     int from = sq_get(p, j);
-    uint64_t b = piece_att(T, M.pt[j], from, occ) & ~own;
+    uint64_t b = piece_att(T, M.pt[j], from, occ) & ~own & block;
     while (b) {
       int to = ctz64(b);
       b &= b - 1;
-      int cap = (occ & bit(to)) ? slot_of(p, to, enemy80) : -1;
-      if ((occ & bit(to)) && cap < 0) continue;  // the enemy king: never legal to reach
+      int cap = (occ & bit(to)) ? slot_of(p, to, M.col80[o]) : -1;
       uint64_t q = sq_set(p, j, to);
       uint64_t occ2 = (occ ^ bit(from)) | bit(to);
-      int ksq = j == c ? to : sq_get(p, c);
-      if (!attacked_by(T, M, q, occ2, ksq, o, cap)) return true;
+      if (!attacked_by(T, M, q, occ2, k, o, cap)) return true;
     }
   }
   return false;
@@ -918,7 +969,7 @@ __global__ __launch_bounds__(256) void k_resolve(const Mat *__restrict__ Mp,
       ill = attacked_by(T, M, p, occ, sq_get(p, c ^ 1), c, m);
       if (!ill) {
         if (M.synth) {
-          v = synth_value(synth_hash(M.seed, c, 0, k, idx));
+          v = synth_value(synth_hash(M.seedmix, c, 0, k, idx));
         } else if (R.present) {
           uint64_t tsq = 0;
           for (int j = 0; j < MAXP; j++) {
