@@ -53,6 +53,11 @@ def run_steps(table, subs, form, steps, tbg):
     return secs, timing, lines
 
 
+def scatter_mode(name, tbg):
+    mode = tbg.SCATTER_BLOCK if name.startswith("block") else tbg.SCATTER_GLOBAL
+    return mode | (tbg.SCATTER_COMBINE if name.endswith("+combine") else 0)
+
+
 def pawnless_materials(extra):
     """Every pawnless material with `extra` non-king pieces, in the colour
     orientation the engine generates (stronger side white)."""
@@ -214,7 +219,7 @@ def run_synth(args, tbg):
     material = args.material if args.material != "KQRvKR" else "KQRBvKQRN"
     t0 = time.perf_counter()
     table = tbg.Table(material, synth_slice=args.slice, seed=args.seed)
-    table.set_scatter(tbg.SCATTER_BLOCK if args.scatter == "block" else tbg.SCATTER_GLOBAL)
+    table.set_scatter(scatter_mode(args.scatter, tbg))
     alloc_s = time.perf_counter() - t0
     n = table.size
     print(f"synth {material} slice {args.slice}: {n} positions per side, "
@@ -283,7 +288,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-form comparison")
     ap.add_argument("--cpu-plies", type=int, default=2)
-    ap.add_argument("--scatter", default="block", choices=["block", "global"],
+    ap.add_argument("--scatter", default="block",
+                    choices=["block", "global", "block+combine", "global+combine"],
                     help="scatter mode when not autotuned")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="keep the built-in launch geometry (table and stream workloads)")
@@ -334,7 +340,7 @@ def main():
         reg.get(sub)
     subs = list(reg.tables.values())
     table = tbg.Table(args.material)
-    table.set_scatter(tbg.SCATTER_BLOCK if args.scatter == "block" else tbg.SCATTER_GLOBAL)
+    table.set_scatter(scatter_mode(args.scatter, tbg))
     n_total = tbg.NSLICES * table.size
 
     tuned = None

@@ -188,10 +188,16 @@ int tbg_get_sub_bitmap(tbg_ctx *ctx, int kind, int colour, int slice, uint64_t *
 /* Predecessor scatter strategy (the reference scatters with byte-wide
  * relaxed atomic ORs, src/kslice.h:40-45):
  *   TBG_SCATTER_GLOBAL: one 64-bit device atomic OR per unmove;
- *   TBG_SCATTER_BLOCK (default): a work-group owns a block of the index and
- *     ORs same-block unmoves into an LDS bitmap, merged once per word. */
+ *   TBG_SCATTER_BLOCK: a work-group owns a block of the index and
+ *     ORs same-block unmoves into an LDS bitmap, merged once per word;
+ *   | TBG_SCATTER_COMBINE: the unmoves that still need device atomics (king
+ *     unmoves, pieces whose moves leave the block) are combined per word
+ *     across a wave first (a segmented OR over lanes whose targets share a
+ *     word: sort-then-OR without the sort), one atomic per run.
+ * Default: TBG_SCATTER_BLOCK (the autotuner also tries the others). */
 #define TBG_SCATTER_GLOBAL 0
 #define TBG_SCATTER_BLOCK 1
+#define TBG_SCATTER_COMBINE 2
 int tbg_set_scatter(tbg_ctx *ctx, int mode);
 
 /* Launch geometry of the ply kernels.  Results never depend on it; only

@@ -34,7 +34,7 @@ ABI_SYMBOLS = [
     "tbg_server_run", "tbg_server_result", "tbg_server_codes", "tbg_server_lines",
     "tbg_set_geometry", "tbg_get_geometry", "tbg_autotune",
 ]
-SCATTER_GLOBAL, SCATTER_BLOCK = 0, 1
+SCATTER_GLOBAL, SCATTER_BLOCK, SCATTER_COMBINE = 0, 1, 2
 
 
 class TbgError(RuntimeError):

@@ -293,17 +293,22 @@ namespace {
 // digits 0..d-1 (factor[d-1] positions) for the smallest d that fits.
 constexpr uint64_t kBlkMaxPos = 40 * 1024 * 8 - 128;
 
-void choose_block(const Mat &M, uint64_t *blk_pos, uint64_t *nblk)
+// A move of a piece of set k >= d changes digits k.. only (the digits
+// before depend on the kings and the earlier sets), so it stays in its
+// block: *blk_set = d.
+void choose_block(const Mat &M, uint64_t *blk_pos, uint64_t *nblk, int *blk_set)
 {
-  if (M.ii.size <= kBlkMaxPos) { *blk_pos = M.ii.size; *nblk = 1; return; }
+  if (M.ii.size <= kBlkMaxPos) { *blk_pos = M.ii.size; *nblk = 1; *blk_set = 0; return; }
   for (int d = 1; d <= M.ii.numsets; d++)
     if (M.ii.factor[d - 1] <= kBlkMaxPos) {
       *blk_pos = M.ii.factor[d - 1];
       *nblk = M.ii.size / M.ii.factor[d - 1];
+      *blk_set = d;
       return;
     }
   *blk_pos = 1;
   *nblk = M.ii.size;
+  *blk_set = M.ii.numsets;
 }
 
 // window_slices 0: each job's own slices (462, or 1 for a synthetic job)
@@ -328,7 +333,7 @@ int build_launchset(LaunchSet &L, const std::vector<const JobDev *> &hjobs,
   bool block = false;
   for (const JobDev *j : hjobs) {
     maxbw = j->blk_words > maxbw ? j->blk_words : maxbw;
-    block |= j->scatter == TBG_SCATTER_BLOCK;
+    block |= (j->scatter & TBG_SCATTER_BLOCK) != 0;
   }
   // the queues follow the block bitmap, which only the block scatter uses
   L.pred_lds = (block ? maxbw * 8 : 0) + PB_QUEUE_BYTES;
@@ -365,7 +370,7 @@ void reset_job(tbg_ctx *c)
   J.more = 0xf;
   J.scatter = c->scatter;
   J.max_ply = c->max_ply;
-  choose_block(c->h, &J.blk_pos, &J.nblk);
+  choose_block(c->h, &J.blk_pos, &J.nblk, &J.blk_set);
   J.blk_words = (J.blk_pos + 63) / 64 + 1;
 }
 
@@ -1113,7 +1118,7 @@ int tbg_get_sub_bitmap(tbg_ctx *c, int kind, int colour, int slice, uint64_t *ds
 int tbg_set_scatter(tbg_ctx *c, int mode)
 {
   if (!c) return fail(TBG_E_INVAL, "null context");
-  if (mode != TBG_SCATTER_GLOBAL && mode != TBG_SCATTER_BLOCK) return fail(TBG_E_INVAL, "bad scatter mode");
+  if (mode & ~(TBG_SCATTER_BLOCK | TBG_SCATTER_COMBINE)) return fail(TBG_E_INVAL, "bad scatter mode");
   HIPCHK(hipStreamSynchronize(c->st));
   c->scatter = mode;
   TRY(push_scatter(c));
@@ -1131,7 +1136,7 @@ int check_geometry(const tbg_geometry *g)
   const int wg[4] = { g->pred_wg_per_cu, g->uncap_wg_per_cu, g->verify_wg_per_cu, g->sweep_wg_per_cu };
   for (int x : wg)
     if (x < 1 || x > 64) return fail(TBG_E_INVAL, "work-groups per CU must be in [1, 64]");
-  if (g->scatter != TBG_SCATTER_GLOBAL && g->scatter != TBG_SCATTER_BLOCK)
+  if (g->scatter & ~(TBG_SCATTER_BLOCK | TBG_SCATTER_COMBINE))
     return fail(TBG_E_INVAL, "bad scatter mode");
   return TBG_OK;
 }
@@ -1191,10 +1196,10 @@ namespace {
 const tbg_geometry kTune[] = {
   // verify uw, sweep uw, pred wg, uncap wg, verify wg, sweep wg, scatter
   { 256, 4096, 3, 16, 16, 16, TBG_SCATTER_BLOCK },  // the built-in default
-  { 1024, 2048, 2, 8, 16, 16, TBG_SCATTER_BLOCK },
-  { 4096, 16384, 4, 32, 8, 8, TBG_SCATTER_BLOCK },
+  { 1024, 2048, 2, 8, 16, 16, TBG_SCATTER_BLOCK | TBG_SCATTER_COMBINE },
+  { 4096, 16384, 4, 32, 8, 8, TBG_SCATTER_BLOCK | TBG_SCATTER_COMBINE },
   { 512, 8192, 8, 16, 32, 16, TBG_SCATTER_GLOBAL },
-  { 2048, 1024, 3, 4, 16, 32, TBG_SCATTER_GLOBAL },
+  { 2048, 1024, 3, 4, 16, 32, TBG_SCATTER_GLOBAL | TBG_SCATTER_COMBINE },
   { 8192, 32768, 6, 16, 8, 8, TBG_SCATTER_BLOCK },
 };
 constexpr int kNTune = (int)(sizeof kTune / sizeof kTune[0]);
@@ -1498,6 +1503,31 @@ extern "C" int tbg_host_selftest(char *msg, int len)
           if (occ & bit(b)) continue;
           uint64_t want = sq_to_idx(T, M.ii, sq_set(p, 2 + kj, b));
           if (moved_idx(M.ii, idx, p, kj, a, b, ob, ra) != want) ST_FAIL("%s: moved_idx", nm);
+        }
+      }
+    }
+  }
+  // pred blocks: a move of a piece of a set >= blk_set stays in its block
+  {
+    const char *bm[] = { "KQRvKRN", "KQRBvKQRN", "KRRvKQ", "KQQvKRR", "KBBNvKN" };
+    for (const char *nm : bm) {
+      Mat M;
+      if (setup_mat(M, nm) != TBG_OK) ST_FAIL("setup %s", nm);
+      uint64_t bp, nb;
+      int bs;
+      choose_block(M, &bp, &nb, &bs);
+      if (bp * nb != M.ii.size) ST_FAIL("%s: blocks do not tile the slice", nm);
+      for (int it = 0; it < 20000; it++) {
+        int sl = (int)(rnd() % NSL);
+        uint64_t idx = rnd() % M.ii.size;
+        uint64_t p = (uint64_t)T.kksq[sl][0] | ((uint64_t)T.kksq[sl][1] << 8);
+        uint64_t occ = idx_to_sq(T, M.ii, idx, &p);
+        int j = 2 + (int)(rnd() % (uint64_t)(M.num - 2));
+        if (sq_get(M.set_of, j) < bs) continue;
+        for (int b = 0; b < 64; b++) {
+          if (occ & bit(b)) continue;
+          uint64_t i2 = sq_to_idx(T, M.ii, sq_set(p, j, b));
+          if (i2 / bp != idx / bp) ST_FAIL("%s: a move of set %d leaves its block", nm, sq_get(M.set_of, j));
         }
       }
     }

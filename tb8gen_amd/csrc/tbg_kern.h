@@ -33,6 +33,7 @@ struct JobDev {
   int scatter;                // TBG_SCATTER_*
   int max_ply;
   uint64_t blk_pos, nblk, blk_words;
+  int blk_set;                // pieces of sets >= blk_set move within a pred block
 };
 
 // One (job, side to move) pair of a launch.

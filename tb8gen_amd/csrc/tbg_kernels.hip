@@ -26,15 +26,28 @@ __device__ __forceinline__ void load_tabs(Tabs &S)
   __syncthreads();
 }
 
+// Bitmaps and codes live in device global memory.  Their pointers come
+// from job records, so the compiler sees generic (flat) pointers; the hot
+// accesses cast them to the global address space (global loads and
+// atomics, whose waits do not also cover LDS traffic).
+typedef __attribute__((address_space(1))) uint64_t gu64;
+typedef __attribute__((address_space(1))) const uint64_t cgu64;
+typedef __attribute__((address_space(1))) unsigned long long gull;
+
+__device__ __forceinline__ uint64_t gload(const uint64_t *p, uint64_t i)
+{
+  return ((cgu64 *)p)[i];
+}
+
 __device__ __forceinline__ void atomic_or64(uint64_t *a, uint64_t v)
 {
-  __hip_atomic_fetch_or(reinterpret_cast<unsigned long long *>(a), (unsigned long long)v,
-                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_fetch_or((gull *)a, (unsigned long long)v, __ATOMIC_RELAXED,
+                        __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __device__ __forceinline__ bool bm_test(const uint64_t *p, uint64_t i)
 {
-  return (p[i >> 6] >> (i & 63)) & 1;
+  return (gload(p, i >> 6) >> (i & 63)) & 1;
 }
 
 __device__ __forceinline__ int wave_excl_scan(int v, int lane, int *total)
@@ -64,28 +77,115 @@ __device__ __forceinline__ uint64_t kings_of(const Tabs &T, int s)
 __device__ __forceinline__ uint64_t phys(const Mat &M, int s) { return M.synth ? 0 : (uint64_t)s; }
 
 // ------------------------------------------------------------- unmoves
+// Up to 4 scatter targets (global bit indices into a plane; ~0: none):
+// their words of `won` are loaded together, then each target not already
+// won gets its 64-bit atomic OR.  One memory round trip per group instead
+// of one per target.
+__device__ __forceinline__ void mark4(uint64_t *plane, const uint64_t *__restrict__ won,
+                                      const uint64_t (&g)[4])
+{
+  uint64_t w[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) w[k] = g[k] != ~0ull ? gload(won, g[k] >> 6) : ~0ull;
+#pragma unroll
+  for (int k = 0; k < 4; k++)
+    if (!((w[k] >> (g[k] & 63)) & 1)) atomic_or64(plane + (g[k] >> 6), bit(g[k] & 63));
+}
+
 // mark_king_unmoves (src/tb8gen.c:246-262)
 // Targets already in wins(stm) (illegal or won faster) can never become new
 // W_n or L_n positions (W_n = C & ~wins; a won position fails verification),
 // so they are tested with a cached read before the memory-side atomic.
+// The targets of a position go out in groups of 4 (mark4).
 __device__ void mark_king_unmoves(const Tabs &T, const Mat &M, int stm, uint64_t occ, uint64_t p,
                                   int s, uint64_t *cand, const uint64_t *__restrict__ won)
 {
   uint64_t b = T.king[sq_get(p, stm)] & ~(occ | T.king[sq_get(p, stm ^ 1)]);
-  while (b) {
-    int t = ctz64(b);
-    b &= b - 1;
-    uint64_t q;
-    int s2 = normalize(T, sq_set(p, stm, t), &q);
-    if (M.synth && s2 != M.synth_slice) continue;  // leaves a single-slice job
-    uint64_t *base = cand + phys(M, s2) * M.words;
-    const uint64_t *wb = won + phys(M, s2) * M.words;
-    uint64_t i2 = sq_to_idx(T, M.ii, q);
-    if (!bm_test(wb, i2)) atomic_or64(base + (i2 >> 6), bit(i2 & 63));
-    if (s < 441 && s2 >= 441) {
-      i2 = sq_to_idx(T, M.ii, mirror_nonkings(q));
-      if (!bm_test(wb, i2)) atomic_or64(base + (i2 >> 6), bit(i2 & 63));
+  bool dual = false;  // entering a diagonal slice from another: also the mirror
+  uint64_t dq = 0;
+  int ds = 0;
+  while (b || dual) {
+    uint64_t g[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      g[k] = ~0ull;
+      if (dual) {
+        g[k] = phys(M, ds) * M.words * 64 + sq_to_idx(T, M.ii, mirror_nonkings(dq));
+        dual = false;
+      } else if (b) {
+        int t = ctz64(b);
+        b &= b - 1;
+        uint64_t q;
+        int s2 = normalize(T, sq_set(p, stm, t), &q);
+        if (M.synth && s2 != M.synth_slice) continue;  // leaves a single-slice job
+        g[k] = phys(M, s2) * M.words * 64 + sq_to_idx(T, M.ii, q);
+        if (s < 441 && s2 >= 441) {
+          dual = true;
+          dq = q;
+          ds = s2;
+        }
+      }
     }
+    mark4(cand, won, g);
+  }
+}
+
+// Scatter with combining ("sort-then-OR" within a wave): every lane brings
+// one target (global bit index into `plane`, ~0: none); a segmented OR-scan
+// over runs of lanes whose targets share a 64-bit word (lanes hold
+// consecutive frontier positions under the same unmove, so their targets
+// cluster) gathers each run's bits into its last lane, which alone reads
+// the word of `won` and issues one atomic OR.  A word that recurs in
+// separate runs just gets one atomic per run.  Call with the whole wave.
+__device__ __forceinline__ void wave_mark(uint64_t *plane, const uint64_t *__restrict__ won,
+                                          uint64_t g, int lane)
+{
+  uint64_t key = g == ~0ull ? ~0ull : g >> 6;
+  uint64_t val = g == ~0ull ? 0 : bit(g & 63);
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    uint64_t ku = __shfl_up(key, d);
+    uint64_t vu = __shfl_up(val, d);
+    if (lane >= d && ku == key) val |= vu;
+  }
+  uint64_t kd = __shfl_down(key, 1);
+  if (key != ~0ull && (lane == 63 || kd != key)) {
+    uint64_t m = val & ~gload(won, key);
+    if (m) atomic_or64(plane + key, m);
+  }
+}
+
+// mark_king_unmoves for the whole wave with combining: one target per lane
+// per round (the kings are the same for every lane of a slice, so the
+// lanes' rounds line up).  ok: the lane holds a position.
+__device__ void king_unmoves_wave(const Tabs &T, const Mat &M, int stm, bool ok, uint64_t occ,
+                                  uint64_t p, int s, uint64_t *cand,
+                                  const uint64_t *__restrict__ won, int lane)
+{
+  uint64_t b = ok ? T.king[sq_get(p, stm)] & ~(occ | T.king[sq_get(p, stm ^ 1)]) : 0;
+  bool dual = false;
+  uint64_t dq = 0;
+  int ds = 0;
+  while (__ballot(b != 0 || dual)) {
+    uint64_t g = ~0ull;
+    if (dual) {
+      g = phys(M, ds) * M.words * 64 + sq_to_idx(T, M.ii, mirror_nonkings(dq));
+      dual = false;
+    } else if (b) {
+      int t = ctz64(b);
+      b &= b - 1;
+      uint64_t q;
+      int s2 = normalize(T, sq_set(p, stm, t), &q);
+      if (!M.synth || s2 == M.synth_slice) {  // else it leaves a single-slice job
+        g = phys(M, s2) * M.words * 64 + sq_to_idx(T, M.ii, q);
+        if (s < 441 && s2 >= 441) {
+          dual = true;
+          dq = q;
+          ds = s2;
+        }
+      }
+    }
+    wave_mark(cand, won, g, lane);
   }
 }
 
@@ -171,7 +271,7 @@ __device__ __forceinline__ int unit_slice(const Mat &M, int sw0, uint64_t local,
 __device__ __forceinline__ uint64_t block_word(const uint64_t *srcs, uint64_t wlo, uint64_t wi,
                                                uint64_t nbw, uint64_t lo, uint64_t hi)
 {
-  uint64_t x = srcs[wlo + wi];
+  uint64_t x = gload(srcs, wlo + wi);
   if (wi == 0) x &= ~0ull << (lo & 63);
   if (wi + 1 == nbw && (hi & 63)) x &= (1ull << (hi & 63)) - 1;
   return x;
@@ -218,7 +318,11 @@ __global__ __launch_bounds__(512) void k_pred_j(JobDev *const *__restrict__ jobs
     uint64_t bi;
     int s = unit_slice(M, sw0, u - P[pi], J.nblk, &bi);
     uint64_t blk_words = J.blk_words;
-    bool lds = J.scatter == TBG_SCATTER_BLOCK;
+    bool lds = (J.scatter & TBG_SCATTER_BLOCK) != 0;
+    bool comb = (J.scatter & TBG_SCATTER_COMBINE) != 0;
+    // read once per unit: a job field read in the loops below would be
+    // reloaded after every atomic, waiting for all of them
+    const int blk_set = J.blk_set;
     // LDS: [block bitmap (block scatter only)][8 wave queues]
     uint64_t *B = dyn;
     uint16_t *q = reinterpret_cast<uint16_t *>(dyn + (lds ? blk_words : 0)) + wave * PB_CAP;
@@ -252,34 +356,69 @@ __global__ __launch_bounds__(512) void k_pred_j(JobDev *const *__restrict__ jobs
         fill_round(q, x, off, base, lane);
         wave_sync();
         int cnt = tot - base < QCAP ? tot - base : QCAP;
-        for (int e = lane; e < cnt; e += 64) {
-          uint16_t v = q[e];
-          uint64_t idx = (wlo + c0 + (v >> 6)) * 64 + (v & 63);
-          uint64_t p = kings_of(T, s);
-          uint64_t occ = idx_to_sq(T, M.ii, idx, &p);
-          mark_king_unmoves(T, M, stm, occ, p, s, cand, won);
+        // all lanes step through the entries together (the combining
+        // scatter works on whole waves); lanes past the end hold nothing
+        for (int e0 = 0; e0 < cnt; e0 += 64) {
+          int e = e0 + lane;
+          bool ok = e < cnt;
+          uint64_t idx = 0, occ = 0, p = kings_of(T, s);
+          if (ok) {
+            uint16_t v = q[e];
+            idx = (wlo + c0 + (v >> 6)) * 64 + (v & 63);
+            occ = idx_to_sq(T, M.ii, idx, &p);
+          }
+          if (comb) king_unmoves_wave(T, M, stm, ok, occ, p, s, cand, won, lane);
+          else if (ok) mark_king_unmoves(T, M, stm, occ, p, s, cand, won);
           for (int t = 0; t < MAXP - 1; t++) {
             if (t >= M.nk[stm]) break;
             int j = M.pcs[stm][t];
             int a = sq_get(p, j);
-            uint64_t bb = piece_att(T, M.pt[j], a, occ) & ~occ;
+            uint64_t bb = ok ? piece_att(T, M.pt[j], a, occ) & ~occ : 0;
             uint64_t ob = 0;
             uint32_t ra = 0;
             if (M.all_single) {
               ob = occ_before(p, j - 2);
               ra = (uint32_t)(a - popc64(ob & (bit(a) - 1)));
             }
-            while (bb) {
-              int tq = ctz64(bb);
-              bb &= bb - 1;
-              uint64_t i2 = M.all_single ? moved_idx(M.ii, idx, p, j - 2, a, tq, ob, ra)
-                                         : sq_to_idx(T, M.ii, sq_set(p, j, tq));
-              if (lds && i2 >= lo && i2 < hi)
+            if (lds && sq_get(M.set_of, j) >= blk_set) {
+              // the piece's set lies inside the block: every target is in LDS
+              while (bb) {
+                int tq = ctz64(bb);
+                bb &= bb - 1;
+                uint64_t i2 = M.all_single ? moved_idx(M.ii, idx, p, j - 2, a, tq, ob, ra)
+                                           : sq_to_idx(T, M.ii, sq_set(p, j, tq));
                 __hip_atomic_fetch_or(reinterpret_cast<unsigned long long *>(&B[(i2 >> 6) - wlo]),
                                       (unsigned long long)bit(i2 & 63), __ATOMIC_RELAXED,
                                       __HIP_MEMORY_SCOPE_WORKGROUP);
-              else if (!bm_test(ownw, i2))
-                atomic_or64(own + (i2 >> 6), bit(i2 & 63));
+              }
+            } else if (comb) {
+              // device atomics combined per word across the wave, one
+              // target per lane per round
+              while (__ballot(bb != 0)) {
+                uint64_t g = ~0ull;
+                if (bb) {
+                  int tq = ctz64(bb);
+                  bb &= bb - 1;
+                  g = M.all_single ? moved_idx(M.ii, idx, p, j - 2, a, tq, ob, ra)
+                                   : sq_to_idx(T, M.ii, sq_set(p, j, tq));
+                }
+                wave_mark(own, ownw, g, lane);
+              }
+            } else {
+              while (bb) {  // device atomics, targets in groups of 4
+                uint64_t g[4];
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                  g[k] = ~0ull;
+                  if (bb) {
+                    int tq = ctz64(bb);
+                    bb &= bb - 1;
+                    g[k] = M.all_single ? moved_idx(M.ii, idx, p, j - 2, a, tq, ob, ra)
+                                        : sq_to_idx(T, M.ii, sq_set(p, j, tq));
+                  }
+                }
+                mark4(own, ownw, g);
+              }
             }
           }
         }
@@ -399,6 +538,11 @@ __device__ int synth_exit(const Tabs &T, const Mat &M, int stm, uint64_t q0, uin
 
 // check_successors_worker + check_moves + check_king_moves
 // (src/tb8gen.c:529-650): is candidate idx of slice s a loss for stm?
+// Candidates come from cand & ~wins(stm), and wins(stm) holds every position
+// that is illegal with stm to move from init on, so the legality test of
+// the reference is never needed here; each piece's attack set is computed
+// only when its moves are reached (most candidates fail at their first
+// move).  Captures look up the captured piece's set from its slot.
 __device__ bool verify_pos(const Tabs &T, const Mat &M, int stm, int s, uint64_t idx,
                            const uint64_t *__restrict__ wins_o, const uint64_t *__restrict__ sub_o)
 {
@@ -406,43 +550,30 @@ __device__ bool verify_pos(const Tabs &T, const Mat &M, int stm, int s, uint64_t
   uint64_t p = kings_of(T, s);
   uint64_t occ = idx_to_sq(T, M.ii, idx, &p);
   int ks = sq_get(p, stm), oks = sq_get(p, o);
-  uint64_t att[MAXS];
-  uint64_t all = 0;
-#pragma unroll
-  for (int t = 0; t < MAXS; t++) {
-    att[t] = 0;
-    if (t < M.nk[stm]) {
-      int j = M.pcs[stm][t];
-      att[t] = piece_att(T, M.pt[j], sq_get(p, j), occ);
-    }
-    all |= att[t];
+  uint64_t oppocc = 0;
+  for (int t = 0; t < MAXP - 1; t++) {
+    if (t >= M.nk[o]) break;
+    oppocc |= bit(sq_get(p, M.pcs[o][t]));
   }
-  if (all & bit(oks)) return false;  // illegal: stm attacks the enemy king
-  uint64_t so[MAXS];
-  set_occ(M.ii, p, so);
   const uint64_t *wins_s = wins_o + phys(M, s) * M.words;
   const uint64_t *sub_s = sub_o + phys(M, s) * M.ctotal[o];
-#pragma unroll
-  for (int t = 0; t < MAXS; t++) {
+  for (int t = 0; t < MAXP - 1; t++) {
     if (t >= M.nk[stm]) break;
     int j = M.pcs[stm][t];
-    uint64_t a = att[t];
-    for (int l = 0; l < MAXS; l++) {
-      if (l >= M.numsets) break;
-      if (M.setcol[l] != o) continue;
-      uint64_t c = a & so[l];
-      while (c) {
-        int to = ctz64(c);
-        c &= c - 1;
-        int jj = slot_of(p, to, M.setmask80[l]);
-        uint64_t q = sq_set(p, j, to);
-        q = sq_set(q, jj, sq_get(q, M.ii.last[l]));
-        if (!bm_test(sub_s + M.coff[l], sq_to_idx(T, M.capt[l], q))) return false;
-      }
+    int fr = sq_get(p, j);
+    uint64_t a = piece_att(T, M.pt[j], fr, occ);
+    uint64_t c = a & oppocc;
+    while (c) {
+      int to = ctz64(c);
+      c &= c - 1;
+      int jj = slot_of(p, to, M.col80[o]);
+      int l = sq_get(M.set_of, jj);
+      uint64_t q = sq_set(p, j, to);
+      q = sq_set(q, jj, sq_get(q, M.ii.last[l]));
+      if (!bm_test(sub_s + M.coff[l], sq_to_idx(T, M.capt[l], q))) return false;
     }
     uint64_t qm = a & ~occ;
     if (M.all_single) {
-      int fr = sq_get(p, j);
       uint64_t ob = occ_before(p, j - 2);
       uint32_t ra = (uint32_t)(fr - popc64(ob & (bit(fr) - 1)));
       while (qm) {
@@ -458,25 +589,22 @@ __device__ bool verify_pos(const Tabs &T, const Mat &M, int stm, int s, uint64_t
     }
   }
   uint64_t kb = T.king[ks] & ~T.king[oks];
-  for (int l = 0; l < MAXS; l++) {
-    if (l >= M.numsets) break;
-    if (M.setcol[l] != o) continue;
-    uint64_t c = kb & so[l];
-    while (c) {
-      int to = ctz64(c);
-      c &= c - 1;
-      int jj = slot_of(p, to, M.setmask80[l]);
-      uint64_t q0 = sq_set(p, stm, to), q;
-      int s2 = normalize(T, q0, &q);
-      if (M.synth && s2 != M.synth_slice) {
-        int v = synth_exit(T, M, stm, q0, q, occ, ks, jj, l, s2);
-        if (v == 3 || v == 2) continue;
-        return false;
-      }
-      q = sq_set(q, jj, sq_get(q, M.ii.last[l]));
-      const uint64_t *sb = sub_o + phys(M, s2) * M.ctotal[o] + M.coff[l];
-      if (!bm_test(sb, sq_to_idx(T, M.capt[l], q))) return false;
+  uint64_t c = kb & oppocc;
+  while (c) {
+    int to = ctz64(c);
+    c &= c - 1;
+    int jj = slot_of(p, to, M.col80[o]);
+    int l = sq_get(M.set_of, jj);
+    uint64_t q0 = sq_set(p, stm, to), q;
+    int s2 = normalize(T, q0, &q);
+    if (M.synth && s2 != M.synth_slice) {
+      int v = synth_exit(T, M, stm, q0, q, occ, ks, jj, l, s2);
+      if (v == 3 || v == 2) continue;
+      return false;
     }
+    q = sq_set(q, jj, sq_get(q, M.ii.last[l]));
+    const uint64_t *sb = sub_o + phys(M, s2) * M.ctotal[o] + M.coff[l];
+    if (!bm_test(sb, sq_to_idx(T, M.capt[l], q))) return false;
   }
   uint64_t kq = kb & ~occ;
   while (kq) {
@@ -542,10 +670,10 @@ __global__ __launch_bounds__(256, 4) void k_verify_j(JobDev *const *__restrict__
       uint64_t a = phys(M, s) * M.words + w;
       uint64_t x0 = 0, x = 0;
       if (live) {
-        x0 = J.cand[stm][a];
+        x0 = gload(J.cand[stm], a);
         uint64_t c = x0;
-        if (exits) c |= J.newW[stm][a];
-        x = run && c ? c & ~J.wins[stm][a] : 0;
+        if (exits) c |= gload(J.newW[stm], a);
+        x = run && c ? c & ~gload(J.wins[stm], a) : 0;
       }
       uint64_t r = 0;
       if (__ballot(x != 0)) {

@@ -136,7 +136,9 @@ def test_synthetic_single_slice(material, slice_, seed):
     t.close()
 
 
-@pytest.mark.parametrize("mode", [tbg.SCATTER_GLOBAL, tbg.SCATTER_BLOCK])
+@pytest.mark.parametrize("mode", [tbg.SCATTER_GLOBAL, tbg.SCATTER_BLOCK,
+                                  tbg.SCATTER_GLOBAL | tbg.SCATTER_COMBINE,
+                                  tbg.SCATTER_BLOCK | tbg.SCATTER_COMBINE])
 def test_scatter_modes_agree(registry, mode):
     """Global-atomic and LDS block-privatised scatter give the same table."""
     material = "KQvKR"
@@ -158,11 +160,13 @@ def _tune_candidates():
         dict(verify_unit_words=256, sweep_unit_words=512, pred_wg_per_cu=1, uncap_wg_per_cu=1,
              verify_wg_per_cu=1, sweep_wg_per_cu=1, scatter=tbg.SCATTER_BLOCK),
         dict(verify_unit_words=4096, sweep_unit_words=16384, pred_wg_per_cu=4, uncap_wg_per_cu=32,
-             verify_wg_per_cu=8, sweep_wg_per_cu=8, scatter=tbg.SCATTER_BLOCK),
+             verify_wg_per_cu=8, sweep_wg_per_cu=8,
+             scatter=tbg.SCATTER_BLOCK | tbg.SCATTER_COMBINE),
         dict(verify_unit_words=65536, sweep_unit_words=65536, pred_wg_per_cu=8, uncap_wg_per_cu=4,
              verify_wg_per_cu=64, sweep_wg_per_cu=64, scatter=tbg.SCATTER_GLOBAL),
         dict(verify_unit_words=768, sweep_unit_words=1536, pred_wg_per_cu=3, uncap_wg_per_cu=16,
-             verify_wg_per_cu=16, sweep_wg_per_cu=16, scatter=tbg.SCATTER_GLOBAL),
+             verify_wg_per_cu=16, sweep_wg_per_cu=16,
+             scatter=tbg.SCATTER_GLOBAL | tbg.SCATTER_COMBINE),
     ]
 
 
@@ -239,7 +243,8 @@ def test_synthetic_block_scatter_multi_block():
                 max_plies=12)
     with open("/tmp/oracle_synth_mb.log") as f:
         olines = [l.strip() for l in f if l.strip()]
-    for mode in (tbg.SCATTER_BLOCK, tbg.SCATTER_GLOBAL):
+    for mode in (tbg.SCATTER_BLOCK, tbg.SCATTER_GLOBAL, tbg.SCATTER_BLOCK | tbg.SCATTER_COMBINE,
+                 tbg.SCATTER_GLOBAL | tbg.SCATTER_COMBINE):
         t = tbg.Table(material, synth_slice=s, seed=seed)
         t.set_scatter(mode)
         t.resolve([])
