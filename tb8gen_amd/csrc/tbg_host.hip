@@ -239,6 +239,7 @@ struct LaunchSet {
   uint64_t *P[4] = { nullptr, nullptr, nullptr, nullptr };
   uint64_t total[4] = { 0, 0, 0, 0 };
   int np = 0;
+  size_t cap = 0;  // pairs the device buffers hold
   size_t pred_lds = 0;
   Geom g;
   void release()
@@ -247,6 +248,7 @@ struct LaunchSet {
     for (auto &x : P) (void)hipFree(x);
     pairs = nullptr;
     for (auto &x : P) x = nullptr;
+    cap = 0;
   }
 };
 
@@ -269,6 +271,10 @@ struct tbg_ctx {
   long long *dhist = nullptr;
   std::vector<long long> hist;
   int max_ply = 4096;
+  // a server job: memory from the stream-ordered pool of the server's setup
+  // stream (which it shares, so does not own), set up without host waits
+  bool pooled = false;
+  hipEvent_t ready = nullptr;  // pooled: set-up and init complete
   hipStream_t st = nullptr;
   int resolved = 0, inited = 0, generated = 0;
   uint64_t bytes = 0;
@@ -316,7 +322,7 @@ int build_launchset(LaunchSet &L, const std::vector<const JobDev *> &hjobs,
                     const std::vector<const Mat *> &mats, const std::vector<Pair> &pairs,
                     uint64_t window_slices, const Geom &g, hipStream_t st)
 {
-  L.release();
+  if (pairs.size() > L.cap) L.release();  // else the device buffers are reused
   L.np = (int)pairs.size();
   L.g = g;
   uint64_t maxbw = 1;
@@ -337,14 +343,16 @@ int build_launchset(LaunchSet &L, const std::vector<const JobDev *> &hjobs,
   }
   // the queues follow the block bitmap, which only the block scatter uses
   L.pred_lds = (block ? maxbw * 8 : 0) + PB_QUEUE_BYTES;
-  HIPCHK(hipMalloc(&L.pairs, sizeof(Pair) * pairs.size()));
+  if (!L.cap) {
+    L.cap = pairs.size();
+    HIPCHK(hipMalloc(&L.pairs, sizeof(Pair) * L.cap));
+    for (int kt = 0; kt < 4; kt++) HIPCHK(hipMalloc(&L.P[kt], sizeof(uint64_t) * (L.cap + 1)));
+  }
+  // pageable sources: the copies are staged before the calls return
   HIPCHK(hipMemcpyAsync(L.pairs, pairs.data(), sizeof(Pair) * pairs.size(), hipMemcpyHostToDevice, st));
-  for (int kt = 0; kt < 4; kt++) {
-    HIPCHK(hipMalloc(&L.P[kt], sizeof(uint64_t) * P[kt].size()));
+  for (int kt = 0; kt < 4; kt++)
     HIPCHK(hipMemcpyAsync(L.P[kt], P[kt].data(), sizeof(uint64_t) * P[kt].size(),
                           hipMemcpyHostToDevice, st));
-  }
-  HIPCHK(hipStreamSynchronize(st));
   return TBG_OK;
 }
 
@@ -404,15 +412,37 @@ int build_ctx_launchsets(tbg_ctx *c)
   return TBG_OK;
 }
 
-int ctx_alloc(tbg_ctx *c, bool with_codes)
+// Device memory of a context: hipMalloc, or for a pooled (server) context
+// the stream-ordered pool of its stream (freed back to the pool, so
+// admission after warm-up makes no driver allocation).
+int dalloc(tbg_ctx *c, void **p, size_t bytes)
+{
+  HIPCHK(c->pooled ? hipMallocAsync(p, bytes, c->st) : hipMalloc(p, bytes));
+  return TBG_OK;
+}
+
+void dfree(tbg_ctx *c, void *p)
+{
+  if (!p) return;
+  if (c->pooled) (void)hipFreeAsync(p, c->st);
+  else (void)hipFree(p);
+}
+
+// ext: the stream of a pooled context (server jobs), else nullptr
+int ctx_alloc(tbg_ctx *c, bool with_codes, hipStream_t ext = nullptr)
 {
   Mat &M = c->h;
   c->nsl = M.synth ? 1 : NSL;
   M.nslices_phys = c->nsl;
   c->pw = c->nsl * M.words;
-  HIPCHK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+  if (ext) {
+    c->st = ext;
+    c->pooled = true;
+  } else {
+    HIPCHK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+  }
   size_t pb = c->pw * 8;
-  HIPCHK(hipMalloc(&c->arena, pb * kPlanes));
+  TRY(dalloc(c, (void **)&c->arena, pb * kPlanes));
   c->bytes += pb * kPlanes;
   uint64_t *p = c->arena;
   for (int s = 0; s < 2; s++) {
@@ -425,7 +455,7 @@ int ctx_alloc(tbg_ctx *c, bool with_codes)
   }
   HIPCHK(hipMemsetAsync(c->arena, 0, pb * kPlanes, c->st));
   uint64_t sw = 4 * c->nsl * (M.ctotal[0] + M.ctotal[1]);
-  HIPCHK(hipMalloc(&c->subarena, sw * 8 + 64));
+  TRY(dalloc(c, (void **)&c->subarena, sw * 8 + 64));
   c->bytes += sw * 8;
   HIPCHK(hipMemsetAsync(c->subarena, 0, sw * 8 + 64, c->st));
   p = c->subarena;
@@ -437,28 +467,26 @@ int ctx_alloc(tbg_ctx *c, bool with_codes)
   if (with_codes)
     for (int s = 0; s < 2; s++) {
       size_t cb = c->nsl * M.ii.size * 2;
-      HIPCHK(hipMalloc(&c->codes[s], cb));
+      TRY(dalloc(c, (void **)&c->codes[s], cb));
       c->bytes += cb;
     }
-  HIPCHK(hipMalloc(&c->dM, sizeof(Mat)));
+  TRY(dalloc(c, (void **)&c->dM, sizeof(Mat)));
   HIPCHK(hipMemcpyAsync(c->dM, &c->h, sizeof(Mat), hipMemcpyHostToDevice, c->st));
-  HIPCHK(hipMalloc(&c->dR, sizeof(SubRef) * MAXS));
+  TRY(dalloc(c, (void **)&c->dR, sizeof(SubRef) * MAXS));
   HIPCHK(hipMemsetAsync(c->dR, 0, sizeof(SubRef) * MAXS, c->st));
-  HIPCHK(hipMalloc(&c->dcnt, 16 * sizeof(unsigned long long)));
+  TRY(dalloc(c, (void **)&c->dcnt, 16 * sizeof(unsigned long long)));
   HIPCHK(hipMemsetAsync(c->dcnt, 0, 16 * sizeof(unsigned long long), c->st));
-  HIPCHK(hipMalloc(&c->dflags, 4 * sizeof(unsigned)));
-  HIPCHK(hipMalloc(&c->dj, sizeof(JobDev)));
-  HIPCHK(hipMalloc(&c->djp, sizeof(JobDev *)));
+  TRY(dalloc(c, (void **)&c->dflags, 4 * sizeof(unsigned)));
+  TRY(dalloc(c, (void **)&c->dj, sizeof(JobDev)));
+  TRY(dalloc(c, (void **)&c->djp, sizeof(JobDev *)));
   HIPCHK(hipMemcpyAsync(c->djp, &c->dj, sizeof(JobDev *), hipMemcpyHostToDevice, c->st));
-  HIPCHK(hipMalloc(&c->dhist, sizeof(long long) * 4 * c->max_ply));
+  TRY(dalloc(c, (void **)&c->dhist, sizeof(long long) * 4 * c->max_ply));
   manager_order(WHITE, c->mgr[WHITE]);
   manager_order(BLACK, c->mgr[BLACK]);
   for (int i = 0; i < CAT_N; i++) c->ms[i] = 0, c->nl[i] = 0;
   reset_job(c);
-  {
-    int r = build_ctx_launchsets(c);
-    if (r != TBG_OK) return r;
-  }
+  if (c->pooled) return TBG_OK;  // the server batches it; no launch sets of its own, no wait
+  TRY(build_ctx_launchsets(c));
   HIPCHK(hipStreamSynchronize(c->st));
   return TBG_OK;
 }
@@ -622,7 +650,9 @@ int zero_plane(tbg_ctx *c, uint64_t *p)
   return TBG_OK;
 }
 
-int run_init(tbg_ctx *c)
+// illegal and mate positions, codes, job state; nothing read back (the
+// counts stay in dcnt[0..3] until read_init_counts)
+int run_init_async(tbg_ctx *c)
 {
   Mat &M = c->h;
   int s0 = M.synth ? M.synth_slice : 0, ns = M.synth ? 1 : NSL;
@@ -635,17 +665,29 @@ int run_init(tbg_ctx *c)
                        s0, ns, c->dcnt);
     HIPCHK(hipGetLastError());
   }
-  unsigned long long v[4];
-  HIPCHK(hipMemcpyAsync(v, c->dcnt, sizeof v, hipMemcpyDeviceToHost, c->st));
-  HIPCHK(hipStreamSynchronize(c->st));
-  c->counts.broken[0] = v[0];
-  c->counts.broken[1] = v[1];
-  c->counts.mates[0] = v[2];
-  c->counts.mates[1] = v[3];
   c->counts.nplies = 0;
   c->inited = 1;
   c->generated = 0;
   return init_job_state(c);
+}
+
+// the broken / mate counts of the last init, read on stream st
+int read_init_counts(tbg_ctx *c, hipStream_t st)
+{
+  unsigned long long v[4];
+  HIPCHK(hipMemcpyAsync(v, c->dcnt, sizeof v, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  c->counts.broken[0] = v[0];
+  c->counts.broken[1] = v[1];
+  c->counts.mates[0] = v[2];
+  c->counts.mates[1] = v[3];
+  return TBG_OK;
+}
+
+int run_init(tbg_ctx *c)
+{
+  TRY(run_init_async(c));
+  return read_init_counts(c, c->st);
 }
 
 // plain form: the host reads the job's flags each ply and issues one launch
@@ -730,13 +772,15 @@ int run_batch(tbg_ctx *prof, JobDev *const *jobs, int nj, const LaunchSet &L, un
 }
 
 // after a generation: pull the final job state and the per-ply counts
-int finish_job(tbg_ctx *c)
+// st: the stream to read on (a pooled job's own stream is shared)
+int finish_job(tbg_ctx *c, hipStream_t st = nullptr)
 {
-  TRY(download_job(c));
+  if (!st) st = c->st;
+  HIPCHK(hipMemcpyAsync(&c->hj, c->dj, sizeof(JobDev), hipMemcpyDeviceToHost, st));
   c->hist.assign((size_t)4 * c->max_ply, -1);
   HIPCHK(hipMemcpyAsync(c->hist.data(), c->dhist, sizeof(long long) * 4 * c->max_ply,
-                        hipMemcpyDeviceToHost, c->st));
-  HIPCHK(hipStreamSynchronize(c->st));
+                        hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
   int last = 0;
   for (int n = 1; n < c->max_ply; n++)
     for (int i = 0; i < 4; i++)
@@ -829,7 +873,7 @@ const char *tbg_strerror(int err)
 }
 
 static int create_common(tbg_ctx **out, const char *material, int synth, int slice, uint64_t seed,
-                         bool keep_codes = false)
+                         bool keep_codes = false, hipStream_t ext = nullptr)
 {
   if (!out || !material) return fail(TBG_E_INVAL, "null argument");
   *out = nullptr;
@@ -842,7 +886,7 @@ static int create_common(tbg_ctx **out, const char *material, int synth, int sli
   c->h.synth_slice = slice;
   c->h.seed = seed;
   c->h.seedmix = splitmix64(seed);
-  r = ctx_alloc(c, !synth || keep_codes);
+  r = ctx_alloc(c, !synth || keep_codes, ext);
   if (r != TBG_OK) {
     tbg_destroy(c);
     return r == TBG_E_HIP ? fail(TBG_E_NOMEM, "device allocation failed") : r;
@@ -862,22 +906,17 @@ int tbg_create_synth(tbg_ctx **out, const char *material, int slice, uint64_t se
 void tbg_destroy(tbg_ctx *c)
 {
   if (!c) return;
-  if (c->st) (void)hipStreamSynchronize(c->st);
+  // a pooled context's stream is shared: its frees are stream-ordered and
+  // its owner has already waited for the work that used the memory
+  if (c->st && !c->pooled) (void)hipStreamSynchronize(c->st);
   flush_timing(c);
   c->both.release();
   for (int s = 0; s < 2; s++) c->side[s].release(), c->plain[s].release();
-  (void)hipFree(c->arena);
-  (void)hipFree(c->subarena);
-  (void)hipFree(c->codes[0]);
-  (void)hipFree(c->codes[1]);
-  (void)hipFree(c->dM);
-  (void)hipFree(c->dR);
-  (void)hipFree(c->dcnt);
-  (void)hipFree(c->dflags);
-  (void)hipFree(c->dj);
-  (void)hipFree(c->djp);
-  (void)hipFree(c->dhist);
-  if (c->st) (void)hipStreamDestroy(c->st);
+  void *bufs[] = { c->arena, c->subarena, c->codes[0], c->codes[1], c->dM, c->dR,
+                   c->dcnt, c->dflags, c->dj, c->djp, c->dhist };
+  for (void *b : bufs) dfree(c, b);
+  if (c->ready) (void)hipEventDestroy(c->ready);
+  if (c->st && !c->pooled) (void)hipStreamDestroy(c->st);
   delete c;
 }
 
@@ -953,7 +992,7 @@ int tbg_resolve_captures(tbg_ctx *c, tbg_ctx *const *subs, int nsubs)
                        c->sub[col][2], c->sub[col][3], s0, ns);
     HIPCHK(hipGetLastError());
   }
-  HIPCHK(hipStreamSynchronize(c->st));
+  if (!c->pooled) HIPCHK(hipStreamSynchronize(c->st));  // a pooled job is waited for by its event
   c->resolved = 1;
   return TBG_OK;
 }
@@ -1570,7 +1609,8 @@ struct tbg_server {
   int max_active = 1;
   Geom geom = g_geom;
   std::vector<SrvJob> jobs;
-  hipStream_t st = nullptr, side = nullptr;
+  // st: the batch; side: spill copies; setup: job allocation and init
+  hipStream_t st = nullptr, side = nullptr, setup = nullptr;
 };
 
 extern "C" {
@@ -1584,9 +1624,17 @@ int tbg_server_create(tbg_server **out, uint64_t budget_bytes, int max_active)
   s->max_active = max_active;
   hipError_t e1 = hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking);
   hipError_t e2 = hipStreamCreateWithFlags(&s->side, hipStreamNonBlocking);
-  if (e1 != hipSuccess || e2 != hipSuccess) {
+  hipError_t e3 = hipStreamCreateWithFlags(&s->setup, hipStreamNonBlocking);
+  if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
     tbg_server_destroy(s);
     return fail(TBG_E_HIP, "server streams");
+  }
+  // job memory comes from the device's stream-ordered pool; keep what it
+  // has reserved rather than returning it at every synchronisation
+  hipMemPool_t pool;
+  if (hipDeviceGetDefaultMemPool(&pool, g_device) == hipSuccess) {
+    uint64_t keep = ~0ull;
+    (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
   }
   *out = s;
   return TBG_OK;
@@ -1597,13 +1645,16 @@ void tbg_server_destroy(tbg_server *s)
   if (!s) return;
   if (s->st) (void)hipStreamSynchronize(s->st);
   if (s->side) (void)hipStreamSynchronize(s->side);
+  if (s->setup) (void)hipStreamSynchronize(s->setup);
   for (auto &j : s->jobs) {
     if (j.ctx) tbg_destroy(j.ctx);
     for (auto &p : j.pinned) (void)hipHostFree(p);
     if (j.ev) (void)hipEventDestroy(j.ev);
   }
+  if (s->setup) (void)hipStreamSynchronize(s->setup);  // the pooled frees
   if (s->st) (void)hipStreamDestroy(s->st);
   if (s->side) (void)hipStreamDestroy(s->side);
+  if (s->setup) (void)hipStreamDestroy(s->setup);
   delete s;
 }
 
@@ -1611,8 +1662,8 @@ int tbg_server_submit(tbg_server *s, const tbg_job_spec *spec, uint64_t *job_id)
 {
   if (!s || !spec || !spec->material) return fail(TBG_E_INVAL, "null argument");
   if (spec->slice < 0 || spec->slice >= 441) return fail(TBG_E_INVAL, "slice must be in [0, 441)");
-  int err;
-  estimate_bytes(spec->material, &err);
+  Mat M;
+  int err = setup_mat(M, spec->material);
   if (err) return err;
   SrvJob j;
   j.material = spec->material;
@@ -1620,6 +1671,13 @@ int tbg_server_submit(tbg_server *s, const tbg_job_spec *spec, uint64_t *job_id)
   j.spec.material = nullptr;
   memset(&j.res, 0, sizeof j.res);
   j.res.arrival_ms = spec->arrival_ms;
+  // the pinned host buffers its codes spill to (a pinned allocation takes
+  // milliseconds, so it is made here, not while serving)
+  for (int stm = 0; stm < 2; stm++)
+    if (hipHostMalloc((void **)&j.pinned[stm], M.ii.size * 2, hipHostMallocDefault) != hipSuccess) {
+      for (auto &q : j.pinned) (void)hipHostFree(q);
+      return fail(TBG_E_NOMEM, "pinned host memory for a job's codes");
+    }
   s->jobs.push_back(std::move(j));
   if (job_id) *job_id = s->jobs.size() - 1;
   return TBG_OK;
@@ -1631,7 +1689,10 @@ int tbg_server_run(tbg_server *s, int form, tbg_server_stats *stats)
   using clk = std::chrono::steady_clock;
   auto t0 = clk::now();
   auto now_ms = [&]() { return std::chrono::duration<double, std::milli>(clk::now() - t0).count(); };
+  // the plain server runs one job at a time; both check the device every
+  // kPlies plies (a job that finishes meanwhile idles, its kernels skip it)
   int max_active = form == TBG_FORM_PLAIN ? 1 : s->max_active;
+  const int kPlies = 4;
   std::vector<int> pending;
   for (size_t i = 0; i < s->jobs.size(); i++)
     if (s->jobs[i].state == 0) pending.push_back((int)i);
@@ -1639,28 +1700,65 @@ int tbg_server_run(tbg_server *s, int form, tbg_server_stats *stats)
     return s->jobs[a].spec.arrival_ms < s->jobs[b].spec.arrival_ms;
   });
   size_t next = 0;
-  std::vector<int> active, spilling;
+  std::vector<int> active, joining, spilling;
   uint64_t used = 0, peak = 0;
   int peak_active = 0;
   double occ_integral = 0, last_t = 0;
   uint64_t plies = 0, done = 0, rejected = 0;
   double psp = 0;
-  // batch state
+  // batch state, kept across membership changes
   JobDev **dptr = nullptr;
-  unsigned *dflags = nullptr, *ddone = nullptr;
+  unsigned *dflags = nullptr, *ddone = nullptr, *hflags = nullptr, *hdone = nullptr;
+  hipEvent_t evb = nullptr;
   HIPCHK(hipMalloc(&dptr, sizeof(JobDev *) * max_active));
   HIPCHK(hipMalloc(&dflags, 4 * sizeof(unsigned)));
   HIPCHK(hipMalloc(&ddone, sizeof(unsigned) * max_active));
+  HIPCHK(hipHostMalloc((void **)&hflags, 4 * sizeof(unsigned), hipHostMallocDefault));
+  HIPCHK(hipHostMalloc((void **)&hdone, sizeof(unsigned) * max_active, hipHostMallocDefault));
+  HIPCHK(hipEventCreateWithFlags(&evb, hipEventDisableTiming));
   LaunchSet L;
   bool changed = true;
-  unsigned hflags[2] = { 0, 1 };
-  std::vector<unsigned> hdone(max_active);
   int r = TBG_OK;
-  while (r == TBG_OK && (next < pending.size() || !active.empty() || !spilling.empty())) {
+  while (r == TBG_OK &&
+         (next < pending.size() || !active.empty() || !joining.empty() || !spilling.empty())) {
     double t = now_ms();
     occ_integral += (double)used * (t - last_t);
     last_t = t;
-    // spills that reached the host end their jobs
+    // 1. the batch: rebuilt when its membership changed, then kPlies plies
+    //    queued without a host wait
+    int na = (int)active.size();
+    if (na && changed) {
+      std::vector<JobDev *> ptrs;
+      std::vector<const JobDev *> hj;
+      std::vector<const Mat *> mats;
+      std::vector<Pair> pairs;
+      for (int i = 0; i < na; i++) {
+        tbg_ctx *c = s->jobs[active[i]].ctx;
+        ptrs.push_back(c->dj);
+        hj.push_back(&c->hj);
+        mats.push_back(&c->h);
+        pairs.push_back({ i, 0 });
+        pairs.push_back({ i, 1 });
+      }
+      HIPCHK(hipMemcpyAsync(dptr, ptrs.data(), sizeof(JobDev *) * na, hipMemcpyHostToDevice, s->st));
+      unsigned f[4] = { 0, 1, 0, 0 };  // a new job may be at ply 1
+      HIPCHK(hipMemcpyAsync(dflags, f, sizeof f, hipMemcpyHostToDevice, s->st));
+      r = build_launchset(L, hj, mats, pairs, 0, s->geom, s->st);
+      if (r != TBG_OK) break;
+      changed = false;
+    }
+    if (na) {
+      for (int k = 0; k < kPlies && r == TBG_OK; k++) {
+        r = launch_ply(nullptr, dptr, na, L, dflags, s->st, true, ddone);
+        plies++;
+      }
+      if (r != TBG_OK) break;
+      HIPCHK(hipMemcpyAsync(hflags, dflags, 4 * sizeof(unsigned), hipMemcpyDeviceToHost, s->st));
+      HIPCHK(hipMemcpyAsync(hdone, ddone, sizeof(unsigned) * na, hipMemcpyDeviceToHost, s->st));
+      HIPCHK(hipEventRecord(evb, s->st));
+    }
+    // 2. host work while those plies run: finished spills, admission (set-up
+    //    and init queued on the setup stream), jobs whose set-up is done
     for (size_t i = 0; i < spilling.size();) {
       SrvJob &j = s->jobs[spilling[i]];
       if (hipEventQuery(j.ev) == hipSuccess) {
@@ -1675,8 +1773,7 @@ int tbg_server_run(tbg_server *s, int form, tbg_server_stats *stats)
         i++;
       }
     }
-    // admission: arrived jobs in FIFO order while slots and budget allow
-    while (next < pending.size() && (int)active.size() < max_active) {
+    while (next < pending.size() && (int)(active.size() + joining.size()) < max_active) {
       SrvJob &j = s->jobs[pending[next]];
       if (j.spec.arrival_ms > now_ms()) break;
       int err;
@@ -1691,98 +1788,87 @@ int tbg_server_run(tbg_server *s, int form, tbg_server_stats *stats)
       }
       if (used + need > s->budget) break;  // wait for memory
       tbg_ctx *c = nullptr;
-      r = create_common(&c, j.material.c_str(), 1, j.spec.slice, j.spec.seed, true);
+      r = create_common(&c, j.material.c_str(), 1, j.spec.slice, j.spec.seed, true, s->setup);
       if (r != TBG_OK) break;
       j.ctx = c;
       r = tbg_resolve_captures(c, nullptr, 0);
-      if (r == TBG_OK) r = run_init(c);
+      if (r == TBG_OK) r = run_init_async(c);
       if (r != TBG_OK) break;
+      HIPCHK(hipEventCreateWithFlags(&c->ready, hipEventDisableTiming));
+      HIPCHK(hipEventRecord(c->ready, s->setup));
       j.res.device_bytes = c->bytes;
       j.res.positions = c->h.ii.size;
       j.res.admit_ms = now_ms();
       used += c->bytes;
       peak = used > peak ? used : peak;
       j.state = 1;
-      active.push_back(pending[next]);
+      joining.push_back(pending[next]);
       next++;
+    }
+    if (r != TBG_OK) break;
+    // 3. wait: for the plies, else for a set-up or a spill, else for the
+    //    next arrival
+    if (na) {
+      HIPCHK(hipEventSynchronize(evb));
+    } else if (!joining.empty()) {
+      HIPCHK(hipEventSynchronize(s->jobs[joining.front()].ctx->ready));
+    } else if (!spilling.empty()) {
+      HIPCHK(hipStreamSynchronize(s->side));
+    } else if (next < pending.size()) {
+      double wait = s->jobs[pending[next]].spec.arrival_ms - now_ms();
+      if (wait > 0) std::this_thread::sleep_for(std::chrono::microseconds((int64_t)(wait * 1000)));
+    }
+    // 4. finished jobs leave the batch; their codes go to pinned host memory
+    if (na && hflags[0]) {
+      std::vector<int> keep;
+      for (int i = 0; i < na && r == TBG_OK; i++) {
+        SrvJob &j = s->jobs[active[i]];
+        if (!hdone[i]) { keep.push_back(active[i]); continue; }
+        tbg_ctx *c = j.ctx;
+        r = finish_job(c, s->st);
+        if (r == TBG_OK) r = read_init_counts(c, s->st);
+        if (r != TBG_OK) break;
+        j.hist = c->hist;
+        j.counts = c->counts;
+        j.res.nplies = c->counts.nplies;
+        j.res.half_steps = executed_half_steps(c);
+        psp += (double)j.res.half_steps * (double)c->h.ii.size / 2.0;
+        uint64_t n = c->h.ii.size;
+        for (int stm = 0; stm < 2; stm++) {
+          if (!j.pinned[stm]) HIPCHK(hipHostMalloc((void **)&j.pinned[stm], n * 2, hipHostMallocDefault));
+          HIPCHK(hipMemcpyAsync(j.pinned[stm], c->codes[stm], n * 2, hipMemcpyDeviceToHost, s->side));
+        }
+        if (!j.ev) HIPCHK(hipEventCreateWithFlags(&j.ev, hipEventDisableTiming));
+        HIPCHK(hipEventRecord(j.ev, s->side));
+        j.state = 2;
+        spilling.push_back(active[i]);
+      }
+      if (r != TBG_OK) break;
+      active = keep;
       changed = true;
     }
-    if (r != TBG_OK) break;
+    // 5. jobs whose set-up and init completed join the batch
+    for (size_t i = 0; i < joining.size();) {
+      if (hipEventQuery(s->jobs[joining[i]].ctx->ready) == hipSuccess) {
+        active.push_back(joining[i]);
+        joining.erase(joining.begin() + i);
+        changed = true;
+      } else {
+        i++;
+      }
+    }
     peak_active = (int)active.size() > peak_active ? (int)active.size() : peak_active;
-    if (active.empty()) {
-      if (!spilling.empty()) {
-        (void)hipStreamSynchronize(s->side);
-      } else if (next < pending.size()) {
-        double wait = s->jobs[pending[next]].spec.arrival_ms - now_ms();
-        if (wait > 0) std::this_thread::sleep_for(std::chrono::microseconds((int64_t)(wait * 1000)));
-      }
-      continue;
-    }
-    if (changed) {
-      std::vector<JobDev *> ptrs;
-      std::vector<const JobDev *> hj;
-      std::vector<const Mat *> mats;
-      std::vector<Pair> pairs;
-      for (size_t i = 0; i < active.size(); i++) {
-        tbg_ctx *c = s->jobs[active[i]].ctx;
-        HIPCHK(hipStreamSynchronize(c->st));
-        ptrs.push_back(c->dj);
-        hj.push_back(&c->hj);
-        mats.push_back(&c->h);
-        pairs.push_back({ (int)i, 0 });
-        pairs.push_back({ (int)i, 1 });
-      }
-      HIPCHK(hipMemcpyAsync(dptr, ptrs.data(), sizeof(JobDev *) * ptrs.size(), hipMemcpyHostToDevice,
-                            s->st));
-      hflags[0] = 0;
-      hflags[1] = 1;  // a new job may be at ply 1
-      HIPCHK(hipMemcpyAsync(dflags, hflags, sizeof hflags, hipMemcpyHostToDevice, s->st));
-      r = build_launchset(L, hj, mats, pairs, 0, s->geom, s->st);
-      if (r != TBG_OK) break;
-      changed = false;
-    }
-    // one ply of every active job
-    int na = (int)active.size();
-    r = launch_ply(nullptr, dptr, na, L, dflags, s->st, hflags[1] != 0, ddone);
-    if (r != TBG_OK) break;
-    plies++;
-    HIPCHK(hipMemcpyAsync(hflags, dflags, sizeof hflags, hipMemcpyDeviceToHost, s->st));
-    HIPCHK(hipMemcpyAsync(hdone.data(), ddone, sizeof(unsigned) * na, hipMemcpyDeviceToHost, s->st));
-    HIPCHK(hipStreamSynchronize(s->st));
-    if (!hflags[0]) continue;
-    // finished jobs leave the batch; their codes go to pinned host memory
-    std::vector<int> keep;
-    for (int i = 0; i < na; i++) {
-      SrvJob &j = s->jobs[active[i]];
-      if (!hdone[i]) { keep.push_back(active[i]); continue; }
-      tbg_ctx *c = j.ctx;
-      r = finish_job(c);
-      if (r != TBG_OK) break;
-      j.hist = c->hist;
-      j.counts = c->counts;
-      j.res.nplies = c->counts.nplies;
-      j.res.half_steps = executed_half_steps(c);
-      psp += (double)j.res.half_steps * (double)c->h.ii.size / 2.0;
-      uint64_t n = c->h.ii.size;
-      for (int stm = 0; stm < 2; stm++) {
-        if (!j.pinned[stm]) HIPCHK(hipHostMalloc((void **)&j.pinned[stm], n * 2, hipHostMallocDefault));
-        HIPCHK(hipMemcpyAsync(j.pinned[stm], c->codes[stm], n * 2, hipMemcpyDeviceToHost, s->side));
-      }
-      if (!j.ev) HIPCHK(hipEventCreateWithFlags(&j.ev, hipEventDisableTiming));
-      HIPCHK(hipEventRecord(j.ev, s->side));
-      j.state = 2;
-      spilling.push_back(active[i]);
-    }
-    if (r != TBG_OK) break;
-    active = keep;
-    changed = true;
   }
   (void)hipStreamSynchronize(s->st);
   (void)hipStreamSynchronize(s->side);
+  (void)hipStreamSynchronize(s->setup);
   L.release();
   (void)hipFree(dptr);
   (void)hipFree(dflags);
   (void)hipFree(ddone);
+  (void)hipHostFree(hflags);
+  (void)hipHostFree(hdone);
+  (void)hipEventDestroy(evb);
   if (r != TBG_OK) return r;
   if (stats) {
     stats->wall_ms = now_ms();

@@ -345,7 +345,9 @@ def test_served_jobs_same_alone_and_in_the_mix():
     codes and count lines equal the oracle's for that job alone."""
     print("case served mix", flush=True)
     srv = tbg.Server(budget_bytes=8 << 30, max_active=6)
-    ids = [srv.submit(m, s, seed, arrival_ms=3.0 * i) for i, (m, s, seed) in enumerate(SERVED_JOBS)]
+    # four jobs at once, the rest 1 ms later, while the first ones still run
+    ids = [srv.submit(m, s, seed, arrival_ms=0.0 if i < 4 else 1.0)
+           for i, (m, s, seed) in enumerate(SERVED_JOBS)]
     st = srv.run(tbg.FORM_BATCHED)
     assert st.jobs_done == len(SERVED_JOBS) and st.jobs_rejected == 0
     assert st.peak_active > 1  # jobs overlapped on the device
