@@ -11,18 +11,22 @@ One step = one such generation.
 value = position-side-plies per second over the whole step: every executed
 half-step (L or W, one side) covers 462 * 226,920 positions for half a ply,
 so a full ply of both sides is 2 * 104,837,040 position-side-plies.
-roofline: the kernel with the largest device time; algorithmic bytes per
-launch = its bits per position (pred 2, verify 3, W sweep 4; 11 per
-position-side-ply in all) * positions it covers / 8, over its average
-launch time measured with HIP events on the engine's stream.
+roofline: the kernel with the largest device time (kernel_rooflines lists all
+three ply kernels): algorithmic bytes per launch = its bits per position (pred
+2, verify 3, W sweep 4; 11 per position-side-ply in all) x positions x the
+half-steps it executed / its launches, over its average launch time measured
+with HIP events on the engine's stream; traffic = HBM bytes per launch from
+the committed rocprof counter summary of the same workload (profiles/).
 cpu_baseline: the CPU oracle (a plain-C restatement of the reference's
 algorithm) on the host cores, timed over plies 1..2 of the same material.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--form batched|plain]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--form batched|graph|plain]
+       [--workload table|synth|stream|served] [--no-autotune] [--scatter MODE]
 """
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -51,6 +55,39 @@ def run_steps(table, subs, form, steps, tbg):
     timing = table.timing()
     table.set_profiling(False)
     return secs, timing, lines
+
+
+def kernel_rooflines(tm, lines, positions, steps, traffic=None):
+    """Per ply kernel: algorithmic bytes per launch = its bits per position x
+    positions x the half-steps it executed (from the count lines: pred runs
+    for every W step and every L step after ply 1, verify for L steps, sweep
+    for W steps) / its launches; average launch time from the HIP events;
+    achieved GB/s = the two divided.  traffic: measured HBM bytes per launch
+    (copy-calibrated rocprof counters) where a profile is committed."""
+    lh = [l for l in lines if re.match(r"l\d+_[wb] = ", l) and not l.startswith("l0_")]
+    wh = [l for l in lines if re.match(r"w\d+_[wb] = ", l)]
+    l1 = [l for l in lh if l.startswith("l1_")]
+    hs = {"pred": len(lh) - len(l1) + len(wh), "verify": len(lh), "sweep": len(wh)}
+    out = {}
+    for k in ("pred", "verify", "sweep"):
+        ms = getattr(tm, "ms_" + k) / steps
+        n = getattr(tm, "n_" + k) / steps
+        alg = BITS[k] * positions * hs[k] / 8.0 / max(n, 1.0)
+        avg = ms / max(n, 1.0)
+        ach = alg / (avg * 1e-3) / 1e9 if avg > 0 else 0.0
+        out[k] = {"kernel": k, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
+                  "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                  "traffic": (traffic or {}).get(k), "alg_bytes_per_launch": alg,
+                  "avg_launch_ms": avg, "launches": n, "half_steps": hs[k], "ms": ms}
+    return out
+
+
+def load_traffic(name):
+    path = os.path.join(ROOT, "profiles", f"traffic_{name}.json")
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f)
+    return None
 
 
 def scatter_mode(name, tbg):
@@ -240,7 +277,10 @@ def run_synth(args, tbg):
     for step in range(args.warmup + args.steps):
         t0 = time.perf_counter()
 
-        def progress(line, _u, t0=t0):
+        lines = []
+
+        def progress(line, _u, t0=t0, lines=lines):
+            lines.append(line.decode())
             print(f"  [{time.perf_counter() - t0:8.2f} s] {line.decode()}", file=sys.stderr,
                   flush=True)
 
@@ -260,14 +300,9 @@ def run_synth(args, tbg):
     steps = max(1, args.steps)
     half_steps = tm.half_steps / steps
     psp = half_steps * n / 2.0
-    cats = {"pred": (tm.ms_pred, tm.n_pred), "verify": (tm.ms_verify, tm.n_verify),
-            "sweep": (tm.ms_sweep, tm.n_sweep)}
-    dom = max(cats, key=lambda k: cats[k][0])
-    avg_ms = cats[dom][0] / max(1, cats[dom][1])
-    alg = BITS[dom] * n / 8.0
-    achieved = alg / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
-    sweep_ms = tm.ms_sweep / max(1, tm.n_sweep)
-    sweep_gbs = BITS["sweep"] * n / 8.0 / (sweep_ms * 1e-3) / 1e9 if sweep_ms > 0 else 0.0
+    kr = kernel_rooflines(tm, lines, n, steps, load_traffic(f"{material}_synth"))
+    dom = max(kr, key=lambda k: kr[k]["ms"])
+    roof = {k: v for k, v in kr[dom].items() if k not in ("ms", "half_steps")}
     out = {
         "metric": METRIC, "value": psp / (sum(secs) / len(secs)),
         "unit": "position-side-plies/s", "n_gpus": 1, "steps": args.steps,
@@ -277,11 +312,8 @@ def run_synth(args, tbg):
                                f"seeded hash exits (seed {args.seed}), full fixpoint",
                    "material": material, "slice": args.slice, "positions_per_side": n,
                    "half_steps_per_step": half_steps, "plies": table.counts.nplies},
-        "roofline": {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "alg_bytes_per_launch": alg, "avg_launch_ms": avg_ms},
-        "sweep_roofline": {"achieved": sweep_gbs, "frac": sweep_gbs / HBM_PEAK_GBS,
-                           "avg_launch_ms": sweep_ms},
+        "roofline": roof,
+        "kernel_rooflines": kr,
         "kernels_ms_per_step": {k: getattr(tm, "ms_" + k) / steps for k in
                                 ("pred", "verify", "sweep", "uncap", "init", "resolve")},
     }
@@ -367,19 +399,10 @@ def main():
     psp = half_steps * n_total / 2.0  # position-side-plies per step
     value = psp / (sum(secs) / len(secs))
 
-    cats = {"pred": (tm.ms_pred, tm.n_pred), "verify": (tm.ms_verify, tm.n_verify),
-            "sweep": (tm.ms_sweep, tm.n_sweep)}
-    dom = max(cats, key=lambda k: cats[k][0])
-    dom_ms, dom_n = cats[dom]
-    avg_ms = dom_ms / max(1, dom_n)
-    covered = table.size if form == tbg.FORM_PLAIN else n_total
-    alg_bytes = BITS[dom] * covered / 8.0
-    achieved = alg_bytes / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"traffic_{args.material}_{args.form}.json")
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            traffic = json.load(f).get(dom)
+    kr = kernel_rooflines(tm, lines, n_total, args.steps,
+                          load_traffic(f"{args.material}_{args.form}"))
+    dom = max(kr, key=lambda k: kr[k]["ms"])
+    roof = {k: v for k, v in kr[dom].items() if k not in ("ms", "half_steps")}
 
     out = {
         "metric": METRIC,
@@ -403,11 +426,8 @@ def main():
             "positions_per_side": n_total, "half_steps_per_step": half_steps,
             "plies": table.counts.nplies,
         },
-        "roofline": {
-            "kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-            "alg_bytes_per_launch": alg_bytes, "avg_launch_ms": avg_ms, "launches": dom_n / max(1, args.steps),
-        },
+        "roofline": roof,
+        "kernel_rooflines": kr,
         "kernels_ms_per_step": {
             "pred": tm.ms_pred / args.steps, "verify": tm.ms_verify / args.steps,
             "sweep": tm.ms_sweep / args.steps, "uncap": tm.ms_uncap / args.steps,
