@@ -172,22 +172,26 @@ def run_stream(args, tbg):
                    "form": args.form, "levels": per_level},
     }
     if args.stream_plain:
-        lev = levels[1]
-        base = [tables[m] for m in levels[0]]
-        fresh = [tbg.Table(m) for m in lev]
-        t0 = time.perf_counter()
-        for t in fresh:
-            t.resolve(base)
-        plines = tbg.generate_batch(fresh, tbg.FORM_PLAIN)
-        pt = time.perf_counter() - t0
-        psp = sum((len(l) - 4) * tbg.NSLICES * t.size / 2.0 for l, t in zip(plines, fresh))
-        same = all(pl == tables[m].lines for pl, m in zip(plines, lev))
-        lvl4 = per_level[1]
-        out["plain_4piece_level"] = {"s": pt, "value": psp / pt,
-                                     "batched_over_plain": lvl4["value"] / (psp / pt),
-                                     "same_count_lines": same}
-        for t in fresh:
-            t.close()
+        # the plain form (one launch per slice and phase, a host round trip
+        # per ply) on the whole 4-piece level and on a sample of six 5-piece
+        # materials, against the batched rate of the same level
+        for key, li, lev in (("plain_4piece_level", 1, levels[1]),
+                             ("plain_5piece_sample", 2, levels[2][::10])):
+            base = [tables[m] for m in sum(levels[:li], [])]
+            fresh = [tbg.Table(m) for m in lev]
+            t0 = time.perf_counter()
+            for t in fresh:
+                t.resolve(base)
+            plines = tbg.generate_batch(fresh, tbg.FORM_PLAIN)
+            pt = time.perf_counter() - t0
+            psp = sum((len(l) - 4) * tbg.NSLICES * t.size / 2.0 for l, t in zip(plines, fresh))
+            same = all(pl == tables[m].lines for pl, m in zip(plines, lev))
+            out[key] = {"materials": lev, "s": pt, "value": psp / pt,
+                        "batched_over_plain": per_level[li]["value"] / (psp / pt),
+                        "same_count_lines": same}
+            print(f"{key}: {out[key]}", file=sys.stderr, flush=True)
+            for t in fresh:
+                t.close()
     print(json.dumps(out), flush=True)
     for t in tables.values():
         t.close()

@@ -37,6 +37,7 @@ extern "C" {
 #define TBG_E_STATE (-5)   /* call out of order (e.g. step before init)   */
 #define TBG_E_BUDGET (-6)  /* job exceeds the device-memory budget        */
 #define TBG_E_NODEV (-7)   /* no HIP device                               */
+#define TBG_E_IO (-8)      /* a slice file could not be written or read   */
 
 #define TBG_NSLICES 462
 #define TBG_DRAW_RULE 100 /* src/tb8gen.c:36 */
@@ -199,6 +200,42 @@ int tbg_get_sub_bitmap(tbg_ctx *ctx, int kind, int colour, int slice, uint64_t *
 #define TBG_SCATTER_BLOCK 1
 #define TBG_SCATTER_COMBINE 2
 int tbg_set_scatter(tbg_ctx *ctx, int mode);
+
+/* Slice files in the reference's format (src/util.c:179-343): frames of
+ * {compressed size, size, offset} (three little-endian uint64) followed by
+ * the zstd level-6 compression of at most 50 KiB of the slice's bytes, in
+ * any order.  Host only: no device or tbg_init needed.  Replace
+ * write_data / read_data of src/util.c. */
+int tbg_file_write(const char *path, const void *data, uint64_t bytes);
+int tbg_file_read(const char *path, void *data, uint64_t bytes);
+
+/* Checkpoint and resume at ply granularity, in the reference's slice-file
+ * tree under root (names of src/kslice.c:306-315, <wk><bk> per slice):
+ *   0/wins/<w|b>/..     wins(stm): the illegal positions and W_1..W_{n-1}
+ *   <n-1>/W/<w|b>/..    W_{n-1}, the frontier of the next L step
+ *   <n-1>/L/<w|b>/..    L_{n-1}, the frontier of the next W step
+ *   sub/<loss|bloss|cwin|win>/<w|b>/..  capture exits (by colour)
+ * plus this library's own codes/<w|b>/.. (uint16 per position) and
+ * tbg_state (ply, step flags, count history).  Directories are created.
+ * tbg_checkpoint needs a context past ply 1 (after tbg_continue stopped at
+ * a ply, or a finished generation).  tbg_restore loads one into a fresh
+ * context of the same material (and slice, seed), replacing capture
+ * resolution and init; tbg_continue then runs on from the saved ply. */
+int tbg_checkpoint(tbg_ctx *ctx, const char *root);
+int tbg_restore(tbg_ctx *ctx, const char *root);
+
+/* Runs the ply loop from the context's current state (after
+ * tbg_init_illegal_mate, tbg_restore or an earlier stop) until the fixpoint,
+ * or until ply `stop_at` is reached when stop_at > 0 (the state is then at
+ * that ply boundary and can be checkpointed or continued).  Lines of the
+ * plies run so far go to fn.  Continues the loop of src/tb8gen.c:1406-1444. */
+int tbg_continue(tbg_ctx *ctx, int form, int stop_at, tbg_line_fn fn, void *user,
+                 tbg_counts *counts);
+
+/* The reference's intermediate files of a finished generation (with
+ * codes): 0/wins, 0/L (mates) and, for every executed half-step, n/W and
+ * n/L per side and slice, derived on the device from the codes. */
+int tbg_export_plies(tbg_ctx *ctx, const char *root);
 
 /* Launch geometry of the ply kernels.  Results never depend on it; only
  * speed does.  tbg_autotune picks one by measurement; tbg_set_geometry with

@@ -32,7 +32,8 @@ ABI_SYMBOLS = [
     "tbg_set_profiling", "tbg_host_selftest", "tbg_set_scatter", "tbg_generate_batch",
     "tbg_emit_lines", "tbg_server_create", "tbg_server_destroy", "tbg_server_submit",
     "tbg_server_run", "tbg_server_result", "tbg_server_codes", "tbg_server_lines",
-    "tbg_set_geometry", "tbg_get_geometry", "tbg_autotune",
+    "tbg_set_geometry", "tbg_get_geometry", "tbg_autotune", "tbg_file_write", "tbg_file_read",
+    "tbg_checkpoint", "tbg_restore", "tbg_continue", "tbg_export_plies",
 ]
 SCATTER_GLOBAL, SCATTER_BLOCK, SCATTER_COMBINE = 0, 1, 2
 
@@ -148,6 +149,13 @@ def lib():
         L.tbg_get_geometry.argtypes = [vp, ctypes.POINTER(Geometry)]
         L.tbg_autotune.argtypes = [vp, ctypes.c_int, ctypes.c_char_p, ctypes.POINTER(Geometry),
                                    ctypes.POINTER(ctypes.c_double)]
+        L.tbg_file_write.argtypes = [ctypes.c_char_p, vp, ctypes.c_uint64]
+        L.tbg_file_read.argtypes = [ctypes.c_char_p, vp, ctypes.c_uint64]
+        L.tbg_checkpoint.argtypes = [vp, ctypes.c_char_p]
+        L.tbg_restore.argtypes = [vp, ctypes.c_char_p]
+        L.tbg_continue.argtypes = [vp, ctypes.c_int, ctypes.c_int, LINE_FN, vp,
+                                   ctypes.POINTER(Counts)]
+        L.tbg_export_plies.argtypes = [vp, ctypes.c_char_p]
         _lib = L
     return _lib
 
@@ -217,6 +225,29 @@ class Table:
         check(lib().tbg_generate(self.ptr, form, fn, None, ctypes.byref(self.counts)),
               f"generate {self.material}")
         return self.lines
+
+    def continue_(self, form=FORM_BATCHED, stop_at=0):
+        """Run the ply loop on from the current state (after init_illegal_mate,
+        restore or an earlier stop), to the fixpoint or to ply stop_at; returns
+        the count lines of the plies run so far."""
+        self.lines = []
+
+        def cb(line, _user):
+            self.lines.append(line.decode())
+
+        fn = LINE_FN(cb)
+        check(lib().tbg_continue(self.ptr, form, int(stop_at), fn, None,
+                                 ctypes.byref(self.counts)), f"continue {self.material}")
+        return self.lines
+
+    def checkpoint(self, root):
+        check(lib().tbg_checkpoint(self.ptr, str(root).encode()), f"checkpoint {self.material}")
+
+    def restore(self, root):
+        check(lib().tbg_restore(self.ptr, str(root).encode()), f"restore {self.material}")
+
+    def export_plies(self, root):
+        check(lib().tbg_export_plies(self.ptr, str(root).encode()), f"export {self.material}")
 
     def emit_lines(self):
         self.lines = []
@@ -299,6 +330,25 @@ class Table:
         t = Timing()
         check(lib().tbg_get_timing(self.ptr, ctypes.byref(t)), "timing")
         return t
+
+
+def file_write(path, data):
+    """Write bytes (a numpy array or bytes) as a slice file in the reference's
+    zstd frame format."""
+    import numpy as np
+    a = np.ascontiguousarray(np.frombuffer(bytes(data), dtype=np.uint8)
+                             if isinstance(data, (bytes, bytearray)) else data)
+    check(lib().tbg_file_write(str(path).encode(), a.ctypes.data_as(ctypes.c_void_p), a.nbytes),
+          f"write {path}")
+
+
+def file_read(path, nbytes):
+    """Read a slice file of nbytes decompressed bytes; returns a uint8 array."""
+    import numpy as np
+    out = np.zeros(nbytes, dtype=np.uint8)
+    check(lib().tbg_file_read(str(path).encode(), out.ctypes.data_as(ctypes.c_void_p), nbytes),
+          f"read {path}")
+    return out
 
 
 def default_geometry():

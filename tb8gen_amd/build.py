@@ -1,6 +1,7 @@
 """Builds the native parts of tb8gen_amd in-tree.
 
-  libtbg.so   -- HIP kernels + C ABI (include/tbg.h), gfx950 code objects
+  libtbg.so   -- HIP kernels + C ABI (include/tbg.h), gfx950 code objects, and the
+                 slice-file codec (zstd)
   tb8gen      -- the C host driver (tb8gen_main.c) linked against libtbg.so
 
 Run: python -m tb8gen_amd.build   (or __graft_entry__.build()).
@@ -32,6 +33,15 @@ def run(cmd):
     subprocess.run(cmd, check=True)
 
 
+# host-only sources (the slice-file codec), built with g++ against zstd: the
+# header from the conda tree, the library the system's (its stable API only;
+# linked by path so no other library directory enters the search)
+CXX_SOURCES = ["tbg_io.cpp"]
+ZSTD_INC = os.environ.get("TBG_ZSTD_INC", "/opt/conda/include")
+ZSTD_LIB = os.environ.get("TBG_ZSTD_LIB", "/usr/lib/x86_64-linux-gnu/libzstd.so.1")
+CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-I", ZSTD_INC]
+
+
 def build_lib():
     objs = []
     os.makedirs(os.path.join(PKG, "_obj"), exist_ok=True)
@@ -39,8 +49,12 @@ def build_lib():
         obj = os.path.join(PKG, "_obj", src.replace(".hip", ".o"))
         run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
+    for src in CXX_SOURCES:
+        obj = os.path.join(PKG, "_obj", src.replace(".cpp", ".o"))
+        run(["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+        objs.append(obj)
     tmp = LIB + ".tmp"
-    run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs])
+    run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs, ZSTD_LIB])
     os.replace(tmp, LIB)
 
 

@@ -9,7 +9,10 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <filesystem>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -836,6 +839,114 @@ int push_scatter(tbg_ctx *c)
 // C ABI
 // ====================================================================
 
+// slice-file codec (tbg_io.cpp)
+namespace tbg {
+int io_write(const char *path, const void *src, uint64_t bytes, int threads, std::string *err);
+int io_read(const char *path, void *dst, uint64_t bytes, int threads, std::string *err);
+}  // namespace tbg
+
+namespace {
+
+// <wk><bk> of slice s, as the reference names its slice files
+std::string slice_name(int s)
+{
+  char b[8];
+  int wk = g_htabs.kksq[s][0], bk = g_htabs.kksq[s][1];
+  snprintf(b, sizeof b, "%c%c%c%c", 'a' + (wk & 7), '1' + (wk >> 3), 'a' + (bk & 7),
+           '1' + (bk >> 3));
+  return b;
+}
+
+struct FileTask {
+  std::string path;
+  uint8_t *ptr;
+  uint64_t bytes;
+};
+
+// Writes (or reads) every task's file: a single large file gets the codec's
+// threads, many small ones are spread over a pool of file threads.
+int run_files(std::vector<FileTask> &tasks, bool write)
+{
+  std::atomic<size_t> next{ 0 };
+  std::mutex mu;
+  std::string err;
+  auto one = [&](const FileTask &t, int threads) {
+    std::string e;
+    int r = write ? io_write(t.path.c_str(), t.ptr, t.bytes, threads, &e)
+                  : io_read(t.path.c_str(), t.ptr, t.bytes, threads, &e);
+    if (r) {
+      std::lock_guard<std::mutex> g(mu);
+      if (err.empty()) err = e;
+    }
+  };
+  if (tasks.size() == 1) {
+    one(tasks[0], 16);
+  } else {
+    std::vector<std::thread> pool;
+    int nt = (int)std::min<size_t>(16, tasks.size());
+    for (int t = 0; t < nt; t++)
+      pool.emplace_back([&]() {
+        for (size_t i; (i = next++) < tasks.size();) one(tasks[i], 1);
+      });
+    for (auto &t : pool) t.join();
+  }
+  if (!err.empty()) return fail(TBG_E_IO, "%s", err.c_str());
+  return TBG_OK;
+}
+
+int make_dirs(const std::string &d)
+{
+  std::error_code ec;
+  std::filesystem::create_directories(d, ec);
+  if (ec) return fail(TBG_E_IO, "cannot create %s", d.c_str());
+  return TBG_OK;
+}
+
+// One plane ([slices][words] words, or [slices][n] of another element
+// size) between the device and per-slice files <dir>/<stm>/<slice>.
+int plane_files(tbg_ctx *c, void *dev, uint64_t slice_bytes, const std::string &dir, int stm,
+                bool write, std::vector<uint8_t> &buf)
+{
+  uint64_t ns = c->nsl, total = ns * slice_bytes;
+  std::string d = dir + "/" + "wb"[stm];
+  if (write) TRY(make_dirs(d));
+  buf.resize(total);
+  if (write) HIPCHK(hipMemcpy(buf.data(), dev, total, hipMemcpyDeviceToHost));
+  std::vector<FileTask> tasks;
+  for (uint64_t i = 0; i < ns; i++) {
+    int s = c->h.synth ? c->h.synth_slice : (int)i;
+    tasks.push_back({ d + "/" + slice_name(s), buf.data() + i * slice_bytes, slice_bytes });
+  }
+  TRY(run_files(tasks, write));
+  if (!write) HIPCHK(hipMemcpy(dev, buf.data(), total, hipMemcpyHostToDevice));
+  return TBG_OK;
+}
+
+const char *kSubNames[4] = { "loss", "bloss", "cwin", "win" };
+
+// the planes of a checkpoint, in a fixed order (both directions use it)
+int checkpoint_planes(tbg_ctx *c, const std::string &root, int n, bool write)
+{
+  std::vector<uint8_t> buf;
+  JobDev &J = c->hj;
+  uint64_t sb = c->h.words * 8;
+  for (int stm = 0; stm < 2; stm++) {
+    TRY(plane_files(c, J.wins[stm], sb, root + "/0/wins", stm, write, buf));
+    TRY(plane_files(c, J.prevW[stm], sb, root + "/" + std::to_string(n - 1) + "/W", stm, write, buf));
+    TRY(plane_files(c, J.prevL[stm], sb, root + "/" + std::to_string(n - 1) + "/L", stm, write, buf));
+    if (c->codes[stm])
+      TRY(plane_files(c, c->codes[stm], c->h.ii.size * 2, root + "/codes", stm, write, buf));
+  }
+  for (int col = 0; col < 2; col++)
+    if (c->h.ctotal[col])
+      for (int k = 0; k < 4; k++)
+        TRY(plane_files(c, c->sub[col][k], c->h.ctotal[col] * 8,
+                        root + "/sub/" + kSubNames[k], col, write, buf));
+  return TBG_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 int tbg_init(int device)
@@ -868,6 +979,7 @@ const char *tbg_strerror(int err)
   case TBG_E_STATE: return g_errbuf[0] ? g_errbuf : "call out of order";
   case TBG_E_BUDGET: return g_errbuf[0] ? g_errbuf : "device memory budget exceeded";
   case TBG_E_NODEV: return "no HIP device";
+  case TBG_E_IO: return g_errbuf[0] ? g_errbuf : "slice file error";
   default: return "unknown error";
   }
 }
@@ -1109,6 +1221,166 @@ int tbg_generate(tbg_ctx *c, int form, tbg_line_fn fn, void *user, tbg_counts *c
   TRY(tbg_generate_batch(one, 1, form, counts));
   if (fn) emit_lines(c, fn, user);
   return TBG_OK;
+}
+
+int tbg_file_write(const char *path, const void *data, uint64_t bytes)
+{
+  if (!path || (!data && bytes)) return fail(TBG_E_INVAL, "null argument");
+  std::string e;
+  if (io_write(path, data, bytes, 0, &e)) return fail(TBG_E_IO, "%s", e.c_str());
+  return TBG_OK;
+}
+
+int tbg_file_read(const char *path, void *data, uint64_t bytes)
+{
+  if (!path || (!data && bytes)) return fail(TBG_E_INVAL, "null argument");
+  std::string e;
+  if (io_read(path, data, bytes, 0, &e)) return fail(TBG_E_IO, "%s", e.c_str());
+  return TBG_OK;
+}
+
+int tbg_checkpoint(tbg_ctx *c, const char *root)
+{
+  if (!c || !root) return fail(TBG_E_INVAL, "null argument");
+  if (!c->inited) return fail(TBG_E_STATE, "nothing to checkpoint: init or restore first");
+  TRY(download_job(c));  // the current planes, ply and flags
+  JobDev &J = c->hj;
+  if (J.n < 2)
+    return fail(TBG_E_STATE, "checkpoint after ply 1 (a synthetic job's exits are in scratch planes at ply 1)");
+  std::string r(root);
+  TRY(make_dirs(r));
+  TRY(checkpoint_planes(c, r, J.n, true));
+  std::vector<long long> hist((size_t)4 * J.n);
+  HIPCHK(hipMemcpy(hist.data(), c->dhist, hist.size() * sizeof(long long), hipMemcpyDeviceToHost));
+  unsigned long long cnt[4];
+  HIPCHK(hipMemcpy(cnt, c->dcnt, sizeof cnt, hipMemcpyDeviceToHost));
+  std::string path = r + "/tbg_state";
+  FILE *F = fopen(path.c_str(), "w");
+  if (!F) return fail(TBG_E_IO, "cannot write %s", path.c_str());
+  fprintf(F, "tbg-checkpoint 1\nmaterial %s\nslice %d\nseed %llu\n", c->name,
+          c->h.synth ? c->h.synth_slice : -1, (unsigned long long)c->h.seed);
+  fprintf(F, "n %d\nmore %u\nmask %u\ndone %d\n", J.n, J.more, J.mask, J.done == 1 ? 1 : 0);
+  fprintf(F, "init %llu %llu %llu %llu\n", cnt[0], cnt[1], cnt[2], cnt[3]);
+  for (int n = 1; n < J.n; n++)
+    fprintf(F, "ply %d %lld %lld %lld %lld\n", n, hist[n * 4], hist[n * 4 + 1], hist[n * 4 + 2],
+            hist[n * 4 + 3]);
+  if (fclose(F) != 0) return fail(TBG_E_IO, "cannot write %s", path.c_str());
+  return TBG_OK;
+}
+
+int tbg_restore(tbg_ctx *c, const char *root)
+{
+  if (!c || !root) return fail(TBG_E_INVAL, "null argument");
+  std::string r(root);
+  std::string path = r + "/tbg_state";
+  FILE *F = fopen(path.c_str(), "r");
+  if (!F) return fail(TBG_E_IO, "cannot read %s", path.c_str());
+  char mat[32] = { 0 };
+  int ver = 0, slice = 0, n = 0, done = 0;
+  unsigned long long seed = 0, cnt[4] = { 0, 0, 0, 0 };
+  unsigned more = 0, mask = 0;
+  bool ok = fscanf(F, "tbg-checkpoint %d material %31s slice %d seed %llu n %d more %u mask %u "
+                      "done %d init %llu %llu %llu %llu",
+                   &ver, mat, &slice, &seed, &n, &more, &mask, &done, &cnt[0], &cnt[1], &cnt[2],
+                   &cnt[3]) == 12;
+  std::vector<long long> hist((size_t)4 * c->max_ply, -1);
+  int pn;
+  long long h[4];
+  while (ok && fscanf(F, " ply %d %lld %lld %lld %lld", &pn, &h[0], &h[1], &h[2], &h[3]) == 5) {
+    if (pn < 1 || pn >= c->max_ply) { ok = false; break; }
+    for (int i = 0; i < 4; i++) hist[(size_t)pn * 4 + i] = h[i];
+  }
+  fclose(F);
+  if (!ok || ver != 1 || n < 2 || n >= c->max_ply) return fail(TBG_E_IO, "%s: bad state file", path.c_str());
+  if (strcmp(mat, c->name) != 0 || slice != (c->h.synth ? c->h.synth_slice : -1) ||
+      (c->h.synth && seed != c->h.seed))
+    return fail(TBG_E_INVAL, "checkpoint of %s slice %d does not match this context", mat, slice);
+  HIPCHK(hipStreamSynchronize(c->st));
+  reset_job(c);  // the canonical planes: the files go to wins, prevW, prevL
+  JobDev &J = c->hj;
+  uint64_t pb = c->pw * 8;
+  for (int s = 0; s < 2; s++) {
+    HIPCHK(hipMemsetAsync(c->newW[s], 0, pb, c->st));
+    HIPCHK(hipMemsetAsync(c->newL[s], 0, pb, c->st));
+    HIPCHK(hipMemsetAsync(c->cand[s], 0, pb, c->st));
+  }
+  HIPCHK(hipStreamSynchronize(c->st));
+  TRY(checkpoint_planes(c, r, n, false));
+  J.n = n;
+  J.more = more;
+  J.mask = mask;
+  J.done = done ? 1 : 0;
+  HIPCHK(hipMemcpy(c->dhist, hist.data(), hist.size() * sizeof(long long), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->dcnt, cnt, sizeof cnt, hipMemcpyHostToDevice));
+  c->counts.broken[0] = cnt[0];
+  c->counts.broken[1] = cnt[1];
+  c->counts.mates[0] = cnt[2];
+  c->counts.mates[1] = cnt[3];
+  c->resolved = 1;
+  c->inited = 1;
+  c->generated = done ? 1 : 0;
+  TRY(upload_job(c));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return TBG_OK;
+}
+
+int tbg_continue(tbg_ctx *c, int form, int stop_at, tbg_line_fn fn, void *user, tbg_counts *counts)
+{
+  if (!c) return fail(TBG_E_INVAL, "null context");
+  if (form < TBG_FORM_PLAIN || form > TBG_FORM_GRAPH) return fail(TBG_E_INVAL, "bad form");
+  if (stop_at < 0) return fail(TBG_E_INVAL, "stop_at must be >= 0");
+  if (!c->inited) return fail(TBG_E_STATE, "init or restore first");
+  TRY(download_job(c));
+  JobDev &J = c->hj;
+  if (J.done != 1 && !(stop_at > 0 && J.n >= stop_at)) {
+    J.done = 0;
+    J.stop_at = stop_at;
+    TRY(upload_job(c));
+    unsigned f[4] = { 0, (J.n == 1 || J.n == DRAW_RULE + 1) ? 1u : 0u, 0, 0 };
+    HIPCHK(hipMemcpyAsync(c->dflags, f, sizeof f, hipMemcpyHostToDevice, c->st));
+    if (form == TBG_FORM_PLAIN) TRY(run_plain(c));
+    else TRY(run_batch(c, c->djp, 1, c->both, c->dflags, c->st, form == TBG_FORM_GRAPH, &c->graph_ms));
+  }
+  TRY(finish_job(c));
+  c->generated = c->hj.done == 1;
+  if (counts) *counts = c->counts;
+  if (fn) emit_lines(c, fn, user);
+  return TBG_OK;
+}
+
+int tbg_export_plies(tbg_ctx *c, const char *root)
+{
+  if (!c || !root) return fail(TBG_E_INVAL, "null argument");
+  if (!c->generated || !c->codes[0] || !c->codes[1])
+    return fail(TBG_E_STATE, "export needs a finished generation that keeps codes");
+  std::string r(root);
+  uint64_t *tmp = nullptr;
+  HIPCHK(hipMalloc(&tmp, c->pw * 8));
+  std::vector<uint8_t> buf;
+  uint64_t sb = c->h.words * 8;
+  int rc = TBG_OK;
+  auto code_plane = [&](int stm, uint16_t code) -> int {
+    hipLaunchKernelGGL(k_code_plane, dim3(cap_grid((c->pw + 3) / 4, (uint64_t)g_cus * 16)), dim3(256),
+                       0, c->st, c->codes[stm], c->h.ii.size, c->h.words, (int)c->nsl, code, tmp);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->st));
+    return TBG_OK;
+  };
+  for (int stm = 0; stm < 2 && rc == TBG_OK; stm++) {
+    rc = plane_files(c, c->hj.wins[stm], sb, r + "/0/wins", stm, true, buf);
+    if (rc == TBG_OK) rc = code_plane(stm, 2);  // mates: L_0
+    if (rc == TBG_OK) rc = plane_files(c, tmp, sb, r + "/0/L", stm, true, buf);
+    for (int n = 1; n <= c->counts.nplies && rc == TBG_OK; n++)
+      for (int kind = 0; kind < 2 && rc == TBG_OK; kind++) {
+        if (c->hist[(size_t)n * 4 + kind * 2 + stm] < 0) continue;  // the step did not run
+        rc = code_plane(stm, (uint16_t)((n << 2) | (kind == 0 ? 2 : 1)));
+        if (rc == TBG_OK)
+          rc = plane_files(c, tmp, sb, r + "/" + std::to_string(n) + (kind == 0 ? "/L" : "/W"), stm,
+                           true, buf);
+      }
+  }
+  (void)hipFree(tmp);
+  return rc;
 }
 
 int tbg_get_codes(tbg_ctx *c, int stm, int slice, uint16_t *dst, uint64_t n)

@@ -27,11 +27,12 @@ struct JobDev {
   uint16_t *codes[2];
   long long *hist;            // [max_ply][4] counts per step, -1: step did not run
   unsigned long long cnt[4];  // this ply: L(W), L(B), W(W), W(B)
-  int n, done;
+  int n, done;                // done: 1 finished, 2 paused at stop_at
   unsigned mask;              // steps of this ply: bit 0 L(W), 1 L(B), 2 W(W), 3 W(B)
   unsigned more;              // bit 0 more_ww, 1 more_wb, 2 more_lw, 3 more_lb
   int scatter;                // TBG_SCATTER_*
   int max_ply;
+  int stop_at;                // > 0: pause (done = 2) on reaching this ply
   uint64_t blk_pos, nblk, blk_words;
   int blk_set;                // pieces of sets >= blk_set move within a pred block
 };
@@ -93,6 +94,11 @@ __global__ void k_init(const Mat *Mp, uint64_t *winsW, uint64_t *winsB, uint64_t
                        unsigned long long *cnt);
 __global__ void k_resolve(const Mat *Mp, const SubRef *Rp, int c, int k, uint64_t *loss,
                           uint64_t *bloss, uint64_t *cwin, uint64_t *win, int s0, int ns);
+
+// bitmap of the positions whose code is `code`, one bit per position, for
+// `nslices` slices of `words` words (the reference's W_n / L_n files)
+__global__ void k_code_plane(const uint16_t *codes, uint64_t size, uint64_t words, int nslices,
+                             uint16_t code, uint64_t *out);
 
 // copies the shared tables to the device (defined with the kernels)
 hipError_t upload_tabs(const Tabs *h);

@@ -853,6 +853,26 @@ __device__ void ctrl_one(JobDev &J, unsigned *flags, unsigned *any_edge)
            ((forced || (more & 1)) ? 4u : 0u) | ((forced || (more & 2)) ? 8u : 0u);
   J.n = n;
   if (forced) atomicOr(any_edge, 1u);
+  if (J.stop_at > 0 && n >= J.stop_at) {  // a ply boundary the host asked to stop at
+    J.done = 2;
+    atomicAdd(&flags[0], 1u);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_code_plane(const uint16_t *codes, uint64_t size,
+                                                    uint64_t words, int nslices, uint16_t code,
+                                                    uint64_t *out)
+{
+  int lane = threadIdx.x & 63;
+  uint64_t nw = (uint64_t)nslices * words;
+  for (uint64_t g = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; g < nw;
+       g += ((uint64_t)gridDim.x * blockDim.x) >> 6) {
+    uint64_t s = g / words, w = g % words;
+    uint64_t idx = w * 64 + lane;
+    bool hit = idx < size && codes[s * size + idx] == code;
+    uint64_t b = __ballot(hit);
+    if (lane == 0) out[g] = b;
+  }
 }
 
 // Synthetic job, ply 1: does stm's king have a legal move or capture into

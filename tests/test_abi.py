@@ -35,7 +35,7 @@ def test_host_selftest_of_device_primitives():
 
 def test_error_codes_have_messages():
     L = tbg.lib()
-    for code in (0, -1, -2, -3, -4, -5, -6, -7):
+    for code in (0, -1, -2, -3, -4, -5, -6, -7, -8):
         assert L.tbg_strerror(code)
 
 

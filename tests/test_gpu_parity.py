@@ -380,3 +380,104 @@ def test_served_admission_under_small_budget():
         assert srv.lines(jid) == olines
         assert np.array_equal(srv.codes(jid, 0), o.codes(0)[0])
     srv.close()
+
+
+def _slice_file(s):
+    wk, bk = O.kksquare(s)
+    return f"{'abcdefgh'[wk & 7]}{wk // 8 + 1}{'abcdefgh'[bk & 7]}{bk // 8 + 1}"
+
+
+@pytest.mark.parametrize("stop", [2, 5, 101])
+def test_checkpoint_restore_continue(registry, tmp_path, stop):
+    """Stop a generation at a ply boundary, write it in the reference's
+    slice-file tree, restore it into a fresh context and continue: the table
+    and the count lines equal the uninterrupted generation's."""
+    material = "KRvKN"
+    registry.get(material)
+    subs = list(registry.tables.values())
+    print(f"case checkpoint {material} stop {stop}", flush=True)
+    t = tbg.Table(material)
+    t.resolve(subs)
+    t.init_illegal_mate()
+    t.continue_(tbg.FORM_BATCHED, stop_at=stop)
+    t.checkpoint(tmp_path)
+    state = (tmp_path / "tbg_state").read_text()
+    n = int([l for l in state.splitlines() if l.startswith("n ")][0].split()[1])
+    assert n >= stop
+    # the frontier files carry the reference's names and the plane's bytes
+    w = t.bitmap(tbg.PLANE_W, 0, 100).view(np.uint8)
+    got = tbg.file_read(tmp_path / str(n - 1) / "W" / "w" / _slice_file(100), w.nbytes)
+    assert np.array_equal(got, w)
+    lines_a = t.continue_(tbg.FORM_GRAPH)
+    t2 = tbg.Table(material)
+    t2.restore(tmp_path)
+    lines_b = t2.continue_(tbg.FORM_BATCHED)
+    assert lines_a == oracle_lines(material)
+    assert lines_b == oracle_lines(material)
+    assert_same_table(material, t)
+    assert_same_table(material, t2)
+    t.close()
+    t2.close()
+
+
+def test_checkpoint_synthetic_multi_block(tmp_path):
+    """A synthetic 6-piece slice stopped at ply 4, checkpointed, restored and
+    finished equals the uninterrupted run (lines and final wins)."""
+    material, s, seed = "KQRvKRN", 170, 5
+    print(f"case checkpoint synth {material}", flush=True)
+    ref = tbg.Table(material, synth_slice=s, seed=seed)
+    ref.resolve([])
+    ref_lines = ref.generate(tbg.FORM_BATCHED)
+    ref_wins = [ref.bitmap(tbg.PLANE_WINS, stm, s) for stm in range(2)]
+    ref.close()
+    t = tbg.Table(material, synth_slice=s, seed=seed)
+    t.resolve([])
+    t.init_illegal_mate()
+    t.continue_(tbg.FORM_BATCHED, stop_at=4)
+    t.checkpoint(tmp_path)
+    t.close()
+    t2 = tbg.Table(material, synth_slice=s, seed=seed)
+    t2.restore(tmp_path)
+    assert t2.continue_(tbg.FORM_BATCHED) == ref_lines
+    for stm in range(2):
+        assert np.array_equal(t2.bitmap(tbg.PLANE_WINS, stm, s), ref_wins[stm])
+    t2.close()
+    # a checkpoint of another seed does not restore into this job
+    t3 = tbg.Table(material, synth_slice=s, seed=seed + 1)
+    with pytest.raises(tbg.TbgError):
+        t3.restore(tmp_path)
+    t3.close()
+
+
+def test_export_plies_reference_files(registry, tmp_path):
+    """Every executed half-step's W_n / L_n slice files, 0/L (mates) and
+    0/wins of a finished table, in the reference's names and frame format,
+    equal the positions the oracle's codes give."""
+    material = "KRvK"
+    t = registry.get(material)
+    print(f"case export {material}", flush=True)
+    t.export_plies(tmp_path)
+    ora = oracle_table(material)
+    words = t.words
+    for stm in range(2):
+        codes = ora.codes(stm)
+        for s in (0, 37, 200, 440, 455):
+            name = _slice_file(s)
+
+            def bits(mask):  # bit idx & 7 of byte idx >> 3 (src/kslice.h:30-55)
+                b = np.zeros(words * 64, dtype=bool)
+                b[:codes.shape[1]] = mask
+                return np.packbits(b, bitorder="little")
+
+            c = codes[s]
+            got = tbg.file_read(tmp_path / "0" / "L" / "wb"[stm] / name, words * 8)
+            assert np.array_equal(got, bits(c == 2)), (stm, s, "L0")
+            got = tbg.file_read(tmp_path / "0" / "wins" / "wb"[stm] / name, words * 8)
+            assert np.array_equal(got, bits((c == 3) | ((c & 3) == 1))), (stm, s, "wins")
+            for n in range(1, t.counts.nplies + 1):
+                for kind, val in (("W", 1), ("L", 2)):
+                    f = tmp_path / str(n) / kind / "wb"[stm] / name
+                    if f.exists():
+                        got = tbg.file_read(f, words * 8)
+                        assert np.array_equal(got, bits(c == ((n << 2) | val))), (stm, s, n, kind)
+    assert (tmp_path / "1" / "L" / "w" / _slice_file(0)).exists()
