@@ -204,7 +204,8 @@ def run_served(args, tbg):
     job at a time, FIFO) on the same arrival stream; p50 / p99 latency
     (arrival -> codes on the host) and throughput."""
     import numpy as np
-    mats = pawnless_materials(4)
+    pieces = [int(x) for x in args.served_pieces.split(",")]
+    mats = sum((pawnless_materials(p - 2) for p in pieces), [])
     rng = np.random.default_rng(args.seed)
     jobs = [(mats[int(rng.integers(len(mats)))], int(rng.integers(0, 441)),
              int(rng.integers(1, 1 << 62))) for _ in range(args.jobs)]
@@ -248,12 +249,13 @@ def run_served(args, tbg):
         "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": cap["batched"]["wall_ms"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
         "data": "synthetic",
-        "config": {"workload": f"served: {len(jobs)} king-slice jobs of 6-piece pawnless materials "
+        "config": {"workload": f"served: {len(jobs)} king-slice jobs of "
+                               f"{'/'.join(map(str, pieces))}-piece pawnless materials "
                                f"with seeded hash exits; value = capacity (all jobs queued at "
                                f"once); latency at seeded Poisson arrivals of 70% of the batched "
                                f"capacity ({lam:.1f} jobs/s); {args.budget_gb:g} GB budget, "
                                f"{args.max_active} slots", "seed": args.seed,
-                   "lambda_jobs_per_s": lam, "budget_gb": args.budget_gb},
+                   "lambda_jobs_per_s": lam, "budget_gb": args.budget_gb, "pieces": pieces},
         "capacity": cap,
         "batched_over_plain": cap["batched"]["value"] / cap["plain"]["value"],
         "latency_ms": {"batched_p50": b["p50_ms"], "batched_p99": b["p99_ms"],
@@ -346,6 +348,8 @@ def main():
     ap.add_argument("--jobs", type=int, default=96, help="served: number of jobs")
     ap.add_argument("--budget-gb", type=float, default=64.0, help="served: HBM admission budget")
     ap.add_argument("--max-active", type=int, default=64, help="served: job slots")
+    ap.add_argument("--served-pieces", default="6",
+                    help="served: piece counts of the job materials, e.g. 5,6")
     ap.add_argument("--stream-plain", action="store_true",
                     help="stream workload: also time the 4-piece level in the plain form")
     ap.add_argument("--seed", type=int, default=20261203)

@@ -201,6 +201,23 @@ int tbg_get_sub_bitmap(tbg_ctx *ctx, int kind, int colour, int slice, uint64_t *
 #define TBG_SCATTER_COMBINE 2
 int tbg_set_scatter(tbg_ctx *ctx, int mode);
 
+/* Material scheduler: generates a material and every smaller material its
+ * captures reach, smallest first, each piece count of the dependency graph
+ * as one batch (tbg_generate_batch), and keeps the tables resident for the
+ * materials asked for later.  Smaller materials are generated in one colour
+ * orientation (the side with more, then stronger, pieces white);
+ * tbg_resolve_captures maps colour-swapped names onto them (probe.c:
+ * 1554-1565).  This replaces the pre-built Syzygy files the reference's
+ * probe_wdl reads for capture exits (probe.c:83-97, 1522-1684).  The
+ * registry owns the contexts it returns. */
+typedef struct tbg_registry tbg_registry;
+int tbg_registry_create(tbg_registry **out, int form);
+int tbg_registry_get(tbg_registry *reg, const char *material, tbg_ctx **out);
+int tbg_registry_count(const tbg_registry *reg);  /* materials resident */
+/* the i-th resident material, in generation order (smallest first) */
+int tbg_registry_entry(const tbg_registry *reg, int i, const char **name, tbg_ctx **ctx);
+void tbg_registry_destroy(tbg_registry *reg);
+
 /* Slice files in the reference's format (src/util.c:179-343): frames of
  * {compressed size, size, offset} (three little-endian uint64) followed by
  * the zstd level-6 compression of at most 50 KiB of the slice's bytes, in

@@ -34,6 +34,8 @@ ABI_SYMBOLS = [
     "tbg_server_run", "tbg_server_result", "tbg_server_codes", "tbg_server_lines",
     "tbg_set_geometry", "tbg_get_geometry", "tbg_autotune", "tbg_file_write", "tbg_file_read",
     "tbg_checkpoint", "tbg_restore", "tbg_continue", "tbg_export_plies",
+    "tbg_registry_create", "tbg_registry_get", "tbg_registry_count", "tbg_registry_entry",
+    "tbg_registry_destroy",
 ]
 SCATTER_GLOBAL, SCATTER_BLOCK, SCATTER_COMBINE = 0, 1, 2
 
@@ -156,6 +158,11 @@ def lib():
         L.tbg_continue.argtypes = [vp, ctypes.c_int, ctypes.c_int, LINE_FN, vp,
                                    ctypes.POINTER(Counts)]
         L.tbg_export_plies.argtypes = [vp, ctypes.c_char_p]
+        L.tbg_registry_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int]
+        L.tbg_registry_get.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(vp)]
+        L.tbg_registry_count.argtypes = [vp]
+        L.tbg_registry_destroy.argtypes = [vp]
+        L.tbg_registry_destroy.restype = None
         _lib = L
     return _lib
 
@@ -182,11 +189,16 @@ def host_selftest():
 class Table:
     """Device-resident state of one material (a tbg_ctx)."""
 
-    def __init__(self, material, synth_slice=None, seed=0):
+    def __init__(self, material, synth_slice=None, seed=0, ptr=None):
+        """ptr: wrap a context owned elsewhere (a NativeRegistry's); close()
+        then leaves it alone."""
         init()
         self.material = material
-        self.ptr = ctypes.c_void_p()
-        if synth_slice is None:
+        self.owned = ptr is None
+        self.ptr = ctypes.c_void_p(ptr) if ptr is not None else ctypes.c_void_p()
+        if ptr is not None:
+            pass
+        elif synth_slice is None:
             check(lib().tbg_create(ctypes.byref(self.ptr), material.encode()), material)
         else:
             check(lib().tbg_create_synth(ctypes.byref(self.ptr), material.encode(),
@@ -201,9 +213,9 @@ class Table:
         self.words = info.slice_words
 
     def close(self):
-        if self.ptr:
+        if self.ptr and self.owned:
             lib().tbg_destroy(self.ptr)
-            self.ptr = ctypes.c_void_p()
+        self.ptr = ctypes.c_void_p()
 
     def __del__(self):
         try:
@@ -462,6 +474,32 @@ def sub_materials(material):
                 if c not in out:
                     out.append(c)
     return out
+
+
+class NativeRegistry:
+    """The library's material scheduler (tbg_registry_*): a material and every
+    smaller material its captures reach, generated smallest first, one batch
+    per piece count, kept resident."""
+
+    def __init__(self, form=FORM_GRAPH):
+        init()
+        self.ptr = ctypes.c_void_p()
+        check(lib().tbg_registry_create(ctypes.byref(self.ptr), form), "registry")
+
+    def get(self, material):
+        c = ctypes.c_void_p()
+        check(lib().tbg_registry_get(self.ptr, material.encode(), ctypes.byref(c)), material)
+        t = Table(material, ptr=c.value)
+        t.emit_lines()
+        return t
+
+    def count(self):
+        return lib().tbg_registry_count(self.ptr)
+
+    def close(self):
+        if self.ptr:
+            lib().tbg_registry_destroy(self.ptr)
+            self.ptr = ctypes.c_void_p()
 
 
 class Registry:

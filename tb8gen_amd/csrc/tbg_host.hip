@@ -12,6 +12,7 @@
 #include <atomic>
 #include <chrono>
 #include <filesystem>
+#include <functional>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -1221,6 +1222,159 @@ int tbg_generate(tbg_ctx *c, int form, tbg_line_fn fn, void *user, tbg_counts *c
   TRY(tbg_generate_batch(one, 1, form, counts));
   if (fn) emit_lines(c, fn, user);
   return TBG_OK;
+}
+
+}  // extern "C"
+
+struct tbg_registry {
+  int form = TBG_FORM_GRAPH;
+  std::vector<std::pair<std::string, tbg_ctx *>> tables;  // in generation order
+};
+
+namespace {
+
+// colour orientation of generated smaller materials: the side with more
+// pieces, then the stronger piece string (K Q R B N), is white
+int piece_rank(char ch)
+{
+  const char *o = "KQRBN";
+  const char *p = strchr(o, ch);
+  return p ? (int)(p - o) : 99;
+}
+
+std::string sorted_side(std::string s)
+{
+  std::stable_sort(s.begin(), s.end(), [](char a, char b) { return piece_rank(a) < piece_rank(b); });
+  return s;
+}
+
+bool side_ge(const std::string &a, const std::string &b)
+{
+  if (a.size() != b.size()) return a.size() > b.size();
+  for (size_t i = 0; i < a.size(); i++)
+    if (a[i] != b[i]) return piece_rank(a[i]) < piece_rank(b[i]);
+  return true;
+}
+
+std::string canonical_material(const std::string &m)
+{
+  size_t v = m.find('v');
+  std::string w = sorted_side(m.substr(0, v)), b = sorted_side(m.substr(v + 1));
+  return side_ge(w, b) ? w + "v" + b : b + "v" + w;
+}
+
+// materials one capture away, without a bare pair of kings
+std::vector<std::string> sub_material_names(const std::string &m)
+{
+  std::vector<std::string> out;
+  size_t v = m.find('v');
+  if (v == std::string::npos) return out;
+  std::string w = m.substr(0, v), b = m.substr(v + 1);
+  for (int white = 1; white >= 0; white--) {
+    const std::string &side = white ? w : b, &other = white ? b : w;
+    for (size_t i = 0; i < side.size(); i++) {
+      if (side[i] == 'K') continue;
+      std::string rest = side.substr(0, i) + side.substr(i + 1);
+      std::string sub = white ? rest + "v" + other : other + "v" + rest;
+      if (sub.size() - 1 <= 2) continue;
+      std::string c = canonical_material(sub);
+      if (std::find(out.begin(), out.end(), c) == out.end()) out.push_back(c);
+    }
+  }
+  return out;
+}
+
+tbg_ctx *registry_find(tbg_registry *r, const std::string &m)
+{
+  for (auto &e : r->tables)
+    if (e.first == m) return e.second;
+  return nullptr;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tbg_registry_create(tbg_registry **out, int form)
+{
+  if (!out) return fail(TBG_E_INVAL, "null argument");
+  if (form < TBG_FORM_PLAIN || form > TBG_FORM_GRAPH) return fail(TBG_E_INVAL, "bad form");
+  *out = new tbg_registry();
+  (*out)->form = form;
+  return TBG_OK;
+}
+
+int tbg_registry_get(tbg_registry *r, const char *material, tbg_ctx **out)
+{
+  if (!r || !material || !out) return fail(TBG_E_INVAL, "null argument");
+  *out = nullptr;
+  std::string want(material);
+  if (tbg_ctx *c = registry_find(r, want)) {
+    *out = c;
+    return TBG_OK;
+  }
+  // post-order walk of the dependency graph: every smaller material first
+  std::vector<std::string> need;
+  std::vector<std::string> stack;
+  std::function<int(const std::string &)> visit = [&](const std::string &m) -> int {
+    if (registry_find(r, m) || std::find(need.begin(), need.end(), m) != need.end()) return TBG_OK;
+    if (std::find(stack.begin(), stack.end(), m) != stack.end()) return TBG_OK;
+    stack.push_back(m);
+    for (const std::string &sub : sub_material_names(m)) TRY(visit(sub));
+    stack.pop_back();
+    need.push_back(m);
+    return TBG_OK;
+  };
+  TRY(visit(want));
+  // one batch per piece count, smallest first
+  size_t lo = 99, hi = 0;
+  for (auto &m : need) lo = std::min(lo, m.size()), hi = std::max(hi, m.size());
+  for (size_t len = lo; len <= hi; len++) {
+    std::vector<tbg_ctx *> batch;
+    std::vector<std::string> names;
+    int rc = TBG_OK;
+    for (auto &m : need) {
+      if (m.size() != len) continue;
+      tbg_ctx *c = nullptr;
+      rc = tbg_create(&c, m.c_str());
+      if (rc != TBG_OK) break;
+      batch.push_back(c);
+      names.push_back(m);
+    }
+    std::vector<tbg_ctx *> subs;
+    for (auto &e : r->tables) subs.push_back(e.second);
+    for (size_t i = 0; i < batch.size() && rc == TBG_OK; i++)
+      rc = tbg_resolve_captures(batch[i], subs.data(), (int)subs.size());
+    if (rc == TBG_OK && !batch.empty())
+      rc = tbg_generate_batch(batch.data(), (int)batch.size(), r->form, nullptr);
+    if (rc != TBG_OK) {
+      for (tbg_ctx *c : batch) tbg_destroy(c);
+      return rc;
+    }
+    for (size_t i = 0; i < batch.size(); i++) r->tables.push_back({ names[i], batch[i] });
+  }
+  *out = registry_find(r, want);
+  return *out ? TBG_OK : fail(TBG_E_STATE, "%s was not generated", material);
+}
+
+int tbg_registry_count(const tbg_registry *r)
+{
+  return r ? (int)r->tables.size() : 0;
+}
+
+int tbg_registry_entry(const tbg_registry *r, int i, const char **name, tbg_ctx **ctx)
+{
+  if (!r || i < 0 || i >= (int)r->tables.size()) return fail(TBG_E_INVAL, "no such registry entry");
+  if (name) *name = r->tables[i].first.c_str();
+  if (ctx) *ctx = r->tables[i].second;
+  return TBG_OK;
+}
+
+void tbg_registry_destroy(tbg_registry *r)
+{
+  if (!r) return;
+  for (auto &e : r->tables) tbg_destroy(e.second);
+  delete r;
 }
 
 int tbg_file_write(const char *path, const void *data, uint64_t bytes)

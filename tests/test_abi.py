@@ -60,6 +60,16 @@ def test_geometry_is_validated_on_the_host():
     assert L.tbg_autotune(None, 0, None, None, None) == -1
 
 
+def test_registry_arguments_are_checked():
+    L = tbg.lib()
+    r = ctypes.c_void_p()
+    assert L.tbg_registry_create(None, 2) == -1
+    assert L.tbg_registry_create(ctypes.byref(r), 9) == -1
+    assert L.tbg_registry_count(None) == 0
+    L.tbg_registry_destroy(None)
+    assert L.tbg_registry_get(None, b"KRvK", None) == -1
+
+
 def test_calls_before_init_are_rejected():
     L = tbg.lib()
     if tbg._inited:

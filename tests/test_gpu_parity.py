@@ -481,3 +481,42 @@ def test_export_plies_reference_files(registry, tmp_path):
                         got = tbg.file_read(f, words * 8)
                         assert np.array_equal(got, bits(c == ((n << 2) | val))), (stm, s, n, kind)
     assert (tmp_path / "1" / "L" / "w" / _slice_file(0)).exists()
+
+
+def test_native_registry_schedules_dependencies():
+    """The library's material scheduler builds every smaller material a
+    table's captures reach, one batch per piece count, and the result equals
+    the oracle: 3/4-piece tables live, KQRvKR through its digest."""
+    print("case native registry", flush=True)
+    reg = tbg.NativeRegistry(tbg.FORM_GRAPH)
+    t = reg.get("KQvKR")
+    assert reg.count() == 3  # KQvK, KRvK (for KvKR), KQvKR
+    assert t.lines == oracle_lines("KQvKR")
+    assert_same_table("KQvKR", t)
+    t = reg.get("KRRvK")
+    assert reg.count() == 4
+    assert t.lines == oracle_lines("KRRvK")
+    with open(os.path.join(os.path.dirname(__file__), "golden", "oracle_KQRvKR.json")) as f:
+        ref = json.load(f)
+    t = reg.get("KQRvKR")  # adds KRvKR and KQRvK
+    assert reg.count() == 7
+    assert t.lines == ref["lines"]
+    for stm in range(2):
+        c = t.codes(stm)
+        crc = [zlib.crc32(c[s].tobytes()) for s in range(c.shape[0])]
+        assert crc == ref["crc32"][str(stm)]
+    assert reg.get("KQvKR").ptr.value is not None and reg.count() == 7  # resident, not rebuilt
+    reg.close()
+
+
+@pytest.mark.parametrize("form", ["graph", "plain"])
+def test_c_driver_prints_the_reference_lines(form):
+    """The C driver (the reference's command line over libtbg) prints the
+    oracle's count lines for a 4-piece material, its smaller materials
+    generated by the library's scheduler."""
+    import subprocess
+    drv = os.path.join(os.path.dirname(tbg.__file__), "tb8gen")
+    print(f"case driver {form}", flush=True)
+    r = subprocess.run([drv, "-f", form, "KRvKN"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert [l for l in r.stdout.splitlines() if l.strip()] == oracle_lines("KRvKN")
