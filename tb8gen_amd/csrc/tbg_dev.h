@@ -284,6 +284,23 @@ TBG_HD uint64_t moved_idx(const Idx &ii, uint64_t idx, uint64_t p, int kj, int a
   return a < b ? r + acc : r - acc;
 }
 
+// Index after a king moves from `from` to `to` when the position needs no
+// symmetry (normalize returned it unchanged), from the index before: the
+// kings precede every set, so each piece strictly between the two squares
+// shifts its rank by one and nothing else changes.  All sets single; p is
+// the position before or after the move (the pieces are the same).
+TBG_HD uint64_t king_moved_idx(const Idx &ii, uint64_t idx, uint64_t p, int from, int to)
+{
+  int lo = from < to ? from : to, hi = from < to ? to : from;
+  uint32_t acc = 0;  // at most the sum of the factors: below 2^30
+  for (int k = 0; k < MAXS; k++) {
+    if (k >= ii.numsets) break;
+    int s = sq_get(p, 2 + k);
+    acc += (s > lo && s < hi) ? ii.factor[k] : 0u;
+  }
+  return from < to ? idx + acc : idx - acc;
+}
+
 // kings + pieces of the sets before kj (all sets single)
 TBG_HD uint64_t occ_before(uint64_t p, int kj)
 {

@@ -1972,6 +1972,34 @@ extern "C" int tbg_host_selftest(char *msg, int len)
       }
     }
   }
+  // a king move that needs no symmetry: incremental index equals re-indexing
+  {
+    const char *km[] = { "KQRvKR", "KQRBvKQRN", "KRvKN" };
+    for (const char *nm : km) {
+      Mat M;
+      if (setup_mat(M, nm) != TBG_OK || !M.all_single) ST_FAIL("setup %s", nm);
+      int tested = 0;
+      for (int it = 0; it < 20000; it++) {
+        int sl = (int)(rnd() % NSL);
+        uint64_t idx = rnd() % M.ii.size;
+        uint64_t p = (uint64_t)T.kksq[sl][0] | ((uint64_t)T.kksq[sl][1] << 8);
+        uint64_t occ = idx_to_sq(T, M.ii, idx, &p);
+        int stm = (int)(rnd() & 1);
+        int from = sq_get(p, stm);
+        uint64_t b = T.king[from] & ~occ & ~T.king[sq_get(p, stm ^ 1)];
+        while (b) {
+          int to = ctz64(b);
+          b &= b - 1;
+          uint64_t q0 = sq_set(p, stm, to), q;
+          if (normalize(T, q0, &q) < 0 || q != q0) continue;
+          tested++;
+          if (king_moved_idx(M.ii, idx, q0, from, to) != sq_to_idx(T, M.ii, q0))
+            ST_FAIL("%s: king_moved_idx", nm);
+        }
+      }
+      if (tested < 1000) ST_FAIL("%s: too few identity king moves (%d)", nm, tested);
+    }
+  }
   // pred blocks: a move of a piece of a set >= blk_set stays in its block
   {
     const char *bm[] = { "KQRvKRN", "KQRBvKQRN", "KRRvKQ", "KQQvKRR", "KBBNvKN" };

@@ -97,10 +97,16 @@ __device__ __forceinline__ void mark4(uint64_t *plane, const uint64_t *__restric
 // W_n or L_n positions (W_n = C & ~wins; a won position fails verification),
 // so they are tested with a cached read before the memory-side atomic.
 // The targets of a position go out in groups of 4 (mark4).
+// idx: the position's index in slice s, or ~0 when unknown (an uncapture's
+// constructed position); with it, a king move that needs no symmetry is
+// indexed incrementally (king_moved_idx).
 __device__ void mark_king_unmoves(const Tabs &T, const Mat &M, int stm, uint64_t occ, uint64_t p,
-                                  int s, uint64_t *cand, const uint64_t *__restrict__ won)
+                                  int s, uint64_t *cand, const uint64_t *__restrict__ won,
+                                  uint64_t idx)
 {
-  uint64_t b = T.king[sq_get(p, stm)] & ~(occ | T.king[sq_get(p, stm ^ 1)]);
+  int from = sq_get(p, stm);
+  bool inc = M.all_single && idx != ~0ull;
+  uint64_t b = T.king[from] & ~(occ | T.king[sq_get(p, stm ^ 1)]);
   bool dual = false;  // entering a diagonal slice from another: also the mirror
   uint64_t dq = 0;
   int ds = 0;
@@ -115,10 +121,11 @@ __device__ void mark_king_unmoves(const Tabs &T, const Mat &M, int stm, uint64_t
       } else if (b) {
         int t = ctz64(b);
         b &= b - 1;
-        uint64_t q;
-        int s2 = normalize(T, sq_set(p, stm, t), &q);
+        uint64_t q0 = sq_set(p, stm, t), q;
+        int s2 = normalize(T, q0, &q);
         if (M.synth && s2 != M.synth_slice) continue;  // leaves a single-slice job
-        g[k] = phys(M, s2) * M.words * 64 + sq_to_idx(T, M.ii, q);
+        g[k] = phys(M, s2) * M.words * 64 +
+               (inc && q == q0 ? king_moved_idx(M.ii, idx, q0, from, t) : sq_to_idx(T, M.ii, q));
         if (s < 441 && s2 >= 441) {
           dual = true;
           dq = q;
@@ -160,9 +167,10 @@ __device__ __forceinline__ void wave_mark(uint64_t *plane, const uint64_t *__res
 // lanes' rounds line up).  ok: the lane holds a position.
 __device__ void king_unmoves_wave(const Tabs &T, const Mat &M, int stm, bool ok, uint64_t occ,
                                   uint64_t p, int s, uint64_t *cand,
-                                  const uint64_t *__restrict__ won, int lane)
+                                  const uint64_t *__restrict__ won, int lane, uint64_t idx)
 {
-  uint64_t b = ok ? T.king[sq_get(p, stm)] & ~(occ | T.king[sq_get(p, stm ^ 1)]) : 0;
+  int from = sq_get(p, stm);
+  uint64_t b = ok ? T.king[from] & ~(occ | T.king[sq_get(p, stm ^ 1)]) : 0;
   bool dual = false;
   uint64_t dq = 0;
   int ds = 0;
@@ -174,10 +182,12 @@ __device__ void king_unmoves_wave(const Tabs &T, const Mat &M, int stm, bool ok,
     } else if (b) {
       int t = ctz64(b);
       b &= b - 1;
-      uint64_t q;
-      int s2 = normalize(T, sq_set(p, stm, t), &q);
+      uint64_t q0 = sq_set(p, stm, t), q;
+      int s2 = normalize(T, q0, &q);
       if (!M.synth || s2 == M.synth_slice) {  // else it leaves a single-slice job
-        g = phys(M, s2) * M.words * 64 + sq_to_idx(T, M.ii, q);
+        g = phys(M, s2) * M.words * 64 +
+            (M.all_single && q == q0 ? king_moved_idx(M.ii, idx, q0, from, t)
+                                     : sq_to_idx(T, M.ii, q));
         if (s < 441 && s2 >= 441) {
           dual = true;
           dq = q;
@@ -367,8 +377,8 @@ __global__ __launch_bounds__(512) void k_pred_j(JobDev *const *__restrict__ jobs
             idx = (wlo + c0 + (v >> 6)) * 64 + (v & 63);
             occ = idx_to_sq(T, M.ii, idx, &p);
           }
-          if (comb) king_unmoves_wave(T, M, stm, ok, occ, p, s, cand, won, lane);
-          else if (ok) mark_king_unmoves(T, M, stm, occ, p, s, cand, won);
+          if (comb) king_unmoves_wave(T, M, stm, ok, occ, p, s, cand, won, lane, idx);
+          else if (ok) mark_king_unmoves(T, M, stm, occ, p, s, cand, won, idx);
           for (int t = 0; t < MAXP - 1; t++) {
             if (t >= M.nk[stm]) break;
             int j = M.pcs[stm][t];
@@ -507,7 +517,7 @@ __global__ __launch_bounds__(256) void k_uncap_j(JobDev *const *__restrict__ job
         uint64_t occ = idx_to_sq(T, ci, idx, &p);
         if (legality && attacked_by(T, M, p, occ, sq_get(p, stm), o, m)) continue;
         // uncapture by the king
-        mark_king_unmoves(T, M, stm, occ, sq_set(p, m, sq_get(p, stm)), s, cand, won);
+        mark_king_unmoves(T, M, stm, occ, sq_set(p, m, sq_get(p, stm)), s, cand, won, ~0ull);
         // uncapture by the non-king pieces
         for (int t = 0; t < MAXP - 1; t++) {
           if (t >= M.nk[stm]) break;
@@ -524,7 +534,7 @@ __global__ __launch_bounds__(256) void k_uncap_j(JobDev *const *__restrict__ job
 // (position q0 not normalised, q normalised into slice s2); captured slot
 // jj of set l, or jj < 0.  3 = illegal move.
 __device__ int synth_exit(const Tabs &T, const Mat &M, int stm, uint64_t q0, uint64_t q, uint64_t occ,
-                          int from, int jj, int l, int s2)
+                          int from, int jj, int l, int s2, uint64_t idx)
 {
   int o = stm ^ 1, to = sq_get(q0, stm);
   if (jj >= 0) {
@@ -533,7 +543,9 @@ __device__ int synth_exit(const Tabs &T, const Mat &M, int stm, uint64_t q0, uin
     return synth_value(synth_hash(M.seedmix, o, 2, s2 * 8 + l, sq_to_idx(T, M.capt[l], c)));
   }
   if (attacked_by(T, M, q0, occ ^ bit(from) ^ bit(to), to, o, -1)) return 3;
-  return synth_value(synth_hash(M.seedmix, o, 1, s2, sq_to_idx(T, M.ii, q)));
+  uint64_t i2 = M.all_single && q == q0 ? king_moved_idx(M.ii, idx, q0, from, to)
+                                        : sq_to_idx(T, M.ii, q);
+  return synth_value(synth_hash(M.seedmix, o, 1, s2, i2));
 }
 
 // check_successors_worker + check_moves + check_king_moves
@@ -598,7 +610,7 @@ __device__ bool verify_pos(const Tabs &T, const Mat &M, int stm, int s, uint64_t
     uint64_t q0 = sq_set(p, stm, to), q;
     int s2 = normalize(T, q0, &q);
     if (M.synth && s2 != M.synth_slice) {
-      int v = synth_exit(T, M, stm, q0, q, occ, ks, jj, l, s2);
+      int v = synth_exit(T, M, stm, q0, q, occ, ks, jj, l, s2, idx);
       if (v == 3 || v == 2) continue;
       return false;
     }
@@ -613,11 +625,13 @@ __device__ bool verify_pos(const Tabs &T, const Mat &M, int stm, int s, uint64_t
     uint64_t q0 = sq_set(p, stm, to), q;
     int s2 = normalize(T, q0, &q);
     if (M.synth && s2 != M.synth_slice) {
-      int v = synth_exit(T, M, stm, q0, q, occ, ks, -1, 0, s2);
+      int v = synth_exit(T, M, stm, q0, q, occ, ks, -1, 0, s2, idx);
       if (v == 3 || v == 2) continue;
       return false;
     }
-    if (!bm_test(wins_o + phys(M, s2) * M.words, sq_to_idx(T, M.ii, q))) return false;
+    uint64_t i2 = M.all_single && q == q0 ? king_moved_idx(M.ii, idx, q0, ks, to)
+                                          : sq_to_idx(T, M.ii, q);
+    if (!bm_test(wins_o + phys(M, s2) * M.words, i2)) return false;
   }
   return true;
 }
@@ -881,7 +895,7 @@ __global__ __launch_bounds__(256) void k_code_plane(const uint16_t *codes, uint6
 // The opponent's attack sets are computed once with stm's king lifted, so
 // each king move's legality is a bit test.
 __device__ void synth_exits_pos(const Tabs &T, const Mat &M, uint64_t p, uint64_t occ, int stm,
-                                bool *w, bool *l)
+                                uint64_t idx, bool *w, bool *l)
 {
   int o = stm ^ 1, s = M.synth_slice, from = sq_get(p, stm), oks = sq_get(p, o);
   uint64_t occk = occ ^ bit(from);
@@ -922,7 +936,9 @@ __device__ void synth_exits_pos(const Tabs &T, const Mat &M, uint64_t p, uint64_
       uint64_t c = sq_set(q, jj, sq_get(q, M.ii.last[l]));
       v = synth_value(synth_hash(M.seedmix, o, 2, s2 * 8 + l, sq_to_idx(T, M.capt[l], c)));
     } else {
-      v = synth_value(synth_hash(M.seedmix, o, 1, s2, sq_to_idx(T, M.ii, q)));
+      uint64_t i2 = M.all_single && q == q0 ? king_moved_idx(M.ii, idx, q0, from, to)
+                                            : sq_to_idx(T, M.ii, q);
+      v = synth_value(synth_hash(M.seedmix, o, 1, s2, i2));
     }
     ww |= v == -2;
     ll |= v == 2;
@@ -1045,8 +1061,8 @@ __global__ __launch_bounds__(256) void k_init(const Mat *__restrict__ Mp, uint64
       // an illegal position is in wins(stm) from the start, which masks its
       // exit candidates in both the W sweep and verification
       if (M.synth) {
-        if (!wi) synth_exits_pos(T, M, p, occ, WHITE, &xw0, &xl0);
-        if (!bi) synth_exits_pos(T, M, p, occ, BLACK, &xw1, &xl1);
+        if (!wi) synth_exits_pos(T, M, p, occ, WHITE, idx, &xw0, &xl0);
+        if (!bi) synth_exits_pos(T, M, p, occ, BLACK, idx, &xw1, &xl1);
       }
       if (wi != bi) {
         int c = bi ? WHITE : BLACK;
