@@ -889,48 +889,55 @@ __global__ __launch_bounds__(256) void k_code_plane(const uint16_t *codes, uint6
   }
 }
 
+// Attacks of colour c's non-king pieces on occupancy `o` (the caller lifts
+// the enemy king from it, so rays pass through the square that king
+// leaves), without the piece in slot `skip` (-1: none).
+__device__ __forceinline__ uint64_t side_attacks(const Tabs &T, const Mat &M, uint64_t p,
+                                                 uint64_t o, int c, int skip)
+{
+  uint64_t a = 0;
+  for (int t = 0; t < MAXP - 1; t++) {
+    if (t >= M.nk[c]) break;
+    int j = M.pcs[c][t];
+    if (j == skip) continue;
+    a |= piece_att(T, M.pt[j], sq_get(p, j), o);
+  }
+  return a;
+}
+
 // Synthetic job, ply 1: does stm's king have a legal move or capture into
 // an exit (a position of another slice, valued by the seeded hash) that is
 // a loss (*w: W_1 candidate) or a win (*l: L_1 candidate) for the opponent?
-// The opponent's attack sets are computed once with stm's king lifted, so
-// each king move's legality is a bit test.
+// uo: the opponent's piece attacks with stm's king lifted, so a quiet king
+// move's legality is a bit test; a capture recomputes them without the
+// captured piece.
 __device__ void synth_exits_pos(const Tabs &T, const Mat &M, uint64_t p, uint64_t occ, int stm,
-                                uint64_t idx, bool *w, bool *l)
+                                uint64_t idx, uint64_t uo, bool *w, bool *l)
 {
   int o = stm ^ 1, s = M.synth_slice, from = sq_get(p, stm), oks = sq_get(p, o);
-  uint64_t occk = occ ^ bit(from);
-  uint64_t att[MAXS];
   uint64_t own = bit(from);
-#pragma unroll
-  for (int t = 0; t < MAXS; t++) {
-    att[t] = 0;
-    if (t < M.nk[o]) {
-      int j = M.pcs[o][t];
-      att[t] = piece_att(T, M.pt[j], sq_get(p, j), occk);
-    }
-    if (t < M.nk[stm]) own |= bit(sq_get(p, M.pcs[stm][t]));
+  for (int t = 0; t < MAXP - 1; t++) {
+    if (t >= M.nk[stm]) break;
+    own |= bit(sq_get(p, M.pcs[stm][t]));
   }
-  uint64_t so[MAXS];
-  set_occ(M.ii, p, so);
   uint64_t b = T.king[from] & ~T.king[oks] & ~own;
+  uint64_t unsafe = T.king[oks] | uo;
   bool ww = false, ll = false;
   while (b) {
     int to = ctz64(b);
     b &= b - 1;
     int jj = -1, l = 0;
     if (occ & bit(to)) {
-      for (int x = 0; x < MAXS; x++)
-        if (x < M.numsets && (so[x] & bit(to))) l = x;
-      jj = slot_of(p, to, M.setmask80[l]);
+      jj = slot_of(p, to, M.col80[o]);
+      l = sq_get(M.set_of, jj);
+      uint64_t a = T.king[oks] | side_attacks(T, M, p, occ ^ bit(from), o, jj);
+      if (a & bit(to)) continue;  // the king would stand in check
+    } else if (unsafe & bit(to)) {
+      continue;
     }
     uint64_t q0 = sq_set(p, stm, to), q;
     int s2 = normalize(T, q0, &q);
     if (s2 == s) continue;
-    uint64_t a = T.king[oks];
-#pragma unroll
-    for (int t = 0; t < MAXS; t++)
-      if (t < M.nk[o] && M.pcs[o][t] != jj) a |= att[t];
-    if (a & bit(to)) continue;  // the king would stand in check
     int v;
     if (jj >= 0) {
       uint64_t c = sq_set(q, jj, sq_get(q, M.ii.last[l]));
@@ -948,14 +955,14 @@ __device__ void synth_exits_pos(const Tabs &T, const Mat &M, uint64_t p, uint64_
 }
 
 // has_legal_moves || has_legal_caps (src/movegen.c:75-111) for colour c:
-// the same answer, found with less work.  King moves are tested against the
-// opponent's attacks computed once with c's king lifted (so a slider's ray
-// passes through the square the king leaves); a capturing king move drops
-// the captured piece's attacks.  Only if no king move is legal are the other
-// pieces' moves tried, each with a full attack test (pins): under a double
-// check none can be legal, under a single check only moves onto the checker
-// or between it and the king can be.
-__device__ bool has_legal(const Tabs &T, const Mat &M, uint64_t p, uint64_t occ, int c)
+// the same answer, found with less work.  uo: the opponent's piece attacks
+// with c's king lifted (a slider's ray passes through the square the king
+// leaves), so a quiet king move's legality is a bit test; a capturing king
+// move recomputes them without the captured piece.  Only if no king move
+// is legal are the other pieces' moves tried, each with a full attack test
+// (pins): under a double check none can be legal, under a single check only
+// moves onto the checker or between it and the king can be.
+__device__ bool has_legal(const Tabs &T, const Mat &M, uint64_t p, uint64_t occ, int c, uint64_t uo)
 {
   int o = c ^ 1;
   int k = sq_get(p, c), ok = sq_get(p, o);
@@ -964,53 +971,38 @@ __device__ bool has_legal(const Tabs &T, const Mat &M, uint64_t p, uint64_t occ,
     if (t >= M.nk[c]) break;
     own |= bit(sq_get(p, M.pcs[c][t]));
   }
-  uint64_t occk = occ ^ bit(k);
-  uint64_t att[MAXS];
-  uint64_t all = T.king[ok];
-#pragma unroll
-  for (int t = 0; t < MAXS; t++) {
-    att[t] = 0;
-    if (t < M.nk[o]) {
-      int j = M.pcs[o][t];
-      att[t] = piece_att(T, M.pt[j], sq_get(p, j), occk);
-    }
-    all |= att[t];
-  }
   uint64_t kb = T.king[k] & ~own & ~T.king[ok];
-  if (kb & ~occ & ~all) return true;  // a quiet king move to a safe square
+  if (kb & ~occ & ~uo) return true;  // a quiet king move to a safe square
   uint64_t caps = kb & occ;           // king captures (the enemy king is never adjacent)
   while (caps) {
     int to = ctz64(caps);
     caps &= caps - 1;
     int cap = slot_of(p, to, M.col80[o]);
-    uint64_t a = T.king[ok];
-#pragma unroll
-    for (int t = 0; t < MAXS; t++)
-      if (t < M.nk[o] && M.pcs[o][t] != cap) a |= att[t];
+    uint64_t a = T.king[ok] | side_attacks(T, M, p, occ ^ bit(k), o, cap);
     if (!(a & bit(to))) return true;
   }
   // checkers (the lifted king does not block rays onto its own square)
+  uint64_t occk = occ ^ bit(k);
   uint64_t block = ~0ull;
   int nchk = 0;
-#pragma unroll
-  for (int t = 0; t < MAXS; t++) {
-    if (t < M.nk[o] && (att[t] & bit(k))) {
-      int j = M.pcs[o][t];
-      int s = sq_get(p, j);
-      uint64_t b = bit(s);
-      int pt = M.pt[j] & 7;
-      if (pt == ROOK || pt == QUEEN) {
-        uint64_t r = rook_att(s, occ) & rook_att(k, occ);
-        if ((s >> 3) == (k >> 3) || (s & 7) == (k & 7)) b |= r;
-      }
-      if (pt == BISHOP || pt == QUEEN) {
-        uint64_t r = bishop_att(s, occ) & bishop_att(k, occ);
-        if ((s >> 3) - (k >> 3) == (s & 7) - (k & 7) || (s >> 3) + (s & 7) == (k >> 3) + (k & 7))
-          b |= r;
-      }
-      block = nchk ? 0 : b;
-      nchk++;
+  for (int t = 0; t < MAXP - 1; t++) {
+    if (t >= M.nk[o]) break;
+    int j = M.pcs[o][t];
+    int s = sq_get(p, j);
+    int pt = M.pt[j] & 7;
+    if (!(piece_att(T, M.pt[j], s, occk) & bit(k))) continue;
+    uint64_t b = bit(s);
+    if (pt == ROOK || pt == QUEEN) {
+      uint64_t r = rook_att(s, occ) & rook_att(k, occ);
+      if ((s >> 3) == (k >> 3) || (s & 7) == (k & 7)) b |= r;
     }
+    if (pt == BISHOP || pt == QUEEN) {
+      uint64_t r = bishop_att(s, occ) & bishop_att(k, occ);
+      if ((s >> 3) - (k >> 3) == (s & 7) - (k & 7) || (s >> 3) + (s & 7) == (k >> 3) + (k & 7))
+        b |= r;
+    }
+    block = nchk ? 0 : b;
+    nchk++;
   }
   if (nchk > 1) return false;
   block &= ~bit(ok);  // the enemy king is never captured
@@ -1056,17 +1048,22 @@ __global__ __launch_bounds__(256) void k_init(const Mat *__restrict__ Mp, uint64
     if (idx < M.ii.size) {
       uint64_t p = kings_of(T, s);
       uint64_t occ = idx_to_sq(T, M.ii, idx, &p);
-      wi = attacked_by(T, M, p, occ, sq_get(p, BLACK), WHITE, -1);
-      bi = attacked_by(T, M, p, occ, sq_get(p, WHITE), BLACK, -1);
+      int wk = sq_get(p, WHITE), bk = sq_get(p, BLACK);
+      // each side's piece attacks with the enemy king lifted: the legality
+      // tests, the exits' king moves and the mate search all read them
+      uint64_t uw = side_attacks(T, M, p, occ ^ bit(bk), WHITE, -1);
+      uint64_t ub = side_attacks(T, M, p, occ ^ bit(wk), BLACK, -1);
+      wi = ((uw | T.king[wk]) >> bk) & 1;  // white to move could take the black king
+      bi = ((ub | T.king[bk]) >> wk) & 1;
       // an illegal position is in wins(stm) from the start, which masks its
       // exit candidates in both the W sweep and verification
       if (M.synth) {
-        if (!wi) synth_exits_pos(T, M, p, occ, WHITE, idx, &xw0, &xl0);
-        if (!bi) synth_exits_pos(T, M, p, occ, BLACK, idx, &xw1, &xl1);
+        if (!wi) synth_exits_pos(T, M, p, occ, WHITE, idx, ub, &xw0, &xl0);
+        if (!bi) synth_exits_pos(T, M, p, occ, BLACK, idx, uw, &xw1, &xl1);
       }
       if (wi != bi) {
         int c = bi ? WHITE : BLACK;
-        bool mate = !has_legal(T, M, p, occ, c);
+        bool mate = !has_legal(T, M, p, occ, c, c == WHITE ? ub : uw);
         wm = mate && c == WHITE;
         bm = mate && c == BLACK;
       }
