@@ -254,6 +254,20 @@ int tbg_continue(tbg_ctx *ctx, int form, int stop_at, tbg_line_fn fn, void *user
  * n/L per side and slice, derived on the device from the codes. */
 int tbg_export_plies(tbg_ctx *ctx, const char *root);
 
+/* Final table writer: the finished table as a Syzygy WDL file
+ * <dir>/<name>.rtbw, in the format the reference's probe_wdl maps and
+ * decodes (src/probe.c:1210-1345 layout, 603-687 index, 1388-1456
+ * decompression), so the reference can resolve the captures of a larger
+ * material from this engine's tables.  Fills in the merge() stub of
+ * src/tb8gen.c:1233-1235,1448-1452 for WDL.  Values: 2 win, 1 cursed win
+ * (conversion after more than 100 plies), 0 draw, -1 blessed loss, -2 loss,
+ * stored as value + 2; broken and illegal positions hold the sub-table's
+ * most frequent value.  name is the file's material name in the probing
+ * code's colour convention (src/probe.c:304-318); at most 7 pieces.  path
+ * (nullable, cap bytes) receives the file's path. */
+int tbg_rtbw_name(tbg_ctx *ctx, char *name, size_t cap);
+int tbg_write_rtbw(tbg_ctx *ctx, const char *dir, char *path, size_t cap);
+
 /* Launch geometry of the ply kernels.  Results never depend on it; only
  * speed does.  tbg_autotune picks one by measurement; tbg_set_geometry with
  * a NULL context sets the default that later contexts and servers start

@@ -56,6 +56,22 @@ def lib():
                                      ctypes.c_uint64]
         L.tbo_time_plies.argtypes = [ctypes.c_char_p, ctypes.c_int,
                                      ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)]
+        L.tbo_rtbw_open.argtypes = [ctypes.c_char_p, ctypes.c_char_p]
+        L.tbo_rtbw_close.argtypes = [ctypes.c_int]
+        L.tbo_rtbw_size.restype = ctypes.c_uint64
+        L.tbo_rtbw_size.argtypes = [ctypes.c_int, ctypes.c_int]
+        L.tbo_rtbw_probe.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, u8p, u8p]
+        L.tbo_rtbw_roundtrip.restype = ctypes.c_uint64
+        L.tbo_rtbw_roundtrip.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_uint64,
+                                         ctypes.POINTER(ctypes.c_uint64)]
+        L.tbo_rtbw_decode.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_uint64, u8p, u8p]
+        L.tbo_rtbw_tables.argtypes = [u8p, ctypes.POINTER(ctypes.c_int8), u8p, u8p]
+        L.tbo_rtbw_verify.restype = ctypes.c_uint64
+        L.tbo_rtbw_verify.argtypes = [ctypes.c_int, ctypes.c_char_p,
+                                      ctypes.POINTER(ctypes.c_uint16),
+                                      ctypes.POINTER(ctypes.c_uint16), ctypes.c_uint64,
+                                      ctypes.POINTER(ctypes.c_uint64),
+                                      ctypes.POINTER(ctypes.c_int64)]
         _lib = L
     return _lib
 
@@ -185,3 +201,72 @@ def time_plies(material, max_plies):
     if r:
         raise RuntimeError(f"oracle timing of {material} failed ({r})")
     return sec.value, hs.value
+
+
+class WdlFile:
+    """A Syzygy WDL file read by the oracle's restatement of the probing
+    code's reader (pawnless tables)."""
+
+    def __init__(self, path, name):
+        h = lib().tbo_rtbw_open(str(path).encode(), name.encode())
+        if h < 0:
+            raise RuntimeError(f"cannot read {path} as {name} ({h})")
+        self.h = h
+        self.name = name
+
+    def close(self):
+        if self.h >= 0:
+            lib().tbo_rtbw_close(self.h)
+            self.h = -1
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def size(self, side):
+        return int(lib().tbo_rtbw_size(self.h, side))
+
+    def probe(self, stm, pt, sq):
+        n = len(pt)
+        a = (ctypes.c_uint8 * 8)(*pt)
+        b = (ctypes.c_uint8 * 8)(*sq)
+        return lib().tbo_rtbw_probe(self.h, stm, n, a, b)
+
+    def roundtrip(self, side, stride=1):
+        """(indices that decode and encode back to themselves, others)"""
+        bad = ctypes.c_uint64()
+        ok = lib().tbo_rtbw_roundtrip(self.h, side, int(stride), ctypes.byref(bad))
+        return int(ok), bad.value
+
+    def decode(self, side, idx):
+        pt = (ctypes.c_uint8 * 8)()
+        sq = (ctypes.c_uint8 * 8)()
+        if lib().tbo_rtbw_decode(self.h, side, int(idx), pt, sq):
+            raise ValueError("index out of range")
+        return list(pt), list(sq)
+
+    def verify(self, material, codes_w, codes_b, stride=1):
+        """Probes every stride-th legal position of a table given by its
+        codes ([462, slice_size] per side, the material's slot order);
+        returns (checked, mismatches, first mismatch or None)."""
+        cw = np.ascontiguousarray(codes_w, dtype=np.uint16).ravel()
+        cb = np.ascontiguousarray(codes_b, dtype=np.uint16).ravel()
+        bad = ctypes.c_uint64()
+        fb = (ctypes.c_int64 * 5)()
+        u16p = ctypes.POINTER(ctypes.c_uint16)
+        n = lib().tbo_rtbw_verify(self.h, material.encode(), cw.ctypes.data_as(u16p),
+                                  cb.ctypes.data_as(u16p), int(stride), ctypes.byref(bad), fb)
+        first = tuple(fb) if bad.value else None
+        return int(n), bad.value, first
+
+
+def rtbw_tables():
+    """(Triangle, OffDiag, Diag, Lower) of the oracle's file reader, 64 each."""
+    tri = (ctypes.c_uint8 * 64)()
+    off = (ctypes.c_int8 * 64)()
+    dg = (ctypes.c_uint8 * 64)()
+    lo = (ctypes.c_uint8 * 64)()
+    lib().tbo_rtbw_tables(tri, off, dg, lo)
+    return list(tri), list(off), list(dg), list(lo)

@@ -1304,3 +1304,494 @@ int tbo_time_plies(const char *material, int max_plies, double *loop_seconds, in
 }
 
 int tbo_get_threads(void) { return g_threads; }
+
+/* ------------------------------------------------------------------ */
+/* Syzygy WDL files: a restatement of the reader the reference probes    */
+/* its capture exits with (probe.c), for pawnless tables.  Used to read  */
+/* the engine's final-table writer output.  The piece-encoding tables     */
+/* (Triangle, OffDiag, Diag, Lower, probe.c:300-440) are built by rule.   */
+
+static uint8_t SzTri[64], SzDiag[64], SzLower[64], SzInvDiag[16], SzInvLower[36];
+static int8_t SzOffDiag[64];
+
+static void sz_tables(void)
+{
+  static int ready;
+  if (ready) return;
+  init_tables();
+  for (int sq = 0; sq < 64; sq++) {
+    int f = sq & 7, r = sq >> 3;
+    /* Triangle: the a1-d1-d4 class of sq under the board symmetries */
+    int cf = f > 3 ? 7 - f : f, cr = r > 3 ? 7 - r : r;
+    if (cr > cf) { int x = cf; cf = cr; cr = x; }
+    SzTri[sq] = (uint8_t)TriOf[cr * 8 + cf];
+    SzOffDiag[sq] = (int8_t)(r > f ? 1 : r < f ? -1 : 0);
+    SzDiag[sq] = (uint8_t)(r == f ? f : r + f == 7 ? 8 + r : 0);
+    /* Lower: squares below a1-h8 numbered rank by rank (0..27), squares
+     * above it as their mirror image, a1-h8 itself 28..35 */
+    int lf = r > f ? r : f, lr = r > f ? f : r;
+    SzLower[sq] = (uint8_t)(lf == lr ? 28 + lf : lr * 7 - lr * (lr - 1) / 2 + (lf - lr - 1));
+  }
+  for (int i = 0; i < 8; i++) {
+    SzInvDiag[i] = (uint8_t)(9 * i);
+    SzInvDiag[8 + i] = (uint8_t)(7 * (i + 1));
+  }
+  for (int sq = 0; sq < 64; sq++)
+    if ((sq >> 3) <= (sq & 7)) SzInvLower[SzLower[sq]] = (uint8_t)sq;
+  ready = 1;
+}
+
+typedef struct {
+  int constant, value;                 /* a constant sub-table (flag 0x80) */
+  int block_size, idx_bits, min_len, max_len;
+  uint32_t num_syms;
+  uint64_t base[64];                   /* by code length */
+  uint32_t offset[64];
+  uint8_t symlen[4096];
+  const uint8_t *sympat, *index_table, *size_table, *data;
+  uint64_t tb_size;
+  uint8_t pieces[MAX_PIECES], norm[MAX_PIECES];
+  uint64_t factor[MAX_PIECES];
+} SzSub;
+
+typedef struct {
+  uint8_t *file;
+  uint64_t len;
+  int num, kk_enc, symmetric, split;
+  int wcnt[16];                        /* the name's piece counts, colour bit 8 */
+  SzSub sub[2];
+} SzFile;
+
+#define SZ_MAX 16
+static SzFile *g_sz[SZ_MAX];
+
+static uint32_t rd16(const uint8_t *p) { return (uint32_t)p[0] | ((uint32_t)p[1] << 8); }
+static uint32_t rd32(const uint8_t *p) { return rd16(p) | (rd16(p + 2) << 16); }
+
+/* calc_symlen (probe.c:1045-1059) */
+static void sz_symlen(SzSub *d, uint32_t s, bool *done)
+{
+  const uint8_t *w = d->sympat + 3 * s;
+  uint32_t s2 = ((uint32_t)w[2] << 4) | (w[1] >> 4);
+  if (s2 == 0xfff) d->symlen[s] = 0;
+  else {
+    uint32_t s1 = ((uint32_t)(w[1] & 0xf) << 8) | w[0];
+    if (!done[s1]) sz_symlen(d, s1, done);
+    if (!done[s2]) sz_symlen(d, s2, done);
+    d->symlen[s] = (uint8_t)(d->symlen[s1] + d->symlen[s2] + 1);
+  }
+  done[s] = true;
+}
+
+/* set_enc_info (probe.c:725-770), pawnless */
+static uint64_t sz_enc_info(SzSub *e, const SzFile *f, const uint8_t *tb, int shift)
+{
+  for (int i = 0; i < f->num; i++) {
+    e->pieces[i] = (tb[i + 1] >> shift) & 0x0f;
+    e->norm[i] = 0;
+  }
+  int order = (tb[0] >> shift) & 0x0f;
+  int k = e->norm[0] = (uint8_t)(f->kk_enc ? 2 : 3);
+  for (int i = k; i < f->num; i += e->norm[i])
+    for (int j = i; j < f->num && e->pieces[j] == e->pieces[i]; j++) e->norm[i]++;
+  int n = 64 - k;
+  uint64_t fac = 1;
+  for (int i = 0; k < f->num || i == order; i++) {
+    if (i == order) {
+      e->factor[0] = fac;
+      fac *= f->kk_enc ? 462 : 31332;
+    } else {
+      e->factor[k] = fac;
+      fac *= Binom[e->norm[k]][n];
+      n -= e->norm[k];
+      k += e->norm[k];
+    }
+  }
+  return fac;
+}
+
+/* setup_pairs + setup_huffman (probe.c:1061-1208): parses one sub-table's
+ * header at *pp; returns -1 for a form this restatement does not read
+ * (rANS).  sizes[] = index, size-table and data bytes. */
+static int sz_pairs(SzSub *d, const uint8_t **pp, uint64_t *sizes)
+{
+  const uint8_t *p = *pp;
+  sizes[0] = sizes[1] = sizes[2] = 0;
+  if (p[0] & 0x80) {
+    d->constant = 1;
+    d->value = p[1];
+    *pp = p + 2;
+    return 0;
+  }
+  if (p[0] & 0x40) return -1;
+  d->constant = 0;
+  d->block_size = p[1];
+  d->idx_bits = p[2];
+  uint32_t real_blocks = rd32(p + 4), nblocks = real_blocks + p[3];
+  d->max_len = p[8];
+  d->min_len = p[9];
+  int h = d->max_len - d->min_len + 1;
+  if (h < 1 || d->max_len > 32) return -1;
+  uint64_t tmp[64] = { 0 };
+  uint32_t off[64];
+  for (int i = 0; i < h; i++) off[i] = rd16(p + 10 + 2 * i);
+  d->num_syms = rd16(p + 10 + 2 * h);
+  d->sympat = p + 12 + 2 * h;
+  for (int i = h - 2; i >= 0; i--) tmp[i] = (tmp[i + 1] + off[i] - off[i + 1]) / 2;
+  for (int i = 0; i < h; i++) {
+    d->base[d->min_len + i] = tmp[i] << (64 - (d->min_len + i));
+    d->offset[d->min_len + i] = off[i];
+  }
+  *pp = p + 12 + 2 * h + 3 * d->num_syms + (d->num_syms & 1);
+  bool done[4096] = { false };
+  for (uint32_t s = 0; s < d->num_syms; s++)
+    if (!done[s]) sz_symlen(d, s, done);
+  sizes[0] = 6 * ((d->tb_size + (1ULL << d->idx_bits) - 1) >> d->idx_bits);
+  sizes[1] = 2ULL * nblocks;
+  sizes[2] = (uint64_t)real_blocks << d->block_size;
+  return 0;
+}
+
+/* 64 bits of the block's bit stream from bit position pos, most
+ * significant first (read_be_u64, util.h:75) */
+static uint64_t sz_bits(const uint8_t *p, uint64_t pos)
+{
+  const uint8_t *b = p + (pos >> 3);
+  uint64_t x = 0;
+  for (int i = 0; i < 8; i++) x = (x << 8) | b[i];
+  int sh = (int)(pos & 7);
+  return sh ? (x << sh) | (b[8] >> (8 - sh)) : x;
+}
+
+/* decompress_pairs / decompress_huff (probe.c:1388-1470): the stored byte
+ * of position idx */
+static int sz_value(const SzSub *d, uint64_t idx)
+{
+  if (d->constant) return d->value;
+  uint32_t main_idx = (uint32_t)(idx >> d->idx_bits);
+  int litidx = (int)(idx & ((1u << d->idx_bits) - 1)) - (int)(1u << (d->idx_bits - 1));
+  uint32_t block = rd32(d->index_table + 6 * (uint64_t)main_idx);
+  litidx += (int)rd16(d->index_table + 6 * (uint64_t)main_idx + 4);
+  if (litidx < 0)
+    while (litidx < 0) litidx += (int)rd16(d->size_table + 2 * (uint64_t)--block) + 1;
+  else
+    while (litidx > (int)rd16(d->size_table + 2 * (uint64_t)block))
+      litidx -= (int)rd16(d->size_table + 2 * (uint64_t)block++) + 1;
+  const uint8_t *ptr = d->data + ((uint64_t)block << d->block_size);
+  uint64_t pos = 0;
+  uint32_t sym;
+  for (;;) {
+    uint64_t code = sz_bits(ptr, pos);
+    int l = d->min_len;
+    while (code < d->base[l]) l++;
+    sym = d->offset[l] + (uint32_t)((code - d->base[l]) >> (64 - l));
+    if (litidx < (int)d->symlen[sym] + 1) break;
+    litidx -= (int)d->symlen[sym] + 1;
+    pos += (uint64_t)l;
+  }
+  while (d->symlen[sym] != 0) {
+    const uint8_t *w = d->sympat + 3 * sym;
+    uint32_t s1 = ((uint32_t)(w[1] & 0xf) << 8) | w[0];
+    if (litidx < (int)d->symlen[s1] + 1) sym = s1;
+    else {
+      litidx -= (int)d->symlen[s1] + 1;
+      sym = ((uint32_t)w[2] << 4) | (w[1] >> 4);
+    }
+  }
+  return d->sympat[3 * sym];
+}
+
+/* Opens <path> as the WDL file of material `name` (e.g. "KQvKR", white as
+ * the file's name has it): init_table (probe.c:1210-1345) and the entry
+ * set-up of detect_tb (probe.c:143-191).  Returns a handle, or < 0. */
+int tbo_rtbw_open(const char *path, const char *name)
+{
+  sz_tables();
+  int h = -1;
+  for (int i = 0; i < SZ_MAX; i++)
+    if (!g_sz[i]) { h = i; break; }
+  if (h < 0) return -2;
+  FILE *F = fopen(path, "rb");
+  if (!F) return -3;
+  fseek(F, 0, SEEK_END);
+  long len = ftell(F);
+  fseek(F, 0, SEEK_SET);
+  SzFile *f = calloc(1, sizeof(SzFile));
+  f->file = calloc((size_t)len + 128, 1);  /* slack for reads past a block */
+  f->len = (uint64_t)len;
+  if (fread(f->file, 1, (size_t)len, F) != (size_t)len) { fclose(F); free(f->file); free(f); return -3; }
+  fclose(F);
+  int color = 0, cnt[16] = { 0 }, num = 0;
+  for (const char *s = name; *s; s++) {
+    if (*s == 'v') { color = 8; continue; }
+    for (int i = KNIGHT; i <= KING; i++)
+      if (*s == PieceChar[i]) { cnt[i | color]++; num++; }
+  }
+  memcpy(f->wcnt, cnt, sizeof cnt);
+  f->num = num;
+  int uniq = 0;
+  for (int i = 0; i < 16; i++) uniq += cnt[i] == 1;
+  f->kk_enc = uniq == 2;
+  f->symmetric = 1;
+  for (int i = 0; i < 8; i++) f->symmetric &= cnt[i] == cnt[i + 8];
+  const uint8_t *d = f->file;
+  if (rd32(d) != 0x5d23e871u) { free(f->file); free(f); return -4; }
+  f->split = d[4] & 1;
+  d += 5;
+  uint64_t sz0 = sz_enc_info(&f->sub[0], f, d, 0);
+  f->sub[0].tb_size = sz0;
+  if (f->split) f->sub[1].tb_size = sz_enc_info(&f->sub[1], f, d, 4);
+  d += f->num + 1;
+  d += (uintptr_t)(d - f->file) & 1;
+  uint64_t sizes[2][3];
+  for (int t = 0; t < 1 + f->split; t++)
+    if (sz_pairs(&f->sub[t], &d, sizes[t])) { free(f->file); free(f); return -5; }
+  for (int t = 0; t < 1 + f->split; t++) { f->sub[t].index_table = d; d += sizes[t][0]; }
+  for (int t = 0; t < 1 + f->split; t++) { f->sub[t].size_table = d; d += sizes[t][1]; }
+  for (int t = 0; t < 1 + f->split; t++) {
+    d = f->file + (((uint64_t)(d - f->file) + 63) & ~63ULL);
+    f->sub[t].data = d;
+    d += sizes[t][2];
+  }
+  if ((uint64_t)(d - f->file) > f->len) { free(f->file); free(f); return -6; }
+  g_sz[h] = f;
+  return h;
+}
+
+void tbo_rtbw_close(int h)
+{
+  if (h < 0 || h >= SZ_MAX || !g_sz[h]) return;
+  free(g_sz[h]->file);
+  free(g_sz[h]);
+  g_sz[h] = NULL;
+}
+
+uint64_t tbo_rtbw_size(int h, int side)
+{
+  if (h < 0 || h >= SZ_MAX || !g_sz[h] || side < 0 || side > g_sz[h]->split) return 0;
+  return g_sz[h]->sub[side].tb_size;
+}
+
+/* encode, PIECE_ENC (probe.c:603-687); p[] in header order, modified */
+static uint64_t sz_encode(const SzSub *e, const SzFile *f, uint8_t *p)
+{
+  int n = f->num, k;
+  uint64_t idx;
+  if (p[0] & 0x04)
+    for (int i = 0; i < n; i++) p[i] ^= 0x07;
+  if (p[0] & 0x20)
+    for (int i = 0; i < n; i++) p[i] ^= 0x38;
+  for (int i = 0; i < n; i++)
+    if (SzOffDiag[p[i]]) {
+      if (SzOffDiag[p[i]] > 0 && i < (f->kk_enc ? 2 : 3))
+        for (int j = 0; j < n; j++) p[j] = FlipDiag[p[j]];
+      break;
+    }
+  if (f->kk_enc) {
+    idx = (uint64_t)KKIdx[SzTri[p[0]]][p[1]];
+    k = 2;
+  } else {
+    int s1 = p[1] > p[0], s2 = (p[2] > p[0]) + (p[2] > p[1]);
+    if (SzOffDiag[p[0]])
+      idx = (uint64_t)SzTri[p[0]] * 63 * 62 + (uint64_t)(p[1] - s1) * 62 + (uint64_t)(p[2] - s2);
+    else if (SzOffDiag[p[1]])
+      idx = 6 * 63 * 62 + (uint64_t)SzDiag[p[0]] * 28 * 62 + (uint64_t)SzLower[p[1]] * 62 +
+            (uint64_t)(p[2] - s2);
+    else if (SzOffDiag[p[2]])
+      idx = 6 * 63 * 62 + 4 * 28 * 62 + (uint64_t)SzDiag[p[0]] * 7 * 28 +
+            (uint64_t)(SzDiag[p[1]] - s1) * 28 + SzLower[p[2]];
+    else
+      idx = 6 * 63 * 62 + 4 * 28 * 62 + 4 * 7 * 28 + (uint64_t)SzDiag[p[0]] * 7 * 6 +
+            (uint64_t)(SzDiag[p[1]] - s1) * 6 + (uint64_t)(SzDiag[p[2]] - s2);
+    k = 3;
+  }
+  idx *= e->factor[0];
+  bb_t occ = 0;
+  for (int i = 0; i < k; i++) occ |= bit(p[i]);
+  while (k < n) {
+    int t = k + e->norm[k];
+    for (int a = k; a < t; a++)
+      for (int b = a + 1; b < t; b++)
+        if (p[a] > p[b]) { uint8_t x = p[a]; p[a] = p[b]; p[b] = x; }
+    uint64_t s = 0;
+    for (int i = k; i < t; i++) s += Binom[i - k + 1][p[i] - popcnt(occ & (bit(p[i]) - 1))];
+    idx += s * e->factor[k];
+    for (; k < t; k++) occ |= bit(p[k]);
+  }
+  return idx;
+}
+
+/* decode_helper + decode, PIECE_ENC (probe.c:859-980): squares in header
+ * order of sub-table index idx; 0, or -1 for an index outside the table */
+static int sz_decode(const SzSub *e, const SzFile *f, uint64_t idx, uint8_t *p)
+{
+  if (idx >= e->tb_size) return -1;
+  int n = f->num, lead = f->kk_enc ? 2 : 3;
+  /* mixed radix in any digit order: digit = (idx / factor) % range */
+  uint64_t q = (idx / e->factor[0]) % (lead == 2 ? 462 : 31332);
+  bb_t occ = 0;
+  if (lead == 2) {
+    p[0] = KKSq[q][0];
+    p[1] = KKSq[q][1];
+    occ = bit(p[0]) | bit(p[1]);
+  } else {
+    const uint64_t A = 6 * 63 * 62, B = 4 * 28 * 62, C = 4 * 7 * 28;
+    const bb_t DIAG = 0x8040201008040201ULL;
+    uint64_t r;
+    if (q < A) {
+      r = q / (63 * 62); q -= r * 63 * 62;
+      p[0] = InvTri[r];
+      r = q / 62; q -= r * 62;
+      p[1] = (uint8_t)(r + (r >= p[0]));
+      occ = bit(p[0]) | bit(p[1]);
+      p[2] = (uint8_t)select_bit(~occ, (int)q);
+    } else if (q < A + B) {
+      q -= A;
+      r = q / (28 * 62); q -= r * 28 * 62;
+      p[0] = SzInvDiag[r];
+      r = q / 62; q -= r * 62;
+      p[1] = SzInvLower[r];
+      occ = bit(p[0]) | bit(p[1]);
+      p[2] = (uint8_t)select_bit(~occ, (int)q);
+    } else if (q < A + B + C) {
+      q -= A + B;
+      r = q / (7 * 28); q -= r * 7 * 28;
+      p[0] = SzInvDiag[r];
+      r = q / 28; q -= r * 28;
+      r += SzInvDiag[r] >= p[0];
+      p[1] = SzInvDiag[r];
+      p[2] = SzInvLower[q];
+    } else {
+      q -= A + B + C;
+      r = q / 42; q -= r * 42;
+      p[0] = SzInvDiag[r];
+      r = q / 6; q -= r * 6;
+      r += SzInvDiag[r] >= p[0];
+      p[1] = SzInvDiag[r];
+      occ = bit(p[0]) | bit(p[1]);
+      p[2] = (uint8_t)select_bit(~occ & DIAG, (int)q);
+    }
+    occ = bit(p[0]) | bit(p[1]) | bit(p[2]);
+  }
+  for (int k = lead; k < n; k += e->norm[k]) {
+    uint64_t range = Binom[e->norm[k]][64 - popcnt(occ)];
+    occ = unrank_set((idx / e->factor[k]) % range, e->norm[k], p + k, occ);
+  }
+  return 0;
+}
+
+/* probe_table's WDL path (probe.c:1522-1611) for a pawnless position:
+ * pt[i] piece code (colour bit 8), sq[i] its square, i < num.  Returns
+ * the stored WDL value (-2..2), or -9 on bad input. */
+int tbo_rtbw_probe(int h, int stm, int num, const uint8_t *pt, const uint8_t *sq)
+{
+  if (h < 0 || h >= SZ_MAX || !g_sz[h] || num != g_sz[h]->num) return -9;
+  const SzFile *f = g_sz[h];
+  int cnt[16] = { 0 };
+  for (int i = 0; i < num; i++) cnt[pt[i]]++;
+  bool same = !memcmp(cnt, f->wcnt, sizeof cnt), flip, bside;
+  if (f->symmetric) {
+    flip = stm != WHITE;
+    bside = false;
+  } else {
+    flip = !same;
+    bside = (stm == WHITE) == flip;
+  }
+  const SzSub *e = &f->sub[f->split ? bside : 0];
+  uint8_t p[MAX_PIECES];
+  /* list_squares (probe.c:1513-1521) */
+  for (int i = 0; i < num;) {
+    int t = e->pieces[i] ^ (flip ? 8 : 0), i0 = i;
+    for (int j = 0; j < num; j++)
+      if (pt[j] == t) p[i++] = (uint8_t)(sq[j] ^ (flip ? 0x38 : 0));
+    if (i == i0) return -9;
+  }
+  uint64_t idx = sz_encode(e, f, p);
+  if (idx >= e->tb_size) return -9;
+  return sz_value(e, idx) - 2;
+}
+
+/* Round trip of the index restatement: for every index of sub-table side
+ * (step `stride`), decode then encode; returns the number of indices whose
+ * position has distinct squares and encodes back to itself, and counts the
+ * others in *bad (must stay 0). */
+uint64_t tbo_rtbw_roundtrip(int h, int side, uint64_t stride, uint64_t *bad)
+{
+  *bad = 0;
+  if (h < 0 || h >= SZ_MAX || !g_sz[h]) return 0;
+  const SzFile *f = g_sz[h];
+  const SzSub *e = &f->sub[side];
+  uint64_t ok = 0, nbad = 0;
+#pragma omp parallel for num_threads(g_threads) reduction(+ : ok, nbad) schedule(static)
+  for (uint64_t i = 0; i < e->tb_size; i += stride) {
+    uint8_t p[MAX_PIECES], p2[MAX_PIECES];
+    sz_decode(e, f, i, p);
+    memcpy(p2, p, sizeof p);
+    if (sz_encode(e, f, p2) == i) ok++;
+    else nbad++;
+  }
+  *bad = nbad;
+  return ok;
+}
+
+/* A table's entries for positions decoded from the file index space:
+ * squares in header order and header piece codes, for tests. */
+int tbo_rtbw_decode(int h, int side, uint64_t idx, uint8_t *pt, uint8_t *sq)
+{
+  if (h < 0 || h >= SZ_MAX || !g_sz[h] || side < 0 || side > g_sz[h]->split) return -1;
+  const SzFile *f = g_sz[h];
+  if (sz_decode(&f->sub[side], f, idx, sq)) return -1;
+  for (int i = 0; i < f->num; i++) pt[i] = f->sub[side].pieces[i];
+  return 0;
+}
+
+/* Checks a WDL file against a table's codes (material as the codes are
+ * laid out, reference slot order; codes[stm][slice * slice_size + idx]):
+ * every stride-th position of every slice, both sides, legal ones only.
+ * Returns the number checked; *bad counts mismatches, first_bad (5
+ * entries, may be NULL) = stm, slice, idx, code, file value of the first. */
+uint64_t tbo_rtbw_verify(int h, const char *material, const uint16_t *codes_w,
+                         const uint16_t *codes_b, uint64_t stride, uint64_t *bad,
+                         int64_t *first_bad)
+{
+  *bad = 0;
+  Table *t = calloc(1, sizeof(Table));
+  if (!t || setup_material(t, material)) { free(t); return 0; }
+  const uint16_t *codes[2] = { codes_w, codes_b };
+  uint64_t checked = 0, nbad = 0;
+  int64_t fb[5] = { -1, -1, -1, -1, -1 };
+  for (int stm = 0; stm < 2; stm++)
+#pragma omp parallel for num_threads(g_threads) reduction(+ : checked, nbad) schedule(dynamic)
+    for (int s = 0; s < NSLICES; s++)
+      for (uint64_t idx = (uint64_t)s % stride; idx < t->size; idx += stride) {
+        uint16_t c = codes[stm][(uint64_t)s * t->size + idx];
+        if (c == TBO_ILLEGAL) continue;
+        uint8_t sq[MAX_PIECES] = { 0 };
+        sq[0] = KKSq[s][0];
+        sq[1] = KKSq[s][1];
+        idx_to_sq(&t->ii, idx, sq);
+        int v = tbo_rtbw_probe(h, stm, t->num, t->pt, sq);
+        checked++;
+        if (v != code_to_wdl(c)) {
+          nbad++;
+#pragma omp critical
+          if (fb[0] < 0) {
+            fb[0] = stm; fb[1] = s; fb[2] = (int64_t)idx; fb[3] = c; fb[4] = v;
+          }
+        }
+      }
+  if (first_bad) memcpy(first_bad, fb, sizeof fb);
+  free(t);
+  *bad = nbad;
+  return checked;
+}
+
+/* The rule-built encoding tables, for tests against the reference's values */
+void tbo_rtbw_tables(uint8_t *tri, int8_t *offdiag, uint8_t *diag, uint8_t *lower)
+{
+  sz_tables();
+  memcpy(tri, SzTri, 64);
+  memcpy(offdiag, SzOffDiag, 64);
+  memcpy(diag, SzDiag, 64);
+  memcpy(lower, SzLower, 64);
+}

@@ -88,6 +88,20 @@ int tbo_synth_slice(const char *material, int s, uint64_t seed,
                     const char *log_path, int max_plies);
 uint64_t tbo_synth_hash(uint64_t seed, int stm, int kind, int region, uint64_t idx);
 
+/* Syzygy WDL files (the reader of probe.c, restated; pawnless only).
+ * name: the file's material, white as in the file name. */
+int tbo_rtbw_open(const char *path, const char *name);
+void tbo_rtbw_close(int h);
+uint64_t tbo_rtbw_size(int h, int side);
+/* stored WDL (-2..2) of a position: pt[i] piece code, sq[i] square */
+int tbo_rtbw_probe(int h, int stm, int num, const uint8_t *pt, const uint8_t *sq);
+uint64_t tbo_rtbw_roundtrip(int h, int side, uint64_t stride, uint64_t *bad);
+int tbo_rtbw_decode(int h, int side, uint64_t idx, uint8_t *pt, uint8_t *sq);
+void tbo_rtbw_tables(uint8_t *tri, int8_t *offdiag, uint8_t *diag, uint8_t *lower);
+uint64_t tbo_rtbw_verify(int h, const char *material, const uint16_t *codes_w,
+                         const uint16_t *codes_b, uint64_t stride, uint64_t *bad,
+                         int64_t *first_bad);
+
 /* CPU baseline sample (see tbo.c). */
 int tbo_time_plies(const char *material, int max_plies, double *loop_seconds, int64_t *half_steps);
 int tbo_get_threads(void);

@@ -33,7 +33,8 @@ ABI_SYMBOLS = [
     "tbg_emit_lines", "tbg_server_create", "tbg_server_destroy", "tbg_server_submit",
     "tbg_server_run", "tbg_server_result", "tbg_server_codes", "tbg_server_lines",
     "tbg_set_geometry", "tbg_get_geometry", "tbg_autotune", "tbg_file_write", "tbg_file_read",
-    "tbg_checkpoint", "tbg_restore", "tbg_continue", "tbg_export_plies",
+    "tbg_checkpoint", "tbg_restore", "tbg_continue", "tbg_export_plies", "tbg_rtbw_name",
+    "tbg_write_rtbw",
     "tbg_registry_create", "tbg_registry_get", "tbg_registry_count", "tbg_registry_entry",
     "tbg_registry_destroy",
 ]
@@ -158,6 +159,8 @@ def lib():
         L.tbg_continue.argtypes = [vp, ctypes.c_int, ctypes.c_int, LINE_FN, vp,
                                    ctypes.POINTER(Counts)]
         L.tbg_export_plies.argtypes = [vp, ctypes.c_char_p]
+        L.tbg_rtbw_name.argtypes = [vp, ctypes.c_char_p, ctypes.c_size_t]
+        L.tbg_write_rtbw.argtypes = [vp, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_size_t]
         L.tbg_registry_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int]
         L.tbg_registry_get.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(vp)]
         L.tbg_registry_count.argtypes = [vp]
@@ -260,6 +263,20 @@ class Table:
 
     def export_plies(self, root):
         check(lib().tbg_export_plies(self.ptr, str(root).encode()), f"export {self.material}")
+
+    def rtbw_name(self):
+        """Name of the table's WDL file in the probing code's colour convention."""
+        buf = ctypes.create_string_buffer(32)
+        check(lib().tbg_rtbw_name(self.ptr, buf, 32), f"rtbw name {self.material}")
+        return buf.value.decode()
+
+    def write_rtbw(self, directory):
+        """Writes the finished table as <directory>/<name>.rtbw (Syzygy WDL
+        format); returns the path."""
+        buf = ctypes.create_string_buffer(4096)
+        check(lib().tbg_write_rtbw(self.ptr, str(directory).encode(), buf, 4096),
+              f"write rtbw {self.material}")
+        return buf.value.decode()
 
     def emit_lines(self):
         self.lines = []

@@ -9,8 +9,11 @@
  * then the ply loop with the reference's more_* flags, printing the same
  * count lines.
  *
- * Usage: tb8gen [-f plain|batched|graph] [-d device] [-v] MATERIAL
+ * Usage: tb8gen [-f plain|batched|graph] [-d device] [-v] [-w DIR] MATERIAL
  *   -v   also print the count lines of the smaller materials
+ *   -w   write the finished tables (the material and every smaller one)
+ *        as Syzygy WDL files DIR/<name>.rtbw, the files the reference's
+ *        probe_wdl reads (the merge() stub of src/tb8gen.c:1448-1452)
  */
 #include <getopt.h>
 #include <stdio.h>
@@ -35,7 +38,8 @@ static void print_line(const char *line, void *user)
 int main(int argc, char **argv)
 {
   int opt, form = TBG_FORM_BATCHED, device = 0, verbose = 0;
-  while ((opt = getopt(argc, argv, "f:d:t:v")) != -1) {
+  const char *wdl_dir = NULL;
+  while ((opt = getopt(argc, argv, "f:d:t:vw:")) != -1) {
     switch (opt) {
     case 'f':
       form = !strcmp(optarg, "plain") ? TBG_FORM_PLAIN
@@ -45,6 +49,7 @@ int main(int argc, char **argv)
     case 'd': device = atoi(optarg); break;
     case 't': break; /* accepted for compatibility: the device replaces the thread pool */
     case 'v': verbose = 1; break;
+    case 'w': wdl_dir = optarg; break;
     default: return 2;
     }
   }
@@ -78,6 +83,15 @@ int main(int argc, char **argv)
   if (r) die(argv[optind], r);
   fprintf(stderr, "generated %d tables in %.3f s\n", n,
           (t1.tv_sec - t0.tv_sec) + 1e-9 * (t1.tv_nsec - t0.tv_nsec));
+  for (int i = 0; wdl_dir && i < n; i++) {
+    const char *name;
+    tbg_ctx *c;
+    char path[4096];
+    if (tbg_registry_entry(reg, i, &name, &c)) continue;
+    r = tbg_write_rtbw(c, wdl_dir, path, sizeof path);
+    if (r) die(name, r);
+    fprintf(stderr, "wrote %s\n", path);
+  }
   tbg_registry_destroy(reg);
   return 0;
 }

@@ -1537,6 +1537,202 @@ int tbg_export_plies(tbg_ctx *c, const char *root)
   return rc;
 }
 
+}  // extern "C"
+
+namespace tbg {
+int rtbw_write(const char *path, const uint8_t *pieces, int num, uint8_t order, int nsides,
+               uint8_t *const *vals, const uint64_t *n, int threads, std::string *err);
+}
+
+namespace {
+
+uint64_t binom64(int n, int k)
+{
+  if (k < 0 || k > n) return 0;
+  uint64_t r = 1;
+  for (int i = 0; i < k; i++) r = r * (uint64_t)(n - i) / (uint64_t)(i + 1);
+  return r;
+}
+
+// create_piece_string (src/probe.c:243-255): the idx-th multiset of n
+// non-king pieces in the probing code's enumeration, strongest first
+std::string syz_piece_string(int n, uint64_t idx)
+{
+  static const char pc[] = "PNBRQ";
+  std::string s((size_t)n, ' ');
+  for (int k = n - 1; k > 0; k--) {
+    int l = 0;
+    while (idx >= binom64(k + 1 + l, k + 1)) l++;
+    idx -= binom64(k + l, k + 1);
+    s[(size_t)(n - 1 - k)] = pc[l];
+  }
+  if (n) s[(size_t)(n - 1)] = pc[idx];
+  return s;
+}
+
+// non-king pieces of colour c, strongest first
+std::string side_pieces(const Mat &M, int c)
+{
+  std::string s;
+  for (int t = QUEEN; t >= KNIGHT; t--)
+    for (int i = 2; i < M.num; i++)
+      if (M.pt[i] == (t | (c << 3))) s += kPieceChar[t];
+  return s;
+}
+
+// The name under which the probing code looks for this material's WDL file
+// (init_tablebases' enumeration, src/probe.c:304-318: the side with more
+// pieces, or with equal counts the later multiset, is white) and whether
+// its colours are the reverse of ours.  False beyond 7 pieces.
+bool syzygy_name(const Mat &M, std::string *name, int *swap)
+{
+  std::string w = side_pieces(M, WHITE), b = side_pieces(M, BLACK);
+  for (int p = 1; p <= 5; p++)
+    for (int q = 0; q <= std::min(p, 5 - p); q++)
+      for (int64_t k = (int64_t)binom64(p + 4, 4) - 1; k >= 0; k--) {
+        std::string ws = syz_piece_string(p, (uint64_t)k);
+        for (int64_t l = q < p ? (int64_t)binom64(q + 4, 4) - 1 : k; l >= 0; l--) {
+          std::string bs = syz_piece_string(q, (uint64_t)l);
+          if ((ws == w && bs == b) || (ws == b && bs == w)) {
+            *name = "K" + ws + "vK" + bs;
+            *swap = !(ws == w && bs == b);
+            return true;
+          }
+        }
+      }
+  return false;
+}
+
+// Header piece order and index layout of the file (set_enc_info,
+// src/probe.c:725-770): both kings first, then, unless every non-king
+// piece has a twin, one unique piece to make the leading triple; then the
+// remaining pieces in runs of like pieces, strongest first, white first.
+// The leading group is the most significant digit (order = number of
+// groups after it).
+int syz_layout(const Mat &M, int swap, SyzLayout *L, uint8_t *pieces, uint8_t *order)
+{
+  memset(L, 0, sizeof *L);
+  int hp[MAXP], n = 0;
+  hp[n++] = KING;
+  hp[n++] = KING | 8;
+  for (int nc = 0; nc < 2; nc++)
+    for (int t = QUEEN; t >= KNIGHT; t--)
+      for (int i = 2; i < M.num; i++)
+        if (M.pt[i] == (t | ((nc ^ swap) << 3))) hp[n++] = t | (nc << 3);
+  int cnt[16] = { 0 };
+  for (int i = 0; i < n; i++) cnt[hp[i]]++;
+  int lead = 2;
+  for (int i = 2; i < n; i++)
+    if (cnt[hp[i]] == 1) {  // a unique piece joins the kings
+      int u = hp[i];
+      for (int j = i; j > 2; j--) hp[j] = hp[j - 1];
+      hp[2] = u;
+      lead = 3;
+      break;
+    }
+  L->num = n;
+  L->lead = lead;
+  bool used[MAXP] = { false };
+  for (int i = 0; i < n; i++) {
+    pieces[i] = (uint8_t)hp[i];
+    int want = hp[i] ^ (swap ? 8 : 0), slot = -1;
+    for (int j = 0; j < M.num && slot < 0; j++)
+      if (!used[j] && M.pt[j] == want) slot = j;
+    if (slot < 0) return fail(TBG_E_INVAL, "internal error: header piece without a slot");
+    used[slot] = true;
+    L->slot[i] = slot;
+  }
+  uint64_t f = 1;
+  int free_sq = 64 - lead;
+  for (int i = lead; i < n;) {
+    int g = L->ngroups++, m = 0;
+    while (i + m < n && hp[i + m] == hp[i]) m++;
+    L->gfirst[g] = i;
+    L->gmult[g] = m;
+    L->gfactor[g] = f;
+    L->gsize[g] = binom64(free_sq, m);
+    f *= L->gsize[g];
+    free_sq -= m;
+    i += m;
+  }
+  L->lead_factor = f;
+  L->tb_size = f * (lead == 2 ? 462 : 31332);
+  *order = (uint8_t)L->ngroups;
+  for (int i = 0; i < 10; i++) L->inv_tri[i] = kInvTri[i];
+  for (int i = 0; i < 8; i++) {
+    L->inv_diag[i] = (uint8_t)(9 * i);        // a1 .. h8
+    L->inv_diag[8 + i] = (uint8_t)(7 * (i + 1));  // h1 .. a8
+  }
+  int k = 0;
+  for (int r = 0; r < 8; r++)  // squares below a1-h8, rank by rank, then a1-h8
+    for (int fl = r + 1; fl < 8; fl++) L->inv_lower[k++] = (uint8_t)(r * 8 + fl);
+  for (int i = 0; i < 8; i++) L->inv_lower[k++] = (uint8_t)(9 * i);
+  return TBG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tbg_rtbw_name(tbg_ctx *c, char *name, size_t cap)
+{
+  if (!c || !name) return fail(TBG_E_INVAL, "null argument");
+  std::string nm;
+  int swap;
+  if (c->h.synth || !syzygy_name(c->h, &nm, &swap))
+    return fail(TBG_E_INVAL, "%s has no WDL file name (at most 7 pieces, whole tables)", c->name);
+  if (nm.size() + 1 > cap) return fail(TBG_E_INVAL, "name buffer too small");
+  memcpy(name, nm.c_str(), nm.size() + 1);
+  return TBG_OK;
+}
+
+int tbg_write_rtbw(tbg_ctx *c, const char *dir, char *path, size_t cap)
+{
+  if (!c || !dir) return fail(TBG_E_INVAL, "null argument");
+  if (c->h.synth) return fail(TBG_E_INVAL, "a single-slice job is not a whole table");
+  if (!c->generated || !c->codes[0] || !c->codes[1])
+    return fail(TBG_E_STATE, "writing a table needs a finished generation that keeps codes");
+  std::string nm;
+  int swap;
+  if (!syzygy_name(c->h, &nm, &swap))
+    return fail(TBG_E_INVAL, "%s has no WDL file name (at most 7 pieces)", c->name);
+  std::string file = std::string(dir) + "/" + nm + ".rtbw";
+  if (path && file.size() + 1 > cap) return fail(TBG_E_INVAL, "path buffer too small");
+  SyzLayout L;
+  uint8_t pieces[MAXP], order;
+  TRY(syz_layout(c->h, swap, &L, pieces, &order));
+  // a material equal to its colour reversal has one sub-table (white to
+  // move); otherwise one per side to move, in the name's colours
+  int nsides = side_pieces(c->h, WHITE) == side_pieces(c->h, BLACK) ? 1 : 2;
+  std::vector<uint8_t> vals[2];
+  const uint64_t chunk = std::min<uint64_t>(L.tb_size, 1ull << 28);
+  uint8_t *dbuf = nullptr;
+  HIPCHK(hipMalloc(&dbuf, chunk));
+  int rc = TBG_OK;
+  for (int t = 0; t < nsides && rc == TBG_OK; t++) {
+    L.stm = t ^ swap;
+    vals[t].resize(L.tb_size);
+    for (uint64_t i0 = 0; i0 < L.tb_size && rc == TBG_OK; i0 += chunk) {
+      uint64_t n = std::min(chunk, L.tb_size - i0);
+      hipLaunchKernelGGL(k_syz_gather, dim3(cap_grid((n + 255) / 256, (uint64_t)g_cus * 32)), dim3(256),
+                         0, c->st, c->dM, c->codes[L.stm], L, i0, n, dbuf);
+      hipError_t e = hipGetLastError();
+      if (e == hipSuccess) e = hipMemcpyAsync(vals[t].data() + i0, dbuf, n, hipMemcpyDeviceToHost, c->st);
+      if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+      if (e != hipSuccess) rc = fail(TBG_E_HIP, "HIP error: %s", hipGetErrorString(e));
+    }
+  }
+  (void)hipFree(dbuf);
+  if (rc != TBG_OK) return rc;
+  uint8_t *vp[2] = { vals[0].data(), nsides == 2 ? vals[1].data() : nullptr };
+  uint64_t ns[2] = { L.tb_size, L.tb_size };
+  std::string e;
+  if (rtbw_write(file.c_str(), pieces, L.num, order, nsides, vp, ns, 0, &e))
+    return fail(TBG_E_IO, "%s", e.c_str());
+  if (path) memcpy(path, file.c_str(), file.size() + 1);
+  return TBG_OK;
+}
+
 int tbg_get_codes(tbg_ctx *c, int stm, int slice, uint16_t *dst, uint64_t n)
 {
   if (!c || !dst || (stm & ~1)) return fail(TBG_E_INVAL, "bad argument");

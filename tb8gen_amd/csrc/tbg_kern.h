@@ -100,6 +100,30 @@ __global__ void k_resolve(const Mat *Mp, const SubRef *Rp, int c, int k, uint64_
 __global__ void k_code_plane(const uint16_t *codes, uint64_t size, uint64_t words, int nslices,
                              uint16_t code, uint64_t *out);
 
+// One sub-table (one side to move) of a Syzygy WDL file, the format
+// src/probe.c reads (encode, src/probe.c:603-687; set_enc_info, 725-770).
+// Header piece i stands on our slot slot[i].  The leading group (both
+// kings, or three unique pieces) is the most significant digit; group g
+// after it has digit (rem / gfactor[g]) % gsize[g], members gfirst[g] ..
+// gfirst[g] + gmult[g] - 1, ranked among the squares left free by the
+// leading group and the groups before g.
+struct SyzLayout {
+  uint64_t tb_size;
+  uint64_t lead_factor;
+  uint64_t gfactor[MAXP];
+  uint64_t gsize[MAXP];
+  int num, lead, ngroups, stm;
+  int gfirst[MAXP], gmult[MAXP];
+  int slot[MAXP];
+  uint8_t inv_tri[10], inv_diag[16], inv_lower[36];
+};
+
+// out[k] = WDL + 2 (0..4) of index i0 + k of sub-table L, or 0xff where the
+// position is broken (kings touch) or illegal: a value the file may choose
+// freely
+__global__ void k_syz_gather(const Mat *Mp, const uint16_t *codes, SyzLayout L, uint64_t i0,
+                             uint64_t n, uint8_t *out);
+
 // copies the shared tables to the device (defined with the kernels)
 hipError_t upload_tabs(const Tabs *h);
 

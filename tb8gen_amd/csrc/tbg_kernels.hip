@@ -1156,6 +1156,122 @@ __global__ __launch_bounds__(256) void k_resolve(const Mat *__restrict__ Mp,
   }
 }
 
+// ------------------------------------------------------- Syzygy WDL files
+// r-th free square (zero bit of occ) among the squares of `among`
+__device__ __forceinline__ int nth_free(uint64_t occ, uint64_t among, int r)
+{
+  uint64_t f = ~occ & among;
+  for (int i = 0; i < r; i++) f &= f - 1;
+  return ctz64(f);
+}
+
+// Squares of the leading group for its digit q: decode_helper's PIECE_ENC
+// branch (src/probe.c:884-946) for three unique pieces (the 31332
+// placements: first piece in the a1-d1-d4 triangle, off the diagonal, or
+// the later pieces' diagonal cases) or the two kings (slice numbering).
+__device__ uint64_t syz_lead(const Tabs &T, const SyzLayout &L, uint32_t q, uint64_t *occ)
+{
+  const uint64_t DIAG = 0x8040201008040201ull;
+  int p0, p1, p2 = 0;
+  if (L.lead == 2) {
+    p0 = T.kksq[q][0];
+    p1 = T.kksq[q][1];
+    *occ = bit(p0) | bit(p1);
+    return (uint64_t)p0 | ((uint64_t)p1 << 8);
+  }
+  const uint32_t A = 6 * 63 * 62, B = 4 * 28 * 62, C = 4 * 7 * 28;
+  if (q < A) {  // first piece off the diagonal
+    uint32_t r = q / (63 * 62);
+    q -= r * 63 * 62;
+    p0 = L.inv_tri[r];
+    r = q / 62;
+    q -= r * 62;
+    p1 = (int)r + ((int)r >= p0);
+    p2 = nth_free(bit(p0) | bit(p1), ~0ull, (int)q);
+  } else if (q < A + B) {  // first on the diagonal, second below it
+    q -= A;
+    uint32_t r = q / (28 * 62);
+    q -= r * 28 * 62;
+    p0 = L.inv_diag[r];
+    r = q / 62;
+    q -= r * 62;
+    p1 = L.inv_lower[r];
+    p2 = nth_free(bit(p0) | bit(p1), ~0ull, (int)q);
+  } else if (q < A + B + C) {  // first two on the diagonal, third below it
+    q -= A + B;
+    uint32_t r = q / (7 * 28);
+    q -= r * 7 * 28;
+    p0 = L.inv_diag[r];
+    r = q / 28;
+    q -= r * 28;
+    r += L.inv_diag[r] >= p0;
+    p1 = L.inv_diag[r];
+    p2 = L.inv_lower[q];
+  } else {  // all three on the diagonal
+    q -= A + B + C;
+    uint32_t r = q / 42;
+    q -= r * 42;
+    p0 = L.inv_diag[r];
+    r = q / 6;
+    q -= r * 6;
+    r += L.inv_diag[r] >= p0;
+    p1 = L.inv_diag[r];
+    p2 = nth_free(bit(p0) | bit(p1), DIAG, (int)q);
+  }
+  *occ = bit(p0) | bit(p1) | bit(p2);
+  return (uint64_t)p0 | ((uint64_t)p1 << 8) | ((uint64_t)p2 << 16);
+}
+
+__global__ __launch_bounds__(256) void k_syz_gather(const Mat *__restrict__ Mp,
+                                                     const uint16_t *__restrict__ codes,
+                                                     SyzLayout L, uint64_t i0, uint64_t n,
+                                                     uint8_t *out)
+{
+  __shared__ Tabs T;
+  load_tabs(T);
+  const Mat &M = *Mp;
+  for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+       k += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t I = i0 + k;
+    uint64_t lead = I / L.lead_factor, rem = I - lead * L.lead_factor;
+    uint64_t occ;
+    uint64_t h = syz_lead(T, L, (uint32_t)lead, &occ);  // header squares, packed
+    for (int g = 0; g < MAXP; g++) {
+      if (g >= L.ngroups) break;
+      uint32_t d = (uint32_t)((rem / L.gfactor[g]) % L.gsize[g]);
+      int f = L.gfirst[g], m = L.gmult[g];
+      uint64_t occ0 = occ;
+      // unrank_binomial (src/probe.c:809-830): member i is the largest r
+      // with C(r, i + 1) <= d, ranked among the squares free of occ0
+      for (int i = MAXS - 1; i > 0; i--) {
+        if (i >= m) continue;
+        int r = i;
+        while (r + 1 < 64 && T.binom[(i + 1) * 64 + r + 1] <= d) r++;
+        d -= T.binom[(i + 1) * 64 + r];
+        int s = nth_free(occ0, ~0ull, r);
+        h = sq_set(h, f + i, s);
+        occ |= bit(s);
+      }
+      int s = nth_free(occ0, ~0ull, (int)d);
+      h = sq_set(h, f, s);
+      occ |= bit(s);
+    }
+    uint64_t p = 0;
+    for (int i = 0; i < MAXP; i++) {
+      if (i >= L.num) break;
+      p = sq_set(p, L.slot[i], sq_get(h, i));
+    }
+    uint64_t q;
+    int s = normalize(T, p, &q);
+    uint8_t v = 0xff;
+    if (s >= 0) {
+      uint16_t c = codes[(uint64_t)s * M.ii.size + sq_to_idx(T, M.ii, q)];
+      if (c != 3) v = (uint8_t)(code_wdl(c) + 2);
+    }
+    out[k] = v;
+  }
+}
+
 }  // namespace tbg
 
 namespace tbg {
