@@ -39,6 +39,19 @@ __device__ __forceinline__ uint64_t gload(const uint64_t *p, uint64_t i)
   return ((cgu64 *)p)[i];
 }
 
+// a value that is the same in every lane, moved to a scalar register
+__device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ uint64_t uni64(uint64_t x)
+{
+  uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)x);
+  uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(x >> 32));
+  return (uint64_t)lo | ((uint64_t)hi << 32);
+}
+template <class X> __device__ __forceinline__ X *uniptr(X *p)
+{
+  return reinterpret_cast<X *>(uni64(reinterpret_cast<uint64_t>(p)));
+}
+
 __device__ __forceinline__ void atomic_or64(uint64_t *a, uint64_t v)
 {
   __hip_atomic_fetch_or((gull *)a, (unsigned long long)v, __ATOMIC_RELAXED,
@@ -658,44 +671,69 @@ __global__ __launch_bounds__(256, 4) void k_verify_j(JobDev *const *__restrict__
   int cached = -1;
   int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   uint64_t total = P[np];
+  // The (job, side) pair of the current unit and what is read from its job
+  // record, refreshed only when a unit leaves the pair's range: the record
+  // does not change during a launch, and a sparse ply then costs one
+  // candidate load per unit instead of a chain of dependent loads.  The
+  // plane pointers are uniform and kept in scalar registers.
+  uint64_t plo = 1, phi = 0;
+  bool skip = true, run = false, exits = false;
+  int stm = 0, n = 0;
+  JobDev *Jp = nullptr;
+  const uint64_t *candr = nullptr, *extra = nullptr, *wins_s = nullptr, *wins_o = nullptr,
+                 *sub_o = nullptr;
+  uint64_t *candw = nullptr, *newL = nullptr;
+  uint16_t *codes = nullptr;
   for (uint64_t u = blockIdx.x; u < total; u += gridDim.x) {
-    int pi = find_pair(P, np, u);
-    Pair pr = pairs[pi];
-    JobDev &J = *jobs[pr.job];
-    if (J.done) continue;  // uniform across the work-group
-    cache_mat(Ms, J.M, pr.job, &cached);
+    if (u < plo || u >= phi) {  // uniform across the work-group
+      int pi = find_pair(P, np, u);
+      plo = P[pi];
+      phi = P[pi + 1];
+      Pair pr = pairs[pi];
+      Jp = uniptr(jobs[pr.job]);
+      skip = uni(Jp->done) != 0;
+      if (!skip) {
+        cache_mat(Ms, Jp->M, pr.job, &cached);
+        stm = pr.side;
+        n = uni(Jp->n);
+        run = (uni(Jp->mask) >> stm) & 1;
+        exits = run && Ms.synth && n == 1;
+        int o = stm ^ 1;
+        candw = uniptr(Jp->cand[stm]);
+        candr = candw;
+        extra = uniptr(Jp->newW[stm]);
+        wins_s = uniptr(Jp->wins[stm]);
+        wins_o = uniptr(Jp->wins[o]);
+        sub_o = uniptr(Jp->sub[o][n <= DRAW_RULE ? TBG_SUB_WIN : TBG_SUB_CWIN]);
+        newL = uniptr(Jp->newL[stm]);
+        codes = uniptr(Jp->codes[stm]);
+      }
+    }
+    if (skip) continue;
     const Mat &M = Ms;
-    int stm = pr.side, o = stm ^ 1, n = J.n;
-    bool run = (J.mask >> stm) & 1;
-    bool exits = run && M.synth && n == 1;
     uint64_t within;
-    int s = unit_slice(M, sw0, u - P[pi], (M.words + uw - 1) / uw, &within);
+    int s = unit_slice(M, sw0, u - plo, (M.words + uw - 1) / uw, &within);
     uint64_t u0 = within * (uint64_t)uw;
     uint64_t uend = u0 + uw < M.words ? u0 + uw : M.words;
     uint64_t mysum = 0;
     // one 64-word chunk per wave per pass (the 4 waves cover 256 words); a
-    // chunk without candidates costs one load and one store per lane.  The
-    // plane pointers are read from the job where used rather than held
-    // across the verification loop, which keeps the register count (and so
-    // the occupancy) of this latency-bound kernel where it is.
+    // chunk without candidates costs one load and one store per lane
     for (uint64_t wbase = u0 + (uint64_t)wave * 64; wbase < uend; wbase += 256) {
       uint64_t w = wbase + lane;
       bool live = w < uend;
       uint64_t a = phys(M, s) * M.words + w;
       uint64_t x0 = 0, x = 0;
       if (live) {
-        x0 = gload(J.cand[stm], a);
+        x0 = gload(candr, a);
         uint64_t c = x0;
-        if (exits) c |= gload(J.newW[stm], a);
-        x = run && c ? c & ~gload(J.wins[stm], a) : 0;
+        if (exits) c |= gload(extra, a);
+        x = run && c ? c & ~gload(wins_s, a) : 0;
       }
       uint64_t r = 0;
       if (__ballot(x != 0)) {
         Q.res[wave][lane] = x;
         int tot;
         int off = wave_excl_scan(popc64(x), lane, &tot);
-        const uint64_t *wins_o = J.wins[o];
-        const uint64_t *sub_o = J.sub[o][n <= DRAW_RULE ? TBG_SUB_WIN : TBG_SUB_CWIN];
         for (int base = 0; base < tot; base += QCAP) {
           fill_round(Q.e[wave], x, off, base, lane);
           wave_sync();
@@ -715,9 +753,8 @@ __global__ __launch_bounds__(256, 4) void k_verify_j(JobDev *const *__restrict__
         wave_sync();  // the next chunk overwrites Q.res
       }
       if (live) {
-        J.newL[stm][a] = r;
-        if (x0) J.cand[stm][a] = 0;
-        uint16_t *codes = J.codes[stm];
+        newL[a] = r;
+        if (x0) candw[a] = 0;
         if (codes && r) {
           uint16_t *c = codes + phys(M, s) * M.ii.size + w * 64;
           uint16_t code = (uint16_t)((n << 2) | 2);
@@ -732,7 +769,7 @@ __global__ __launch_bounds__(256, 4) void k_verify_j(JobDev *const *__restrict__
       mysum += popc64(r);
     }
     uint64_t sum = wave_sum64(mysum);
-    if (lane == 0 && sum) atomicAdd(&J.cnt[stm], (unsigned long long)sum);
+    if (lane == 0 && sum) atomicAdd(&Jp->cnt[stm], (unsigned long long)sum);
   }
 }
 
