@@ -327,33 +327,52 @@ __global__ __launch_bounds__(512) void k_pred_j(JobDev *const *__restrict__ jobs
   int cached = -1;
   int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   uint64_t total = P[np];
+  // The (job, side) pair of the current unit and what is read from its job
+  // record, refreshed only when a unit leaves the pair's range (the record
+  // does not change during a launch); uniform, so held in scalar registers.
+  // A job field read in the loops below would be reloaded after every
+  // atomic, waiting for all of them.
+  uint64_t plo = 1, phi = 0;
+  bool en = false, lds = false, comb = false;
+  int stm = 0, blk_set = 0;
+  uint64_t nblk = 1, blk_words = 0, blk_pos = 0;
+  const uint64_t *src = nullptr, *won = nullptr;
+  uint64_t *cand = nullptr;
   for (uint64_t u = blockIdx.x; u < total; u += gridDim.x) {
-    int pi = find_pair(P, np, u);
-    Pair pr = pairs[pi];
-    const JobDev &J = *jobs[pr.job];
-    int stm = pr.side, o = stm ^ 1;
-    bool en = !J.done && ((J.mask >> (kind * 2 + stm)) & 1);
-    if (kind == 0) en = en && J.n > 1 && ((J.more >> (2 + stm)) & 1);  // more_lw / more_lb
-    else en = en && ((J.more >> stm) & 1);                            // more_ww / more_wb
-    if (!en) continue;  // uniform across the work-group
-    cache_mat(Ms, J.M, pr.job, &cached);
+    if (u < plo || u >= phi) {  // uniform across the work-group
+      int pi = find_pair(P, np, u);
+      plo = P[pi];
+      phi = P[pi + 1];
+      Pair pr = pairs[pi];
+      const JobDev *Jp = uniptr(jobs[pr.job]);
+      stm = pr.side;
+      int o = stm ^ 1, more = uni(Jp->more);
+      en = !uni(Jp->done) && ((uni(Jp->mask) >> (kind * 2 + stm)) & 1);
+      if (kind == 0) en = en && uni(Jp->n) > 1 && ((more >> (2 + stm)) & 1);  // more_lw / more_lb
+      else en = en && ((more >> stm) & 1);                                   // more_ww / more_wb
+      if (en) {
+        cache_mat(Ms, Jp->M, pr.job, &cached);
+        nblk = uni64(Jp->nblk);
+        blk_words = uni64(Jp->blk_words);
+        blk_pos = uni64(Jp->blk_pos);
+        int sc = uni(Jp->scatter);
+        lds = (sc & TBG_SCATTER_BLOCK) != 0;
+        comb = (sc & TBG_SCATTER_COMBINE) != 0;
+        blk_set = uni(Jp->blk_set);
+        src = uniptr(kind == 0 ? Jp->prevW[o] : Jp->prevL[o]);
+        cand = uniptr(Jp->cand[stm]);
+        won = uniptr(Jp->wins[stm]);
+      }
+    }
+    if (!en) continue;
     const Mat &M = Ms;
     uint64_t bi;
-    int s = unit_slice(M, sw0, u - P[pi], J.nblk, &bi);
-    uint64_t blk_words = J.blk_words;
-    bool lds = (J.scatter & TBG_SCATTER_BLOCK) != 0;
-    bool comb = (J.scatter & TBG_SCATTER_COMBINE) != 0;
-    // read once per unit: a job field read in the loops below would be
-    // reloaded after every atomic, waiting for all of them
-    const int blk_set = J.blk_set;
+    int s = unit_slice(M, sw0, u - plo, nblk, &bi);
     // LDS: [block bitmap (block scatter only)][8 wave queues]
     uint64_t *B = dyn;
     uint16_t *q = reinterpret_cast<uint16_t *>(dyn + (lds ? blk_words : 0)) + wave * PB_CAP;
-    const uint64_t *src = kind == 0 ? J.prevW[o] : J.prevL[o];
-    uint64_t *cand = J.cand[stm];
-    const uint64_t *won = J.wins[stm];
-    uint64_t lo = bi * J.blk_pos;
-    uint64_t hi = lo + J.blk_pos < M.ii.size ? lo + J.blk_pos : M.ii.size;
+    uint64_t lo = bi * blk_pos;
+    uint64_t hi = lo + blk_pos < M.ii.size ? lo + blk_pos : M.ii.size;
     uint64_t wlo = lo >> 6, nbw = ((hi + 63) >> 6) - wlo;
     const uint64_t *srcs = src + phys(M, s) * M.words;
     uint64_t *own = cand + phys(M, s) * M.words;
