@@ -245,6 +245,7 @@ struct LaunchSet {
   int np = 0;
   size_t cap = 0;  // pairs the device buffers hold
   size_t pred_lds = 0;
+  bool pred_wide = false;  // some job scatters high-set unmoves to device memory
   Geom g;
   void release()
   {
@@ -341,9 +342,11 @@ int build_launchset(LaunchSet &L, const std::vector<const JobDev *> &hjobs,
     L.total[kt] = P[kt][pairs.size()];
   }
   bool block = false;
+  L.pred_wide = false;
   for (const JobDev *j : hjobs) {
     maxbw = j->blk_words > maxbw ? j->blk_words : maxbw;
     block |= (j->scatter & TBG_SCATTER_BLOCK) != 0;
+    L.pred_wide |= !(j->scatter & TBG_SCATTER_BLOCK) || j->blk_set > 0;
   }
   // the queues follow the block bitmap, which only the block scatter uses
   L.pred_lds = (block ? maxbw * 8 : 0) + PB_QUEUE_BYTES;
@@ -546,8 +549,12 @@ int launch(tbg_ctx *prof, JobDev *const *jobs, const LaunchSet &L, int kt, int k
   switch (kt) {
   case KT_PRED: {
     Timed t(prof, CAT_PRED);
-    hipLaunchKernelGGL(k_pred_j, dim3(cap_grid(total, (uint64_t)g_cus * L.g.wg[kt])), dim3(PB_THREADS),
-                       L.pred_lds, st, jobs, L.pairs, L.P[kt], L.np, sw0, kind);
+    if (L.pred_wide)
+      hipLaunchKernelGGL(k_pred_wide_j, dim3(cap_grid(total, (uint64_t)g_cus * L.g.wg[kt])),
+                         dim3(PB_THREADS), L.pred_lds, st, jobs, L.pairs, L.P[kt], L.np, sw0, kind);
+    else
+      hipLaunchKernelGGL(k_pred_j, dim3(cap_grid(total, (uint64_t)g_cus * L.g.wg[kt])),
+                         dim3(PB_THREADS), L.pred_lds, st, jobs, L.pairs, L.P[kt], L.np, sw0, kind);
     break;
   }
   case KT_UNCAP: {

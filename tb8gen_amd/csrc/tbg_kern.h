@@ -79,6 +79,8 @@ constexpr int PB_QUEUE_BYTES = 8 * 1024 * 2;
 
 __global__ void k_pred_j(JobDev *const *jobs, const Pair *pairs, const uint64_t *P, int np, int sw0,
                          int kind);
+__global__ void k_pred_wide_j(JobDev *const *jobs, const Pair *pairs, const uint64_t *P, int np,
+                              int sw0, int kind);
 __global__ void k_uncap_j(JobDev *const *jobs, const Pair *pairs, const uint64_t *P, int np, int sw0,
                           int kind, const unsigned *flags);
 __global__ void k_verify_j(JobDev *const *jobs, const Pair *pairs, const uint64_t *P, int np, int sw0,

@@ -113,7 +113,7 @@ __device__ __forceinline__ void mark4(uint64_t *plane, const uint64_t *__restric
 // idx: the position's index in slice s, or ~0 when unknown (an uncapture's
 // constructed position); with it, a king move that needs no symmetry is
 // indexed incrementally (king_moved_idx).
-__device__ void mark_king_unmoves(const Tabs &T, const Mat &M, int stm, uint64_t occ, uint64_t p,
+__device__ __forceinline__ void mark_king_unmoves(const Tabs &T, const Mat &M, int stm, uint64_t occ, uint64_t p,
                                   int s, uint64_t *cand, const uint64_t *__restrict__ won,
                                   uint64_t idx)
 {
@@ -312,6 +312,11 @@ __device__ __forceinline__ uint64_t block_word(const uint64_t *srcs, uint64_t wl
 // with the global scatter) use 64-bit device atomics.  Device-scope atomics
 // on gfx950 travel to the memory side, so privatising cuts the scatter's HBM
 // write traffic by the share of same-block unmoves.
+//
+// Two register budgets: k_pred_j (no spills, 4 waves per SIMD) when every
+// unmove but the king's lands in LDS; k_pred_wide_j (80 VGPRs, a few
+// spilled, 6 waves per SIMD) when pieces of high sets scatter to device
+// memory, whose atomics' latency more resident waves hide.
 constexpr int PB_WAVES = 8;   // waves per work-group
 constexpr int PB_CAP = QCAP;  // queue entries per wave per round
 
@@ -323,158 +328,18 @@ __global__ __launch_bounds__(512) void k_pred_j(JobDev *const *__restrict__ jobs
   __shared__ Tabs T;
   __shared__ Mat Ms;
   extern __shared__ uint64_t dyn[];
-  load_tabs(T);
-  int cached = -1;
-  int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  uint64_t total = P[np];
-  // The (job, side) pair of the current unit and what is read from its job
-  // record, refreshed only when a unit leaves the pair's range (the record
-  // does not change during a launch); uniform, so held in scalar registers.
-  // A job field read in the loops below would be reloaded after every
-  // atomic, waiting for all of them.
-  uint64_t plo = 1, phi = 0;
-  bool en = false, lds = false, comb = false;
-  int stm = 0, blk_set = 0;
-  uint64_t nblk = 1, blk_words = 0, blk_pos = 0;
-  const uint64_t *src = nullptr, *won = nullptr;
-  uint64_t *cand = nullptr;
-  for (uint64_t u = blockIdx.x; u < total; u += gridDim.x) {
-    if (u < plo || u >= phi) {  // uniform across the work-group
-      int pi = find_pair(P, np, u);
-      plo = P[pi];
-      phi = P[pi + 1];
-      Pair pr = pairs[pi];
-      const JobDev *Jp = uniptr(jobs[pr.job]);
-      stm = pr.side;
-      int o = stm ^ 1, more = uni(Jp->more);
-      en = !uni(Jp->done) && ((uni(Jp->mask) >> (kind * 2 + stm)) & 1);
-      if (kind == 0) en = en && uni(Jp->n) > 1 && ((more >> (2 + stm)) & 1);  // more_lw / more_lb
-      else en = en && ((more >> stm) & 1);                                   // more_ww / more_wb
-      if (en) {
-        cache_mat(Ms, Jp->M, pr.job, &cached);
-        nblk = uni64(Jp->nblk);
-        blk_words = uni64(Jp->blk_words);
-        blk_pos = uni64(Jp->blk_pos);
-        int sc = uni(Jp->scatter);
-        lds = (sc & TBG_SCATTER_BLOCK) != 0;
-        comb = (sc & TBG_SCATTER_COMBINE) != 0;
-        blk_set = uni(Jp->blk_set);
-        src = uniptr(kind == 0 ? Jp->prevW[o] : Jp->prevL[o]);
-        cand = uniptr(Jp->cand[stm]);
-        won = uniptr(Jp->wins[stm]);
-      }
-    }
-    if (!en) continue;
-    const Mat &M = Ms;
-    uint64_t bi;
-    int s = unit_slice(M, sw0, u - plo, nblk, &bi);
-    // LDS: [block bitmap (block scatter only)][8 wave queues]
-    uint64_t *B = dyn;
-    uint16_t *q = reinterpret_cast<uint16_t *>(dyn + (lds ? blk_words : 0)) + wave * PB_CAP;
-    uint64_t lo = bi * blk_pos;
-    uint64_t hi = lo + blk_pos < M.ii.size ? lo + blk_pos : M.ii.size;
-    uint64_t wlo = lo >> 6, nbw = ((hi + 63) >> 6) - wlo;
-    const uint64_t *srcs = src + phys(M, s) * M.words;
-    uint64_t *own = cand + phys(M, s) * M.words;
-    const uint64_t *ownw = won + phys(M, s) * M.words;
-    // a block with no frontier bit is skipped whole (sparse plies); the
-    // words read here are found in cache by the per-wave pass
-    int nz = 0;
-    for (uint64_t wi = threadIdx.x; wi < nbw; wi += blockDim.x)
-      nz |= block_word(srcs, wlo, wi, nbw, lo, hi) != 0;
-    if (!__syncthreads_or(nz)) continue;
-    if (lds)
-      for (uint64_t w = threadIdx.x; w < nbw; w += blockDim.x) B[w] = 0;
-    __syncthreads();
-    // each wave walks its own 64-word chunks with a private queue: no
-    // work-group barrier until the merge
-    for (uint64_t c0 = (uint64_t)wave * 64; c0 < nbw; c0 += (uint64_t)PB_WAVES * 64) {
-      uint64_t wi = c0 + lane;
-      uint64_t x = wi < nbw ? block_word(srcs, wlo, wi, nbw, lo, hi) : 0;
-      if (!__ballot(x != 0)) continue;
-      int tot;
-      int off = wave_excl_scan(popc64(x), lane, &tot);
-      for (int base = 0; base < tot; base += QCAP) {
-        fill_round(q, x, off, base, lane);
-        wave_sync();
-        int cnt = tot - base < QCAP ? tot - base : QCAP;
-        // all lanes step through the entries together (the combining
-        // scatter works on whole waves); lanes past the end hold nothing
-        for (int e0 = 0; e0 < cnt; e0 += 64) {
-          int e = e0 + lane;
-          bool ok = e < cnt;
-          uint64_t idx = 0, occ = 0, p = kings_of(T, s);
-          if (ok) {
-            uint16_t v = q[e];
-            idx = (wlo + c0 + (v >> 6)) * 64 + (v & 63);
-            occ = idx_to_sq(T, M.ii, idx, &p);
-          }
-          if (comb) king_unmoves_wave(T, M, stm, ok, occ, p, s, cand, won, lane, idx);
-          else if (ok) mark_king_unmoves(T, M, stm, occ, p, s, cand, won, idx);
-          for (int t = 0; t < MAXP - 1; t++) {
-            if (t >= M.nk[stm]) break;
-            int j = M.pcs[stm][t];
-            int a = sq_get(p, j);
-            uint64_t bb = ok ? piece_att(T, M.pt[j], a, occ) & ~occ : 0;
-            uint64_t ob = 0;
-            uint32_t ra = 0;
-            if (M.all_single) {
-              ob = occ_before(p, j - 2);
-              ra = (uint32_t)(a - popc64(ob & (bit(a) - 1)));
-            }
-            if (lds && sq_get(M.set_of, j) >= blk_set) {
-              // the piece's set lies inside the block: every target is in LDS
-              while (bb) {
-                int tq = ctz64(bb);
-                bb &= bb - 1;
-                uint64_t i2 = M.all_single ? moved_idx(M.ii, idx, p, j - 2, a, tq, ob, ra)
-                                           : sq_to_idx(T, M.ii, sq_set(p, j, tq));
-                __hip_atomic_fetch_or(reinterpret_cast<unsigned long long *>(&B[(i2 >> 6) - wlo]),
-                                      (unsigned long long)bit(i2 & 63), __ATOMIC_RELAXED,
-                                      __HIP_MEMORY_SCOPE_WORKGROUP);
-              }
-            } else if (comb) {
-              // device atomics combined per word across the wave, one
-              // target per lane per round
-              while (__ballot(bb != 0)) {
-                uint64_t g = ~0ull;
-                if (bb) {
-                  int tq = ctz64(bb);
-                  bb &= bb - 1;
-                  g = M.all_single ? moved_idx(M.ii, idx, p, j - 2, a, tq, ob, ra)
-                                   : sq_to_idx(T, M.ii, sq_set(p, j, tq));
-                }
-                wave_mark(own, ownw, g, lane);
-              }
-            } else {
-              while (bb) {  // device atomics, targets in groups of 4
-                uint64_t g[4];
-#pragma unroll
-                for (int k = 0; k < 4; k++) {
-                  g[k] = ~0ull;
-                  if (bb) {
-                    int tq = ctz64(bb);
-                    bb &= bb - 1;
-                    g[k] = M.all_single ? moved_idx(M.ii, idx, p, j - 2, a, tq, ob, ra)
-                                        : sq_to_idx(T, M.ii, sq_set(p, j, tq));
-                  }
-                }
-                mark4(own, ownw, g);
-              }
-            }
-          }
-        }
-        wave_sync();
-      }
-    }
-    __syncthreads();
-    if (lds)
-      for (uint64_t w = threadIdx.x; w < nbw; w += blockDim.x) {
-        uint64_t v = B[w];
-        if (v) atomic_or64(own + wlo + w, v);
-      }
-    __syncthreads();
-  }
+#include "tbg_pred.inc"
+}
+
+__global__ __launch_bounds__(512, 6) void k_pred_wide_j(JobDev *const *__restrict__ jobs,
+                                                        const Pair *__restrict__ pairs,
+                                                        const uint64_t *__restrict__ P, int np,
+                                                        int sw0, int kind)
+{
+  __shared__ Tabs T;
+  __shared__ Mat Ms;
+  extern __shared__ uint64_t dyn[];
+#include "tbg_pred.inc"
 }
 
 // predecessors_sub_worker (src/tb8gen.c:346-414) over every (job, side):
