@@ -246,6 +246,7 @@ struct LaunchSet {
   size_t cap = 0;  // pairs the device buffers hold
   size_t pred_lds = 0;
   bool pred_wide = false;  // some job scatters high-set unmoves to device memory
+  uint64_t uncap_lds = 0;  // words of the largest whole-slice LDS bitmap of the uncapture
   Geom g;
   void release()
   {
@@ -347,6 +348,14 @@ int build_launchset(LaunchSet &L, const std::vector<const JobDev *> &hjobs,
     maxbw = j->blk_words > maxbw ? j->blk_words : maxbw;
     block |= (j->scatter & TBG_SCATTER_BLOCK) != 0;
     L.pred_wide |= !(j->scatter & TBG_SCATTER_BLOCK) || j->blk_set > 0;
+  }
+  // the uncapture privatises slices of up to 40 KB (those pred handles as
+  // one block)
+  L.uncap_lds = 0;
+  for (size_t i = 0; i < hjobs.size(); i++) {
+    uint64_t nbw = (mats[i]->ii.size + 63) >> 6;
+    if ((hjobs[i]->scatter & TBG_SCATTER_BLOCK) && hjobs[i]->nblk == 1 && nbw > L.uncap_lds)
+      L.uncap_lds = nbw;
   }
   // the queues follow the block bitmap, which only the block scatter uses
   L.pred_lds = (block ? maxbw * 8 : 0) + PB_QUEUE_BYTES;
@@ -559,8 +568,9 @@ int launch(tbg_ctx *prof, JobDev *const *jobs, const LaunchSet &L, int kt, int k
   }
   case KT_UNCAP: {
     Timed t(prof, CAT_UNCAP);
-    hipLaunchKernelGGL(k_uncap_j, dim3(cap_grid(total, (uint64_t)g_cus * L.g.wg[kt])), dim3(256), 0, st,
-                       jobs, L.pairs, L.P[kt], L.np, sw0, kind, flags);
+    hipLaunchKernelGGL(k_uncap_j, dim3(cap_grid(total, (uint64_t)g_cus * L.g.wg[kt])), dim3(256),
+                       L.uncap_lds * 8, st, jobs, L.pairs, L.P[kt], L.np, sw0, kind, flags,
+                       L.uncap_lds);
     break;
   }
   case KT_VERIFY: {

@@ -82,7 +82,7 @@ __global__ void k_pred_j(JobDev *const *jobs, const Pair *pairs, const uint64_t 
 __global__ void k_pred_wide_j(JobDev *const *jobs, const Pair *pairs, const uint64_t *P, int np,
                               int sw0, int kind);
 __global__ void k_uncap_j(JobDev *const *jobs, const Pair *pairs, const uint64_t *P, int np, int sw0,
-                          int kind, const unsigned *flags);
+                          int kind, const unsigned *flags, uint64_t lds_words);
 __global__ void k_verify_j(JobDev *const *jobs, const Pair *pairs, const uint64_t *P, int np, int sw0,
                            int uw);
 __global__ void k_sweep_j(JobDev *const *jobs, const Pair *pairs, const uint64_t *P, int np, int sw0,

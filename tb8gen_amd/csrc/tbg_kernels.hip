@@ -346,14 +346,21 @@ __global__ __launch_bounds__(512, 6) void k_pred_wide_j(JobDev *const *__restric
 // uncapture from the stm^1 capture-exit bitmaps, at n == 1 (L: sub/win with
 // the legality filter; W: sub/loss) and n == 101 (L: sub/cwin & ~sub/win;
 // W: sub/bloss).
+// With the block scatter, a job whose whole slice bitmap fits in lds_words
+// (the dynamic LDS of the launch) OR-s the non-king uncaptures of a unit
+// into an LDS copy of the slice and merges it with one device atomic per
+// nonzero word, as pred's block scatter does; king uncaptures, and every
+// uncapture of larger slices, use device atomics.
 __global__ __launch_bounds__(256) void k_uncap_j(JobDev *const *__restrict__ jobs,
                                                   const Pair *__restrict__ pairs,
                                                   const uint64_t *__restrict__ P, int np, int sw0,
-                                                  int kind, const unsigned *__restrict__ flags)
+                                                  int kind, const unsigned *__restrict__ flags,
+                                                  uint64_t lds_words)
 {
   __shared__ Tabs T;
   __shared__ Queue Q;
   __shared__ Mat Ms;
+  extern __shared__ uint64_t dyn[];
   if (flags && !flags[1]) return;  // no job is at ply 1 or 101 (uniform)
   load_tabs(T);
   int cached = -1;
@@ -386,8 +393,13 @@ __global__ __launch_bounds__(256) void k_uncap_j(JobDev *const *__restrict__ job
     } else {
       src0 = J.sub[o][n == 1 ? TBG_SUB_LOSS : TBG_SUB_BLOSS];
     }
-    uint64_t wbase = within * UNIT_WORDS + (uint64_t)wave * 64;
-    uint64_t w = wbase + lane;
+    // a wave per 64 words; a capture space of fewer than 256 words has its
+    // words interleaved over the 4 waves instead (word = base + 4 * lane +
+    // wave), so that every wave has work
+    uint64_t ubase = within * UNIT_WORDS;
+    bool il = M.cwords[k] < UNIT_WORDS;
+    uint64_t wstep = il ? 4 : 1, woff = il ? (uint64_t)wave : (uint64_t)wave * 64;
+    uint64_t w = ubase + (uint64_t)lane * wstep + woff;
     uint64_t x = 0;
     if (w < M.cwords[k]) {
       uint64_t a = phys(M, s) * M.ctotal[o] + M.coff[k] + w;
@@ -402,13 +414,21 @@ __global__ __launch_bounds__(256) void k_uncap_j(JobDev *const *__restrict__ job
     const uint64_t *won = J.wins[stm];
     uint64_t *own = cand + phys(M, s) * M.words;
     const uint64_t *ownw = won + phys(M, s) * M.words;
+    uint64_t nbw = (M.ii.size + 63) >> 6;
+    bool lds = (J.scatter & TBG_SCATTER_BLOCK) && nbw <= lds_words;  // uniform
+    uint64_t *B = dyn;
+    if (lds) {
+      if (!__syncthreads_or(tot != 0)) continue;  // nothing to uncapture in this unit
+      for (uint64_t i = threadIdx.x; i < nbw; i += blockDim.x) B[i] = 0;
+      __syncthreads();
+    }
     for (int base = 0; base < tot; base += QCAP) {
       fill_round(Q.e[wave], x, off, base, lane);
       wave_sync();
       int cnt = tot - base < QCAP ? tot - base : QCAP;
       for (int e = lane; e < cnt; e += 64) {
         uint16_t v = Q.e[wave][e];
-        uint64_t idx = (wbase + (v >> 6)) * 64 + (v & 63);
+        uint64_t idx = (ubase + (uint64_t)(v >> 6) * wstep + woff) * 64 + (v & 63);
         if (idx >= ci.size) continue;
         uint64_t p = kings_of(T, s);
         uint64_t occ = idx_to_sq(T, ci, idx, &p);
@@ -419,10 +439,31 @@ __global__ __launch_bounds__(256) void k_uncap_j(JobDev *const *__restrict__ job
         for (int t = 0; t < MAXP - 1; t++) {
           if (t >= M.nk[stm]) break;
           int j = M.pcs[stm][t];
-          mark_unmoves(T, M, j, occ, sq_set(p, m, sq_get(p, j)), own, ownw);
+          uint64_t q = sq_set(p, m, sq_get(p, j));
+          if (lds) {
+            uint64_t b = piece_att(T, M.pt[j], sq_get(q, j), occ) & ~occ;
+            while (b) {
+              int tq = ctz64(b);
+              b &= b - 1;
+              uint64_t i2 = sq_to_idx(T, M.ii, sq_set(q, j, tq));
+              __hip_atomic_fetch_or(reinterpret_cast<unsigned long long *>(&B[i2 >> 6]),
+                                    (unsigned long long)bit(i2 & 63), __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+          } else {
+            mark_unmoves(T, M, j, occ, q, own, ownw);
+          }
         }
       }
       wave_sync();
+    }
+    if (lds) {
+      __syncthreads();
+      for (uint64_t i = threadIdx.x; i < nbw; i += blockDim.x) {
+        uint64_t v = B[i];
+        if (v) atomic_or64(own + i, v);
+      }
+      __syncthreads();
     }
   }
 }
