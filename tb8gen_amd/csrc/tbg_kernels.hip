@@ -523,19 +523,31 @@ __device__ bool verify_pos(const Tabs &T, const Mat &M, int stm, int s, uint64_t
       if (!bm_test(sub_s + M.coff[l], sq_to_idx(T, M.capt[l], q))) return false;
     }
     uint64_t qm = a & ~occ;
+    // quiet moves four at a time: the four bitmap reads are in flight
+    // together instead of one round trip after another
+    uint64_t ob = 0;
+    uint32_t ra = 0;
     if (M.all_single) {
-      uint64_t ob = occ_before(p, j - 2);
-      uint32_t ra = (uint32_t)(fr - popc64(ob & (bit(fr) - 1)));
-      while (qm) {
-        int to = ctz64(qm);
-        qm &= qm - 1;
-        if (!bm_test(wins_s, moved_idx(M.ii, idx, p, j - 2, fr, to, ob, ra))) return false;
-      }
+      ob = occ_before(p, j - 2);
+      ra = (uint32_t)(fr - popc64(ob & (bit(fr) - 1)));
     }
     while (qm) {
-      int to = ctz64(qm);
-      qm &= qm - 1;
-      if (!bm_test(wins_s, sq_to_idx(T, M.ii, sq_set(p, j, to)))) return false;
+      uint64_t i4[4], w4[4];
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        i4[k] = ~0ull;
+        if (qm) {
+          int to = ctz64(qm);
+          qm &= qm - 1;
+          i4[k] = M.all_single ? moved_idx(M.ii, idx, p, j - 2, fr, to, ob, ra)
+                               : sq_to_idx(T, M.ii, sq_set(p, j, to));
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < 4; k++) w4[k] = i4[k] != ~0ull ? gload(wins_s, i4[k] >> 6) : ~0ull;
+#pragma unroll
+      for (int k = 0; k < 4; k++)
+        if (!((w4[k] >> (i4[k] & 63)) & 1)) return false;
     }
   }
   uint64_t kb = T.king[ks] & ~T.king[oks];
