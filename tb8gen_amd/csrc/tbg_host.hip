@@ -247,6 +247,7 @@ struct LaunchSet {
   size_t pred_lds = 0;
   bool pred_wide = false;  // some job scatters high-set unmoves to device memory
   uint64_t uncap_lds = 0;  // words of the largest whole-slice LDS bitmap of the uncapture
+  bool verify_split = false;  // some job's slices take two-pass verification
   Geom g;
   void release()
   {
@@ -352,7 +353,9 @@ int build_launchset(LaunchSet &L, const std::vector<const JobDev *> &hjobs,
   // the uncapture privatises slices of up to 40 KB (those pred handles as
   // one block)
   L.uncap_lds = 0;
+  L.verify_split = false;
   for (size_t i = 0; i < hjobs.size(); i++) {
+    L.verify_split |= mats[i]->ii.size > VERIFY_SPLIT_SIZE;
     uint64_t nbw = (mats[i]->ii.size + 63) >> 6;
     if ((hjobs[i]->scatter & TBG_SCATTER_BLOCK) && hjobs[i]->nblk == 1 && nbw > L.uncap_lds)
       L.uncap_lds = nbw;
@@ -575,8 +578,12 @@ int launch(tbg_ctx *prof, JobDev *const *jobs, const LaunchSet &L, int kt, int k
   }
   case KT_VERIFY: {
     Timed t(prof, CAT_VERIFY);
-    hipLaunchKernelGGL(k_verify_j, dim3(cap_grid(total, (uint64_t)g_cus * L.g.wg[kt])), dim3(256), 0, st,
-                       jobs, L.pairs, L.P[kt], L.np, sw0, (int)L.g.verify_uw);
+    if (L.verify_split)
+      hipLaunchKernelGGL(k_verify_j<true>, dim3(cap_grid(total, (uint64_t)g_cus * L.g.wg[kt])),
+                         dim3(256), 0, st, jobs, L.pairs, L.P[kt], L.np, sw0, (int)L.g.verify_uw);
+    else
+      hipLaunchKernelGGL(k_verify_j<false>, dim3(cap_grid(total, (uint64_t)g_cus * L.g.wg[kt])),
+                         dim3(256), 0, st, jobs, L.pairs, L.P[kt], L.np, sw0, (int)L.g.verify_uw);
     break;
   }
   default: {

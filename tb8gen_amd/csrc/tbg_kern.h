@@ -83,8 +83,10 @@ __global__ void k_pred_wide_j(JobDev *const *jobs, const Pair *pairs, const uint
                               int sw0, int kind);
 __global__ void k_uncap_j(JobDev *const *jobs, const Pair *pairs, const uint64_t *P, int np, int sw0,
                           int kind, const unsigned *flags, uint64_t lds_words);
-__global__ void k_verify_j(JobDev *const *jobs, const Pair *pairs, const uint64_t *P, int np, int sw0,
-                           int uw);
+constexpr uint64_t VERIFY_SPLIT_SIZE = 1ull << 22;  // positions per slice
+template <bool SPLIT>
+__global__ __launch_bounds__(256, 4) void k_verify_j(JobDev *const *jobs, const Pair *pairs,
+                                                   const uint64_t *P, int np, int sw0, int uw);
 __global__ void k_sweep_j(JobDev *const *jobs, const Pair *pairs, const uint64_t *P, int np, int sw0,
                           int uw);
 // flags[0]: jobs done; flags[1]: some job is at ply 1 or 101 (uncapture)

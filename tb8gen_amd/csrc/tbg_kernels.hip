@@ -229,10 +229,17 @@ __device__ __forceinline__ void mark_unmoves(const Tabs &T, const Mat &M, int j,
 
 // ----------------------------------------------------------- frontier
 constexpr int QCAP = 1024;  // queue entries per wave per round
+constexpr int VERIFY_QUICK = 4;    // bitmap reads of verification's first pass
+constexpr int VERIFY_SPLIT = 256;  // candidates per round worth two passes
+// ... in slices of more positions than VERIFY_SPLIT_SIZE (tbg_kern.h): in
+// smaller tables the bitmaps stay in the caches, their reads are cheap, and
+// the second decode of the open candidates costs more than the idle lanes
+// it saves
 
 struct Queue {
   uint16_t e[4][QCAP];
   uint64_t res[4][64];
+  int n[4];  // verification: candidates left open by the first pass
 };
 
 // LDS written by some lanes of a wave becomes visible to the other lanes
@@ -493,9 +500,13 @@ __device__ int synth_exit(const Tabs &T, const Mat &M, int stm, uint64_t q0, uin
 // the reference is never needed here; each piece's attack set is computed
 // only when its moves are reached (most candidates fail at their first
 // move).  Captures look up the captured piece's set from its slot.
-__device__ bool verify_pos(const Tabs &T, const Mat &M, int stm, int s, uint64_t idx,
-                           const uint64_t *__restrict__ wins_o, const uint64_t *__restrict__ sub_o)
+// Returns 0 (not lost), 1 (lost), or 2 when `budget` bitmap reads were
+// spent without an answer.
+__device__ __forceinline__ int verify_pos(const Tabs &T, const Mat &M, int stm, int s, uint64_t idx,
+                          const uint64_t *__restrict__ wins_o, const uint64_t *__restrict__ sub_o,
+                          int budget)
 {
+  int used = 0;  // bitmap reads so far; at budget the answer is left open (2)
   int o = stm ^ 1;
   uint64_t p = kings_of(T, s);
   uint64_t occ = idx_to_sq(T, M.ii, idx, &p);
@@ -520,7 +531,8 @@ __device__ bool verify_pos(const Tabs &T, const Mat &M, int stm, int s, uint64_t
       int l = sq_get(M.set_of, jj);
       uint64_t q = sq_set(p, j, to);
       q = sq_set(q, jj, sq_get(q, M.ii.last[l]));
-      if (!bm_test(sub_s + M.coff[l], sq_to_idx(T, M.capt[l], q))) return false;
+      if (used++ >= budget) return 2;
+      if (!bm_test(sub_s + M.coff[l], sq_to_idx(T, M.capt[l], q))) return 0;
     }
     uint64_t qm = a & ~occ;
     // quiet moves four at a time: the four bitmap reads are in flight
@@ -532,6 +544,8 @@ __device__ bool verify_pos(const Tabs &T, const Mat &M, int stm, int s, uint64_t
       ra = (uint32_t)(fr - popc64(ob & (bit(fr) - 1)));
     }
     while (qm) {
+      if (used >= budget) return 2;
+      used += 4;
       uint64_t i4[4], w4[4];
 #pragma unroll
       for (int k = 0; k < 4; k++) {
@@ -547,7 +561,7 @@ __device__ bool verify_pos(const Tabs &T, const Mat &M, int stm, int s, uint64_t
       for (int k = 0; k < 4; k++) w4[k] = i4[k] != ~0ull ? gload(wins_s, i4[k] >> 6) : ~0ull;
 #pragma unroll
       for (int k = 0; k < 4; k++)
-        if (!((w4[k] >> (i4[k] & 63)) & 1)) return false;
+        if (!((w4[k] >> (i4[k] & 63)) & 1)) return 0;
     }
   }
   uint64_t kb = T.king[ks] & ~T.king[oks];
@@ -559,14 +573,15 @@ __device__ bool verify_pos(const Tabs &T, const Mat &M, int stm, int s, uint64_t
     int l = sq_get(M.set_of, jj);
     uint64_t q0 = sq_set(p, stm, to), q;
     int s2 = normalize(T, q0, &q);
+    if (used++ >= budget) return 2;
     if (M.synth && s2 != M.synth_slice) {
       int v = synth_exit(T, M, stm, q0, q, occ, ks, jj, l, s2, idx);
       if (v == 3 || v == 2) continue;
-      return false;
+      return 0;
     }
     q = sq_set(q, jj, sq_get(q, M.ii.last[l]));
     const uint64_t *sb = sub_o + phys(M, s2) * M.ctotal[o] + M.coff[l];
-    if (!bm_test(sb, sq_to_idx(T, M.capt[l], q))) return false;
+    if (!bm_test(sb, sq_to_idx(T, M.capt[l], q))) return 0;
   }
   uint64_t kq = kb & ~occ;
   while (kq) {
@@ -574,16 +589,17 @@ __device__ bool verify_pos(const Tabs &T, const Mat &M, int stm, int s, uint64_t
     kq &= kq - 1;
     uint64_t q0 = sq_set(p, stm, to), q;
     int s2 = normalize(T, q0, &q);
+    if (used++ >= budget) return 2;
     if (M.synth && s2 != M.synth_slice) {
       int v = synth_exit(T, M, stm, q0, q, occ, ks, -1, 0, s2, idx);
       if (v == 3 || v == 2) continue;
-      return false;
+      return 0;
     }
     uint64_t i2 = M.all_single && q == q0 ? king_moved_idx(M.ii, idx, q0, ks, to)
                                           : sq_to_idx(T, M.ii, q);
-    if (!bm_test(wins_o + phys(M, s2) * M.words, i2)) return false;
+    if (!bm_test(wins_o + phys(M, s2) * M.words, i2)) return 0;
   }
-  return true;
+  return 1;
 }
 
 // check_successors (src/tb8gen.c:607-661) over every (job, side): prune
@@ -596,6 +612,9 @@ __device__ bool verify_pos(const Tabs &T, const Mat &M, int stm, int s, uint64_t
 // Verification is latency-bound (a dependent chain of random bitmap tests
 // per candidate): 4 waves per SIMD (at most 128 VGPRs, a few spilled) beat
 // 3 waves without spills.
+// SPLIT: the two-pass form for slices larger than VERIFY_SPLIT_SIZE (one
+// pass otherwise, compiled without the second pass's bookkeeping).
+template <bool SPLIT>
 __global__ __launch_bounds__(256, 4) void k_verify_j(JobDev *const *__restrict__ jobs,
                                                    const Pair *__restrict__ pairs,
                                                    const uint64_t *__restrict__ P, int np, int sw0,
@@ -675,16 +694,32 @@ __global__ __launch_bounds__(256, 4) void k_verify_j(JobDev *const *__restrict__
           fill_round(Q.e[wave], x, off, base, lane);
           wave_sync();
           int cnt = tot - base < QCAP ? tot - base : QCAP;
-          for (int e = lane; e < cnt; e += 64) {
-            uint16_t v = Q.e[wave][e];
-            uint64_t idx = (wbase + (v >> 6)) * 64 + (v & 63);
-            bool ok = idx < M.ii.size && verify_pos(T, M, stm, s, idx, wins_o, sub_o);
-            if (!ok)
-              __hip_atomic_fetch_and(reinterpret_cast<unsigned long long *>(&Q.res[wave][v >> 6]),
-                                     ~(1ull << (v & 63)), __ATOMIC_RELAXED,
-                                     __HIP_MEMORY_SCOPE_WORKGROUP);
+          // Two passes: most candidates are refuted by one of their first
+          // moves, a lost one reads every move.  The first pass spends at
+          // most VERIFY_QUICK reads per candidate and queues the open ones
+          // in place (a slot never passes the entry being read); the second
+          // finishes them with the lanes evenly loaded with long work.
+          // A round of fewer than VERIFY_SPLIT candidates takes one pass.
+          int todo = cnt, first = SPLIT && cnt >= VERIFY_SPLIT ? 0 : 1;
+          for (int pass = first; pass < 2; pass++) {
+            if (lane == 0) Q.n[wave] = 0;
+            wave_sync();
+            int budget = pass == 0 ? VERIFY_QUICK : 1 << 30;
+            for (int e = lane; e < todo; e += 64) {
+              uint16_t v = Q.e[wave][e];
+              uint64_t idx = (wbase + (v >> 6)) * 64 + (v & 63);
+              int r = idx < M.ii.size ? verify_pos(T, M, stm, s, idx, wins_o, sub_o, budget) : 0;
+              if (r == 0)
+                __hip_atomic_fetch_and(reinterpret_cast<unsigned long long *>(&Q.res[wave][v >> 6]),
+                                       ~(1ull << (v & 63)), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+              else if (r == 2)
+                Q.e[wave][__hip_atomic_fetch_add(&Q.n[wave], 1, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_WORKGROUP)] = v;
+            }
+            wave_sync();
+            todo = Q.n[wave];
           }
-          wave_sync();
         }
         r = Q.res[wave][lane];
         wave_sync();  // the next chunk overwrites Q.res
@@ -709,6 +744,11 @@ __global__ __launch_bounds__(256, 4) void k_verify_j(JobDev *const *__restrict__
     if (lane == 0 && sum) atomicAdd(&Jp->cnt[stm], (unsigned long long)sum);
   }
 }
+
+template __global__ void k_verify_j<false>(JobDev *const *, const Pair *, const uint64_t *, int, int,
+                                           int);
+template __global__ void k_verify_j<true>(JobDev *const *, const Pair *, const uint64_t *, int, int,
+                                          int);
 
 // calc_W_n tail (src/tb8gen.c:1072-1077) fused, over every (job, side):
 // W = C & ~wins; wins |= W; count; codes; C cleared.  uw words per unit
