@@ -175,40 +175,46 @@ __device__ __forceinline__ void wave_mark(uint64_t *plane, const uint64_t *__res
   }
 }
 
-// mark_king_unmoves for the whole wave with combining: one target per lane
-// per round (the kings are the same for every lane of a slice, so the
-// lanes' rounds line up).  ok: the lane holds a position.
+// mark_king_unmoves for the whole wave with combining.  Every lane of a
+// pred unit has the same kings, so the loop runs over the king's target
+// squares in step across the wave: each target's slice and symmetry are
+// computed once (scalar), and for a move that needs no symmetry the lanes'
+// targets keep the order of their positions, so the segmented OR folds
+// neighbouring lanes into one atomic per word.  ok: the lane holds a
+// position.
 __device__ void king_unmoves_wave(const Tabs &T, const Mat &M, int stm, bool ok, uint64_t occ,
                                   uint64_t p, int s, uint64_t *cand,
                                   const uint64_t *__restrict__ won, int lane, uint64_t idx)
 {
-  int from = sq_get(p, stm);
-  uint64_t b = ok ? T.king[from] & ~(occ | T.king[sq_get(p, stm ^ 1)]) : 0;
-  bool dual = false;
-  uint64_t dq = 0;
-  int ds = 0;
-  while (__ballot(b != 0 || dual)) {
-    uint64_t g = ~0ull;
-    if (dual) {
-      g = phys(M, ds) * M.words * 64 + sq_to_idx(T, M.ii, mirror_nonkings(dq));
-      dual = false;
-    } else if (b) {
-      int t = ctz64(b);
-      b &= b - 1;
-      uint64_t q0 = sq_set(p, stm, t), q;
-      int s2 = normalize(T, q0, &q);
-      if (!M.synth || s2 == M.synth_slice) {  // else it leaves a single-slice job
-        g = phys(M, s2) * M.words * 64 +
-            (M.all_single && q == q0 ? king_moved_idx(M.ii, idx, q0, from, t)
-                                     : sq_to_idx(T, M.ii, q));
-        if (s < 441 && s2 >= 441) {
-          dual = true;
-          dq = q;
-          ds = s2;
-        }
-      }
+  int from = uni(sq_get(p, stm)), oks = uni(sq_get(p, stm ^ 1));
+  uint64_t targets = uni64(T.king[from] & ~T.king[oks]);
+  while (targets) {
+    int t = ctz64(targets);
+    targets &= targets - 1;
+    // normalize (src/tb8gen.c:230-238) of the moved king pair
+    int wk = stm == WHITE ? t : oks, bk = stm == WHITE ? oks : t;
+    int m = mirror_mask(wk);
+    bool fl = flip_test(wk, bk);
+    int w2 = wk ^ m, b2 = bk ^ m;
+    if (fl) {
+      w2 = ((w2 & 7) << 3) | (w2 >> 3);
+      b2 = ((b2 & 7) << 3) | (b2 >> 3);
+    }
+    int s2 = uni(T.kkidx[T.tri[w2] * 64 + b2]);
+    if (M.synth && s2 != M.synth_slice) continue;  // it leaves a single-slice job
+    bool dual = s < 441 && s2 >= 441;              // mark_king_unmoves' mirror image
+    uint64_t g = ~0ull, gd = ~0ull;
+    if (ok && !(occ & bit(t))) {
+      uint64_t q0 = sq_set(p, stm, t);
+      uint64_t q = q0 ^ (0x0101010101010101ull * (uint64_t)m);
+      if (fl) q = flip_diag_bytes(q);
+      uint64_t base = phys(M, s2) * M.words * 64;
+      g = base + (M.all_single && m == 0 && !fl ? king_moved_idx(M.ii, idx, q0, from, t)
+                                                 : sq_to_idx(T, M.ii, q));
+      if (dual) gd = base + sq_to_idx(T, M.ii, mirror_nonkings(q));
     }
     wave_mark(cand, won, g, lane);
+    if (dual) wave_mark(cand, won, gd, lane);
   }
 }
 
