@@ -336,7 +336,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-form comparison")
     ap.add_argument("--cpu-plies", type=int, default=2)
-    ap.add_argument("--scatter", default="block",
+    ap.add_argument("--scatter", default="block+combine",
                     choices=["block", "global", "block+combine", "global+combine"],
                     help="scatter mode when not autotuned")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",

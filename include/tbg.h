@@ -195,7 +195,8 @@ int tbg_get_sub_bitmap(tbg_ctx *ctx, int kind, int colour, int slice, uint64_t *
  *     unmoves, pieces whose moves leave the block) are combined per word
  *     across a wave first (a segmented OR over lanes whose targets share a
  *     word: sort-then-OR without the sort), one atomic per run.
- * Default: TBG_SCATTER_BLOCK (the autotuner also tries the others). */
+ * Default: TBG_SCATTER_BLOCK | TBG_SCATTER_COMBINE (the autotuner also tries
+ * the others). */
 #define TBG_SCATTER_GLOBAL 0
 #define TBG_SCATTER_BLOCK 1
 #define TBG_SCATTER_COMBINE 2

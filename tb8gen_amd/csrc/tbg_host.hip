@@ -33,7 +33,7 @@ constexpr uint64_t kPlanes = 12;
 Tabs g_htabs;
 // geometry and scatter given to new contexts and servers (tbg_set_geometry(NULL, ..))
 Geom g_geom = { 256, 4096, { 3, 16, 16, 16 } };
-int g_scatter = TBG_SCATTER_BLOCK;
+int g_scatter = TBG_SCATTER_BLOCK | TBG_SCATTER_COMBINE;
 char g_arch[64] = "none";
 
 int fail(int code, const char *fmt, ...)
