@@ -928,9 +928,8 @@ __device__ __forceinline__ uint64_t side_attacks(const Tabs &T, const Mat &M, ui
 // Synthetic job, ply 1: does stm's king have a legal move or capture into
 // an exit (a position of another slice, valued by the seeded hash) that is
 // a loss (*w: W_1 candidate) or a win (*l: L_1 candidate) for the opponent?
-// uo: the opponent's piece attacks with stm's king lifted, so a quiet king
-// move's legality is a bit test; a capture recomputes them without the
-// captured piece.
+// uo: the opponent's piece attacks with stm's king lifted, so a king
+// move's legality, capture or not, is a bit test.
 __device__ void synth_exits_pos(const Tabs &T, const Mat &M, uint64_t p, uint64_t occ, int stm,
                                 uint64_t idx, uint64_t uo, bool *w, bool *l)
 {
@@ -946,14 +945,14 @@ __device__ void synth_exits_pos(const Tabs &T, const Mat &M, uint64_t p, uint64_
   while (b) {
     int to = ctz64(b);
     b &= b - 1;
+    // the king would stand in check; for a capture the same test holds,
+    // since the captured piece neither attacks its own square nor blocks a
+    // ray that ends there
+    if (unsafe & bit(to)) continue;
     int jj = -1, l = 0;
     if (occ & bit(to)) {
       jj = slot_of(p, to, M.col80[o]);
       l = sq_get(M.set_of, jj);
-      uint64_t a = T.king[oks] | side_attacks(T, M, p, occ ^ bit(from), o, jj);
-      if (a & bit(to)) continue;  // the king would stand in check
-    } else if (unsafe & bit(to)) {
-      continue;
     }
     uint64_t q0 = sq_set(p, stm, to), q;
     int s2 = normalize(T, q0, &q);
@@ -977,9 +976,9 @@ __device__ void synth_exits_pos(const Tabs &T, const Mat &M, uint64_t p, uint64_
 // has_legal_moves || has_legal_caps (src/movegen.c:75-111) for colour c:
 // the same answer, found with less work.  uo: the opponent's piece attacks
 // with c's king lifted (a slider's ray passes through the square the king
-// leaves), so a quiet king move's legality is a bit test; a capturing king
-// move recomputes them without the captured piece.  Only if no king move
-// is legal are the other pieces' moves tried, each with a full attack test
+// leaves), so a king move's legality, capture or not, is a bit test.  Only
+// if no king move is legal are the other pieces' moves tried, each with a
+// full attack test
 // (pins): under a double check none can be legal, under a single check only
 // moves onto the checker or between it and the king can be.
 __device__ bool has_legal(const Tabs &T, const Mat &M, uint64_t p, uint64_t occ, int c, uint64_t uo)
@@ -992,15 +991,9 @@ __device__ bool has_legal(const Tabs &T, const Mat &M, uint64_t p, uint64_t occ,
     own |= bit(sq_get(p, M.pcs[c][t]));
   }
   uint64_t kb = T.king[k] & ~own & ~T.king[ok];
-  if (kb & ~occ & ~uo) return true;  // a quiet king move to a safe square
-  uint64_t caps = kb & occ;           // king captures (the enemy king is never adjacent)
-  while (caps) {
-    int to = ctz64(caps);
-    caps &= caps - 1;
-    int cap = slot_of(p, to, M.col80[o]);
-    uint64_t a = T.king[ok] | side_attacks(T, M, p, occ ^ bit(k), o, cap);
-    if (!(a & bit(to))) return true;
-  }
+  // a king move or capture to a safe square (a captured piece neither
+  // attacks its own square nor blocks a ray that ends there)
+  if (kb & ~uo) return true;
   // checkers (the lifted king does not block rays onto its own square)
   uint64_t occk = occ ^ bit(k);
   uint64_t block = ~0ull;
