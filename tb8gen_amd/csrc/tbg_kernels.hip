@@ -481,19 +481,37 @@ __global__ __launch_bounds__(256) void k_uncap_j(JobDev *const *__restrict__ job
   }
 }
 
+// Attacks of colour c's non-king pieces on occupancy `o` (the caller lifts
+// the enemy king from it, so rays pass through the square that king
+// leaves), without the piece in slot `skip` (-1: none).
+__device__ __forceinline__ uint64_t side_attacks(const Tabs &T, const Mat &M, uint64_t p,
+                                                 uint64_t o, int c, int skip)
+{
+  uint64_t a = 0;
+  for (int t = 0; t < MAXP - 1; t++) {
+    if (t >= M.nk[c]) break;
+    int j = M.pcs[c][t];
+    if (j == skip) continue;
+    a |= piece_att(T, M.pt[j], sq_get(p, j), o);
+  }
+  return a;
+}
+
 // Synthetic king-move exit value for o after stm's king moved from `from`
 // (position q0 not normalised, q normalised into slice s2); captured slot
-// jj of set l, or jj < 0.  3 = illegal move.
-__device__ int synth_exit(const Tabs &T, const Mat &M, int stm, uint64_t q0, uint64_t q, uint64_t occ,
-                          int from, int jj, int l, int s2, uint64_t idx)
+// jj of set l, or jj < 0.  3 = illegal move: the target is in `unsafe`, the
+// opponent's attacks with stm's king lifted plus its king's squares (a
+// captured piece neither attacks its own square nor blocks a ray that ends
+// there).
+__device__ int synth_exit(const Tabs &T, const Mat &M, int stm, uint64_t q0, uint64_t q,
+                          uint64_t unsafe, int from, int jj, int l, int s2, uint64_t idx)
 {
   int o = stm ^ 1, to = sq_get(q0, stm);
+  if ((unsafe >> to) & 1) return 3;
   if (jj >= 0) {
-    if (attacked_by(T, M, q0, occ ^ bit(from), to, o, jj)) return 3;
     uint64_t c = sq_set(q, jj, sq_get(q, M.ii.last[l]));
     return synth_value(synth_hash(M.seedmix, o, 2, s2 * 8 + l, sq_to_idx(T, M.capt[l], c)));
   }
-  if (attacked_by(T, M, q0, occ ^ bit(from) ^ bit(to), to, o, -1)) return 3;
   uint64_t i2 = M.all_single && q == q0 ? king_moved_idx(M.ii, idx, q0, from, to)
                                         : sq_to_idx(T, M.ii, q);
   return synth_value(synth_hash(M.seedmix, o, 1, s2, i2));
@@ -571,6 +589,9 @@ __device__ __forceinline__ int verify_pos(const Tabs &T, const Mat &M, int stm, 
     }
   }
   uint64_t kb = T.king[ks] & ~T.king[oks];
+  // a synthetic job's king moves mostly leave the slice, and each such exit
+  // needs a legality test: the opponent's attacks once, with the king lifted
+  uint64_t unsafe = M.synth ? T.king[oks] | side_attacks(T, M, p, occ ^ bit(ks), o, -1) : 0;
   uint64_t c = kb & oppocc;
   while (c) {
     int to = ctz64(c);
@@ -581,7 +602,7 @@ __device__ __forceinline__ int verify_pos(const Tabs &T, const Mat &M, int stm, 
     int s2 = normalize(T, q0, &q);
     if (used++ >= budget) return 2;
     if (M.synth && s2 != M.synth_slice) {
-      int v = synth_exit(T, M, stm, q0, q, occ, ks, jj, l, s2, idx);
+      int v = synth_exit(T, M, stm, q0, q, unsafe, ks, jj, l, s2, idx);
       if (v == 3 || v == 2) continue;
       return 0;
     }
@@ -597,7 +618,7 @@ __device__ __forceinline__ int verify_pos(const Tabs &T, const Mat &M, int stm, 
     int s2 = normalize(T, q0, &q);
     if (used++ >= budget) return 2;
     if (M.synth && s2 != M.synth_slice) {
-      int v = synth_exit(T, M, stm, q0, q, occ, ks, -1, 0, s2, idx);
+      int v = synth_exit(T, M, stm, q0, q, unsafe, ks, -1, 0, s2, idx);
       if (v == 3 || v == 2) continue;
       return 0;
     }
@@ -907,22 +928,6 @@ __global__ __launch_bounds__(256) void k_code_plane(const uint16_t *codes, uint6
     uint64_t b = __ballot(hit);
     if (lane == 0) out[g] = b;
   }
-}
-
-// Attacks of colour c's non-king pieces on occupancy `o` (the caller lifts
-// the enemy king from it, so rays pass through the square that king
-// leaves), without the piece in slot `skip` (-1: none).
-__device__ __forceinline__ uint64_t side_attacks(const Tabs &T, const Mat &M, uint64_t p,
-                                                 uint64_t o, int c, int skip)
-{
-  uint64_t a = 0;
-  for (int t = 0; t < MAXP - 1; t++) {
-    if (t >= M.nk[c]) break;
-    int j = M.pcs[c][t];
-    if (j == skip) continue;
-    a |= piece_att(T, M.pt[j], sq_get(p, j), o);
-  }
-  return a;
 }
 
 // Synthetic job, ply 1: does stm's king have a legal move or capture into
