@@ -49,6 +49,12 @@ struct Mat {
   int synth, synth_slice;
   uint64_t seed, seedmix;    // synthetic exits: seed and splitmix64(seed)
   int all_single;            // every set holds one piece (set k in slot 2 + k)
+  // synthetic job: side c's king moves that leave the slice (the kings are
+  // fixed, so each target's slice and symmetry are too): target square,
+  // target slice, mirror mask and diagonal flip of the moved king pair
+  int nexit[2];
+  uint8_t exit_to[2][8], exit_m[2][8], exit_fl[2][8];
+  int16_t exit_s2[2][8];
 };
 
 // Tables shared by all kernels (copied to LDS per work-group).

@@ -1023,6 +1023,28 @@ static int create_common(tbg_ctx **out, const char *material, int synth, int sli
   c->h.synth_slice = slice;
   c->h.seed = seed;
   c->h.seedmix = splitmix64(seed);
+  if (synth) {
+    const Tabs &T = g_htabs;
+    int kq[2] = { T.kksq[slice][0], T.kksq[slice][1] };
+    for (int stm = 0; stm < 2; stm++) {
+      int from = kq[stm], oks = kq[stm ^ 1], n = 0;
+      uint64_t b = T.king[from] & ~T.king[oks];
+      while (b) {
+        int to = ctz64(b);
+        b &= b - 1;
+        int wk = stm == WHITE ? to : oks, bk = stm == WHITE ? oks : to;
+        uint64_t q;
+        int s2 = normalize(T, (uint64_t)wk | ((uint64_t)bk << 8), &q);
+        if (s2 == slice) continue;
+        c->h.exit_to[stm][n] = (uint8_t)to;
+        c->h.exit_m[stm][n] = (uint8_t)mirror_mask(wk);
+        c->h.exit_fl[stm][n] = flip_test(wk, bk) ? 1 : 0;
+        c->h.exit_s2[stm][n] = (int16_t)s2;
+        n++;
+      }
+      c->h.nexit[stm] = n;
+    }
+  }
   r = ctx_alloc(c, !synth || keep_codes, ext);
   if (r != TBG_OK) {
     tbg_destroy(c);

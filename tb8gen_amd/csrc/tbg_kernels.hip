@@ -938,37 +938,34 @@ __global__ __launch_bounds__(256) void k_code_plane(const uint16_t *codes, uint6
 __device__ void synth_exits_pos(const Tabs &T, const Mat &M, uint64_t p, uint64_t occ, int stm,
                                 uint64_t idx, uint64_t uo, bool *w, bool *l)
 {
-  int o = stm ^ 1, s = M.synth_slice, from = sq_get(p, stm), oks = sq_get(p, o);
-  uint64_t own = bit(from);
+  int o = stm ^ 1, from = sq_get(p, stm), oks = sq_get(p, o);
+  uint64_t own = 0;
   for (int t = 0; t < MAXP - 1; t++) {
     if (t >= M.nk[stm]) break;
     own |= bit(sq_get(p, M.pcs[stm][t]));
   }
-  uint64_t b = T.king[from] & ~T.king[oks] & ~own;
-  uint64_t unsafe = T.king[oks] | uo;
+  uint64_t blocked = own | T.king[oks] | uo;
   bool ww = false, ll = false;
-  while (b) {
-    int to = ctz64(b);
-    b &= b - 1;
-    // the king would stand in check; for a capture the same test holds,
-    // since the captured piece neither attacks its own square nor blocks a
-    // ray that ends there
-    if (unsafe & bit(to)) continue;
-    int jj = -1, l = 0;
-    if (occ & bit(to)) {
-      jj = slot_of(p, to, M.col80[o]);
-      l = sq_get(M.set_of, jj);
-    }
-    uint64_t q0 = sq_set(p, stm, to), q;
-    int s2 = normalize(T, q0, &q);
-    if (s2 == s) continue;
+  // the king's moves out of the slice, their slice and symmetry prepared on
+  // the host
+  for (int e = 0; e < 8; e++) {
+    if (e >= M.nexit[stm]) break;
+    int to = M.exit_to[stm][e];
+    if (blocked & bit(to)) continue;  // own piece, or the king would stand in check
+    int m = M.exit_m[stm][e], s2 = M.exit_s2[stm][e];
+    bool fl = M.exit_fl[stm][e] != 0;
+    uint64_t q0 = sq_set(p, stm, to);
+    uint64_t q = q0 ^ (0x0101010101010101ull * (uint64_t)m);
+    if (fl) q = flip_diag_bytes(q);
     int v;
-    if (jj >= 0) {
-      uint64_t c = sq_set(q, jj, sq_get(q, M.ii.last[l]));
-      v = synth_value(synth_hash(M.seedmix, o, 2, s2 * 8 + l, sq_to_idx(T, M.capt[l], c)));
+    if (occ & bit(to)) {
+      int jj = slot_of(p, to, M.col80[o]);
+      int k = sq_get(M.set_of, jj);
+      uint64_t c = sq_set(q, jj, sq_get(q, M.ii.last[k]));
+      v = synth_value(synth_hash(M.seedmix, o, 2, s2 * 8 + k, sq_to_idx(T, M.capt[k], c)));
     } else {
-      uint64_t i2 = M.all_single && q == q0 ? king_moved_idx(M.ii, idx, q0, from, to)
-                                            : sq_to_idx(T, M.ii, q);
+      uint64_t i2 = M.all_single && m == 0 && !fl ? king_moved_idx(M.ii, idx, q0, from, to)
+                                                  : sq_to_idx(T, M.ii, q);
       v = synth_value(synth_hash(M.seedmix, o, 1, s2, i2));
     }
     ww |= v == -2;
