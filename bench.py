@@ -353,6 +353,10 @@ def main():
     ap.add_argument("--stream-plain", action="store_true",
                     help="stream workload: also time the 4-piece level in the plain form")
     ap.add_argument("--seed", type=int, default=20261203)
+    ap.add_argument("--geometry", default="",
+                    help="launch geometry of the contexts created, e.g. "
+                         "verify_unit_words=1024,pred_wg_per_cu=2 (fields of tbg_geometry; "
+                         "the table workload's autotuner replaces it)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters against known bytes)")
@@ -378,6 +382,12 @@ def main():
     import tb8gen_amd as tbg
 
     tbg.init(0)
+    if args.geometry:
+        g = tbg.default_geometry()
+        for kv in args.geometry.split(","):
+            k, v = kv.split("=")
+            setattr(g, k.strip(), int(v))
+        tbg.set_default_geometry(g)
     if args.workload == "synth":
         return run_synth(args, tbg)
     if args.workload == "stream":

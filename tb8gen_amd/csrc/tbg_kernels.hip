@@ -298,8 +298,17 @@ __device__ __forceinline__ void cache_mat(Mat &Ms, const Mat *src, int j, int *c
 __device__ __forceinline__ int unit_slice(const Mat &M, int sw0, uint64_t local, uint64_t ups,
                                           uint64_t *within)
 {
+  if (M.synth) {  // one slice: no division
+    *within = local;
+    return M.synth_slice;
+  }
+  if (local < (1ull << 32)) {  // a 32-bit division is a fraction of a 64-bit one
+    uint32_t q = (uint32_t)local / (uint32_t)ups;
+    *within = local - (uint64_t)q * ups;
+    return sw0 + (int)q;
+  }
   *within = local % ups;
-  return M.synth ? M.synth_slice : sw0 + (int)(local / ups);
+  return sw0 + (int)(local / ups);
 }
 
 // Word wi of a pred block's frontier: bits below lo and from hi on (the
@@ -791,21 +800,40 @@ __global__ __launch_bounds__(256) void k_sweep_j(JobDev *const *__restrict__ job
   int cached = -1;
   int lane = threadIdx.x & 63;
   uint64_t total = P[np];
+  // the (job, side) pair's fields, re-read only when a unit leaves its range
+  // (the job record does not change during a launch), in scalar registers
+  uint64_t plo = 1, phi = 0;
+  bool skip = true, run = false, exits = false;
+  int stm = 0, n = 0;
+  JobDev *Jp = nullptr;
+  uint64_t *cand = nullptr, *wins = nullptr, *newW = nullptr;
+  const uint64_t *kex = nullptr;
+  uint16_t *codes = nullptr;
   for (uint64_t u = blockIdx.x; u < total; u += gridDim.x) {
-    int pi = find_pair(P, np, u);
-    Pair pr = pairs[pi];
-    JobDev &J = *jobs[pr.job];
-    if (J.done) continue;  // uniform across the work-group
-    cache_mat(Ms, J.M, pr.job, &cached);
+    if (u < plo || u >= phi) {  // uniform across the work-group
+      int pi = find_pair(P, np, u);
+      plo = P[pi];
+      phi = P[pi + 1];
+      Pair pr = pairs[pi];
+      Jp = uniptr(jobs[pr.job]);
+      skip = uni(Jp->done) != 0;
+      if (!skip) {
+        cache_mat(Ms, Jp->M, pr.job, &cached);
+        stm = pr.side;
+        n = uni(Jp->n);
+        run = (uni(Jp->mask) >> (2 + stm)) & 1;
+        exits = run && Ms.synth && n == 1;
+        cand = uniptr(Jp->cand[stm]);
+        wins = uniptr(Jp->wins[stm]);
+        newW = uniptr(Jp->newW[stm]);
+        kex = uniptr(Jp->prevW[stm]);
+        codes = uniptr(Jp->codes[stm]);
+      }
+    }
+    if (skip) continue;
     const Mat &M = Ms;
-    int stm = pr.side, n = J.n;
-    bool run = (J.mask >> (2 + stm)) & 1;
-    bool exits = run && M.synth && n == 1;
     uint64_t within;
-    int s = unit_slice(M, sw0, u - P[pi], (M.words + uw - 1) / uw, &within);
-    uint64_t *cand = J.cand[stm], *wins = J.wins[stm], *newW = J.newW[stm];
-    const uint64_t *kex = J.prevW[stm];
-    uint16_t *codes = J.codes[stm];
+    int s = unit_slice(M, sw0, u - plo, (M.words + uw - 1) / uw, &within);
     uint64_t base = phys(M, s) * M.words;
     uint64_t wend = (within + 1) * uw < M.words ? (within + 1) * uw : M.words;
     uint64_t mycnt = 0;
@@ -848,7 +876,7 @@ __global__ __launch_bounds__(256) void k_sweep_j(JobDev *const *__restrict__ job
       }
     }
     mycnt = wave_sum64(mycnt);
-    if (lane == 0 && mycnt) atomicAdd(&J.cnt[2 + stm], (unsigned long long)mycnt);
+    if (lane == 0 && mycnt) atomicAdd(&Jp->cnt[2 + stm], (unsigned long long)mycnt);
   }
 }
 
