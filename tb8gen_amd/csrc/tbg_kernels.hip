@@ -388,17 +388,34 @@ __global__ __launch_bounds__(256) void k_uncap_j(JobDev *const *__restrict__ job
   int cached = -1;
   int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   uint64_t total = P[np];
+  // the (job, side) pair's fields, re-read only when a unit leaves its range
+  // (the job record does not change during a launch)
+  uint64_t plo = 1, phi = 0;
+  bool en = false;
+  int stm = 0, o = 1, n = 0, scatter = 0;
+  const JobDev *Jp = nullptr;
   for (uint64_t u = blockIdx.x; u < total; u += gridDim.x) {
-    int pi = find_pair(P, np, u);
-    Pair pr = pairs[pi];
-    const JobDev &J = *jobs[pr.job];
-    int stm = pr.side, o = stm ^ 1, n = J.n;
-    bool en = !J.done && ((J.mask >> (kind * 2 + stm)) & 1) && (n == 1 || n == DRAW_RULE + 1);
-    if (!en) continue;  // uniform across the work-group
-    cache_mat(Ms, J.M, pr.job, &cached);
+    if (u < plo || u >= phi) {  // uniform across the work-group
+      int pi = find_pair(P, np, u);
+      plo = P[pi];
+      phi = P[pi + 1];
+      Pair pr = pairs[pi];
+      Jp = uniptr(jobs[pr.job]);
+      stm = pr.side;
+      o = stm ^ 1;
+      n = uni(Jp->n);
+      en = !uni(Jp->done) && ((uni(Jp->mask) >> (kind * 2 + stm)) & 1) &&
+           (n == 1 || n == DRAW_RULE + 1);
+      if (en) {
+        cache_mat(Ms, Jp->M, pr.job, &cached);
+        scatter = uni(Jp->scatter);
+      }
+    }
+    if (!en) continue;
+    const JobDev &J = *Jp;
     const Mat &M = Ms;
     uint64_t within;
-    int s = unit_slice(M, sw0, u - P[pi], uncap_units(M, stm), &within);
+    int s = unit_slice(M, sw0, u - plo, uncap_units(M, stm), &within);
     int k;
     for (k = 0; k < M.numsets; k++) {
       if (M.setcol[k] == stm) continue;
@@ -437,7 +454,7 @@ __global__ __launch_bounds__(256) void k_uncap_j(JobDev *const *__restrict__ job
     uint64_t *own = cand + phys(M, s) * M.words;
     const uint64_t *ownw = won + phys(M, s) * M.words;
     uint64_t nbw = (M.ii.size + 63) >> 6;
-    bool lds = (J.scatter & TBG_SCATTER_BLOCK) && nbw <= lds_words;  // uniform
+    bool lds = (scatter & TBG_SCATTER_BLOCK) && nbw <= lds_words;  // uniform
     uint64_t *B = dyn;
     if (lds) {
       if (!__syncthreads_or(tot != 0)) continue;  // nothing to uncapture in this unit
