@@ -154,11 +154,13 @@ __device__ __forceinline__ void mark_king_unmoves(const Tabs &T, const Mat &M, i
 // one target (global bit index into `plane`, ~0: none); a segmented OR-scan
 // over runs of lanes whose targets share a 64-bit word (lanes hold
 // consecutive frontier positions under the same unmove, so their targets
-// cluster) gathers each run's bits into its last lane, which alone reads
-// the word of `won` and issues one atomic OR.  A word that recurs in
-// separate runs just gets one atomic per run.  Call with the whole wave.
-__device__ __forceinline__ void wave_mark(uint64_t *plane, const uint64_t *__restrict__ won,
-                                          uint64_t g, int lane)
+// cluster) gathers each run's bits into its last lane, which alone issues
+// one atomic OR.  A word that recurs in separate runs just gets one atomic
+// per run.  Targets already in wins(stm) are not filtered here: the
+// verification and the W sweep mask them, and a read of `won` before each
+// combined atomic costs more than the atomics it saves.  Call with the
+// whole wave.
+__device__ __forceinline__ void wave_mark(uint64_t *plane, uint64_t g, int lane)
 {
   uint64_t key = g == ~0ull ? ~0ull : g >> 6;
   uint64_t val = g == ~0ull ? 0 : bit(g & 63);
@@ -169,10 +171,7 @@ __device__ __forceinline__ void wave_mark(uint64_t *plane, const uint64_t *__res
     if (lane >= d && ku == key) val |= vu;
   }
   uint64_t kd = __shfl_down(key, 1);
-  if (key != ~0ull && (lane == 63 || kd != key)) {
-    uint64_t m = val & ~gload(won, key);
-    if (m) atomic_or64(plane + key, m);
-  }
+  if (key != ~0ull && (lane == 63 || kd != key)) atomic_or64(plane + key, val);
 }
 
 // mark_king_unmoves for the whole wave with combining.  Every lane of a
@@ -183,8 +182,8 @@ __device__ __forceinline__ void wave_mark(uint64_t *plane, const uint64_t *__res
 // neighbouring lanes into one atomic per word.  ok: the lane holds a
 // position.
 __device__ void king_unmoves_wave(const Tabs &T, const Mat &M, int stm, bool ok, uint64_t occ,
-                                  uint64_t p, int s, uint64_t *cand,
-                                  const uint64_t *__restrict__ won, int lane, uint64_t idx)
+                                  uint64_t p, int s, uint64_t *cand, int lane,
+                                  uint64_t idx)
 {
   int from = uni(sq_get(p, stm)), oks = uni(sq_get(p, stm ^ 1));
   uint64_t targets = uni64(T.king[from] & ~T.king[oks]);
@@ -213,8 +212,8 @@ __device__ void king_unmoves_wave(const Tabs &T, const Mat &M, int stm, bool ok,
                                                  : sq_to_idx(T, M.ii, q));
       if (dual) gd = base + sq_to_idx(T, M.ii, mirror_nonkings(q));
     }
-    wave_mark(cand, won, g, lane);
-    if (dual) wave_mark(cand, won, gd, lane);
+    wave_mark(cand, g, lane);
+    if (dual) wave_mark(cand, gd, lane);
   }
 }
 
