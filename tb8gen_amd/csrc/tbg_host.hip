@@ -1902,7 +1902,7 @@ namespace {
 // each kernel's best candidate is read from its own time.
 const tbg_geometry kTune[] = {
   // verify uw, sweep uw, pred wg, uncap wg, verify wg, sweep wg, scatter
-  { 256, 4096, 3, 16, 16, 16, TBG_SCATTER_BLOCK },  // the built-in default
+  { 256, 4096, 3, 16, 16, 16, TBG_SCATTER_BLOCK | TBG_SCATTER_COMBINE },  // the built-in default
   { 1024, 2048, 2, 8, 16, 16, TBG_SCATTER_BLOCK | TBG_SCATTER_COMBINE },
   { 4096, 16384, 4, 32, 8, 8, TBG_SCATTER_BLOCK | TBG_SCATTER_COMBINE },
   { 512, 8192, 8, 16, 32, 16, TBG_SCATTER_GLOBAL },
