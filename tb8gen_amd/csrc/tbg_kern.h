@@ -85,7 +85,7 @@ __global__ void k_uncap_j(JobDev *const *jobs, const Pair *pairs, const uint64_t
                           int kind, const unsigned *flags, uint64_t lds_words);
 constexpr uint64_t VERIFY_SPLIT_SIZE = 1ull << 22;  // positions per slice
 template <bool SPLIT>
-__global__ __launch_bounds__(256, 4) void k_verify_j(JobDev *const *jobs, const Pair *pairs,
+__global__ __launch_bounds__(256, 5) void k_verify_j(JobDev *const *jobs, const Pair *pairs,
                                                    const uint64_t *P, int np, int sw0, int uw);
 __global__ void k_sweep_j(JobDev *const *jobs, const Pair *pairs, const uint64_t *P, int np, int sw0,
                           int uw);

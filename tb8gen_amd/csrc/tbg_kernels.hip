@@ -661,13 +661,13 @@ __device__ __forceinline__ int verify_pos(const Tabs &T, const Mat &M, int stm, 
 // a decode.  A step that does not run this ply writes an empty newL.  For a
 // synthetic job at n == 1 the king-move exit candidates (computed by k_init
 // into newW[stm]) join the candidates.
-// Verification is latency-bound (a dependent chain of random bitmap tests
-// per candidate): 4 waves per SIMD (at most 128 VGPRs, a few spilled) beat
-// 3 waves without spills.
+// Verification waits on chains of random bitmap reads: 5 waves per SIMD
+// (at most 96 VGPRs, a few spilled) beat 4 waves without spills and 6 with
+// more spills.
 // SPLIT: the two-pass form for slices larger than VERIFY_SPLIT_SIZE (one
 // pass otherwise, compiled without the second pass's bookkeeping).
 template <bool SPLIT>
-__global__ __launch_bounds__(256, 4) void k_verify_j(JobDev *const *__restrict__ jobs,
+__global__ __launch_bounds__(256, 5) void k_verify_j(JobDev *const *__restrict__ jobs,
                                                    const Pair *__restrict__ pairs,
                                                    const uint64_t *__restrict__ P, int np, int sw0,
                                                    int uw)
