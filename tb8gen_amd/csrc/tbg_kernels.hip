@@ -372,7 +372,7 @@ __global__ __launch_bounds__(512, 6) void k_pred_wide_j(JobDev *const *__restric
 // into an LDS copy of the slice and merges it with one device atomic per
 // nonzero word, as pred's block scatter does; king uncaptures, and every
 // uncapture of larger slices, use device atomics.
-__global__ __launch_bounds__(256) void k_uncap_j(JobDev *const *__restrict__ jobs,
+__global__ __launch_bounds__(256, 5) void k_uncap_j(JobDev *const *__restrict__ jobs,
                                                   const Pair *__restrict__ pairs,
                                                   const uint64_t *__restrict__ P, int np, int sw0,
                                                   int kind, const unsigned *__restrict__ flags,
@@ -1085,7 +1085,7 @@ __device__ bool has_legal(const Tabs &T, const Mat &M, uint64_t p, uint64_t occ,
 // calc_illegal_worker + calc_mate_worker (src/tb8gen.c:664-727), dense: one
 // position per lane.  Writes illegal(c) into wins[c], mates into mate[c],
 // and every code.
-__global__ __launch_bounds__(256) void k_init(const Mat *__restrict__ Mp, uint64_t *winsW,
+__global__ __launch_bounds__(256, 6) void k_init(const Mat *__restrict__ Mp, uint64_t *winsW,
                                                uint64_t *winsB, uint64_t *mateW, uint64_t *mateB,
                                                uint16_t *codesW, uint16_t *codesB,
                                                uint64_t *kexW0, uint64_t *kexW1, uint64_t *kexL0,
