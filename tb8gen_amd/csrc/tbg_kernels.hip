@@ -150,28 +150,60 @@ __device__ __forceinline__ void mark_king_unmoves(const Tabs &T, const Mat &M, i
   }
 }
 
+// A 32-bit value moved between lanes by DPP (gfx9 controls: 0x138 wave_shr:1,
+// 0x110 + n row_shr:n within rows of 16 lanes, 0x142 row_bcast:15, 0x143
+// row_bcast:31); lanes with no source lane, or in rows outside ROWS, read
+// `old`.  No LDS round trip, unlike a shuffle.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ uint32_t dpp_move(uint32_t old, uint32_t v)
+{
+  return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)v, CTRL, ROWS, 0xf, false);
+}
+
+// one step of the segmented OR-scan: take the moved lane's bits if it has
+// the same key
+template <int CTRL, int ROWS>
+__device__ __forceinline__ void seg_or_step(uint32_t key, uint32_t ku, uint64_t &val)
+{
+  uint32_t lo = dpp_move<CTRL, ROWS>(0u, (uint32_t)val);
+  uint32_t hi = dpp_move<CTRL, ROWS>(0u, (uint32_t)(val >> 32));
+  if (ku == key) val |= (uint64_t)hi << 32 | lo;
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ void seg_or_step(uint32_t key, uint64_t &val)
+{
+  seg_or_step<CTRL, ROWS>(key, dpp_move<CTRL, ROWS>(~0u, key), val);
+}
+
 // Scatter with combining ("sort-then-OR" within a wave): every lane brings
-// one target (global bit index into `plane`, ~0: none); a segmented OR-scan
-// over runs of lanes whose targets share a 64-bit word (lanes hold
-// consecutive frontier positions under the same unmove, so their targets
-// cluster) gathers each run's bits into its last lane, which alone issues
-// one atomic OR.  A word that recurs in separate runs just gets one atomic
-// per run.  Targets already in wins(stm) are not filtered here: the
-// verification and the W sweep mask them, and a read of `won` before each
-// combined atomic costs more than the atomics it saves.  Call with the
-// whole wave.
+// one target (bit index into the slice bitmap `plane`, ~0: none); a
+// segmented OR-scan over runs of lanes whose targets share a 64-bit word
+// (lanes hold consecutive frontier positions under the same unmove, so
+// their targets cluster) gathers each run's bits into its last lane, which
+// alone issues one atomic OR.  A word that recurs in separate runs just
+// gets one atomic per run (lanes of another run with the same word may add
+// their bits too: the same word, so harmless).  The scan is the usual
+// wave64 DPP scan: lane shift 1, then shifts 2, 4, 8 inside each row of 16
+// lanes, then the row totals broadcast to the rows above.  A slice has at
+// most 62*61*60*59*58*57 < 2^36 positions (six non-king pieces), so its
+// word indices fit 32 bits below the ~0u of "no target".  Targets already
+// in wins(stm) are not filtered here: the verification and the W sweep mask
+// them, and a read of `won` before each combined atomic costs more than the
+// atomics it saves.  Call with the whole wave.
 __device__ __forceinline__ void wave_mark(uint64_t *plane, uint64_t g, int lane)
 {
-  uint64_t key = g == ~0ull ? ~0ull : g >> 6;
+  uint32_t key = g == ~0ull ? ~0u : (uint32_t)(g >> 6);
   uint64_t val = g == ~0ull ? 0 : bit(g & 63);
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    uint64_t ku = __shfl_up(key, d);
-    uint64_t vu = __shfl_up(val, d);
-    if (lane >= d && ku == key) val |= vu;
-  }
-  uint64_t kd = __shfl_down(key, 1);
-  if (key != ~0ull && (lane == 63 || kd != key)) atomic_or64(plane + key, val);
+  uint32_t kp = dpp_move<0x138, 0xf>(~0u, key);  // the key of lane - 1
+  uint64_t heads = __ballot(kp != key);         // lanes that start a run
+  seg_or_step<0x138, 0xf>(key, kp, val);
+  seg_or_step<0x112, 0xf>(key, val);
+  seg_or_step<0x114, 0xf>(key, val);
+  seg_or_step<0x118, 0xf>(key, val);
+  seg_or_step<0x142, 0xa>(key, val);  // lane 15 of rows 0, 2 into rows 1, 3
+  seg_or_step<0x143, 0xc>(key, val);  // lane 31 into rows 2, 3
+  bool last = ((heads >> 1 | 1ull << 63) >> lane) & 1;
+  if (key != ~0u && last) atomic_or64(plane + key, val);
 }
 
 // mark_king_unmoves for the whole wave with combining.  Every lane of a
@@ -207,13 +239,13 @@ __device__ void king_unmoves_wave(const Tabs &T, const Mat &M, int stm, bool ok,
       uint64_t q0 = sq_set(p, stm, t);
       uint64_t q = q0 ^ (0x0101010101010101ull * (uint64_t)m);
       if (fl) q = flip_diag_bytes(q);
-      uint64_t base = phys(M, s2) * M.words * 64;
-      g = base + (M.all_single && m == 0 && !fl ? king_moved_idx(M.ii, idx, q0, from, t)
-                                                 : sq_to_idx(T, M.ii, q));
-      if (dual) gd = base + sq_to_idx(T, M.ii, mirror_nonkings(q));
+      g = M.all_single && m == 0 && !fl ? king_moved_idx(M.ii, idx, q0, from, t)
+                                         : sq_to_idx(T, M.ii, q);
+      if (dual) gd = sq_to_idx(T, M.ii, mirror_nonkings(q));
     }
-    wave_mark(cand, g, lane);
-    if (dual) wave_mark(cand, gd, lane);
+    uint64_t *plane = cand + phys(M, s2) * M.words;
+    wave_mark(plane, g, lane);
+    if (dual) wave_mark(plane, gd, lane);
   }
 }
 
