@@ -324,6 +324,21 @@ void choose_block(const Mat &M, uint64_t *blk_pos, uint64_t *nblk, int *blk_set)
   *blk_set = M.ii.numsets;
 }
 
+// A table of few large slices (one block each) has few pred units per
+// launch (462 per side), each long: the work-groups that draw a second unit
+// set the launch's length while the rest of the chip idles.  Such a block's
+// frontier is split into up to kMaxParts parts of at least one 64-word
+// chunk per wave each, every part a unit of its own that scatters into its
+// own LDS copy of the whole block.
+constexpr int kMaxParts = 4;
+int block_parts(uint64_t nblk, uint64_t blk_pos)
+{
+  if (nblk != 1) return 1;
+  uint64_t chunks = (blk_pos + 64 * 64 - 1) / (64 * 64);
+  uint64_t parts = chunks / (PB_THREADS / 64);
+  return parts < 1 ? 1 : parts > (uint64_t)kMaxParts ? kMaxParts : (int)parts;
+}
+
 // window_slices 0: each job's own slices (462, or 1 for a synthetic job)
 int build_launchset(LaunchSet &L, const std::vector<const JobDev *> &hjobs,
                     const std::vector<const Mat *> &mats, const std::vector<Pair> &pairs,
@@ -339,7 +354,8 @@ int build_launchset(LaunchSet &L, const std::vector<const JobDev *> &hjobs,
     for (size_t i = 0; i < pairs.size(); i++) {
       const Mat &M = *mats[pairs[i].job];
       uint64_t ns = window_slices ? window_slices : (M.synth ? 1 : NSL);
-      P[kt][i + 1] = P[kt][i] + ns * units_per_slice(M, hjobs[pairs[i].job]->nblk, kt, pairs[i].side, g);
+      const JobDev *J = hjobs[pairs[i].job];
+      P[kt][i + 1] = P[kt][i] + ns * units_per_slice(M, J->nblk * J->blk_parts, kt, pairs[i].side, g);
     }
     L.total[kt] = P[kt][pairs.size()];
   }
@@ -399,6 +415,7 @@ void reset_job(tbg_ctx *c)
   J.max_ply = c->max_ply;
   choose_block(c->h, &J.blk_pos, &J.nblk, &J.blk_set);
   J.blk_words = (J.blk_pos + 63) / 64 + 1;
+  J.blk_parts = block_parts(J.nblk, J.blk_pos);
 }
 
 int upload_job(tbg_ctx *c)

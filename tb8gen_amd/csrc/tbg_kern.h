@@ -35,6 +35,7 @@ struct JobDev {
   int stop_at;                // > 0: pause (done = 2) on reaching this ply
   uint64_t blk_pos, nblk, blk_words;
   int blk_set;                // pieces of sets >= blk_set move within a pred block
+  int blk_parts;              // pred units per block (> 1 only for one block per slice)
 };
 
 // One (job, side to move) pair of a launch.
@@ -43,7 +44,8 @@ struct Pair {
 };
 
 // Work units per slice of each launch type, one unit per work-group at a
-// time: pred = one index block; uncap = 256 bitmap words (64 per wave);
+// time: pred = one part of an index block (nblk = blocks x parts here);
+// uncap = 256 bitmap words (64 per wave);
 // verify and sweep = a number of words chosen by the launch geometry.
 enum { KT_PRED = 0, KT_UNCAP = 1, KT_VERIFY = 2, KT_SWEEP = 3 };
 constexpr uint64_t UNIT_WORDS = 256;
