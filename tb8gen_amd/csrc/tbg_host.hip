@@ -146,10 +146,11 @@ void calc_idx(Idx &ii)
   }
   uint64_t f = 1;
   for (int i = MAXS - 1; i >= 0; i--) {
-    if (i >= ii.numsets) { ii.factor[i] = 0; ii.inv[i] = 0; continue; }
+    if (i >= ii.numsets) { ii.factor[i] = 0; ii.inv[i] = 0; ii.invf[i] = 0; continue; }
     assert(f < (1ull << 32));  // the sets after the first hold at most 5 of 8 pieces
     ii.factor[i] = (uint32_t)f;
     ii.inv[i] = 1.0 / (double)f;
+    ii.invf[i] = 1.0f / (float)f;
     f *= sub[i];
   }
   ii.size = f;
@@ -2163,6 +2164,17 @@ extern "C" int tbg_host_selftest(char *msg, int len)
     int wk = T.kksq[s][0], bk = T.kksq[s][1];
     if (kkmap(wk, bk) != s) ST_FAIL("kksq/kkmap disagree at %d", s);
     if ((s >= 441) != (on_diag(wk) && on_diag(bk))) ST_FAIL("diagonal slice numbering");
+  }
+  // div24 against integer division around quotient boundaries
+  for (uint32_t d = 2; d < (1u << 24); d = d * 3 / 2 + 1) {
+    float inv = 1.0f / (float)d;
+    for (uint64_t q = 0; q * d < (1u << 24); q = q * 2 + 1)
+      for (int e = -1; e <= 1; e++) {
+        int64_t n = (int64_t)(q * d) + e;
+        if (n < 0 || n >= (1 << 24)) continue;
+        uint32_t r, qq = div24((uint32_t)n, d, inv, &r);
+        if (qq != (uint64_t)n / d || r != (uint64_t)n % d) ST_FAIL("div24 %lld / %u", (long long)n, d);
+      }
   }
   // index bijection (exhaustive on small materials) and round trips
   const char *mats[] = { "KRvK", "KRRvK", "KBNvK", "KQRvKR", "KNNNvKB", "KQRBvKQRN", "KRRRvKRR" };

@@ -574,6 +574,32 @@ __device__ int synth_exit(const Tabs &T, const Mat &M, int stm, uint64_t q0, uin
   return synth_value(synth_hash(M.seedmix, o, 1, s2, i2));
 }
 
+// The king's moves of a verification unit: every candidate of a slice has
+// the same kings, so where stm's king can step and how each step is
+// normalised (slice, mirror, diagonal flip: src/tb8gen.c:230-238, a function
+// of the kings alone) is worked out once per (slice, side) and kept in LDS,
+// one entry per target square: s2 | m << 9 | fl << 15.  Lane i fills
+// square i; call with the whole wave.
+__device__ __forceinline__ void king_table(const Tabs &T, int s, int stm, uint32_t *kt, int lane)
+{
+  int ks = T.kksq[s][stm], oks = T.kksq[s][stm ^ 1];
+  uint32_t e = 0;
+  if (((T.king[ks] & ~T.king[oks]) >> lane) & 1) {
+    int wk = stm == WHITE ? lane : oks, bk = stm == WHITE ? oks : lane;
+    uint64_t q;
+    int s2 = normalize(T, (uint64_t)wk | ((uint64_t)bk << 8), &q);
+    e = (uint32_t)s2 | (uint32_t)mirror_mask(wk) << 9 | (uint32_t)flip_test(wk, bk) << 15;
+  }
+  kt[lane] = e;
+}
+
+// q0 (stm's king moved to a target of kt) normalised by its table entry
+__device__ __forceinline__ uint64_t king_norm(uint32_t e, uint64_t q0)
+{
+  uint64_t q = q0 ^ (0x0101010101010101ull * (uint64_t)((e >> 9) & 63));
+  return (e >> 15) & 1 ? flip_diag_bytes(q) : q;
+}
+
 // check_successors_worker + check_moves + check_king_moves
 // (src/tb8gen.c:529-650): is candidate idx of slice s a loss for stm?
 // Candidates come from cand & ~wins(stm), and wins(stm) holds every position
@@ -585,7 +611,7 @@ __device__ int synth_exit(const Tabs &T, const Mat &M, int stm, uint64_t q0, uin
 // spent without an answer.
 __device__ __forceinline__ int verify_pos(const Tabs &T, const Mat &M, int stm, int s, uint64_t idx,
                           const uint64_t *__restrict__ wins_o, const uint64_t *__restrict__ sub_o,
-                          int budget)
+                          const uint32_t *kt, int budget)
 {
   int used = 0;  // bitmap reads so far; at budget the answer is left open (2)
   int o = stm ^ 1;
@@ -655,8 +681,9 @@ __device__ __forceinline__ int verify_pos(const Tabs &T, const Mat &M, int stm, 
     c &= c - 1;
     int jj = slot_of(p, to, M.col80[o]);
     int l = sq_get(M.set_of, jj);
-    uint64_t q0 = sq_set(p, stm, to), q;
-    int s2 = normalize(T, q0, &q);
+    uint32_t e = kt[to];
+    uint64_t q0 = sq_set(p, stm, to), q = king_norm(e, q0);
+    int s2 = (int)(e & 511);
     if (used++ >= budget) return 2;
     if (M.synth && s2 != M.synth_slice) {
       int v = synth_exit(T, M, stm, q0, q, unsafe, ks, jj, l, s2, idx);
@@ -667,21 +694,37 @@ __device__ __forceinline__ int verify_pos(const Tabs &T, const Mat &M, int stm, 
     const uint64_t *sb = sub_o + phys(M, s2) * M.ctotal[o] + M.coff[l];
     if (!bm_test(sb, sq_to_idx(T, M.capt[l], q))) return 0;
   }
+  // quiet king moves four at a time, like the pieces' (a lost candidate
+  // reads every one); a synthetic job's exits need no read
   uint64_t kq = kb & ~occ;
   while (kq) {
-    int to = ctz64(kq);
-    kq &= kq - 1;
-    uint64_t q0 = sq_set(p, stm, to), q;
-    int s2 = normalize(T, q0, &q);
-    if (used++ >= budget) return 2;
-    if (M.synth && s2 != M.synth_slice) {
-      int v = synth_exit(T, M, stm, q0, q, unsafe, ks, -1, 0, s2, idx);
-      if (v == 3 || v == 2) continue;
-      return 0;
+    if (used >= budget) return 2;
+    used += 4;
+    uint64_t i4[4], w4[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      i4[k] = ~0ull;
+      if (kq) {
+        int to = ctz64(kq);
+        kq &= kq - 1;
+        uint32_t e = kt[to];
+        uint64_t q0 = sq_set(p, stm, to), q = king_norm(e, q0);
+        int s2 = (int)(e & 511);
+        if (M.synth && s2 != M.synth_slice) {
+          int v = synth_exit(T, M, stm, q0, q, unsafe, ks, -1, 0, s2, idx);
+          if (v != 3 && v != 2) return 0;
+        } else {
+          i4[k] = phys(M, s2) * M.words * 64 +
+                  (M.all_single && q == q0 ? king_moved_idx(M.ii, idx, q0, ks, to)
+                                           : sq_to_idx(T, M.ii, q));
+        }
+      }
     }
-    uint64_t i2 = M.all_single && q == q0 ? king_moved_idx(M.ii, idx, q0, ks, to)
-                                          : sq_to_idx(T, M.ii, q);
-    if (!bm_test(wins_o + phys(M, s2) * M.words, i2)) return 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) w4[k] = i4[k] != ~0ull ? gload(wins_o, i4[k] >> 6) : ~0ull;
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+      if (!((w4[k] >> (i4[k] & 63)) & 1)) return 0;
   }
   return 1;
 }
@@ -706,6 +749,7 @@ __global__ __launch_bounds__(256, 5) void k_verify_j(JobDev *const *__restrict__
 {
   __shared__ Tabs T;
   __shared__ Queue Q;
+  __shared__ uint32_t KT[4][64];  // king_table per wave
   __shared__ Mat Ms;
   load_tabs(T);
   int cached = -1;
@@ -718,7 +762,7 @@ __global__ __launch_bounds__(256, 5) void k_verify_j(JobDev *const *__restrict__
   // plane pointers are uniform and kept in scalar registers.
   uint64_t plo = 1, phi = 0;
   bool skip = true, run = false, exits = false;
-  int stm = 0, n = 0;
+  int stm = 0, n = 0, kt_key = -1;
   JobDev *Jp = nullptr;
   const uint64_t *candr = nullptr, *extra = nullptr, *wins_s = nullptr, *wins_o = nullptr,
                  *sub_o = nullptr;
@@ -756,6 +800,11 @@ __global__ __launch_bounds__(256, 5) void k_verify_j(JobDev *const *__restrict__
     uint64_t u0 = within * (uint64_t)uw;
     uint64_t uend = u0 + uw < M.words ? u0 + uw : M.words;
     uint64_t mysum = 0;
+    if (s * 2 + stm != kt_key) {  // uniform; the wave's earlier reads of KT are done
+      kt_key = s * 2 + stm;
+      king_table(T, s, stm, KT[wave], lane);
+      wave_sync();
+    }
     // one 64-word chunk per wave per pass (the 4 waves cover 256 words); a
     // chunk without candidates costs one load and one store per lane
     for (uint64_t wbase = u0 + (uint64_t)wave * 64; wbase < uend; wbase += 256) {
@@ -792,7 +841,8 @@ __global__ __launch_bounds__(256, 5) void k_verify_j(JobDev *const *__restrict__
             for (int e = lane; e < todo; e += 64) {
               uint16_t v = Q.e[wave][e];
               uint64_t idx = (wbase + (v >> 6)) * 64 + (v & 63);
-              int r = idx < M.ii.size ? verify_pos(T, M, stm, s, idx, wins_o, sub_o, budget) : 0;
+              int r = idx < M.ii.size ? verify_pos(T, M, stm, s, idx, wins_o, sub_o, KT[wave], budget)
+                                      : 0;
               if (r == 0)
                 __hip_atomic_fetch_and(reinterpret_cast<unsigned long long *>(&Q.res[wave][v >> 6]),
                                        ~(1ull << (v & 63)), __ATOMIC_RELAXED,
