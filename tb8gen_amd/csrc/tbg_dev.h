@@ -295,24 +295,45 @@ TBG_HD uint64_t idx_to_sq(const Tabs &T, const Idx &ii, uint64_t idx, uint64_t *
   return occ;
 }
 
+// What moved_idx_l reads about the sets after kj (all sets single): their
+// squares in position p and their factors (0 past the last set), and kj's
+// own factor, gathered once per moving piece so that the loop over its
+// targets reads no table (a table read there would also wait on every LDS
+// atomic the loop has queued).
+struct Later {
+  uint32_t fj;
+  uint32_t f[MAXS - 1];
+  int s[MAXS - 1];
+};
+TBG_HD Later later_sets(const Idx &ii, uint64_t p, int kj)
+{
+  Later L;
+  L.fj = ii.factor[kj];
+  for (int k = 0; k < MAXS - 1; k++) {
+    int kk = kj + 1 + k;
+    L.f[k] = 0;
+    L.s[k] = 0;
+    if (kk < ii.numsets) {
+      L.f[k] = ii.factor[kk];
+      L.s[k] = sq_get(p, 2 + kk);
+    }
+  }
+  return L;
+}
 // Index after the piece of set kj (all sets single) moves from a to b, from
 // the index before: its own rank changes, and each later piece strictly
 // between a and b shifts its rank by one (its set of lower occupied squares
 // gains or loses a).  occb = kings + pieces of the sets before kj;
-// rank_a = rank of a among the squares free of occb.
-TBG_HD uint64_t moved_idx(const Idx &ii, uint64_t idx, uint64_t p, int kj, int a, int b,
-                          uint64_t occb, uint32_t rank_a)
+// rank_a = rank of a among the squares free of occb; L = later_sets(kj).
+TBG_HD uint64_t moved_idx_l(const Later &L, uint64_t idx, int a, int b, uint64_t occb,
+                            uint32_t rank_a)
 {
-  uint32_t f = ii.factor[kj];
   uint32_t rank_b = (uint32_t)(b - popc64(occb & (bit(b) - 1)));
   int lo = a < b ? a : b, hi = a < b ? b : a;
-  uint32_t acc = 0;  // sum of the later factors: below 2 * factor[kj + 1]
-  for (int k = kj + 1; k < MAXS; k++) {
-    if (k >= ii.numsets) break;
-    int s = sq_get(p, 2 + k);
-    acc += (s > lo && s < hi) ? ii.factor[k] : 0u;
-  }
-  uint64_t r = idx + (uint64_t)rank_b * f - (uint64_t)rank_a * f;  // modulo 2^64
+  uint32_t acc = 0;
+  for (int k = 0; k < MAXS - 1; k++) acc += (L.s[k] > lo && L.s[k] < hi) ? L.f[k] : 0u;
+  int64_t dr = (int64_t)(int32_t)(rank_b - rank_a) * (int64_t)L.fj;
+  uint64_t r = idx + (uint64_t)dr;
   return a < b ? r + acc : r - acc;
 }
 

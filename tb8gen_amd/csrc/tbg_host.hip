@@ -2238,7 +2238,8 @@ extern "C" int tbg_host_selftest(char *msg, int len)
         for (int b = 0; b < 64; b++) {
           if (occ & bit(b)) continue;
           uint64_t want = sq_to_idx(T, M.ii, sq_set(p, 2 + kj, b));
-          if (moved_idx(M.ii, idx, p, kj, a, b, ob, ra) != want) ST_FAIL("%s: moved_idx", nm);
+          if (moved_idx_l(later_sets(M.ii, p, kj), idx, a, b, ob, ra) != want)
+            ST_FAIL("%s: moved_idx", nm);
         }
       }
     }

@@ -47,6 +47,17 @@ __device__ __forceinline__ uint64_t uni64(uint64_t x)
   uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(x >> 32));
   return (uint64_t)lo | ((uint64_t)hi << 32);
 }
+// later_sets of a piece with its factors in scalar registers (the same for
+// every lane): the loads of the factors complete before the caller's loop
+// over targets, not inside it
+__device__ __forceinline__ Later later_sets_u(const Idx &ii, uint64_t p, int kj)
+{
+  Later L = later_sets(ii, p, kj);
+  L.fj = (uint32_t)uni((int)L.fj);
+#pragma unroll
+  for (int k = 0; k < MAXS - 1; k++) L.f[k] = (uint32_t)uni((int)L.f[k]);
+  return L;
+}
 template <class X> __device__ __forceinline__ X *uniptr(X *p)
 {
   return reinterpret_cast<X *>(uni64(reinterpret_cast<uint64_t>(p)));
@@ -646,9 +657,11 @@ __device__ __forceinline__ int verify_pos(const Tabs &T, const Mat &M, int stm, 
     // together instead of one round trip after another
     uint64_t ob = 0;
     uint32_t ra = 0;
+    Later LS = {};
     if (M.all_single) {
       ob = occ_before(p, j - 2);
       ra = (uint32_t)(fr - popc64(ob & (bit(fr) - 1)));
+      LS = later_sets_u(M.ii, p, j - 2);
     }
     while (qm) {
       if (used >= budget) return 2;
@@ -660,7 +673,7 @@ __device__ __forceinline__ int verify_pos(const Tabs &T, const Mat &M, int stm, 
         if (qm) {
           int to = ctz64(qm);
           qm &= qm - 1;
-          i4[k] = M.all_single ? moved_idx(M.ii, idx, p, j - 2, fr, to, ob, ra)
+          i4[k] = M.all_single ? moved_idx_l(LS, idx, fr, to, ob, ra)
                                : sq_to_idx(T, M.ii, sq_set(p, j, to));
         }
       }
