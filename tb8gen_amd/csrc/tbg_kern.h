@@ -66,12 +66,27 @@ TBG_HD uint64_t uncap_units(const Mat &M, int side)
   return u;
 }
 
+// Verification units of a slice larger than VERIFY_SPLIT_SIZE positions are
+// at least VERIFY_BIG_UNIT words: most of its plies have few candidates,
+// and a unit with none is then read with many loads in flight and its L
+// words cleared in one go (small units kept one load per wave in flight).
+// Such a slice still has tens of thousands of units.
+constexpr uint64_t VERIFY_SPLIT_SIZE = 1ull << 22;  // positions per slice
+constexpr uint32_t VERIFY_BIG_UNIT = 4096;
+TBG_HD uint32_t verify_unit(const Mat &M, uint32_t uw)
+{
+  return M.ii.size > VERIFY_SPLIT_SIZE && uw < VERIFY_BIG_UNIT ? VERIFY_BIG_UNIT : uw;
+}
+
 TBG_HD uint64_t units_per_slice(const Mat &M, uint64_t nblk, int kt, int side, const Geom &g)
 {
   switch (kt) {
   case KT_PRED: return nblk;
   case KT_UNCAP: return uncap_units(M, side);
-  case KT_VERIFY: return (M.words + g.verify_uw - 1) / g.verify_uw;
+  case KT_VERIFY: {
+    uint32_t uw = verify_unit(M, g.verify_uw);
+    return (M.words + uw - 1) / uw;
+  }
   default: return (M.words + g.sweep_uw - 1) / g.sweep_uw;
   }
 }
@@ -85,7 +100,6 @@ __global__ void k_pred_wide_j(JobDev *const *jobs, const Pair *pairs, const uint
                               int sw0, int kind);
 __global__ void k_uncap_j(JobDev *const *jobs, const Pair *pairs, const uint64_t *P, int np, int sw0,
                           int kind, const unsigned *flags, uint64_t lds_words);
-constexpr uint64_t VERIFY_SPLIT_SIZE = 1ull << 22;  // positions per slice
 template <bool SPLIT>
 __global__ __launch_bounds__(256, 5) void k_verify_j(JobDev *const *jobs, const Pair *pairs,
                                                    const uint64_t *P, int np, int sw0, int uw);

@@ -355,13 +355,42 @@ __device__ __forceinline__ int unit_slice(const Mat &M, int sw0, uint64_t local,
 
 // Word wi of a pred block's frontier: bits below lo and from hi on (the
 // neighbouring blocks' positions sharing its first and last words) masked.
-__device__ __forceinline__ uint64_t block_word(const uint64_t *srcs, uint64_t wlo, uint64_t wi,
-                                               uint64_t nbw, uint64_t lo, uint64_t hi)
+__device__ __forceinline__ uint64_t block_mask(uint64_t x, uint64_t wi, uint64_t nbw, uint64_t lo,
+                                               uint64_t hi)
 {
-  uint64_t x = gload(srcs, wlo + wi);
   if (wi == 0) x &= ~0ull << (lo & 63);
   if (wi + 1 == nbw && (hi & 63)) x &= (1ull << (hi & 63)) - 1;
   return x;
+}
+__device__ __forceinline__ uint64_t block_word(const uint64_t *srcs, uint64_t wlo, uint64_t wi,
+                                               uint64_t nbw, uint64_t lo, uint64_t hi)
+{
+  return block_mask(gload(srcs, wlo + wi), wi, nbw, lo, hi);
+}
+
+// OR of words [w0, w1) of p (w0 < w1) over the work-group (thread t of nt
+// reads w0 + t, w0 + t + nt, ...), eight loads per thread in flight at a
+// time: the addresses past w1 are clamped to w1 - 1 and their words
+// dropped, so no branch separates the loads.  edge(x, w) masks word w.
+template <class Edge>
+__device__ __forceinline__ uint64_t or_words(const uint64_t *p, uint64_t w0, uint64_t w1, int t,
+                                             int nt, Edge edge)
+{
+  uint64_t acc = 0;
+  for (uint64_t b = w0 + (uint64_t)t; b < w1; b += (uint64_t)nt * 8) {
+    uint64_t v[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      uint64_t w = b + (uint64_t)k * nt;
+      v[k] = gload(p, w < w1 ? w : w1 - 1);
+    }
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      uint64_t w = b + (uint64_t)k * nt;
+      acc |= w < w1 ? edge(v[k], w) : 0;
+    }
+  }
+  return acc;
 }
 
 // predecessors_worker (src/tb8gen.c:484-516) over every (job, side) of a
@@ -809,10 +838,24 @@ __global__ __launch_bounds__(256, 5) void k_verify_j(JobDev *const *__restrict__
     if (skip) continue;
     const Mat &M = Ms;
     uint64_t within;
-    int s = unit_slice(M, sw0, u - plo, (M.words + uw - 1) / uw, &within);
-    uint64_t u0 = within * (uint64_t)uw;
-    uint64_t uend = u0 + uw < M.words ? u0 + uw : M.words;
+    uint32_t uwe = verify_unit(M, (uint32_t)uw);
+    int s = unit_slice(M, sw0, u - plo, (M.words + uwe - 1) / uwe, &within);
+    uint64_t u0 = within * (uint64_t)uwe;
+    uint64_t uend = u0 + uwe < M.words ? u0 + uwe : M.words;
     uint64_t mysum = 0;
+    // A large unit without candidates (and exits) only has its L words
+    // cleared.  Every thread's candidate loads go out together; a unit with
+    // some is then verified as below, its words found in cache.
+    if (uwe >= VERIFY_BIG_UNIT) {
+      uint64_t sb = phys(M, s) * M.words;
+      auto none = [](uint64_t x, uint64_t) { return x; };
+      uint64_t any = or_words(candr + sb, u0, uend, threadIdx.x, 256, none);
+      if (exits) any |= or_words(extra + sb, u0, uend, threadIdx.x, 256, none);
+      if (!__syncthreads_or(any != 0)) {
+        for (uint64_t w = u0 + threadIdx.x; w < uend; w += 256) newL[sb + w] = 0;
+        continue;
+      }
+    }
     if (s * 2 + stm != kt_key) {  // uniform; the wave's earlier reads of KT are done
       kt_key = s * 2 + stm;
       king_table(T, s, stm, KT[wave], lane);
@@ -887,7 +930,7 @@ __global__ __launch_bounds__(256, 5) void k_verify_j(JobDev *const *__restrict__
       }
       mysum += popc64(r);
     }
-    uint64_t sum = wave_sum64(mysum);
+    uint64_t sum = __ballot(mysum != 0) ? wave_sum64(mysum) : 0;
     if (lane == 0 && sum) atomicAdd(&Jp->cnt[stm], (unsigned long long)sum);
   }
 }
@@ -986,7 +1029,7 @@ __global__ __launch_bounds__(256) void k_sweep_j(JobDev *const *__restrict__ job
         }
       }
     }
-    mycnt = wave_sum64(mycnt);
+    mycnt = __ballot(mycnt != 0) ? wave_sum64(mycnt) : 0;
     if (lane == 0 && mycnt) atomicAdd(&Jp->cnt[2 + stm], (unsigned long long)mycnt);
   }
 }
