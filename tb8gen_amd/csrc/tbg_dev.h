@@ -30,7 +30,8 @@ struct Idx {
   uint64_t size;
   uint32_t factor[MAXS];     // below 2^32 for up to 8 pieces (at most 61*60*59*58*57)
   double inv[MAXS];          // 1.0 / factor[k], for digit extraction
-  float invf[MAXS];          // the same in single precision (indices below 2^24)
+  float invf[MAXS];          // the same in single precision
+  int small_q;               // every set's radix (its digit's range) is below 2^21
 };
 
 // Everything a kernel needs to know about a material.
@@ -194,17 +195,19 @@ TBG_HD uint64_t div_fix(uint64_t n, uint64_t d, double inv, uint64_t *rem)
   return q;
 }
 
-// floor(n / d) for n < 2^24 and d >= 2: n is exact in single precision and
-// the product with the rounded 1/d is within 1 of n/d, so one fix-up step
-// either way gives the quotient; q and d both fit 24 bits, so their product
-// is a single 24-bit multiply.
-TBG_HD uint32_t div24(uint32_t n, uint32_t d, float inv, uint32_t *rem)
+// floor(n / d) for a quotient below 2^21 (a digit of the index: at most its
+// set's radix).  n is converted in two halves (relative error below 2^-23),
+// the product with the rounded 1/d adds 2^-23 more, so the estimate is
+// within 2^21 * 2^-22 = 0.5 of n / d and one fix-up step either way gives
+// the quotient; q * d is one 32 x 32 -> 64-bit multiply.
+TBG_HD uint32_t div_small(uint64_t n, uint32_t d, float inv, uint64_t *rem)
 {
-  uint32_t q = (uint32_t)((float)n * inv);
-  int32_t r = (int32_t)(n - (q & 0xffffffu) * (d & 0xffffffu));
-  if (r < 0) { q--; r += (int32_t)d; }
-  else if ((uint32_t)r >= d) { q++; r -= (int32_t)d; }
-  *rem = (uint32_t)r;
+  float fn = (float)(uint32_t)(n >> 32) * 4294967296.0f + (float)(uint32_t)n;
+  uint32_t q = (uint32_t)(fn * inv);
+  int64_t r = (int64_t)(n - (uint64_t)q * d);
+  if (r < 0) { q--; r += (int64_t)d; }
+  else if ((uint64_t)r >= d) { q++; r -= (int64_t)d; }
+  *rem = (uint64_t)r;
   return q;
 }
 
@@ -248,16 +251,15 @@ TBG_HD uint64_t idx_to_sq(const Tabs &T, const Idx &ii, uint64_t idx, uint64_t *
 {
   uint64_t p = *pp;
   uint64_t occ = bit(sq_get(p, 0)) | bit(sq_get(p, 1));
-  bool small = ii.size < (1u << 24);  // every table of up to 6 pieces
   for (int k = 0; k < MAXS; k++) {
     if (k >= ii.numsets) break;
     uint64_t d;
     if (ii.factor[k] == 1) {
       d = idx;
       idx = 0;
-    } else if (small) {
-      uint32_t rem;
-      d = div24((uint32_t)idx, ii.factor[k], ii.invf[k], &rem);
+    } else if (ii.small_q) {
+      uint64_t rem;
+      d = div_small(idx, ii.factor[k], ii.invf[k], &rem);
       idx = rem;
     } else {
       uint64_t rem;

@@ -145,6 +145,8 @@ void calc_idx(Idx &ii)
     n -= ii.mult[i];
   }
   uint64_t f = 1;
+  ii.small_q = 1;
+  for (int i = 0; i < ii.numsets; i++) ii.small_q &= sub[i] < (1u << 21);
   for (int i = MAXS - 1; i >= 0; i--) {
     if (i >= ii.numsets) { ii.factor[i] = 0; ii.inv[i] = 0; ii.invf[i] = 0; continue; }
     assert(f < (1ull << 32));  // the sets after the first hold at most 5 of 8 pieces
@@ -2165,15 +2167,18 @@ extern "C" int tbg_host_selftest(char *msg, int len)
     if (kkmap(wk, bk) != s) ST_FAIL("kksq/kkmap disagree at %d", s);
     if ((s >= 441) != (on_diag(wk) && on_diag(bk))) ST_FAIL("diagonal slice numbering");
   }
-  // div24 against integer division around quotient boundaries
-  for (uint32_t d = 2; d < (1u << 24); d = d * 3 / 2 + 1) {
+  // div_small against integer division around quotient boundaries, for
+  // divisors up to 2^32 and quotients up to 2^21
+  for (uint64_t d = 2; d < (1ull << 32); d = d * 3 / 2 + 1) {
     float inv = 1.0f / (float)d;
-    for (uint64_t q = 0; q * d < (1u << 24); q = q * 2 + 1)
+    for (uint64_t q = 0; q < (1u << 21); q = q * 2 + 1)
       for (int e = -1; e <= 1; e++) {
         int64_t n = (int64_t)(q * d) + e;
-        if (n < 0 || n >= (1 << 24)) continue;
-        uint32_t r, qq = div24((uint32_t)n, d, inv, &r);
-        if (qq != (uint64_t)n / d || r != (uint64_t)n % d) ST_FAIL("div24 %lld / %u", (long long)n, d);
+        if (n < 0 || (uint64_t)n / d >= (1u << 21)) continue;
+        uint64_t r;
+        uint32_t qq = div_small((uint64_t)n, (uint32_t)d, inv, &r);
+        if (qq != (uint64_t)n / d || r != (uint64_t)n % d)
+          ST_FAIL("div_small %lld / %llu", (long long)n, (unsigned long long)d);
       }
   }
   // index bijection (exhaustive on small materials) and round trips
