@@ -845,8 +845,9 @@ __global__ __launch_bounds__(256, 5) void k_verify_j(JobDev *const *__restrict__
     uint64_t mysum = 0;
     // A large unit without candidates (and exits) only has its L words
     // cleared.  Every thread's candidate loads go out together; a unit with
-    // some is then verified as below, its words found in cache.
-    if (uwe >= VERIFY_BIG_UNIT) {
+    // some is then verified as below, its words found in cache.  (Large
+    // units come with large slices, which only the SPLIT form runs.)
+    if (SPLIT && uwe >= VERIFY_BIG_UNIT) {
       uint64_t sb = phys(M, s) * M.words;
       auto none = [](uint64_t x, uint64_t) { return x; };
       uint64_t any = or_words(candr + sb, u0, uend, threadIdx.x, 256, none);
