@@ -356,6 +356,30 @@ TBG_HD uint64_t king_moved_idx(const Idx &ii, uint64_t idx, uint64_t p, int from
   return from < to ? idx + acc : idx - acc;
 }
 
+// king_moved_idx with the sets' squares (of the position before or after
+// the move) and factors gathered beforehand: a king move's rank shifts
+// come from every set, each factor 0 past the last set
+struct AllSets {
+  uint32_t f[MAXS];
+  int s[MAXS];
+};
+TBG_HD AllSets all_sets(const Idx &ii, uint64_t p)
+{
+  AllSets A;
+  for (int k = 0; k < MAXS; k++) {
+    A.f[k] = k < ii.numsets ? ii.factor[k] : 0u;
+    A.s[k] = sq_get(p, 2 + k);
+  }
+  return A;
+}
+TBG_HD uint64_t king_moved_idx_a(const AllSets &A, uint64_t idx, int from, int to)
+{
+  int lo = from < to ? from : to, hi = from < to ? to : from;
+  uint32_t acc = 0;
+  for (int k = 0; k < MAXS; k++) acc += (A.s[k] > lo && A.s[k] < hi) ? A.f[k] : 0u;
+  return from < to ? idx + acc : idx - acc;
+}
+
 // kings + pieces of the sets before kj (all sets single)
 TBG_HD uint64_t occ_before(uint64_t p, int kj)
 {

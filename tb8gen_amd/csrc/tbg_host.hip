@@ -2182,7 +2182,9 @@ extern "C" int tbg_host_selftest(char *msg, int len)
       }
   }
   // index bijection (exhaustive on small materials) and round trips
-  const char *mats[] = { "KRvK", "KRRvK", "KBNvK", "KQRvKR", "KNNNvKB", "KQRBvKQRN", "KRRRvKRR" };
+  // KRRRRRvK: a set whose radix (C(62, 5)) takes the double-precision digits
+  const char *mats[] = { "KRvK", "KRRvK", "KBNvK", "KQRvKR", "KNNNvKB", "KQRBvKQRN", "KRRRvKRR",
+                         "KRRRRRvK" };
   for (const char *nm : mats) {
     Mat M;
     if (setup_mat(M, nm) != TBG_OK) ST_FAIL("setup %s", nm);
@@ -2270,6 +2272,8 @@ extern "C" int tbg_host_selftest(char *msg, int len)
           uint64_t q0 = sq_set(p, stm, to), q;
           if (normalize(T, q0, &q) < 0 || q != q0) continue;
           tested++;
+          if (king_moved_idx_a(all_sets(M.ii, q0), idx, from, to) != sq_to_idx(T, M.ii, q0))
+            ST_FAIL("%s: king_moved_idx_a", nm);
           if (king_moved_idx(M.ii, idx, q0, from, to) != sq_to_idx(T, M.ii, q0))
             ST_FAIL("%s: king_moved_idx", nm);
         }

@@ -230,6 +230,10 @@ __device__ void king_unmoves_wave(const Tabs &T, const Mat &M, int stm, bool ok,
 {
   int from = uni(sq_get(p, stm)), oks = uni(sq_get(p, stm ^ 1));
   uint64_t targets = uni64(T.king[from] & ~T.king[oks]);
+  // the sets' squares and (scalar) factors, gathered once for every target
+  AllSets A = all_sets(M.ii, p);
+#pragma unroll
+  for (int k = 0; k < MAXS; k++) A.f[k] = (uint32_t)uni((int)A.f[k]);
   while (targets) {
     int t = ctz64(targets);
     targets &= targets - 1;
@@ -250,7 +254,7 @@ __device__ void king_unmoves_wave(const Tabs &T, const Mat &M, int stm, bool ok,
       uint64_t q0 = sq_set(p, stm, t);
       uint64_t q = q0 ^ (0x0101010101010101ull * (uint64_t)m);
       if (fl) q = flip_diag_bytes(q);
-      g = M.all_single && m == 0 && !fl ? king_moved_idx(M.ii, idx, q0, from, t)
+      g = M.all_single && m == 0 && !fl ? king_moved_idx_a(A, idx, from, t)
                                          : sq_to_idx(T, M.ii, q);
       if (dual) gd = sq_to_idx(T, M.ii, mirror_nonkings(q));
     }
