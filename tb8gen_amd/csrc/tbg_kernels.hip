@@ -1133,6 +1133,9 @@ __device__ void synth_exits_pos(const Tabs &T, const Mat &M, uint64_t p, uint64_
   }
   uint64_t blocked = own | T.king[oks] | uo;
   bool ww = false, ll = false;
+  AllSets A = all_sets(M.ii, p);
+#pragma unroll
+  for (int k = 0; k < MAXS; k++) A.f[k] = (uint32_t)uni((int)A.f[k]);
   // the king's moves out of the slice, their slice and symmetry prepared on
   // the host
   for (int e = 0; e < 8; e++) {
@@ -1151,7 +1154,7 @@ __device__ void synth_exits_pos(const Tabs &T, const Mat &M, uint64_t p, uint64_
       uint64_t c = sq_set(q, jj, sq_get(q, M.ii.last[k]));
       v = synth_value(synth_hash(M.seedmix, o, 2, s2 * 8 + k, sq_to_idx(T, M.capt[k], c)));
     } else {
-      uint64_t i2 = M.all_single && m == 0 && !fl ? king_moved_idx(M.ii, idx, q0, from, to)
+      uint64_t i2 = M.all_single && m == 0 && !fl ? king_moved_idx_a(A, idx, from, to)
                                                   : sq_to_idx(T, M.ii, q);
       v = synth_value(synth_hash(M.seedmix, o, 1, s2, i2));
     }
