@@ -372,6 +372,19 @@ TBG_HD AllSets all_sets(const Idx &ii, uint64_t p)
   }
   return A;
 }
+// sq_to_idx of an all-single material from its gathered factors
+TBG_HD uint64_t sq_to_idx_a(const AllSets &A, uint64_t p)
+{
+  uint64_t occ = bit(sq_get(p, 0)) | bit(sq_get(p, 1));
+  uint64_t idx = 0;
+  for (int k = 0; k < MAXS; k++) {
+    if (A.f[k] == 0) break;
+    int s = sq_get(p, 2 + k);
+    idx += (uint64_t)(uint32_t)(s - popc64(occ & (bit(s) - 1))) * A.f[k];
+    occ |= bit(s);
+  }
+  return idx;
+}
 TBG_HD uint64_t king_moved_idx_a(const AllSets &A, uint64_t idx, int from, int to)
 {
   int lo = from < to ? from : to, hi = from < to ? to : from;

@@ -254,9 +254,13 @@ __device__ void king_unmoves_wave(const Tabs &T, const Mat &M, int stm, bool ok,
       uint64_t q0 = sq_set(p, stm, t);
       uint64_t q = q0 ^ (0x0101010101010101ull * (uint64_t)m);
       if (fl) q = flip_diag_bytes(q);
-      g = M.all_single && m == 0 && !fl ? king_moved_idx_a(A, idx, from, t)
-                                         : sq_to_idx(T, M.ii, q);
-      if (dual) gd = sq_to_idx(T, M.ii, mirror_nonkings(q));
+      g = !M.all_single ? sq_to_idx(T, M.ii, q)
+          : m == 0 && !fl ? king_moved_idx_a(A, idx, from, t)
+                          : sq_to_idx_a(A, q);
+      if (dual) {
+        uint64_t qd = mirror_nonkings(q);
+        gd = M.all_single ? sq_to_idx_a(A, qd) : sq_to_idx(T, M.ii, qd);
+      }
     }
     uint64_t *plane = cand + phys(M, s2) * M.words;
     wave_mark(plane, g, lane);
@@ -1154,8 +1158,9 @@ __device__ void synth_exits_pos(const Tabs &T, const Mat &M, uint64_t p, uint64_
       uint64_t c = sq_set(q, jj, sq_get(q, M.ii.last[k]));
       v = synth_value(synth_hash(M.seedmix, o, 2, s2 * 8 + k, sq_to_idx(T, M.capt[k], c)));
     } else {
-      uint64_t i2 = M.all_single && m == 0 && !fl ? king_moved_idx_a(A, idx, from, to)
-                                                  : sq_to_idx(T, M.ii, q);
+      uint64_t i2 = !M.all_single ? sq_to_idx(T, M.ii, q)
+                    : m == 0 && !fl ? king_moved_idx_a(A, idx, from, to)
+                                    : sq_to_idx_a(A, q);
       v = synth_value(synth_hash(M.seedmix, o, 1, s2, i2));
     }
     ww |= v == -2;
