@@ -339,6 +339,33 @@ TBG_HD uint64_t moved_idx_l(const Later &L, uint64_t idx, int a, int b, uint64_t
   return a < b ? r + acc : r - acc;
 }
 
+// idx_to_sq of an all-single material whose digits take div_small
+// (ii.small_q): set k's one piece is slot 2 + k, the last set's factor is 1.
+// Written per set index so that the table reads can all be issued before
+// the first is needed.
+TBG_HD uint64_t idx_to_sq_1(const Idx &ii, uint64_t idx, uint64_t *pp)
+{
+  uint64_t p = *pp;
+  uint64_t occ = bit(sq_get(p, 0)) | bit(sq_get(p, 1));
+  int n = ii.numsets;
+  uint32_t f[MAXS];
+  float inv[MAXS];
+  for (int k = 0; k < MAXS; k++) {
+    f[k] = ii.factor[k];
+    inv[k] = ii.invf[k];
+  }
+  for (int k = 0; k < MAXS; k++) {
+    if (k >= n) break;
+    uint64_t d = idx;
+    if (k + 1 < n) d = div_small(idx, f[k], inv[k], &idx);
+    int s = select_zero(occ, (int)d);
+    p = sq_set(p, 2 + k, s);
+    occ |= bit(s);
+  }
+  *pp = p;
+  return occ;
+}
+
 // Index after a king moves from `from` to `to` when the position needs no
 // symmetry (normalize returned it unchanged), from the index before: the
 // kings precede every set, so each piece strictly between the two squares

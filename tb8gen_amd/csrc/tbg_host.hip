@@ -2216,6 +2216,12 @@ extern "C" int tbg_host_selftest(char *msg, int len)
             for (int a = 1; a < ii.mult[k]; a++)
               if (sq_get(p, ii.first[k] + a) <= sq_get(p, ii.first[k] + a - 1)) ST_FAIL("%s: set order", nm);
           if (sq_to_idx(T, ii, p) != idx) ST_FAIL("%s space %zu: round trip idx %llu slice %d", nm, sp, (unsigned long long)idx, s);
+          bool single = true;
+          for (int k = 0; k < ii.numsets; k++) single &= ii.mult[k] == 1;
+          if (single && ii.small_q) {
+            uint64_t p1 = kings;
+            if (idx_to_sq_1(ii, idx, &p1) != occ || p1 != p) ST_FAIL("%s space %zu: idx_to_sq_1", nm, sp);
+          }
           // permuting members within a set does not change the index
           for (int k = 0; k < ii.numsets; k++)
             if (ii.mult[k] >= 2) {

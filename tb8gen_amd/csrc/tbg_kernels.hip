@@ -429,6 +429,7 @@ __global__ __launch_bounds__(512) void k_pred_j(JobDev *const *__restrict__ jobs
   __shared__ Tabs T;
   __shared__ Mat Ms;
   extern __shared__ uint64_t dyn[];
+  constexpr bool kDecode1 = true;  // registers to spare at 4 waves/SIMD
 #include "tbg_pred.inc"
 }
 
@@ -440,6 +441,7 @@ __global__ __launch_bounds__(512, 6) void k_pred_wide_j(JobDev *const *__restric
   __shared__ Tabs T;
   __shared__ Mat Ms;
   extern __shared__ uint64_t dyn[];
+  constexpr bool kDecode1 = false;  // its spills would grow
 #include "tbg_pred.inc"
 }
 
