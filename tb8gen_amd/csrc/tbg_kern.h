@@ -66,16 +66,20 @@ TBG_HD uint64_t uncap_units(const Mat &M, int side)
   return u;
 }
 
-// Verification units of a slice larger than VERIFY_SPLIT_SIZE positions are
-// at least VERIFY_BIG_UNIT words: most of its plies have few candidates,
-// and a unit with none is then read with many loads in flight and its L
-// words cleared in one go (small units kept one load per wave in flight).
-// Such a slice still has tens of thousands of units.
+// Verification units of a slice of at least VERIFY_BIG_WORDS bitmap words
+// (7 and 8 pieces) are at least VERIFY_BIG_UNIT words: most of its plies
+// have few candidates, and a unit with none is then read with many loads in
+// flight and its L words cleared in one go (small units kept one load per
+// wave in flight).  Such a slice still has over a thousand units; a smaller
+// one keeps the geometry's unit, or a launch would have too few units to
+// fill the chip.  Only the two-pass form (slices above VERIFY_SPLIT_SIZE
+// positions) handles large units.
 constexpr uint64_t VERIFY_SPLIT_SIZE = 1ull << 22;  // positions per slice
 constexpr uint32_t VERIFY_BIG_UNIT = 4096;
+constexpr uint64_t VERIFY_BIG_WORDS = 1ull << 22;
 TBG_HD uint32_t verify_unit(const Mat &M, uint32_t uw)
 {
-  return M.ii.size > VERIFY_SPLIT_SIZE && uw < VERIFY_BIG_UNIT ? VERIFY_BIG_UNIT : uw;
+  return M.words >= VERIFY_BIG_WORDS && uw < VERIFY_BIG_UNIT ? VERIFY_BIG_UNIT : uw;
 }
 
 TBG_HD uint64_t units_per_slice(const Mat &M, uint64_t nblk, int kt, int side, const Geom &g)
