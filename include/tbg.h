@@ -274,7 +274,8 @@ int tbg_write_rtbw(tbg_ctx *ctx, const char *dir, char *path, size_t cap);
  * a NULL context sets the default that later contexts and servers start
  * from (a batch runs with the geometry of its first context). */
 typedef struct tbg_geometry {
-  int verify_unit_words;  /* bitmap words per verification work unit: multiple of 256, <= 65536 */
+  int verify_unit_words;  /* bitmap words per verification work unit: multiple of 256, <= 65536;
+                             slices of 2^22 words or more use at least 4096 */
   int sweep_unit_words;   /* bitmap words per W-sweep work unit: multiple of 512, <= 65536 */
   int pred_wg_per_cu;     /* grid caps, work-groups per CU, 1..64 */
   int uncap_wg_per_cu;
