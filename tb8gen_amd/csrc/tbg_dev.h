@@ -82,7 +82,6 @@ TBG_HD uint64_t sq_set(uint64_t p, int i, int v)
   int sh = i << 3;
   return (p & ~(0xffull << sh)) | ((uint64_t)v << sh);
 }
-TBG_HD uint64_t bytes_mask(int n) { return n >= 8 ? ~0ull : (1ull << (8 * n)) - 1; }
 
 // Slot holding square `to` among the slots selected by byte mask `valid`
 // (0x80 in each candidate byte); -1 if none.
@@ -429,19 +428,6 @@ TBG_HD uint64_t occ_before(uint64_t p, int kj)
     o |= bit(sq_get(p, 2 + k));
   }
   return o;
-}
-
-// Occupancy of each set (bitboard of its members), for set-wise capture
-// enumeration.
-TBG_HD void set_occ(const Idx &ii, uint64_t p, uint64_t *so)
-{
-  for (int k = 0; k < MAXS; k++) {
-    uint64_t b = 0;
-    if (k < ii.numsets)
-      for (int a = 0; a < MAXS; a++)
-        if (a < ii.mult[k]) b |= bit(sq_get(p, ii.first[k] + a));
-    so[k] = b;
-  }
 }
 
 // is `target` attacked by the non-king pieces of colour c (plus the king of
