@@ -305,11 +305,13 @@ struct Later {
   uint32_t fj;
   uint32_t f[MAXS - 1];
   int s[MAXS - 1];
+  int n;  // number of later sets
 };
 TBG_HD Later later_sets(const Idx &ii, uint64_t p, int kj)
 {
   Later L;
   L.fj = ii.factor[kj];
+  L.n = ii.numsets - 1 - kj;
   for (int k = 0; k < MAXS - 1; k++) {
     int kk = kj + 1 + k;
     L.f[k] = 0;
@@ -332,7 +334,11 @@ TBG_HD uint64_t moved_idx_l(const Later &L, uint64_t idx, int a, int b, uint64_t
   uint32_t rank_b = (uint32_t)(b - popc64(occb & (bit(b) - 1)));
   int lo = a < b ? a : b, hi = a < b ? b : a;
   uint32_t acc = 0;
-  for (int k = 0; k < MAXS - 1; k++) acc += (L.s[k] > lo && L.s[k] < hi) ? L.f[k] : 0u;
+#pragma unroll
+  for (int k = 0; k < MAXS - 1; k++) {
+    if (k >= L.n) break;  // the same for every lane: a scalar branch
+    acc += (L.s[k] > lo && L.s[k] < hi) ? L.f[k] : 0u;
+  }
   int64_t dr = (int64_t)(int32_t)(rank_b - rank_a) * (int64_t)L.fj;
   uint64_t r = idx + (uint64_t)dr;
   return a < b ? r + acc : r - acc;
@@ -415,7 +421,11 @@ TBG_HD uint64_t king_moved_idx_a(const AllSets &A, uint64_t idx, int from, int t
 {
   int lo = from < to ? from : to, hi = from < to ? to : from;
   uint32_t acc = 0;
-  for (int k = 0; k < MAXS; k++) acc += (A.s[k] > lo && A.s[k] < hi) ? A.f[k] : 0u;
+#pragma unroll
+  for (int k = 0; k < MAXS; k++) {
+    if (A.f[k] == 0) break;  // past the last set
+    acc += (A.s[k] > lo && A.s[k] < hi) ? A.f[k] : 0u;
+  }
   return from < to ? idx + acc : idx - acc;
 }
 

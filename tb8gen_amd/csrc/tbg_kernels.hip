@@ -54,6 +54,7 @@ __device__ __forceinline__ Later later_sets_u(const Idx &ii, uint64_t p, int kj)
 {
   Later L = later_sets(ii, p, kj);
   L.fj = (uint32_t)uni((int)L.fj);
+  L.n = uni(L.n);
 #pragma unroll
   for (int k = 0; k < MAXS - 1; k++) L.f[k] = (uint32_t)uni((int)L.f[k]);
   return L;
