@@ -208,12 +208,16 @@ __device__ __forceinline__ void wave_mark(uint64_t *plane, uint64_t g, int lane)
   uint64_t val = g == ~0ull ? 0 : bit(g & 63);
   uint32_t kp = dpp_move<0x138, 0xf>(~0u, key);  // the key of lane - 1
   uint64_t heads = __ballot(kp != key);         // lanes that start a run
-  seg_or_step<0x138, 0xf>(key, kp, val);
-  seg_or_step<0x112, 0xf>(key, val);
-  seg_or_step<0x114, 0xf>(key, val);
-  seg_or_step<0x118, 0xf>(key, val);
-  seg_or_step<0x142, 0xa>(key, val);  // lane 15 of rows 0, 2 into rows 1, 3
-  seg_or_step<0x143, 0xc>(key, val);  // lane 31 into rows 2, 3
+  // every target alone in its run (scattered targets, as a piece's moves
+  // across unrelated positions are): no scan
+  if (__ballot(key != ~0u && kp == key)) {
+    seg_or_step<0x138, 0xf>(key, kp, val);
+    seg_or_step<0x112, 0xf>(key, val);
+    seg_or_step<0x114, 0xf>(key, val);
+    seg_or_step<0x118, 0xf>(key, val);
+    seg_or_step<0x142, 0xa>(key, val);  // lane 15 of rows 0, 2 into rows 1, 3
+    seg_or_step<0x143, 0xc>(key, val);  // lane 31 into rows 2, 3
+  }
   bool last = ((heads >> 1 | 1ull << 63) >> lane) & 1;
   if (key != ~0u && last) atomic_or64(plane + key, val);
 }
