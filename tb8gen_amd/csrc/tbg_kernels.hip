@@ -59,6 +59,24 @@ __device__ __forceinline__ Later later_sets_u(const Idx &ii, uint64_t p, int kj)
   for (int k = 0; k < MAXS - 1; k++) L.f[k] = (uint32_t)uni((int)L.f[k]);
   return L;
 }
+// moved_idx_l as a 32-bit offset from the position's own index, for a
+// piece whose set lies inside a pred block: its factor is below the block
+// size (at most 327,552 positions, under 2^19) and the rank change below
+// 62, so the product fits a 24-bit multiply and the offset 32 bits.
+__device__ __forceinline__ int moved_delta(const Later &L, int a, int b, uint64_t occb,
+                                           uint32_t rank_a)
+{
+  int rank_b = b - popc64(occb & (bit(b) - 1));
+  int lo = a < b ? a : b, hi = a < b ? b : a;
+  int acc = 0;
+#pragma unroll
+  for (int k = 0; k < MAXS - 1; k++) {
+    if (k >= L.n) break;
+    acc += (L.s[k] > lo && L.s[k] < hi) ? (int)L.f[k] : 0;
+  }
+  int d = __mul24(rank_b - (int)rank_a, (int)L.fj);
+  return a < b ? d + acc : d - acc;
+}
 template <class X> __device__ __forceinline__ X *uniptr(X *p)
 {
   return reinterpret_cast<X *>(uni64(reinterpret_cast<uint64_t>(p)));
