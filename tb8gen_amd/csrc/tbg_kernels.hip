@@ -60,9 +60,10 @@ __device__ __forceinline__ Later later_sets_u(const Idx &ii, uint64_t p, int kj)
   return L;
 }
 // moved_idx_l as a 32-bit offset from the position's own index, for a
-// piece whose set lies inside a pred block: its factor is below the block
-// size (at most 327,552 positions, under 2^19) and the rank change below
-// 62, so the product fits a 24-bit multiply and the offset 32 bits.
+// piece whose set's factor is below 2^23 (a set inside a pred block, whose
+// factor is below the block size of at most 327,552 positions; any set of a
+// slice of up to 2^22 positions): the rank change is below 62, so the
+// product fits a 24-bit multiply and the offset 32 bits.
 __device__ __forceinline__ int moved_delta(const Later &L, int a, int b, uint64_t occb,
                                            uint32_t rank_a)
 {
@@ -682,6 +683,9 @@ __device__ __forceinline__ uint64_t king_norm(uint32_t e, uint64_t q0)
 // move).  Captures look up the captured piece's set from its slot.
 // Returns 0 (not lost), 1 (lost), or 2 when `budget` bitmap reads were
 // spent without an answer.
+// SMALL: the slice has at most VERIFY_SPLIT_SIZE positions, so quiet-move
+// targets are found with 32-bit offsets (moved_delta).
+template <bool SMALL>
 __device__ __forceinline__ int verify_pos(const Tabs &T, const Mat &M, int stm, int s, uint64_t idx,
                           const uint64_t *__restrict__ wins_o, const uint64_t *__restrict__ sub_o,
                           const uint32_t *kt, int budget)
@@ -735,8 +739,11 @@ __device__ __forceinline__ int verify_pos(const Tabs &T, const Mat &M, int stm, 
         if (qm) {
           int to = ctz64(qm);
           qm &= qm - 1;
-          i4[k] = M.all_single ? moved_idx_l(LS, idx, fr, to, ob, ra)
-                               : sq_to_idx(T, M.ii, sq_set(p, j, to));
+          if (SMALL && M.all_single)
+            i4[k] = (uint32_t)((int)idx + moved_delta(LS, fr, to, ob, ra));
+          else
+            i4[k] = M.all_single ? moved_idx_l(LS, idx, fr, to, ob, ra)
+                                 : sq_to_idx(T, M.ii, sq_set(p, j, to));
         }
       }
 #pragma unroll
@@ -931,8 +938,9 @@ __global__ __launch_bounds__(256, 5) void k_verify_j(JobDev *const *__restrict__
             for (int e = lane; e < todo; e += 64) {
               uint16_t v = Q.e[wave][e];
               uint64_t idx = (wbase + (v >> 6)) * 64 + (v & 63);
-              int r = idx < M.ii.size ? verify_pos(T, M, stm, s, idx, wins_o, sub_o, KT[wave], budget)
-                                      : 0;
+              int r = idx < M.ii.size
+                          ? verify_pos<!SPLIT>(T, M, stm, s, idx, wins_o, sub_o, KT[wave], budget)
+                          : 0;
               if (r == 0)
                 __hip_atomic_fetch_and(reinterpret_cast<unsigned long long *>(&Q.res[wave][v >> 6]),
                                        ~(1ull << (v & 63)), __ATOMIC_RELAXED,
