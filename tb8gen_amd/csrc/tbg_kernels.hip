@@ -94,17 +94,6 @@ __device__ __forceinline__ bool bm_test(const uint64_t *p, uint64_t i)
   return (gload(p, i >> 6) >> (i & 63)) & 1;
 }
 
-__device__ __forceinline__ int wave_excl_scan(int v, int lane, int *total)
-{
-  int x = v;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    int t = __shfl_up(x, d, 64);
-    if (lane >= d) x += t;
-  }
-  *total = __shfl(x, 63, 64);
-  return x - v;
-}
 
 __device__ __forceinline__ uint64_t wave_sum64(uint64_t v)
 {
@@ -189,6 +178,23 @@ template <int CTRL, int ROWS>
 __device__ __forceinline__ uint32_t dpp_move(uint32_t old, uint32_t v)
 {
   return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)v, CTRL, ROWS, 0xf, false);
+}
+
+// Exclusive prefix sum over the wave by DPP lane moves (shifts 1, 2, 4, 8
+// inside each row of 16 lanes, then the row totals broadcast to the rows
+// above); *total: the whole wave's sum.
+__device__ __forceinline__ int wave_excl_scan(int v, int lane, int *total)
+{
+  (void)lane;
+  uint32_t x = (uint32_t)v;
+  x += dpp_move<0x111, 0xf>(0u, x);
+  x += dpp_move<0x112, 0xf>(0u, x);
+  x += dpp_move<0x114, 0xf>(0u, x);
+  x += dpp_move<0x118, 0xf>(0u, x);
+  x += dpp_move<0x142, 0xa>(0u, x);
+  x += dpp_move<0x143, 0xc>(0u, x);
+  *total = __builtin_amdgcn_readlane((int)x, 63);
+  return (int)x - v;
 }
 
 // one step of the segmented OR-scan: take the moved lane's bits if it has
