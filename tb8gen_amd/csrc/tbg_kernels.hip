@@ -596,8 +596,9 @@ __global__ __launch_bounds__(256, 5) void k_uncap_j(JobDev *const *__restrict__ 
               int tq = ctz64(b);
               b &= b - 1;
               uint64_t i2 = sq_to_idx(T, M.ii, sq_set(q, j, tq));
-              __hip_atomic_fetch_or(reinterpret_cast<unsigned long long *>(&B[i2 >> 6]),
-                                    (unsigned long long)bit(i2 & 63), __ATOMIC_RELAXED,
+              // into the 32-bit half-word holding the bit: one LDS bank
+              __hip_atomic_fetch_or(reinterpret_cast<unsigned *>(B) + (i2 >> 5),
+                                    1u << (i2 & 31), __ATOMIC_RELAXED,
                                     __HIP_MEMORY_SCOPE_WORKGROUP);
             }
           } else {
@@ -948,8 +949,8 @@ __global__ __launch_bounds__(256, 5) void k_verify_j(JobDev *const *__restrict__
                           ? verify_pos<!SPLIT>(T, M, stm, s, idx, wins_o, sub_o, KT[wave], budget)
                           : 0;
               if (r == 0)
-                __hip_atomic_fetch_and(reinterpret_cast<unsigned long long *>(&Q.res[wave][v >> 6]),
-                                       ~(1ull << (v & 63)), __ATOMIC_RELAXED,
+                __hip_atomic_fetch_and(reinterpret_cast<unsigned *>(Q.res[wave]) + (v >> 5),
+                                       ~(1u << (v & 31)), __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_WORKGROUP);
               else if (r == 2)
                 Q.e[wave][__hip_atomic_fetch_add(&Q.n[wave], 1, __ATOMIC_RELAXED,
