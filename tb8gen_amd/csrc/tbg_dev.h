@@ -371,6 +371,24 @@ TBG_HD uint64_t idx_to_sq_1(const Idx &ii, uint64_t idx, uint64_t *pp)
   return occ;
 }
 
+// sq_to_idx of an all-single material (set k's piece in slot 2 + k),
+// written per set index so that the factor reads can all be issued first
+TBG_HD uint64_t sq_to_idx_1(const Idx &ii, uint64_t p)
+{
+  uint64_t occ = bit(sq_get(p, 0)) | bit(sq_get(p, 1));
+  uint64_t idx = 0;
+  int n = ii.numsets;
+  uint32_t f[MAXS];
+  for (int k = 0; k < MAXS; k++) f[k] = ii.factor[k];
+  for (int k = 0; k < MAXS; k++) {
+    if (k >= n) break;
+    int s = sq_get(p, 2 + k);
+    idx += (uint64_t)(uint32_t)(s - popc64(occ & (bit(s) - 1))) * f[k];
+    occ |= bit(s);
+  }
+  return idx;
+}
+
 // Index after a king moves from `from` to `to` when the position needs no
 // symmetry (normalize returned it unchanged), from the index before: the
 // kings precede every set, so each piece strictly between the two squares

@@ -2282,6 +2282,7 @@ extern "C" int tbg_host_selftest(char *msg, int len)
             ST_FAIL("%s: king_moved_idx_a", nm);
           if (sq_to_idx_a(all_sets(M.ii, q0), q0) != sq_to_idx(T, M.ii, q0))
             ST_FAIL("%s: sq_to_idx_a", nm);
+          if (sq_to_idx_1(M.ii, q0) != sq_to_idx(T, M.ii, q0)) ST_FAIL("%s: sq_to_idx_1", nm);
           if (king_moved_idx(M.ii, idx, q0, from, to) != sq_to_idx(T, M.ii, q0))
             ST_FAIL("%s: king_moved_idx", nm);
         }

@@ -804,8 +804,9 @@ __device__ __forceinline__ int verify_pos(const Tabs &T, const Mat &M, int stm, 
           if (v != 3 && v != 2) return 0;
         } else {
           i4[k] = phys(M, s2) * M.words * 64 +
-                  (M.all_single && q == q0 ? king_moved_idx(M.ii, idx, q0, ks, to)
-                                           : sq_to_idx(T, M.ii, q));
+                  (!M.all_single ? sq_to_idx(T, M.ii, q)
+                   : q == q0     ? king_moved_idx(M.ii, idx, q0, ks, to)
+                                 : sq_to_idx_1(M.ii, q));
         }
       }
     }
