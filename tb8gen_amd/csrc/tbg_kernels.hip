@@ -182,7 +182,8 @@ __device__ __forceinline__ uint32_t dpp_move(uint32_t old, uint32_t v)
 
 // Exclusive prefix sum over the wave by DPP lane moves (shifts 1, 2, 4, 8
 // inside each row of 16 lanes, then the row totals broadcast to the rows
-// above); *total: the whole wave's sum.
+// above); *total: the whole wave's sum.  Call with the whole wave (a lane
+// left out would pass on a stale value).
 __device__ __forceinline__ int wave_excl_scan(int v, int lane, int *total)
 {
   (void)lane;
