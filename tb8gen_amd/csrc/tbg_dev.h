@@ -389,6 +389,26 @@ TBG_HD uint64_t sq_to_idx_1(const Idx &ii, uint64_t p)
   return idx;
 }
 
+// sq_to_idx in the capture space of set l of an all-single material (its
+// Idx: the material's, set l emptied): the other sets' pieces in slots
+// 2 + k, set l's slot ignored
+TBG_HD uint64_t capt_idx_1(const Idx &ci, uint64_t p, int l)
+{
+  uint64_t occ = bit(sq_get(p, 0)) | bit(sq_get(p, 1));
+  uint64_t idx = 0;
+  int n = ci.numsets;
+  uint32_t f[MAXS];
+  for (int k = 0; k < MAXS; k++) f[k] = ci.factor[k];
+  for (int k = 0; k < MAXS; k++) {
+    if (k >= n) break;
+    if (k == l) continue;
+    int s = sq_get(p, 2 + k);
+    idx += (uint64_t)(uint32_t)(s - popc64(occ & (bit(s) - 1))) * f[k];
+    occ |= bit(s);
+  }
+  return idx;
+}
+
 // Index after a king moves from `from` to `to` when the position needs no
 // symmetry (normalize returned it unchanged), from the index before: the
 // kings precede every set, so each piece strictly between the two squares

@@ -2283,6 +2283,9 @@ extern "C" int tbg_host_selftest(char *msg, int len)
           if (sq_to_idx_a(all_sets(M.ii, q0), q0) != sq_to_idx(T, M.ii, q0))
             ST_FAIL("%s: sq_to_idx_a", nm);
           if (sq_to_idx_1(M.ii, q0) != sq_to_idx(T, M.ii, q0)) ST_FAIL("%s: sq_to_idx_1", nm);
+          for (int l = 0; l < M.numsets; l++)
+            if (capt_idx_1(M.capt[l], q0, l) != sq_to_idx(T, M.capt[l], q0))
+              ST_FAIL("%s: capt_idx_1 set %d", nm, l);
           if (king_moved_idx(M.ii, idx, q0, from, to) != sq_to_idx(T, M.ii, q0))
             ST_FAIL("%s: king_moved_idx", nm);
         }

@@ -649,7 +649,8 @@ __device__ int synth_exit(const Tabs &T, const Mat &M, int stm, uint64_t q0, uin
   if ((unsafe >> to) & 1) return 3;
   if (jj >= 0) {
     uint64_t c = sq_set(q, jj, sq_get(q, M.ii.last[l]));
-    return synth_value(synth_hash(M.seedmix, o, 2, s2 * 8 + l, sq_to_idx(T, M.capt[l], c)));
+    uint64_t ci = M.all_single ? capt_idx_1(M.capt[l], c, l) : sq_to_idx(T, M.capt[l], c);
+    return synth_value(synth_hash(M.seedmix, o, 2, s2 * 8 + l, ci));
   }
   uint64_t i2 = M.all_single && q == q0 ? king_moved_idx(M.ii, idx, q0, from, to)
                                         : sq_to_idx(T, M.ii, q);
@@ -724,7 +725,8 @@ __device__ __forceinline__ int verify_pos(const Tabs &T, const Mat &M, int stm, 
       uint64_t q = sq_set(p, j, to);
       q = sq_set(q, jj, sq_get(q, M.ii.last[l]));
       if (used++ >= budget) return 2;
-      if (!bm_test(sub_s + M.coff[l], sq_to_idx(T, M.capt[l], q))) return 0;
+      uint64_t ci = M.all_single ? capt_idx_1(M.capt[l], q, l) : sq_to_idx(T, M.capt[l], q);
+      if (!bm_test(sub_s + M.coff[l], ci)) return 0;
     }
     uint64_t qm = a & ~occ;
     // quiet moves four at a time: the four bitmap reads are in flight
@@ -782,7 +784,8 @@ __device__ __forceinline__ int verify_pos(const Tabs &T, const Mat &M, int stm, 
     }
     q = sq_set(q, jj, sq_get(q, M.ii.last[l]));
     const uint64_t *sb = sub_o + phys(M, s2) * M.ctotal[o] + M.coff[l];
-    if (!bm_test(sb, sq_to_idx(T, M.capt[l], q))) return 0;
+    uint64_t ci = M.all_single ? capt_idx_1(M.capt[l], q, l) : sq_to_idx(T, M.capt[l], q);
+    if (!bm_test(sb, ci)) return 0;
   }
   // quiet king moves four at a time, like the pieces' (a lost candidate
   // reads every one); a synthetic job's exits need no read
@@ -1198,7 +1201,8 @@ __device__ void synth_exits_pos(const Tabs &T, const Mat &M, uint64_t p, uint64_
       int jj = slot_of(p, to, M.col80[o]);
       int k = sq_get(M.set_of, jj);
       uint64_t c = sq_set(q, jj, sq_get(q, M.ii.last[k]));
-      v = synth_value(synth_hash(M.seedmix, o, 2, s2 * 8 + k, sq_to_idx(T, M.capt[k], c)));
+      uint64_t ci = M.all_single ? capt_idx_1(M.capt[k], c, k) : sq_to_idx(T, M.capt[k], c);
+      v = synth_value(synth_hash(M.seedmix, o, 2, s2 * 8 + k, ci));
     } else {
       uint64_t i2 = !M.all_single ? sq_to_idx(T, M.ii, q)
                     : m == 0 && !fl ? king_moved_idx_a(A, idx, from, to)
