@@ -309,7 +309,8 @@ __device__ __forceinline__ void mark_unmoves(const Tabs &T, const Mat &M, int j,
   while (b) {
     int t = ctz64(b);
     b &= b - 1;
-    uint64_t i2 = sq_to_idx(T, M.ii, sq_set(q, j, t));
+    uint64_t q2 = sq_set(q, j, t);
+    uint64_t i2 = M.all_single ? sq_to_idx_1(M.ii, q2) : sq_to_idx(T, M.ii, q2);
     if (!bm_test(wb, i2)) atomic_or64(base + (i2 >> 6), bit(i2 & 63));
   }
 }
@@ -596,7 +597,8 @@ __global__ __launch_bounds__(256, 5) void k_uncap_j(JobDev *const *__restrict__ 
             while (b) {
               int tq = ctz64(b);
               b &= b - 1;
-              uint64_t i2 = sq_to_idx(T, M.ii, sq_set(q, j, tq));
+              uint64_t q2 = sq_set(q, j, tq);
+              uint64_t i2 = M.all_single ? sq_to_idx_1(M.ii, q2) : sq_to_idx(T, M.ii, q2);
               // into the 32-bit half-word holding the bit: one LDS bank
               __hip_atomic_fetch_or(reinterpret_cast<unsigned *>(B) + (i2 >> 5),
                                     1u << (i2 & 31), __ATOMIC_RELAXED,
@@ -652,8 +654,9 @@ __device__ int synth_exit(const Tabs &T, const Mat &M, int stm, uint64_t q0, uin
     uint64_t ci = M.all_single ? capt_idx_1(M.capt[l], c, l) : sq_to_idx(T, M.capt[l], c);
     return synth_value(synth_hash(M.seedmix, o, 2, s2 * 8 + l, ci));
   }
-  uint64_t i2 = M.all_single && q == q0 ? king_moved_idx(M.ii, idx, q0, from, to)
-                                        : sq_to_idx(T, M.ii, q);
+  uint64_t i2 = !M.all_single ? sq_to_idx(T, M.ii, q)
+                : q == q0     ? king_moved_idx(M.ii, idx, q0, from, to)
+                              : sq_to_idx_1(M.ii, q);
   return synth_value(synth_hash(M.seedmix, o, 1, s2, i2));
 }
 
