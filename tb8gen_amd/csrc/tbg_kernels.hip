@@ -149,7 +149,9 @@ __device__ __forceinline__ void mark_king_unmoves(const Tabs &T, const Mat &M, i
     for (int k = 0; k < 4; k++) {
       g[k] = ~0ull;
       if (dual) {
-        g[k] = phys(M, ds) * M.words * 64 + sq_to_idx(T, M.ii, mirror_nonkings(dq));
+        uint64_t qd = mirror_nonkings(dq);
+        g[k] = phys(M, ds) * M.words * 64 +
+               (M.all_single ? sq_to_idx_1(M.ii, qd) : sq_to_idx(T, M.ii, qd));
         dual = false;
       } else if (b) {
         int t = ctz64(b);
@@ -158,7 +160,9 @@ __device__ __forceinline__ void mark_king_unmoves(const Tabs &T, const Mat &M, i
         int s2 = normalize(T, q0, &q);
         if (M.synth && s2 != M.synth_slice) continue;  // leaves a single-slice job
         g[k] = phys(M, s2) * M.words * 64 +
-               (inc && q == q0 ? king_moved_idx(M.ii, idx, q0, from, t) : sq_to_idx(T, M.ii, q));
+               (inc && q == q0  ? king_moved_idx(M.ii, idx, q0, from, t)
+                : M.all_single ? sq_to_idx_1(M.ii, q)
+                               : sq_to_idx(T, M.ii, q));
         if (s < 441 && s2 >= 441) {
           dual = true;
           dq = q;
