@@ -409,6 +409,52 @@ TBG_HD uint64_t capt_idx_1(const Idx &ci, uint64_t p, int l)
   return idx;
 }
 
+// Stepping through consecutive indices of an all-single material: set k's
+// digit is the rank of its square among the 62 - k squares left by the kings
+// and the sets before it (a radix of 62 - k), six bits per digit in D.
+TBG_HD uint64_t digits_of(uint64_t p, int n)
+{
+  uint64_t occ = bit(sq_get(p, 0)) | bit(sq_get(p, 1));
+  uint64_t D = 0;
+  for (int k = 0; k < MAXS; k++) {
+    if (k >= n) break;
+    int s = sq_get(p, 2 + k);
+    D |= (uint64_t)(s - popc64(occ & (bit(s) - 1))) << (6 * k);
+    occ |= bit(s);
+  }
+  return D;
+}
+// The position of index + 64 from that of index (its digits D and squares
+// p, both updated): 64 is added to the last digit with carries upwards, and
+// the squares of the sets from the lowest changed digit on are placed again.
+// Returns the occupancy.  Past the table's end the result is meaningless.
+TBG_HD uint64_t step64(int n, uint64_t *dp, uint64_t *pp)
+{
+  uint64_t D = *dp, p = *pp;
+  int add = 64, low = n - 1;
+  for (int j = MAXS - 1; j >= 0; j--) {
+    if (j >= n) continue;
+    int r = 62 - j, d = (int)((D >> (6 * j)) & 63) + add;
+    add = 0;
+    while (d >= r) {  // twice at most: 64 < 2 r
+      d -= r;
+      add++;
+    }
+    D = (D & ~(63ull << (6 * j))) | ((uint64_t)d << (6 * j));
+    low = j;
+    if (!add) break;
+  }
+  uint64_t occ = bit(sq_get(p, 0)) | bit(sq_get(p, 1));
+  for (int j = 0; j < MAXS; j++) {
+    if (j >= n) break;
+    if (j >= low) p = sq_set(p, 2 + j, select_zero(occ, (int)((D >> (6 * j)) & 63)));
+    occ |= bit(sq_get(p, 2 + j));
+  }
+  *dp = D;
+  *pp = p;
+  return occ;
+}
+
 // Index after a king moves from `from` to `to` when the position needs no
 // symmetry (normalize returned it unchanged), from the index before: the
 // kings precede every set, so each piece strictly between the two squares

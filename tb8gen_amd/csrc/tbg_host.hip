@@ -707,7 +707,8 @@ int run_init_async(tbg_ctx *c)
   HIPCHK(hipMemsetAsync(c->dcnt, 0, 4 * sizeof(unsigned long long), c->st));
   {
     Timed t(c, CAT_INIT);
-    hipLaunchKernelGGL(k_init, dim3(cap_grid(((uint64_t)ns * M.words + 3) / 4, (uint64_t)g_cus * 16)),
+    uint64_t chunks = ((uint64_t)ns * M.words + 4 * INIT_CHUNK - 1) / (4 * INIT_CHUNK);
+    hipLaunchKernelGGL(k_init, dim3(cap_grid(chunks, (uint64_t)g_cus * 16)),
                        dim3(256), 0, c->st, c->dM, c->wins[0], c->wins[1], c->prevL[0], c->prevL[1],
                        c->codes[0], c->codes[1], c->prevW[0], c->prevW[1], c->newW[0], c->newW[1],
                        s0, ns, c->dcnt);
@@ -2221,6 +2222,12 @@ extern "C" int tbg_host_selftest(char *msg, int len)
           if (single && ii.small_q) {
             uint64_t p1 = kings;
             if (idx_to_sq_1(ii, idx, &p1) != occ || p1 != p) ST_FAIL("%s space %zu: idx_to_sq_1", nm, sp);
+          }
+          if (sp == 0 && M.all_single && idx + 64 < ii.size) {  // step64 against a full decode
+            uint64_t D = digits_of(p, ii.numsets), p1 = p, p2 = kings;
+            uint64_t o1 = step64(ii.numsets, &D, &p1), o2 = idx_to_sq(T, ii, idx + 64, &p2);
+            if (o1 != o2 || p1 != p2 || D != digits_of(p2, ii.numsets))
+              ST_FAIL("%s: step64 from %llu", nm, (unsigned long long)idx);
           }
           // permuting members within a set does not change the index
           for (int k = 0; k < ii.numsets; k++)

@@ -112,6 +112,9 @@ __global__ void k_sweep_j(JobDev *const *jobs, const Pair *pairs, const uint64_t
 // flags[0]: jobs done; flags[1]: some job is at ply 1 or 101 (uncapture)
 // done_out (nullable): done_out[j] = job j finished
 __global__ void k_ctrl(JobDev *const *jobs, int nj, unsigned *flags, unsigned *done_out);
+// k_init: consecutive bitmap words per wave (positions step by 64 between
+// them, for all-single materials without a full decode)
+constexpr int INIT_CHUNK = 8;
 __global__ void k_init(const Mat *Mp, uint64_t *winsW, uint64_t *winsB, uint64_t *mateW,
                        uint64_t *mateB, uint16_t *codesW, uint16_t *codesB, uint64_t *kexW0,
                        uint64_t *kexW1, uint64_t *kexL0, uint64_t *kexL1, int s0, int ns,

@@ -1302,15 +1302,32 @@ __global__ __launch_bounds__(256, 6) void k_init(const Mat *__restrict__ Mp, uin
   int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   uint64_t nw = (uint64_t)ns * M.words;
   uint64_t c0 = 0, c1 = 0, c2 = 0, c3 = 0;
-  for (uint64_t g = (uint64_t)blockIdx.x * 4 + wave; g < nw; g += (uint64_t)gridDim.x * 4) {
+  // each wave takes INIT_CHUNK consecutive words; for an all-single material
+  // a lane's position then steps by 64 from word to word (step64) instead of
+  // a full decode
+  bool stepping = M.all_single;
+  int nsets = M.ii.numsets;
+  for (uint64_t g0 = ((uint64_t)blockIdx.x * 4 + wave) * INIT_CHUNK; g0 < nw;
+       g0 += (uint64_t)gridDim.x * 4 * INIT_CHUNK)
+  for (uint64_t g = g0, D = 0, pk = 0; g < g0 + INIT_CHUNK && g < nw; g++) {
     int s = s0 + (int)(g / M.words);
     uint64_t w = g % M.words;
     uint64_t idx = w * 64 + lane;
     bool wi = false, bi = false, wm = false, bm = false;
     bool xw0 = false, xw1 = false, xl0 = false, xl1 = false;
     if (idx < M.ii.size) {
-      uint64_t p = kings_of(T, s);
-      uint64_t occ = idx_to_sq(T, M.ii, idx, &p);
+      // the previous word's position of this lane is index - 64 of the same
+      // slice unless the chunk or the slice starts here
+      uint64_t p, occ;
+      if (stepping && g != g0 && w != 0) {
+        p = pk;
+        occ = step64(nsets, &D, &p);
+      } else {
+        p = kings_of(T, s);
+        occ = idx_to_sq(T, M.ii, idx, &p);
+        if (stepping) D = digits_of(p, nsets);
+      }
+      pk = p;
       int wk = sq_get(p, WHITE), bk = sq_get(p, BLACK);
       // each side's piece attacks with the enemy king lifted: the legality
       // tests, the exits' king moves and the mate search all read them
