@@ -455,6 +455,37 @@ TBG_HD uint64_t step64(int n, uint64_t *dp, uint64_t *pp)
   return occ;
 }
 
+// The position of index + delta from that of index (digits D and squares
+// p, both updated), for 0 < delta < 2^24: delta is added to the last digit
+// and the carries go upwards (division by each set's radix), then the
+// squares from the lowest changed digit on are placed again.  Returns the
+// occupancy.  Past the table's end the result is meaningless.
+TBG_HD uint64_t step_by(int n, uint64_t *dp, uint64_t *pp, uint32_t delta)
+{
+  uint64_t D = *dp, p = *pp;
+  uint32_t add = delta;
+  int low = n - 1;
+#pragma unroll
+  for (int j = MAXS - 1; j >= 0; j--) {
+    if (j >= n) continue;
+    uint32_t r = 62 - j, d = (uint32_t)((D >> (6 * j)) & 63) + add;
+    add = d / r;  // a constant divisor per digit
+    d -= add * r;
+    D = (D & ~(63ull << (6 * j))) | ((uint64_t)d << (6 * j));
+    low = j;
+    if (!add) break;
+  }
+  uint64_t occ = bit(sq_get(p, 0)) | bit(sq_get(p, 1));
+  for (int j = 0; j < MAXS; j++) {
+    if (j >= n) break;
+    if (j >= low) p = sq_set(p, 2 + j, select_zero(occ, (int)((D >> (6 * j)) & 63)));
+    occ |= bit(sq_get(p, 2 + j));
+  }
+  *dp = D;
+  *pp = p;
+  return occ;
+}
+
 // Index after a king moves from `from` to `to` when the position needs no
 // symmetry (normalize returned it unchanged), from the index before: the
 // kings precede every set, so each piece strictly between the two squares

@@ -2223,6 +2223,16 @@ extern "C" int tbg_host_selftest(char *msg, int len)
             uint64_t p1 = kings;
             if (idx_to_sq_1(ii, idx, &p1) != occ || p1 != p) ST_FAIL("%s space %zu: idx_to_sq_1", nm, sp);
           }
+          if (sp == 0 && M.all_single) {  // step_by against a full decode
+            uint32_t deltas[4] = { 1, 63, 4097, (uint32_t)(rnd() % 100000) + 1 };
+            for (uint32_t dl : deltas) {
+              if (idx + dl >= ii.size) continue;
+              uint64_t D = digits_of(p, ii.numsets), p1 = p, p2 = kings;
+              uint64_t o1 = step_by(ii.numsets, &D, &p1, dl), o2 = idx_to_sq(T, ii, idx + dl, &p2);
+              if (o1 != o2 || p1 != p2 || D != digits_of(p2, ii.numsets))
+                ST_FAIL("%s: step_by %u from %llu", nm, dl, (unsigned long long)idx);
+            }
+          }
           if (sp == 0 && M.all_single && idx + 64 < ii.size) {  // step64 against a full decode
             uint64_t D = digits_of(p, ii.numsets), p1 = p, p2 = kings;
             uint64_t o1 = step64(ii.numsets, &D, &p1), o2 = idx_to_sq(T, ii, idx + 64, &p2);

@@ -704,12 +704,24 @@ __device__ __forceinline__ uint64_t king_norm(uint32_t e, uint64_t q0)
 template <bool SMALL>
 __device__ __forceinline__ int verify_pos(const Tabs &T, const Mat &M, int stm, int s, uint64_t idx,
                           const uint64_t *__restrict__ wins_o, const uint64_t *__restrict__ sub_o,
-                          const uint32_t *kt, int budget)
+                          const uint32_t *kt, int budget, uint64_t &hidx, uint64_t &hD,
+                          uint64_t &hp)
 {
   int used = 0;  // bitmap reads so far; at budget the answer is left open (2)
   int o = stm ^ 1;
-  uint64_t p = kings_of(T, s);
-  uint64_t occ = idx_to_sq(T, M.ii, idx, &p);
+  // the lane's previous candidate of the slice (hidx, its digits hD and
+  // squares hp) is stepped forward when it lies shortly before this one
+  uint64_t p, occ;
+  if (M.all_single && hidx != ~0ull && idx > hidx && idx - hidx < (1u << 24)) {
+    p = hp;
+    occ = step_by(M.ii.numsets, &hD, &p, (uint32_t)(idx - hidx));
+  } else {
+    p = kings_of(T, s);
+    occ = idx_to_sq(T, M.ii, idx, &p);
+    if (M.all_single) hD = digits_of(p, M.ii.numsets);
+  }
+  hidx = idx;
+  hp = p;
   int ks = sq_get(p, stm), oks = sq_get(p, o);
   uint64_t oppocc = 0;
   for (int t = 0; t < MAXP - 1; t++) {
@@ -902,6 +914,7 @@ __global__ __launch_bounds__(256, 5) void k_verify_j(JobDev *const *__restrict__
     uint64_t u0 = within * (uint64_t)uwe;
     uint64_t uend = u0 + uwe < M.words ? u0 + uwe : M.words;
     uint64_t mysum = 0;
+    uint64_t hidx = ~0ull, hD = 0, hp = 0;  // the lane's last decoded candidate of the unit
     // A large unit without candidates (and exits) only has its L words
     // cleared.  Every thread's candidate loads go out together; a unit with
     // some is then verified as below, its words found in cache.  (Large
@@ -958,7 +971,8 @@ __global__ __launch_bounds__(256, 5) void k_verify_j(JobDev *const *__restrict__
               uint16_t v = Q.e[wave][e];
               uint64_t idx = (wbase + (v >> 6)) * 64 + (v & 63);
               int r = idx < M.ii.size
-                          ? verify_pos<!SPLIT>(T, M, stm, s, idx, wins_o, sub_o, KT[wave], budget)
+                          ? verify_pos<!SPLIT>(T, M, stm, s, idx, wins_o, sub_o, KT[wave], budget,
+                                               hidx, hD, hp)
                           : 0;
               if (r == 0)
                 __hip_atomic_fetch_and(reinterpret_cast<unsigned *>(Q.res[wave]) + (v >> 5),
