@@ -477,7 +477,7 @@ __global__ __launch_bounds__(512, 6) void k_pred_wide_j(JobDev *const *__restric
   __shared__ Tabs T;
   __shared__ Mat Ms;
   extern __shared__ uint64_t dyn[];
-  constexpr bool kDecode1 = false;  // its spills would grow
+  constexpr bool kDecode1 = false;  // stepped decodes instead
 #include "tbg_pred.inc"
 }
 
