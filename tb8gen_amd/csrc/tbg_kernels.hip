@@ -826,10 +826,12 @@ __device__ __forceinline__ int verify_pos(const Tabs &T, const Mat &M, int stm, 
           int v = synth_exit(T, M, stm, q0, q, unsafe, ks, -1, 0, s2, idx);
           if (v != 3 && v != 2) return 0;
         } else {
-          i4[k] = phys(M, s2) * M.words * 64 +
-                  (!M.all_single ? sq_to_idx(T, M.ii, q)
-                   : q == q0     ? king_moved_idx(M.ii, idx, q0, ks, to)
-                                 : sq_to_idx_1(M.ii, q));
+          uint64_t i2 = !M.all_single ? sq_to_idx(T, M.ii, q)
+                        : q == q0     ? king_moved_idx(M.ii, idx, q0, ks, to)
+                                      : sq_to_idx_1(M.ii, q);
+          // SMALL: at most 462 slices of 2^22 positions, below 2^32 bits
+          i4[k] = SMALL ? (uint64_t)((uint32_t)phys(M, s2) * (uint32_t)M.words * 64u + (uint32_t)i2)
+                        : phys(M, s2) * M.words * 64 + i2;
         }
       }
     }
