@@ -275,6 +275,7 @@ struct tbg_ctx {
   uint16_t *codes[2] = { nullptr, nullptr };
   unsigned long long *dcnt = nullptr;  // init counts
   unsigned *dflags = nullptr;          // [0] jobs done, [1] uncapture ply pending
+  unsigned *hflags = nullptr;          // pinned host copy of a batch's flags
   JobDev hj;                           // host mirror of the job
   JobDev *dj = nullptr;                // the job on the device
   JobDev **djp = nullptr;              // device array holding dj (a batch of one)
@@ -779,13 +780,18 @@ int run_batch(tbg_ctx *prof, JobDev *const *jobs, int nj, const LaunchSet &L, un
 {
   unsigned f[2] = { 0, 1 };
   if (!graph) {
+    // the flags come back into pinned memory: a direct copy, no staging
+    if (!prof->hflags)
+      HIPCHK(hipHostMalloc((void **)&prof->hflags, 2 * sizeof(unsigned), hipHostMallocDefault));
+    unsigned *hf = prof->hflags;
+    hf[1] = 1;
     for (;;) {
       // the host reads the flags after every ply, so it knows whether any
       // job is at ply 1 or 101 and needs the uncapture launches
-      TRY(launch_ply(prof, jobs, nj, L, flags, st, f[1] != 0));
-      HIPCHK(hipMemcpyAsync(f, flags, sizeof f, hipMemcpyDeviceToHost, st));
+      TRY(launch_ply(prof, jobs, nj, L, flags, st, hf[1] != 0));
+      HIPCHK(hipMemcpyAsync(hf, flags, 2 * sizeof(unsigned), hipMemcpyDeviceToHost, st));
       HIPCHK(hipStreamSynchronize(st));
-      if ((int)f[0] >= nj) break;
+      if ((int)hf[0] >= nj) break;
     }
     return TBG_OK;
   }
@@ -1096,6 +1102,7 @@ void tbg_destroy(tbg_ctx *c)
                    c->dcnt, c->dflags, c->dj, c->djp, c->dhist };
   for (void *b : bufs) dfree(c, b);
   if (c->ready) (void)hipEventDestroy(c->ready);
+  if (c->hflags) (void)hipHostFree(c->hflags);
   if (c->st && !c->pooled) (void)hipStreamDestroy(c->st);
   delete c;
 }
