@@ -778,7 +778,6 @@ int run_plain(tbg_ctx *c)
 int run_batch(tbg_ctx *prof, JobDev *const *jobs, int nj, const LaunchSet &L, unsigned *flags,
               hipStream_t st, bool graph, double *graph_ms)
 {
-  unsigned f[2] = { 0, 1 };
   if (!graph) {
     // the flags come back into pinned memory: a direct copy, no staging
     if (!prof->hflags)
@@ -808,11 +807,14 @@ int run_batch(tbg_ctx *prof, JobDev *const *jobs, int nj, const LaunchSet &L, un
   HIPCHK(hipEventCreate(&a));
   HIPCHK(hipEventCreate(&b));
   HIPCHK(hipEventRecord(a, st));
+  if (!prof->hflags)
+    HIPCHK(hipHostMalloc((void **)&prof->hflags, 2 * sizeof(unsigned), hipHostMallocDefault));
+  unsigned *hf = prof->hflags;
   for (;;) {
     for (int i = 0; i < 4; i++) HIPCHK(hipGraphLaunch(ge, st));
-    HIPCHK(hipMemcpyAsync(f, flags, sizeof f, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(hf, flags, 2 * sizeof(unsigned), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
-    if ((int)f[0] >= nj) break;
+    if ((int)hf[0] >= nj) break;
   }
   HIPCHK(hipEventRecord(b, st));
   HIPCHK(hipEventSynchronize(b));
