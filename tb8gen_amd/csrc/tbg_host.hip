@@ -28,8 +28,9 @@ namespace {
 thread_local char g_errbuf[256];
 int g_device = -1;
 int g_cus = 256;
-// bitmap planes per job: wins, prevW, prevL, newW, newL, cand per side
-constexpr uint64_t kPlanes = 12;
+// bitmap planes per job: wins, prevW, prevL, newW, newL and the L and W
+// steps' candidates per side
+constexpr uint64_t kPlanes = 14;
 Tabs g_htabs;
 // geometry and scatter given to new contexts and servers (tbg_set_geometry(NULL, ..))
 Geom g_geom = { 256, 4096, { 3, 16, 16, 16 } };
@@ -270,7 +271,8 @@ struct tbg_ctx {
   SubRef *dR = nullptr;
   uint64_t nsl = 0, pw = 0;
   uint64_t *arena = nullptr, *subarena = nullptr;
-  uint64_t *wins[2], *prevW[2], *prevL[2], *newW[2], *newL[2], *cand[2];  // canonical planes
+  uint64_t *wins[2], *prevW[2], *prevL[2], *newW[2], *newL[2];  // canonical planes
+  uint64_t *cand[2], *candW[2];
   uint64_t *sub[2][4];
   uint16_t *codes[2] = { nullptr, nullptr };
   unsigned long long *dcnt = nullptr;  // init counts
@@ -408,6 +410,7 @@ void reset_job(tbg_ctx *c)
     J.newW[s] = c->newW[s];
     J.newL[s] = c->newL[s];
     J.cand[s] = c->cand[s];
+    J.candW[s] = c->candW[s];
     J.codes[s] = c->codes[s];
     for (int k = 0; k < 4; k++) J.sub[s][k] = c->sub[s][k];
   }
@@ -492,6 +495,7 @@ int ctx_alloc(tbg_ctx *c, bool with_codes, hipStream_t ext = nullptr)
     c->newW[s] = p; p += c->pw;
     c->newL[s] = p; p += c->pw;
     c->cand[s] = p; p += c->pw;
+    c->candW[s] = p; p += c->pw;
   }
   HIPCHK(hipMemsetAsync(c->arena, 0, pb * kPlanes, c->st));
   uint64_t sw = 4 * c->nsl * (M.ctotal[0] + M.ctotal[1]);
@@ -582,11 +586,12 @@ int launch(tbg_ctx *prof, JobDev *const *jobs, const LaunchSet &L, int kt, int k
   switch (kt) {
   case KT_PRED: {
     Timed t(prof, CAT_PRED);
+    uint64_t units = kind == 2 ? 2 * total : total;  // kind 2: both steps
     if (L.pred_wide)
-      hipLaunchKernelGGL(k_pred_wide_j, dim3(cap_grid(total, (uint64_t)g_cus * L.g.wg[kt])),
+      hipLaunchKernelGGL(k_pred_wide_j, dim3(cap_grid(units, (uint64_t)g_cus * L.g.wg[kt])),
                          dim3(PB_THREADS), L.pred_lds, st, jobs, L.pairs, L.P[kt], L.np, sw0, kind);
     else
-      hipLaunchKernelGGL(k_pred_j, dim3(cap_grid(total, (uint64_t)g_cus * L.g.wg[kt])),
+      hipLaunchKernelGGL(k_pred_j, dim3(cap_grid(units, (uint64_t)g_cus * L.g.wg[kt])),
                          dim3(PB_THREADS), L.pred_lds, st, jobs, L.pairs, L.P[kt], L.np, sw0, kind);
     break;
   }
@@ -619,16 +624,18 @@ int launch(tbg_ctx *prof, JobDev *const *jobs, const LaunchSet &L, int kt, int k
 }
 
 // One whole ply of every job of a launch set (batched form; also what a
-// graph captures): L candidates and verification for both sides, then W
-// candidates and the sweep, then the control kernel.
+// graph captures): the L and W candidates of both sides (one pred launch),
+// then verification, the sweep and the control kernel.
 int launch_ply(tbg_ctx *prof, JobDev *const *jobs, int nj, const LaunchSet &L, unsigned *flags,
                hipStream_t st, bool edge = true, unsigned *done_out = nullptr)
 {
+  // the L and W steps mark separate candidate planes from different
+  // frontiers, so their uncaptures and preds all come first (the preds in
+  // one launch); verification must see wins(opp) before this ply's sweep
   if (edge) TRY(launch(prof, jobs, L, KT_UNCAP, 0, 0, flags, st));
-  TRY(launch(prof, jobs, L, KT_PRED, 0, 0, flags, st));
-  TRY(launch(prof, jobs, L, KT_VERIFY, 0, 0, flags, st));
   if (edge) TRY(launch(prof, jobs, L, KT_UNCAP, 1, 0, flags, st));
-  TRY(launch(prof, jobs, L, KT_PRED, 1, 0, flags, st));
+  TRY(launch(prof, jobs, L, KT_PRED, 2, 0, flags, st));
+  TRY(launch(prof, jobs, L, KT_VERIFY, 0, 0, flags, st));
   TRY(launch(prof, jobs, L, KT_SWEEP, 1, 0, flags, st));
   hipLaunchKernelGGL(k_ctrl, dim3(1), dim3(1024), 0, st, jobs, nj, flags, done_out);
   HIPCHK(hipGetLastError());
@@ -1533,6 +1540,7 @@ int tbg_restore(tbg_ctx *c, const char *root)
     HIPCHK(hipMemsetAsync(c->newW[s], 0, pb, c->st));
     HIPCHK(hipMemsetAsync(c->newL[s], 0, pb, c->st));
     HIPCHK(hipMemsetAsync(c->cand[s], 0, pb, c->st));
+    HIPCHK(hipMemsetAsync(c->candW[s], 0, pb, c->st));
   }
   HIPCHK(hipStreamSynchronize(c->st));
   TRY(checkpoint_planes(c, r, n, false));

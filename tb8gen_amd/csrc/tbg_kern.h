@@ -22,7 +22,8 @@ struct SubRef {
 // replayed; k_ctrl advances it (the more_* logic of src/tb8gen.c:1406-1444).
 struct JobDev {
   const Mat *M;
-  uint64_t *wins[2], *prevW[2], *prevL[2], *newW[2], *newL[2], *cand[2];
+  uint64_t *wins[2], *prevW[2], *prevL[2], *newW[2], *newL[2];
+  uint64_t *cand[2], *candW[2];  // candidates of the L and of the W step
   uint64_t *sub[2][4];
   uint16_t *codes[2];
   long long *hist;            // [max_ply][4] counts per step, -1: step did not run

@@ -566,7 +566,7 @@ __global__ __launch_bounds__(256, 5) void k_uncap_j(JobDev *const *__restrict__ 
     int off = wave_excl_scan(popc64(x), lane, &tot);
     const Idx &ci = M.capt[k];
     int m = M.ii.last[k];
-    uint64_t *cand = J.cand[stm];
+    uint64_t *cand = kind == 0 ? J.cand[stm] : J.candW[stm];
     const uint64_t *won = J.wins[stm];
     uint64_t *own = cand + phys(M, s) * M.words;
     const uint64_t *ownw = won + phys(M, s) * M.words;
@@ -1054,7 +1054,7 @@ __global__ __launch_bounds__(256) void k_sweep_j(JobDev *const *__restrict__ job
         n = uni(Jp->n);
         run = (uni(Jp->mask) >> (2 + stm)) & 1;
         exits = run && Ms.synth && n == 1;
-        cand = uniptr(Jp->cand[stm]);
+        cand = uniptr(Jp->candW[stm]);
         wins = uniptr(Jp->wins[stm]);
         newW = uniptr(Jp->newW[stm]);
         kex = uniptr(Jp->prevW[stm]);
