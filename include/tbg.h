@@ -103,6 +103,12 @@ int tbg_create(tbg_ctx **out, const char *material);
  * `material`; capture and king-move exits come from a seeded splitmix64
  * hash (win below 0.03, loss in [0.03, 0.06)). */
 int tbg_create_synth(tbg_ctx **out, const char *material, int slice, uint64_t seed);
+/* tbg_create_synth with options: TBG_SYNTH_CODES keeps the per-position
+ * codes of the slice (2 bytes per position and side, read by tbg_get_codes)
+ * as a whole-table job does; without it only the bitmaps are kept. */
+#define TBG_SYNTH_CODES 1u
+int tbg_create_synth_ex(tbg_ctx **out, const char *material, int slice, uint64_t seed,
+                        unsigned flags);
 void tbg_destroy(tbg_ctx *ctx);
 int tbg_get_info(const tbg_ctx *ctx, tbg_info *info);
 

@@ -56,6 +56,11 @@ def lib():
                                      ctypes.c_uint64]
         L.tbo_time_plies.argtypes = [ctypes.c_char_p, ctypes.c_int,
                                      ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)]
+        L.tbo_synth_check.restype = ctypes.c_int64
+        L.tbo_synth_check.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_uint64,
+                                      ctypes.POINTER(ctypes.c_void_p), ctypes.c_uint64,
+                                      ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64),
+                                      ctypes.POINTER(ctypes.c_int64)]
         L.tbo_rtbw_open.argtypes = [ctypes.c_char_p, ctypes.c_char_p]
         L.tbo_rtbw_close.argtypes = [ctypes.c_int]
         L.tbo_rtbw_size.restype = ctypes.c_uint64
@@ -190,6 +195,43 @@ def kksquare(s):
 
 def synth_hash(seed, stm, kind, region, idx):
     return int(lib().tbo_synth_hash(seed, stm, kind, region, idx))
+
+
+def planes_from_codes(codes_w, codes_b, words):
+    """The ten planes tbo_synth_check reads (wins after ply 1, W_1, L_1,
+    W_2, L_2, each [white, black]) derived from a synthetic slice's codes."""
+    out = []
+    cs = (np.asarray(codes_w).ravel(), np.asarray(codes_b).ravel())
+
+    def plane(mask):
+        b = np.zeros(words * 64, dtype=bool)
+        b[:mask.size] = mask
+        return np.packbits(b, bitorder="little").view(np.uint64)
+
+    for which in ("wins1", "W1", "L1", "W2", "L2"):
+        for c in cs:
+            if which == "wins1":
+                m = (c == 3) | (c == 5)
+            else:
+                n = int(which[1])
+                m = c == ((n << 2) | (1 if which[0] == "W" else 2))
+            out.append(plane(m))
+    return out
+
+
+def synth_check(material, s, seed, planes, nsamples, sample_seed=1):
+    """Sampled forward check of plies 1 and 2 of a synthetic slice against
+    the engine's planes (see tbo_synth_check); returns (mismatches, checked,
+    first mismatch (stm, plane, index, engine bit) or None)."""
+    arrs = [np.ascontiguousarray(p, dtype=np.uint64) for p in planes]
+    ptrs = (ctypes.c_void_p * 10)(*[a.ctypes.data for a in arrs])
+    chk = ctypes.c_uint64()
+    fb = (ctypes.c_int64 * 4)()
+    bad = lib().tbo_synth_check(material.encode(), int(s), int(seed), ptrs, int(nsamples),
+                                int(sample_seed), ctypes.byref(chk), fb)
+    if bad < 0:
+        raise RuntimeError(f"synth_check of {material} failed ({bad})")
+    return int(bad), chk.value, (tuple(fb) if bad else None)
 
 
 def time_plies(material, max_plies):

@@ -1303,6 +1303,230 @@ int tbo_time_plies(const char *material, int max_plies, double *loop_seconds, in
   return 0;
 }
 
+/* ------------------------------------------------------------------ */
+/* Sampled forward check of a synthetic slice too large to generate     */
+/* here (the 8-piece slice): plies 1 and 2 recomputed position by       */
+/* position from the moves out of each sampled position, the way the     */
+/* recurrences of calc_L_n / calc_W_n (tb8gen.c:943-1084) read forward.  */
+/* The engine's own ply-1 planes stand in for the targets' ply-1 sets.   */
+
+/* One legal move of stm out of position sq (slice s) and what it reaches:
+ * kind 0 = quiet move inside the slice to index *tgt; kind 1 = a terminal
+ * value *val for o (a capture inside the slice, or a king move or king
+ * capture out of the slice, valued by the seeded hash).  Returns the
+ * number of legal moves; fn receives each. */
+typedef void (*move_fn)(void *u, int kind, uint64_t tgt, int val, const uint8_t *q, bb_t qocc);
+
+static int synth_moves(const Table *t, int stm, int s, const uint8_t *sq_in, bb_t occ, move_fn fn,
+                       void *u)
+{
+  int o = stm ^ 1, nlegal = 0;
+  uint8_t sq[MAX_PIECES], sq2[MAX_PIECES];
+  memcpy(sq, sq_in, MAX_PIECES);
+  /* non-king pieces: captures (check_moves, tb8gen.c:571-590), then quiet moves */
+  for (int i = 1; t->pcs[stm][i] >= 0; i++) {
+    int k = t->pcs[stm][i], from = sq[k];
+    bb_t b = piece_att(t->pt[k], from, occ);
+    bb_t caps = b & occ;
+    while (caps) {
+      int to = pop_lsb(&caps), j = 0;
+      while (sq[j] != to) j++;
+      if (!((t->pt[k] ^ t->pt[j]) & 8)) continue;
+      /* legal iff stm's king is not attacked once piece j is gone */
+      uint8_t xpt[MAX_PIECES], xsq[MAX_PIECES];
+      int nx = 0;
+      for (int x = 0; x < t->num; x++)
+        if (x != j) { xpt[nx] = t->pt[x]; xsq[nx] = x == k ? (uint8_t)to : sq[x]; nx++; }
+      if (attacked_by(xpt, xsq, nx, occ ^ bit(from), sq[stm], o)) continue;
+      nlegal++;
+      int l = t->pc_to_set[j];
+      memcpy(sq2, sq, MAX_PIECES);
+      sq2[k] = (uint8_t)to;
+      sq2[j] = sq2[t->ii.last[l]];
+      uint64_t ci = sq_to_idx(&t->capt[l], sq2);
+      fn(u, 1, 0, synth_value(synth_hash_mixed(t->seedmix, o, 0, l, ci)), NULL, 0);
+    }
+    bb_t q = b & ~occ;
+    while (q) {
+      int to = pop_lsb(&q);
+      sq[k] = (uint8_t)to;
+      bb_t occ2 = occ ^ bit(from) ^ bit(to);
+      if (!attacked_by(t->pt, sq, t->num, occ2, sq[stm], o)) {
+        nlegal++;
+        memcpy(sq2, sq, MAX_PIECES);
+        fn(u, 0, sq_to_idx(&t->ii, sq2), 0, sq, occ2);
+      }
+      sq[k] = (uint8_t)from;
+    }
+  }
+  /* the king (check_king_moves, tb8gen.c:537-569) */
+  int from = sq[stm];
+  bb_t b = KingAtt[from] & ~KingAtt[sq[o]];
+  while (b) {
+    int to = pop_lsb(&b), j = -1;
+    if (occ & bit(to)) {
+      j = 0;
+      while (sq[j] != to) j++;
+      if ((t->pt[j] >> 3) == stm) continue;
+    }
+    sq[stm] = (uint8_t)to;
+    normalize(sq, sq2);
+    int s2 = KKMap[sq2[0]][sq2[1]];
+    if (s2 != s) {
+      int v = synth_king_exit(t, stm, sq, occ, from, j, sq2, s2);
+      if (v != 3) { nlegal++; fn(u, 1, 0, v, NULL, 0); }
+    } else if (j >= 0) {
+      uint8_t xpt[MAX_PIECES], xsq[MAX_PIECES];
+      int nx = 0;
+      for (int x = 0; x < t->num; x++)
+        if (x != j) { xpt[nx] = t->pt[x]; xsq[nx] = sq[x]; nx++; }
+      if (!attacked_by(xpt, xsq, nx, occ ^ bit(from), to, o)) {
+        nlegal++;
+        int l = t->pc_to_set[j];
+        sq2[j] = sq2[t->ii.last[l]];
+        uint64_t ci = sq_to_idx(&t->capt[l], sq2);
+        fn(u, 1, 0, synth_value(synth_hash_mixed(t->seedmix, o, 0, l, ci)), NULL, 0);
+      }
+    } else {
+      bb_t occ2 = occ ^ bit(from) ^ bit(to);
+      if (!attacked_by(t->pt, sq, t->num, occ2, to, o)) {
+        nlegal++;
+        fn(u, 0, sq_to_idx(&t->ii, sq2), 0, sq, occ2);
+      }
+    }
+    sq[stm] = (uint8_t)from;
+  }
+  return nlegal;
+}
+
+typedef struct {
+  const Table *t;
+  int stm;
+  const uint64_t *W1o, *L1o;
+  /* any legal move to a loss for o: via a mate (L_0), a terminal -2, or
+   * a quiet move into the engine's L_1(o); every legal move to a win for
+   * o: terminal 2 (ply 1), or terminal 2 / quiet into W_1(o) (ply 2) */
+  bool w1, all_win1, w2, all_win2, any_w1o;
+} Fwd;
+
+static void fwd_move(void *u, int kind, uint64_t tgt, int val, const uint8_t *q, bb_t qocc)
+{
+  Fwd *f = (Fwd *)u;
+  const Table *t = f->t;
+  int o = f->stm ^ 1;
+  if (kind == 1) {
+    if (val == -2) f->w1 = true;
+    if (val != 2) f->all_win1 = f->all_win2 = false;
+    return;
+  }
+  /* a quiet move inside the slice: o to move at q, which is legal for o */
+  f->all_win1 = false;
+  if (opp_king_attacked(t->pt, q, t->num, qocc, f->stm) &&
+      !has_legal_move_or_cap(t->pt, q, t->num, qocc, o))
+    f->w1 = true; /* q in L_0(o): a mate */
+  if (bm_test(f->L1o, tgt)) f->w2 = true;
+  if (bm_test(f->W1o, tgt)) f->any_w1o = true;
+  else f->all_win2 = false;
+}
+
+static uint64_t rng_next(uint64_t *st)
+{
+  *st += 0x9e3779b97f4a7c15ULL;
+  uint64_t z = *st;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+
+/* A random set bit of plane (a random word, then the next nonzero word at
+ * most 2^20 words on, wrapping); 0 when none was found. */
+static int pick_bit(const uint64_t *p, uint64_t words, uint64_t size, uint64_t *st, uint64_t *idx)
+{
+  uint64_t w = rng_next(st) % words;
+  for (uint64_t k = 0; k < (1u << 20); k++) {
+    uint64_t x = p[w];
+    if (x) {
+      int n = popcnt(x), r = (int)(rng_next(st) % (uint64_t)n);
+      uint64_t i = w * 64 + (uint64_t)select_bit(x, r);
+      if (i < size) { *idx = i; return 1; }
+    }
+    w = w + 1 == words ? 0 : w + 1;
+  }
+  return 0;
+}
+
+int64_t tbo_synth_check(const char *material, int s, uint64_t seed, const uint64_t *const *planes,
+                        uint64_t nsamples, uint64_t sample_seed, uint64_t *checked,
+                        int64_t *first_bad)
+{
+  init_tables();
+  if (s < 0 || s >= 441) return -1;
+  Table tab;
+  memset(&tab, 0, sizeof tab);
+  Table *t = &tab;
+  if (setup_material(t, material)) return -1;
+  t->synth = 1;
+  t->synth_slice = s;
+  t->seed = seed;
+  t->seedmix = splitmix64(seed);
+  /* planes: wins1[2], W1[2], L1[2] after ply 1, W2[2], L2[2] after ply 2 */
+  const uint64_t *wins1[2] = { planes[0], planes[1] }, *W1[2] = { planes[2], planes[3] };
+  const uint64_t *L1[2] = { planes[4], planes[5] }, *W2[2] = { planes[6], planes[7] };
+  const uint64_t *L2[2] = { planes[8], planes[9] };
+  int64_t bad = 0;
+  uint64_t nchk = 0;
+  if (first_bad) first_bad[0] = -1;
+  /* sample list: per side, half uniform, half set bits of the engine's
+   * W1, L1, W2, L2 planes of that side (an eighth each) */
+  uint64_t total = 2 * nsamples;
+  uint64_t *sidx = xalloc(total);
+  uint64_t st = sample_seed;
+  for (uint64_t i = 0; i < total; i++) {
+    int stm = (int)(i / nsamples);
+    uint64_t j = i % nsamples, idx = rng_next(&st) % t->size;
+    if (j >= nsamples / 2) {
+      const uint64_t *pl[4] = { W1[stm], L1[stm], W2[stm], L2[stm] };
+      pick_bit(pl[j % 4], t->words, t->size, &st, &idx);
+    }
+    sidx[i] = idx;
+  }
+#pragma omp parallel for schedule(dynamic, 64) num_threads(g_threads) reduction(+ : bad, nchk)
+  for (uint64_t i = 0; i < total; i++) {
+    int stm = (int)(i / nsamples), o = stm ^ 1;
+    uint64_t idx = sidx[i];
+    uint8_t sq[MAX_PIECES] = { 0 };
+    sq[0] = KKSq[s][0];
+    sq[1] = KKSq[s][1];
+    bb_t occ = idx_to_sq(&t->ii, idx, sq);
+    bool ill = opp_king_attacked(t->pt, sq, t->num, occ, stm);
+    Fwd f = { t, stm, W1[o], L1[o], false, true, false, true, false };
+    int nl = ill ? 0 : synth_moves(t, stm, s, sq, occ, fwd_move, &f);
+    bool w1 = !ill && f.w1;
+    bool l1 = !ill && nl > 0 && f.all_win1;
+    bool w2 = !ill && !w1 && f.w2;
+    bool l2 = !ill && f.any_w1o && f.all_win2;
+    bool exp[5] = { ill || w1, w1, l1, w2, l2 };
+    const uint64_t *got[5] = { wins1[stm], W1[stm], L1[stm], W2[stm], L2[stm] };
+    nchk++;
+    for (int k = 0; k < 5; k++) {
+      bool g = bm_test(got[k], idx);
+      if (g != exp[k]) {
+        bad++;
+#pragma omp critical
+        if (first_bad && first_bad[0] < 0) {
+          first_bad[0] = stm;
+          first_bad[1] = k;
+          first_bad[2] = (int64_t)idx;
+          first_bad[3] = g;
+        }
+      }
+    }
+  }
+  free(sidx);
+  if (checked) *checked = nchk;
+  return bad;
+}
+
 int tbo_get_threads(void) { return g_threads; }
 
 /* ------------------------------------------------------------------ */

@@ -102,6 +102,20 @@ uint64_t tbo_rtbw_verify(int h, const char *material, const uint16_t *codes_w,
                          const uint16_t *codes_b, uint64_t stride, uint64_t *bad,
                          int64_t *first_bad);
 
+/* Sampled forward check of the first two plies of a synthetic slice
+ * (tbo_synth_slice's exits) too large to generate here.  planes[10] are the
+ * engine's bitmaps of the slice: wins[W,B], W_1[W,B], L_1[W,B] after ply 1,
+ * W_2[W,B], L_2[W,B] after ply 2.  For nsamples seeded positions per side
+ * (half uniform, half set bits of the engine's W_1, L_1, W_2, L_2), each of
+ * the five memberships is recomputed from the legal moves out of the
+ * position (targets' ply-1 sets read from the engine's planes, mates and
+ * exits computed here).  Returns the number of mismatching memberships;
+ * first_bad[4] = stm, plane (0 wins, 1 W_1, 2 L_1, 3 W_2, 4 L_2), index,
+ * the engine's bit. */
+int64_t tbo_synth_check(const char *material, int s, uint64_t seed, const uint64_t *const *planes,
+                        uint64_t nsamples, uint64_t sample_seed, uint64_t *checked,
+                        int64_t *first_bad);
+
 /* CPU baseline sample (see tbo.c). */
 int tbo_time_plies(const char *material, int max_plies, double *loop_seconds, int64_t *half_steps);
 int tbo_get_threads(void);

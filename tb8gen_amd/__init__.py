@@ -25,7 +25,8 @@ DRAW_RULE = 100
 
 # the exported entry points of include/tbg.h
 ABI_SYMBOLS = [
-    "tbg_init", "tbg_strerror", "tbg_create", "tbg_create_synth", "tbg_destroy",
+    "tbg_init", "tbg_strerror", "tbg_create", "tbg_create_synth", "tbg_create_synth_ex",
+    "tbg_destroy",
     "tbg_get_info", "tbg_resolve_captures", "tbg_init_illegal_mate", "tbg_step_L",
     "tbg_step_W", "tbg_step_L_plain", "tbg_step_W_plain", "tbg_end_ply", "tbg_generate",
     "tbg_get_codes", "tbg_get_bitmap", "tbg_get_sub_bitmap", "tbg_get_timing",
@@ -39,6 +40,7 @@ ABI_SYMBOLS = [
     "tbg_registry_destroy",
 ]
 SCATTER_GLOBAL, SCATTER_BLOCK, SCATTER_COMBINE = 0, 1, 2
+SYNTH_CODES = 1
 
 
 class TbgError(RuntimeError):
@@ -116,6 +118,8 @@ def lib():
         L.tbg_create.argtypes = [ctypes.POINTER(vp), ctypes.c_char_p]
         L.tbg_create_synth.argtypes = [ctypes.POINTER(vp), ctypes.c_char_p, ctypes.c_int,
                                        ctypes.c_uint64]
+        L.tbg_create_synth_ex.argtypes = [ctypes.POINTER(vp), ctypes.c_char_p, ctypes.c_int,
+                                          ctypes.c_uint64, ctypes.c_uint]
         L.tbg_destroy.argtypes = [vp]
         L.tbg_destroy.restype = None
         L.tbg_get_info.argtypes = [vp, ctypes.POINTER(Info)]
@@ -192,9 +196,10 @@ def host_selftest():
 class Table:
     """Device-resident state of one material (a tbg_ctx)."""
 
-    def __init__(self, material, synth_slice=None, seed=0, ptr=None):
+    def __init__(self, material, synth_slice=None, seed=0, ptr=None, keep_codes=False):
         """ptr: wrap a context owned elsewhere (a NativeRegistry's); close()
-        then leaves it alone."""
+        then leaves it alone.  keep_codes: a synthetic slice also keeps its
+        per-position codes (whole tables always do)."""
         init()
         self.material = material
         self.owned = ptr is None
@@ -204,8 +209,9 @@ class Table:
         elif synth_slice is None:
             check(lib().tbg_create(ctypes.byref(self.ptr), material.encode()), material)
         else:
-            check(lib().tbg_create_synth(ctypes.byref(self.ptr), material.encode(),
-                                         int(synth_slice), int(seed)), material)
+            check(lib().tbg_create_synth_ex(ctypes.byref(self.ptr), material.encode(),
+                                            int(synth_slice), int(seed),
+                                            SYNTH_CODES if keep_codes else 0), material)
         self.synth_slice = synth_slice
         self.lines = []
         self.counts = Counts()

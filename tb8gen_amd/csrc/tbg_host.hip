@@ -1094,8 +1094,15 @@ int tbg_create(tbg_ctx **out, const char *material) { return create_common(out, 
 
 int tbg_create_synth(tbg_ctx **out, const char *material, int slice, uint64_t seed)
 {
+  return tbg_create_synth_ex(out, material, slice, seed, 0);
+}
+
+int tbg_create_synth_ex(tbg_ctx **out, const char *material, int slice, uint64_t seed,
+                        unsigned flags)
+{
   if (slice < 0 || slice >= 441) return fail(TBG_E_INVAL, "synthetic slice must be in [0, 441)");
-  return create_common(out, material, 1, slice, seed);
+  if (flags & ~(unsigned)TBG_SYNTH_CODES) return fail(TBG_E_INVAL, "unknown flags");
+  return create_common(out, material, 1, slice, seed, (flags & TBG_SYNTH_CODES) != 0);
 }
 
 void tbg_destroy(tbg_ctx *c)

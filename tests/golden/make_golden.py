@@ -9,8 +9,14 @@
    too large to regenerate inside a GPU test (per-slice CRC32 of the codes
    and the reference-format count lines).  These pin the HIP engine at that
    size (tests/test_gpu_parity.py).
+3. synth_<M>_s<slice>_<seed>.json -- the same digest of one synthetic king
+   slice (seeded hash exits) run by the oracle to its full fixpoint: the
+   count lines and the CRC32 of every run of CHUNK positions of each side's
+   codes.  The 7-piece KQRBvKQR slice 170 (776,520,240 positions per side)
+   exercises the code that only slices above 2^22 bitmap words reach.
 
-Usage: python tests/golden/make_golden.py [naive|oracle] [MATERIAL ...]
+Usage: python tests/golden/make_golden.py [naive|oracle|synth] [MATERIAL ...]
+       (synth: MATERIAL:SLICE:SEED)
 """
 import hashlib
 import json
@@ -29,7 +35,9 @@ sys.path.insert(0, ROOT)
 NAIVE_3 = ["KQvK", "KRvK", "KBvK", "KNvK"]
 NAIVE_4 = ["KQvKR", "KRvKN", "KBNvK", "KRRvK", "KBBvK", "KQvKQ"]
 ORACLE = ["KQRvKR"]
+SYNTH = ["KQRBvKQR:170:20261203"]
 SAMPLE = 4096
+CHUNK = 1 << 24
 
 
 def build_naive(outdir):
@@ -80,6 +88,25 @@ def oracle_fixture(material):
                    "crc32": crc}, f, separators=(",", ":"))
 
 
+def synth_fixture(spec):
+    from oracle import oracle as O
+    material, s, seed = spec.split(":")
+    s, seed = int(s), int(seed)
+    O.set_threads(os.cpu_count() or 8)
+    log = os.path.join(tempfile.gettempdir(), f"golden_synth_{material}.log")
+    t = O.Table(material, synth_slice=s, seed=seed, log_path=log)
+    with open(log) as f:
+        lines = [l.strip() for l in f if l.strip()]
+    crc = {}
+    for stm in range(2):
+        c = t.codes(stm).ravel()
+        crc[str(stm)] = [zlib.crc32(c[i:i + CHUNK].tobytes()) for i in range(0, c.size, CHUNK)]
+    name = f"synth_{material}_s{s}_{seed}.json"
+    with open(os.path.join(HERE, name), "w") as f:
+        json.dump({"material": material, "slice": s, "seed": seed, "slice_size": int(t.size),
+                   "chunk": CHUNK, "lines": lines, "crc32": crc}, f, separators=(",", ":"))
+
+
 def main():
     what = sys.argv[1] if len(sys.argv) > 1 else "all"
     mats = sys.argv[2:]
@@ -93,6 +120,10 @@ def main():
         for m in mats or ORACLE:
             print("oracle", m, flush=True)
             oracle_fixture(m)
+    if what in ("all", "synth"):
+        for spec in mats or SYNTH:
+            print("synth", spec, flush=True)
+            synth_fixture(spec)
 
 
 if __name__ == "__main__":

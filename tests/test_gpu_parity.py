@@ -123,16 +123,20 @@ def test_five_piece_against_oracle_digests(registry, path):
                                                    ("KRRvKB", 300, 3)])
 def test_synthetic_single_slice(material, slice_, seed):
     """Single-slice jobs with hashed capture and king-move exits (the form
-    of the 8-piece workload) agree with the oracle's synthetic mode."""
+    of the 8-piece workload): count lines and every code of the slice at the
+    fixpoint equal the oracle's synthetic mode."""
     print(f"case synth {material} s{slice_} seed{seed}", flush=True)
     o = O.Table(material, synth_slice=slice_, seed=seed,
                 log_path=f"/tmp/oracle_synth_{material}.log")
     with open(f"/tmp/oracle_synth_{material}.log") as f:
         olines = [l.strip() for l in f if l.strip()]
-    t = tbg.Table(material, synth_slice=slice_, seed=seed)
+    t = tbg.Table(material, synth_slice=slice_, seed=seed, keep_codes=True)
     t.resolve([])
     lines = t.generate(tbg.FORM_BATCHED)
     assert lines == olines
+    for stm in range(2):
+        g, c = t.codes(stm)[0], o.codes(stm)[0]
+        assert np.array_equal(g, c), (stm, int((g != c).sum()))
     t.close()
 
 
@@ -189,27 +193,6 @@ def test_geometry_does_not_change_results(registry, gi):
     t.close()
 
 
-def test_geometry_synthetic_multi_block():
-    """A multi-block 6-piece synthetic slice under two extreme geometries."""
-    material, s, seed = "KQRvKRN", 170, 5
-    print(f"case geometry synth {material}", flush=True)
-    ref = None
-    for geo in _tune_candidates()[:3]:
-        t = tbg.Table(material, synth_slice=s, seed=seed)
-        t.set_geometry(tbg.Geometry(**geo))
-        t.resolve([])
-        lines = t.generate(tbg.FORM_BATCHED)
-        # synthetic jobs keep bitmaps only: the final wins(stm) of the slice
-        wins = [t.bitmap(tbg.PLANE_WINS, stm, s) for stm in range(2)]
-        if ref is None:
-            ref = (lines, wins)
-        else:
-            assert lines == ref[0]
-            for stm in range(2):
-                assert np.array_equal(wins[stm], ref[1][stm])
-        t.close()
-
-
 def test_autotune_picks_and_caches(registry, tmp_path):
     """The autotuner times its candidates, applies one, appends it to the
     cache file, and a second call is a cache hit; the table generated
@@ -232,26 +215,6 @@ def test_autotune_picks_and_caches(registry, tmp_path):
     assert lines == oracle_lines(material)
     assert_same_table(material, t)
     t.close()
-
-
-def test_synthetic_block_scatter_multi_block():
-    """A 6-piece single slice is larger than one LDS block, so the block
-    scatter splits it by its leading digit."""
-    material, s, seed = "KQRvKRN", 170, 5
-    print(f"case synth multi-block {material}", flush=True)
-    o = O.Table(material, synth_slice=s, seed=seed, log_path="/tmp/oracle_synth_mb.log",
-                max_plies=12)
-    with open("/tmp/oracle_synth_mb.log") as f:
-        olines = [l.strip() for l in f if l.strip()]
-    for mode in (tbg.SCATTER_BLOCK, tbg.SCATTER_GLOBAL, tbg.SCATTER_BLOCK | tbg.SCATTER_COMBINE,
-                 tbg.SCATTER_GLOBAL | tbg.SCATTER_COMBINE):
-        t = tbg.Table(material, synth_slice=s, seed=seed)
-        t.set_scatter(mode)
-        t.resolve([])
-        lines = t.generate(tbg.FORM_BATCHED)
-        # the oracle was cut at ply 12; compare the common prefix
-        assert lines[:len(olines)] == olines
-        t.close()
 
 
 def test_graph_form(registry):
@@ -330,7 +293,8 @@ def test_host_driven_steps_follow_main_loop(registry):
 
 
 SERVED_JOBS = [("KQRvKR", 170, 7), ("KRBvKN", 57, 11), ("KRRvKB", 300, 3), ("KQNvKB", 12, 5),
-               ("KQRvKR", 440, 9), ("KBNvKN", 200, 2), ("KRRvKB", 13, 4), ("KQvKRN", 99, 8)]
+               ("KQRvKR", 440, 9), ("KBNvKN", 200, 2), ("KRRvKB", 13, 4), ("KQvKRN", 99, 8),
+               ("KQRvKRN", 170, 5), ("KRBvKRN", 57, 9)]
 
 
 def _oracle_synth(material, s, seed):
@@ -346,6 +310,7 @@ def test_served_jobs_same_alone_and_in_the_mix():
     print("case served mix", flush=True)
     srv = tbg.Server(budget_bytes=8 << 30, max_active=6)
     # four jobs at once, the rest 1 ms later, while the first ones still run
+    # (the last two are 6-piece slices of 13,388,280 positions)
     ids = [srv.submit(m, s, seed, arrival_ms=0.0 if i < 4 else 1.0)
            for i, (m, s, seed) in enumerate(SERVED_JOBS)]
     st = srv.run(tbg.FORM_BATCHED)
@@ -418,35 +383,6 @@ def test_checkpoint_restore_continue(registry, tmp_path, stop):
     assert_same_table(material, t2)
     t.close()
     t2.close()
-
-
-def test_checkpoint_synthetic_multi_block(tmp_path):
-    """A synthetic 6-piece slice stopped at ply 4, checkpointed, restored and
-    finished equals the uninterrupted run (lines and final wins)."""
-    material, s, seed = "KQRvKRN", 170, 5
-    print(f"case checkpoint synth {material}", flush=True)
-    ref = tbg.Table(material, synth_slice=s, seed=seed)
-    ref.resolve([])
-    ref_lines = ref.generate(tbg.FORM_BATCHED)
-    ref_wins = [ref.bitmap(tbg.PLANE_WINS, stm, s) for stm in range(2)]
-    ref.close()
-    t = tbg.Table(material, synth_slice=s, seed=seed)
-    t.resolve([])
-    t.init_illegal_mate()
-    t.continue_(tbg.FORM_BATCHED, stop_at=4)
-    t.checkpoint(tmp_path)
-    t.close()
-    t2 = tbg.Table(material, synth_slice=s, seed=seed)
-    t2.restore(tmp_path)
-    assert t2.continue_(tbg.FORM_BATCHED) == ref_lines
-    for stm in range(2):
-        assert np.array_equal(t2.bitmap(tbg.PLANE_WINS, stm, s), ref_wins[stm])
-    t2.close()
-    # a checkpoint of another seed does not restore into this job
-    t3 = tbg.Table(material, synth_slice=s, seed=seed + 1)
-    with pytest.raises(tbg.TbgError):
-        t3.restore(tmp_path)
-    t3.close()
 
 
 def test_export_plies_reference_files(registry, tmp_path):

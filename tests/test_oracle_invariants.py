@@ -143,3 +143,27 @@ def test_synthetic_slice_deterministic_and_seeded():
     h = np.array([O.synth_hash(7, 0, 0, 0, i) for i in range(20000)], dtype=np.uint64)
     win = (h < np.uint64(553402322211286548)).mean()
     assert 0.02 < win < 0.04
+
+
+def test_synth_forward_check_pins_itself():
+    """The sampled forward check used on the 8-piece slice (moves forward
+    from each position) agrees with the oracle's retrograde generation
+    (unmoves backward) on a 6-piece synthetic slice, and catches planted
+    mistakes in the planes it is given: an off-by-one shift of one plane,
+    and a plane emptied."""
+    m, s, seed = "KQRvKRN", 170, 5
+    o = O.Table(m, synth_slice=s, seed=seed, max_plies=3)
+    words = ((o.size + 511) // 512) * 8
+    pl = O.planes_from_codes(o.codes(0), o.codes(1), words)
+    assert all(int(np.bitwise_count(p).sum()) > 0 for p in pl)
+    bad, checked, first = O.synth_check(m, s, seed, pl, 20000, sample_seed=3)
+    assert (bad, checked) == (0, 40000), first
+    shifted = [p.copy() for p in pl]
+    shifted[5] = (shifted[5] << np.uint64(1)) | np.concatenate(
+        [[np.uint64(0)], shifted[5][:-1] >> np.uint64(63)])  # L_1(black), index + 1
+    bad, _, first = O.synth_check(m, s, seed, shifted, 20000, sample_seed=3)
+    assert bad > 0 and first is not None
+    emptied = [p.copy() for p in pl]
+    emptied[6][:] = 0  # W_2(white)
+    bad, _, first = O.synth_check(m, s, seed, emptied, 20000, sample_seed=3)
+    assert bad > 0 and first[:2] == (0, 3)
