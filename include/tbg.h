@@ -141,6 +141,18 @@ int tbg_generate(tbg_ctx *ctx, int form, tbg_line_fn fn, void *user, tbg_counts 
  * context must have its capture exits resolved; they must not depend on
  * each other.  counts receives nc entries (or is null). */
 int tbg_generate_batch(tbg_ctx *const *ctxs, int nc, int form, tbg_counts *counts);
+/* tbg_generate_batch in the batched form for callers that cannot allocate
+ * device memory (tight serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  The batch's per-call device state (job
+ * pointers, flags, work-unit prefix sums) lives in a fixed library array of
+ * TBG_NOALLOC_SCRATCH_BYTES per GPU instead of memory allocated per call; it
+ * calls no hipMalloc, hipFree or other allocating routine.  At most
+ * TBG_NOALLOC_MAX_BATCH contexts (TBG_E_INVAL beyond); profiling must be off.
+ * Every call on one GPU shares that array: calls that overlap on one GPU
+ * (from several host threads) race and must not be made. */
+#define TBG_NOALLOC_MAX_BATCH 256
+#define TBG_NOALLOC_SCRATCH_BYTES 32768
+int tbg_generate_batch_noalloc(tbg_ctx *const *ctxs, int nc, tbg_counts *counts);
 /* The count lines of a generated context, in the reference's order. */
 int tbg_emit_lines(const tbg_ctx *ctx, tbg_line_fn fn, void *user);
 

@@ -31,7 +31,7 @@ ABI_SYMBOLS = [
     "tbg_step_W", "tbg_step_L_plain", "tbg_step_W_plain", "tbg_end_ply", "tbg_generate",
     "tbg_get_codes", "tbg_get_bitmap", "tbg_get_sub_bitmap", "tbg_get_timing",
     "tbg_set_profiling", "tbg_host_selftest", "tbg_set_scatter", "tbg_generate_batch",
-    "tbg_emit_lines", "tbg_server_create", "tbg_server_destroy", "tbg_server_submit",
+    "tbg_emit_lines", "tbg_generate_batch_noalloc", "tbg_server_create", "tbg_server_destroy", "tbg_server_submit",
     "tbg_server_run", "tbg_server_result", "tbg_server_codes", "tbg_server_lines",
     "tbg_set_geometry", "tbg_get_geometry", "tbg_autotune", "tbg_file_write", "tbg_file_read",
     "tbg_checkpoint", "tbg_restore", "tbg_continue", "tbg_export_plies", "tbg_rtbw_name",
@@ -41,6 +41,7 @@ ABI_SYMBOLS = [
 ]
 SCATTER_GLOBAL, SCATTER_BLOCK, SCATTER_COMBINE = 0, 1, 2
 SYNTH_CODES = 1
+NOALLOC_MAX_BATCH = 256
 
 
 class TbgError(RuntimeError):
@@ -142,6 +143,8 @@ def lib():
         L.tbg_generate_batch.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.c_int,
                                          ctypes.POINTER(Counts)]
         L.tbg_emit_lines.argtypes = [vp, LINE_FN, vp]
+        L.tbg_generate_batch_noalloc.argtypes = [ctypes.POINTER(vp), ctypes.c_int,
+                                                 ctypes.POINTER(Counts)]
         L.tbg_server_create.argtypes = [ctypes.POINTER(vp), ctypes.c_uint64, ctypes.c_int]
         L.tbg_server_destroy.argtypes = [vp]
         L.tbg_server_destroy.restype = None
@@ -403,6 +406,18 @@ def generate_batch(tables, form=FORM_GRAPH):
     arr = (ctypes.c_void_p * len(tables))(*[t.ptr.value for t in tables])
     counts = (Counts * len(tables))()
     check(lib().tbg_generate_batch(arr, len(tables), form, counts), "generate_batch")
+    for t, c in zip(tables, counts):
+        t.counts = c
+        t.emit_lines()
+    return [t.lines for t in tables]
+
+
+def generate_batch_noalloc(tables):
+    """generate_batch in the batched form without any device allocation
+    (tbg_generate_batch_noalloc; at most NOALLOC_MAX_BATCH tables)."""
+    arr = (ctypes.c_void_p * len(tables))(*[t.ptr.value for t in tables])
+    counts = (Counts * len(tables))()
+    check(lib().tbg_generate_batch_noalloc(arr, len(tables), counts), "generate_batch_noalloc")
     for t, c in zip(tables, counts):
         t.counts = c
         t.emit_lines()

@@ -23,6 +23,10 @@
 
 using namespace tbg;
 
+// Device memory of the entry point that allocates nothing: one copy per GPU,
+// held until the process ends.  Every call on a GPU shares it.
+static __device__ __align__(256) uint8_t scratch[TBG_NOALLOC_SCRATCH_BYTES];
+
 namespace {
 
 thread_local char g_errbuf[256];
@@ -36,6 +40,22 @@ Tabs g_htabs;
 Geom g_geom = { 256, 4096, { 3, 16, 16, 16 } };
 int g_scatter = TBG_SCATTER_BLOCK | TBG_SCATTER_COMBINE;
 char g_arch[64] = "none";
+
+// Scratch of tbg_generate_batch_noalloc: the batch's job-pointer array, its
+// flags, its (job, side) pairs and the four prefix-sum arrays of its work
+// units, at fixed offsets (256-byte aligned) inside `scratch` below.
+constexpr int kNaMax = TBG_NOALLOC_MAX_BATCH;
+constexpr size_t kNaJobs = 0;
+constexpr size_t kNaFlags = kNaJobs + ((sizeof(JobDev *) * kNaMax + 255) & ~(size_t)255);
+constexpr size_t kNaPairs = kNaFlags + 256;
+constexpr size_t kNaPStride = (sizeof(uint64_t) * (2 * kNaMax + 1) + 255) & ~(size_t)255;
+constexpr size_t kNaP = kNaPairs + ((sizeof(Pair) * 2 * kNaMax + 255) & ~(size_t)255);
+constexpr size_t kNaBytes = kNaP + 4 * kNaPStride;
+static_assert(kNaBytes <= TBG_NOALLOC_SCRATCH_BYTES, "no-allocation scratch layout");
+// device address of `scratch` on each device (hipGetSymbolAddress at
+// tbg_init: the array's name is not a device address in host code)
+constexpr int kMaxDevices = 64;
+uint8_t *g_scratch[kMaxDevices];
 
 int fail(int code, const char *fmt, ...)
 {
@@ -345,29 +365,31 @@ int block_parts(uint64_t nblk, uint64_t blk_pos)
   return parts < 1 ? 1 : parts > (uint64_t)kMaxParts ? kMaxParts : (int)parts;
 }
 
-// window_slices 0: each job's own slices (462, or 1 for a synthetic job)
-int build_launchset(LaunchSet &L, const std::vector<const JobDev *> &hjobs,
-                    const std::vector<const Mat *> &mats, const std::vector<Pair> &pairs,
-                    uint64_t window_slices, const Geom &g, hipStream_t st)
+// Host side of a launch configuration: for np (job, side) pairs, the
+// prefix sums of their work units per kernel type into P[kt * (np + 1) ..]
+// and the launch parameters that depend on the jobs.  window_slices 0: each
+// job's own slices (462, or 1 for a synthetic job).
+void plan_launchset(LaunchSet &L, const JobDev *const *hjobs, const Mat *const *mats, int nj,
+                    const Pair *pairs, int np, uint64_t window_slices, const Geom &g, uint64_t *P)
 {
-  if (pairs.size() > L.cap) L.release();  // else the device buffers are reused
-  L.np = (int)pairs.size();
+  L.np = np;
   L.g = g;
   uint64_t maxbw = 1;
-  std::vector<uint64_t> P[4];
   for (int kt = 0; kt < 4; kt++) {
-    P[kt].assign(pairs.size() + 1, 0);
-    for (size_t i = 0; i < pairs.size(); i++) {
+    uint64_t *Pk = P + (size_t)kt * (np + 1);
+    Pk[0] = 0;
+    for (int i = 0; i < np; i++) {
       const Mat &M = *mats[pairs[i].job];
       uint64_t ns = window_slices ? window_slices : (M.synth ? 1 : NSL);
       const JobDev *J = hjobs[pairs[i].job];
-      P[kt][i + 1] = P[kt][i] + ns * units_per_slice(M, J->nblk * J->blk_parts, kt, pairs[i].side, g);
+      Pk[i + 1] = Pk[i] + ns * units_per_slice(M, J->nblk * J->blk_parts, kt, pairs[i].side, g);
     }
-    L.total[kt] = P[kt][pairs.size()];
+    L.total[kt] = Pk[np];
   }
   bool block = false;
   L.pred_wide = false;
-  for (const JobDev *j : hjobs) {
+  for (int i = 0; i < nj; i++) {
+    const JobDev *j = hjobs[i];
     maxbw = j->blk_words > maxbw ? j->blk_words : maxbw;
     block |= (j->scatter & TBG_SCATTER_BLOCK) != 0;
     L.pred_wide |= !(j->scatter & TBG_SCATTER_BLOCK) || j->blk_set > 0;
@@ -376,7 +398,7 @@ int build_launchset(LaunchSet &L, const std::vector<const JobDev *> &hjobs,
   // one block)
   L.uncap_lds = 0;
   L.verify_split = false;
-  for (size_t i = 0; i < hjobs.size(); i++) {
+  for (int i = 0; i < nj; i++) {
     L.verify_split |= mats[i]->ii.size > VERIFY_SPLIT_SIZE;
     uint64_t nbw = (mats[i]->ii.size + 63) >> 6;
     if ((hjobs[i]->scatter & TBG_SCATTER_BLOCK) && hjobs[i]->nblk == 1 && nbw > L.uncap_lds)
@@ -384,17 +406,34 @@ int build_launchset(LaunchSet &L, const std::vector<const JobDev *> &hjobs,
   }
   // the queues follow the block bitmap, which only the block scatter uses
   L.pred_lds = (block ? maxbw * 8 : 0) + PB_QUEUE_BYTES;
+}
+
+// Copies the pairs and the prefix sums of a planned launch set to its
+// device arrays.  The sources may be pageable: the copies are staged
+// before the calls return.
+int upload_launchset(const LaunchSet &L, const Pair *pairs, const uint64_t *P, hipStream_t st)
+{
+  HIPCHK(hipMemcpyAsync(L.pairs, pairs, sizeof(Pair) * L.np, hipMemcpyHostToDevice, st));
+  for (int kt = 0; kt < 4; kt++)
+    HIPCHK(hipMemcpyAsync(L.P[kt], P + (size_t)kt * (L.np + 1), sizeof(uint64_t) * (L.np + 1),
+                          hipMemcpyHostToDevice, st));
+  return TBG_OK;
+}
+
+int build_launchset(LaunchSet &L, const std::vector<const JobDev *> &hjobs,
+                    const std::vector<const Mat *> &mats, const std::vector<Pair> &pairs,
+                    uint64_t window_slices, const Geom &g, hipStream_t st)
+{
+  if (pairs.size() > L.cap) L.release();  // else the device buffers are reused
+  std::vector<uint64_t> P(4 * (pairs.size() + 1));
+  plan_launchset(L, hjobs.data(), mats.data(), (int)hjobs.size(), pairs.data(), (int)pairs.size(),
+                 window_slices, g, P.data());
   if (!L.cap) {
     L.cap = pairs.size();
     HIPCHK(hipMalloc(&L.pairs, sizeof(Pair) * L.cap));
     for (int kt = 0; kt < 4; kt++) HIPCHK(hipMalloc(&L.P[kt], sizeof(uint64_t) * (L.cap + 1)));
   }
-  // pageable sources: the copies are staged before the calls return
-  HIPCHK(hipMemcpyAsync(L.pairs, pairs.data(), sizeof(Pair) * pairs.size(), hipMemcpyHostToDevice, st));
-  for (int kt = 0; kt < 4; kt++)
-    HIPCHK(hipMemcpyAsync(L.P[kt], P[kt].data(), sizeof(uint64_t) * P[kt].size(),
-                          hipMemcpyHostToDevice, st));
-  return TBG_OK;
+  return upload_launchset(L, pairs.data(), P.data(), st);
 }
 
 // canonical job state: planes in their allocation order, ply 1, all steps
@@ -525,6 +564,7 @@ int ctx_alloc(tbg_ctx *c, bool with_codes, hipStream_t ext = nullptr)
   TRY(dalloc(c, (void **)&c->djp, sizeof(JobDev *)));
   HIPCHK(hipMemcpyAsync(c->djp, &c->dj, sizeof(JobDev *), hipMemcpyHostToDevice, c->st));
   TRY(dalloc(c, (void **)&c->dhist, sizeof(long long) * 4 * c->max_ply));
+  c->hist.assign((size_t)4 * c->max_ply, -1);  // sized once: finish_job refills it in place
   manager_order(WHITE, c->mgr[WHITE]);
   manager_order(BLACK, c->mgr[BLACK]);
   for (int i = 0; i < CAT_N; i++) c->ms[i] = 0, c->nl[i] = 0;
@@ -1024,6 +1064,13 @@ int tbg_init(int device)
     if (*q == ' ' || *q == ':') *q = '_';
   build_tabs(g_htabs);
   HIPCHK(upload_tabs(&g_htabs));
+  if (device < kMaxDevices && !g_scratch[device]) {
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(TBG_E_HIP, "address of the no-allocation scratch: %s", hipGetErrorString(e));
+    g_scratch[device] = static_cast<uint8_t *>(p);
+  }
   HIPCHK(hipDeviceSynchronize());
   g_device = device;
   return TBG_OK;
@@ -1292,6 +1339,65 @@ int tbg_generate_batch(tbg_ctx *const *cs, int nc, int form, tbg_counts *counts)
     if (r != TBG_OK) return r;
     for (int i = 0; i < nc; i++) TRY(finish_job(cs[i]));
   }
+  if (counts)
+    for (int i = 0; i < nc; i++) counts[i] = cs[i]->counts;
+  return TBG_OK;
+}
+
+int tbg_generate_batch_noalloc(tbg_ctx *const *cs, int nc, tbg_counts *counts)
+{
+  if (!cs || nc < 1) return fail(TBG_E_INVAL, "bad batch");
+  if (nc > TBG_NOALLOC_MAX_BATCH)
+    return fail(TBG_E_INVAL, "batch of %d contexts: the no-allocation scratch holds at most %d",
+                nc, TBG_NOALLOC_MAX_BATCH);
+  if (g_device < 0 || g_device >= kMaxDevices || !g_scratch[g_device])
+    return fail(TBG_E_STATE, "tbg_init not called");
+  for (int i = 0; i < nc; i++) {
+    if (!cs[i]) return fail(TBG_E_INVAL, "null context in batch");
+    if (!cs[i]->resolved) return fail(TBG_E_STATE, "resolve captures of %s first", cs[i]->name);
+    if (cs[i]->prof) return fail(TBG_E_STATE, "profiling of %s is on (it creates events)", cs[i]->name);
+    if (cs[i]->pooled) return fail(TBG_E_STATE, "%s belongs to a server", cs[i]->name);
+    for (int j = 0; j < i; j++)
+      if (cs[j] == cs[i]) return fail(TBG_E_INVAL, "context twice in a batch");
+  }
+  for (int i = 0; i < nc; i++) TRY(run_init(cs[i]));
+  tbg_ctx *c0 = cs[0];
+  uint8_t *sp = g_scratch[g_device];
+  JobDev **dj = reinterpret_cast<JobDev **>(sp + kNaJobs);
+  unsigned *df = reinterpret_cast<unsigned *>(sp + kNaFlags);
+  // host staging on the stack: no heap either
+  JobDev *ptrs[kNaMax];
+  const JobDev *hj[kNaMax];
+  const Mat *mats[kNaMax];
+  Pair pairs[2 * kNaMax];
+  uint64_t P[4 * (2 * kNaMax + 1)];
+  for (int i = 0; i < nc; i++) {
+    HIPCHK(hipStreamSynchronize(cs[i]->st));
+    ptrs[i] = cs[i]->dj;
+    hj[i] = &cs[i]->hj;
+    mats[i] = &cs[i]->h;
+    pairs[2 * i] = { i, 0 };
+    pairs[2 * i + 1] = { i, 1 };
+  }
+  LaunchSet L;  // its arrays live in the scratch: never released
+  plan_launchset(L, hj, mats, nc, pairs, 2 * nc, 0, c0->geom, P);
+  L.pairs = reinterpret_cast<Pair *>(sp + kNaPairs);
+  for (int kt = 0; kt < 4; kt++) L.P[kt] = reinterpret_cast<uint64_t *>(sp + kNaP + kt * kNaPStride);
+  L.cap = 2 * nc;
+  // the scratch keeps the previous call's contents: every field read below
+  // is written first (flags start at {0 done, 1 uncapture pending})
+  unsigned f[4] = { 0, 1, 0, 0 };
+  HIPCHK(hipMemcpyAsync(dj, ptrs, sizeof(JobDev *) * nc, hipMemcpyHostToDevice, c0->st));
+  HIPCHK(hipMemcpyAsync(df, f, sizeof f, hipMemcpyHostToDevice, c0->st));
+  TRY(upload_launchset(L, pairs, P, c0->st));
+  unsigned hf[2] = { 0, 1 };
+  for (;;) {
+    TRY(launch_ply(nullptr, dj, nc, L, df, c0->st, hf[1] != 0));
+    HIPCHK(hipMemcpyAsync(hf, df, sizeof hf, hipMemcpyDeviceToHost, c0->st));
+    HIPCHK(hipStreamSynchronize(c0->st));
+    if ((int)hf[0] >= nc) break;
+  }
+  for (int i = 0; i < nc; i++) TRY(finish_job(cs[i], c0->st));
   if (counts)
     for (int i = 0; i < nc; i++) counts[i] = cs[i]->counts;
   return TBG_OK;
