@@ -1,27 +1,32 @@
 #!/usr/bin/env python3
 """bench.py -- retrograde throughput of the MI355X king-slice engine.
 
-Workload (default, one GPU): the KQRvKR table -- 462 king slices x 226,920
-positions per side (62*61*60), exhaustive, no input data -- generated
-end to end on the device: capture exits resolved against the resident
-KQvKR / KRvKR / KQRvK tables (generated beforehand, untimed), illegal and
-mate init, then the ply loop of the reference's main() to its fixpoint.
-One step = one such generation.
+Workload (default, one GPU): one king slice of the 8-piece material
+KQRBvKQRN at real shape -- slice 170 (white king d1, black king e8), six
+distinct non-kings over 62 squares, 44,261,653,680 positions per side,
+1-bit bitmaps -- with capture and king-move exits drawn from a seeded
+splitmix64 hash (win below 0.03, loss in [0.03, 0.06)), run to its full
+fixpoint.  One step = capture exits + init + the ply loop to the fixpoint.
 
-value = position-side-plies per second over the whole step: every executed
-half-step (L or W, one side) covers 462 * 226,920 positions for half a ply,
-so a full ply of both sides is 2 * 104,837,040 position-side-plies.
-roofline: the kernel with the largest device time (kernel_rooflines lists all
-three ply kernels): algorithmic bytes per launch = its bits per position (pred
-2, verify 3, W sweep 4; 11 per position-side-ply in all) x positions x the
-half-steps it executed / its launches, over its average launch time measured
-with HIP events on the engine's stream; traffic = HBM bytes per launch from
-the committed rocprof counter summary of the same workload (profiles/).
+value = position-side-plies per second, the mean over the timed steps:
+every executed half-step (L or W, one side) covers the slice's positions
+for half a ply.
+roofline: the kernel with the largest device time (pred, verify, sweep or
+init): its algorithmic bytes per launch (bits per position: pred 2, verify 3,
+W sweep 4 per executed half-step; init 8 output planes) over its average
+launch time from HIP events on the engine's stream; traffic = HBM bytes per
+launch from the committed rocprof counter summary (profiles/).
+secondary: the KQRvKR table (462 slices x 226,920 positions per side)
+generated end to end in the batched form.
 cpu_baseline: the CPU oracle (a plain-C restatement of the reference's
-algorithm) on the host cores, timed over plies 1..2 of the same material.
+algorithm, OpenMP) on the host cores, on a bounded sample of the same kind
+of job: the full fixpoint, init included, of the 6-piece synthetic slice
+KQRvKRN 170 with the same exit hash (also timed on the GPU for a
+like-for-like ratio), plus the init of 2^24 positions of the 8-piece slice
+itself; the extrapolation to the whole 8-piece slice is stated.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--form batched|graph|plain]
-       [--workload table|synth|stream|served] [--no-autotune] [--scatter MODE]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+       [--workload synth|table|stream|served] [--form batched|graph|plain] ...
 """
 import argparse
 import json
@@ -39,8 +44,27 @@ METRIC = "retro positions/s per GPU + HBM GB/s fraction; p50/p99 slice latency v
 
 
 def cpu_threads():
-    n = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
-    return max(1, min(n, 16))
+    """Threads for the CPU oracle: every core the process may use, within
+    the OMP_NUM_THREADS share when one is set."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    lim = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return max(1, min(n, lim) if lim > 0 else n)
+
+
+def host_cpu():
+    """nproc and the lscpu model line of this host."""
+    import subprocess
+    info = {"nproc": os.cpu_count()}
+    try:
+        info["nproc"] = int(subprocess.run(["nproc"], capture_output=True, text=True).stdout)
+        out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
+        for l in out.splitlines():
+            k, _, v = l.partition(":")
+            if k.strip() in ("Model name", "CPU(s)", "Socket(s)", "Thread(s) per core"):
+                info[k.strip()] = v.strip()
+    except OSError:
+        pass
+    return info
 
 
 def run_steps(table, subs, form, steps, tbg):
@@ -57,28 +81,33 @@ def run_steps(table, subs, form, steps, tbg):
     return secs, timing, lines
 
 
-def kernel_rooflines(tm, lines, positions, steps, traffic=None):
-    """Per ply kernel: algorithmic bytes per launch = its bits per position x
+def kernel_rooflines(tm, lines, positions, steps, traffic=None, init_bits=4):
+    """Per kernel: algorithmic bytes per launch = its bits per position x
     positions x the half-steps it executed (from the count lines: pred runs
     for every W step and every L step after ply 1, verify for L steps, sweep
-    for W steps) / its launches; average launch time from the HIP events;
-    achieved GB/s = the two divided.  traffic: measured HBM bytes per launch
+    for W steps) / its launches; init writes init_bits planes once per step
+    (illegal and mates per side, plus the four exit-candidate planes of a
+    synthetic job).  Average launch time from the HIP events; achieved GB/s
+    = the two divided.  traffic: measured HBM bytes per launch
     (copy-calibrated rocprof counters) where a profile is committed."""
     lh = [l for l in lines if re.match(r"l\d+_[wb] = ", l) and not l.startswith("l0_")]
     wh = [l for l in lines if re.match(r"w\d+_[wb] = ", l)]
     l1 = [l for l in lh if l.startswith("l1_")]
     hs = {"pred": len(lh) - len(l1) + len(wh), "verify": len(lh), "sweep": len(wh)}
     out = {}
-    for k in ("pred", "verify", "sweep"):
+    for k in ("pred", "verify", "sweep", "init"):
         ms = getattr(tm, "ms_" + k) / steps
         n = getattr(tm, "n_" + k) / steps
-        alg = BITS[k] * positions * hs[k] / 8.0 / max(n, 1.0)
+        if k == "init":
+            alg = init_bits * positions / 8.0
+        else:
+            alg = BITS[k] * positions * hs[k] / 8.0 / max(n, 1.0)
         avg = ms / max(n, 1.0)
         ach = alg / (avg * 1e-3) / 1e9 if avg > 0 else 0.0
         out[k] = {"kernel": k, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
                   "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
                   "traffic": (traffic or {}).get(k), "alg_bytes_per_launch": alg,
-                  "avg_launch_ms": avg, "launches": n, "half_steps": hs[k], "ms": ms}
+                  "avg_launch_ms": avg, "launches": n, "half_steps": hs.get(k, 1), "ms": ms}
     return out
 
 
@@ -265,11 +294,160 @@ def run_served(args, tbg):
     print(json.dumps(out), flush=True)
 
 
+def synth_steps(table, form, steps, warmup, tbg, progress=True):
+    """warmup + steps full generations of a synthetic slice; returns the
+    timed steps' seconds and the last step's count lines."""
+    secs, lines = [], []
+    table.set_profiling(True)
+    for step in range(warmup + steps):
+        t0 = time.perf_counter()
+        lines = []
+
+        def cb(line, _u, t0=t0, lines=lines):
+            lines.append(line.decode())
+            if progress:
+                print(f"  [{time.perf_counter() - t0:8.2f} s] {line.decode()}", file=sys.stderr,
+                      flush=True)
+
+        fn = tbg.LINE_FN(cb)
+        table.resolve([])
+        tbg.check(tbg.lib().tbg_generate(table.ptr, form, fn, None,
+                                         tbg.ctypes.byref(table.counts)), "generate")
+        dt = time.perf_counter() - t0
+        if step == warmup - 1:  # the timings cover the timed steps only
+            table.set_profiling(False)
+            table.set_profiling(True)
+        if step >= warmup:
+            secs.append(dt)
+    return secs, lines
+
+
+def cpu_baseline_synth(args, tbg, material8, n8, psp8):
+    """The CPU oracle on a bounded sample of the same kind of job, on the
+    host's cores: the full fixpoint (init included, no codes) of the 6-piece
+    synthetic slice KQRvKRN 170 with the same exit hash -- also generated on
+    the GPU for a like-for-like ratio -- and the init of 2^24 positions of the
+    8-piece slice itself.  The whole 8-piece slice on the CPU is extrapolated
+    from the two rates."""
+    from oracle import oracle as O
+    threads = cpu_threads()
+    info = host_cpu()
+    print(f"cpu baseline: {threads} threads; host {info}", file=sys.stderr, flush=True)
+    O.set_threads(threads)
+    O.set_keep_synth_codes(False)
+    m6, s6 = "KQRvKRN", args.slice
+    log = f"/tmp/bench_cpu_{m6}.log"
+    t0 = time.perf_counter()
+    o = O.Table(m6, synth_slice=s6, seed=args.seed, log_path=log)
+    sec6 = time.perf_counter() - t0
+    with open(log) as f:
+        olines = [l.strip() for l in f if l.strip()]
+    hs6 = sum(1 for l in olines if re.match(r"[lw]\d+_[wb] = ", l) and not l.startswith("l0_"))
+    cpu6 = hs6 * o.size / 2.0 / sec6
+    # the same job on the GPU
+    t = tbg.Table(m6, synth_slice=s6, seed=args.seed)
+    gsecs, glines = synth_steps(t, tbg.FORM_BATCHED, 5, 1, tbg, progress=False)
+    t.close()
+    gpu6 = hs6 * o.size / 2.0 / (sum(gsecs) / len(gsecs))
+    n_init = 1 << 24
+    sec_i, _ = O.synth_init_time(material8, args.slice, args.seed, 0, n_init)
+    init_rate = n_init / sec_i
+    extrap = n8 / init_rate + psp8 / cpu6
+    print(f"cpu baseline: {m6} {sec6:.2f} s ({cpu6:.3e}/s; GPU {gpu6:.3e}/s), "
+          f"8-piece init sample {init_rate:.3e} positions/s", file=sys.stderr, flush=True)
+    return {
+        "value": cpu6, "unit": "position-side-plies/s", "cores": threads, "kind": "port",
+        "sample": f"CPU oracle (C restatement of the reference algorithm, OpenMP, {threads} "
+                  f"threads) on the full fixpoint, init included, of the synthetic 6-piece "
+                  f"slice {m6} {s6} (seed {args.seed}, same exit hash): {hs6} half-steps x "
+                  f"{o.size} positions in {sec6:.2f} s; the same job on the GPU runs at "
+                  f"{gpu6:.3e}/s ({gpu6 / cpu6:.0f}x)",
+        "host": info,
+        "like_for_like": {"job": f"{m6} slice {s6} synthetic, full fixpoint",
+                          "cpu": cpu6, "gpu": gpu6, "gpu_over_cpu": gpu6 / cpu6,
+                          "same_count_lines": glines == olines},
+        "init_sample": {"job": f"{material8} slice {args.slice}, indices [0, 2^24)",
+                        "seconds": sec_i, "positions_per_s": init_rate},
+        "extrapolated_8piece_s": extrap,
+        "extrapolation": "8-piece slice positions / the init sample's rate + the slice's "
+                         "position-side-plies / the 6-piece job's rate (a lower bound: the "
+                         "per-position cost grows with the piece count)",
+    }
+
+
+def run_table(args, tbg, steps, warmup, extras=True, material=None):
+    """A whole table (default KQRvKR) generated end to end (smaller tables
+    resident and generated untimed); returns the bench dict."""
+    form = {"batched": tbg.FORM_BATCHED, "graph": tbg.FORM_GRAPH, "plain": tbg.FORM_PLAIN}[args.form]
+    material = material or args.material or "KQRvKR"
+    reg = tbg.NativeRegistry(tbg.FORM_GRAPH)
+    subs = [reg.get(m) for m in tbg.sub_materials(material)]
+    table = tbg.Table(material)
+    table.set_scatter(scatter_mode(args.scatter, tbg))
+    n_total = tbg.NSLICES * table.size
+    tuned = None
+    if args.autotune:
+        table.resolve(subs)
+        g, tms = table.autotune()
+        tuned = dict(g.as_dict(), trial_ms=tms)
+    for _ in range(warmup):
+        table.resolve(subs)
+        table.generate(form)
+    secs, tm, lines = run_steps(table, subs, form, steps, tbg)
+    mean = sum(secs) / len(secs)
+    half_steps = tm.half_steps / max(1, steps)
+    psp = half_steps * n_total / 2.0  # position-side-plies per step
+    kr = kernel_rooflines(tm, lines, n_total, steps, load_traffic(f"{material}_{args.form}"))
+    dom = max(kr, key=lambda k: kr[k]["ms"])
+    out = {
+        "metric": METRIC, "value": psp / mean, "unit": "position-side-plies/s", "n_gpus": 1,
+        "steps": steps, "warmup": warmup, "ms_per_step": 1000.0 * mean,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{material} table generation, {args.form} form: 462 king slices x "
+                        f"{table.size} positions per side, capture exits from resident smaller "
+                        "tables, init + ply loop to the fixpoint",
+            "material": material, "form": args.form, "slices": 462,
+            "geometry": table.geometry().as_dict(), "autotuned": tuned is not None,
+            "positions_per_side": n_total, "half_steps_per_step": half_steps,
+            "plies": table.counts.nplies,
+        },
+        "roofline": {k: v for k, v in kr[dom].items() if k not in ("ms", "half_steps")},
+        "kernel_rooflines": kr,
+        "kernels_ms_per_step": {k: getattr(tm, "ms_" + k) / steps for k in
+                                ("pred", "verify", "sweep", "uncap", "init", "resolve")},
+    }
+    if extras and not args.no_plain and form != tbg.FORM_PLAIN:
+        t0 = time.perf_counter()
+        table.resolve(subs)
+        plines = table.generate(tbg.FORM_PLAIN)
+        pt = time.perf_counter() - t0
+        out["plain"] = {"ms_per_step": 1000 * pt, "value": psp / pt,
+                        "batched_over_plain": (psp / mean) / (psp / pt),
+                        "same_count_lines": plines == lines}
+    if extras and not args.no_cpu:
+        from oracle import oracle as O
+        threads = cpu_threads()
+        O.set_threads(threads)
+        sec, hs = O.time_plies(material, args.cpu_plies)
+        out["cpu_baseline"] = {
+            "value": hs * n_total / 2.0 / sec, "unit": "position-side-plies/s", "cores": threads,
+            "kind": "port", "host": host_cpu(),
+            "sample": f"CPU oracle (C restatement of the reference algorithm, OpenMP) on plies "
+                      f"1..{args.cpu_plies} of {material}: {hs} half-steps over all 462 "
+                      f"slices in {sec:.2f} s; smaller tables generated untimed",
+        }
+    table.close()
+    reg.close()
+    return out
+
+
 def run_synth(args, tbg):
     """One king slice at real shape with seeded Bernoulli(0.03) win / loss
     exits for captures and king moves; one step = capture exits + init + the
     ply loop to the fixpoint."""
-    material = args.material if args.material != "KQRvKR" else "KQRBvKQRN"
+    material = args.material or "KQRBvKQRN"
     t0 = time.perf_counter()
     table = tbg.Table(material, synth_slice=args.slice, seed=args.seed)
     table.set_scatter(scatter_mode(args.scatter, tbg))
@@ -278,72 +456,66 @@ def run_synth(args, tbg):
     print(f"synth {material} slice {args.slice}: {n} positions per side, "
           f"{table.info.device_bytes / 2**30:.1f} GiB resident, alloc {alloc_s:.1f} s",
           file=sys.stderr, flush=True)
-    secs = []
-    table.set_profiling(True)
-    for step in range(args.warmup + args.steps):
-        t0 = time.perf_counter()
-
-        lines = []
-
-        def progress(line, _u, t0=t0, lines=lines):
-            lines.append(line.decode())
-            print(f"  [{time.perf_counter() - t0:8.2f} s] {line.decode()}", file=sys.stderr,
-                  flush=True)
-
-        fn = tbg.LINE_FN(progress)
-        table.resolve([])
-        form = {"batched": tbg.FORM_BATCHED, "graph": tbg.FORM_GRAPH,
-                "plain": tbg.FORM_PLAIN}[args.form]
-        tbg.check(tbg.lib().tbg_generate(table.ptr, form, fn, None,
-                                         tbg.ctypes.byref(table.counts)), "generate")
-        dt = time.perf_counter() - t0
-        if step == args.warmup - 1:
-            table.set_profiling(False)
-            table.set_profiling(True)
-        if step >= args.warmup:
-            secs.append(dt)
+    form = {"batched": tbg.FORM_BATCHED, "graph": tbg.FORM_GRAPH, "plain": tbg.FORM_PLAIN}[args.form]
+    secs, lines = synth_steps(table, form, args.steps, args.warmup, tbg)
     tm = table.timing()
     steps = max(1, args.steps)
     half_steps = tm.half_steps / steps
     psp = half_steps * n / 2.0
-    kr = kernel_rooflines(tm, lines, n, steps, load_traffic(f"{material}_synth"))
+    mean = sum(secs) / len(secs)
+    kr = kernel_rooflines(tm, lines, n, steps, load_traffic(f"{material}_synth"), init_bits=8)
     dom = max(kr, key=lambda k: kr[k]["ms"])
     roof = {k: v for k, v in kr[dom].items() if k not in ("ms", "half_steps")}
+    table.close()
     out = {
-        "metric": METRIC, "value": psp / (sum(secs) / len(secs)),
+        "metric": METRIC, "value": psp / mean,
         "unit": "position-side-plies/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": 1000 * max(secs), "higher_is_better": True,
+        "warmup": args.warmup, "ms_per_step": 1000 * mean, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "u64", "data": "synthetic",
         "config": {"workload": f"{material} single king slice {args.slice} at real shape, "
                                f"seeded hash exits (seed {args.seed}), full fixpoint",
                    "material": material, "slice": args.slice, "positions_per_side": n,
-                   "half_steps_per_step": half_steps, "plies": table.counts.nplies},
+                   "half_steps_per_step": half_steps, "plies": table.counts.nplies,
+                   "form": args.form},
         "roofline": roof,
         "kernel_rooflines": kr,
         "kernels_ms_per_step": {k: getattr(tm, "ms_" + k) / steps for k in
                                 ("pred", "verify", "sweep", "uncap", "init", "resolve")},
+        "ms_per_ply": 1000 * mean / max(1, table.counts.nplies),
+        "step_s": secs,
     }
+    if not args.no_secondary:
+        sec = run_table(args, tbg, steps=5, warmup=1, extras=False, material="KQRvKR")
+        out["secondary"] = {k: sec[k] for k in ("value", "ms_per_step", "steps", "config",
+                                                 "roofline", "kernels_ms_per_step")}
+        print(f"secondary KQRvKR table: {sec['ms_per_step']:.1f} ms per table, "
+              f"{sec['value']:.3e}/s", file=sys.stderr, flush=True)
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline_synth(args, tbg, material, n, psp)
     print(json.dumps(out), flush=True)
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--material", default="KQRvKR")
+    ap.add_argument("--material", default="",
+                    help="synth: KQRBvKQRN (default); table: KQRvKR (default)")
     ap.add_argument("--form", default="batched", choices=["batched", "graph", "plain"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-form comparison")
+    ap.add_argument("--no-secondary", action="store_true",
+                    help="synth: skip the secondary KQRvKR table line")
     ap.add_argument("--cpu-plies", type=int, default=2)
     ap.add_argument("--scatter", default="block+combine",
                     choices=["block", "global", "block+combine", "global+combine"],
                     help="scatter mode when not autotuned")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="keep the built-in launch geometry (table and stream workloads)")
-    ap.add_argument("--workload", default="table", choices=["table", "synth", "stream", "served"],
-                    help="table: a whole material (default KQRvKR); synth: one king slice of "
-                         "--material with seeded hash exits (default KQRBvKQRN slice 170)")
+    ap.add_argument("--workload", default="synth", choices=["table", "synth", "stream", "served"],
+                    help="synth (default): one king slice of --material with seeded hash "
+                         "exits (KQRBvKQRN slice 170); table: a whole material (KQRvKR)")
     ap.add_argument("--slice", type=int, default=170)
     ap.add_argument("--jobs", type=int, default=96, help="served: number of jobs")
     ap.add_argument("--budget-gb", type=float, default=64.0, help="served: HBM admission budget")
@@ -394,88 +566,7 @@ def main():
         return run_stream(args, tbg)
     if args.workload == "served":
         return run_served(args, tbg)
-    form = {"batched": tbg.FORM_BATCHED, "graph": tbg.FORM_GRAPH, "plain": tbg.FORM_PLAIN}[args.form]
-    reg = tbg.Registry(tbg.FORM_BATCHED)
-    for sub in tbg.sub_materials(args.material):
-        reg.get(sub)
-    subs = list(reg.tables.values())
-    table = tbg.Table(args.material)
-    table.set_scatter(scatter_mode(args.scatter, tbg))
-    n_total = tbg.NSLICES * table.size
-
-    tuned = None
-    if args.autotune:
-        table.resolve(subs)
-        g, tms = table.autotune()
-        tuned = dict(g.as_dict(), trial_ms=tms)
-    for _ in range(args.warmup):
-        table.resolve(subs)
-        table.generate(form)
-    secs, tm, lines = run_steps(table, subs, form, args.steps, tbg)
-    ms_step = 1000.0 * max(secs)
-    half_steps = tm.half_steps / max(1, args.steps)
-    psp = half_steps * n_total / 2.0  # position-side-plies per step
-    value = psp / (sum(secs) / len(secs))
-
-    kr = kernel_rooflines(tm, lines, n_total, args.steps,
-                          load_traffic(f"{args.material}_{args.form}"))
-    dom = max(kr, key=lambda k: kr[k]["ms"])
-    roof = {k: v for k, v in kr[dom].items() if k not in ("ms", "half_steps")}
-
-    out = {
-        "metric": METRIC,
-        "value": value,
-        "unit": "position-side-plies/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "u64",
-        "data": "synthetic",
-        "config": {
-            "workload": f"{args.material} table generation, {args.form} form: 462 king slices x "
-                        f"{table.size} positions per side, capture exits from resident smaller "
-                        "tables, init + ply loop to the fixpoint",
-            "material": args.material, "form": args.form, "slices": 462,
-            "geometry": table.geometry().as_dict(), "autotuned": tuned is not None,
-            "positions_per_side": n_total, "half_steps_per_step": half_steps,
-            "plies": table.counts.nplies,
-        },
-        "roofline": roof,
-        "kernel_rooflines": kr,
-        "kernels_ms_per_step": {
-            "pred": tm.ms_pred / args.steps, "verify": tm.ms_verify / args.steps,
-            "sweep": tm.ms_sweep / args.steps, "uncap": tm.ms_uncap / args.steps,
-            "init": tm.ms_init / args.steps, "resolve": tm.ms_resolve / args.steps,
-            "launches": tm.launches / args.steps,
-        },
-    }
-
-    if not args.no_plain and form != tbg.FORM_PLAIN:
-        t0 = time.perf_counter()
-        table.resolve(subs)
-        plines = table.generate(tbg.FORM_PLAIN)
-        pt = time.perf_counter() - t0
-        out["plain"] = {"ms_per_step": 1000 * pt, "value": psp / pt,
-                        "batched_over_plain": (psp / (sum(secs) / len(secs))) / (psp / pt),
-                        "same_count_lines": plines == lines}
-
-    if not args.no_cpu:
-        from oracle import oracle as O
-        threads = cpu_threads()
-        O.set_threads(threads)
-        sec, hs = O.time_plies(args.material, args.cpu_plies)
-        out["cpu_baseline"] = {
-            "value": hs * n_total / 2.0 / sec, "unit": "position-side-plies/s", "cores": threads,
-            "kind": "port",
-            "sample": f"CPU oracle (C restatement of the reference algorithm, OpenMP) on plies "
-                      f"1..{args.cpu_plies} of {args.material}: {hs} half-steps over all 462 "
-                      f"slices in {sec:.2f} s; smaller tables generated untimed",
-        }
-    print(json.dumps(out), flush=True)
+    print(json.dumps(run_table(args, tbg, args.steps, args.warmup)), flush=True)
 
 
 if __name__ == "__main__":

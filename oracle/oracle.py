@@ -56,6 +56,10 @@ def lib():
                                      ctypes.c_uint64]
         L.tbo_time_plies.argtypes = [ctypes.c_char_p, ctypes.c_int,
                                      ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)]
+        L.tbo_synth_init_time.restype = ctypes.c_double
+        L.tbo_synth_init_time.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_uint64,
+                                          ctypes.c_uint64, ctypes.c_uint64,
+                                          ctypes.POINTER(ctypes.c_int64)]
         L.tbo_synth_check.restype = ctypes.c_int64
         L.tbo_synth_check.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_uint64,
                                       ctypes.POINTER(ctypes.c_void_p), ctypes.c_uint64,
@@ -87,6 +91,10 @@ def set_threads(n):
 
 def set_ref_king_capture_index(on):
     lib().tbo_set_ref_king_capture_index(1 if on else 0)
+
+
+def set_keep_synth_codes(on):
+    lib().tbo_set_keep_synth_codes(1 if on else 0)
 
 
 def free_all():
@@ -232,6 +240,16 @@ def synth_check(material, s, seed, planes, nsamples, sample_seed=1):
     if bad < 0:
         raise RuntimeError(f"synth_check of {material} failed ({bad})")
     return int(bad), chk.value, (tuple(fb) if bad else None)
+
+
+def synth_init_time(material, s, seed, lo, n):
+    """Seconds of the oracle's init (illegal, mates, synthetic exits) over
+    indices [lo, lo + n) of a synthetic slice, and the four counts."""
+    c = (ctypes.c_int64 * 4)()
+    sec = lib().tbo_synth_init_time(material.encode(), int(s), int(seed), int(lo), int(n), c)
+    if sec < 0:
+        raise RuntimeError(f"oracle init sample of {material} failed")
+    return sec, list(c)
 
 
 def time_plies(material, max_plies):

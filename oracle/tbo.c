@@ -44,9 +44,11 @@ enum { KNIGHT = 2, BISHOP = 3, ROOK = 4, QUEEN = 5, KING = 6 }; /* types.h:17 */
 
 static int g_threads = 1;
 static int g_ref_kcap = 0;
+static int g_keep_synth_codes = 1;
 
 void tbo_set_threads(int n) { g_threads = n > 0 ? n : 1; }
 void tbo_set_ref_king_capture_index(int on) { g_ref_kcap = on != 0; }
+void tbo_set_keep_synth_codes(int on) { g_keep_synth_codes = on != 0; }
 
 /* ------------------------------------------------------------------ */
 /* Board geometry (movegen.c:14-73 semantics, own construction).       */
@@ -1163,7 +1165,7 @@ int tbo_synth_slice(const char *material, int s, uint64_t seed, const char *log_
   t->seed = seed;
   t->seedmix = splitmix64(seed);
   snprintf(t->name, sizeof t->name, "#%d", g_ntab);
-  for (int c = 0; c < 2; c++) {
+  for (int c = 0; c < 2 && g_keep_synth_codes; c++) {
     t->code[c] = calloc(t->size, sizeof(uint16_t));
     if (!t->code[c]) { fprintf(stderr, "tbo: out of memory\n"); abort(); }
   }
@@ -1525,6 +1527,79 @@ int64_t tbo_synth_check(const char *material, int s, uint64_t seed, const uint64
   free(sidx);
   if (checked) *checked = nchk;
   return bad;
+}
+
+/* CPU baseline sample of the init stage on a slice too large to generate
+ * here: calc_illegal_and_mate (tb8gen.c:664-768) and the synthetic
+ * king-move exits, position by position over indices [lo, lo + n) of slice
+ * s, into local bitmaps.  Returns the wall seconds; counts[4] = illegal W,
+ * illegal B, mates W, mates B in the range. */
+double tbo_synth_init_time(const char *material, int s, uint64_t seed, uint64_t lo, uint64_t n,
+                           int64_t *counts)
+{
+  init_tables();
+  Table tab;
+  memset(&tab, 0, sizeof tab);
+  Table *t = &tab;
+  if (s < 0 || s >= 441 || setup_material(t, material)) return -1.0;
+  t->synth = 1;
+  t->synth_slice = s;
+  t->seed = seed;
+  t->seedmix = splitmix64(seed);
+  if (lo >= t->size) return -1.0;
+  if (n > t->size - lo) n = t->size - lo;
+  uint64_t nw = (n + 63) / 64;
+  uint64_t *bm = xalloc(8 * nw);
+  int64_t c0 = 0, c1 = 0, c2 = 0, c3 = 0;
+  double t0 = omp_get_wtime();
+#pragma omp parallel for schedule(dynamic, 64) num_threads(g_threads) reduction(+ : c0, c1, c2, c3)
+  for (uint64_t w = 0; w < nw; w++) {
+    uint8_t sq[MAX_PIECES] = { 0 }, sq2[MAX_PIECES];
+    uint64_t out[8] = { 0 };
+    for (int bt = 0; bt < 64; bt++) {
+      uint64_t i = w * 64 + (uint64_t)bt;
+      if (i >= n) break;
+      sq[0] = KKSq[s][0];
+      sq[1] = KKSq[s][1];
+      bb_t occ = idx_to_sq(&t->ii, lo + i, sq);
+      bool ill[2];
+      for (int c = 0; c < 2; c++) ill[c] = opp_king_attacked(t->pt, sq, t->num, occ, c);
+      if (ill[0]) { out[0] |= 1ULL << bt; c0++; }
+      if (ill[1]) { out[1] |= 1ULL << bt; c1++; }
+      if (ill[0] != ill[1]) {
+        int c = ill[1] ? WHITE : BLACK;
+        if (!has_legal_move_or_cap(t->pt, sq, t->num, occ, c)) {
+          out[2 + c] |= 1ULL << bt;
+          if (c) c3++; else c2++;
+        }
+      }
+      /* the exits of synth_king_exits */
+      for (int stm = 0; stm < 2; stm++) {
+        int o = stm ^ 1, from = sq[stm];
+        bb_t b = KingAtt[from] & ~KingAtt[sq[o]];
+        while (b) {
+          int to = pop_lsb(&b), j = -1;
+          if (occ & bit(to)) {
+            j = 0;
+            while (sq[j] != to) j++;
+            if ((t->pt[j] >> 3) == stm) continue;
+          }
+          sq[stm] = (uint8_t)to;
+          normalize(sq, sq2);
+          int s2 = KKMap[sq2[0]][sq2[1]];
+          int v = s2 == s ? 3 : synth_king_exit(t, stm, sq, occ, from, j, sq2, s2);
+          sq[stm] = (uint8_t)from;
+          if (v == -2) out[4 + stm] |= 1ULL << bt;
+          if (v == 2) out[6 + stm] |= 1ULL << bt;
+        }
+      }
+    }
+    for (int k = 0; k < 8; k++) bm[k * nw + w] = out[k];
+  }
+  double sec = omp_get_wtime() - t0;
+  free(bm);
+  if (counts) { counts[0] = c0; counts[1] = c1; counts[2] = c2; counts[3] = c3; }
+  return sec;
 }
 
 int tbo_get_threads(void) { return g_threads; }

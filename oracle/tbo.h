@@ -37,6 +37,8 @@ void tbo_set_threads(int nthreads);
  * sets (check_king_moves, tb8gen.c:543-555, which omits the fix-up that
  * check_moves applies at tb8gen.c:586-588).  0 (default): correct index. */
 void tbo_set_ref_king_capture_index(int on);
+/* 0: synthetic slices keep no codes (only their count lines); default 1. */
+void tbo_set_keep_synth_codes(int on);
 
 /* Generate a material (e.g. "KQRvKR") and, recursively, every smaller
  * material its captures reach.  Count lines in the reference's format
@@ -116,6 +118,10 @@ int64_t tbo_synth_check(const char *material, int s, uint64_t seed, const uint64
                         uint64_t nsamples, uint64_t sample_seed, uint64_t *checked,
                         int64_t *first_bad);
 
+/* CPU baseline sample of init (illegal, mates, synthetic exits) over
+ * indices [lo, lo + n) of a synthetic slice; returns wall seconds. */
+double tbo_synth_init_time(const char *material, int s, uint64_t seed, uint64_t lo, uint64_t n,
+                           int64_t *counts);
 /* CPU baseline sample (see tbo.c). */
 int tbo_time_plies(const char *material, int max_plies, double *loop_seconds, int64_t *half_steps);
 int tbo_get_threads(void);
