@@ -755,11 +755,22 @@ int run_init_async(tbg_ctx *c)
   HIPCHK(hipMemsetAsync(c->dcnt, 0, 4 * sizeof(unsigned long long), c->st));
   {
     Timed t(c, CAT_INIT);
-    uint64_t chunks = ((uint64_t)ns * M.words + 4 * INIT_CHUNK - 1) / (4 * INIT_CHUNK);
-    hipLaunchKernelGGL(k_init, dim3(cap_grid(chunks, (uint64_t)g_cus * 16)),
-                       dim3(256), 0, c->st, c->dM, c->wins[0], c->wins[1], c->prevL[0], c->prevL[1],
-                       c->codes[0], c->codes[1], c->prevW[0], c->prevW[1], c->newW[0], c->newW[1],
-                       s0, ns, c->dcnt);
+    // all sets single and at least 64 placements (runs of the last set's
+    // squares) per slice: one placement per lane
+    uint64_t R = 62 - (uint64_t)(M.ii.numsets - 1);
+    if (M.all_single && M.ii.numsets >= 2 && M.ii.size / R >= 64) {
+      uint64_t chunks = (uint64_t)ns * ((M.words + R - 1) / R);
+      hipLaunchKernelGGL(k_init_pl, dim3(cap_grid((chunks + 3) / 4, (uint64_t)g_cus * 16)),
+                         dim3(256), 0, c->st, c->dM, c->wins[0], c->wins[1], c->prevL[0],
+                         c->prevL[1], c->codes[0], c->codes[1], c->prevW[0], c->prevW[1],
+                         c->newW[0], c->newW[1], s0, ns, c->dcnt);
+    } else {
+      uint64_t chunks = ((uint64_t)ns * M.words + 4 * INIT_CHUNK - 1) / (4 * INIT_CHUNK);
+      hipLaunchKernelGGL(k_init, dim3(cap_grid(chunks, (uint64_t)g_cus * 16)),
+                         dim3(256), 0, c->st, c->dM, c->wins[0], c->wins[1], c->prevL[0],
+                         c->prevL[1], c->codes[0], c->codes[1], c->prevW[0], c->prevW[1],
+                         c->newW[0], c->newW[1], s0, ns, c->dcnt);
+    }
     HIPCHK(hipGetLastError());
   }
   c->counts.nplies = 0;

@@ -120,6 +120,12 @@ __global__ void k_init(const Mat *Mp, uint64_t *winsW, uint64_t *winsB, uint64_t
                        uint64_t *mateB, uint16_t *codesW, uint16_t *codesB, uint64_t *kexW0,
                        uint64_t *kexW1, uint64_t *kexL0, uint64_t *kexL1, int s0, int ns,
                        unsigned long long *cnt);
+// the same outputs for an all-single material of at least 64 placements
+// per slice, one placement (every set's square but the last) per lane
+__global__ void k_init_pl(const Mat *Mp, uint64_t *winsW, uint64_t *winsB, uint64_t *mateW,
+                          uint64_t *mateB, uint16_t *codesW, uint16_t *codesB, uint64_t *kexW0,
+                          uint64_t *kexW1, uint64_t *kexL0, uint64_t *kexL1, int s0, int ns,
+                          unsigned long long *cnt);
 __global__ void k_resolve(const Mat *Mp, const SubRef *Rp, int c, int k, uint64_t *loss,
                           uint64_t *bloss, uint64_t *cwin, uint64_t *win, int s0, int ns);
 

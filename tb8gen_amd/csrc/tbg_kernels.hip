@@ -1397,6 +1397,319 @@ __global__ __launch_bounds__(256, 6) void k_init(const Mat *__restrict__ Mp, uin
   }
 }
 
+// ------------------------------------------------- init by placements
+// For an all-single material, the positions that share the kings and the
+// pieces of every set but the last differ only in the last set's square t,
+// and form a run of R = 62 - L consecutive indices (L = the last set; its
+// digit is the rank of t among the R squares the others leave free).  The
+// placement-wise init evaluates such a run (a "placement") per lane:
+// legality and king safety become 64-square masks over t, computed once per
+// placement, and each synthetic exit's index is a per-placement base plus
+// the rank of the moved position's last square.
+
+// squares strictly between a and b on a shared rank, file or diagonal (0
+// when a and b share none)
+__device__ __forceinline__ uint64_t between_sq(int a, int b)
+{
+  int dr = (b >> 3) - (a >> 3), df = (b & 7) - (a & 7);
+  if (dr == 0 || df == 0) return rook_att(a, bit(b)) & rook_att(b, bit(a));
+  if (dr == df || dr == -df) return bishop_att(a, bit(b)) & bishop_att(b, bit(a));
+  return 0;
+}
+
+// The squares t of the last set's piece (slot jx) at which square y is
+// attacked by colour c's other non-king pieces over occupancy occ (which
+// leaves out slot jx): a piece at t blocks a slider's ray onto y exactly
+// when t lies strictly between them.  All squares if a knight attacks y,
+// none if nothing does.
+__device__ __forceinline__ uint64_t fixed_attacks_on(const Tabs &T, const Mat &M, uint64_t p,
+                                                     uint64_t occ, int y, int c, int jx)
+{
+  uint64_t inter = ~0ull;
+  bool any = false;
+  for (int i = 0; i < MAXP - 1; i++) {
+    if (i >= M.nk[c]) break;
+    int j = M.pcs[c][i];
+    if (j == jx) continue;
+    int s = sq_get(p, j);
+    if (!(piece_att(T, M.pt[j], s, occ) & bit(y))) continue;
+    any = true;
+    inter &= (M.pt[j] & 7) == KNIGHT ? 0ull : between_sq(s, y);
+  }
+  return any ? ~inter : 0ull;
+}
+
+// The squares t from which the last set's piece (type pt, colour c) would
+// attack y, the others standing on occ: reverse attacks from y, y excluded.
+__device__ __forceinline__ uint64_t last_attacks_on(const Tabs &T, int pt, int y, uint64_t occ)
+{
+  return piece_att(T, pt, y, occ) & ~bit(y);
+}
+
+// Is square y attacked by colour o, as a mask over t: o's other pieces
+// (blockable at t) and, if the last piece is o's, that piece from t.
+// occ: the occupancy without the last piece (and without a king lifted by
+// the caller).
+__device__ __forceinline__ uint64_t attacked_mask(const Tabs &T, const Mat &M, uint64_t p,
+                                                  uint64_t occ, int y, int o, int jx, int cx,
+                                                  int ptx)
+{
+  uint64_t a = fixed_attacks_on(T, M, p, occ, y, o, jx);
+  if (cx == o) a |= last_attacks_on(T, ptx, y, occ);
+  return a;
+}
+
+// sq_to_idx over the sets before set L of packed squares q (set `skip`
+// left out, -1: none) with the factors of ii; *ob receives the occupancy
+// of the kings and those sets, against which the last set's square is
+// ranked
+__device__ __forceinline__ uint64_t partial_idx(const Idx &ii, uint64_t q, int L, int skip,
+                                                uint64_t *ob)
+{
+  uint64_t occ = bit(sq_get(q, 0)) | bit(sq_get(q, 1));
+  uint64_t idx = 0;
+  for (int k = 0; k < MAXS; k++) {
+    if (k >= L) break;
+    if (k == skip) continue;
+    int s = sq_get(q, 2 + k);
+    idx += (uint64_t)(uint32_t)(s - popc64(occ & (bit(s) - 1))) * ii.factor[k];
+    occ |= bit(s);
+  }
+  *ob = occ;
+  return idx;
+}
+
+// the square t moved by a king exit's symmetry (mirror mask m, then the
+// a1-h8 flip)
+__device__ __forceinline__ int sym_sq(int t, int m, bool fl)
+{
+  int u = t ^ m;
+  return fl ? ((u & 7) << 3) | (u >> 3) : u;
+}
+
+constexpr int IB_PLANES = 8;  // wins W/B, mates W/B, exit W candidates W/B, exit L candidates W/B
+constexpr int IB_WORDS = 64;  // words per wave of one plane (R <= 62 of them used)
+
+// calc_illegal_worker + calc_mate_worker (src/tb8gen.c:664-727) and the
+// synthetic exits, one placement per lane, for an all-single material with
+// at least 64 placements per slice.  A wave takes 64 consecutive placements
+// = 64 R positions = R bitmap words of each plane (a "chunk"), assembles
+// them in LDS and stores each word once.  Same outputs as k_init.
+__global__ __launch_bounds__(256) void k_init_pl(const Mat *__restrict__ Mp, uint64_t *winsW,
+                                                 uint64_t *winsB, uint64_t *mateW, uint64_t *mateB,
+                                                 uint16_t *codesW, uint16_t *codesB,
+                                                 uint64_t *kexW0, uint64_t *kexW1,
+                                                 uint64_t *kexL0, uint64_t *kexL1, int s0, int ns,
+                                                 unsigned long long *cnt)
+{
+  __shared__ Tabs T;
+  __shared__ Mat Ms;
+  __shared__ uint64_t buf[4][IB_PLANES][IB_WORDS];
+  load_tabs(T);
+  {
+    const uint32_t *a = reinterpret_cast<const uint32_t *>(Mp);
+    uint32_t *b = reinterpret_cast<uint32_t *>(&Ms);
+    for (int i = threadIdx.x; i < (int)(sizeof(Mat) / 4); i += blockDim.x) b[i] = a[i];
+    __syncthreads();
+  }
+  const Mat &M = Ms;
+  int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  uint64_t(*B)[IB_WORDS] = buf[wave];
+  const int L = M.ii.numsets - 1, jx = 2 + L, R = 62 - L;
+  const int cx = (M.pt[jx] >> 3) & 1, ptx = M.pt[jx];
+  const uint64_t np = M.ii.size / (uint64_t)R;    // placements per slice
+  const uint64_t cps = (M.words + R - 1) / R;     // chunks per slice (padding words included)
+  uint64_t cw[4] = { 0, 0, 0, 0 };                // counts: illegal W, B, mates W, B
+  for (uint64_t gc = (uint64_t)blockIdx.x * 4 + wave; gc < (uint64_t)ns * cps;
+       gc += (uint64_t)gridDim.x * 4) {
+    int s = s0 + (int)(gc / cps);
+    uint64_t g = gc % cps;
+    uint64_t P = g * 64 + lane;
+    bool valid = P < np;
+    // this lane's placement: the squares of every set but the last
+    uint64_t p = kings_of(T, s);
+    uint64_t occF = 0;
+    if (valid) occF = idx_to_sq_1(M.ii, P * (uint64_t)R, &p) & ~bit(sq_get(p, jx));
+    else occF = bit(sq_get(p, 0)) | bit(sq_get(p, 1));
+    uint64_t F = ~occF;  // the R squares of the last piece
+    int wk = sq_get(p, WHITE), bk = sq_get(p, BLACK);
+    // illegal(c): c to move can take the other king
+    uint64_t WI = attacked_mask(T, M, p, occF, bk, WHITE, jx, cx, ptx);
+    uint64_t BI = attacked_mask(T, M, p, occF, wk, BLACK, jx, cx, ptx);
+    if (T.king[wk] & bit(bk)) WI = BI = ~0ull;
+    // king moves of each side that stay legal (own piece not on the
+    // target, target not attacked with the king lifted), as masks over t,
+    // OR-ed into the escapes for the mate test
+    uint64_t esc[2];
+    for (int c = 0; c < 2; c++) {
+      int o = c ^ 1, k = sq_get(p, c);
+      uint64_t own = 0;
+      for (int i = 0; i < MAXP - 1; i++) {
+        if (i >= M.nk[c]) break;
+        int j = M.pcs[c][i];
+        if (j != jx) own |= bit(sq_get(p, j));
+      }
+      uint64_t ys = T.king[k] & ~own & ~T.king[sq_get(p, o)];
+      uint64_t e = 0;
+      while (ys) {
+        int y = ctz64(ys);
+        ys &= ys - 1;
+        uint64_t ok = ~attacked_mask(T, M, p, occF ^ bit(k), y, o, jx, cx, ptx);
+        if (cx == c) ok &= ~bit(y);
+        e |= ok;
+      }
+      esc[c] = e;
+    }
+    // mates: in check, not illegal, no king move; the rest of the moves
+    // only for those t (rare), by the full test
+    uint64_t MW = 0, MB = 0;
+    if (valid) {
+      for (int c = 0; c < 2; c++) {
+        uint64_t cand = (c == WHITE ? BI & ~WI : WI & ~BI) & ~esc[c] & F;
+        int k = sq_get(p, c), o = c ^ 1;
+        while (cand) {
+          int t = ctz64(cand);
+          cand &= cand - 1;
+          uint64_t pt_ = sq_set(p, jx, t), occ = occF | bit(t);
+          uint64_t uo = side_attacks(T, M, pt_, occ ^ bit(k), o, -1);
+          if (!has_legal(T, M, pt_, occ, c, uo)) {
+            if (c == WHITE) MW |= bit(t);
+            else MB |= bit(t);
+          }
+        }
+      }
+    }
+    // rank-domain masks (bit r = position P R + r)
+    uint64_t rw = 0, rb = 0, rmw = 0, rmb = 0;
+    {
+      uint64_t f = F;
+      for (int r = 0; r < R; r++) {
+        int t = ctz64(f);
+        f &= f - 1;
+        rw |= ((WI >> t) & 1) << r;
+        rb |= ((BI >> t) & 1) << r;
+        rmw |= ((MW >> t) & 1) << r;
+        rmb |= ((MB >> t) & 1) << r;
+      }
+    }
+    if (!valid) rw = rb = rmw = rmb = 0;
+    // synthetic exits: each of the king's moves out of the slice, its
+    // index a per-placement base plus the rank of the last square
+    uint64_t rx[4] = { 0, 0, 0, 0 };  // W candidates W, B; L candidates W, B
+    if (M.synth && valid) {
+      for (int c = 0; c < 2; c++) {
+        int o = c ^ 1, k = sq_get(p, c);
+        uint64_t ill = c == WHITE ? WI : BI;
+        uint64_t own = 0;
+        for (int i = 0; i < MAXP - 1; i++) {
+          if (i >= M.nk[c]) break;
+          int j = M.pcs[c][i];
+          if (j != jx) own |= bit(sq_get(p, j));
+        }
+        for (int e = 0; e < 8; e++) {
+          if (e >= M.nexit[c]) break;
+          int to = M.exit_to[c][e], m = M.exit_m[c][e], s2 = M.exit_s2[c][e];
+          bool fl = M.exit_fl[c][e] != 0;
+          if (own & bit(to)) continue;  // every t: an own piece stands there
+          uint64_t ok = ~attacked_mask(T, M, p, occF ^ bit(k), to, o, jx, cx, ptx) & ~ill;
+          if (cx == c) ok &= ~bit(to);
+          uint64_t q0 = sq_set(p, c, to);
+          uint64_t q = q0 ^ (0x0101010101010101ull * (uint64_t)m);
+          if (fl) q = flip_diag_bytes(q);
+          uint64_t ob, base;
+          int kind, region;
+          if (occF & bit(to)) {  // a capture of one of o's other pieces
+            int jj = slot_of(p, to, M.col80[o]);
+            int l = sq_get(M.set_of, jj);
+            base = partial_idx(M.capt[l], q, L, l, &ob);
+            kind = 2;
+            region = s2 * 8 + l;
+          } else {
+            base = partial_idx(M.ii, q, L, -1, &ob);
+            kind = 1;
+            region = s2;
+          }
+          // the last piece taken on `to` itself (it is o's): its own set's
+          // capture space, the same index for that one t
+          uint64_t xcap = 0;
+          bool xt = cx == o && (F & bit(to));
+          if (xt) {
+            uint64_t ob2;
+            xcap = partial_idx(M.capt[L], q, L, -1, &ob2);
+          }
+          uint64_t f = F;
+          for (int r = 0; r < R; r++) {
+            int t = ctz64(f);
+            f &= f - 1;
+            if (!((ok >> t) & 1)) continue;
+            uint64_t h;
+            if (xt && t == to) {
+              h = synth_hash(M.seedmix, o, 2, s2 * 8 + L, xcap);
+            } else {
+              int u = sym_sq(t, m, fl);
+              uint64_t idx = base + (uint64_t)(uint32_t)(u - popc64(ob & (bit(u) - 1)));
+              h = synth_hash(M.seedmix, o, kind, region, idx);
+            }
+            int v = synth_value(h);
+            rx[c] |= (uint64_t)(v == -2) << r;
+            rx[2 + c] |= (uint64_t)(v == 2) << r;
+          }
+        }
+      }
+    }
+    // assemble the chunk's words in LDS (a lane's R bits start at bit
+    // lane * R) and store each word of each plane once
+    for (int i = lane; i < IB_PLANES * IB_WORDS; i += 64) B[i / IB_WORDS][i % IB_WORDS] = 0;
+    wave_sync();
+    uint64_t rm[IB_PLANES] = { rw, rb, rmw, rmb, rx[0], rx[1], rx[2], rx[3] };
+    int off = lane * R, w0 = off >> 6, sh = off & 63;
+#pragma unroll
+    for (int k = 0; k < IB_PLANES; k++) {
+      uint64_t v = rm[k];
+      if (!v) continue;
+      atomicOr(reinterpret_cast<unsigned long long *>(&B[k][w0]), (unsigned long long)(v << sh));
+      if (sh && sh + R > 64)
+        atomicOr(reinterpret_cast<unsigned long long *>(&B[k][w0 + 1]),
+                 (unsigned long long)(v >> (64 - sh)));
+    }
+    wave_sync();
+    uint64_t wbase = phys(M, s) * M.words + g * (uint64_t)R;
+    uint64_t nwc = M.words - g * (uint64_t)R < (uint64_t)R ? M.words - g * (uint64_t)R : (uint64_t)R;
+    if ((uint64_t)lane < nwc) {
+      winsW[wbase + lane] = B[0][lane];
+      winsB[wbase + lane] = B[1][lane];
+      mateW[wbase + lane] = B[2][lane];
+      mateB[wbase + lane] = B[3][lane];
+      if (M.synth) {
+        kexW0[wbase + lane] = B[4][lane];
+        kexW1[wbase + lane] = B[5][lane];
+        kexL0[wbase + lane] = B[6][lane];
+        kexL1[wbase + lane] = B[7][lane];
+      }
+    }
+    // codes, 64 consecutive positions per store
+    if (codesW) {
+      uint64_t cbase = phys(M, s) * M.ii.size + g * 64 * (uint64_t)R;
+      for (int k = 0; k < R; k++) {
+        uint64_t i = cbase + (uint64_t)k * 64 + lane;
+        if (g * 64 * (uint64_t)R + (uint64_t)k * 64 + lane >= M.ii.size) break;
+        uint64_t a = B[0][k], b = B[1][k], ma = B[2][k], mb = B[3][k];
+        codesW[i] = ((a >> lane) & 1) ? 3 : ((ma >> lane) & 1) ? 2 : 0;
+        codesB[i] = ((b >> lane) & 1) ? 3 : ((mb >> lane) & 1) ? 2 : 0;
+      }
+    }
+    wave_sync();  // the next chunk clears B
+    cw[0] += popc64(rw);
+    cw[1] += popc64(rb);
+    cw[2] += popc64(rmw);
+    cw[3] += popc64(rmb);
+  }
+  for (int k = 0; k < 4; k++) {
+    uint64_t v = __ballot(cw[k] != 0) ? wave_sum64(cw[k]) : 0;
+    if (lane == 0 && v) atomicAdd(cnt + k, (unsigned long long)v);
+  }
+}
+
 // calc_sub_worker (src/tb8gen.c:280-314) for set k of colour c: classify
 // every position with one piece of set k removed, c to move, by the
 // smaller material's finished table (probe_wdl, src/probe.c:1677-1684).
