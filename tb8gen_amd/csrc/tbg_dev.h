@@ -592,6 +592,33 @@ constexpr uint64_t SYN_T_WIN = 553402322211286548ull;    // 0.03 * 2^64
 constexpr uint64_t SYN_T_LOSS = 1106804644422573097ull;  // 0.06 * 2^64
 TBG_HD int synth_value(uint64_t h) { return h < SYN_T_WIN ? 2 : h < SYN_T_LOSS ? -2 : 0; }
 
+// synth_hash's key without the index, XOR the premixed seed: the hash of
+// index idx (below 2^36) is splitmix64(idx ^ key)
+TBG_HD uint64_t synth_key(uint64_t seedmix, int stm, int kind, int region)
+{
+  return (((uint64_t)(region & 0xfff) << 36) | ((uint64_t)(kind & 3) << 48) |
+          ((uint64_t)(stm & 1) << 50)) ^ seedmix;
+}
+// synth_value(splitmix64(idx ^ key)) from the high half of the hash: the
+// last multiply's low half is needed only when the high half ties a
+// threshold's
+TBG_HD int synth_value_k(uint64_t key, uint64_t idx)
+{
+  uint64_t z = (idx ^ key) + 0x9e3779b97f4a7c15ull;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z ^= z >> 27;
+  uint32_t zl = (uint32_t)z, zh = (uint32_t)(z >> 32);
+  uint32_t ph = (uint32_t)(((uint64_t)zl * 0x133111ebu) >> 32) + zl * 0x94d049bbu + zh * 0x133111ebu;
+  uint32_t hh = ph ^ (ph >> 31);
+  constexpr uint32_t TW = (uint32_t)(SYN_T_WIN >> 32), TL = (uint32_t)(SYN_T_LOSS >> 32);
+  int v = hh < TW ? 2 : hh < TL ? -2 : 0;
+  if (hh == TW || hh == TL) {  // about one hash in 2^31
+    uint64_t h = z * 0x94d049bb133111ebull;
+    v = synth_value(h ^ (h >> 31));
+  }
+  return v;
+}
+
 // code <-> WDL (tb8gen.c:300-312 classes; W_n with n <= 100 is a win)
 TBG_HD int code_wdl(uint16_t c)
 {

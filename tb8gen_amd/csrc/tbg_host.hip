@@ -2279,6 +2279,41 @@ extern "C" int tbg_host_selftest(char *msg, int len)
       if (!(occ >> q & 1) && seen++ == r) { s = q; break; }
     if (select_zero(occ, r) != s) ST_FAIL("select_zero");
   }
+  // synth_value_k against synth_value(synth_hash(..)): random keys and
+  // indices, and hashes whose high half ties a threshold's (preimages by
+  // inverting splitmix64's steps)
+  {
+    auto inv_xs = [](uint64_t y, int sh) {
+      uint64_t x = y;
+      for (int i = 0; i < 64 / sh + 1; i++) x = y ^ (x >> sh);
+      return x;
+    };
+    auto inv_mul = [](uint64_t a) {
+      uint64_t x = a;
+      for (int i = 0; i < 6; i++) x *= 2 - a * x;
+      return x;
+    };
+    for (int it = 0; it < 400000; it++) {
+      uint64_t seedmix = splitmix64(rnd());
+      int stm = (int)(rnd() & 1), kind = (int)(rnd() % 3), region = (int)(rnd() % 4096);
+      uint64_t key = synth_key(seedmix, stm, kind, region), idx;
+      if (it & 1) {
+        idx = rnd() & 0xfffffffffull;
+      } else {  // an index whose hash ties a threshold in its high half
+        uint64_t hi = (it & 2) ? (SYN_T_WIN >> 32) : (SYN_T_LOSS >> 32);
+        uint64_t h = hi << 32 | (rnd() >> 32);
+        uint64_t z = inv_xs(h, 31) * inv_mul(0x94d049bb133111ebull);
+        z = inv_xs(z, 27) * inv_mul(0xbf58476d1ce4e5b9ull);
+        uint64_t x = inv_xs(z, 30) - 0x9e3779b97f4a7c15ull;
+        if (splitmix64(x) != h) ST_FAIL("splitmix64 preimage");
+        idx = x ^ key;  // any 64-bit value: the function takes idx ^ key
+      }
+      int want = synth_value(splitmix64(idx ^ key));
+      if ((idx >> 36) == 0 && want != synth_value(synth_hash(seedmix, stm, kind, region, idx)))
+        ST_FAIL("synth_key");
+      if (synth_value_k(key, idx) != want) ST_FAIL("synth_value_k");
+    }
+  }
   for (int it = 0; it < 200000; it++) {
     uint64_t d = 1 + rnd() % 0xffffffffull, n = rnd() % (1ull << 40), rem;
     uint64_t q = div_fix(n, d, 1.0 / (double)d, &rem);

@@ -1495,7 +1495,7 @@ constexpr int IB_WORDS = 64;  // words per wave of one plane (R <= 62 of them us
 // at least 64 placements per slice.  A wave takes 64 consecutive placements
 // = 64 R positions = R bitmap words of each plane (a "chunk"), assembles
 // them in LDS and stores each word once.  Same outputs as k_init.
-__global__ __launch_bounds__(256) void k_init_pl(const Mat *__restrict__ Mp, uint64_t *winsW,
+__global__ __launch_bounds__(256, 5) void k_init_pl(const Mat *__restrict__ Mp, uint64_t *winsW,
                                                  uint64_t *winsB, uint64_t *mateW, uint64_t *mateB,
                                                  uint16_t *codesW, uint16_t *codesB,
                                                  uint64_t *kexW0, uint64_t *kexW1,
@@ -1560,42 +1560,50 @@ __global__ __launch_bounds__(256) void k_init_pl(const Mat *__restrict__ Mp, uin
       }
       esc[c] = e;
     }
-    // mates: in check, not illegal, no king move; the rest of the moves
-    // only for those t (rare), by the full test
+    // mates: in check, not illegal, no king move.  The other moves are
+    // tried only for those t, by the full test, with the wave's candidates
+    // queued in LDS (the wave's assembly buffer, not yet in use) so that
+    // every lane tests one: a lane's own candidates number 0 to R.
     uint64_t MW = 0, MB = 0;
-    if (valid) {
+    {
+      uint8_t *wb = reinterpret_cast<uint8_t *>(B);
+      uint16_t *qe = reinterpret_cast<uint16_t *>(wb);          // QCAP entries: lane << 6 | t
+      uint64_t *qp = reinterpret_cast<uint64_t *>(wb + 2 * QCAP);  // each lane's placement
+      uint64_t *qm = qp + 64;                                    // mates found, [side][lane]
+      static_assert(2 * QCAP + 3 * 64 * 8 <= IB_PLANES * IB_WORDS * 8, "mate queue fits");
+      qp[lane] = p;
+      qm[lane] = 0;
+      qm[64 + lane] = 0;
       for (int c = 0; c < 2; c++) {
-        uint64_t cand = (c == WHITE ? BI & ~WI : WI & ~BI) & ~esc[c] & F;
-        int k = sq_get(p, c), o = c ^ 1;
-        while (cand) {
-          int t = ctz64(cand);
-          cand &= cand - 1;
-          uint64_t pt_ = sq_set(p, jx, t), occ = occF | bit(t);
-          uint64_t uo = side_attacks(T, M, pt_, occ ^ bit(k), o, -1);
-          if (!has_legal(T, M, pt_, occ, c, uo)) {
-            if (c == WHITE) MW |= bit(t);
-            else MB |= bit(t);
+        uint64_t cand = valid ? (c == WHITE ? BI & ~WI : WI & ~BI) & ~esc[c] & F : 0;
+        int tot;
+        int off = wave_excl_scan(popc64(cand), lane, &tot);
+        for (int base = 0; base < tot; base += QCAP) {
+          fill_round(qe, cand, off, base, lane);
+          wave_sync();
+          int cntq = tot - base < QCAP ? tot - base : QCAP;
+          for (int e = lane; e < cntq; e += 64) {
+            uint16_t v = qe[e];
+            int owner = v >> 6, t = v & 63;
+            uint64_t pt_ = sq_set(qp[owner], jx, t), occ = 0;
+            for (int i = 0; i < MAXP; i++)
+              if (i < M.num) occ |= bit(sq_get(pt_, i));
+            // no king move is legal here: only the other moves are tried
+            if (!has_legal(T, M, pt_, occ, c, ~0ull))
+              __hip_atomic_fetch_or(&qm[c * 64 + owner], bit(t), __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_WORKGROUP);
           }
+          wave_sync();
         }
       }
+      MW = qm[lane];
+      MB = qm[64 + lane];
+      wave_sync();  // the buffer is cleared next
     }
-    // rank-domain masks (bit r = position P R + r)
-    uint64_t rw = 0, rb = 0, rmw = 0, rmb = 0;
-    {
-      uint64_t f = F;
-      for (int r = 0; r < R; r++) {
-        int t = ctz64(f);
-        f &= f - 1;
-        rw |= ((WI >> t) & 1) << r;
-        rb |= ((BI >> t) & 1) << r;
-        rmw |= ((MW >> t) & 1) << r;
-        rmb |= ((MB >> t) & 1) << r;
-      }
-    }
-    if (!valid) rw = rb = rmw = rmb = 0;
     // synthetic exits: each of the king's moves out of the slice, its
-    // index a per-placement base plus the rank of the last square
-    uint64_t rx[4] = { 0, 0, 0, 0 };  // W candidates W, B; L candidates W, B
+    // index a per-placement base plus the rank of the moved position's last
+    // square; the candidates collect over squares t
+    uint64_t xs[4] = { 0, 0, 0, 0 };  // W candidates W, B; L candidates W, B
     if (M.synth && valid) {
       for (int c = 0; c < 2; c++) {
         int o = c ^ 1, k = sq_get(p, c);
@@ -1611,51 +1619,89 @@ __global__ __launch_bounds__(256) void k_init_pl(const Mat *__restrict__ Mp, uin
           int to = M.exit_to[c][e], m = M.exit_m[c][e], s2 = M.exit_s2[c][e];
           bool fl = M.exit_fl[c][e] != 0;
           if (own & bit(to)) continue;  // every t: an own piece stands there
-          uint64_t ok = ~attacked_mask(T, M, p, occF ^ bit(k), to, o, jx, cx, ptx) & ~ill;
-          if (cx == c) ok &= ~bit(to);
+          uint64_t go = ~attacked_mask(T, M, p, occF ^ bit(k), to, o, jx, cx, ptx) & ~ill & F;
+          uint64_t xw = 0, xl = 0;  // this exit's W and L candidates
+          if (cx == c) go &= ~bit(to);
           uint64_t q0 = sq_set(p, c, to);
           uint64_t q = q0 ^ (0x0101010101010101ull * (uint64_t)m);
           if (fl) q = flip_diag_bytes(q);
-          uint64_t ob, base;
-          int kind, region;
+          uint64_t ob, base, key;
           if (occF & bit(to)) {  // a capture of one of o's other pieces
             int jj = slot_of(p, to, M.col80[o]);
             int l = sq_get(M.set_of, jj);
             base = partial_idx(M.capt[l], q, L, l, &ob);
-            kind = 2;
-            region = s2 * 8 + l;
+            key = synth_key(M.seedmix, o, 2, s2 * 8 + l);
           } else {
             base = partial_idx(M.ii, q, L, -1, &ob);
-            kind = 1;
-            region = s2;
-          }
-          // the last piece taken on `to` itself (it is o's): its own set's
-          // capture space, the same index for that one t
-          uint64_t xcap = 0;
-          bool xt = cx == o && (F & bit(to));
-          if (xt) {
-            uint64_t ob2;
-            xcap = partial_idx(M.capt[L], q, L, -1, &ob2);
-          }
-          uint64_t f = F;
-          for (int r = 0; r < R; r++) {
-            int t = ctz64(f);
-            f &= f - 1;
-            if (!((ok >> t) & 1)) continue;
-            uint64_t h;
-            if (xt && t == to) {
-              h = synth_hash(M.seedmix, o, 2, s2 * 8 + L, xcap);
-            } else {
-              int u = sym_sq(t, m, fl);
-              uint64_t idx = base + (uint64_t)(uint32_t)(u - popc64(ob & (bit(u) - 1)));
-              h = synth_hash(M.seedmix, o, kind, region, idx);
+            key = synth_key(M.seedmix, o, 1, s2);
+            if (cx == o && (go & bit(to))) {
+              // the last piece (o's) taken on `to` itself: its set's
+              // capture space, one index
+              uint64_t ob2;
+              uint64_t ci = partial_idx(M.capt[L], q, L, -1, &ob2);
+              int v = synth_value_k(synth_key(M.seedmix, o, 2, s2 * 8 + L), ci);
+              xw |= (uint64_t)(v == -2) << to;
+              xl |= (uint64_t)(v == 2) << to;
+              go &= ~bit(to);
             }
-            int v = synth_value(h);
-            rx[c] |= (uint64_t)(v == -2) << r;
-            rx[2 + c] |= (uint64_t)(v == 2) << r;
+          }
+          while (go) {
+            int t = ctz64(go);
+            go &= go - 1;
+            int u = sym_sq(t, m, fl);
+            uint64_t idx = base + (uint64_t)(uint32_t)(u - popc64(ob & (bit(u) - 1)));
+            int v = synth_value_k(key, idx);
+            xw |= (uint64_t)(v == -2) << t;
+            xl |= (uint64_t)(v == 2) << t;
+          }
+          if (c == WHITE) {
+            xs[0] |= xw;
+            xs[2] |= xl;
+          } else {
+            xs[1] |= xw;
+            xs[3] |= xl;
           }
         }
       }
+    }
+    // rank-domain masks (bit r = position P R + r): the squares' bits
+    // compressed through the free squares F (Hacker's Delight 7-4, the six
+    // move masks of F computed once)
+    uint64_t mv[6];
+    {
+      uint64_t mm = F, mk = ~F << 1;
+#pragma unroll
+      for (int i = 0; i < 6; i++) {
+        uint64_t mp = mk ^ (mk << 1);
+        mp ^= mp << 2;
+        mp ^= mp << 4;
+        mp ^= mp << 8;
+        mp ^= mp << 16;
+        mp ^= mp << 32;
+        uint64_t v = mp & mm;
+        mm = (mm ^ v) | (v >> (1 << i));
+        mk &= ~mp;
+        mv[i] = v;
+      }
+    }
+    auto compress = [&](uint64_t x) {
+      x &= F;
+#pragma unroll
+      for (int i = 0; i < 6; i++) {
+        uint64_t t = x & mv[i];
+        x = (x ^ t) | (t >> (1 << i));
+      }
+      return x;
+    };
+    uint64_t rw = 0, rb = 0, rmw = 0, rmb = 0;
+    uint64_t rx[4] = { 0, 0, 0, 0 };
+    if (valid) {
+      rw = compress(WI);
+      rb = compress(BI);
+      rmw = compress(MW);
+      rmb = compress(MB);
+      if (M.synth)
+        for (int k = 0; k < 4; k++) rx[k] = compress(xs[k]);
     }
     // assemble the chunk's words in LDS (a lane's R bits start at bit
     // lane * R) and store each word of each plane once
