@@ -113,6 +113,21 @@ class Table:
                                   log_path.encode() if log_path else None, int(max_plies))
         if h < 0:
             raise RuntimeError(f"oracle generation of {material} failed ({h})")
+        self._wrap(h, material, synth_slice)
+
+    @classmethod
+    def find(cls, material):
+        """A table the oracle already generated (a smaller material comes
+        with the larger ones), or None."""
+        h = lib().tbo_find(material.encode())
+        if h < 0:
+            return None
+        t = cls.__new__(cls)
+        t._wrap(h, material, None)
+        return t
+
+    def _wrap(self, h, material, synth_slice):
+        L = lib()
         self.h = h
         self.material = material
         self.synth_slice = synth_slice

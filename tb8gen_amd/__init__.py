@@ -171,6 +171,8 @@ def lib():
         L.tbg_registry_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int]
         L.tbg_registry_get.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(vp)]
         L.tbg_registry_count.argtypes = [vp]
+        L.tbg_registry_entry.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_char_p),
+                                         ctypes.POINTER(vp)]
         L.tbg_registry_destroy.argtypes = [vp]
         L.tbg_registry_destroy.restype = None
         _lib = L
@@ -533,6 +535,16 @@ class NativeRegistry:
 
     def count(self):
         return lib().tbg_registry_count(self.ptr)
+
+    def entries(self):
+        """(material, Table) of every resident table, smallest first."""
+        out = []
+        for i in range(self.count()):
+            name, c = ctypes.c_char_p(), ctypes.c_void_p()
+            check(lib().tbg_registry_entry(self.ptr, i, ctypes.byref(name), ctypes.byref(c)),
+                  "registry entry")
+            out.append((name.value.decode(), Table(name.value.decode(), ptr=c.value)))
+        return out
 
     def close(self):
         if self.ptr:

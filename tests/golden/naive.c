@@ -32,6 +32,8 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include "naive_moves.h"
+
 #define UNK (-1)
 #define INVALID 0x7fff
 
@@ -45,37 +47,10 @@ typedef struct Solved {
 
 static Solved *g_solved;
 
-static const int DF[8] = { 1, -1, 0, 0, 1, 1, -1, -1 }; /* 0-3 straight, 4-7 diagonal */
-static const int DR[8] = { 0, 0, 1, -1, 1, -1, 1, -1 };
-static const int NF[8] = { 1, 2, 2, 1, -1, -2, -2, -1 };
-static const int NR[8] = { 2, 1, -1, -2, -2, -1, 1, 2 };
-
-static int onb(int f, int r) { return f >= 0 && f < 8 && r >= 0 && r < 8; }
-
-/* Is square x attacked by colour `by`?  Walks outward from x. */
+/* Is square x attacked by colour `by`? */
 static int attacked(const Solved *m, const int *board, int x, int by)
 {
-  int f = x & 7, r = x >> 3;
-  for (int d = 0; d < 8; d++) {
-    int cf = f + DF[d], cr = r + DR[d], dist = 1;
-    for (; onb(cf, cr); cf += DF[d], cr += DR[d], dist++) {
-      int j = board[cr * 8 + cf];
-      if (j < 0) continue;
-      if (m->col[j] == by) {
-        int t = m->typ[j];
-        if (t == 'Q' || (t == 'R' && d < 4) || (t == 'B' && d >= 4) || (t == 'K' && dist == 1))
-          return 1;
-      }
-      break;
-    }
-  }
-  for (int d = 0; d < 8; d++) {
-    int cf = f + NF[d], cr = r + NR[d];
-    if (!onb(cf, cr)) continue;
-    int j = board[cr * 8 + cf];
-    if (j >= 0 && m->col[j] == by && m->typ[j] == 'N') return 1;
-  }
-  return 0;
+  return nm_attacked(m->col, m->typ, board, x, by);
 }
 
 static int king_of(const Solved *m, int c)
@@ -111,36 +86,34 @@ typedef struct {
   int cw[64];
 } Moves;
 
-static void try_move(const Solved *m, Solved **sub, int *sq, int *board, int i, int to, int stm,
-                     Moves *mv)
+typedef struct {
+  const Solved *m;
+  Solved **sub;
+  int o;
+  Moves *mv;
+} MoveCtx;
+
+static void take_move(void *u, int i, int to, int j, const int *sq, const int *board)
 {
-  int o = stm ^ 1, from = sq[i], j = board[to];
-  if (j >= 0 && (m->col[j] == stm || m->typ[j] == 'K')) return;
-  board[from] = -1;
-  board[to] = i;
-  sq[i] = to;
-  if (j >= 0) sq[j] = -1;
-  int ks = sq[king_of(m, stm)];
-  if (!attacked(m, board, ks, o)) {
-    if (j < 0) {
+  (void)i;
+  (void)to;
+  (void)board;
+  MoveCtx *x = (MoveCtx *)u;
+  const Solved *m = x->m;
+  if (j < 0) {
+    uint64_t q = 0;
+    for (int a = m->k - 1; a >= 0; a--) q = q * 64 + (uint64_t)sq[a];
+    x->mv->q[x->mv->nq++] = q;
+  } else {
+    int wdl = 0;
+    if (x->sub[j]) {
       uint64_t q = 0;
-      for (int a = m->k - 1; a >= 0; a--) q = q * 64 + (uint64_t)sq[a];
-      mv->q[mv->nq++] = q;
-    } else {
-      int wdl = 0;
-      if (sub[j]) {
-        uint64_t q = 0;
-        for (int a = m->k - 1; a >= 0; a--)
-          if (a != j) q = q * 64 + (uint64_t)sq[a];
-        wdl = wdl_of(sub[j]->v[o][q]);
-      }
-      mv->cw[mv->nc++] = wdl;
+      for (int a = m->k - 1; a >= 0; a--)
+        if (a != j) q = q * 64 + (uint64_t)sq[a];
+      wdl = wdl_of(x->sub[j]->v[x->o][q]);
     }
+    x->mv->cw[x->mv->nc++] = wdl;
   }
-  sq[i] = from;
-  if (j >= 0) sq[j] = to;
-  board[to] = j;
-  board[from] = i;
 }
 
 static void gen_moves(const Solved *m, Solved **sub, uint64_t p, int stm, Moves *mv)
@@ -150,23 +123,8 @@ static void gen_moves(const Solved *m, Solved **sub, uint64_t p, int stm, Moves 
   uint64_t x = p;
   for (int i = 0; i < m->k; i++) { sq[i] = (int)(x & 63); x >>= 6; board[sq[i]] = i; }
   mv->nq = mv->nc = 0;
-  for (int i = 0; i < m->k; i++) {
-    if (m->col[i] != stm) continue;
-    int t = m->typ[i], f = sq[i] & 7, r = sq[i] >> 3;
-    if (t == 'N') {
-      for (int d = 0; d < 8; d++)
-        if (onb(f + NF[d], r + NR[d])) try_move(m, sub, sq, board, i, (r + NR[d]) * 8 + f + NF[d], stm, mv);
-      continue;
-    }
-    int d0 = t == 'B' ? 4 : 0, d1 = t == 'R' ? 4 : 8;
-    for (int d = d0; d < d1; d++) {
-      for (int cf = f + DF[d], cr = r + DR[d]; onb(cf, cr); cf += DF[d], cr += DR[d]) {
-        int to = cr * 8 + cf, occupied = board[to] >= 0;
-        try_move(m, sub, sq, board, i, to, stm, mv);
-        if (occupied || t == 'K') break;
-      }
-    }
-  }
+  MoveCtx ctx = { m, sub, stm ^ 1, mv };
+  nm_legal_moves(m->col, m->typ, m->k, sq, board, stm, take_move, &ctx);
 }
 
 static Solved *solve(const int *col, const int *typ, int k)
