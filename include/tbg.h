@@ -157,46 +157,78 @@ int tbg_generate_batch_noalloc(tbg_ctx *const *ctxs, int nc, tbg_counts *counts)
 int tbg_emit_lines(const tbg_ctx *ctx, tbg_line_fn fn, void *user);
 
 /* ---------------------------------------------------------------------
- * Served form: a continuous-batching job server.  Jobs are synthetic
- * single-king-slice jobs (tbg_create_synth) with their codes kept.  run()
- * admits arrived jobs in FIFO order while the device-memory budget and the
- * slot limit allow, runs one ply of every admitted job per launch sequence
- * (jobs join and leave the running batch at ply boundaries), and spills
- * each finished job's codes to pinned host memory on a side stream.  With
- * TBG_FORM_PLAIN it serves one job at a time instead (the comparison).
- * No counterpart in the reference, which generates one material per run.
+ * Served form: a continuous-batching job server.  A job is one or several
+ * king slices of a material, each a synthetic slice job (tbg_create_synth:
+ * seeded hash exits, codes kept).  The serving loop admits arrived jobs in
+ * FIFO order while the device-memory budget and the slot limit allow, runs
+ * one ply of every admitted slice per launch sequence (slices join and
+ * leave the running batch at ply boundaries), and copies each finished
+ * slice's codes to pinned host memory on a side stream.  When the oldest
+ * waiting job does not fit, admitted jobs larger than it are preempted: at
+ * a ply boundary their whole device state (bitmap planes, capture exits,
+ * codes, job record) is copied to pinned host memory and their device
+ * memory released; they are copied back and rejoin once memory allows,
+ * bit-exact.  With TBG_FORM_PLAIN it serves one job at a time instead (the
+ * comparison).  No counterpart in the reference, which generates one
+ * material per run.
+ *
+ * Jobs may be submitted before or while the loop runs (tbg_server_serve on
+ * one host thread, tbg_server_submit / tbg_job_poll from others).
  * --------------------------------------------------------------------- */
 typedef struct tbg_server tbg_server;
 typedef struct tbg_job_spec {
   const char *material;  /* e.g. "KQRvKQR"                                  */
-  int slice;             /* king slice, 0 <= slice < 441                    */
-  uint64_t seed;         /* exit hash seed                                  */
-  double arrival_ms;     /* arrival time after the start of tbg_server_run  */
+  int slice;             /* first king slice, 0 <= slice < 441              */
+  uint64_t seed;         /* exit hash seed (the same for every slice)       */
+  double arrival_ms;     /* arrival after the start of serving; < 0: the
+                            moment of submission                            */
+  int nslices;           /* king slices slice, slice + 1, ... (mod 441); 0: 1,
+                            at most TBG_JOB_MAX_SLICES                      */
 } tbg_job_spec;
+#define TBG_JOB_MAX_SLICES 16
 typedef struct tbg_job_result {
   int status;            /* TBG_OK, TBG_E_BUDGET (never fits), ...          */
-  int nplies;
-  double arrival_ms, admit_ms, done_ms; /* done = codes on the host        */
-  uint64_t positions;    /* per side                                        */
+  int nplies;            /* the largest of the job's slices                 */
+  double arrival_ms, admit_ms, done_ms; /* done = all codes on the host     */
+  uint64_t positions;    /* per side and slice                              */
   uint64_t device_bytes;
-  uint64_t half_steps;
+  uint64_t half_steps;   /* summed over the job's slices                    */
+  int nslices;
+  int spills;            /* times the job was preempted to host memory      */
 } tbg_job_result;
 typedef struct tbg_server_stats {
   double wall_ms;
   uint64_t jobs_done, jobs_rejected;
   double mean_resident_bytes;  /* time-averaged device bytes of admitted jobs */
   uint64_t peak_resident_bytes;
-  int peak_active;
+  int peak_active;             /* slices in the running batch                 */
   uint64_t plies;              /* launch sequences run                        */
   double position_side_plies;  /* sum over jobs of half-steps x positions / 2 */
+  uint64_t spills, restores;   /* preemptions to pinned host memory and back  */
+  double spill_ms;             /* host wall time spent spilling and restoring */
 } tbg_server_stats;
+/* tbg_job_poll: the job is not finished yet (not an error) */
+#define TBG_PENDING 1
 int tbg_server_create(tbg_server **out, uint64_t budget_bytes, int max_active);
 void tbg_server_destroy(tbg_server *srv);
+/* Thread-safe; may be called while tbg_server_serve runs. */
 int tbg_server_submit(tbg_server *srv, const tbg_job_spec *spec, uint64_t *job_id);
+/* The serving loop: runs until tbg_server_close has been called and every
+ * submitted job is done (thread-safe to call from another thread). */
+int tbg_server_serve(tbg_server *srv, int form, tbg_server_stats *stats);
+int tbg_server_close(tbg_server *srv);
+/* tbg_server_close, then tbg_server_serve: every job submitted beforehand. */
 int tbg_server_run(tbg_server *srv, int form, tbg_server_stats *stats);
+/* TBG_OK and the result once the job is done (or rejected), TBG_PENDING
+ * before; thread-safe. */
+int tbg_job_poll(tbg_server *srv, uint64_t job_id, tbg_job_result *out);
 int tbg_server_result(tbg_server *srv, uint64_t job_id, tbg_job_result *out);
+/* codes of slice number `part` (0 .. nslices - 1) of a finished job */
 int tbg_server_codes(tbg_server *srv, uint64_t job_id, int stm, uint16_t *dst, uint64_t n);
+int tbg_server_codes_part(tbg_server *srv, uint64_t job_id, int part, int stm, uint16_t *dst,
+                          uint64_t n);
 int tbg_server_lines(tbg_server *srv, uint64_t job_id, tbg_line_fn fn, void *user);
+int tbg_server_lines_part(tbg_server *srv, uint64_t job_id, int part, tbg_line_fn fn, void *user);
 
 /* Data plane (device -> host, synchronous). */
 int tbg_get_codes(tbg_ctx *ctx, int stm, int slice, uint16_t *dst, uint64_t n);

@@ -33,6 +33,8 @@ ABI_SYMBOLS = [
     "tbg_set_profiling", "tbg_host_selftest", "tbg_set_scatter", "tbg_generate_batch",
     "tbg_emit_lines", "tbg_generate_batch_noalloc", "tbg_server_create", "tbg_server_destroy", "tbg_server_submit",
     "tbg_server_run", "tbg_server_result", "tbg_server_codes", "tbg_server_lines",
+    "tbg_server_serve", "tbg_server_close", "tbg_job_poll", "tbg_server_codes_part",
+    "tbg_server_lines_part",
     "tbg_set_geometry", "tbg_get_geometry", "tbg_autotune", "tbg_file_write", "tbg_file_read",
     "tbg_checkpoint", "tbg_restore", "tbg_continue", "tbg_export_plies", "tbg_rtbw_name",
     "tbg_write_rtbw",
@@ -84,21 +86,24 @@ class Geometry(ctypes.Structure):
 
 class JobSpec(ctypes.Structure):
     _fields_ = [("material", ctypes.c_char_p), ("slice", ctypes.c_int), ("seed", ctypes.c_uint64),
-                ("arrival_ms", ctypes.c_double)]
+                ("arrival_ms", ctypes.c_double), ("nslices", ctypes.c_int)]
 
 
 class JobResult(ctypes.Structure):
     _fields_ = [("status", ctypes.c_int), ("nplies", ctypes.c_int), ("arrival_ms", ctypes.c_double),
                 ("admit_ms", ctypes.c_double), ("done_ms", ctypes.c_double),
                 ("positions", ctypes.c_uint64), ("device_bytes", ctypes.c_uint64),
-                ("half_steps", ctypes.c_uint64)]
+                ("half_steps", ctypes.c_uint64), ("nslices", ctypes.c_int),
+                ("spills", ctypes.c_int)]
 
 
 class ServerStats(ctypes.Structure):
     _fields_ = [("wall_ms", ctypes.c_double), ("jobs_done", ctypes.c_uint64),
                 ("jobs_rejected", ctypes.c_uint64), ("mean_resident_bytes", ctypes.c_double),
                 ("peak_resident_bytes", ctypes.c_uint64), ("peak_active", ctypes.c_int),
-                ("plies", ctypes.c_uint64), ("position_side_plies", ctypes.c_double)]
+                ("plies", ctypes.c_uint64), ("position_side_plies", ctypes.c_double),
+                ("spills", ctypes.c_uint64), ("restores", ctypes.c_uint64),
+                ("spill_ms", ctypes.c_double)]
 
 
 LINE_FN = ctypes.CFUNCTYPE(None, ctypes.c_char_p, ctypes.c_void_p)
@@ -150,6 +155,12 @@ def lib():
         L.tbg_server_destroy.restype = None
         L.tbg_server_submit.argtypes = [vp, ctypes.POINTER(JobSpec), ctypes.POINTER(ctypes.c_uint64)]
         L.tbg_server_run.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ServerStats)]
+        L.tbg_server_serve.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ServerStats)]
+        L.tbg_server_close.argtypes = [vp]
+        L.tbg_job_poll.argtypes = [vp, ctypes.c_uint64, ctypes.POINTER(JobResult)]
+        L.tbg_server_codes_part.argtypes = [vp, ctypes.c_uint64, ctypes.c_int, ctypes.c_int,
+                                            ctypes.POINTER(ctypes.c_uint16), ctypes.c_uint64]
+        L.tbg_server_lines_part.argtypes = [vp, ctypes.c_uint64, ctypes.c_int, LINE_FN, vp]
         L.tbg_server_result.argtypes = [vp, ctypes.c_uint64, ctypes.POINTER(JobResult)]
         L.tbg_server_codes.argtypes = [vp, ctypes.c_uint64, ctypes.c_int,
                                        ctypes.POINTER(ctypes.c_uint16), ctypes.c_uint64]
@@ -427,7 +438,9 @@ def generate_batch_noalloc(tables):
 
 
 class Server:
-    """Continuous-batching server of synthetic king-slice jobs (libtbg)."""
+    """Continuous-batching server of synthetic king-slice jobs (libtbg).
+    Jobs may be submitted before run()/serve() or, with serve() running on
+    another thread, while it runs; poll() tells whether one is done."""
 
     def __init__(self, budget_bytes, max_active=64):
         init()
@@ -436,39 +449,61 @@ class Server:
               "server")
         self.ids = []
 
-    def submit(self, material, slice_, seed, arrival_ms=0.0):
-        spec = JobSpec(material.encode(), int(slice_), int(seed), float(arrival_ms))
+    def submit(self, material, slice_, seed, arrival_ms=0.0, nslices=1):
+        """arrival_ms < 0: the moment of submission (live serving)."""
+        spec = JobSpec(material.encode(), int(slice_), int(seed), float(arrival_ms), int(nslices))
         jid = ctypes.c_uint64()
         check(lib().tbg_server_submit(self.ptr, ctypes.byref(spec), ctypes.byref(jid)), "submit")
         self.ids.append(jid.value)
         return jid.value
 
     def run(self, form=FORM_BATCHED):
+        """Serve every job submitted so far (closes the server to new jobs)."""
         st = ServerStats()
         check(lib().tbg_server_run(self.ptr, form, ctypes.byref(st)), "server run")
         return st
+
+    def serve(self, form=FORM_BATCHED):
+        """The serving loop; returns once close() was called and every job
+        is done.  ctypes releases the interpreter lock for its duration, so
+        other threads submit and poll meanwhile."""
+        st = ServerStats()
+        check(lib().tbg_server_serve(self.ptr, form, ctypes.byref(st)), "server serve")
+        return st
+
+    def close_submissions(self):
+        check(lib().tbg_server_close(self.ptr), "server close")
+
+    def poll(self, jid):
+        """The job's result once it is done, else None."""
+        r = JobResult()
+        rc = lib().tbg_job_poll(self.ptr, jid, ctypes.byref(r))
+        if rc == 1:  # TBG_PENDING
+            return None
+        check(rc, "poll")
+        return r
 
     def result(self, jid):
         r = JobResult()
         check(lib().tbg_server_result(self.ptr, jid, ctypes.byref(r)), "result")
         return r
 
-    def codes(self, jid, stm):
+    def codes(self, jid, stm, part=0):
         r = self.result(jid)
         out = np.zeros(r.positions, dtype=np.uint16)
-        check(lib().tbg_server_codes(self.ptr, jid, stm,
-                                     out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)),
-                                     r.positions), "codes")
+        check(lib().tbg_server_codes_part(self.ptr, jid, int(part), stm,
+                                          out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)),
+                                          r.positions), "codes")
         return out
 
-    def lines(self, jid):
+    def lines(self, jid, part=0):
         out = []
 
         def cb(line, _u):
             out.append(line.decode())
 
         fn = LINE_FN(cb)
-        check(lib().tbg_server_lines(self.ptr, jid, fn, None), "lines")
+        check(lib().tbg_server_lines_part(self.ptr, jid, int(part), fn, None), "lines")
         return out
 
     def close(self):

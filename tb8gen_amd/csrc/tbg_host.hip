@@ -11,6 +11,8 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <condition_variable>
+#include <deque>
 #include <filesystem>
 #include <functional>
 #include <mutex>
@@ -2521,16 +2523,128 @@ extern "C" int tbg_host_selftest(char *msg, int len)
 // ====================================================================
 namespace {
 
+// Preemption: a resident context's whole device state (bitmap planes,
+// capture-exit planes, codes and job record) goes to pinned host memory and
+// its device memory back to the pool; the copy back rebases the job
+// record's plane pointers onto the new allocations.  Runs on the context's
+// own stream (the server's set-up stream), with the batch idle.
+struct CtxImage {
+  void *host = nullptr;
+  size_t bytes = 0;
+  uint64_t *arena = nullptr, *subarena = nullptr;  // the released allocations
+  uint16_t *codes[2] = { nullptr, nullptr };
+};
+
+size_t ctx_arena_bytes(const tbg_ctx *c) { return c->pw * 8 * kPlanes; }
+size_t ctx_sub_bytes(const tbg_ctx *c)
+{
+  return 4 * c->nsl * (c->h.ctotal[0] + c->h.ctotal[1]) * 8 + 64;
+}
+size_t ctx_codes_bytes(const tbg_ctx *c) { return c->codes[0] ? c->nsl * c->h.ii.size * 2 : 0; }
+
+int ctx_spill(tbg_ctx *c, CtxImage &im)
+{
+  size_t ab = ctx_arena_bytes(c), sb = ctx_sub_bytes(c), cb = ctx_codes_bytes(c);
+  size_t total = ab + sb + 2 * cb;
+  if (!im.host) {
+    HIPCHK(hipHostMalloc(&im.host, total, hipHostMallocDefault));
+    im.bytes = total;
+  }
+  char *h = static_cast<char *>(im.host);
+  HIPCHK(hipMemcpyAsync(h, c->arena, ab, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipMemcpyAsync(h + ab, c->subarena, sb, hipMemcpyDeviceToHost, c->st));
+  for (int s = 0; s < 2 && cb; s++)
+    HIPCHK(hipMemcpyAsync(h + ab + sb + s * cb, c->codes[s], cb, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipMemcpyAsync(&c->hj, c->dj, sizeof(JobDev), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  im.arena = c->arena;
+  im.subarena = c->subarena;
+  dfree(c, c->arena);
+  dfree(c, c->subarena);
+  c->arena = c->subarena = nullptr;
+  for (int s = 0; s < 2; s++) {
+    im.codes[s] = c->codes[s];
+    dfree(c, c->codes[s]);
+    c->codes[s] = nullptr;
+  }
+  return TBG_OK;
+}
+
+template <class X> X *rebase(X *p, const void *old, size_t n, void *now)
+{
+  const char *a = reinterpret_cast<const char *>(p), *o = static_cast<const char *>(old);
+  if (!p || a < o || a >= o + n) return p;
+  return reinterpret_cast<X *>(static_cast<char *>(now) + (a - o));
+}
+
+int ctx_restore_image(tbg_ctx *c, CtxImage &im)
+{
+  size_t ab = c->pw * 8 * kPlanes;
+  size_t sb = 4 * c->nsl * (c->h.ctotal[0] + c->h.ctotal[1]) * 8 + 64;
+  size_t cb = im.codes[0] ? c->nsl * c->h.ii.size * 2 : 0;
+  TRY(dalloc(c, (void **)&c->arena, ab));
+  TRY(dalloc(c, (void **)&c->subarena, sb));
+  for (int s = 0; s < 2 && cb; s++) TRY(dalloc(c, (void **)&c->codes[s], cb));
+  const char *h = static_cast<const char *>(im.host);
+  HIPCHK(hipMemcpyAsync(c->arena, h, ab, hipMemcpyHostToDevice, c->st));
+  HIPCHK(hipMemcpyAsync(c->subarena, h + ab, sb, hipMemcpyHostToDevice, c->st));
+  for (int s = 0; s < 2 && cb; s++)
+    HIPCHK(hipMemcpyAsync(c->codes[s], h + ab + sb + s * cb, cb, hipMemcpyHostToDevice, c->st));
+  // the canonical planes in their allocation order (ctx_alloc), and the job
+  // record's (swapped) plane pointers moved onto the new allocations
+  uint64_t *p = c->arena;
+  for (int s = 0; s < 2; s++) {
+    c->wins[s] = p; p += c->pw;
+    c->prevW[s] = p; p += c->pw;
+    c->prevL[s] = p; p += c->pw;
+    c->newW[s] = p; p += c->pw;
+    c->newL[s] = p; p += c->pw;
+    c->cand[s] = p; p += c->pw;
+    c->candW[s] = p; p += c->pw;
+  }
+  p = c->subarena;
+  for (int col = 0; col < 2; col++)
+    for (int k = 0; k < 4; k++) {
+      c->sub[col][k] = p;
+      p += c->nsl * c->h.ctotal[col];
+    }
+  JobDev &J = c->hj;
+  for (int s = 0; s < 2; s++) {
+    J.wins[s] = rebase(J.wins[s], im.arena, ab, c->arena);
+    J.prevW[s] = rebase(J.prevW[s], im.arena, ab, c->arena);
+    J.prevL[s] = rebase(J.prevL[s], im.arena, ab, c->arena);
+    J.newW[s] = rebase(J.newW[s], im.arena, ab, c->arena);
+    J.newL[s] = rebase(J.newL[s], im.arena, ab, c->arena);
+    J.cand[s] = rebase(J.cand[s], im.arena, ab, c->arena);
+    J.candW[s] = rebase(J.candW[s], im.arena, ab, c->arena);
+    for (int k = 0; k < 4; k++) J.sub[s][k] = rebase(J.sub[s][k], im.subarena, sb, c->subarena);
+    J.codes[s] = c->codes[s];
+  }
+  HIPCHK(hipMemcpyAsync(c->dj, &c->hj, sizeof(JobDev), hipMemcpyHostToDevice, c->st));
+  return TBG_OK;
+}
+
+// one king slice of a served job
+struct SrvSlice {
+  int slice = 0;
+  tbg_ctx *ctx = nullptr;
+  uint16_t *pinned[2] = { nullptr, nullptr };  // its final codes
+  CtxImage image;                              // its device state while preempted
+  std::vector<long long> hist;
+  tbg_counts counts;
+  bool done = false;    // finished: codes copied (or copying) to pinned memory
+  bool landed = false;  // those copies complete
+  hipEvent_t ev = nullptr;
+};
+
 struct SrvJob {
   std::string material;
   tbg_job_spec spec;
   tbg_job_result res;
-  int state = 0;  // 0 queued, 1 active, 2 spilling, 3 done
-  tbg_ctx *ctx = nullptr;
-  uint16_t *pinned[2] = { nullptr, nullptr };
-  hipEvent_t ev = nullptr;
-  std::vector<long long> hist;
-  tbg_counts counts;
+  int state = 0;  // 0 waiting, 1 resident, 2 preempted, 3 done
+  std::vector<SrvSlice> sl;
+  uint64_t bytes = 0;  // admission estimate, all slices
+  int left = 0;        // slices whose codes have not landed
 };
 
 uint64_t estimate_bytes(const char *material, int *err)
@@ -2548,10 +2662,155 @@ struct tbg_server {
   uint64_t budget = 0;
   int max_active = 1;
   Geom geom = g_geom;
-  std::vector<SrvJob> jobs;
-  // st: the batch; side: spill copies; setup: job allocation and init
+  std::deque<SrvJob> jobs;  // appended by submit (references stay valid)
+  std::mutex mu;            // jobs' length, states and results; closed; serving
+  std::condition_variable cv;
+  bool closed = false, serving = false, started = false;
+  std::chrono::steady_clock::time_point t0;
+  // st: the batch; side: codes of finished slices; setup: job allocation,
+  // init, preemption and restore
   hipStream_t st = nullptr, side = nullptr, setup = nullptr;
+  double now_ms() const
+  {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
 };
+
+namespace {
+
+// the serving loop (tbg_server_serve)
+struct Serve {
+  tbg_server *s;
+  bool plain;
+  int max_slices;
+  std::vector<int> waiting;                    // jobs in arrival order
+  std::vector<std::pair<int, int>> active;     // (job, slice) in the batch
+  std::vector<std::pair<int, int>> joining;    // set-up or restore queued
+  std::vector<std::pair<int, int>> spilling;   // codes on their way to pinned memory
+  std::vector<int> preempted;                  // jobs on the host, oldest first
+  uint64_t used = 0, peak = 0;
+  int peak_active = 0;
+  double occ_integral = 0, last_t = 0;
+  uint64_t plies = 0, done = 0, rejected = 0, spills = 0, restores = 0;
+  double psp = 0, spill_ms = 0;
+
+  SrvJob &J(int i) { return s->jobs[i]; }
+  int slots_used() const { return (int)(active.size() + joining.size()); }
+
+  void complete(int ji, int status)
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    SrvJob &j = J(ji);
+    j.res.status = status;
+    j.res.done_ms = s->now_ms();
+    j.state = 3;
+    s->cv.notify_all();
+  }
+
+  // set-up of one slice: pooled allocation, capture exits, init, queued on
+  // the set-up stream
+  int admit_slice(int ji, int k)
+  {
+    SrvJob &j = J(ji);
+    SrvSlice &q = j.sl[k];
+    tbg_ctx *c = nullptr;
+    TRY(create_common(&c, j.material.c_str(), 1, q.slice, j.spec.seed, true, s->setup));
+    q.ctx = c;
+    TRY(tbg_resolve_captures(c, nullptr, 0));
+    TRY(run_init_async(c));
+    HIPCHK(hipEventCreateWithFlags(&c->ready, hipEventDisableTiming));
+    HIPCHK(hipEventRecord(c->ready, s->setup));
+    used += c->bytes;
+    joining.push_back({ ji, k });
+    return TBG_OK;
+  }
+
+  int admit(int ji)
+  {
+    SrvJob &j = J(ji);
+    j.res.admit_ms = s->now_ms();
+    j.res.device_bytes = 0;
+    for (int k = 0; k < (int)j.sl.size(); k++) {
+      TRY(admit_slice(ji, k));
+      j.res.device_bytes += j.sl[k].ctx->bytes;
+      j.res.positions = j.sl[k].ctx->h.ii.size;
+    }
+    peak = used > peak ? used : peak;
+    std::lock_guard<std::mutex> lk(s->mu);
+    j.state = 1;
+    return TBG_OK;
+  }
+
+  // with the batch idle: job ji's unfinished slices to pinned host memory
+  int preempt(int ji)
+  {
+    SrvJob &j = J(ji);
+    double t = s->now_ms();
+    for (int k = 0; k < (int)j.sl.size(); k++) {
+      SrvSlice &q = j.sl[k];
+      if (q.done) continue;
+      TRY(ctx_spill(q.ctx, q.image));
+      used -= q.ctx->bytes;
+      active.erase(std::remove(active.begin(), active.end(), std::make_pair(ji, k)), active.end());
+    }
+    j.res.spills++;
+    spills++;
+    spill_ms += s->now_ms() - t;
+    preempted.push_back(ji);
+    std::lock_guard<std::mutex> lk(s->mu);
+    j.state = 2;
+    return TBG_OK;
+  }
+
+  int restore(int ji)
+  {
+    SrvJob &j = J(ji);
+    double t = s->now_ms();
+    for (int k = 0; k < (int)j.sl.size(); k++) {
+      SrvSlice &q = j.sl[k];
+      if (q.done) continue;
+      TRY(ctx_restore_image(q.ctx, q.image));
+      HIPCHK(hipEventRecord(q.ctx->ready, s->setup));
+      used += q.ctx->bytes;
+      joining.push_back({ ji, k });
+    }
+    peak = used > peak ? used : peak;
+    restores++;
+    spill_ms += s->now_ms() - t;
+    std::lock_guard<std::mutex> lk(s->mu);
+    j.state = 1;
+    return TBG_OK;
+  }
+
+  // admitted jobs larger than waiting job wi whose release lets it in
+  // (largest first; a job is preempted at most twice), or none
+  std::vector<int> victims(int wi)
+  {
+    std::vector<int> cand;
+    SrvJob &w = J(wi);
+    for (auto &a : active) {
+      SrvJob &j = J(a.first);
+      if (j.state != 1 || j.bytes <= w.bytes || j.res.spills >= 2) continue;
+      if (std::find(cand.begin(), cand.end(), a.first) != cand.end()) continue;
+      bool all_active = true;  // none of its slices still joining
+      for (auto &q : joining) all_active &= q.first != a.first;
+      if (all_active) cand.push_back(a.first);
+    }
+    std::sort(cand.begin(), cand.end(), [&](int x, int y) { return J(x).bytes > J(y).bytes; });
+    std::vector<int> out;
+    uint64_t freed = 0;
+    for (int v : cand) {
+      if (used - freed + w.bytes <= s->budget) break;
+      out.push_back(v);
+      for (auto &q : J(v).sl)
+        if (!q.done) freed += q.ctx->bytes;
+    }
+    if (used - freed + w.bytes > s->budget) out.clear();
+    return out;
+  }
+};
+
+}  // namespace
 
 extern "C" {
 
@@ -2586,11 +2845,13 @@ void tbg_server_destroy(tbg_server *s)
   if (s->st) (void)hipStreamSynchronize(s->st);
   if (s->side) (void)hipStreamSynchronize(s->side);
   if (s->setup) (void)hipStreamSynchronize(s->setup);
-  for (auto &j : s->jobs) {
-    if (j.ctx) tbg_destroy(j.ctx);
-    for (auto &p : j.pinned) (void)hipHostFree(p);
-    if (j.ev) (void)hipEventDestroy(j.ev);
-  }
+  for (auto &j : s->jobs)
+    for (auto &q : j.sl) {
+      if (q.ctx) tbg_destroy(q.ctx);
+      for (auto &p : q.pinned) (void)hipHostFree(p);
+      if (q.image.host) (void)hipHostFree(q.image.host);
+      if (q.ev) (void)hipEventDestroy(q.ev);
+    }
   if (s->setup) (void)hipStreamSynchronize(s->setup);  // the pooled frees
   if (s->st) (void)hipStreamDestroy(s->st);
   if (s->side) (void)hipStreamDestroy(s->side);
@@ -2602,6 +2863,9 @@ int tbg_server_submit(tbg_server *s, const tbg_job_spec *spec, uint64_t *job_id)
 {
   if (!s || !spec || !spec->material) return fail(TBG_E_INVAL, "null argument");
   if (spec->slice < 0 || spec->slice >= 441) return fail(TBG_E_INVAL, "slice must be in [0, 441)");
+  int ns = spec->nslices ? spec->nslices : 1;
+  if (ns < 1 || ns > TBG_JOB_MAX_SLICES)
+    return fail(TBG_E_INVAL, "a job has 1 to %d king slices", TBG_JOB_MAX_SLICES);
   Mat M;
   int err = setup_mat(M, spec->material);
   if (err) return err;
@@ -2609,71 +2873,113 @@ int tbg_server_submit(tbg_server *s, const tbg_job_spec *spec, uint64_t *job_id)
   j.material = spec->material;
   j.spec = *spec;
   j.spec.material = nullptr;
+  j.spec.nslices = ns;
   memset(&j.res, 0, sizeof j.res);
-  j.res.arrival_ms = spec->arrival_ms;
-  // the pinned host buffers its codes spill to (a pinned allocation takes
+  j.res.nslices = ns;
+  j.bytes = estimate_bytes(spec->material, &err) * (uint64_t)ns;
+  j.left = ns;
+  j.sl.resize(ns);
+  // the pinned host buffers its codes go to (a pinned allocation takes
   // milliseconds, so it is made here, not while serving)
-  for (int stm = 0; stm < 2; stm++)
-    if (hipHostMalloc((void **)&j.pinned[stm], M.ii.size * 2, hipHostMallocDefault) != hipSuccess) {
-      for (auto &q : j.pinned) (void)hipHostFree(q);
-      return fail(TBG_E_NOMEM, "pinned host memory for a job's codes");
-    }
+  for (int k = 0; k < ns; k++) {
+    j.sl[k].slice = (spec->slice + k) % 441;
+    for (int stm = 0; stm < 2; stm++)
+      if (hipHostMalloc((void **)&j.sl[k].pinned[stm], M.ii.size * 2, hipHostMallocDefault) !=
+          hipSuccess) {
+        for (auto &q : j.sl)
+          for (auto &p : q.pinned) (void)hipHostFree(p);
+        return fail(TBG_E_NOMEM, "pinned host memory for a job's codes");
+      }
+  }
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (s->closed) {
+    for (auto &q : j.sl)
+      for (auto &p : q.pinned) (void)hipHostFree(p);
+    return fail(TBG_E_STATE, "the server takes no more jobs");
+  }
+  double a = spec->arrival_ms;
+  if (a < 0) a = s->started ? s->now_ms() : 0.0;
+  j.spec.arrival_ms = j.res.arrival_ms = a;
   s->jobs.push_back(std::move(j));
   if (job_id) *job_id = s->jobs.size() - 1;
+  s->cv.notify_all();
   return TBG_OK;
 }
 
-int tbg_server_run(tbg_server *s, int form, tbg_server_stats *stats)
+int tbg_server_close(tbg_server *s)
 {
   if (!s) return fail(TBG_E_INVAL, "null server");
-  using clk = std::chrono::steady_clock;
-  auto t0 = clk::now();
-  auto now_ms = [&]() { return std::chrono::duration<double, std::milli>(clk::now() - t0).count(); };
-  // the plain server runs one job at a time; both check the device every
-  // kPlies plies (a job that finishes meanwhile idles, its kernels skip it)
-  int max_active = form == TBG_FORM_PLAIN ? 1 : s->max_active;
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->closed = true;
+  s->cv.notify_all();
+  return TBG_OK;
+}
+
+int tbg_server_serve(tbg_server *s, int form, tbg_server_stats *stats)
+{
+  if (!s) return fail(TBG_E_INVAL, "null server");
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->serving) return fail(TBG_E_STATE, "the server is already serving");
+    s->serving = true;
+    if (!s->started) {
+      s->t0 = std::chrono::steady_clock::now();
+      s->started = true;
+    }
+  }
+  Serve v;
+  v.s = s;
+  v.plain = form == TBG_FORM_PLAIN;
+  // the plain server runs one slice at a time; both check the device every
+  // kPlies plies (a slice that finishes meanwhile idles, its kernels skip it)
+  v.max_slices = v.plain ? 1 : s->max_active;
   const int kPlies = 4;
-  std::vector<int> pending;
-  for (size_t i = 0; i < s->jobs.size(); i++)
-    if (s->jobs[i].state == 0) pending.push_back((int)i);
-  std::stable_sort(pending.begin(), pending.end(), [&](int a, int b) {
-    return s->jobs[a].spec.arrival_ms < s->jobs[b].spec.arrival_ms;
-  });
-  size_t next = 0;
-  std::vector<int> active, joining, spilling;
-  uint64_t used = 0, peak = 0;
-  int peak_active = 0;
-  double occ_integral = 0, last_t = 0;
-  uint64_t plies = 0, done = 0, rejected = 0;
-  double psp = 0;
-  // batch state, kept across membership changes
+  size_t seen = 0;
   JobDev **dptr = nullptr;
   unsigned *dflags = nullptr, *ddone = nullptr, *hflags = nullptr, *hdone = nullptr;
   hipEvent_t evb = nullptr;
-  HIPCHK(hipMalloc(&dptr, sizeof(JobDev *) * max_active));
-  HIPCHK(hipMalloc(&dflags, 4 * sizeof(unsigned)));
-  HIPCHK(hipMalloc(&ddone, sizeof(unsigned) * max_active));
-  HIPCHK(hipHostMalloc((void **)&hflags, 4 * sizeof(unsigned), hipHostMallocDefault));
-  HIPCHK(hipHostMalloc((void **)&hdone, sizeof(unsigned) * max_active, hipHostMallocDefault));
-  HIPCHK(hipEventCreateWithFlags(&evb, hipEventDisableTiming));
+  int r = TBG_OK;
+  auto setup_fail = [&](hipError_t e) {
+    if (e != hipSuccess) r = fail(TBG_E_HIP, "server batch buffers: %s", hipGetErrorString(e));
+  };
+  setup_fail(hipMalloc(&dptr, sizeof(JobDev *) * v.max_slices));
+  if (r == TBG_OK) setup_fail(hipMalloc(&dflags, 4 * sizeof(unsigned)));
+  if (r == TBG_OK) setup_fail(hipMalloc(&ddone, sizeof(unsigned) * v.max_slices));
+  if (r == TBG_OK) setup_fail(hipHostMalloc((void **)&hflags, 4 * sizeof(unsigned), hipHostMallocDefault));
+  if (r == TBG_OK)
+    setup_fail(hipHostMalloc((void **)&hdone, sizeof(unsigned) * v.max_slices, hipHostMallocDefault));
+  if (r == TBG_OK) setup_fail(hipEventCreateWithFlags(&evb, hipEventDisableTiming));
   LaunchSet L;
   bool changed = true;
-  int r = TBG_OK;
-  while (r == TBG_OK &&
-         (next < pending.size() || !active.empty() || !joining.empty() || !spilling.empty())) {
-    double t = now_ms();
-    occ_integral += (double)used * (t - last_t);
-    last_t = t;
+  while (r == TBG_OK) {
+    // 0. jobs submitted since the last look join the waiting list
+    bool closed;
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      for (; seen < s->jobs.size(); seen++) {
+        double a = s->jobs[seen].spec.arrival_ms;
+        auto it = std::upper_bound(v.waiting.begin(), v.waiting.end(), a,
+                                   [&](double x, int ji) { return x < s->jobs[ji].spec.arrival_ms; });
+        v.waiting.insert(it, (int)seen);
+      }
+      closed = s->closed;
+    }
+    if (closed && v.waiting.empty() && v.active.empty() && v.joining.empty() &&
+        v.spilling.empty() && v.preempted.empty())
+      break;
+    double t = s->now_ms();
+    v.occ_integral += (double)v.used * (t - v.last_t);
+    v.last_t = t;
     // 1. the batch: rebuilt when its membership changed, then kPlies plies
     //    queued without a host wait
-    int na = (int)active.size();
+    int na = (int)v.active.size();
     if (na && changed) {
       std::vector<JobDev *> ptrs;
       std::vector<const JobDev *> hj;
       std::vector<const Mat *> mats;
       std::vector<Pair> pairs;
       for (int i = 0; i < na; i++) {
-        tbg_ctx *c = s->jobs[active[i]].ctx;
+        tbg_ctx *c = v.J(v.active[i].first).sl[v.active[i].second].ctx;
         ptrs.push_back(c->dj);
         hj.push_back(&c->hj);
         mats.push_back(&c->h);
@@ -2681,7 +2987,7 @@ int tbg_server_run(tbg_server *s, int form, tbg_server_stats *stats)
         pairs.push_back({ i, 1 });
       }
       HIPCHK(hipMemcpyAsync(dptr, ptrs.data(), sizeof(JobDev *) * na, hipMemcpyHostToDevice, s->st));
-      unsigned f[4] = { 0, 1, 0, 0 };  // a new job may be at ply 1
+      unsigned f[4] = { 0, 1, 0, 0 };  // a new slice may be at ply 1
       HIPCHK(hipMemcpyAsync(dflags, f, sizeof f, hipMemcpyHostToDevice, s->st));
       r = build_launchset(L, hj, mats, pairs, 0, s->geom, s->st);
       if (r != TBG_OK) break;
@@ -2690,114 +2996,145 @@ int tbg_server_run(tbg_server *s, int form, tbg_server_stats *stats)
     if (na) {
       for (int k = 0; k < kPlies && r == TBG_OK; k++) {
         r = launch_ply(nullptr, dptr, na, L, dflags, s->st, true, ddone);
-        plies++;
+        v.plies++;
       }
       if (r != TBG_OK) break;
       HIPCHK(hipMemcpyAsync(hflags, dflags, 4 * sizeof(unsigned), hipMemcpyDeviceToHost, s->st));
       HIPCHK(hipMemcpyAsync(hdone, ddone, sizeof(unsigned) * na, hipMemcpyDeviceToHost, s->st));
       HIPCHK(hipEventRecord(evb, s->st));
     }
-    // 2. host work while those plies run: finished spills, admission (set-up
-    //    and init queued on the setup stream), jobs whose set-up is done
-    for (size_t i = 0; i < spilling.size();) {
-      SrvJob &j = s->jobs[spilling[i]];
-      if (hipEventQuery(j.ev) == hipSuccess) {
-        j.res.done_ms = now_ms();
-        used -= j.res.device_bytes;
-        tbg_destroy(j.ctx);
-        j.ctx = nullptr;
-        j.state = 3;
-        done++;
-        spilling.erase(spilling.begin() + i);
-      } else {
-        i++;
+    // 2. host work while those plies run: landed codes, admission (set-up
+    //    and init queued on the set-up stream)
+    for (size_t i = 0; i < v.spilling.size();) {
+      auto pr = v.spilling[i];
+      SrvSlice &q = v.J(pr.first).sl[pr.second];
+      if (hipEventQuery(q.ev) != hipSuccess) { i++; continue; }
+      v.used -= q.ctx->bytes;
+      tbg_destroy(q.ctx);
+      q.ctx = nullptr;
+      q.landed = true;
+      v.spilling.erase(v.spilling.begin() + i);
+      if (--v.J(pr.first).left == 0) {
+        v.done++;
+        v.complete(pr.first, TBG_OK);
       }
     }
-    while (next < pending.size() && (int)(active.size() + joining.size()) < max_active) {
-      SrvJob &j = s->jobs[pending[next]];
-      if (j.spec.arrival_ms > now_ms()) break;
-      int err;
-      uint64_t need = estimate_bytes(j.material.c_str(), &err);
-      if (need > s->budget) {
-        j.res.status = TBG_E_BUDGET;
-        j.res.admit_ms = j.res.done_ms = now_ms();
-        j.state = 3;
-        rejected++;
-        next++;
+    int blocked = -1;  // an arrived job that waits for memory
+    while (!v.waiting.empty()) {
+      int ji = v.waiting.front();
+      SrvJob &j = v.J(ji);
+      if (j.spec.arrival_ms > s->now_ms()) break;
+      if (j.bytes > s->budget) {
+        v.waiting.erase(v.waiting.begin());
+        v.rejected++;
+        v.complete(ji, TBG_E_BUDGET);
         continue;
       }
-      if (used + need > s->budget) break;  // wait for memory
-      tbg_ctx *c = nullptr;
-      r = create_common(&c, j.material.c_str(), 1, j.spec.slice, j.spec.seed, true, s->setup);
+      int ns = (int)j.sl.size();
+      bool slots = v.plain ? v.active.empty() && v.joining.empty() && v.preempted.empty()
+                           : v.slots_used() + ns <= v.max_slices;
+      if (!slots) break;
+      if (v.used + j.bytes > s->budget) {
+        blocked = ji;
+        break;
+      }
+      v.waiting.erase(v.waiting.begin());
+      r = v.admit(ji);
       if (r != TBG_OK) break;
-      j.ctx = c;
-      r = tbg_resolve_captures(c, nullptr, 0);
-      if (r == TBG_OK) r = run_init_async(c);
-      if (r != TBG_OK) break;
-      HIPCHK(hipEventCreateWithFlags(&c->ready, hipEventDisableTiming));
-      HIPCHK(hipEventRecord(c->ready, s->setup));
-      j.res.device_bytes = c->bytes;
-      j.res.positions = c->h.ii.size;
-      j.res.admit_ms = now_ms();
-      used += c->bytes;
-      peak = used > peak ? used : peak;
-      j.state = 1;
-      joining.push_back(pending[next]);
-      next++;
     }
     if (r != TBG_OK) break;
-    // 3. wait: for the plies, else for a set-up or a spill, else for the
-    //    next arrival
+    // preempted jobs come back when they fit and no arrived job is blocked
+    while (!v.plain && blocked < 0 && !v.preempted.empty()) {
+      int ji = v.preempted.front();
+      uint64_t need = 0;
+      for (auto &q : v.J(ji).sl)
+        if (!q.done) need += q.ctx->bytes;
+      int ns = 0;
+      for (auto &q : v.J(ji).sl) ns += !q.done;
+      if (v.used + need > s->budget || v.slots_used() + ns > v.max_slices) break;
+      v.preempted.erase(v.preempted.begin());
+      r = v.restore(ji);
+      if (r != TBG_OK) break;
+    }
+    if (r != TBG_OK) break;
+    // 3. wait: for the plies, else for a set-up or a copy, else for the next
+    //    arrival or submission
     if (na) {
       HIPCHK(hipEventSynchronize(evb));
-    } else if (!joining.empty()) {
-      HIPCHK(hipEventSynchronize(s->jobs[joining.front()].ctx->ready));
-    } else if (!spilling.empty()) {
+    } else if (!v.joining.empty()) {
+      auto pr = v.joining.front();
+      HIPCHK(hipEventSynchronize(v.J(pr.first).sl[pr.second].ctx->ready));
+    } else if (!v.spilling.empty()) {
       HIPCHK(hipStreamSynchronize(s->side));
-    } else if (next < pending.size()) {
-      double wait = s->jobs[pending[next]].spec.arrival_ms - now_ms();
-      if (wait > 0) std::this_thread::sleep_for(std::chrono::microseconds((int64_t)(wait * 1000)));
+    } else if (blocked < 0) {
+      std::unique_lock<std::mutex> lk(s->mu);
+      double wait = 50.0;  // look again for submissions at least this often
+      if (!v.waiting.empty())
+        wait = std::min(wait, s->jobs[v.waiting.front()].spec.arrival_ms - s->now_ms());
+      if (wait > 0 && seen == s->jobs.size() && !s->closed)
+        s->cv.wait_for(lk, std::chrono::microseconds((int64_t)(wait * 1000)));
+      else if (wait > 0 && !v.waiting.empty())
+        std::this_thread::sleep_for(std::chrono::microseconds((int64_t)(wait * 1000)));
     }
-    // 4. finished jobs leave the batch; their codes go to pinned host memory
+    // 4. finished slices leave the batch; their codes go to pinned memory
     if (na && hflags[0]) {
-      std::vector<int> keep;
+      std::vector<std::pair<int, int>> keep;
       for (int i = 0; i < na && r == TBG_OK; i++) {
-        SrvJob &j = s->jobs[active[i]];
-        if (!hdone[i]) { keep.push_back(active[i]); continue; }
-        tbg_ctx *c = j.ctx;
+        auto pr = v.active[i];
+        if (!hdone[i]) { keep.push_back(pr); continue; }
+        SrvJob &j = v.J(pr.first);
+        SrvSlice &q = j.sl[pr.second];
+        tbg_ctx *c = q.ctx;
         r = finish_job(c, s->st);
         if (r == TBG_OK) r = read_init_counts(c, s->st);
         if (r != TBG_OK) break;
-        j.hist = c->hist;
-        j.counts = c->counts;
-        j.res.nplies = c->counts.nplies;
-        j.res.half_steps = executed_half_steps(c);
-        psp += (double)j.res.half_steps * (double)c->h.ii.size / 2.0;
+        q.hist = c->hist;
+        q.counts = c->counts;
+        j.res.nplies = std::max(j.res.nplies, c->counts.nplies);
+        uint64_t hs = executed_half_steps(c);
+        j.res.half_steps += hs;
+        v.psp += (double)hs * (double)c->h.ii.size / 2.0;
         uint64_t n = c->h.ii.size;
-        for (int stm = 0; stm < 2; stm++) {
-          if (!j.pinned[stm]) HIPCHK(hipHostMalloc((void **)&j.pinned[stm], n * 2, hipHostMallocDefault));
-          HIPCHK(hipMemcpyAsync(j.pinned[stm], c->codes[stm], n * 2, hipMemcpyDeviceToHost, s->side));
-        }
-        if (!j.ev) HIPCHK(hipEventCreateWithFlags(&j.ev, hipEventDisableTiming));
-        HIPCHK(hipEventRecord(j.ev, s->side));
-        j.state = 2;
-        spilling.push_back(active[i]);
+        for (int stm = 0; stm < 2; stm++)
+          HIPCHK(hipMemcpyAsync(q.pinned[stm], c->codes[stm], n * 2, hipMemcpyDeviceToHost, s->side));
+        if (!q.ev) HIPCHK(hipEventCreateWithFlags(&q.ev, hipEventDisableTiming));
+        HIPCHK(hipEventRecord(q.ev, s->side));
+        q.done = true;
+        v.spilling.push_back(pr);
       }
       if (r != TBG_OK) break;
-      active = keep;
+      v.active = keep;
       changed = true;
     }
-    // 5. jobs whose set-up and init completed join the batch
-    for (size_t i = 0; i < joining.size();) {
-      if (hipEventQuery(s->jobs[joining[i]].ctx->ready) == hipSuccess) {
-        active.push_back(joining[i]);
-        joining.erase(joining.begin() + i);
+    // 5. a blocked job preempts larger admitted jobs, the batch being idle
+    if (blocked >= 0 && !v.plain) {
+      std::vector<int> vic = v.victims(blocked);
+      if (!vic.empty()) {
+        for (int ji : vic) {
+          r = v.preempt(ji);
+          if (r != TBG_OK) break;
+        }
+        if (r != TBG_OK) break;
+        changed = true;
+        v.waiting.erase(std::find(v.waiting.begin(), v.waiting.end(), blocked));
+        r = v.admit(blocked);
+        if (r != TBG_OK) break;
+      }
+    }
+    // 6. slices whose set-up (or restore) completed join the batch (the
+    //    plain server: one slice at a time)
+    for (size_t i = 0; i < v.joining.size();) {
+      if (v.plain && !v.active.empty()) break;
+      auto pr = v.joining[i];
+      if (hipEventQuery(v.J(pr.first).sl[pr.second].ctx->ready) == hipSuccess) {
+        v.active.push_back(pr);
+        v.joining.erase(v.joining.begin() + i);
         changed = true;
       } else {
         i++;
       }
     }
-    peak_active = (int)active.size() > peak_active ? (int)active.size() : peak_active;
+    v.peak_active = (int)v.active.size() > v.peak_active ? (int)v.active.size() : v.peak_active;
   }
   (void)hipStreamSynchronize(s->st);
   (void)hipStreamSynchronize(s->side);
@@ -2808,48 +3145,91 @@ int tbg_server_run(tbg_server *s, int form, tbg_server_stats *stats)
   (void)hipFree(ddone);
   (void)hipHostFree(hflags);
   (void)hipHostFree(hdone);
-  (void)hipEventDestroy(evb);
+  if (evb) (void)hipEventDestroy(evb);
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->serving = false;
+  }
   if (r != TBG_OK) return r;
   if (stats) {
-    stats->wall_ms = now_ms();
-    stats->jobs_done = done;
-    stats->jobs_rejected = rejected;
-    stats->mean_resident_bytes = last_t > 0 ? occ_integral / last_t : 0;
-    stats->peak_resident_bytes = peak;
-    stats->peak_active = peak_active;
-    stats->plies = plies;
-    stats->position_side_plies = psp;
+    stats->wall_ms = s->now_ms();
+    stats->jobs_done = v.done;
+    stats->jobs_rejected = v.rejected;
+    stats->mean_resident_bytes = v.last_t > 0 ? v.occ_integral / v.last_t : 0;
+    stats->peak_resident_bytes = v.peak;
+    stats->peak_active = v.peak_active;
+    stats->plies = v.plies;
+    stats->position_side_plies = v.psp;
+    stats->spills = v.spills;
+    stats->restores = v.restores;
+    stats->spill_ms = v.spill_ms;
   }
+  return TBG_OK;
+}
+
+int tbg_server_run(tbg_server *s, int form, tbg_server_stats *stats)
+{
+  TRY(tbg_server_close(s));
+  return tbg_server_serve(s, form, stats);
+}
+
+int tbg_job_poll(tbg_server *s, uint64_t id, tbg_job_result *out)
+{
+  if (!s || !out) return fail(TBG_E_INVAL, "null argument");
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (id >= s->jobs.size()) return fail(TBG_E_INVAL, "bad job id");
+  if (s->jobs[id].state != 3) return TBG_PENDING;
+  *out = s->jobs[id].res;
   return TBG_OK;
 }
 
 int tbg_server_result(tbg_server *s, uint64_t id, tbg_job_result *out)
 {
-  if (!s || !out || id >= s->jobs.size()) return fail(TBG_E_INVAL, "bad job id");
+  if (!s || !out) return fail(TBG_E_INVAL, "null argument");
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (id >= s->jobs.size()) return fail(TBG_E_INVAL, "bad job id");
   *out = s->jobs[id].res;
+  return TBG_OK;
+}
+
+int tbg_server_codes_part(tbg_server *s, uint64_t id, int part, int stm, uint16_t *dst, uint64_t n)
+{
+  if (!s || !dst || (stm & ~1)) return fail(TBG_E_INVAL, "bad argument");
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (id >= s->jobs.size()) return fail(TBG_E_INVAL, "bad job id");
+  SrvJob &j = s->jobs[id];
+  if (j.state != 3 || j.res.status != TBG_OK) return fail(TBG_E_STATE, "job has no result");
+  if (part < 0 || part >= (int)j.sl.size()) return fail(TBG_E_INVAL, "no such slice of the job");
+  if (n > j.res.positions) return fail(TBG_E_INVAL, "too many codes");
+  memcpy(dst, j.sl[part].pinned[stm], n * 2);
   return TBG_OK;
 }
 
 int tbg_server_codes(tbg_server *s, uint64_t id, int stm, uint16_t *dst, uint64_t n)
 {
-  if (!s || !dst || id >= s->jobs.size() || (stm & ~1)) return fail(TBG_E_INVAL, "bad argument");
-  SrvJob &j = s->jobs[id];
-  if (j.state != 3 || j.res.status != TBG_OK) return fail(TBG_E_STATE, "job has no result");
-  if (n > j.res.positions) return fail(TBG_E_INVAL, "too many codes");
-  memcpy(dst, j.pinned[stm], n * 2);
+  return tbg_server_codes_part(s, id, 0, stm, dst, n);
+}
+
+int tbg_server_lines_part(tbg_server *s, uint64_t id, int part, tbg_line_fn fn, void *user)
+{
+  if (!s || !fn) return fail(TBG_E_INVAL, "bad argument");
+  tbg_ctx tmp;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (id >= s->jobs.size()) return fail(TBG_E_INVAL, "bad job id");
+    SrvJob &j = s->jobs[id];
+    if (j.state != 3 || j.res.status != TBG_OK) return fail(TBG_E_STATE, "job has no result");
+    if (part < 0 || part >= (int)j.sl.size()) return fail(TBG_E_INVAL, "no such slice of the job");
+    tmp.counts = j.sl[part].counts;
+    tmp.hist = j.sl[part].hist;
+  }
+  emit_lines(&tmp, fn, user);
   return TBG_OK;
 }
 
 int tbg_server_lines(tbg_server *s, uint64_t id, tbg_line_fn fn, void *user)
 {
-  if (!s || !fn || id >= s->jobs.size()) return fail(TBG_E_INVAL, "bad argument");
-  SrvJob &j = s->jobs[id];
-  if (j.state != 3 || j.res.status != TBG_OK) return fail(TBG_E_STATE, "job has no result");
-  tbg_ctx tmp;
-  tmp.counts = j.counts;
-  tmp.hist = j.hist;
-  emit_lines(&tmp, fn, user);
-  return TBG_OK;
+  return tbg_server_lines_part(s, id, 0, fn, user);
 }
 
 }  // extern "C"
