@@ -227,50 +227,67 @@ def run_stream(args, tbg):
 
 
 def run_served(args, tbg):
-    """Config 4: seeded Poisson arrivals of king-slice jobs (one king slice
-    of a 6-piece pawnless material each, seeded hash exits), served with
-    continuous batching under an HBM budget, against the plain server (one
-    job at a time, FIFO) on the same arrival stream; p50 / p99 latency
-    (arrival -> codes on the host) and throughput."""
+    """Config 4: seeded arrivals of king-slice jobs of 6-piece pawnless
+    materials (1 to --max-slices king slices each, seeded hash exits),
+    served with continuous batching under an HBM budget, against the plain
+    server (one slice at a time, FIFO).  The job list is drawn until its
+    admission estimates reach --fill of the budget, so that the capacity run
+    (every job queued at once) loads HBM to about that fraction; latency
+    (arrival -> codes on the host, p50 / p99) is measured at seeded Poisson
+    arrivals of 70% of the batched capacity, and once more under a budget
+    that forces preemption to pinned host memory."""
     import numpy as np
     pieces = [int(x) for x in args.served_pieces.split(",")]
     mats = sum((pawnless_materials(p - 2) for p in pieces), [])
     rng = np.random.default_rng(args.seed)
-    jobs = [(mats[int(rng.integers(len(mats)))], int(rng.integers(0, 441)),
-             int(rng.integers(1, 1 << 62))) for _ in range(args.jobs)]
     budget = int(args.budget_gb * (1 << 30))
+    sizes = {}
+    for m in mats:
+        t = tbg.Table(m, synth_slice=0, seed=1, keep_codes=True)
+        sizes[m] = (t.info.device_bytes, t.size)
+        t.close()
+    jobs, total = [], 0
+    while total < args.fill * budget and len(jobs) < args.jobs:
+        m = mats[int(rng.integers(len(mats)))]
+        k = int(rng.integers(1, args.max_slices + 1))
+        jobs.append((m, int(rng.integers(0, 441)), int(rng.integers(1, 1 << 62)), k))
+        total += sizes[m][0] * k
+    positions = [sizes[m][1] * k for m, _, _, k in jobs]
+    print(f"served: {len(jobs)} jobs, {sum(k for *_, k in jobs)} slices, "
+          f"{min(positions):.3g}-{max(positions):.3g} positions per job, "
+          f"{total / 2**30:.1f} GiB of job state", file=sys.stderr, flush=True)
 
-    def serve(arrivals, form, slots):
-        srv = tbg.Server(budget, slots)
-        ids = [srv.submit(m, s, seed, a) for (m, s, seed), a in zip(jobs, arrivals)]
+    def serve(arrivals, form, slots, bud):
+        srv = tbg.Server(bud, slots)
+        ids = [srv.submit(m, s, seed, a, k) for (m, s, seed, k), a in zip(jobs, arrivals)]
         st = srv.run(form)
         res = [srv.result(i) for i in ids]
         srv.close()
         lat = np.array([r.done_ms - r.arrival_ms for r in res if r.status == 0])
         return st, res, lat
 
-    # capacity of each server: the whole job list arriving at once
+    def summary(st, lat):
+        return {"wall_ms": st.wall_ms, "p50_ms": float(np.percentile(lat, 50)),
+                "p99_ms": float(np.percentile(lat, 99)),
+                "mean_resident_gb": st.mean_resident_bytes / 2**30,
+                "peak_resident_gb": st.peak_resident_bytes / 2**30,
+                "peak_active_slices": st.peak_active, "plies": st.plies,
+                "jobs_done": st.jobs_done, "spills": st.spills, "restores": st.restores,
+                "spill_ms": st.spill_ms,
+                "value": st.position_side_plies / (st.wall_ms / 1000.0)}
+
     cap = {}
-    for name, form, slots in (("batched", tbg.FORM_BATCHED, args.max_active),
-                              ("plain", tbg.FORM_PLAIN, 1)):
-        st, _, lat = serve([0.0] * len(jobs), form, slots)
-        cap[name] = {"wall_ms": st.wall_ms, "jobs_per_s": len(jobs) / (st.wall_ms / 1000.0),
-                     "value": st.position_side_plies / (st.wall_ms / 1000.0),
-                     "peak_active": st.peak_active, "plies": st.plies}
+    for name, form in (("batched", tbg.FORM_BATCHED), ("plain", tbg.FORM_PLAIN)):
+        st, _, lat = serve([0.0] * len(jobs), form, args.max_active, budget)
+        cap[name] = dict(summary(st, lat), jobs_per_s=len(jobs) / (st.wall_ms / 1000.0))
         print(f"capacity {name}: {cap[name]}", file=sys.stderr, flush=True)
     lam = 0.7 * cap["batched"]["jobs_per_s"]  # offered load: 70 % of the batched capacity
-    gaps = rng.exponential(1.0 / lam, len(jobs))
-    arrivals = list(np.cumsum(gaps) * 1000.0)
+    arrivals = list(np.cumsum(rng.exponential(1.0 / lam, len(jobs))) * 1000.0)
     out_modes = {}
-    for name, form, slots in (("batched", tbg.FORM_BATCHED, args.max_active),
-                              ("plain", tbg.FORM_PLAIN, 1)):
-        st, res, lat = serve(arrivals, form, slots)
-        out_modes[name] = {
-            "wall_ms": st.wall_ms, "p50_ms": float(np.percentile(lat, 50)),
-            "p99_ms": float(np.percentile(lat, 99)),
-            "mean_resident_gb": st.mean_resident_bytes / 2**30,
-            "peak_resident_gb": st.peak_resident_bytes / 2**30, "peak_active": st.peak_active,
-            "plies": st.plies, "jobs_done": st.jobs_done}
+    for name, form, bud in (("batched", tbg.FORM_BATCHED, budget), ("plain", tbg.FORM_PLAIN, budget),
+                            ("batched_small_budget", tbg.FORM_BATCHED, budget // 64)):
+        st, _, lat = serve(arrivals, form, args.max_active, bud)
+        out_modes[name] = dict(summary(st, lat), budget_gb=bud / 2**30)
         print(f"{name} at {lam:.1f} jobs/s: {out_modes[name]}", file=sys.stderr, flush=True)
     b, p = out_modes["batched"], out_modes["plain"]
     out = {
@@ -278,13 +295,16 @@ def run_served(args, tbg):
         "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": cap["batched"]["wall_ms"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
         "data": "synthetic",
-        "config": {"workload": f"served: {len(jobs)} king-slice jobs of "
-                               f"{'/'.join(map(str, pieces))}-piece pawnless materials "
-                               f"with seeded hash exits; value = capacity (all jobs queued at "
-                               f"once); latency at seeded Poisson arrivals of 70% of the batched "
-                               f"capacity ({lam:.1f} jobs/s); {args.budget_gb:g} GB budget, "
-                               f"{args.max_active} slots", "seed": args.seed,
-                   "lambda_jobs_per_s": lam, "budget_gb": args.budget_gb, "pieces": pieces},
+        "config": {"workload": f"served: {len(jobs)} jobs of 1-{args.max_slices} king slices of "
+                               f"{'/'.join(map(str, pieces))}-piece pawnless materials with seeded "
+                               f"hash exits ({min(positions):.3g}-{max(positions):.3g} positions "
+                               f"per job, {total / 2**30:.1f} GiB of state = "
+                               f"{100 * total / budget:.0f}% of the budget); value = capacity "
+                               f"(all jobs queued at once); latency at seeded Poisson arrivals of "
+                               f"70% of the batched capacity ({lam:.1f} jobs/s); "
+                               f"{args.budget_gb:g} GB budget, {args.max_active} slots",
+                   "seed": args.seed, "lambda_jobs_per_s": lam, "budget_gb": args.budget_gb,
+                   "pieces": pieces, "jobs": len(jobs), "job_state_gb": total / 2**30},
         "capacity": cap,
         "batched_over_plain": cap["batched"]["value"] / cap["plain"]["value"],
         "latency_ms": {"batched_p50": b["p50_ms"], "batched_p99": b["p99_ms"],
@@ -517,9 +537,12 @@ def main():
                     help="synth (default): one king slice of --material with seeded hash "
                          "exits (KQRBvKQRN slice 170); table: a whole material (KQRvKR)")
     ap.add_argument("--slice", type=int, default=170)
-    ap.add_argument("--jobs", type=int, default=96, help="served: number of jobs")
+    ap.add_argument("--jobs", type=int, default=1000, help="served: number of jobs, at most")
     ap.add_argument("--budget-gb", type=float, default=64.0, help="served: HBM admission budget")
-    ap.add_argument("--max-active", type=int, default=64, help="served: job slots")
+    ap.add_argument("--max-active", type=int, default=1024, help="served: slice slots")
+    ap.add_argument("--max-slices", type=int, default=8, help="served: king slices per job, at most")
+    ap.add_argument("--fill", type=float, default=0.7,
+                    help="served: job state drawn up to this fraction of the budget")
     ap.add_argument("--served-pieces", default="6",
                     help="served: piece counts of the job materials, e.g. 5,6")
     ap.add_argument("--stream-plain", action="store_true",

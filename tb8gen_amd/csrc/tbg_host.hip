@@ -2634,6 +2634,7 @@ struct SrvSlice {
   tbg_counts counts;
   bool done = false;    // finished: codes copied (or copying) to pinned memory
   bool landed = false;  // those copies complete
+  bool narrow = false;  // the pinned codes are one byte each (every ply below 64)
   hipEvent_t ev = nullptr;
 };
 
@@ -3095,8 +3096,26 @@ int tbg_server_serve(tbg_server *s, int form, tbg_server_stats *stats)
         j.res.half_steps += hs;
         v.psp += (double)hs * (double)c->h.ii.size / 2.0;
         uint64_t n = c->h.ii.size;
-        for (int stm = 0; stm < 2; stm++)
-          HIPCHK(hipMemcpyAsync(q.pinned[stm], c->codes[stm], n * 2, hipMemcpyDeviceToHost, s->side));
+        // every ply below 64: the codes fit a byte, narrowed on the device
+        // into the (no longer needed) plane arena, one side after the other
+        // (the side stream orders each narrowing after the previous copy)
+        int maxn = 0;
+        for (int pn = 1; pn < c->max_ply; pn++)
+          for (int i2 = 0; i2 < 4; i2++)
+            if (c->hist[(size_t)pn * 4 + i2] > 0) maxn = pn;
+        q.narrow = maxn < 64 && ctx_arena_bytes(c) >= n;
+        for (int stm = 0; stm < 2; stm++) {
+          if (q.narrow) {
+            uint8_t *tmp = reinterpret_cast<uint8_t *>(c->arena);
+            hipLaunchKernelGGL(k_codes8, dim3(cap_grid((n + 2047) / 2048, (uint64_t)g_cus * 8)),
+                               dim3(256), 0, s->side, c->codes[stm], n, tmp);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipMemcpyAsync(q.pinned[stm], tmp, n, hipMemcpyDeviceToHost, s->side));
+          } else {
+            HIPCHK(hipMemcpyAsync(q.pinned[stm], c->codes[stm], n * 2, hipMemcpyDeviceToHost,
+                                  s->side));
+          }
+        }
         if (!q.ev) HIPCHK(hipEventCreateWithFlags(&q.ev, hipEventDisableTiming));
         HIPCHK(hipEventRecord(q.ev, s->side));
         q.done = true;
@@ -3201,7 +3220,13 @@ int tbg_server_codes_part(tbg_server *s, uint64_t id, int part, int stm, uint16_
   if (j.state != 3 || j.res.status != TBG_OK) return fail(TBG_E_STATE, "job has no result");
   if (part < 0 || part >= (int)j.sl.size()) return fail(TBG_E_INVAL, "no such slice of the job");
   if (n > j.res.positions) return fail(TBG_E_INVAL, "too many codes");
-  memcpy(dst, j.sl[part].pinned[stm], n * 2);
+  const SrvSlice &q = j.sl[part];
+  if (q.narrow) {  // one byte per code on the host: widened here
+    const uint8_t *b = reinterpret_cast<const uint8_t *>(q.pinned[stm]);
+    for (uint64_t i = 0; i < n; i++) dst[i] = b[i];
+  } else {
+    memcpy(dst, q.pinned[stm], n * 2);
+  }
   return TBG_OK;
 }
 

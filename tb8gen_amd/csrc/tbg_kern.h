@@ -134,6 +134,9 @@ __global__ void k_resolve(const Mat *Mp, const SubRef *Rp, int c, int k, uint64_
 __global__ void k_code_plane(const uint16_t *codes, uint64_t size, uint64_t words, int nslices,
                              uint16_t code, uint64_t *out);
 
+// codes narrowed to one byte each (every ply below 64)
+__global__ void k_codes8(const uint16_t *codes, uint64_t n, uint8_t *out);
+
 // One sub-table (one side to move) of a Syzygy WDL file, the format
 // src/probe.c reads (encode, src/probe.c:603-687; set_enc_info, 725-770).
 // Header piece i stands on our slot slot[i].  The leading group (both

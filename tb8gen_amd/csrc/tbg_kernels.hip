@@ -1189,6 +1189,28 @@ __global__ __launch_bounds__(256) void k_code_plane(const uint16_t *codes, uint6
   }
 }
 
+// Finished codes narrowed to one byte each (a table whose plies are all
+// below 64 keeps n << 2 | tag below 256): the served form's copy of a
+// finished slice to host memory then moves half the bytes.  Eight codes per
+// thread: a 16-byte load and an 8-byte store.
+__global__ __launch_bounds__(256) void k_codes8(const uint16_t *__restrict__ codes, uint64_t n,
+                                                uint8_t *__restrict__ out)
+{
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x * 8;
+  for (uint64_t i = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * 8; i < n; i += stride) {
+    if (i + 8 <= n) {
+      uint4 v = *reinterpret_cast<const uint4 *>(codes + i);
+      uint32_t lo = (v.x & 0xff) | ((v.x >> 16) & 0xff) << 8 | (v.y & 0xff) << 16 |
+                    ((v.y >> 16) & 0xff) << 24;
+      uint32_t hi = (v.z & 0xff) | ((v.z >> 16) & 0xff) << 8 | (v.w & 0xff) << 16 |
+                    ((v.w >> 16) & 0xff) << 24;
+      *reinterpret_cast<uint2 *>(out + i) = make_uint2(lo, hi);
+    } else {
+      for (uint64_t k = i; k < n; k++) out[k] = (uint8_t)codes[k];
+    }
+  }
+}
+
 // Synthetic job, ply 1: does stm's king have a legal move or capture into
 // an exit (a position of another slice, valued by the seeded hash) that is
 // a loss (*w: W_1 candidate) or a win (*l: L_1 candidate) for the opponent?
