@@ -581,59 +581,12 @@ class NativeRegistry:
             out.append((name.value.decode(), Table(name.value.decode(), ptr=c.value)))
         return out
 
+    @property
+    def tables(self):
+        """{material: Table} of every resident table."""
+        return dict(self.entries())
+
     def close(self):
         if self.ptr:
             lib().tbg_registry_destroy(self.ptr)
             self.ptr = ctypes.c_void_p()
-
-
-class Registry:
-    """Generates materials in dependency order and keeps their tables
-    resident (the engine's replacement for Syzygy files)."""
-
-    def __init__(self, form=FORM_BATCHED):
-        self.form = form
-        self.tables = {}
-
-    def get(self, material, form=None):
-        if material in self.tables:
-            return self.tables[material]
-        for sub in sub_materials(material):
-            self.get(sub, form)
-        t = Table(material)
-        t.resolve(list(self.tables.values()))
-        t.generate(self.form if form is None else form)
-        self.tables[material] = t
-        return t
-
-    def get_many(self, materials, form=FORM_GRAPH):
-        """Generate materials level by level (every smaller material first),
-        each level as one batch."""
-        need = []
-
-        def visit(m):
-            if m in self.tables or m in need:
-                return
-            for sub in sub_materials(m):
-                visit(sub)
-            need.append(m)
-
-        for m in materials:
-            visit(m)
-        levels = {}
-        for m in need:
-            levels.setdefault(len(m) - 1, []).append(m)
-        for k in sorted(levels):
-            batch = [Table(m) for m in levels[k]]
-            subs = list(self.tables.values())
-            for t in batch:
-                t.resolve(subs)
-            generate_batch(batch, form)
-            for m, t in zip(levels[k], batch):
-                self.tables[m] = t
-        return [self.tables[m] for m in materials]
-
-    def close(self):
-        for t in self.tables.values():
-            t.close()
-        self.tables = {}

@@ -37,7 +37,7 @@ static void print_line(const char *line, void *user)
 
 int main(int argc, char **argv)
 {
-  int opt, form = TBG_FORM_BATCHED, device = 0, verbose = 0;
+  int opt, form = TBG_FORM_BATCHED, device = 0, verbose = 0, threads = 1;
   const char *wdl_dir = NULL;
   while ((opt = getopt(argc, argv, "f:d:t:vw:")) != -1) {
     switch (opt) {
@@ -47,7 +47,7 @@ int main(int argc, char **argv)
                                       : TBG_FORM_BATCHED;
       break;
     case 'd': device = atoi(optarg); break;
-    case 't': break; /* accepted for compatibility: the device replaces the thread pool */
+    case 't': threads = atoi(optarg); break; /* printed only: the device replaces the pool */
     case 'v': verbose = 1; break;
     case 'w': wdl_dir = optarg; break;
     default: return 2;
@@ -59,6 +59,8 @@ int main(int argc, char **argv)
   }
   int r = tbg_init(device);
   if (r) die("tbg_init", r);
+  /* the reference's first line (tb8gen.c:1322-1324) */
+  printf("number of threads = %d\n", threads > 1 ? threads : 1);
   struct timespec t0, t1;
   clock_gettime(CLOCK_MONOTONIC, &t0);
   /* the material and every smaller one its captures reach, smallest first,

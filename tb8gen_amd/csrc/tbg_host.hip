@@ -306,6 +306,7 @@ struct tbg_ctx {
   long long *dhist = nullptr;
   std::vector<long long> hist;
   int max_ply = 4096;
+  bool ply_cap_asked = false;  // a bounded run (tune trials) may stop at max_ply
   // a server job: memory from the stream-ordered pool of the server's setup
   // stream (which it shares, so does not own), set up without host waits
   bool pooled = false;
@@ -898,6 +899,11 @@ int finish_job(tbg_ctx *c, hipStream_t st = nullptr)
   HIPCHK(hipMemcpyAsync(c->hist.data(), c->dhist, sizeof(long long) * 4 * c->max_ply,
                         hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
+  // the reference's loop has no ply cap: reaching it with moves still to
+  // resolve would leave a truncated table
+  if (c->hj.done == 1 && c->hj.more && c->hj.n >= c->max_ply && !c->ply_cap_asked)
+    return fail(TBG_E_STATE, "%s: ply limit %d reached with positions still resolving",
+                c->name, c->max_ply);
   int last = 0;
   for (int n = 1; n < c->max_ply; n++)
     for (int i = 0; i < 4; i++)
@@ -2114,9 +2120,11 @@ int tune_trial(tbg_ctx *c, const tbg_geometry &g, int max_plies, double *ms, dou
   TRY(tbg_set_profiling(c, 1));
   int saved = c->max_ply;
   if (max_plies > 0 && max_plies + 1 < saved) c->max_ply = max_plies + 1;
+  c->ply_cap_asked = true;
   tbg_ctx *one[1] = { c };
   int r = tbg_generate_batch(one, 1, TBG_FORM_BATCHED, nullptr);
   c->max_ply = saved;
+  c->ply_cap_asked = false;
   if (r != TBG_OK) return r;
   flush_timing(c);
   *total = 0;

@@ -116,6 +116,9 @@ int io_read(const char *path, void *dst, uint64_t bytes, int threads, std::strin
   int rc = 0;
   size_t bound = ZSTD_compressBound(kChunk);
   std::vector<Frame> fr;
+  // frames cover disjoint whole chunks: each offset is a multiple of kChunk,
+  // seen once, and its frame holds exactly that chunk's bytes
+  std::vector<uint8_t> seen((bytes + kChunk - 1) / kChunk, 0);
   while (left && !rc) {
     // a batch of frames from the file, then decompressed in parallel
     fr.clear();
@@ -126,12 +129,13 @@ int io_read(const char *path, void *dst, uint64_t bytes, int threads, std::strin
         rc = -1;
         break;
       }
-      if (h[1] > kChunk || h[1] == 0 || h[0] > bound || h[2] > bytes || h[1] > bytes - h[2] ||
-          h[1] > left) {
+      if (h[1] > kChunk || h[1] == 0 || h[0] > bound || h[2] >= bytes || h[2] % kChunk ||
+          h[1] != std::min<uint64_t>(kChunk, bytes - h[2]) || h[1] > left || seen[h[2] / kChunk]) {
         *err = std::string(path) + ": bad frame header";
         rc = -1;
         break;
       }
+      seen[h[2] / kChunk] = 1;
       Frame f;
       f.csize = h[0];
       f.chunk = h[1];

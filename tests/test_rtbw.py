@@ -248,7 +248,7 @@ def test_colour_flip_probe(tmp_path):
 def registry():
     import tb8gen_amd as tbg
     print("registry: batched form", flush=True)
-    r = tbg.Registry(tbg.FORM_BATCHED)
+    r = tbg.NativeRegistry(tbg.FORM_BATCHED)
     yield r
     r.close()
 

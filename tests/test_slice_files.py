@@ -95,3 +95,36 @@ def test_damaged_files_are_errors(tmp_path):
     assert L.tbg_file_read(str(p).encode(), ptr, size - 1) == -8
     assert L.tbg_file_read(str(tmp_path / "none").encode(), ptr, size) == -8
     assert L.tbg_strerror(-8)
+
+
+def test_duplicated_and_misplaced_frames_are_errors(tmp_path):
+    """Frames must cover disjoint whole chunks: a file with the first frame
+    twice (and the last chunk missing), or with a frame moved off its chunk
+    boundary, is rejected instead of read with part of the slice unwritten
+    or decompressed twice into the same bytes."""
+    size = 3 * CHUNK
+    data = bitmap_like(size, np.random.default_rng(4))
+    p = tmp_path / "s"
+    tbg.file_write(p, data)
+    fr = frames(p)
+    L = tbg.lib()
+    buf = np.zeros(size, dtype=np.uint8)
+    ptr = buf.ctypes.data_as(ctypes.c_void_p)
+
+    def write(name, frs):
+        q = tmp_path / name
+        with open(q, "wb") as f:
+            for csize, chunk, off, payload in frs:
+                f.write(struct.pack("<QQQ", csize, chunk, off) + payload)
+        return str(q).encode()
+
+    # every header alone is in bounds; the byte counts add up to the size
+    assert L.tbg_file_read(write("dup", [fr[0], fr[1], fr[0]]), ptr, size) == -8
+    assert L.tbg_file_read(write("ok", [fr[2], fr[0], fr[1]]), ptr, size) == 0
+    assert np.array_equal(buf, data)
+    csize, chunk, off, payload = fr[1]
+    assert L.tbg_file_read(write("mis", [fr[0], (csize, chunk, off + 8, payload), fr[2]]),
+                           ptr, size) == -8
+    # a short chunk inside the slice (only the last may be short)
+    assert L.tbg_file_read(write("short", [fr[0], (csize, chunk - 8, off, payload), fr[2]]),
+                           ptr, size) == -8
