@@ -360,6 +360,15 @@ typedef struct tbg_timing {
 int tbg_get_timing(const tbg_ctx *ctx, tbg_timing *t);
 int tbg_set_profiling(tbg_ctx *ctx, int on);
 
+/* King-slice numbering (host only, no device needed).  tbg_king_slice:
+ * the slice of white king wk and black king bk after the mirror and a1-h8
+ * flip, as KKMap[wk][bk] (src/tb8gen.c:38, 69-89); -1 for squares outside
+ * 0..63, equal or adjacent kings.  tbg_slice_kings: the two squares that
+ * number slice s (0 <= s < 462), as KKSquare[s] (src/probe.c:524-538);
+ * TBG_E_INVAL otherwise.  Slice s names the files of a slice-file tree. */
+int tbg_king_slice(int wk, int bk);
+int tbg_slice_kings(int s, int *wk, int *bk);
+
 /* Self-test of the index, symmetry and attack primitives compiled for the
  * host (no device needed).  0 on success; msg receives a description. */
 int tbg_host_selftest(char *msg, int len);

@@ -34,7 +34,7 @@ ABI_SYMBOLS = [
     "tbg_emit_lines", "tbg_generate_batch_noalloc", "tbg_server_create", "tbg_server_destroy", "tbg_server_submit",
     "tbg_server_run", "tbg_server_result", "tbg_server_codes", "tbg_server_lines",
     "tbg_server_serve", "tbg_server_close", "tbg_job_poll", "tbg_server_codes_part",
-    "tbg_server_lines_part",
+    "tbg_server_lines_part", "tbg_king_slice", "tbg_slice_kings",
     "tbg_set_geometry", "tbg_get_geometry", "tbg_autotune", "tbg_file_write", "tbg_file_read",
     "tbg_checkpoint", "tbg_restore", "tbg_continue", "tbg_export_plies", "tbg_rtbw_name",
     "tbg_write_rtbw",
@@ -166,6 +166,9 @@ def lib():
                                        ctypes.POINTER(ctypes.c_uint16), ctypes.c_uint64]
         L.tbg_server_lines.argtypes = [vp, ctypes.c_uint64, LINE_FN, vp]
         L.tbg_host_selftest.argtypes = [ctypes.c_char_p, ctypes.c_int]
+        L.tbg_king_slice.argtypes = [ctypes.c_int, ctypes.c_int]
+        L.tbg_slice_kings.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                      ctypes.POINTER(ctypes.c_int)]
         L.tbg_set_geometry.argtypes = [vp, ctypes.POINTER(Geometry)]
         L.tbg_get_geometry.argtypes = [vp, ctypes.POINTER(Geometry)]
         L.tbg_autotune.argtypes = [vp, ctypes.c_int, ctypes.c_char_p, ctypes.POINTER(Geometry),
@@ -207,6 +210,18 @@ def host_selftest():
     buf = ctypes.create_string_buffer(256)
     r = lib().tbg_host_selftest(buf, 256)
     return r, buf.value.decode()
+
+
+def king_slice(wk, bk):
+    """Slice number of white king wk and black king bk (-1: no slice)."""
+    return lib().tbg_king_slice(int(wk), int(bk))
+
+
+def slice_kings(s):
+    """(white king, black king) squares numbering slice s."""
+    wk, bk = ctypes.c_int(), ctypes.c_int()
+    check(lib().tbg_slice_kings(int(s), ctypes.byref(wk), ctypes.byref(bk)), f"slice {s}")
+    return wk.value, bk.value
 
 
 class Table:

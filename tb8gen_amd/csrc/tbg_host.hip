@@ -2263,6 +2263,33 @@ uint64_t rnd()
 
 }  // namespace
 
+namespace {
+const Tabs &numbering_tabs()
+{
+  static const Tabs T = [] {
+    Tabs t;
+    build_tabs(t);
+    return t;
+  }();
+  return T;
+}
+}  // namespace
+
+extern "C" int tbg_king_slice(int wk, int bk)
+{
+  if (wk < 0 || wk > 63 || bk < 0 || bk > 63 || wk == bk || touching(wk, bk)) return -1;
+  uint64_t q;
+  return normalize(numbering_tabs(), (uint64_t)wk | ((uint64_t)bk << 8), &q);
+}
+
+extern "C" int tbg_slice_kings(int s, int *wk, int *bk)
+{
+  if (s < 0 || s >= NSL || !wk || !bk) return fail(TBG_E_INVAL, "no such slice");
+  *wk = numbering_tabs().kksq[s][0];
+  *bk = numbering_tabs().kksq[s][1];
+  return TBG_OK;
+}
+
 extern "C" int tbg_host_selftest(char *msg, int len)
 {
 #define ST_FAIL(...)                       \

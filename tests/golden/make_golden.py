@@ -15,7 +15,11 @@
    codes.  The 7-piece KQRBvKQR slice 170 (776,520,240 positions per side)
    exercises the code that only slices above 2^22 bitmap words reach.
 
-Usage: python tests/golden/make_golden.py [naive|oracle|synth] [MATERIAL ...]
+4. kkidx_probe_c.json -- the values of the reference's KKIdx table
+   (src/probe.c:441-522, the king-slice numbering), read from the
+   reference's source text as data: 10 rows of 64 entries.
+
+Usage: python tests/golden/make_golden.py [naive|oracle|synth|kkidx] [MATERIAL ...]
        (synth: MATERIAL:SLICE:SEED)
 """
 import hashlib
@@ -107,6 +111,27 @@ def synth_fixture(spec):
                    "chunk": CHUNK, "lines": lines, "crc32": crc}, f, separators=(",", ":"))
 
 
+def kkidx_fixture(probe_c="/root/reference/src/probe.c"):
+    """The numbers of the KKIdx and InvTriangle initialisers, in order, read
+    from the reference's text as data."""
+    import re
+    with open(probe_c) as f:
+        text = f.read()
+
+    def table(decl, n):
+        start = text.index(decl)
+        end = text.index("};", start)
+        vals = [int(v) for v in re.findall(r"-?\d+", text[start + len(decl):end])]
+        assert len(vals) == n, (decl, len(vals))
+        return vals, f"src/probe.c:{text[:start].count(chr(10)) + 1}-{text[:end].count(chr(10)) + 1}"
+
+    kk, kk_src = table("KKIdx[10][64] = {", 640)
+    inv, inv_src = table("InvTriangle[10] = {", 10)
+    with open(os.path.join(HERE, "kkidx_probe_c.json"), "w") as f:
+        json.dump({"kkidx_source": kk_src, "kkidx": [kk[i * 64:(i + 1) * 64] for i in range(10)],
+                   "inv_triangle_source": inv_src, "inv_triangle": inv}, f)
+
+
 def main():
     what = sys.argv[1] if len(sys.argv) > 1 else "all"
     mats = sys.argv[2:]
@@ -120,6 +145,8 @@ def main():
         for m in mats or ORACLE:
             print("oracle", m, flush=True)
             oracle_fixture(m)
+    if what in ("all", "kkidx"):
+        kkidx_fixture()
     if what in ("all", "synth"):
         for spec in mats or SYNTH:
             print("synth", spec, flush=True)

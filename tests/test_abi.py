@@ -33,6 +33,38 @@ def test_host_selftest_of_device_primitives():
     assert r == 0, msg
 
 
+def test_king_slice_numbering_equals_the_reference_table():
+    """tbg_king_slice / tbg_slice_kings against every entry of the
+    reference's KKIdx and InvTriangle tables (src/probe.c:327-329, 441-522,
+    committed as data), and KKMap's symmetry classes over all king pairs."""
+    import json
+    with open(os.path.join(ROOT, "tests", "golden", "kkidx_probe_c.json")) as f:
+        ref = json.load(f)
+    seen = set()
+    for tri, wk in enumerate(ref["inv_triangle"]):
+        for bk, s in enumerate(ref["kkidx"][tri]):
+            if s < 0:
+                continue
+            assert tbg.king_slice(wk, bk) == s, (wk, bk)
+            assert tbg.slice_kings(s) == (wk, bk), s
+            seen.add(s)
+    assert seen == set(range(462))
+    f = lambda x: x ^ 7          # file mirror
+    r = lambda x: x ^ 56         # rank mirror
+    d = lambda x: ((x & 7) << 3) | (x >> 3)  # a1-h8 flip
+    for wk in range(64):
+        for bk in range(64):
+            s = tbg.king_slice(wk, bk)
+            touching = max(abs((wk & 7) - (bk & 7)), abs((wk >> 3) - (bk >> 3))) <= 1
+            assert (s == -1) == touching
+            if s >= 0:
+                for g in (f, r, d):
+                    assert tbg.king_slice(g(wk), g(bk)) == s
+    assert tbg.king_slice(-1, 5) == -1 and tbg.king_slice(3, 64) == -1
+    with pytest.raises(tbg.TbgError):
+        tbg.slice_kings(462)
+
+
 def test_error_codes_have_messages():
     L = tbg.lib()
     for code in (0, -1, -2, -3, -4, -5, -6, -7, -8):

@@ -1,4 +1,5 @@
 """Closed forms, invariants and known answers that pin the oracle."""
+import json
 import math
 import os
 
@@ -9,19 +10,33 @@ from oracle import oracle as O
 
 O.set_threads(min(8, os.cpu_count() or 1))
 
-# Entries of the reference's KKIdx table (src/probe.c:441-522): the slice
-# numbering, including the 21 slices with both kings on a1-h8 (441..461).
-KKIDX_VECTORS = [
-    (0, 3, 0), (0, 63, 57), (1, 0, 58), (1, 4, 59), (2, 0, 116), (2, 60, 170),
-    (3, 0, 174), (4, 0, 229), (5, 0, 284), (5, 63, 338), (6, 2, 339), (6, 18, 441),
-    (6, 63, 446), (7, 3, 366), (7, 27, 447), (8, 0, 452), (8, 1, 391), (8, 36, 453),
-    (9, 0, 457), (9, 9, 458), (9, 63, 461), (9, 55, 440), (9, 62, -1), (0, 0, -1), (6, 8, -1),
-]
+# The reference's KKIdx and InvTriangle tables (src/probe.c:441-522,
+# 327-329) as data, every entry: the slice numbering, including the 21
+# slices with both kings on a1-h8 (441..461).
+with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                       "kkidx_probe_c.json")) as _f:
+    KKIDX = json.load(_f)
 
 
-@pytest.mark.parametrize("tri,bk,want", KKIDX_VECTORS)
-def test_kkidx_vectors(tri, bk, want):
-    assert O.kkidx(tri, bk) == want
+def test_kkidx_whole_table():
+    for tri in range(10):
+        for bk in range(64):
+            assert O.kkidx(tri, bk) == KKIDX["kkidx"][tri][bk], (tri, bk)
+
+
+def test_kkmap_agrees_with_kkidx():
+    """KKMap (src/tb8gen.c:69-89) of each table entry's king pair is that
+    entry; the pairs the table leaves out are equal, touching or, with the
+    white king on a1-d4, have the black king above the diagonal."""
+    for tri, wk in enumerate(KKIDX["inv_triangle"]):
+        for bk in range(64):
+            want = KKIDX["kkidx"][tri][bk]
+            if want >= 0:
+                assert O.kkmap(wk, bk) == want, (wk, bk)
+            else:
+                touching = max(abs((wk & 7) - (bk & 7)), abs((wk >> 3) - (bk >> 3))) <= 1
+                above = (wk & 7) == (wk >> 3) and (bk & 7) < (bk >> 3)
+                assert touching or above, (wk, bk)
 
 
 def test_slices_partition_king_pairs():
