@@ -81,13 +81,14 @@ def run_steps(table, subs, form, steps, tbg):
     return secs, timing, lines
 
 
-def kernel_rooflines(tm, lines, positions, steps, traffic=None, init_bits=4):
+def kernel_rooflines(tm, lines, positions, steps, traffic=None, init_bits=10):
     """Per kernel: algorithmic bytes per launch = its bits per position x
     positions x the half-steps it executed (from the count lines: pred runs
     for every W step and every L step after ply 1, verify for L steps, sweep
-    for W steps) / its launches; init writes init_bits planes once per step
-    (illegal and mates per side, plus the four exit-candidate planes of a
-    synthetic job).  Average launch time from the HIP events; achieved GB/s
+    for W steps) / its launches; the init phase moves init_bits bits per
+    position once per step (illegal and mates per side, the four
+    exit-candidate planes of a synthetic job, and the static refutations:
+    written by the init kernel, then read and written back by k_xr_pl).  Average launch time from the HIP events; achieved GB/s
     = the two divided.  traffic: measured HBM bytes per launch
     (copy-calibrated rocprof counters) where a profile is committed."""
     lh = [l for l in lines if re.match(r"l\d+_[wb] = ", l) and not l.startswith("l0_")]
@@ -483,7 +484,7 @@ def run_synth(args, tbg):
     half_steps = tm.half_steps / steps
     psp = half_steps * n / 2.0
     mean = sum(secs) / len(secs)
-    kr = kernel_rooflines(tm, lines, n, steps, load_traffic(f"{material}_synth"), init_bits=8)
+    kr = kernel_rooflines(tm, lines, n, steps, load_traffic(f"{material}_synth"), init_bits=14)
     dom = max(kr, key=lambda k: kr[k]["ms"])
     roof = {k: v for k, v in kr[dom].items() if k not in ("ms", "half_steps")}
     table.close()

@@ -35,8 +35,19 @@ thread_local char g_errbuf[256];
 int g_device = -1;
 int g_cus = 256;
 // bitmap planes per job: wins, prevW, prevL, newW, newL and the L and W
-// steps' candidates per side
+// steps' candidates per side, and for the placement layout (pl_layout) the
+// static refutations per side
 constexpr uint64_t kPlanes = 14;
+
+// all sets single and at least 64 placements (runs of the last set's R
+// squares) per slice: the placement-parallel kernels apply
+bool pl_layout(const Mat &M)
+{
+  uint64_t R = 62 - (uint64_t)(M.ii.numsets - 1);
+  return M.all_single && M.ii.numsets >= 2 && M.ii.size / R >= 64;
+}
+
+uint64_t planes_of(const Mat &M) { return kPlanes + (pl_layout(M) ? 2 : 0); }
 Tabs g_htabs;
 // geometry and scatter given to new contexts and servers (tbg_set_geometry(NULL, ..))
 Geom g_geom = { 256, 4096, { 3, 16, 16, 16 } };
@@ -295,6 +306,8 @@ struct tbg_ctx {
   uint64_t *arena = nullptr, *subarena = nullptr;
   uint64_t *wins[2], *prevW[2], *prevL[2], *newW[2], *newL[2];  // canonical planes
   uint64_t *cand[2], *candW[2];
+  uint64_t *xr[2] = { nullptr, nullptr };  // static refutations (pl_layout only)
+  bool xr_ok = false;                      // worked out since the last resolve and init
   uint64_t *sub[2][4];
   uint16_t *codes[2] = { nullptr, nullptr };
   unsigned long long *dcnt = nullptr;  // init counts
@@ -453,6 +466,7 @@ void reset_job(tbg_ctx *c)
     J.newL[s] = c->newL[s];
     J.cand[s] = c->cand[s];
     J.candW[s] = c->candW[s];
+    J.xr[s] = c->xr_ok ? c->xr[s] : nullptr;
     J.codes[s] = c->codes[s];
     for (int k = 0; k < 4; k++) J.sub[s][k] = c->sub[s][k];
   }
@@ -513,6 +527,25 @@ void dfree(tbg_ctx *c, void *p)
   else (void)hipFree(p);
 }
 
+// the canonical planes in their allocation order within the arena
+void assign_planes(tbg_ctx *c)
+{
+  uint64_t *p = c->arena;
+  for (int s = 0; s < 2; s++) {
+    c->wins[s] = p; p += c->pw;
+    c->prevW[s] = p; p += c->pw;
+    c->prevL[s] = p; p += c->pw;
+    c->newW[s] = p; p += c->pw;
+    c->newL[s] = p; p += c->pw;
+    c->cand[s] = p; p += c->pw;
+    c->candW[s] = p; p += c->pw;
+  }
+  for (int s = 0; s < 2; s++) {
+    c->xr[s] = pl_layout(c->h) ? p : nullptr;
+    if (c->xr[s]) p += c->pw;
+  }
+}
+
 // ext: the stream of a pooled context (server jobs), else nullptr
 int ctx_alloc(tbg_ctx *c, bool with_codes, hipStream_t ext = nullptr)
 {
@@ -527,24 +560,16 @@ int ctx_alloc(tbg_ctx *c, bool with_codes, hipStream_t ext = nullptr)
     HIPCHK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
   }
   size_t pb = c->pw * 8;
-  TRY(dalloc(c, (void **)&c->arena, pb * kPlanes));
-  c->bytes += pb * kPlanes;
-  uint64_t *p = c->arena;
-  for (int s = 0; s < 2; s++) {
-    c->wins[s] = p; p += c->pw;
-    c->prevW[s] = p; p += c->pw;
-    c->prevL[s] = p; p += c->pw;
-    c->newW[s] = p; p += c->pw;
-    c->newL[s] = p; p += c->pw;
-    c->cand[s] = p; p += c->pw;
-    c->candW[s] = p; p += c->pw;
-  }
-  HIPCHK(hipMemsetAsync(c->arena, 0, pb * kPlanes, c->st));
+  uint64_t np = planes_of(M);
+  TRY(dalloc(c, (void **)&c->arena, pb * np));
+  c->bytes += pb * np;
+  assign_planes(c);
+  HIPCHK(hipMemsetAsync(c->arena, 0, pb * np, c->st));
   uint64_t sw = 4 * c->nsl * (M.ctotal[0] + M.ctotal[1]);
   TRY(dalloc(c, (void **)&c->subarena, sw * 8 + 64));
   c->bytes += sw * 8;
   HIPCHK(hipMemsetAsync(c->subarena, 0, sw * 8 + 64, c->st));
-  p = c->subarena;
+  uint64_t *p = c->subarena;
   for (int col = 0; col < 2; col++)
     for (int k = 0; k < 4; k++) {
       c->sub[col][k] = p;
@@ -758,15 +783,23 @@ int run_init_async(tbg_ctx *c)
   HIPCHK(hipMemsetAsync(c->dcnt, 0, 4 * sizeof(unsigned long long), c->st));
   {
     Timed t(c, CAT_INIT);
-    // all sets single and at least 64 placements (runs of the last set's
-    // squares) per slice: one placement per lane
+    // one placement per lane (pl_layout); with the capture exits resolved,
+    // the static refutations of the verification after it
     uint64_t R = 62 - (uint64_t)(M.ii.numsets - 1);
-    if (M.all_single && M.ii.numsets >= 2 && M.ii.size / R >= 64) {
+    c->xr_ok = false;
+    if (pl_layout(M)) {
       uint64_t chunks = (uint64_t)ns * ((M.words + R - 1) / R);
-      hipLaunchKernelGGL(k_init_pl, dim3(cap_grid((chunks + 3) / 4, (uint64_t)g_cus * 16)),
-                         dim3(256), 0, c->st, c->dM, c->wins[0], c->wins[1], c->prevL[0],
-                         c->prevL[1], c->codes[0], c->codes[1], c->prevW[0], c->prevW[1],
-                         c->newW[0], c->newW[1], s0, ns, c->dcnt);
+      dim3 grid(cap_grid((chunks + 3) / 4, (uint64_t)g_cus * 16));
+      hipLaunchKernelGGL(k_init_pl, grid, dim3(256), 0, c->st, c->dM, c->wins[0], c->wins[1],
+                         c->prevL[0], c->prevL[1], c->codes[0], c->codes[1], c->prevW[0],
+                         c->prevW[1], c->newW[0], c->newW[1], c->xr[0], c->xr[1], s0, ns,
+                         c->dcnt);
+      if (c->resolved && c->xr[0]) {
+        HIPCHK(hipGetLastError());
+        hipLaunchKernelGGL(k_xr_pl, grid, dim3(256), 0, c->st, c->dM, c->sub[0][TBG_SUB_WIN],
+                           c->sub[1][TBG_SUB_WIN], c->xr[0], c->xr[1], s0, ns);
+        c->xr_ok = true;
+      }
     } else {
       uint64_t chunks = ((uint64_t)ns * M.words + 4 * INIT_CHUNK - 1) / (4 * INIT_CHUNK);
       hipLaunchKernelGGL(k_init, dim3(cap_grid(chunks, (uint64_t)g_cus * 16)),
@@ -1263,6 +1296,11 @@ int tbg_resolve_captures(tbg_ctx *c, tbg_ctx *const *subs, int nsubs)
   }
   if (!c->pooled) HIPCHK(hipStreamSynchronize(c->st));  // a pooled job is waited for by its event
   c->resolved = 1;
+  if (c->xr_ok) {  // worked out from the earlier capture exits: verify every move
+    c->xr_ok = false;
+    c->hj.xr[0] = c->hj.xr[1] = nullptr;
+    if (c->inited) TRY(upload_job(c));
+  }
   return TBG_OK;
 }
 
@@ -1665,6 +1703,7 @@ int tbg_restore(tbg_ctx *c, const char *root)
       (c->h.synth && seed != c->h.seed))
     return fail(TBG_E_INVAL, "checkpoint of %s slice %d does not match this context", mat, slice);
   HIPCHK(hipStreamSynchronize(c->st));
+  c->xr_ok = false;  // not in the files: the restored job verifies every move
   reset_job(c);  // the canonical planes: the files go to wins, prevW, prevL
   JobDev &J = c->hj;
   uint64_t pb = c->pw * 8;
@@ -2570,7 +2609,7 @@ struct CtxImage {
   uint16_t *codes[2] = { nullptr, nullptr };
 };
 
-size_t ctx_arena_bytes(const tbg_ctx *c) { return c->pw * 8 * kPlanes; }
+size_t ctx_arena_bytes(const tbg_ctx *c) { return c->pw * 8 * planes_of(c->h); }
 size_t ctx_sub_bytes(const tbg_ctx *c)
 {
   return 4 * c->nsl * (c->h.ctotal[0] + c->h.ctotal[1]) * 8 + 64;
@@ -2614,7 +2653,7 @@ template <class X> X *rebase(X *p, const void *old, size_t n, void *now)
 
 int ctx_restore_image(tbg_ctx *c, CtxImage &im)
 {
-  size_t ab = c->pw * 8 * kPlanes;
+  size_t ab = ctx_arena_bytes(c);
   size_t sb = 4 * c->nsl * (c->h.ctotal[0] + c->h.ctotal[1]) * 8 + 64;
   size_t cb = im.codes[0] ? c->nsl * c->h.ii.size * 2 : 0;
   TRY(dalloc(c, (void **)&c->arena, ab));
@@ -2627,17 +2666,8 @@ int ctx_restore_image(tbg_ctx *c, CtxImage &im)
     HIPCHK(hipMemcpyAsync(c->codes[s], h + ab + sb + s * cb, cb, hipMemcpyHostToDevice, c->st));
   // the canonical planes in their allocation order (ctx_alloc), and the job
   // record's (swapped) plane pointers moved onto the new allocations
-  uint64_t *p = c->arena;
-  for (int s = 0; s < 2; s++) {
-    c->wins[s] = p; p += c->pw;
-    c->prevW[s] = p; p += c->pw;
-    c->prevL[s] = p; p += c->pw;
-    c->newW[s] = p; p += c->pw;
-    c->newL[s] = p; p += c->pw;
-    c->cand[s] = p; p += c->pw;
-    c->candW[s] = p; p += c->pw;
-  }
-  p = c->subarena;
+  assign_planes(c);
+  uint64_t *p = c->subarena;
   for (int col = 0; col < 2; col++)
     for (int k = 0; k < 4; k++) {
       c->sub[col][k] = p;
@@ -2652,6 +2682,7 @@ int ctx_restore_image(tbg_ctx *c, CtxImage &im)
     J.newL[s] = rebase(J.newL[s], im.arena, ab, c->arena);
     J.cand[s] = rebase(J.cand[s], im.arena, ab, c->arena);
     J.candW[s] = rebase(J.candW[s], im.arena, ab, c->arena);
+    J.xr[s] = rebase(J.xr[s], im.arena, ab, c->arena);
     for (int k = 0; k < 4; k++) J.sub[s][k] = rebase(J.sub[s][k], im.subarena, sb, c->subarena);
     J.codes[s] = c->codes[s];
   }
@@ -2689,7 +2720,7 @@ uint64_t estimate_bytes(const char *material, int *err)
   *err = setup_mat(M, material);
   if (*err) return 0;
   uint64_t pw = M.words, sw = 4 * (M.ctotal[0] + M.ctotal[1]);
-  return pw * 8 * kPlanes + sw * 8 + 2 * M.ii.size * 2 + 65536;
+  return pw * 8 * planes_of(M) + sw * 8 + 2 * M.ii.size * 2 + 65536;
 }
 
 }  // namespace

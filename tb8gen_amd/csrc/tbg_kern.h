@@ -25,12 +25,17 @@ struct JobDev {
   uint64_t *wins[2], *prevW[2], *prevL[2], *newW[2], *newL[2];
   uint64_t *cand[2], *candW[2];  // candidates of the L and of the W step
   uint64_t *sub[2][4];
+  // static refutations of the L step (k_init_pl + k_xr_pl): a position with a
+  // capture or synthetic exit whose result is not a win of the opponent
+  // (within the 50-move rule); nullptr: not worked out, verify every move
+  uint64_t *xr[2];
   uint16_t *codes[2];
   long long *hist;            // [max_ply][4] counts per step, -1: step did not run
   unsigned long long cnt[4];  // this ply: L(W), L(B), W(W), W(B)
   int n, done;                // done: 1 finished, 2 paused at stop_at
   unsigned mask;              // steps of this ply: bit 0 L(W), 1 L(B), 2 W(W), 3 W(B)
   unsigned more;              // bit 0 more_ww, 1 more_wb, 2 more_lw, 3 more_lb
+  unsigned pull;              // bit s: side s's L step of this ply in the pull form
   int scatter;                // TBG_SCATTER_*
   int max_ply;
   int stop_at;                // > 0: pause (done = 2) on reaching this ply
@@ -124,8 +129,13 @@ __global__ void k_init(const Mat *Mp, uint64_t *winsW, uint64_t *winsB, uint64_t
 // per slice, one placement (every set's square but the last) per lane
 __global__ void k_init_pl(const Mat *Mp, uint64_t *winsW, uint64_t *winsB, uint64_t *mateW,
                           uint64_t *mateB, uint16_t *codesW, uint16_t *codesB, uint64_t *kexW0,
-                          uint64_t *kexW1, uint64_t *kexL0, uint64_t *kexL1, int s0, int ns,
-                          unsigned long long *cnt);
+                          uint64_t *kexW1, uint64_t *kexL0, uint64_t *kexL1, uint64_t *xrW,
+                          uint64_t *xrB, int s0, int ns, unsigned long long *cnt);
+// the capture part of the static refutations, OR-ed into xrW / xrB (which
+// k_init_pl cleared or filled with the exit part), from the capture-exit
+// planes sub[colour][TBG_SUB_WIN]; same launch shape as k_init_pl
+__global__ void k_xr_pl(const Mat *Mp, const uint64_t *subwinW, const uint64_t *subwinB,
+                        uint64_t *xrW, uint64_t *xrB, int s0, int ns);
 __global__ void k_resolve(const Mat *Mp, const SubRef *Rp, int c, int k, uint64_t *loss,
                           uint64_t *bloss, uint64_t *cwin, uint64_t *win, int s0, int ns);
 

@@ -323,6 +323,9 @@ __device__ __forceinline__ void mark_unmoves(const Tabs &T, const Mat &M, int j,
 constexpr int QCAP = 1024;  // queue entries per wave per round
 constexpr int VERIFY_QUICK = 4;    // bitmap reads of verification's first pass
 constexpr int VERIFY_SPLIT = 256;  // candidates per round worth two passes
+// an L step runs in the pull form when its frontier holds at least 1 /
+// PULL_DENSITY of the positions
+constexpr unsigned long long PULL_DENSITY = 16;
 // ... in slices of more positions than VERIFY_SPLIT_SIZE (tbg_kern.h): in
 // smaller tables the bitmaps stay in the caches, their reads are cheap, and
 // the second decode of the open candidates costs more than the idle lanes
@@ -701,11 +704,14 @@ __device__ __forceinline__ uint64_t king_norm(uint32_t e, uint64_t q0)
 // spent without an answer.
 // SMALL: the slice has at most VERIFY_SPLIT_SIZE positions, so quiet-move
 // targets are found with 32-bit offsets (moved_delta).
+// xr: the candidate passed the static refutations (k_xr_pl), so every
+// capture by a non-king piece and every synthetic king exit loses; only the
+// quiet moves and the king's captures within the table are read.
 template <bool SMALL>
 __device__ __forceinline__ int verify_pos(const Tabs &T, const Mat &M, int stm, int s, uint64_t idx,
                           const uint64_t *__restrict__ wins_o, const uint64_t *__restrict__ sub_o,
                           const uint32_t *kt, int budget, uint64_t &hidx, uint64_t &hD,
-                          uint64_t &hp)
+                          uint64_t &hp, bool xr)
 {
   int used = 0;  // bitmap reads so far; at budget the answer is left open (2)
   int o = stm ^ 1;
@@ -735,7 +741,7 @@ __device__ __forceinline__ int verify_pos(const Tabs &T, const Mat &M, int stm, 
     int j = M.pcs[stm][t];
     int fr = sq_get(p, j);
     uint64_t a = piece_att(T, M.pt[j], fr, occ);
-    uint64_t c = a & oppocc;
+    uint64_t c = xr ? 0 : a & oppocc;
     while (c) {
       int to = ctz64(c);
       c &= c - 1;
@@ -785,7 +791,8 @@ __device__ __forceinline__ int verify_pos(const Tabs &T, const Mat &M, int stm, 
   uint64_t kb = T.king[ks] & ~T.king[oks];
   // a synthetic job's king moves mostly leave the slice, and each such exit
   // needs a legality test: the opponent's attacks once, with the king lifted
-  uint64_t unsafe = M.synth ? T.king[oks] | side_attacks(T, M, p, occ ^ bit(ks), o, -1) : 0;
+  uint64_t unsafe =
+      M.synth && !xr ? T.king[oks] | side_attacks(T, M, p, occ ^ bit(ks), o, -1) : 0;
   uint64_t c = kb & oppocc;
   while (c) {
     int to = ctz64(c);
@@ -795,6 +802,7 @@ __device__ __forceinline__ int verify_pos(const Tabs &T, const Mat &M, int stm, 
     uint32_t e = kt[to];
     uint64_t q0 = sq_set(p, stm, to), q = king_norm(e, q0);
     int s2 = (int)(e & 511);
+    if (M.synth && s2 != M.synth_slice && xr) continue;
     if (used++ >= budget) return 2;
     if (M.synth && s2 != M.synth_slice) {
       int v = synth_exit(T, M, stm, q0, q, unsafe, ks, jj, l, s2, idx);
@@ -823,8 +831,10 @@ __device__ __forceinline__ int verify_pos(const Tabs &T, const Mat &M, int stm, 
         uint64_t q0 = sq_set(p, stm, to), q = king_norm(e, q0);
         int s2 = (int)(e & 511);
         if (M.synth && s2 != M.synth_slice) {
-          int v = synth_exit(T, M, stm, q0, q, unsafe, ks, -1, 0, s2, idx);
-          if (v != 3 && v != 2) return 0;
+          if (!xr) {
+            int v = synth_exit(T, M, stm, q0, q, unsafe, ks, -1, 0, s2, idx);
+            if (v != 3 && v != 2) return 0;
+          }
         } else {
           uint64_t i2 = !M.all_single ? sq_to_idx(T, M.ii, q)
                         : q == q0     ? king_moved_idx(M.ii, idx, q0, ks, to)
@@ -844,6 +854,57 @@ __device__ __forceinline__ int verify_pos(const Tabs &T, const Mat &M, int stm, 
   return 1;
 }
 
+// The pull form's second condition for L_n: does some quiet move of
+// candidate idx (stm to move, slice s) reach the frontier W_{n-1}(opp)
+// (prev_o), i.e. is idx a predecessor of it (src/tb8gen.c:484-527 run
+// backwards)?  Only candidates that passed the verification ask, so this
+// is not on the hot path.  Captures and synthetic exits lead out of the
+// table and never reach it.
+__device__ bool has_hit(const Tabs &T, const Mat &M, int stm, int s, uint64_t idx,
+                        const uint64_t *__restrict__ prev_o, const uint32_t *kt)
+{
+  int o = stm ^ 1;
+  uint64_t p = kings_of(T, s);
+  uint64_t occ = idx_to_sq(T, M.ii, idx, &p);
+  int ks = sq_get(p, stm), oks = sq_get(p, o);
+  const uint64_t *prev_s = prev_o + phys(M, s) * M.words;
+  for (int t = 0; t < MAXP - 1; t++) {
+    if (t >= M.nk[stm]) break;
+    int j = M.pcs[stm][t];
+    int fr = sq_get(p, j);
+    uint64_t qm = piece_att(T, M.pt[j], fr, occ) & ~occ;
+    uint64_t ob = 0;
+    uint32_t ra = 0;
+    Later LS = {};
+    if (M.all_single) {
+      ob = occ_before(p, j - 2);
+      ra = (uint32_t)(fr - popc64(ob & (bit(fr) - 1)));
+      LS = later_sets(M.ii, p, j - 2);
+    }
+    while (qm) {
+      int to = ctz64(qm);
+      qm &= qm - 1;
+      uint64_t i2 = M.all_single ? moved_idx_l(LS, idx, fr, to, ob, ra)
+                                 : sq_to_idx(T, M.ii, sq_set(p, j, to));
+      if (bm_test(prev_s, i2)) return true;
+    }
+  }
+  uint64_t kq = T.king[ks] & ~T.king[oks] & ~occ;
+  while (kq) {
+    int to = ctz64(kq);
+    kq &= kq - 1;
+    uint32_t e = kt[to];
+    uint64_t q0 = sq_set(p, stm, to), q = king_norm(e, q0);
+    int s2 = (int)(e & 511);
+    if (M.synth && s2 != M.synth_slice) continue;
+    uint64_t i2 = !M.all_single ? sq_to_idx(T, M.ii, q)
+                  : q == q0     ? king_moved_idx(M.ii, idx, q0, ks, to)
+                                : sq_to_idx_1(M.ii, q);
+    if (bm_test(prev_o + phys(M, s2) * M.words, i2)) return true;
+  }
+  return false;
+}
+
 // check_successors (src/tb8gen.c:607-661) over every (job, side): prune
 // the candidates in cand[stm] to L_n, written to newL[stm] (cand is cleared
 // for the next half-step), counting into the job and writing L_n codes.  A
@@ -856,6 +917,12 @@ __device__ __forceinline__ int verify_pos(const Tabs &T, const Mat &M, int stm, 
 // more spills.
 // SPLIT: the two-pass form for slices larger than VERIFY_SPLIT_SIZE (one
 // pass otherwise, compiled without the second pass's bookkeeping).
+// Pull form (the job's pull bit of stm, set by k_ctrl for a dense frontier
+// when the static refutations are in place): no predecessor pass ran; the
+// candidates are every position not won, not statically refuted and not in
+// L_{n-1}, and one that passes the verification must also reach the
+// frontier (has_hit).  The same L_n: a position is a predecessor of the
+// frontier exactly when one of its quiet moves reaches it.
 template <bool SPLIT>
 __global__ __launch_bounds__(256, 5) void k_verify_j(JobDev *const *__restrict__ jobs,
                                                    const Pair *__restrict__ pairs,
@@ -876,11 +943,11 @@ __global__ __launch_bounds__(256, 5) void k_verify_j(JobDev *const *__restrict__
   // candidate load per unit instead of a chain of dependent loads.  The
   // plane pointers are uniform and kept in scalar registers.
   uint64_t plo = 1, phi = 0;
-  bool skip = true, run = false, exits = false;
+  bool skip = true, run = false, exits = false, pull = false;
   int stm = 0, n = 0, kt_key = -1;
   JobDev *Jp = nullptr;
   const uint64_t *candr = nullptr, *extra = nullptr, *wins_s = nullptr, *wins_o = nullptr,
-                 *sub_o = nullptr;
+                 *sub_o = nullptr, *xrp = nullptr, *prevW_o = nullptr, *prevL_s = nullptr;
   uint64_t *candw = nullptr, *newL = nullptr;
   uint16_t *codes = nullptr;
   for (uint64_t u = blockIdx.x; u < total; u += gridDim.x) {
@@ -904,6 +971,11 @@ __global__ __launch_bounds__(256, 5) void k_verify_j(JobDev *const *__restrict__
         wins_s = uniptr(Jp->wins[stm]);
         wins_o = uniptr(Jp->wins[o]);
         sub_o = uniptr(Jp->sub[o][n <= DRAW_RULE ? TBG_SUB_WIN : TBG_SUB_CWIN]);
+        // the static refutations hold for the 50-move rule's plies only
+        xrp = n <= DRAW_RULE ? uniptr(Jp->xr[stm]) : nullptr;
+        pull = run && xrp && ((uni(Jp->pull) >> stm) & 1);
+        prevW_o = uniptr(Jp->prevW[o]);
+        prevL_s = uniptr(Jp->prevL[stm]);
         newL = uniptr(Jp->newL[stm]);
         codes = uniptr(Jp->codes[stm]);
       }
@@ -921,7 +993,7 @@ __global__ __launch_bounds__(256, 5) void k_verify_j(JobDev *const *__restrict__
     // cleared.  Every thread's candidate loads go out together; a unit with
     // some is then verified as below, its words found in cache.  (Large
     // units come with large slices, which only the SPLIT form runs.)
-    if (SPLIT && uwe >= VERIFY_BIG_UNIT) {
+    if (SPLIT && uwe >= VERIFY_BIG_UNIT && !pull) {
       uint64_t sb = phys(M, s) * M.words;
       auto none = [](uint64_t x, uint64_t) { return x; };
       uint64_t any = or_words(candr + sb, u0, uend, threadIdx.x, 256, none);
@@ -946,8 +1018,14 @@ __global__ __launch_bounds__(256, 5) void k_verify_j(JobDev *const *__restrict__
       if (live) {
         x0 = gload(candr, a);
         uint64_t c = x0;
+        if (pull) {  // every position of the word
+          uint64_t b0 = w * 64;
+          c = b0 >= M.ii.size ? 0 : M.ii.size - b0 >= 64 ? ~0ull : bit(M.ii.size - b0) - 1;
+        }
         if (exits) c |= gload(extra, a);
         x = run && c ? c & ~gload(wins_s, a) : 0;
+        if (x && xrp) x &= ~gload(xrp, a);
+        if (x && pull) x &= ~gload(prevL_s, a);
       }
       uint64_t r = 0;
       if (__ballot(x != 0)) {
@@ -974,8 +1052,9 @@ __global__ __launch_bounds__(256, 5) void k_verify_j(JobDev *const *__restrict__
               uint64_t idx = (wbase + (v >> 6)) * 64 + (v & 63);
               int r = idx < M.ii.size
                           ? verify_pos<!SPLIT>(T, M, stm, s, idx, wins_o, sub_o, KT[wave], budget,
-                                               hidx, hD, hp)
+                                               hidx, hD, hp, xrp != nullptr)
                           : 0;
+              if (r == 1 && pull && !has_hit(T, M, stm, s, idx, prevW_o, KT[wave])) r = 0;
               if (r == 0)
                 __hip_atomic_fetch_and(reinterpret_cast<unsigned *>(Q.res[wave]) + (v >> 5),
                                        ~(1u << (v & 31)), __ATOMIC_RELAXED,
@@ -1034,7 +1113,7 @@ __global__ __launch_bounds__(256) void k_sweep_j(JobDev *const *__restrict__ job
   // the (job, side) pair's fields, re-read only when a unit leaves its range
   // (the job record does not change during a launch), in scalar registers
   uint64_t plo = 1, phi = 0;
-  bool skip = true, run = false, exits = false;
+  bool skip = true, run = false, exits = false, pull = false;
   int stm = 0, n = 0;
   JobDev *Jp = nullptr;
   uint64_t *cand = nullptr, *wins = nullptr, *newW = nullptr;
@@ -1151,6 +1230,8 @@ __device__ void ctrl_one(JobDev &J, unsigned *flags, unsigned *any_edge)
     J.prevL[c] = J.newL[c];
     J.newL[c] = x;
   }
+  // W_n of each side: the next ply's L frontier of the other
+  unsigned long long wcnt[2] = { (J.mask & 4) ? J.cnt[2] : 0ull, (J.mask & 8) ? J.cnt[3] : 0ull };
   for (int i = 0; i < 4; i++) J.cnt[i] = 0;
   n++;
   J.more = more;
@@ -1166,6 +1247,16 @@ __device__ void ctrl_one(JobDev &J, unsigned *flags, unsigned *any_edge)
   J.mask = ((forced || (more & 4)) ? 1u : 0u) | ((forced || (more & 8)) ? 2u : 0u) |
            ((forced || (more & 1)) ? 4u : 0u) | ((forced || (more & 2)) ? 8u : 0u);
   J.n = n;
+  // the pull form for an L step whose frontier is dense: every frontier
+  // position has tens of predecessors, so at PULL_DENSITY of the positions
+  // nearly every position is a candidate anyway
+  unsigned pull = 0;
+  if (n >= 2 && n <= DRAW_RULE) {
+    unsigned long long pos = J.M->ii.size * J.M->nslices_phys;
+    for (int c = 0; c < 2; c++)
+      if (((J.mask >> c) & 1) && J.xr[c] && wcnt[c ^ 1] * PULL_DENSITY >= pos) pull |= 1u << c;
+  }
+  J.pull = pull;
   if (forced) atomicOr(any_edge, 1u);
   if (J.stop_at > 0 && n >= J.stop_at) {  // a ply boundary the host asked to stop at
     J.done = 2;
@@ -1509,19 +1600,25 @@ __device__ __forceinline__ int sym_sq(int t, int m, bool fl)
   return fl ? ((u & 7) << 3) | (u >> 3) : u;
 }
 
-constexpr int IB_PLANES = 8;  // wins W/B, mates W/B, exit W candidates W/B, exit L candidates W/B
+// wins W/B, mates W/B, exit W candidates W/B, exit L candidates W/B, exit
+// refutations W/B
+constexpr int IB_PLANES = 10;
 constexpr int IB_WORDS = 64;  // words per wave of one plane (R <= 62 of them used)
 
 // calc_illegal_worker + calc_mate_worker (src/tb8gen.c:664-727) and the
 // synthetic exits, one placement per lane, for an all-single material with
 // at least 64 placements per slice.  A wave takes 64 consecutive placements
 // = 64 R positions = R bitmap words of each plane (a "chunk"), assembles
-// them in LDS and stores each word once.  Same outputs as k_init.
+// them in LDS and stores each word once.  Same outputs as k_init, and for
+// a synthetic job with xrW/xrB the exit refutations: positions with a legal
+// king exit whose value is not a win for the side to move after it (such a
+// position is never lost, whatever the ply).
 __global__ __launch_bounds__(256, 5) void k_init_pl(const Mat *__restrict__ Mp, uint64_t *winsW,
                                                  uint64_t *winsB, uint64_t *mateW, uint64_t *mateB,
                                                  uint16_t *codesW, uint16_t *codesB,
                                                  uint64_t *kexW0, uint64_t *kexW1,
-                                                 uint64_t *kexL0, uint64_t *kexL1, int s0, int ns,
+                                                 uint64_t *kexL0, uint64_t *kexL1, uint64_t *xrW,
+                                                 uint64_t *xrB, int s0, int ns,
                                                  unsigned long long *cnt)
 {
   __shared__ Tabs T;
@@ -1626,6 +1723,7 @@ __global__ __launch_bounds__(256, 5) void k_init_pl(const Mat *__restrict__ Mp, 
     // index a per-placement base plus the rank of the moved position's last
     // square; the candidates collect over squares t
     uint64_t xs[4] = { 0, 0, 0, 0 };  // W candidates W, B; L candidates W, B
+    uint64_t xr[2] = { 0, 0 };        // exit refutations W, B
     if (M.synth && valid) {
       for (int c = 0; c < 2; c++) {
         int o = c ^ 1, k = sq_get(p, c);
@@ -1644,6 +1742,7 @@ __global__ __launch_bounds__(256, 5) void k_init_pl(const Mat *__restrict__ Mp, 
           uint64_t go = ~attacked_mask(T, M, p, occF ^ bit(k), to, o, jx, cx, ptx) & ~ill & F;
           uint64_t xw = 0, xl = 0;  // this exit's W and L candidates
           if (cx == c) go &= ~bit(to);
+          const uint64_t legal = go;
           uint64_t q0 = sq_set(p, c, to);
           uint64_t q = q0 ^ (0x0101010101010101ull * (uint64_t)m);
           if (fl) q = flip_diag_bytes(q);
@@ -1683,6 +1782,7 @@ __global__ __launch_bounds__(256, 5) void k_init_pl(const Mat *__restrict__ Mp, 
             xs[1] |= xw;
             xs[3] |= xl;
           }
+          xr[c] |= legal & ~xl;  // a legal exit not into a win of the opponent
         }
       }
     }
@@ -1716,20 +1816,22 @@ __global__ __launch_bounds__(256, 5) void k_init_pl(const Mat *__restrict__ Mp, 
       return x;
     };
     uint64_t rw = 0, rb = 0, rmw = 0, rmb = 0;
-    uint64_t rx[4] = { 0, 0, 0, 0 };
+    uint64_t rx[4] = { 0, 0, 0, 0 }, rr[2] = { 0, 0 };
     if (valid) {
       rw = compress(WI);
       rb = compress(BI);
       rmw = compress(MW);
       rmb = compress(MB);
-      if (M.synth)
+      if (M.synth) {
         for (int k = 0; k < 4; k++) rx[k] = compress(xs[k]);
+        for (int k = 0; k < 2; k++) rr[k] = compress(xr[k]);
+      }
     }
     // assemble the chunk's words in LDS (a lane's R bits start at bit
     // lane * R) and store each word of each plane once
     for (int i = lane; i < IB_PLANES * IB_WORDS; i += 64) B[i / IB_WORDS][i % IB_WORDS] = 0;
     wave_sync();
-    uint64_t rm[IB_PLANES] = { rw, rb, rmw, rmb, rx[0], rx[1], rx[2], rx[3] };
+    uint64_t rm[IB_PLANES] = { rw, rb, rmw, rmb, rx[0], rx[1], rx[2], rx[3], rr[0], rr[1] };
     int off = lane * R, w0 = off >> 6, sh = off & 63;
 #pragma unroll
     for (int k = 0; k < IB_PLANES; k++) {
@@ -1754,6 +1856,10 @@ __global__ __launch_bounds__(256, 5) void k_init_pl(const Mat *__restrict__ Mp, 
         kexL0[wbase + lane] = B[6][lane];
         kexL1[wbase + lane] = B[7][lane];
       }
+      if (xrW) {
+        xrW[wbase + lane] = B[8][lane];
+        xrB[wbase + lane] = B[9][lane];
+      }
     }
     // codes, 64 consecutive positions per store
     if (codesW) {
@@ -1775,6 +1881,181 @@ __global__ __launch_bounds__(256, 5) void k_init_pl(const Mat *__restrict__ Mp, 
   for (int k = 0; k < 4; k++) {
     uint64_t v = __ballot(cw[k] != 0) ? wave_sum64(cw[k]) : 0;
     if (lane == 0 && v) atomicAdd(cnt + k, (unsigned long long)v);
+  }
+}
+
+// The six move masks of Hacker's Delight 7-4 for compressing square masks
+// through F (bit r of the result = the r-th square of F)
+__device__ __forceinline__ void rank_masks(uint64_t F, uint64_t mv[6])
+{
+  uint64_t mm = F, mk = ~F << 1;
+#pragma unroll
+  for (int i = 0; i < 6; i++) {
+    uint64_t mp = mk ^ (mk << 1);
+    mp ^= mp << 2;
+    mp ^= mp << 4;
+    mp ^= mp << 8;
+    mp ^= mp << 16;
+    mp ^= mp << 32;
+    uint64_t v = mp & mm;
+    mm = (mm ^ v) | (v >> (1 << i));
+    mk &= ~mp;
+    mv[i] = v;
+  }
+}
+
+__device__ __forceinline__ uint64_t compress_f(uint64_t x, uint64_t F, const uint64_t mv[6])
+{
+  x &= F;
+#pragma unroll
+  for (int i = 0; i < 6; i++) {
+    uint64_t t = x & mv[i];
+    x = (x ^ t) | (t >> (1 << i));
+  }
+  return x;
+}
+
+// nb <= 62 bits of the bitmap p from bit pos on
+__device__ __forceinline__ uint64_t load_bits(const uint64_t *p, uint64_t pos, int nb)
+{
+  uint64_t w = pos >> 6;
+  int sh = (int)(pos & 63);
+  uint64_t v = gload(p, w) >> sh;
+  if (sh + nb > 64) v |= gload(p, w + 1) << (64 - sh);
+  return v & (bit(nb) - 1);
+}
+
+// The capture part of check_moves (src/tb8gen.c:563-576): a position, stm
+// to move, with a capture by a non-king piece whose result is not in the
+// capture-exit plane sub[opp][TBG_SUB_WIN] is not lost (at plies up to
+// 100); the exits are fixed for the whole generation, so this is worked out
+// once per position and OR-ed into xr, which k_init_pl wrote (zero, or the
+// synthetic king exits' refutations).  Placement-parallel as k_init_pl: a
+// lane holds every set's square but the last set's, whose R squares t are
+// bit-parallel:
+//  - a piece of stm (not the last set's) capturing one of the other sets'
+//    pieces at d: the results for all t form one run of R + 1 bits of the
+//    capture space (the last set's square ranked among F plus the vacated
+//    square), valid where t does not block the move;
+//  - the same piece capturing the last set's piece (t = d): one bit each;
+//  - the last set's piece (stm's) capturing a piece at d: one result for
+//    every t that reaches d.
+// King captures are the verification's (real tables) or the exits' (k_init_pl).
+__global__ __launch_bounds__(256, 4) void k_xr_pl(const Mat *__restrict__ Mp,
+                                                  const uint64_t *__restrict__ subwinW,
+                                                  const uint64_t *__restrict__ subwinB,
+                                                  uint64_t *xrW, uint64_t *xrB, int s0, int ns)
+{
+  __shared__ Tabs T;
+  __shared__ Mat Ms;
+  __shared__ uint64_t buf[4][2][IB_WORDS];
+  load_tabs(T);
+  {
+    const uint32_t *a = reinterpret_cast<const uint32_t *>(Mp);
+    uint32_t *b = reinterpret_cast<uint32_t *>(&Ms);
+    for (int i = threadIdx.x; i < (int)(sizeof(Mat) / 4); i += blockDim.x) b[i] = a[i];
+    __syncthreads();
+  }
+  const Mat &M = Ms;
+  int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  uint64_t(*B)[IB_WORDS] = buf[wave];
+  const int L = M.ii.numsets - 1, jx = 2 + L, R = 62 - L;
+  const int cx = (M.pt[jx] >> 3) & 1, ptx = M.pt[jx];
+  const uint64_t fullR = bit(R) - 1;
+  const uint64_t np = M.ii.size / (uint64_t)R;
+  const uint64_t cps = (M.words + R - 1) / R;
+  for (uint64_t gc = (uint64_t)blockIdx.x * 4 + wave; gc < (uint64_t)ns * cps;
+       gc += (uint64_t)gridDim.x * 4) {
+    int s = s0 + (int)(gc / cps);
+    uint64_t g = gc % cps;
+    uint64_t P = g * 64 + lane;
+    bool valid = P < np;
+    uint64_t p = kings_of(T, s);
+    uint64_t occF = bit(sq_get(p, 0)) | bit(sq_get(p, 1));
+    if (valid) occF = idx_to_sq_1(M.ii, P * (uint64_t)R, &p) & ~bit(sq_get(p, jx));
+    const uint64_t F = ~occF;
+    uint64_t mv[6];
+    rank_masks(F, mv);
+    uint64_t rr[2] = { 0, 0 };
+    for (int c = 0; c < 2 && valid; c++) {
+      int o = c ^ 1;
+      const uint64_t *subS = (o == WHITE ? subwinW : subwinB) + phys(M, s) * M.ctotal[o];
+      uint64_t opp = 0;
+      for (int i = 0; i < MAXP - 1; i++) {
+        if (i >= M.nk[o]) break;
+        int j = M.pcs[o][i];
+        if (j != jx) opp |= bit(sq_get(p, j));
+      }
+      uint64_t rk = 0, rsq = 0;  // refuted: rank domain, square domain
+      for (int i = 0; i < MAXP - 1; i++) {
+        if (i >= M.nk[c]) break;
+        int j = M.pcs[c][i];
+        if (j == jx) continue;
+        int sj = sq_get(p, j);
+        uint64_t a0 = piece_att(T, M.pt[j], sj, occF);
+        uint64_t cap = a0 & opp;
+        // t above the vacated square sj ranks one higher in the capture space
+        uint64_t lowm = bit(popc64(F & (bit(sj) - 1))) - 1;
+        while (cap) {
+          int d = ctz64(cap);
+          cap &= cap - 1;
+          int l = sq_get(M.set_of, slot_of(p, d, M.col80[o]));
+          uint64_t ob;
+          uint64_t base = partial_idx(M.capt[l], sq_set(p, j, d), L, l, &ob);
+          uint64_t seg = load_bits(subS + M.coff[l], base, R + 1);
+          uint64_t segF = (seg & lowm) | ((seg >> 1) & ~lowm);
+          uint64_t btw = between_sq(sj, d) & F;
+          uint64_t ok = btw ? fullR & ~compress_f(btw, F, mv) : fullR;
+          rk |= ok & ~segF;
+        }
+        if (cx == o) {  // the last set's piece taken on d, t = d
+          uint64_t dd = a0 & F;
+          while (dd) {
+            int d = ctz64(dd);
+            dd &= dd - 1;
+            uint64_t ob;
+            uint64_t ci = partial_idx(M.capt[L], sq_set(p, j, d), L, -1, &ob);
+            if (!bm_test(subS + M.coff[L], ci)) rsq |= bit(d);
+          }
+        }
+      }
+      if (cx == c) {  // the last set's piece takes the piece on d
+        uint64_t oo = opp;
+        while (oo) {
+          int d = ctz64(oo);
+          oo &= oo - 1;
+          uint64_t from = piece_att(T, ptx, d, occF) & F;
+          if (!from) continue;
+          int l = sq_get(M.set_of, slot_of(p, d, M.col80[o]));
+          uint64_t ob;
+          uint64_t ci = partial_idx(M.capt[l], sq_set(p, jx, d), L, l, &ob);
+          ci += (uint64_t)(uint32_t)(d - popc64(ob & (bit(d) - 1)));
+          if (!bm_test(subS + M.coff[l], ci)) rsq |= from;
+        }
+      }
+      rr[c] = rk | compress_f(rsq, F, mv);
+    }
+    // the chunk's R words of each plane: read, OR-ed, written back once
+    uint64_t wbase = phys(M, s) * M.words + g * (uint64_t)R;
+    uint64_t nwc = M.words - g * (uint64_t)R < (uint64_t)R ? M.words - g * (uint64_t)R : (uint64_t)R;
+    B[0][lane] = (uint64_t)lane < nwc ? xrW[wbase + lane] : 0;
+    B[1][lane] = (uint64_t)lane < nwc ? xrB[wbase + lane] : 0;
+    wave_sync();
+    int off = lane * R, w0 = off >> 6, sh = off & 63;
+    for (int k = 0; k < 2; k++) {
+      uint64_t v = rr[k];
+      if (!v) continue;
+      atomicOr(reinterpret_cast<unsigned long long *>(&B[k][w0]), (unsigned long long)(v << sh));
+      if (sh && sh + R > 64)
+        atomicOr(reinterpret_cast<unsigned long long *>(&B[k][w0 + 1]),
+                 (unsigned long long)(v >> (64 - sh)));
+    }
+    wave_sync();
+    if ((uint64_t)lane < nwc) {
+      xrW[wbase + lane] = B[0][lane];
+      xrB[wbase + lane] = B[1][lane];
+    }
+    wave_sync();  // the next chunk overwrites B
   }
 }
 
