@@ -467,6 +467,7 @@ void reset_job(tbg_ctx *c)
     J.cand[s] = c->cand[s];
     J.candW[s] = c->candW[s];
     J.xr[s] = c->xr_ok ? c->xr[s] : nullptr;
+    J.nopen = c->xr_ok ? c->dcnt + 4 : nullptr;
     J.codes[s] = c->codes[s];
     for (int k = 0; k < 4; k++) J.sub[s][k] = c->sub[s][k];
   }
@@ -780,7 +781,7 @@ int run_init_async(tbg_ctx *c)
 {
   Mat &M = c->h;
   int s0 = M.synth ? M.synth_slice : 0, ns = M.synth ? 1 : NSL;
-  HIPCHK(hipMemsetAsync(c->dcnt, 0, 4 * sizeof(unsigned long long), c->st));
+  HIPCHK(hipMemsetAsync(c->dcnt, 0, 6 * sizeof(unsigned long long), c->st));
   {
     Timed t(c, CAT_INIT);
     // one placement per lane (pl_layout); with the capture exits resolved,
@@ -797,7 +798,8 @@ int run_init_async(tbg_ctx *c)
       if (c->resolved && c->xr[0]) {
         HIPCHK(hipGetLastError());
         hipLaunchKernelGGL(k_xr_pl, grid, dim3(256), 0, c->st, c->dM, c->sub[0][TBG_SUB_WIN],
-                           c->sub[1][TBG_SUB_WIN], c->xr[0], c->xr[1], s0, ns);
+                           c->sub[1][TBG_SUB_WIN], c->wins[0], c->wins[1], c->xr[0], c->xr[1],
+                           s0, ns, c->dcnt + 4);
         c->xr_ok = true;
       }
     } else {
@@ -1299,6 +1301,7 @@ int tbg_resolve_captures(tbg_ctx *c, tbg_ctx *const *subs, int nsubs)
   if (c->xr_ok) {  // worked out from the earlier capture exits: verify every move
     c->xr_ok = false;
     c->hj.xr[0] = c->hj.xr[1] = nullptr;
+    c->hj.nopen = nullptr;
     if (c->inited) TRY(upload_job(c));
   }
   return TBG_OK;

@@ -29,6 +29,9 @@ struct JobDev {
   // capture or synthetic exit whose result is not a win of the opponent
   // (within the 50-move rule); nullptr: not worked out, verify every move
   uint64_t *xr[2];
+  // per side: positions neither won (illegal) at init nor statically refuted,
+  // what a pull-form L step verifies at most (k_xr_pl counts them)
+  const unsigned long long *nopen;
   uint16_t *codes[2];
   long long *hist;            // [max_ply][4] counts per step, -1: step did not run
   unsigned long long cnt[4];  // this ply: L(W), L(B), W(W), W(B)
@@ -135,7 +138,8 @@ __global__ void k_init_pl(const Mat *Mp, uint64_t *winsW, uint64_t *winsB, uint6
 // k_init_pl cleared or filled with the exit part), from the capture-exit
 // planes sub[colour][TBG_SUB_WIN]; same launch shape as k_init_pl
 __global__ void k_xr_pl(const Mat *Mp, const uint64_t *subwinW, const uint64_t *subwinB,
-                        uint64_t *xrW, uint64_t *xrB, int s0, int ns);
+                        const uint64_t *winsW, const uint64_t *winsB, uint64_t *xrW,
+                        uint64_t *xrB, int s0, int ns, unsigned long long *nopen);
 __global__ void k_resolve(const Mat *Mp, const SubRef *Rp, int c, int k, uint64_t *loss,
                           uint64_t *bloss, uint64_t *cwin, uint64_t *win, int s0, int ns);
 

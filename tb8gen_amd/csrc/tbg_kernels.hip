@@ -323,9 +323,9 @@ __device__ __forceinline__ void mark_unmoves(const Tabs &T, const Mat &M, int j,
 constexpr int QCAP = 1024;  // queue entries per wave per round
 constexpr int VERIFY_QUICK = 4;    // bitmap reads of verification's first pass
 constexpr int VERIFY_SPLIT = 256;  // candidates per round worth two passes
-// an L step runs in the pull form when its frontier holds at least 1 /
-// PULL_DENSITY of the positions
-constexpr unsigned long long PULL_DENSITY = 16;
+// an L step runs in the pull form when PULL_RATIO times its frontier is at
+// least the positions it would verify (JobDev.nopen)
+constexpr unsigned long long PULL_RATIO = 2;
 // ... in slices of more positions than VERIFY_SPLIT_SIZE (tbg_kern.h): in
 // smaller tables the bitmaps stay in the caches, their reads are cheap, and
 // the second decode of the open candidates costs more than the idle lanes
@@ -1247,14 +1247,15 @@ __device__ void ctrl_one(JobDev &J, unsigned *flags, unsigned *any_edge)
   J.mask = ((forced || (more & 4)) ? 1u : 0u) | ((forced || (more & 8)) ? 2u : 0u) |
            ((forced || (more & 1)) ? 4u : 0u) | ((forced || (more & 2)) ? 8u : 0u);
   J.n = n;
-  // the pull form for an L step whose frontier is dense: every frontier
-  // position has tens of predecessors, so at PULL_DENSITY of the positions
-  // nearly every position is a candidate anyway
+  // the pull form for an L step whose frontier is large beside the
+  // positions it would verify: each frontier position costs the predecessor
+  // pass tens of unmoves, each open position the pull form about one
+  // refuted verification
   unsigned pull = 0;
   if (n >= 2 && n <= DRAW_RULE) {
-    unsigned long long pos = J.M->ii.size * J.M->nslices_phys;
     for (int c = 0; c < 2; c++)
-      if (((J.mask >> c) & 1) && J.xr[c] && wcnt[c ^ 1] * PULL_DENSITY >= pos) pull |= 1u << c;
+      if (((J.mask >> c) & 1) && J.xr[c] && J.nopen && wcnt[c ^ 1] * PULL_RATIO >= J.nopen[c])
+        pull |= 1u << c;
   }
   J.pull = pull;
   if (forced) atomicOr(any_edge, 1u);
@@ -1941,10 +1942,15 @@ __device__ __forceinline__ uint64_t load_bits(const uint64_t *p, uint64_t pos, i
 //  - the last set's piece (stm's) capturing a piece at d: one result for
 //    every t that reaches d.
 // King captures are the verification's (real tables) or the exits' (k_init_pl).
+// nopen[c] counts the positions, c to move, neither in wins (illegal) nor
+// refuted.
 __global__ __launch_bounds__(256, 4) void k_xr_pl(const Mat *__restrict__ Mp,
                                                   const uint64_t *__restrict__ subwinW,
                                                   const uint64_t *__restrict__ subwinB,
-                                                  uint64_t *xrW, uint64_t *xrB, int s0, int ns)
+                                                  const uint64_t *__restrict__ winsW,
+                                                  const uint64_t *__restrict__ winsB,
+                                                  uint64_t *xrW, uint64_t *xrB, int s0, int ns,
+                                                  unsigned long long *nopen)
 {
   __shared__ Tabs T;
   __shared__ Mat Ms;
@@ -1964,6 +1970,7 @@ __global__ __launch_bounds__(256, 4) void k_xr_pl(const Mat *__restrict__ Mp,
   const uint64_t fullR = bit(R) - 1;
   const uint64_t np = M.ii.size / (uint64_t)R;
   const uint64_t cps = (M.words + R - 1) / R;
+  uint64_t open[2] = { 0, 0 };
   for (uint64_t gc = (uint64_t)blockIdx.x * 4 + wave; gc < (uint64_t)ns * cps;
        gc += (uint64_t)gridDim.x * 4) {
     int s = s0 + (int)(gc / cps);
@@ -2054,8 +2061,16 @@ __global__ __launch_bounds__(256, 4) void k_xr_pl(const Mat *__restrict__ Mp,
     if ((uint64_t)lane < nwc) {
       xrW[wbase + lane] = B[0][lane];
       xrB[wbase + lane] = B[1][lane];
+      uint64_t b0 = (g * (uint64_t)R + lane) * 64;  // the word's first index in the slice
+      uint64_t vm = b0 >= M.ii.size ? 0 : M.ii.size - b0 >= 64 ? ~0ull : bit(M.ii.size - b0) - 1;
+      open[0] += popc64(vm & ~B[0][lane] & ~winsW[wbase + lane]);
+      open[1] += popc64(vm & ~B[1][lane] & ~winsB[wbase + lane]);
     }
     wave_sync();  // the next chunk overwrites B
+  }
+  for (int k = 0; k < 2; k++) {
+    uint64_t v = __ballot(open[k] != 0) ? wave_sum64(open[k]) : 0;
+    if (lane == 0 && v) atomicAdd(nopen + k, (unsigned long long)v);
   }
 }
 
