@@ -13,9 +13,10 @@ every executed half-step (L or W, one side) covers the slice's positions
 for half a ply.
 roofline: the kernel with the largest device time (pred, verify, sweep or
 init): its algorithmic bytes per launch (bits per position: pred 2, verify 3,
-W sweep 4 per executed half-step; init 8 output planes) over its average
-launch time from HIP events on the engine's stream; traffic = HBM bytes per
-launch from the committed rocprof counter summary (profiles/).
+W sweep 4 per executed half-step; the init phase 14: ten output planes, two
+of them read and written back once more) over its average launch time from
+HIP events on the engine's stream; traffic = HBM bytes per launch from the
+committed rocprof counter summary (profiles/).
 secondary: the KQRvKR table (462 slices x 226,920 positions per side)
 generated end to end in the batched form.
 cpu_baseline: the CPU oracle (a plain-C restatement of the reference's

@@ -34,10 +34,9 @@ namespace {
 thread_local char g_errbuf[256];
 int g_device = -1;
 int g_cus = 256;
-// bitmap planes per job: wins, prevW, prevL, newW, newL and the L and W
-// steps' candidates per side, and for the placement layout (pl_layout) the
-// static refutations per side
-constexpr uint64_t kPlanes = 14;
+// bitmap planes per job: wins, prevW, prevL, newW, newL, the L and W steps'
+// candidates and the static refutations, per side
+constexpr uint64_t kPlanes = 16;
 
 // all sets single and at least 64 placements (runs of the last set's R
 // squares) per slice: the placement-parallel kernels apply
@@ -47,7 +46,7 @@ bool pl_layout(const Mat &M)
   return M.all_single && M.ii.numsets >= 2 && M.ii.size / R >= 64;
 }
 
-uint64_t planes_of(const Mat &M) { return kPlanes + (pl_layout(M) ? 2 : 0); }
+uint64_t planes_of(const Mat &) { return kPlanes; }
 Tabs g_htabs;
 // geometry and scatter given to new contexts and servers (tbg_set_geometry(NULL, ..))
 Geom g_geom = { 256, 4096, { 3, 16, 16, 16 } };
@@ -306,7 +305,7 @@ struct tbg_ctx {
   uint64_t *arena = nullptr, *subarena = nullptr;
   uint64_t *wins[2], *prevW[2], *prevL[2], *newW[2], *newL[2];  // canonical planes
   uint64_t *cand[2], *candW[2];
-  uint64_t *xr[2] = { nullptr, nullptr };  // static refutations (pl_layout only)
+  uint64_t *xr[2] = { nullptr, nullptr };  // static refutations (captures, exits)
   bool xr_ok = false;                      // worked out since the last resolve and init
   uint64_t *sub[2][4];
   uint16_t *codes[2] = { nullptr, nullptr };
@@ -542,8 +541,8 @@ void assign_planes(tbg_ctx *c)
     c->candW[s] = p; p += c->pw;
   }
   for (int s = 0; s < 2; s++) {
-    c->xr[s] = pl_layout(c->h) ? p : nullptr;
-    if (c->xr[s]) p += c->pw;
+    c->xr[s] = p;
+    p += c->pw;
   }
 }
 
@@ -795,7 +794,7 @@ int run_init_async(tbg_ctx *c)
                          c->prevL[0], c->prevL[1], c->codes[0], c->codes[1], c->prevW[0],
                          c->prevW[1], c->newW[0], c->newW[1], c->xr[0], c->xr[1], s0, ns,
                          c->dcnt);
-      if (c->resolved && c->xr[0]) {
+      if (c->resolved) {
         HIPCHK(hipGetLastError());
         hipLaunchKernelGGL(k_xr_pl, grid, dim3(256), 0, c->st, c->dM, c->sub[0][TBG_SUB_WIN],
                            c->sub[1][TBG_SUB_WIN], c->wins[0], c->wins[1], c->xr[0], c->xr[1],
@@ -804,10 +803,18 @@ int run_init_async(tbg_ctx *c)
       }
     } else {
       uint64_t chunks = ((uint64_t)ns * M.words + 4 * INIT_CHUNK - 1) / (4 * INIT_CHUNK);
-      hipLaunchKernelGGL(k_init, dim3(cap_grid(chunks, (uint64_t)g_cus * 16)),
-                         dim3(256), 0, c->st, c->dM, c->wins[0], c->wins[1], c->prevL[0],
-                         c->prevL[1], c->codes[0], c->codes[1], c->prevW[0], c->prevW[1],
-                         c->newW[0], c->newW[1], s0, ns, c->dcnt);
+      dim3 grid(cap_grid(chunks, (uint64_t)g_cus * 16));
+      hipLaunchKernelGGL(k_init, grid, dim3(256), 0, c->st, c->dM, c->wins[0], c->wins[1],
+                         c->prevL[0], c->prevL[1], c->codes[0], c->codes[1], c->prevW[0],
+                         c->prevW[1], c->newW[0], c->newW[1], c->xr[0], c->xr[1], s0, ns,
+                         c->dcnt);
+      if (c->resolved) {
+        HIPCHK(hipGetLastError());
+        hipLaunchKernelGGL(k_xr, grid, dim3(256), 0, c->st, c->dM, c->sub[0][TBG_SUB_WIN],
+                           c->sub[1][TBG_SUB_WIN], c->wins[0], c->wins[1], c->xr[0], c->xr[1],
+                           s0, ns, c->dcnt + 4);
+        c->xr_ok = true;
+      }
     }
     HIPCHK(hipGetLastError());
   }

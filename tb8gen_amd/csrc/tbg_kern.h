@@ -126,8 +126,13 @@ __global__ void k_ctrl(JobDev *const *jobs, int nj, unsigned *flags, unsigned *d
 constexpr int INIT_CHUNK = 8;
 __global__ void k_init(const Mat *Mp, uint64_t *winsW, uint64_t *winsB, uint64_t *mateW,
                        uint64_t *mateB, uint16_t *codesW, uint16_t *codesB, uint64_t *kexW0,
-                       uint64_t *kexW1, uint64_t *kexL0, uint64_t *kexL1, int s0, int ns,
-                       unsigned long long *cnt);
+                       uint64_t *kexW1, uint64_t *kexL0, uint64_t *kexL1, uint64_t *xrW,
+                       uint64_t *xrB, int s0, int ns, unsigned long long *cnt);
+// the capture part of the static refutations for any material, OR-ed into
+// xrW / xrB after k_init; the same launch shape as k_init
+__global__ void k_xr(const Mat *Mp, const uint64_t *subwinW, const uint64_t *subwinB,
+                     const uint64_t *winsW, const uint64_t *winsB, uint64_t *xrW, uint64_t *xrB,
+                     int s0, int ns, unsigned long long *nopen);
 // the same outputs for an all-single material of at least 64 placements
 // per slice, one placement (every set's square but the last) per lane
 __global__ void k_init_pl(const Mat *Mp, uint64_t *winsW, uint64_t *winsB, uint64_t *mateW,

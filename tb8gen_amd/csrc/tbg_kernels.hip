@@ -1309,7 +1309,7 @@ __global__ __launch_bounds__(256) void k_codes8(const uint16_t *__restrict__ cod
 // uo: the opponent's piece attacks with stm's king lifted, so a king
 // move's legality, capture or not, is a bit test.
 __device__ void synth_exits_pos(const Tabs &T, const Mat &M, uint64_t p, uint64_t occ, int stm,
-                                uint64_t idx, uint64_t uo, bool *w, bool *l)
+                                uint64_t idx, uint64_t uo, bool *w, bool *l, bool *r)
 {
   int o = stm ^ 1, from = sq_get(p, stm), oks = sq_get(p, o);
   uint64_t own = 0;
@@ -1318,7 +1318,7 @@ __device__ void synth_exits_pos(const Tabs &T, const Mat &M, uint64_t p, uint64_
     own |= bit(sq_get(p, M.pcs[stm][t]));
   }
   uint64_t blocked = own | T.king[oks] | uo;
-  bool ww = false, ll = false;
+  bool ww = false, ll = false, rr = false;
   AllSets A = all_sets(M.ii, p);
 #pragma unroll
   for (int k = 0; k < MAXS; k++) A.f[k] = (uint32_t)uni((int)A.f[k]);
@@ -1348,9 +1348,11 @@ __device__ void synth_exits_pos(const Tabs &T, const Mat &M, uint64_t p, uint64_
     }
     ww |= v == -2;
     ll |= v == 2;
+    rr |= v != 2;  // a legal exit not into a win of the opponent
   }
   *w = ww;
   *l = ll;
+  *r = rr;
 }
 
 // has_legal_moves || has_legal_caps (src/movegen.c:75-111) for colour c:
@@ -1423,8 +1425,8 @@ __global__ __launch_bounds__(256, 6) void k_init(const Mat *__restrict__ Mp, uin
                                                uint64_t *winsB, uint64_t *mateW, uint64_t *mateB,
                                                uint16_t *codesW, uint16_t *codesB,
                                                uint64_t *kexW0, uint64_t *kexW1, uint64_t *kexL0,
-                                               uint64_t *kexL1, int s0, int ns,
-                                               unsigned long long *cnt)
+                                               uint64_t *kexL1, uint64_t *xrW, uint64_t *xrB,
+                                               int s0, int ns, unsigned long long *cnt)
 {
   __shared__ Tabs T;
   load_tabs(T);
@@ -1444,7 +1446,7 @@ __global__ __launch_bounds__(256, 6) void k_init(const Mat *__restrict__ Mp, uin
     uint64_t w = g % M.words;
     uint64_t idx = w * 64 + lane;
     bool wi = false, bi = false, wm = false, bm = false;
-    bool xw0 = false, xw1 = false, xl0 = false, xl1 = false;
+    bool xw0 = false, xw1 = false, xl0 = false, xl1 = false, xr0 = false, xr1 = false;
     if (idx < M.ii.size) {
       // the previous word's position of this lane is index - 64 of the same
       // slice unless the chunk or the slice starts here
@@ -1468,8 +1470,8 @@ __global__ __launch_bounds__(256, 6) void k_init(const Mat *__restrict__ Mp, uin
       // an illegal position is in wins(stm) from the start, which masks its
       // exit candidates in both the W sweep and verification
       if (M.synth) {
-        if (!wi) synth_exits_pos(T, M, p, occ, WHITE, idx, ub, &xw0, &xl0);
-        if (!bi) synth_exits_pos(T, M, p, occ, BLACK, idx, uw, &xw1, &xl1);
+        if (!wi) synth_exits_pos(T, M, p, occ, WHITE, idx, ub, &xw0, &xl0, &xr0);
+        if (!bi) synth_exits_pos(T, M, p, occ, BLACK, idx, uw, &xw1, &xl1, &xr1);
       }
       if (wi != bi) {
         int c = bi ? WHITE : BLACK;
@@ -1490,6 +1492,13 @@ __global__ __launch_bounds__(256, 6) void k_init(const Mat *__restrict__ Mp, uin
         kexW1[a] = e1;
         kexL0[a] = e2;
         kexL1[a] = e3;
+      }
+    }
+    if (xrW) {  // the exit part of the static refutations (zero for a table)
+      uint64_t r0 = __ballot(xr0), r1 = __ballot(xr1);
+      if (lane == 0) {
+        xrW[a] = r0;
+        xrB[a] = r1;
       }
     }
     if (lane == 0) {
@@ -1882,6 +1891,93 @@ __global__ __launch_bounds__(256, 5) void k_init_pl(const Mat *__restrict__ Mp, 
   for (int k = 0; k < 4; k++) {
     uint64_t v = __ballot(cw[k] != 0) ? wave_sum64(cw[k]) : 0;
     if (lane == 0 && v) atomicAdd(cnt + k, (unsigned long long)v);
+  }
+}
+
+// The capture part of the static refutations for any material (k_xr_pl's
+// counterpart; k_init wrote the exit part): one position per lane, as in
+// k_init, both sides; a capture by a non-king piece whose result's bit in
+// sub[opp][TBG_SUB_WIN] is clear refutes (the captures of verify_pos,
+// src/tb8gen.c:563-576).  nopen[c]: positions neither illegal for c to move
+// nor refuted.
+__global__ __launch_bounds__(256) void k_xr(const Mat *__restrict__ Mp,
+                                            const uint64_t *__restrict__ subwinW,
+                                            const uint64_t *__restrict__ subwinB,
+                                            const uint64_t *__restrict__ winsW,
+                                            const uint64_t *__restrict__ winsB, uint64_t *xrW,
+                                            uint64_t *xrB, int s0, int ns,
+                                            unsigned long long *nopen)
+{
+  __shared__ Tabs T;
+  load_tabs(T);
+  const Mat &M = *Mp;
+  int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  uint64_t nw = (uint64_t)ns * M.words;
+  uint64_t open[2] = { 0, 0 };
+  bool stepping = M.all_single;
+  int nsets = M.ii.numsets;
+  for (uint64_t g0 = ((uint64_t)blockIdx.x * 4 + wave) * INIT_CHUNK; g0 < nw;
+       g0 += (uint64_t)gridDim.x * 4 * INIT_CHUNK)
+  for (uint64_t g = g0, D = 0, pk = 0; g < g0 + INIT_CHUNK && g < nw; g++) {
+    int s = s0 + (int)(g / M.words);
+    uint64_t w = g % M.words;
+    uint64_t idx = w * 64 + lane;
+    uint64_t a = phys(M, s) * M.words + w;
+    uint64_t win[2] = { winsW[a], winsB[a] };
+    bool ref[2] = { false, false };
+    if (idx < M.ii.size) {
+      uint64_t p, occ;
+      if (stepping && g != g0 && w != 0) {
+        p = pk;
+        occ = step64(nsets, &D, &p);
+      } else {
+        p = kings_of(T, s);
+        occ = idx_to_sq(T, M.ii, idx, &p);
+        if (stepping) D = digits_of(p, nsets);
+      }
+      pk = p;
+      for (int c = 0; c < 2; c++) {
+        if ((win[c] >> lane) & 1) continue;  // illegal with c to move
+        int o = c ^ 1;
+        const uint64_t *subS = (o == WHITE ? subwinW : subwinB) + phys(M, s) * M.ctotal[o];
+        uint64_t oppocc = 0;
+        for (int t = 0; t < MAXP - 1; t++) {
+          if (t >= M.nk[o]) break;
+          oppocc |= bit(sq_get(p, M.pcs[o][t]));
+        }
+        bool r = false;
+        for (int t = 0; t < MAXP - 1 && !r; t++) {
+          if (t >= M.nk[c]) break;
+          int j = M.pcs[c][t];
+          uint64_t cap = piece_att(T, M.pt[j], sq_get(p, j), occ) & oppocc;
+          while (cap && !r) {
+            int to = ctz64(cap);
+            cap &= cap - 1;
+            int jj = slot_of(p, to, M.col80[o]);
+            int l = sq_get(M.set_of, jj);
+            uint64_t q = sq_set(p, j, to);
+            q = sq_set(q, jj, sq_get(q, M.ii.last[l]));
+            uint64_t ci = M.all_single ? capt_idx_1(M.capt[l], q, l) : sq_to_idx(T, M.capt[l], q);
+            r = !bm_test(subS + M.coff[l], ci);
+          }
+        }
+        ref[c] = r;
+      }
+    }
+    uint64_t r0 = __ballot(ref[0]), r1 = __ballot(ref[1]);
+    if (lane == 0) {
+      uint64_t x0 = xrW[a] | r0, x1 = xrB[a] | r1;
+      xrW[a] = x0;
+      xrB[a] = x1;
+      uint64_t b0 = w * 64;
+      uint64_t vm = b0 >= M.ii.size ? 0 : M.ii.size - b0 >= 64 ? ~0ull : bit(M.ii.size - b0) - 1;
+      open[0] += popc64(vm & ~x0 & ~win[0]);
+      open[1] += popc64(vm & ~x1 & ~win[1]);
+    }
+  }
+  if (lane == 0) {
+    if (open[0]) atomicAdd(nopen + 0, (unsigned long long)open[0]);
+    if (open[1]) atomicAdd(nopen + 1, (unsigned long long)open[1]);
   }
 }
 
