@@ -599,20 +599,30 @@ TBG_HD uint64_t synth_key(uint64_t seedmix, int stm, int kind, int region)
   return (((uint64_t)(region & 0xfff) << 36) | ((uint64_t)(kind & 3) << 48) |
           ((uint64_t)(stm & 1) << 50)) ^ seedmix;
 }
+// The high 32 bits of splitmix64(idx ^ key), and in *z the state before
+// the last multiply (to finish the hash with).  They decide synth_value
+// except where they equal a threshold's high half, SYN_TW_HI or SYN_TL_HI
+// (about one hash in 2^31).
+constexpr uint32_t SYN_TW_HI = (uint32_t)(SYN_T_WIN >> 32), SYN_TL_HI = (uint32_t)(SYN_T_LOSS >> 32);
+TBG_HD uint32_t synth_hi(uint64_t key, uint64_t idx, uint64_t *zp)
+{
+  uint64_t z = (idx ^ key) + 0x9e3779b97f4a7c15ull;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z ^= z >> 27;
+  *zp = z;
+  uint32_t zl = (uint32_t)z, zh = (uint32_t)(z >> 32);
+  uint32_t ph = (uint32_t)(((uint64_t)zl * 0x133111ebu) >> 32) + zl * 0x94d049bbu + zh * 0x133111ebu;
+  return ph ^ (ph >> 31);
+}
 // synth_value(splitmix64(idx ^ key)) from the high half of the hash: the
 // last multiply's low half is needed only when the high half ties a
 // threshold's
 TBG_HD int synth_value_k(uint64_t key, uint64_t idx)
 {
-  uint64_t z = (idx ^ key) + 0x9e3779b97f4a7c15ull;
-  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
-  z ^= z >> 27;
-  uint32_t zl = (uint32_t)z, zh = (uint32_t)(z >> 32);
-  uint32_t ph = (uint32_t)(((uint64_t)zl * 0x133111ebu) >> 32) + zl * 0x94d049bbu + zh * 0x133111ebu;
-  uint32_t hh = ph ^ (ph >> 31);
-  constexpr uint32_t TW = (uint32_t)(SYN_T_WIN >> 32), TL = (uint32_t)(SYN_T_LOSS >> 32);
-  int v = hh < TW ? 2 : hh < TL ? -2 : 0;
-  if (hh == TW || hh == TL) {  // about one hash in 2^31
+  uint64_t z;
+  uint32_t hh = synth_hi(key, idx, &z);
+  int v = hh < SYN_TW_HI ? 2 : hh < SYN_TL_HI ? -2 : 0;
+  if (hh == SYN_TW_HI || hh == SYN_TL_HI) {
     uint64_t h = z * 0x94d049bb133111ebull;
     v = synth_value(h ^ (h >> 31));
   }

@@ -37,6 +37,8 @@ int g_cus = 256;
 // bitmap planes per job: wins, prevW, prevL, newW, newL, the L and W steps'
 // candidates and the static refutations, per side
 constexpr uint64_t kPlanes = 16;
+// plies a generation may take (the history's length)
+constexpr int kMaxPly = 4096;
 
 // all sets single and at least 64 placements (runs of the last set's R
 // squares) per slice: the placement-parallel kernels apply
@@ -287,6 +289,7 @@ struct LaunchSet {
   Geom g;
   void release()
   {
+    if (!cap) return;  // nothing allocated (or arrays owned elsewhere)
     (void)hipFree(pairs);
     for (auto &x : P) (void)hipFree(x);
     pairs = nullptr;
@@ -302,7 +305,12 @@ struct tbg_ctx {
   SubRef hR[MAXS];
   SubRef *dR = nullptr;
   uint64_t nsl = 0, pw = 0;
+  // device memory in two blocks: the state block (bitmap planes from its
+  // start, `arena`, then the capture-exit planes, `subarena`, and the codes)
+  // and the record block `rec` (material, capture tables, init counts,
+  // flags, job record, its pointer, per-ply history)
   uint64_t *arena = nullptr, *subarena = nullptr;
+  void *rec = nullptr;
   uint64_t *wins[2], *prevW[2], *prevL[2], *newW[2], *newL[2];  // canonical planes
   uint64_t *cand[2], *candW[2];
   uint64_t *xr[2] = { nullptr, nullptr };  // static refutations (captures, exits)
@@ -317,7 +325,7 @@ struct tbg_ctx {
   JobDev **djp = nullptr;              // device array holding dj (a batch of one)
   long long *dhist = nullptr;
   std::vector<long long> hist;
-  int max_ply = 4096;
+  int max_ply = kMaxPly;
   bool ply_cap_asked = false;  // a bounded run (tune trials) may stop at max_ply
   // a server job: memory from the stream-ordered pool of the server's setup
   // stream (which it shares, so does not own), set up without host waits
@@ -325,7 +333,8 @@ struct tbg_ctx {
   hipEvent_t ready = nullptr;  // pooled: set-up and init complete
   hipStream_t st = nullptr;
   int resolved = 0, inited = 0, generated = 0;
-  uint64_t bytes = 0;
+  uint64_t bytes = 0;        // device bytes held: both blocks
+  uint64_t state_bytes = 0;  // the state block (what preemption moves to the host)
   int mgr[2][NSL];
   LaunchSet both, side[2], plain[2];
   bool prof = false;
@@ -527,10 +536,51 @@ void dfree(tbg_ctx *c, void *p)
   else (void)hipFree(p);
 }
 
-// the canonical planes in their allocation order within the arena
-void assign_planes(tbg_ctx *c)
+// Byte layout of a context's two device blocks (every part 256-byte
+// aligned).  State block: the bitmap planes, the capture-exit planes, the
+// codes of each side; record block: material, capture tables, counts (16
+// words), flags (4), job record, its pointer, the per-ply history.
+struct CtxLayout {
+  size_t arena, sub, codes;                  // state block parts (codes: per side)
+  size_t o_sub, o_codes, state;              // offsets, total
+  size_t o_m, o_r, o_cnt, o_flags, o_j, o_jp, o_hist, rec;
+};
+constexpr size_t kPart = 256;
+size_t part_up(size_t x) { return (x + kPart - 1) & ~(kPart - 1); }
+
+CtxLayout ctx_layout(const Mat &M, uint64_t nsl, int max_ply, bool with_codes)
 {
-  uint64_t *p = c->arena;
+  CtxLayout L;
+  uint64_t pw = nsl * M.words;
+  L.arena = part_up(pw * 8 * planes_of(M));
+  L.sub = part_up(4 * nsl * (M.ctotal[0] + M.ctotal[1]) * 8 + 64);
+  L.codes = with_codes ? part_up(nsl * M.ii.size * 2) : 0;
+  L.o_sub = L.arena;
+  L.o_codes = L.o_sub + L.sub;
+  L.state = L.o_codes + 2 * L.codes;
+  L.o_m = 0;
+  L.o_r = part_up(sizeof(Mat));
+  L.o_cnt = L.o_r + part_up(sizeof(SubRef) * MAXS);
+  L.o_flags = L.o_cnt + part_up(16 * sizeof(unsigned long long));
+  L.o_j = L.o_flags + part_up(4 * sizeof(unsigned));
+  L.o_jp = L.o_j + part_up(sizeof(JobDev));
+  L.o_hist = L.o_jp + part_up(sizeof(JobDev *));
+  L.rec = L.o_hist + part_up(sizeof(long long) * 4 * (size_t)max_ply);
+  return L;
+}
+
+CtxLayout ctx_layout(const tbg_ctx *c)
+{
+  return ctx_layout(c->h, c->nsl, c->max_ply, c->codes[0] != nullptr);
+}
+
+// the canonical planes in their allocation order, the capture-exit planes
+// and the codes at their places in the state block at `state`
+void carve_state(tbg_ctx *c, void *state, const CtxLayout &L, bool with_codes)
+{
+  char *b = static_cast<char *>(state);
+  uint64_t *p = reinterpret_cast<uint64_t *>(b);
+  c->arena = p;
   for (int s = 0; s < 2; s++) {
     c->wins[s] = p; p += c->pw;
     c->prevW[s] = p; p += c->pw;
@@ -544,9 +594,34 @@ void assign_planes(tbg_ctx *c)
     c->xr[s] = p;
     p += c->pw;
   }
+  c->subarena = reinterpret_cast<uint64_t *>(b + L.o_sub);
+  p = c->subarena;
+  for (int col = 0; col < 2; col++)
+    for (int k = 0; k < 4; k++) {
+      c->sub[col][k] = p;
+      p += c->nsl * c->h.ctotal[col];
+    }
+  for (int s = 0; s < 2; s++)
+    c->codes[s] = with_codes ? reinterpret_cast<uint16_t *>(b + L.o_codes + s * L.codes) : nullptr;
 }
 
-// ext: the stream of a pooled context (server jobs), else nullptr
+void carve_rec(tbg_ctx *c, void *rec, const CtxLayout &L)
+{
+  char *b = static_cast<char *>(rec);
+  c->rec = rec;
+  c->dM = reinterpret_cast<Mat *>(b + L.o_m);
+  c->dR = reinterpret_cast<SubRef *>(b + L.o_r);
+  c->dcnt = reinterpret_cast<unsigned long long *>(b + L.o_cnt);
+  c->dflags = reinterpret_cast<unsigned *>(b + L.o_flags);
+  c->dj = reinterpret_cast<JobDev *>(b + L.o_j);
+  c->djp = reinterpret_cast<JobDev **>(b + L.o_jp);
+  c->dhist = reinterpret_cast<long long *>(b + L.o_hist);
+}
+
+// ext: the stream of a pooled context (server jobs), else nullptr.  Two
+// allocations; the planes and capture-exit planes start zero, the history
+// at -1 (no step run).  A pooled context's record block is written by
+// setup_pooled, in one copy; any other's here.
 int ctx_alloc(tbg_ctx *c, bool with_codes, hipStream_t ext = nullptr)
 {
   Mat &M = c->h;
@@ -559,39 +634,22 @@ int ctx_alloc(tbg_ctx *c, bool with_codes, hipStream_t ext = nullptr)
   } else {
     HIPCHK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
   }
-  size_t pb = c->pw * 8;
-  uint64_t np = planes_of(M);
-  TRY(dalloc(c, (void **)&c->arena, pb * np));
-  c->bytes += pb * np;
-  assign_planes(c);
-  HIPCHK(hipMemsetAsync(c->arena, 0, pb * np, c->st));
-  uint64_t sw = 4 * c->nsl * (M.ctotal[0] + M.ctotal[1]);
-  TRY(dalloc(c, (void **)&c->subarena, sw * 8 + 64));
-  c->bytes += sw * 8;
-  HIPCHK(hipMemsetAsync(c->subarena, 0, sw * 8 + 64, c->st));
-  uint64_t *p = c->subarena;
-  for (int col = 0; col < 2; col++)
-    for (int k = 0; k < 4; k++) {
-      c->sub[col][k] = p;
-      p += c->nsl * M.ctotal[col];
-    }
-  if (with_codes)
-    for (int s = 0; s < 2; s++) {
-      size_t cb = c->nsl * M.ii.size * 2;
-      TRY(dalloc(c, (void **)&c->codes[s], cb));
-      c->bytes += cb;
-    }
-  TRY(dalloc(c, (void **)&c->dM, sizeof(Mat)));
-  HIPCHK(hipMemcpyAsync(c->dM, &c->h, sizeof(Mat), hipMemcpyHostToDevice, c->st));
-  TRY(dalloc(c, (void **)&c->dR, sizeof(SubRef) * MAXS));
-  HIPCHK(hipMemsetAsync(c->dR, 0, sizeof(SubRef) * MAXS, c->st));
-  TRY(dalloc(c, (void **)&c->dcnt, 16 * sizeof(unsigned long long)));
-  HIPCHK(hipMemsetAsync(c->dcnt, 0, 16 * sizeof(unsigned long long), c->st));
-  TRY(dalloc(c, (void **)&c->dflags, 4 * sizeof(unsigned)));
-  TRY(dalloc(c, (void **)&c->dj, sizeof(JobDev)));
-  TRY(dalloc(c, (void **)&c->djp, sizeof(JobDev *)));
-  HIPCHK(hipMemcpyAsync(c->djp, &c->dj, sizeof(JobDev *), hipMemcpyHostToDevice, c->st));
-  TRY(dalloc(c, (void **)&c->dhist, sizeof(long long) * 4 * c->max_ply));
+  CtxLayout L = ctx_layout(M, c->nsl, c->max_ply, with_codes);
+  void *state = nullptr, *rec = nullptr;
+  TRY(dalloc(c, &state, L.state));
+  carve_state(c, state, L, with_codes);
+  TRY(dalloc(c, &rec, L.rec));
+  carve_rec(c, rec, L);
+  c->state_bytes = L.state;
+  c->bytes = L.state + L.rec;
+  HIPCHK(hipMemsetAsync(state, 0, L.o_codes, c->st));
+  if (!c->pooled) {
+    HIPCHK(hipMemsetAsync(c->dhist, 0xff, sizeof(long long) * 4 * c->max_ply, c->st));
+    HIPCHK(hipMemcpyAsync(c->dM, &c->h, sizeof(Mat), hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemsetAsync(c->dR, 0, sizeof(SubRef) * MAXS, c->st));
+    HIPCHK(hipMemsetAsync(c->dcnt, 0, 16 * sizeof(unsigned long long), c->st));
+    HIPCHK(hipMemcpyAsync(c->djp, &c->dj, sizeof(JobDev *), hipMemcpyHostToDevice, c->st));
+  }
   c->hist.assign((size_t)4 * c->max_ply, -1);  // sized once: finish_job refills it in place
   manager_order(WHITE, c->mgr[WHITE]);
   manager_order(BLACK, c->mgr[BLACK]);
@@ -646,8 +704,11 @@ unsigned cap_grid(uint64_t want, uint64_t cap) { return (unsigned)(want < 1 ? 1 
 
 // One launch of a job kernel.  kt: KT_*; kind: 0 L step, 1 W step.
 // prof: context receiving timings (null inside graph capture).
+// k_uncap_j's resident work-groups per CU (its launch bounds)
+constexpr uint64_t kUncapResident = 5;
+
 int launch(tbg_ctx *prof, JobDev *const *jobs, const LaunchSet &L, int kt, int kind, int sw0,
-           const unsigned *flags, hipStream_t st)
+           const unsigned *flags, hipStream_t st, bool lean = false)
 {
   uint64_t total = L.total[kt];
   if (!total) return TBG_OK;
@@ -665,7 +726,8 @@ int launch(tbg_ctx *prof, JobDev *const *jobs, const LaunchSet &L, int kt, int k
   }
   case KT_UNCAP: {
     Timed t(prof, CAT_UNCAP);
-    hipLaunchKernelGGL(k_uncap_j, dim3(cap_grid(total, (uint64_t)g_cus * L.g.wg[kt])), dim3(256),
+    uint64_t per_cu = lean && L.g.wg[kt] > kUncapResident ? kUncapResident : L.g.wg[kt];
+    hipLaunchKernelGGL(k_uncap_j, dim3(cap_grid(total, (uint64_t)g_cus * per_cu)), dim3(256),
                        L.uncap_lds * 8, st, jobs, L.pairs, L.P[kt], L.np, sw0, kind, flags,
                        L.uncap_lds);
     break;
@@ -694,14 +756,18 @@ int launch(tbg_ctx *prof, JobDev *const *jobs, const LaunchSet &L, int kt, int k
 // One whole ply of every job of a launch set (batched form; also what a
 // graph captures): the L and W candidates of both sides (one pred launch),
 // then verification, the sweep and the control kernel.
+// lean_uncap: the host does not know whether some job is at ply 1 or 101
+// (the kernel reads it from flags): the uncapture grids are capped at one
+// round of resident work-groups, so that a ply without one costs little.
 int launch_ply(tbg_ctx *prof, JobDev *const *jobs, int nj, const LaunchSet &L, unsigned *flags,
-               hipStream_t st, bool edge = true, unsigned *done_out = nullptr)
+               hipStream_t st, bool edge = true, unsigned *done_out = nullptr,
+               bool lean_uncap = false)
 {
   // the L and W steps mark separate candidate planes from different
   // frontiers, so their uncaptures and preds all come first (the preds in
   // one launch); verification must see wins(opp) before this ply's sweep
-  if (edge) TRY(launch(prof, jobs, L, KT_UNCAP, 0, 0, flags, st));
-  if (edge) TRY(launch(prof, jobs, L, KT_UNCAP, 1, 0, flags, st));
+  if (edge) TRY(launch(prof, jobs, L, KT_UNCAP, 0, 0, flags, st, lean_uncap));
+  if (edge) TRY(launch(prof, jobs, L, KT_UNCAP, 1, 0, flags, st, lean_uncap));
   TRY(launch(prof, jobs, L, KT_PRED, 2, 0, flags, st));
   TRY(launch(prof, jobs, L, KT_VERIFY, 0, 0, flags, st));
   TRY(launch(prof, jobs, L, KT_SWEEP, 1, 0, flags, st));
@@ -776,11 +842,13 @@ int zero_plane(tbg_ctx *c, uint64_t *p)
 
 // illegal and mate positions, codes, job state; nothing read back (the
 // counts stay in dcnt[0..3] until read_init_counts)
-int run_init_async(tbg_ctx *c)
+// The init kernels: illegal positions, mates and exit candidates, and with
+// the capture exits resolved the static refutations; the counts go to
+// dcnt[0..5], which must be zero.
+int launch_init(tbg_ctx *c)
 {
   Mat &M = c->h;
   int s0 = M.synth ? M.synth_slice : 0, ns = M.synth ? 1 : NSL;
-  HIPCHK(hipMemsetAsync(c->dcnt, 0, 6 * sizeof(unsigned long long), c->st));
   {
     Timed t(c, CAT_INIT);
     // one placement per lane (pl_layout); with the capture exits resolved,
@@ -818,6 +886,13 @@ int run_init_async(tbg_ctx *c)
     }
     HIPCHK(hipGetLastError());
   }
+  return TBG_OK;
+}
+
+int run_init_async(tbg_ctx *c)
+{
+  HIPCHK(hipMemsetAsync(c->dcnt, 0, 6 * sizeof(unsigned long long), c->st));
+  TRY(launch_init(c));
   c->counts.nplies = 0;
   c->inited = 1;
   c->generated = 0;
@@ -835,6 +910,55 @@ int read_init_counts(tbg_ctx *c, hipStream_t st)
   c->counts.mates[0] = v[2];
   c->counts.mates[1] = v[3];
   return TBG_OK;
+}
+
+// the capture-exit classification of every set (k_resolve), from the
+// capture tables at dR
+int launch_resolve(tbg_ctx *c)
+{
+  const Mat &M = c->h;
+  int s0 = M.synth ? M.synth_slice : 0, ns = M.synth ? 1 : NSL;
+  for (int k = 0; k < M.numsets; k++) {
+    int col = M.setcol[k];
+    uint64_t waves = (uint64_t)ns * M.cwords[k];
+    Timed t(c, CAT_RESOLVE);
+    hipLaunchKernelGGL(k_resolve, dim3(cap_grid((waves + 3) / 4, (uint64_t)g_cus * 16)), dim3(256), 0,
+                       c->st, c->dM, c->dR + k, col, k, c->sub[col][0], c->sub[col][1],
+                       c->sub[col][2], c->sub[col][3], s0, ns);
+    HIPCHK(hipGetLastError());
+  }
+  return TBG_OK;
+}
+
+// Set-up of a pooled (server) context of a synthetic material, queued on
+// its stream without a host wait or a pageable copy: the whole record block
+// (material, capture tables, zero counts, flags, job record, its pointer, a
+// history of -1: no step run) goes up in one copy from `stage` (pinned host
+// memory of at least the layout's rec bytes, untouched until the copy has
+// run), then the capture-exit classification and the init kernels.
+int setup_pooled(tbg_ctx *c, void *stage)
+{
+  if (!c->pooled || !c->h.synth) return fail(TBG_E_STATE, "not a pooled synthetic context");
+  memset(c->hR, 0, sizeof c->hR);  // hashed exits: no smaller tables
+  c->resolved = 1;
+  c->xr_ok = true;  // launch_init works the static refutations out
+  c->counts.nplies = 0;
+  c->inited = 1;
+  c->generated = 0;
+  reset_job(c);
+  CtxLayout L = ctx_layout(c);
+  char *h = static_cast<char *>(stage);
+  memset(h, 0, L.o_hist);
+  memcpy(h + L.o_m, &c->h, sizeof(Mat));
+  memcpy(h + L.o_r, c->hR, sizeof(SubRef) * MAXS);
+  unsigned f[4] = { 0, 1, 0, 0 };  // no job done; an uncapture ply pending
+  memcpy(h + L.o_flags, f, sizeof f);
+  memcpy(h + L.o_j, &c->hj, sizeof(JobDev));
+  memcpy(h + L.o_jp, &c->dj, sizeof(JobDev *));
+  memset(h + L.o_hist, 0xff, L.rec - L.o_hist);
+  HIPCHK(hipMemcpyAsync(c->rec, h, L.rec, hipMemcpyHostToDevice, c->st));
+  TRY(launch_resolve(c));
+  return launch_init(c);
 }
 
 int run_init(tbg_ctx *c)
@@ -933,14 +1057,10 @@ int run_batch(tbg_ctx *prof, JobDev *const *jobs, int nj, const LaunchSet &L, un
 
 // after a generation: pull the final job state and the per-ply counts
 // st: the stream to read on (a pooled job's own stream is shared)
-int finish_job(tbg_ctx *c, hipStream_t st = nullptr)
+// the host's account of a finished job from its final job record (hj) and
+// per-ply history (hist)
+int finish_counts(tbg_ctx *c)
 {
-  if (!st) st = c->st;
-  HIPCHK(hipMemcpyAsync(&c->hj, c->dj, sizeof(JobDev), hipMemcpyDeviceToHost, st));
-  c->hist.assign((size_t)4 * c->max_ply, -1);
-  HIPCHK(hipMemcpyAsync(c->hist.data(), c->dhist, sizeof(long long) * 4 * c->max_ply,
-                        hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
   // the reference's loop has no ply cap: reaching it with moves still to
   // resolve would leave a truncated table
   if (c->hj.done == 1 && c->hj.more && c->hj.n >= c->max_ply && !c->ply_cap_asked)
@@ -957,6 +1077,17 @@ int finish_job(tbg_ctx *c, hipStream_t st = nullptr)
   c->hs_generated += hs;
   c->generated = 1;
   return TBG_OK;
+}
+
+int finish_job(tbg_ctx *c, hipStream_t st = nullptr)
+{
+  if (!st) st = c->st;
+  HIPCHK(hipMemcpyAsync(&c->hj, c->dj, sizeof(JobDev), hipMemcpyDeviceToHost, st));
+  c->hist.assign((size_t)4 * c->max_ply, -1);
+  HIPCHK(hipMemcpyAsync(c->hist.data(), c->dhist, sizeof(long long) * 4 * c->max_ply,
+                        hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return finish_counts(c);
 }
 
 uint64_t executed_half_steps(const tbg_ctx *c)
@@ -1222,9 +1353,8 @@ void tbg_destroy(tbg_ctx *c)
   flush_timing(c);
   c->both.release();
   for (int s = 0; s < 2; s++) c->side[s].release(), c->plain[s].release();
-  void *bufs[] = { c->arena, c->subarena, c->codes[0], c->codes[1], c->dM, c->dR,
-                   c->dcnt, c->dflags, c->dj, c->djp, c->dhist };
-  for (void *b : bufs) dfree(c, b);
+  dfree(c, c->arena);  // the state block
+  dfree(c, c->rec);
   if (c->ready) (void)hipEventDestroy(c->ready);
   if (c->hflags) (void)hipHostFree(c->hflags);
   if (c->st && !c->pooled) (void)hipStreamDestroy(c->st);
@@ -1293,16 +1423,7 @@ int tbg_resolve_captures(tbg_ctx *c, tbg_ctx *const *subs, int nsubs)
     }
   }
   HIPCHK(hipMemcpyAsync(c->dR, c->hR, sizeof(SubRef) * MAXS, hipMemcpyHostToDevice, c->st));
-  int s0 = M.synth ? M.synth_slice : 0, ns = M.synth ? 1 : NSL;
-  for (int k = 0; k < M.numsets; k++) {
-    int col = M.setcol[k];
-    uint64_t waves = (uint64_t)ns * M.cwords[k];
-    Timed t(c, CAT_RESOLVE);
-    hipLaunchKernelGGL(k_resolve, dim3(cap_grid((waves + 3) / 4, (uint64_t)g_cus * 16)), dim3(256), 0,
-                       c->st, c->dM, c->dR + k, col, k, c->sub[col][0], c->sub[col][1],
-                       c->sub[col][2], c->sub[col][3], s0, ns);
-    HIPCHK(hipGetLastError());
-  }
+  TRY(launch_resolve(c));
   if (!c->pooled) HIPCHK(hipStreamSynchronize(c->st));  // a pooled job is waited for by its event
   c->resolved = 1;
   if (c->xr_ok) {  // worked out from the earlier capture exits: verify every move
@@ -2607,50 +2728,31 @@ extern "C" int tbg_host_selftest(char *msg, int len)
 // ====================================================================
 namespace {
 
-// Preemption: a resident context's whole device state (bitmap planes,
-// capture-exit planes, codes and job record) goes to pinned host memory and
-// its device memory back to the pool; the copy back rebases the job
-// record's plane pointers onto the new allocations.  Runs on the context's
-// own stream (the server's set-up stream), with the batch idle.
+// Preemption: a resident context's state block (bitmap planes, capture-exit
+// planes, codes) goes to pinned host memory in one copy, with the job
+// record, and returns to the pool; the copy back rebases the job record's
+// plane pointers onto the new block.  The record block stays resident.
+// Runs on the context's own stream (the server's set-up stream), with the
+// batch idle.
 struct CtxImage {
   void *host = nullptr;
   size_t bytes = 0;
-  uint64_t *arena = nullptr, *subarena = nullptr;  // the released allocations
-  uint16_t *codes[2] = { nullptr, nullptr };
+  void *state = nullptr;  // the released block
 };
-
-size_t ctx_arena_bytes(const tbg_ctx *c) { return c->pw * 8 * planes_of(c->h); }
-size_t ctx_sub_bytes(const tbg_ctx *c)
-{
-  return 4 * c->nsl * (c->h.ctotal[0] + c->h.ctotal[1]) * 8 + 64;
-}
-size_t ctx_codes_bytes(const tbg_ctx *c) { return c->codes[0] ? c->nsl * c->h.ii.size * 2 : 0; }
 
 int ctx_spill(tbg_ctx *c, CtxImage &im)
 {
-  size_t ab = ctx_arena_bytes(c), sb = ctx_sub_bytes(c), cb = ctx_codes_bytes(c);
-  size_t total = ab + sb + 2 * cb;
+  size_t total = c->state_bytes;
   if (!im.host) {
     HIPCHK(hipHostMalloc(&im.host, total, hipHostMallocDefault));
     im.bytes = total;
   }
-  char *h = static_cast<char *>(im.host);
-  HIPCHK(hipMemcpyAsync(h, c->arena, ab, hipMemcpyDeviceToHost, c->st));
-  HIPCHK(hipMemcpyAsync(h + ab, c->subarena, sb, hipMemcpyDeviceToHost, c->st));
-  for (int s = 0; s < 2 && cb; s++)
-    HIPCHK(hipMemcpyAsync(h + ab + sb + s * cb, c->codes[s], cb, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipMemcpyAsync(im.host, c->arena, total, hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipMemcpyAsync(&c->hj, c->dj, sizeof(JobDev), hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
-  im.arena = c->arena;
-  im.subarena = c->subarena;
+  im.state = c->arena;
   dfree(c, c->arena);
-  dfree(c, c->subarena);
   c->arena = c->subarena = nullptr;
-  for (int s = 0; s < 2; s++) {
-    im.codes[s] = c->codes[s];
-    dfree(c, c->codes[s]);
-    c->codes[s] = nullptr;
-  }
   return TBG_OK;
 }
 
@@ -2663,37 +2765,25 @@ template <class X> X *rebase(X *p, const void *old, size_t n, void *now)
 
 int ctx_restore_image(tbg_ctx *c, CtxImage &im)
 {
-  size_t ab = ctx_arena_bytes(c);
-  size_t sb = 4 * c->nsl * (c->h.ctotal[0] + c->h.ctotal[1]) * 8 + 64;
-  size_t cb = im.codes[0] ? c->nsl * c->h.ii.size * 2 : 0;
-  TRY(dalloc(c, (void **)&c->arena, ab));
-  TRY(dalloc(c, (void **)&c->subarena, sb));
-  for (int s = 0; s < 2 && cb; s++) TRY(dalloc(c, (void **)&c->codes[s], cb));
-  const char *h = static_cast<const char *>(im.host);
-  HIPCHK(hipMemcpyAsync(c->arena, h, ab, hipMemcpyHostToDevice, c->st));
-  HIPCHK(hipMemcpyAsync(c->subarena, h + ab, sb, hipMemcpyHostToDevice, c->st));
-  for (int s = 0; s < 2 && cb; s++)
-    HIPCHK(hipMemcpyAsync(c->codes[s], h + ab + sb + s * cb, cb, hipMemcpyHostToDevice, c->st));
-  // the canonical planes in their allocation order (ctx_alloc), and the job
-  // record's (swapped) plane pointers moved onto the new allocations
-  assign_planes(c);
-  uint64_t *p = c->subarena;
-  for (int col = 0; col < 2; col++)
-    for (int k = 0; k < 4; k++) {
-      c->sub[col][k] = p;
-      p += c->nsl * c->h.ctotal[col];
-    }
+  CtxLayout L = ctx_layout(c->h, c->nsl, c->max_ply, c->codes[0] != nullptr);
+  size_t n = c->state_bytes;
+  void *state = nullptr;
+  TRY(dalloc(c, &state, n));
+  HIPCHK(hipMemcpyAsync(state, im.host, n, hipMemcpyHostToDevice, c->st));
+  // the canonical planes at their places in the new block, and the job
+  // record's (swapped) plane pointers moved with them
+  carve_state(c, state, L, c->codes[0] != nullptr);
   JobDev &J = c->hj;
   for (int s = 0; s < 2; s++) {
-    J.wins[s] = rebase(J.wins[s], im.arena, ab, c->arena);
-    J.prevW[s] = rebase(J.prevW[s], im.arena, ab, c->arena);
-    J.prevL[s] = rebase(J.prevL[s], im.arena, ab, c->arena);
-    J.newW[s] = rebase(J.newW[s], im.arena, ab, c->arena);
-    J.newL[s] = rebase(J.newL[s], im.arena, ab, c->arena);
-    J.cand[s] = rebase(J.cand[s], im.arena, ab, c->arena);
-    J.candW[s] = rebase(J.candW[s], im.arena, ab, c->arena);
-    J.xr[s] = rebase(J.xr[s], im.arena, ab, c->arena);
-    for (int k = 0; k < 4; k++) J.sub[s][k] = rebase(J.sub[s][k], im.subarena, sb, c->subarena);
+    J.wins[s] = rebase(J.wins[s], im.state, n, state);
+    J.prevW[s] = rebase(J.prevW[s], im.state, n, state);
+    J.prevL[s] = rebase(J.prevL[s], im.state, n, state);
+    J.newW[s] = rebase(J.newW[s], im.state, n, state);
+    J.newL[s] = rebase(J.newL[s], im.state, n, state);
+    J.cand[s] = rebase(J.cand[s], im.state, n, state);
+    J.candW[s] = rebase(J.candW[s], im.state, n, state);
+    J.xr[s] = rebase(J.xr[s], im.state, n, state);
+    for (int k = 0; k < 4; k++) J.sub[s][k] = rebase(J.sub[s][k], im.state, n, state);
     J.codes[s] = c->codes[s];
   }
   HIPCHK(hipMemcpyAsync(c->dj, &c->hj, sizeof(JobDev), hipMemcpyHostToDevice, c->st));
@@ -2704,11 +2794,17 @@ int ctx_restore_image(tbg_ctx *c, CtxImage &im)
 struct SrvSlice {
   int slice = 0;
   tbg_ctx *ctx = nullptr;
-  uint16_t *pinned[2] = { nullptr, nullptr };  // its final codes
-  CtxImage image;                              // its device state while preempted
+  // pinned host memory, allocated at submission: the final codes of each
+  // side (pinned[stm]), the staging of the set-up copy (stage) and the
+  // landing place of the finished record (fin: counts, flags, job record,
+  // its pointer and history, as in the record block from o_cnt on)
+  void *host = nullptr;
+  uint16_t *pinned[2] = { nullptr, nullptr };
+  char *stage = nullptr, *fin = nullptr;
+  CtxImage image;  // its device state while preempted
   std::vector<long long> hist;
   tbg_counts counts;
-  bool done = false;    // finished: codes copied (or copying) to pinned memory
+  bool done = false;    // finished: record and codes copied (or copying) to pinned memory
   bool landed = false;  // those copies complete
   bool narrow = false;  // the pinned codes are one byte each (every ply below 64)
   hipEvent_t ev = nullptr;
@@ -2724,13 +2820,26 @@ struct SrvJob {
   int left = 0;        // slices whose codes have not landed
 };
 
+// a served slice's two device blocks (a synthetic context keeping codes)
+CtxLayout served_layout(const Mat &M) { return ctx_layout(M, 1, kMaxPly, true); }
+
 uint64_t estimate_bytes(const char *material, int *err)
 {
   Mat M;
   *err = setup_mat(M, material);
   if (*err) return 0;
-  uint64_t pw = M.words, sw = 4 * (M.ctotal[0] + M.ctotal[1]);
-  return pw * 8 * planes_of(M) + sw * 8 + 2 * M.ii.size * 2 + 65536;
+  CtxLayout L = served_layout(M);
+  return L.state + L.rec;
+}
+
+// pinned bytes of a served slice (codes of both sides, staging, landing)
+size_t served_host_bytes(const Mat &M, size_t *o_stage, size_t *o_fin)
+{
+  CtxLayout L = served_layout(M);
+  size_t cb = part_up(M.ii.size * 2);
+  *o_stage = 2 * cb;
+  *o_fin = *o_stage + part_up(L.rec);
+  return *o_fin + part_up(L.rec - L.o_cnt);
 }
 
 }  // namespace
@@ -2763,7 +2872,7 @@ struct Serve {
   std::vector<int> waiting;                    // jobs in arrival order
   std::vector<std::pair<int, int>> active;     // (job, slice) in the batch
   std::vector<std::pair<int, int>> joining;    // set-up or restore queued
-  std::vector<std::pair<int, int>> spilling;   // codes on their way to pinned memory
+  std::vector<std::pair<int, int>> spilling;   // record and codes on their way to pinned memory
   std::vector<int> preempted;                  // jobs on the host, oldest first
   uint64_t used = 0, peak = 0;
   int peak_active = 0;
@@ -2784,8 +2893,8 @@ struct Serve {
     s->cv.notify_all();
   }
 
-  // set-up of one slice: pooled allocation, capture exits, init, queued on
-  // the set-up stream
+  // set-up of one slice: two pool allocations, one copy, the capture-exit
+  // and init kernels, all queued on the set-up stream
   int admit_slice(int ji, int k)
   {
     SrvJob &j = J(ji);
@@ -2793,8 +2902,7 @@ struct Serve {
     tbg_ctx *c = nullptr;
     TRY(create_common(&c, j.material.c_str(), 1, q.slice, j.spec.seed, true, s->setup));
     q.ctx = c;
-    TRY(tbg_resolve_captures(c, nullptr, 0));
-    TRY(run_init_async(c));
+    TRY(setup_pooled(c, q.stage));
     HIPCHK(hipEventCreateWithFlags(&c->ready, hipEventDisableTiming));
     HIPCHK(hipEventRecord(c->ready, s->setup));
     used += c->bytes;
@@ -2818,7 +2926,8 @@ struct Serve {
     return TBG_OK;
   }
 
-  // with the batch idle: job ji's unfinished slices to pinned host memory
+  // with the batch idle: job ji's unfinished slices' state blocks to pinned
+  // host memory
   int preempt(int ji)
   {
     SrvJob &j = J(ji);
@@ -2827,7 +2936,7 @@ struct Serve {
       SrvSlice &q = j.sl[k];
       if (q.done) continue;
       TRY(ctx_spill(q.ctx, q.image));
-      used -= q.ctx->bytes;
+      used -= q.ctx->state_bytes;
       active.erase(std::remove(active.begin(), active.end(), std::make_pair(ji, k)), active.end());
     }
     j.res.spills++;
@@ -2848,7 +2957,7 @@ struct Serve {
       if (q.done) continue;
       TRY(ctx_restore_image(q.ctx, q.image));
       HIPCHK(hipEventRecord(q.ctx->ready, s->setup));
-      used += q.ctx->bytes;
+      used += q.ctx->state_bytes;
       joining.push_back({ ji, k });
     }
     peak = used > peak ? used : peak;
@@ -2856,6 +2965,75 @@ struct Serve {
     spill_ms += s->now_ms() - t;
     std::lock_guard<std::mutex> lk(s->mu);
     j.state = 1;
+    return TBG_OK;
+  }
+
+  // a slice the batch finished (its done word from k_ctrl: the done value
+  // and the last ply that found positions): its record and codes go to
+  // pinned memory on the side stream, narrowed to a byte each when every
+  // ply is below 64 (on the device, into the no longer needed planes, one
+  // side after the other: the side stream orders each narrowing after the
+  // previous copy).  The batch has completed the plies that touched it.
+  int leave(std::pair<int, int> pr, unsigned word)
+  {
+    SrvSlice &q = J(pr.first).sl[pr.second];
+    tbg_ctx *c = q.ctx;
+    CtxLayout L = ctx_layout(c);
+    uint64_t n = c->h.ii.size;
+    int maxn = (int)(word >> 2);
+    HIPCHK(hipMemcpyAsync(q.fin, reinterpret_cast<char *>(c->rec) + L.o_cnt, L.rec - L.o_cnt,
+                          hipMemcpyDeviceToHost, s->side));
+    q.narrow = maxn < 64 && L.arena >= n;
+    for (int stm = 0; stm < 2; stm++) {
+      if (q.narrow) {
+        uint8_t *tmp = reinterpret_cast<uint8_t *>(c->arena);
+        hipLaunchKernelGGL(k_codes8, dim3(cap_grid((n + 2047) / 2048, (uint64_t)g_cus * 8)),
+                           dim3(256), 0, s->side, c->codes[stm], n, tmp);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(q.pinned[stm], tmp, n, hipMemcpyDeviceToHost, s->side));
+      } else {
+        HIPCHK(hipMemcpyAsync(q.pinned[stm], c->codes[stm], n * 2, hipMemcpyDeviceToHost, s->side));
+      }
+    }
+    if (!q.ev) HIPCHK(hipEventCreateWithFlags(&q.ev, hipEventDisableTiming));
+    HIPCHK(hipEventRecord(q.ev, s->side));
+    q.done = true;
+    spilling.push_back(pr);
+    return TBG_OK;
+  }
+
+  // a left slice whose copies have completed: its account from the landed
+  // record, its device memory back to the pool
+  int land(std::pair<int, int> pr)
+  {
+    SrvJob &j = J(pr.first);
+    SrvSlice &q = j.sl[pr.second];
+    tbg_ctx *c = q.ctx;
+    CtxLayout L = ctx_layout(c);
+    const char *f = q.fin;
+    const unsigned long long *cnt = reinterpret_cast<const unsigned long long *>(f);
+    memcpy(&c->hj, f + (L.o_j - L.o_cnt), sizeof(JobDev));
+    const long long *h = reinterpret_cast<const long long *>(f + (L.o_hist - L.o_cnt));
+    c->hist.assign(h, h + (size_t)4 * c->max_ply);
+    TRY(finish_counts(c));
+    for (int i = 0; i < 2; i++) {
+      c->counts.broken[i] = cnt[i];
+      c->counts.mates[i] = cnt[2 + i];
+    }
+    q.hist = c->hist;
+    q.counts = c->counts;
+    j.res.nplies = std::max(j.res.nplies, c->counts.nplies);
+    uint64_t hs = executed_half_steps(c);
+    j.res.half_steps += hs;
+    psp += (double)hs * (double)c->h.ii.size / 2.0;
+    used -= c->bytes;
+    tbg_destroy(c);
+    q.ctx = nullptr;
+    q.landed = true;
+    if (--j.left == 0) {
+      done++;
+      complete(pr.first, TBG_OK);
+    }
     return TBG_OK;
   }
 
@@ -2880,10 +3058,77 @@ struct Serve {
       if (used - freed + w.bytes <= s->budget) break;
       out.push_back(v);
       for (auto &q : J(v).sl)
-        if (!q.done) freed += q.ctx->bytes;
+        if (!q.done) freed += q.ctx->state_bytes;
     }
     if (used - freed + w.bytes > s->budget) out.clear();
     return out;
+  }
+};
+
+// The running batch's device arrays in one block, written in one copy per
+// rebuild from a pinned mirror: the job pointers, the flags, the (job,
+// side) pairs and the four prefix-sum arrays of the work units; the done
+// words (written by k_ctrl) follow, outside the copy.
+struct BatchArrays {
+  size_t o_flags = 0, o_pairs = 0, o_P = 0, pstride = 0, o_done = 0, bytes = 0;
+  char *dev = nullptr, *host = nullptr;
+  unsigned *hflags = nullptr, *hdone = nullptr;  // pinned: what k_ctrl left
+
+  int alloc(int m)
+  {
+    o_flags = part_up(sizeof(JobDev *) * m);
+    o_pairs = o_flags + kPart;
+    pstride = part_up(sizeof(uint64_t) * (2 * m + 1));
+    o_P = o_pairs + part_up(sizeof(Pair) * 2 * m);
+    o_done = o_P + 4 * pstride;
+    bytes = o_done + part_up(sizeof(unsigned) * m);
+    HIPCHK(hipMalloc(&dev, bytes));
+    HIPCHK(hipHostMalloc((void **)&host, bytes, hipHostMallocDefault));
+    HIPCHK(hipHostMalloc((void **)&hflags, 4 * sizeof(unsigned), hipHostMallocDefault));
+    HIPCHK(hipHostMalloc((void **)&hdone, sizeof(unsigned) * m, hipHostMallocDefault));
+    return TBG_OK;
+  }
+  void release()
+  {
+    (void)hipFree(dev);
+    (void)hipHostFree(host);
+    (void)hipHostFree(hflags);
+    (void)hipHostFree(hdone);
+  }
+  JobDev **djobs() const { return reinterpret_cast<JobDev **>(dev); }
+  unsigned *dflags() const { return reinterpret_cast<unsigned *>(dev + o_flags); }
+  unsigned *ddone() const { return reinterpret_cast<unsigned *>(dev + o_done); }
+
+  // the launch set of the slices `ctx` (both sides of each) planned into
+  // the mirror and copied up on stream st; L's arrays point into the block
+  int build(LaunchSet &L, const std::vector<tbg_ctx *> &ctx, const Geom &g, hipStream_t st)
+  {
+    int na = (int)ctx.size();
+    std::vector<const JobDev *> hj(na);
+    std::vector<const Mat *> mats(na);
+    Pair *pairs = reinterpret_cast<Pair *>(host + o_pairs);
+    JobDev **ptrs = reinterpret_cast<JobDev **>(host);
+    for (int i = 0; i < na; i++) {
+      ptrs[i] = ctx[i]->dj;
+      hj[i] = &ctx[i]->hj;
+      mats[i] = &ctx[i]->h;
+      pairs[2 * i] = { i, 0 };
+      pairs[2 * i + 1] = { i, 1 };
+    }
+    unsigned f[4] = { 0, 1, 0, 0 };  // no job done; a new slice may be at ply 1
+    memcpy(host + o_flags, f, sizeof f);
+    // plan_launchset writes the four prefix arrays back to back; each goes
+    // to its own aligned place
+    std::vector<uint64_t> P(4 * (2 * (size_t)na + 1));
+    plan_launchset(L, hj.data(), mats.data(), na, pairs, 2 * na, 0, g, P.data());
+    for (int kt = 0; kt < 4; kt++)
+      memcpy(host + o_P + kt * pstride, P.data() + (size_t)kt * (2 * na + 1),
+             sizeof(uint64_t) * (2 * na + 1));
+    L.pairs = reinterpret_cast<Pair *>(dev + o_pairs);
+    for (int kt = 0; kt < 4; kt++) L.P[kt] = reinterpret_cast<uint64_t *>(dev + o_P + kt * pstride);
+    L.cap = 0;  // the arrays belong to this block: LaunchSet::release leaves them
+    HIPCHK(hipMemcpyAsync(dev, host, o_done, hipMemcpyHostToDevice, st));
+    return TBG_OK;
   }
 };
 
@@ -2925,7 +3170,7 @@ void tbg_server_destroy(tbg_server *s)
   for (auto &j : s->jobs)
     for (auto &q : j.sl) {
       if (q.ctx) tbg_destroy(q.ctx);
-      for (auto &p : q.pinned) (void)hipHostFree(p);
+      if (q.host) (void)hipHostFree(q.host);
       if (q.image.host) (void)hipHostFree(q.image.host);
       if (q.ev) (void)hipEventDestroy(q.ev);
     }
@@ -2956,22 +3201,32 @@ int tbg_server_submit(tbg_server *s, const tbg_job_spec *spec, uint64_t *job_id)
   j.bytes = estimate_bytes(spec->material, &err) * (uint64_t)ns;
   j.left = ns;
   j.sl.resize(ns);
-  // the pinned host buffers its codes go to (a pinned allocation takes
-  // milliseconds, so it is made here, not while serving)
+  // each slice's pinned host memory: its codes, the set-up staging, the
+  // landing place of its final record (a pinned allocation takes a
+  // fraction of a millisecond, so it is made here, not while serving)
+  size_t o_stage, o_fin, hb = served_host_bytes(M, &o_stage, &o_fin);
+  size_t cb = part_up(M.ii.size * 2);
+  auto free_host = [&]() {
+    for (auto &q : j.sl)
+      if (q.host) (void)hipHostFree(q.host);
+  };
   for (int k = 0; k < ns; k++) {
-    j.sl[k].slice = (spec->slice + k) % 441;
-    for (int stm = 0; stm < 2; stm++)
-      if (hipHostMalloc((void **)&j.sl[k].pinned[stm], M.ii.size * 2, hipHostMallocDefault) !=
-          hipSuccess) {
-        for (auto &q : j.sl)
-          for (auto &p : q.pinned) (void)hipHostFree(p);
-        return fail(TBG_E_NOMEM, "pinned host memory for a job's codes");
-      }
+    SrvSlice &q = j.sl[k];
+    q.slice = (spec->slice + k) % 441;
+    if (hipHostMalloc(&q.host, hb, hipHostMallocDefault) != hipSuccess) {
+      q.host = nullptr;
+      free_host();
+      return fail(TBG_E_NOMEM, "pinned host memory for a job's results");
+    }
+    char *h = static_cast<char *>(q.host);
+    q.pinned[0] = reinterpret_cast<uint16_t *>(h);
+    q.pinned[1] = reinterpret_cast<uint16_t *>(h + cb);
+    q.stage = h + o_stage;
+    q.fin = h + o_fin;
   }
   std::lock_guard<std::mutex> lk(s->mu);
   if (s->closed) {
-    for (auto &q : j.sl)
-      for (auto &p : q.pinned) (void)hipHostFree(p);
+    free_host();
     return fail(TBG_E_STATE, "the server takes no more jobs");
   }
   double a = spec->arrival_ms;
@@ -3012,22 +3267,17 @@ int tbg_server_serve(tbg_server *s, int form, tbg_server_stats *stats)
   v.max_slices = v.plain ? 1 : s->max_active;
   const int kPlies = 4;
   size_t seen = 0;
-  JobDev **dptr = nullptr;
-  unsigned *dflags = nullptr, *ddone = nullptr, *hflags = nullptr, *hdone = nullptr;
+  BatchArrays B;
   hipEvent_t evb = nullptr;
-  int r = TBG_OK;
-  auto setup_fail = [&](hipError_t e) {
-    if (e != hipSuccess) r = fail(TBG_E_HIP, "server batch buffers: %s", hipGetErrorString(e));
-  };
-  setup_fail(hipMalloc(&dptr, sizeof(JobDev *) * v.max_slices));
-  if (r == TBG_OK) setup_fail(hipMalloc(&dflags, 4 * sizeof(unsigned)));
-  if (r == TBG_OK) setup_fail(hipMalloc(&ddone, sizeof(unsigned) * v.max_slices));
-  if (r == TBG_OK) setup_fail(hipHostMalloc((void **)&hflags, 4 * sizeof(unsigned), hipHostMallocDefault));
-  if (r == TBG_OK)
-    setup_fail(hipHostMalloc((void **)&hdone, sizeof(unsigned) * v.max_slices, hipHostMallocDefault));
-  if (r == TBG_OK) setup_fail(hipEventCreateWithFlags(&evb, hipEventDisableTiming));
+  int r = B.alloc(v.max_slices);
+  if (r == TBG_OK && hipEventCreateWithFlags(&evb, hipEventDisableTiming) != hipSuccess)
+    r = fail(TBG_E_HIP, "server event");
   LaunchSet L;
-  bool changed = true;
+  bool changed = true, inflight = false;
+  int blocked = -1;  // an arrived job that waits for memory
+  // The loop keeps the device busy: the next group of plies is queued as
+  // soon as the previous one has ended and the batch is updated; the host's
+  // bookkeeping (landed results, admission, set-up) runs meanwhile.
   while (r == TBG_OK) {
     // 0. jobs submitted since the last look join the waiting list
     bool closed;
@@ -3041,62 +3291,88 @@ int tbg_server_serve(tbg_server *s, int form, tbg_server_stats *stats)
       }
       closed = s->closed;
     }
-    if (closed && v.waiting.empty() && v.active.empty() && v.joining.empty() &&
+    if (closed && !inflight && v.waiting.empty() && v.active.empty() && v.joining.empty() &&
         v.spilling.empty() && v.preempted.empty())
       break;
     double t = s->now_ms();
     v.occ_integral += (double)v.used * (t - v.last_t);
     v.last_t = t;
-    // 1. the batch: rebuilt when its membership changed, then kPlies plies
-    //    queued without a host wait
-    int na = (int)v.active.size();
-    if (na && changed) {
-      std::vector<JobDev *> ptrs;
-      std::vector<const JobDev *> hj;
-      std::vector<const Mat *> mats;
-      std::vector<Pair> pairs;
-      for (int i = 0; i < na; i++) {
-        tbg_ctx *c = v.J(v.active[i].first).sl[v.active[i].second].ctx;
-        ptrs.push_back(c->dj);
-        hj.push_back(&c->hj);
-        mats.push_back(&c->h);
-        pairs.push_back({ i, 0 });
-        pairs.push_back({ i, 1 });
+    // 1. the group in flight has ended: finished slices leave the batch,
+    //    their record and codes on their way to pinned memory
+    if (inflight) {
+      HIPCHK(hipEventSynchronize(evb));
+      inflight = false;
+      if (B.hflags[0]) {
+        std::vector<std::pair<int, int>> keep;
+        for (int i = 0; i < (int)v.active.size() && r == TBG_OK; i++) {
+          if (B.hdone[i] & 3) r = v.leave(v.active[i], B.hdone[i]);
+          else keep.push_back(v.active[i]);
+        }
+        if (r != TBG_OK) break;
+        v.active = keep;
+        changed = true;
       }
-      HIPCHK(hipMemcpyAsync(dptr, ptrs.data(), sizeof(JobDev *) * na, hipMemcpyHostToDevice, s->st));
-      unsigned f[4] = { 0, 1, 0, 0 };  // a new slice may be at ply 1
-      HIPCHK(hipMemcpyAsync(dflags, f, sizeof f, hipMemcpyHostToDevice, s->st));
-      r = build_launchset(L, hj, mats, pairs, 0, s->geom, s->st);
-      if (r != TBG_OK) break;
-      changed = false;
     }
+    // 2. a blocked job preempts larger admitted jobs, the batch being idle
+    if (blocked >= 0 && !v.plain) {
+      std::vector<int> vic = v.victims(blocked);
+      if (!vic.empty()) {
+        for (int ji : vic) {
+          r = v.preempt(ji);
+          if (r != TBG_OK) break;
+        }
+        if (r != TBG_OK) break;
+        changed = true;
+        v.waiting.erase(std::find(v.waiting.begin(), v.waiting.end(), blocked));
+        r = v.admit(blocked);
+        if (r != TBG_OK) break;
+      }
+    }
+    blocked = -1;
+    // 3. slices whose set-up (or restore) completed join the batch (the
+    //    plain server: one slice at a time)
+    for (size_t i = 0; i < v.joining.size();) {
+      if (v.plain && !v.active.empty()) break;
+      auto pr = v.joining[i];
+      if (hipEventQuery(v.J(pr.first).sl[pr.second].ctx->ready) == hipSuccess) {
+        v.active.push_back(pr);
+        v.joining.erase(v.joining.begin() + i);
+        changed = true;
+      } else {
+        i++;
+      }
+    }
+    v.peak_active = (int)v.active.size() > v.peak_active ? (int)v.active.size() : v.peak_active;
+    // 4. the next group: the batch rebuilt when its membership changed, then
+    //    kPlies plies queued and the flags and done words read back
+    int na = (int)v.active.size();
     if (na) {
+      if (changed) {
+        std::vector<tbg_ctx *> ctx;
+        for (auto &pr : v.active) ctx.push_back(v.J(pr.first).sl[pr.second].ctx);
+        r = B.build(L, ctx, s->geom, s->st);
+        if (r != TBG_OK) break;
+        changed = false;
+      }
       for (int k = 0; k < kPlies && r == TBG_OK; k++) {
-        r = launch_ply(nullptr, dptr, na, L, dflags, s->st, true, ddone);
+        r = launch_ply(nullptr, B.djobs(), na, L, B.dflags(), s->st, true, B.ddone(), true);
         v.plies++;
       }
       if (r != TBG_OK) break;
-      HIPCHK(hipMemcpyAsync(hflags, dflags, 4 * sizeof(unsigned), hipMemcpyDeviceToHost, s->st));
-      HIPCHK(hipMemcpyAsync(hdone, ddone, sizeof(unsigned) * na, hipMemcpyDeviceToHost, s->st));
+      HIPCHK(hipMemcpyAsync(B.hflags, B.dflags(), 4 * sizeof(unsigned), hipMemcpyDeviceToHost, s->st));
+      HIPCHK(hipMemcpyAsync(B.hdone, B.ddone(), sizeof(unsigned) * na, hipMemcpyDeviceToHost, s->st));
       HIPCHK(hipEventRecord(evb, s->st));
+      inflight = true;
     }
-    // 2. host work while those plies run: landed codes, admission (set-up
-    //    and init queued on the set-up stream)
-    for (size_t i = 0; i < v.spilling.size();) {
+    // 5. host work while those plies run: landed results, admission (set-up
+    //    and init queued on the set-up stream), restores
+    for (size_t i = 0; i < v.spilling.size() && r == TBG_OK;) {
       auto pr = v.spilling[i];
-      SrvSlice &q = v.J(pr.first).sl[pr.second];
-      if (hipEventQuery(q.ev) != hipSuccess) { i++; continue; }
-      v.used -= q.ctx->bytes;
-      tbg_destroy(q.ctx);
-      q.ctx = nullptr;
-      q.landed = true;
+      if (hipEventQuery(v.J(pr.first).sl[pr.second].ev) != hipSuccess) { i++; continue; }
       v.spilling.erase(v.spilling.begin() + i);
-      if (--v.J(pr.first).left == 0) {
-        v.done++;
-        v.complete(pr.first, TBG_OK);
-      }
+      r = v.land(pr);
     }
-    int blocked = -1;  // an arrived job that waits for memory
+    if (r != TBG_OK) break;
     while (!v.waiting.empty()) {
       int ji = v.waiting.front();
       SrvJob &j = v.J(ji);
@@ -3124,122 +3400,39 @@ int tbg_server_serve(tbg_server *s, int form, tbg_server_stats *stats)
     while (!v.plain && blocked < 0 && !v.preempted.empty()) {
       int ji = v.preempted.front();
       uint64_t need = 0;
-      for (auto &q : v.J(ji).sl)
-        if (!q.done) need += q.ctx->bytes;
       int ns = 0;
-      for (auto &q : v.J(ji).sl) ns += !q.done;
+      for (auto &q : v.J(ji).sl)
+        if (!q.done) need += q.ctx->state_bytes, ns++;
       if (v.used + need > s->budget || v.slots_used() + ns > v.max_slices) break;
       v.preempted.erase(v.preempted.begin());
       r = v.restore(ji);
       if (r != TBG_OK) break;
     }
     if (r != TBG_OK) break;
-    // 3. wait: for the plies, else for a set-up or a copy, else for the next
+    // 6. nothing running: wait for a set-up, else a copy, else the next
     //    arrival or submission
-    if (na) {
-      HIPCHK(hipEventSynchronize(evb));
-    } else if (!v.joining.empty()) {
-      auto pr = v.joining.front();
-      HIPCHK(hipEventSynchronize(v.J(pr.first).sl[pr.second].ctx->ready));
-    } else if (!v.spilling.empty()) {
-      HIPCHK(hipStreamSynchronize(s->side));
-    } else if (blocked < 0) {
-      std::unique_lock<std::mutex> lk(s->mu);
-      double wait = 50.0;  // look again for submissions at least this often
-      if (!v.waiting.empty())
-        wait = std::min(wait, s->jobs[v.waiting.front()].spec.arrival_ms - s->now_ms());
-      if (wait > 0 && seen == s->jobs.size() && !s->closed)
-        s->cv.wait_for(lk, std::chrono::microseconds((int64_t)(wait * 1000)));
-      else if (wait > 0 && !v.waiting.empty())
-        std::this_thread::sleep_for(std::chrono::microseconds((int64_t)(wait * 1000)));
-    }
-    // 4. finished slices leave the batch; their codes go to pinned memory
-    if (na && hflags[0]) {
-      std::vector<std::pair<int, int>> keep;
-      for (int i = 0; i < na && r == TBG_OK; i++) {
-        auto pr = v.active[i];
-        if (!hdone[i]) { keep.push_back(pr); continue; }
-        SrvJob &j = v.J(pr.first);
-        SrvSlice &q = j.sl[pr.second];
-        tbg_ctx *c = q.ctx;
-        r = finish_job(c, s->st);
-        if (r == TBG_OK) r = read_init_counts(c, s->st);
-        if (r != TBG_OK) break;
-        q.hist = c->hist;
-        q.counts = c->counts;
-        j.res.nplies = std::max(j.res.nplies, c->counts.nplies);
-        uint64_t hs = executed_half_steps(c);
-        j.res.half_steps += hs;
-        v.psp += (double)hs * (double)c->h.ii.size / 2.0;
-        uint64_t n = c->h.ii.size;
-        // every ply below 64: the codes fit a byte, narrowed on the device
-        // into the (no longer needed) plane arena, one side after the other
-        // (the side stream orders each narrowing after the previous copy)
-        int maxn = 0;
-        for (int pn = 1; pn < c->max_ply; pn++)
-          for (int i2 = 0; i2 < 4; i2++)
-            if (c->hist[(size_t)pn * 4 + i2] > 0) maxn = pn;
-        q.narrow = maxn < 64 && ctx_arena_bytes(c) >= n;
-        for (int stm = 0; stm < 2; stm++) {
-          if (q.narrow) {
-            uint8_t *tmp = reinterpret_cast<uint8_t *>(c->arena);
-            hipLaunchKernelGGL(k_codes8, dim3(cap_grid((n + 2047) / 2048, (uint64_t)g_cus * 8)),
-                               dim3(256), 0, s->side, c->codes[stm], n, tmp);
-            HIPCHK(hipGetLastError());
-            HIPCHK(hipMemcpyAsync(q.pinned[stm], tmp, n, hipMemcpyDeviceToHost, s->side));
-          } else {
-            HIPCHK(hipMemcpyAsync(q.pinned[stm], c->codes[stm], n * 2, hipMemcpyDeviceToHost,
-                                  s->side));
-          }
-        }
-        if (!q.ev) HIPCHK(hipEventCreateWithFlags(&q.ev, hipEventDisableTiming));
-        HIPCHK(hipEventRecord(q.ev, s->side));
-        q.done = true;
-        v.spilling.push_back(pr);
-      }
-      if (r != TBG_OK) break;
-      v.active = keep;
-      changed = true;
-    }
-    // 5. a blocked job preempts larger admitted jobs, the batch being idle
-    if (blocked >= 0 && !v.plain) {
-      std::vector<int> vic = v.victims(blocked);
-      if (!vic.empty()) {
-        for (int ji : vic) {
-          r = v.preempt(ji);
-          if (r != TBG_OK) break;
-        }
-        if (r != TBG_OK) break;
-        changed = true;
-        v.waiting.erase(std::find(v.waiting.begin(), v.waiting.end(), blocked));
-        r = v.admit(blocked);
-        if (r != TBG_OK) break;
+    if (!inflight) {
+      if (!v.joining.empty()) {
+        auto pr = v.joining.front();
+        HIPCHK(hipEventSynchronize(v.J(pr.first).sl[pr.second].ctx->ready));
+      } else if (!v.spilling.empty()) {
+        HIPCHK(hipStreamSynchronize(s->side));
+      } else if (blocked < 0) {
+        std::unique_lock<std::mutex> lk(s->mu);
+        double wait = 50.0;  // look again for submissions at least this often
+        if (!v.waiting.empty())
+          wait = std::min(wait, s->jobs[v.waiting.front()].spec.arrival_ms - s->now_ms());
+        if (wait > 0 && seen == s->jobs.size() && !s->closed)
+          s->cv.wait_for(lk, std::chrono::microseconds((int64_t)(wait * 1000)));
+        else if (wait > 0 && !v.waiting.empty())
+          std::this_thread::sleep_for(std::chrono::microseconds((int64_t)(wait * 1000)));
       }
     }
-    // 6. slices whose set-up (or restore) completed join the batch (the
-    //    plain server: one slice at a time)
-    for (size_t i = 0; i < v.joining.size();) {
-      if (v.plain && !v.active.empty()) break;
-      auto pr = v.joining[i];
-      if (hipEventQuery(v.J(pr.first).sl[pr.second].ctx->ready) == hipSuccess) {
-        v.active.push_back(pr);
-        v.joining.erase(v.joining.begin() + i);
-        changed = true;
-      } else {
-        i++;
-      }
-    }
-    v.peak_active = (int)v.active.size() > v.peak_active ? (int)v.active.size() : v.peak_active;
   }
   (void)hipStreamSynchronize(s->st);
   (void)hipStreamSynchronize(s->side);
   (void)hipStreamSynchronize(s->setup);
-  L.release();
-  (void)hipFree(dptr);
-  (void)hipFree(dflags);
-  (void)hipFree(ddone);
-  (void)hipHostFree(hflags);
-  (void)hipHostFree(hdone);
+  B.release();
   if (evb) (void)hipEventDestroy(evb);
   {
     std::lock_guard<std::mutex> lk(s->mu);

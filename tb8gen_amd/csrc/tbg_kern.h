@@ -36,6 +36,7 @@ struct JobDev {
   long long *hist;            // [max_ply][4] counts per step, -1: step did not run
   unsigned long long cnt[4];  // this ply: L(W), L(B), W(W), W(B)
   int n, done;                // done: 1 finished, 2 paused at stop_at
+  int maxn;                   // the last ply whose steps found positions (0: none)
   unsigned mask;              // steps of this ply: bit 0 L(W), 1 L(B), 2 W(W), 3 W(B)
   unsigned more;              // bit 0 more_ww, 1 more_wb, 2 more_lw, 3 more_lb
   unsigned pull;              // bit s: side s's L step of this ply in the pull form
@@ -119,7 +120,7 @@ __global__ __launch_bounds__(256, 5) void k_verify_j(JobDev *const *jobs, const 
 __global__ void k_sweep_j(JobDev *const *jobs, const Pair *pairs, const uint64_t *P, int np, int sw0,
                           int uw);
 // flags[0]: jobs done; flags[1]: some job is at ply 1 or 101 (uncapture)
-// done_out (nullable): done_out[j] = job j finished
+// done_out (nullable): done_out[j] = job j's done value | its maxn << 2
 __global__ void k_ctrl(JobDev *const *jobs, int nj, unsigned *flags, unsigned *done_out);
 // k_init: consecutive bitmap words per wave (positions step by 64 between
 // them, for all-single materials without a full decode)

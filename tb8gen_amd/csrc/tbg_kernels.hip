@@ -1113,7 +1113,7 @@ __global__ __launch_bounds__(256) void k_sweep_j(JobDev *const *__restrict__ job
   // the (job, side) pair's fields, re-read only when a unit leaves its range
   // (the job record does not change during a launch), in scalar registers
   uint64_t plo = 1, phi = 0;
-  bool skip = true, run = false, exits = false, pull = false;
+  bool skip = true, run = false, exits = false;
   int stm = 0, n = 0;
   JobDev *Jp = nullptr;
   uint64_t *cand = nullptr, *wins = nullptr, *newW = nullptr;
@@ -1206,7 +1206,7 @@ __global__ __launch_bounds__(1024) void k_ctrl(JobDev *const *jobs, int nj, unsi
   __syncthreads();
   for (int j = threadIdx.x; j < nj; j += blockDim.x) {
     if (!jobs[j]->done) ctrl_one(*jobs[j], flags, &any_edge);
-    if (done_out) done_out[j] = jobs[j]->done;
+    if (done_out) done_out[j] = (unsigned)jobs[j]->done | ((unsigned)jobs[j]->maxn << 2);
   }
   __syncthreads();
   if (threadIdx.x == 0) flags[1] = any_edge;
@@ -1222,6 +1222,7 @@ __device__ void ctrl_one(JobDev &J, unsigned *flags, unsigned *any_edge)
   bool ww = (J.mask & 4) && J.cnt[2], wb = (J.mask & 8) && J.cnt[3];
   // more_ww <- L_n(B), more_wb <- L_n(W), more_lw <- W_n(B), more_lb <- W_n(W)
   unsigned more = (lb ? 1u : 0u) | (lw ? 2u : 0u) | (wb ? 4u : 0u) | (ww ? 8u : 0u);
+  if (more) J.maxn = n;
   for (int c = 0; c < 2; c++) {
     uint64_t *x = J.prevW[c];
     J.prevW[c] = J.newW[c];
@@ -1733,12 +1734,12 @@ __global__ __launch_bounds__(256, 5) void k_init_pl(const Mat *__restrict__ Mp, 
     // index a per-placement base plus the rank of the moved position's last
     // square; the candidates collect over squares t
     uint64_t xs[4] = { 0, 0, 0, 0 };  // W candidates W, B; L candidates W, B
-    uint64_t xr[2] = { 0, 0 };        // exit refutations W, B
+    uint64_t xr0 = 0, xr1 = 0;        // exit refutations W, B
     if (M.synth && valid) {
       for (int c = 0; c < 2; c++) {
         int o = c ^ 1, k = sq_get(p, c);
         uint64_t ill = c == WHITE ? WI : BI;
-        uint64_t own = 0;
+        uint64_t own = 0, xrc = 0;
         for (int i = 0; i < MAXP - 1; i++) {
           if (i >= M.nk[c]) break;
           int j = M.pcs[c][i];
@@ -1776,15 +1777,35 @@ __global__ __launch_bounds__(256, 5) void k_init_pl(const Mat *__restrict__ Mp, 
               go &= ~bit(to);
             }
           }
-          while (go) {
-            int t = ctz64(go);
-            go &= go - 1;
+          // the hash's high half decides each value; where it ties a
+          // threshold's (about once in 2^31) the whole exit is evaluated
+          // again with full hashes, off the common path
+          uint64_t hw = 0, hl = 0;
+          bool tie = false;
+          for (uint64_t g2 = go; g2; g2 &= g2 - 1) {
+            int t = ctz64(g2);
             int u = sym_sq(t, m, fl);
             uint64_t idx = base + (uint64_t)(uint32_t)(u - popc64(ob & (bit(u) - 1)));
-            int v = synth_value_k(key, idx);
-            xw |= (uint64_t)(v == -2) << t;
-            xl |= (uint64_t)(v == 2) << t;
+            uint64_t z;
+            uint32_t hh = synth_hi(key, idx, &z);
+            uint64_t bt = bit(t);
+            hl |= hh < SYN_TW_HI ? bt : 0ull;                          // value 2
+            hw |= hh - SYN_TW_HI < SYN_TL_HI - SYN_TW_HI ? bt : 0ull;  // value -2
+            tie |= hh == SYN_TW_HI || hh == SYN_TL_HI;
           }
+          if (__ballot(tie)) {
+            hw = hl = 0;
+            for (uint64_t g2 = go; g2; g2 &= g2 - 1) {
+              int t = ctz64(g2);
+              int u = sym_sq(t, m, fl);
+              uint64_t idx = base + (uint64_t)(uint32_t)(u - popc64(ob & (bit(u) - 1)));
+              int v = synth_value_k(key, idx);
+              hw |= (uint64_t)(v == -2) << t;
+              hl |= (uint64_t)(v == 2) << t;
+            }
+          }
+          xw |= hw;
+          xl |= hl;
           if (c == WHITE) {
             xs[0] |= xw;
             xs[2] |= xl;
@@ -1792,8 +1813,10 @@ __global__ __launch_bounds__(256, 5) void k_init_pl(const Mat *__restrict__ Mp, 
             xs[1] |= xw;
             xs[3] |= xl;
           }
-          xr[c] |= legal & ~xl;  // a legal exit not into a win of the opponent
+          xrc |= legal & ~xl;  // a legal exit not into a win of the opponent
         }
+        if (c == WHITE) xr0 = xrc;
+        else xr1 = xrc;
       }
     }
     // rank-domain masks (bit r = position P R + r): the squares' bits
@@ -1834,7 +1857,8 @@ __global__ __launch_bounds__(256, 5) void k_init_pl(const Mat *__restrict__ Mp, 
       rmb = compress(MB);
       if (M.synth) {
         for (int k = 0; k < 4; k++) rx[k] = compress(xs[k]);
-        for (int k = 0; k < 2; k++) rr[k] = compress(xr[k]);
+        rr[0] = compress(xr0);
+        rr[1] = compress(xr1);
       }
     }
     // assemble the chunk's words in LDS (a lane's R bits start at bit
