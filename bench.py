@@ -228,20 +228,11 @@ def run_stream(args, tbg):
         t.close()
 
 
-def run_served(args, tbg):
-    """Config 4: seeded arrivals of king-slice jobs of 6-piece pawnless
-    materials (1 to --max-slices king slices each, seeded hash exits),
-    served with continuous batching under an HBM budget, against the plain
-    server (one slice at a time, FIFO).  The job list is drawn until its
-    admission estimates reach --fill of the budget, so that the capacity run
-    (every job queued at once) loads HBM to about that fraction; latency
-    (arrival -> codes on the host, p50 / p99) is measured at seeded Poisson
-    arrivals of 70% of the batched capacity, and once more under a budget
-    that forces preemption to pinned host memory."""
-    import numpy as np
-    pieces = [int(x) for x in args.served_pieces.split(",")]
+def draw_jobs(args, tbg, pieces, rng):
+    """King-slice jobs of the pawnless materials with the given piece
+    counts (1 to --max-slices slices each, seeded hash exits), drawn until
+    their device bytes reach --fill of the budget."""
     mats = sum((pawnless_materials(p - 2) for p in pieces), [])
-    rng = np.random.default_rng(args.seed)
     budget = int(args.budget_gb * (1 << 30))
     sizes = {}
     for m in mats:
@@ -255,13 +246,34 @@ def run_served(args, tbg):
         jobs.append((m, int(rng.integers(0, 441)), int(rng.integers(1, 1 << 62)), k))
         total += sizes[m][0] * k
     positions = [sizes[m][1] * k for m, _, _, k in jobs]
+    return jobs, total, positions
+
+
+def run_served(args, tbg):
+    """Config 4: seeded arrivals of king-slice jobs of 6-piece pawnless
+    materials (1 to --max-slices king slices each, seeded hash exits),
+    served with continuous batching under an HBM budget, against the plain
+    server (one slice at a time, FIFO).  The job list is drawn until its
+    admission estimates reach --fill of the budget, so that the capacity run
+    (every job queued at once) loads HBM to about that fraction; latency
+    (arrival -> codes on the host, p50 / p99) is measured at seeded Poisson
+    arrivals of 70% of the batched capacity, and once more under a budget
+    that forces preemption to pinned host memory.  The capacity comparison
+    is repeated on a mixed stream of 5- and 6-piece jobs (--mixed-pieces),
+    the stream the batched-over-plain target is stated for."""
+    import numpy as np
+    pieces = [int(x) for x in args.served_pieces.split(",")]
+    rng = np.random.default_rng(args.seed)
+    budget = int(args.budget_gb * (1 << 30))
+    jobs, total, positions = draw_jobs(args, tbg, pieces, rng)
     print(f"served: {len(jobs)} jobs, {sum(k for *_, k in jobs)} slices, "
           f"{min(positions):.3g}-{max(positions):.3g} positions per job, "
           f"{total / 2**30:.1f} GiB of job state", file=sys.stderr, flush=True)
 
-    def serve(arrivals, form, slots, bud):
+    def serve(arrivals, form, slots, bud, jl=None):
+        jl = jobs if jl is None else jl
         srv = tbg.Server(bud, slots)
-        ids = [srv.submit(m, s, seed, a, k) for (m, s, seed, k), a in zip(jobs, arrivals)]
+        ids = [srv.submit(m, s, seed, a, k) for (m, s, seed, k), a in zip(jl, arrivals)]
         st = srv.run(form)
         res = [srv.result(i) for i in ids]
         srv.close()
@@ -283,6 +295,18 @@ def run_served(args, tbg):
         st, _, lat = serve([0.0] * len(jobs), form, args.max_active, budget)
         cap[name] = dict(summary(st, lat), jobs_per_s=len(jobs) / (st.wall_ms / 1000.0))
         print(f"capacity {name}: {cap[name]}", file=sys.stderr, flush=True)
+    mixed = {}
+    if args.mixed_pieces:
+        mp = [int(x) for x in args.mixed_pieces.split(",")]
+        mjobs, mtotal, mpos = draw_jobs(args, tbg, mp, np.random.default_rng(args.seed + 1))
+        for name, form in (("batched", tbg.FORM_BATCHED), ("plain", tbg.FORM_PLAIN)):
+            st, _, lat = serve([0.0] * len(mjobs), form, args.max_active, budget, mjobs)
+            mixed[name] = dict(summary(st, lat), jobs_per_s=len(mjobs) / (st.wall_ms / 1000.0))
+            print(f"mixed capacity {name}: {mixed[name]}", file=sys.stderr, flush=True)
+        mixed = {"pieces": mp, "jobs": len(mjobs), "slices": sum(k for *_, k in mjobs),
+                 "job_state_gb": mtotal / 2**30, "positions_per_job": [min(mpos), max(mpos)],
+                 "capacity": mixed,
+                 "batched_over_plain": mixed["batched"]["value"] / mixed["plain"]["value"]}
     lam = 0.7 * cap["batched"]["jobs_per_s"]  # offered load: 70 % of the batched capacity
     arrivals = list(np.cumsum(rng.exponential(1.0 / lam, len(jobs))) * 1000.0)
     out_modes = {}
@@ -312,6 +336,7 @@ def run_served(args, tbg):
         "latency_ms": {"batched_p50": b["p50_ms"], "batched_p99": b["p99_ms"],
                        "plain_p50": p["p50_ms"], "plain_p99": p["p99_ms"]},
         "served": out_modes,
+        "mixed_stream": mixed or None,
     }
     print(json.dumps(out), flush=True)
 
@@ -547,6 +572,8 @@ def main():
                     help="served: job state drawn up to this fraction of the budget")
     ap.add_argument("--served-pieces", default="6",
                     help="served: piece counts of the job materials, e.g. 5,6")
+    ap.add_argument("--mixed-pieces", default="5,6",
+                    help="served: piece counts of the mixed capacity comparison ('' to skip)")
     ap.add_argument("--stream-plain", action="store_true",
                     help="stream workload: also time the 4-piece level in the plain form")
     ap.add_argument("--seed", type=int, default=20261203)

@@ -1624,7 +1624,7 @@ constexpr int IB_WORDS = 64;  // words per wave of one plane (R <= 62 of them us
 // a synthetic job with xrW/xrB the exit refutations: positions with a legal
 // king exit whose value is not a win for the side to move after it (such a
 // position is never lost, whatever the ply).
-__global__ __launch_bounds__(256, 5) void k_init_pl(const Mat *__restrict__ Mp, uint64_t *winsW,
+__global__ __launch_bounds__(256, 4) void k_init_pl(const Mat *__restrict__ Mp, uint64_t *winsW,
                                                  uint64_t *winsB, uint64_t *mateW, uint64_t *mateB,
                                                  uint16_t *codesW, uint16_t *codesB,
                                                  uint64_t *kexW0, uint64_t *kexW1,
@@ -1694,12 +1694,13 @@ __global__ __launch_bounds__(256, 5) void k_init_pl(const Mat *__restrict__ Mp, 
     // tried only for those t, by the full test, with the wave's candidates
     // queued in LDS (the wave's assembly buffer, not yet in use) so that
     // every lane tests one: a lane's own candidates number 0 to R.
-    uint64_t MW = 0, MB = 0;
+    // the mates found stay in the wave's buffer (qm) until the exits are
+    // done: they are not needed before, and registers are short there
+    uint64_t *qm = reinterpret_cast<uint64_t *>(reinterpret_cast<uint8_t *>(B) + 2 * QCAP) + 64;
     {
       uint8_t *wb = reinterpret_cast<uint8_t *>(B);
       uint16_t *qe = reinterpret_cast<uint16_t *>(wb);          // QCAP entries: lane << 6 | t
-      uint64_t *qp = reinterpret_cast<uint64_t *>(wb + 2 * QCAP);  // each lane's placement
-      uint64_t *qm = qp + 64;                                    // mates found, [side][lane]
+      uint64_t *qp = reinterpret_cast<uint64_t *>(wb + 2 * QCAP);  // each lane's placement; qm: mates found, [side][lane]
       static_assert(2 * QCAP + 3 * 64 * 8 <= IB_PLANES * IB_WORDS * 8, "mate queue fits");
       qp[lane] = p;
       qm[lane] = 0;
@@ -1726,9 +1727,6 @@ __global__ __launch_bounds__(256, 5) void k_init_pl(const Mat *__restrict__ Mp, 
           wave_sync();
         }
       }
-      MW = qm[lane];
-      MB = qm[64 + lane];
-      wave_sync();  // the buffer is cleared next
     }
     // synthetic exits: each of the king's moves out of the slice, its
     // index a per-placement base plus the rank of the moved position's last
@@ -1848,6 +1846,8 @@ __global__ __launch_bounds__(256, 5) void k_init_pl(const Mat *__restrict__ Mp, 
       }
       return x;
     };
+    uint64_t MW = qm[lane], MB = qm[64 + lane];
+    wave_sync();  // the buffer is cleared next
     uint64_t rw = 0, rb = 0, rmw = 0, rmb = 0;
     uint64_t rx[4] = { 0, 0, 0, 0 }, rr[2] = { 0, 0 };
     if (valid) {
