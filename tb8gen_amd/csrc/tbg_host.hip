@@ -315,7 +315,7 @@ struct tbg_ctx {
   uint64_t *cand[2], *candW[2];
   uint64_t *xr[2] = { nullptr, nullptr };  // static refutations (captures, exits)
   bool xr_ok = false;                      // worked out since the last resolve and init
-  uint64_t *sub[2][4];
+  uint64_t *sub[2][SUB_KINDS];
   uint16_t *codes[2] = { nullptr, nullptr };
   unsigned long long *dcnt = nullptr;  // init counts
   unsigned *dflags = nullptr;          // [0] jobs done, [1] uncapture ply pending
@@ -477,7 +477,7 @@ void reset_job(tbg_ctx *c)
     J.xr[s] = c->xr_ok ? c->xr[s] : nullptr;
     J.nopen = c->xr_ok ? c->dcnt + 4 : nullptr;
     J.codes[s] = c->codes[s];
-    for (int k = 0; k < 4; k++) J.sub[s][k] = c->sub[s][k];
+    for (int k = 0; k < SUB_KINDS; k++) J.sub[s][k] = c->sub[s][k];
   }
   J.hist = c->dhist;
   J.n = 1;
@@ -553,7 +553,7 @@ CtxLayout ctx_layout(const Mat &M, uint64_t nsl, int max_ply, bool with_codes)
   CtxLayout L;
   uint64_t pw = nsl * M.words;
   L.arena = part_up(pw * 8 * planes_of(M));
-  L.sub = part_up(4 * nsl * (M.ctotal[0] + M.ctotal[1]) * 8 + 64);
+  L.sub = part_up(SUB_KINDS * nsl * (M.ctotal[0] + M.ctotal[1]) * 8 + 64);
   L.codes = with_codes ? part_up(nsl * M.ii.size * 2) : 0;
   L.o_sub = L.arena;
   L.o_codes = L.o_sub + L.sub;
@@ -597,7 +597,7 @@ void carve_state(tbg_ctx *c, void *state, const CtxLayout &L, bool with_codes)
   c->subarena = reinterpret_cast<uint64_t *>(b + L.o_sub);
   p = c->subarena;
   for (int col = 0; col < 2; col++)
-    for (int k = 0; k < 4; k++) {
+    for (int k = 0; k < SUB_KINDS; k++) {
       c->sub[col][k] = p;
       p += c->nsl * c->h.ctotal[col];
     }
@@ -924,7 +924,7 @@ int launch_resolve(tbg_ctx *c)
     Timed t(c, CAT_RESOLVE);
     hipLaunchKernelGGL(k_resolve, dim3(cap_grid((waves + 3) / 4, (uint64_t)g_cus * 16)), dim3(256), 0,
                        c->st, c->dM, c->dR + k, col, k, c->sub[col][0], c->sub[col][1],
-                       c->sub[col][2], c->sub[col][3], s0, ns);
+                       c->sub[col][2], c->sub[col][3], c->sub[col][SUB_LWIN], s0, ns);
     HIPCHK(hipGetLastError());
   }
   return TBG_OK;
@@ -1215,7 +1215,8 @@ int plane_files(tbg_ctx *c, void *dev, uint64_t slice_bytes, const std::string &
   return TBG_OK;
 }
 
-const char *kSubNames[4] = { "loss", "bloss", "cwin", "win" };
+// the reference's four capture-exit files, and the legal wins (an addition)
+const char *kSubNames[SUB_KINDS] = { "loss", "bloss", "cwin", "win", "lwin" };
 
 // the planes of a checkpoint, in a fixed order (both directions use it)
 int checkpoint_planes(tbg_ctx *c, const std::string &root, int n, bool write)
@@ -1232,7 +1233,7 @@ int checkpoint_planes(tbg_ctx *c, const std::string &root, int n, bool write)
   }
   for (int col = 0; col < 2; col++)
     if (c->h.ctotal[col])
-      for (int k = 0; k < 4; k++)
+      for (int k = 0; k < SUB_KINDS; k++)
         TRY(plane_files(c, c->sub[col][k], c->h.ctotal[col] * 8,
                         root + "/sub/" + kSubNames[k], col, write, buf));
   return TBG_OK;
@@ -2783,7 +2784,7 @@ int ctx_restore_image(tbg_ctx *c, CtxImage &im)
     J.cand[s] = rebase(J.cand[s], im.state, n, state);
     J.candW[s] = rebase(J.candW[s], im.state, n, state);
     J.xr[s] = rebase(J.xr[s], im.state, n, state);
-    for (int k = 0; k < 4; k++) J.sub[s][k] = rebase(J.sub[s][k], im.state, n, state);
+    for (int k = 0; k < SUB_KINDS; k++) J.sub[s][k] = rebase(J.sub[s][k], im.state, n, state);
     J.codes[s] = c->codes[s];
   }
   HIPCHK(hipMemcpyAsync(c->dj, &c->hj, sizeof(JobDev), hipMemcpyHostToDevice, c->st));

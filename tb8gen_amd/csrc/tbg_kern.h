@@ -16,6 +16,13 @@ struct SubRef {
   int present;               // 0: a bare pair of kings (always a draw)
 };
 
+// Capture-exit planes per colour: the four of the C ABI (TBG_SUB_LOSS,
+// _BLOSS, _CWIN, _WIN; an illegal result counts as a win, a capture into it
+// being illegal) and the legal wins alone, which the ply-1 uncapture of the
+// L step starts from (the wins plane is about half illegal positions).
+constexpr int SUB_LWIN = 4;
+constexpr int SUB_KINDS = 5;
+
 // Device-resident state of one generation job (a material, or one king
 // slice of one).  The kernels read the ply, the step mask and the plane
 // pointers from here, so a whole ply can be captured in a hipGraph and
@@ -24,7 +31,7 @@ struct JobDev {
   const Mat *M;
   uint64_t *wins[2], *prevW[2], *prevL[2], *newW[2], *newL[2];
   uint64_t *cand[2], *candW[2];  // candidates of the L and of the W step
-  uint64_t *sub[2][4];
+  uint64_t *sub[2][SUB_KINDS];
   // static refutations of the L step (k_init_pl + k_xr_pl): a position with a
   // capture or synthetic exit whose result is not a win of the opponent
   // (within the 50-move rule); nullptr: not worked out, verify every move
@@ -147,7 +154,8 @@ __global__ void k_xr_pl(const Mat *Mp, const uint64_t *subwinW, const uint64_t *
                         const uint64_t *winsW, const uint64_t *winsB, uint64_t *xrW,
                         uint64_t *xrB, int s0, int ns, unsigned long long *nopen);
 __global__ void k_resolve(const Mat *Mp, const SubRef *Rp, int c, int k, uint64_t *loss,
-                          uint64_t *bloss, uint64_t *cwin, uint64_t *win, int s0, int ns);
+                          uint64_t *bloss, uint64_t *cwin, uint64_t *win, uint64_t *lwin, int s0,
+                          int ns);
 
 // bitmap of the positions whose code is `code`, one bit per position, for
 // `nslices` slices of `words` words (the reference's W_n / L_n files)

@@ -543,12 +543,12 @@ __global__ __launch_bounds__(256, 5) void k_uncap_j(JobDev *const *__restrict__ 
       if (within < cpb) break;
       within -= cpb;
     }
+    // at n == 1 the legal wins only: the legality filter of the reference's
+    // uncapture (src/tb8gen.c:964-970) was applied by k_resolve
     const uint64_t *src0, *src1 = nullptr;
-    int legality = 0;
     if (kind == 0) {
-      src0 = J.sub[o][n == 1 ? TBG_SUB_WIN : TBG_SUB_CWIN];
+      src0 = J.sub[o][n == 1 ? SUB_LWIN : TBG_SUB_CWIN];
       if (n != 1) src1 = J.sub[o][TBG_SUB_WIN];
-      legality = n == 1;
     } else {
       src0 = J.sub[o][n == 1 ? TBG_SUB_LOSS : TBG_SUB_BLOSS];
     }
@@ -591,7 +591,6 @@ __global__ __launch_bounds__(256, 5) void k_uncap_j(JobDev *const *__restrict__ 
         if (idx >= ci.size) continue;
         uint64_t p = kings_of(T, s);
         uint64_t occ = idx_to_sq(T, ci, idx, &p);
-        if (legality && attacked_by(T, M, p, occ, sq_get(p, stm), o, m)) continue;
         // uncapture by the king
         mark_king_unmoves(T, M, stm, occ, sq_set(p, m, sq_get(p, stm)), s, cand, won, ~0ull);
         // uncapture by the non-king pieces
@@ -2200,7 +2199,7 @@ __global__ __launch_bounds__(256, 4) void k_xr_pl(const Mat *__restrict__ Mp,
 __global__ __launch_bounds__(256) void k_resolve(const Mat *__restrict__ Mp,
                                                   const SubRef *__restrict__ Rp, int c, int k,
                                                   uint64_t *loss, uint64_t *bloss, uint64_t *cwin,
-                                                  uint64_t *win, int s0, int ns)
+                                                  uint64_t *win, uint64_t *lwin, int s0, int ns)
 {
   __shared__ Tabs T;
   load_tabs(T);
@@ -2239,12 +2238,14 @@ __global__ __launch_bounds__(256) void k_resolve(const Mat *__restrict__ Mp,
     }
     uint64_t b0 = __ballot(!ill && v == -2), b1 = __ballot(!ill && v == -1);
     uint64_t b2 = __ballot(ill || v >= 1), b3 = __ballot(ill || v == 2);
+    uint64_t b4 = __ballot(!ill && v == 2);
     if (lane == 0) {
       uint64_t a = phys(M, s) * M.ctotal[c] + M.coff[k] + w;
       loss[a] = b0;
       bloss[a] = b1;
       cwin[a] = b2;
       win[a] = b3;
+      lwin[a] = b4;
     }
   }
 }
