@@ -11,12 +11,13 @@ fixpoint.  One step = capture exits + init + the ply loop to the fixpoint.
 value = position-side-plies per second, the mean over the timed steps:
 every executed half-step (L or W, one side) covers the slice's positions
 for half a ply.
-roofline: the kernel with the largest device time (pred, verify, sweep or
-init): its algorithmic bytes per launch (bits per position: pred 2, verify 3,
-W sweep 4 per executed half-step; the init phase 14: ten output planes, two
-of them read and written back once more) over its average launch time from
-HIP events on the engine's stream; traffic = HBM bytes per launch from the
-committed rocprof counter summary (profiles/).
+roofline: the kernel with the largest device time (pred, verify, sweep,
+init or the static refutations' kernel xr): its algorithmic bytes per launch
+(bits per position: pred 2, verify 3, W sweep 4 per executed half-step; init
+10 once per step: ten output planes; xr 6: two planes read and written back,
+two read) over its average launch time from HIP events on the engine's
+stream; traffic = HBM bytes per launch from the committed rocprof counter
+summary (profiles/).
 secondary: the KQRvKR table (462 slices x 226,920 positions per side)
 generated end to end in the batched form.
 cpu_baseline: the CPU oracle (a plain-C restatement of the reference's
@@ -41,6 +42,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 BITS = {"pred": 2, "verify": 3, "sweep": 4}
+XR_BITS = 6
 METRIC = "retro positions/s per GPU + HBM GB/s fraction; p50/p99 slice latency vs plain"
 
 
@@ -86,22 +88,26 @@ def kernel_rooflines(tm, lines, positions, steps, traffic=None, init_bits=10):
     """Per kernel: algorithmic bytes per launch = its bits per position x
     positions x the half-steps it executed (from the count lines: pred runs
     for every W step and every L step after ply 1, verify for L steps, sweep
-    for W steps) / its launches; the init phase moves init_bits bits per
-    position once per step (illegal and mates per side, the four
-    exit-candidate planes of a synthetic job, and the static refutations:
-    written by the init kernel, then read and written back by k_xr_pl).  Average launch time from the HIP events; achieved GB/s
-    = the two divided.  traffic: measured HBM bytes per launch
-    (copy-calibrated rocprof counters) where a profile is committed."""
+    for W steps) / its launches; the init kernel writes init_bits bits per
+    position once per step (illegal and mates per side, and for a synthetic
+    job the four exit-candidate planes and the exit part of the static
+    refutations), the refutation kernel (xr) XR_BITS (it reads and writes
+    both refutation planes and reads both wins planes).  Average launch time
+    from the HIP events; achieved GB/s = the two divided.  traffic: measured
+    HBM bytes per launch (copy-calibrated rocprof counters) where a profile
+    is committed."""
     lh = [l for l in lines if re.match(r"l\d+_[wb] = ", l) and not l.startswith("l0_")]
     wh = [l for l in lines if re.match(r"w\d+_[wb] = ", l)]
     l1 = [l for l in lh if l.startswith("l1_")]
     hs = {"pred": len(lh) - len(l1) + len(wh), "verify": len(lh), "sweep": len(wh)}
     out = {}
-    for k in ("pred", "verify", "sweep", "init"):
+    for k in ("pred", "verify", "sweep", "init", "xr"):
         ms = getattr(tm, "ms_" + k) / steps
         n = getattr(tm, "n_" + k) / steps
-        if k == "init":
-            alg = init_bits * positions / 8.0
+        if k == "xr" and n == 0:
+            continue
+        if k in ("init", "xr"):
+            alg = (init_bits if k == "init" else XR_BITS) * positions / 8.0
         else:
             alg = BITS[k] * positions * hs[k] / 8.0 / max(n, 1.0)
         avg = ms / max(n, 1.0)
@@ -463,7 +469,7 @@ def run_table(args, tbg, steps, warmup, extras=True, material=None):
         "roofline": {k: v for k, v in kr[dom].items() if k not in ("ms", "half_steps")},
         "kernel_rooflines": kr,
         "kernels_ms_per_step": {k: getattr(tm, "ms_" + k) / steps for k in
-                                ("pred", "verify", "sweep", "uncap", "init", "resolve")},
+                                ("pred", "verify", "sweep", "uncap", "init", "xr", "resolve")},
     }
     if extras and not args.no_plain and form != tbg.FORM_PLAIN:
         t0 = time.perf_counter()
@@ -510,7 +516,7 @@ def run_synth(args, tbg):
     half_steps = tm.half_steps / steps
     psp = half_steps * n / 2.0
     mean = sum(secs) / len(secs)
-    kr = kernel_rooflines(tm, lines, n, steps, load_traffic(f"{material}_synth"), init_bits=14)
+    kr = kernel_rooflines(tm, lines, n, steps, load_traffic(f"{material}_synth"), init_bits=10)
     dom = max(kr, key=lambda k: kr[k]["ms"])
     roof = {k: v for k, v in kr[dom].items() if k not in ("ms", "half_steps")}
     table.close()
@@ -527,7 +533,7 @@ def run_synth(args, tbg):
         "roofline": roof,
         "kernel_rooflines": kr,
         "kernels_ms_per_step": {k: getattr(tm, "ms_" + k) / steps for k in
-                                ("pred", "verify", "sweep", "uncap", "init", "resolve")},
+                                ("pred", "verify", "sweep", "uncap", "init", "xr", "resolve")},
         "ms_per_ply": 1000 * mean / max(1, table.counts.nplies),
         "step_s": secs,
     }

@@ -356,6 +356,8 @@ typedef struct tbg_timing {
   uint64_t launches;
   uint64_t half_steps;      /* half-steps executed (L or W, one side)       */
   uint64_t n_pred, n_verify, n_sweep, n_uncap, n_init, n_resolve; /* launches */
+  double ms_xr;             /* the static refutations' kernel (after init)  */
+  uint64_t n_xr;
 } tbg_timing;
 int tbg_get_timing(const tbg_ctx *ctx, tbg_timing *t);
 int tbg_set_profiling(tbg_ctx *ctx, int on);

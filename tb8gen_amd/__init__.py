@@ -71,7 +71,8 @@ class Timing(ctypes.Structure):
                 ("half_steps", ctypes.c_uint64), ("n_pred", ctypes.c_uint64),
                 ("n_verify", ctypes.c_uint64), ("n_sweep", ctypes.c_uint64),
                 ("n_uncap", ctypes.c_uint64), ("n_init", ctypes.c_uint64),
-                ("n_resolve", ctypes.c_uint64)]
+                ("n_resolve", ctypes.c_uint64), ("ms_xr", ctypes.c_double),
+                ("n_xr", ctypes.c_uint64)]
 
 
 class Geometry(ctypes.Structure):

@@ -270,7 +270,7 @@ void material_string(const int *pt, int num, bool flip, char *out)
   out[j] = 0;
 }
 
-enum { CAT_PRED, CAT_VERIFY, CAT_SWEEP, CAT_UNCAP, CAT_INIT, CAT_RESOLVE, CAT_N };
+enum { CAT_PRED, CAT_VERIFY, CAT_SWEEP, CAT_UNCAP, CAT_INIT, CAT_RESOLVE, CAT_XR, CAT_N };
 
 }  // namespace
 
@@ -850,41 +850,42 @@ int launch_init(tbg_ctx *c)
   Mat &M = c->h;
   int s0 = M.synth ? M.synth_slice : 0, ns = M.synth ? 1 : NSL;
   {
-    Timed t(c, CAT_INIT);
     // one placement per lane (pl_layout); with the capture exits resolved,
     // the static refutations of the verification after it
     uint64_t R = 62 - (uint64_t)(M.ii.numsets - 1);
+    bool pl = pl_layout(M);
+    uint64_t chunks = pl ? (uint64_t)ns * ((M.words + R - 1) / R)
+                         : ((uint64_t)ns * M.words + 4 * INIT_CHUNK - 1) / (4 * INIT_CHUNK);
+    dim3 grid = pl ? dim3(cap_grid((chunks + 3) / 4, (uint64_t)g_cus * 16))
+                   : dim3(cap_grid(chunks, (uint64_t)g_cus * 16));
     c->xr_ok = false;
-    if (pl_layout(M)) {
-      uint64_t chunks = (uint64_t)ns * ((M.words + R - 1) / R);
-      dim3 grid(cap_grid((chunks + 3) / 4, (uint64_t)g_cus * 16));
-      hipLaunchKernelGGL(k_init_pl, grid, dim3(256), 0, c->st, c->dM, c->wins[0], c->wins[1],
-                         c->prevL[0], c->prevL[1], c->codes[0], c->codes[1], c->prevW[0],
-                         c->prevW[1], c->newW[0], c->newW[1], c->xr[0], c->xr[1], s0, ns,
-                         c->dcnt);
-      if (c->resolved) {
-        HIPCHK(hipGetLastError());
+    {
+      Timed t(c, CAT_INIT);
+      if (pl)
+        hipLaunchKernelGGL(k_init_pl, grid, dim3(256), 0, c->st, c->dM, c->wins[0], c->wins[1],
+                           c->prevL[0], c->prevL[1], c->codes[0], c->codes[1], c->prevW[0],
+                           c->prevW[1], c->newW[0], c->newW[1], c->xr[0], c->xr[1], s0, ns,
+                           c->dcnt);
+      else
+        hipLaunchKernelGGL(k_init, grid, dim3(256), 0, c->st, c->dM, c->wins[0], c->wins[1],
+                           c->prevL[0], c->prevL[1], c->codes[0], c->codes[1], c->prevW[0],
+                           c->prevW[1], c->newW[0], c->newW[1], c->xr[0], c->xr[1], s0, ns,
+                           c->dcnt);
+      HIPCHK(hipGetLastError());
+    }
+    if (c->resolved) {
+      Timed t(c, CAT_XR);
+      if (pl)
         hipLaunchKernelGGL(k_xr_pl, grid, dim3(256), 0, c->st, c->dM, c->sub[0][TBG_SUB_WIN],
                            c->sub[1][TBG_SUB_WIN], c->wins[0], c->wins[1], c->xr[0], c->xr[1],
                            s0, ns, c->dcnt + 4);
-        c->xr_ok = true;
-      }
-    } else {
-      uint64_t chunks = ((uint64_t)ns * M.words + 4 * INIT_CHUNK - 1) / (4 * INIT_CHUNK);
-      dim3 grid(cap_grid(chunks, (uint64_t)g_cus * 16));
-      hipLaunchKernelGGL(k_init, grid, dim3(256), 0, c->st, c->dM, c->wins[0], c->wins[1],
-                         c->prevL[0], c->prevL[1], c->codes[0], c->codes[1], c->prevW[0],
-                         c->prevW[1], c->newW[0], c->newW[1], c->xr[0], c->xr[1], s0, ns,
-                         c->dcnt);
-      if (c->resolved) {
-        HIPCHK(hipGetLastError());
+      else
         hipLaunchKernelGGL(k_xr, grid, dim3(256), 0, c->st, c->dM, c->sub[0][TBG_SUB_WIN],
                            c->sub[1][TBG_SUB_WIN], c->wins[0], c->wins[1], c->xr[0], c->xr[1],
                            s0, ns, c->dcnt + 4);
-        c->xr_ok = true;
-      }
+      HIPCHK(hipGetLastError());
+      c->xr_ok = true;
     }
-    HIPCHK(hipGetLastError());
   }
   return TBG_OK;
 }
@@ -2387,6 +2388,7 @@ int tbg_get_timing(const tbg_ctx *cc, tbg_timing *t)
   t->ms_uncap = c->ms[CAT_UNCAP];
   t->ms_init = c->ms[CAT_INIT];
   t->ms_resolve = c->ms[CAT_RESOLVE];
+  t->ms_xr = c->ms[CAT_XR];
   t->ms_total = 0;
   for (int i = 0; i < CAT_N; i++) t->ms_total += c->ms[i];
   t->launches = c->launches;
@@ -2397,6 +2399,7 @@ int tbg_get_timing(const tbg_ctx *cc, tbg_timing *t)
   t->n_uncap = c->nl[CAT_UNCAP];
   t->n_init = c->nl[CAT_INIT];
   t->n_resolve = c->nl[CAT_RESOLVE];
+  t->n_xr = c->nl[CAT_XR];
   return TBG_OK;
 }
 

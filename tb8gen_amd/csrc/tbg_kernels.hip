@@ -1777,19 +1777,28 @@ __global__ __launch_bounds__(256, 4) void k_init_pl(const Mat *__restrict__ Mp, 
           // the hash's high half decides each value; where it ties a
           // threshold's (about once in 2^31) the whole exit is evaluated
           // again with full hashes, off the common path
-          uint64_t hw = 0, hl = 0;
+          // Each 32-square half of the board in a loop of its own: the
+          // square masks and the lowest set bit are then 32-bit values.
+          uint32_t hwh[2] = { 0, 0 }, hlh[2] = { 0, 0 };
           bool tie = false;
-          for (uint64_t g2 = go; g2; g2 &= g2 - 1) {
-            int t = ctz64(g2);
-            int u = sym_sq(t, m, fl);
-            uint64_t idx = base + (uint64_t)(uint32_t)(u - popc64(ob & (bit(u) - 1)));
-            uint64_t z;
-            uint32_t hh = synth_hi(key, idx, &z);
-            uint64_t bt = bit(t);
-            hl |= hh < SYN_TW_HI ? bt : 0ull;                          // value 2
-            hw |= hh - SYN_TW_HI < SYN_TL_HI - SYN_TW_HI ? bt : 0ull;  // value -2
-            tie |= hh == SYN_TW_HI || hh == SYN_TL_HI;
+#pragma unroll
+          for (int h = 0; h < 2; h++) {
+            uint32_t w32 = 0, l32 = 0;
+            for (uint32_t g2 = (uint32_t)(go >> (32 * h)); g2; g2 &= g2 - 1) {
+              int t = 32 * h + __builtin_ctz(g2);
+              int u = sym_sq(t, m, fl);
+              uint64_t idx = base + (uint64_t)(uint32_t)(u - popc64(ob & (bit(u) - 1)));
+              uint64_t z;
+              uint32_t hh = synth_hi(key, idx, &z);
+              uint32_t bt = g2 & (0u - g2);
+              l32 |= hh < SYN_TW_HI ? bt : 0u;                          // value 2
+              w32 |= hh - SYN_TW_HI < SYN_TL_HI - SYN_TW_HI ? bt : 0u;  // value -2
+              tie |= hh == SYN_TW_HI || hh == SYN_TL_HI;
+            }
+            hwh[h] = w32;
+            hlh[h] = l32;
           }
+          uint64_t hw = hwh[0] | (uint64_t)hwh[1] << 32, hl = hlh[0] | (uint64_t)hlh[1] << 32;
           if (__ballot(tie)) {
             hw = hl = 0;
             for (uint64_t g2 = go; g2; g2 &= g2 - 1) {
