@@ -322,7 +322,8 @@ __device__ __forceinline__ void mark_unmoves(const Tabs &T, const Mat &M, int j,
 // ----------------------------------------------------------- frontier
 constexpr int QCAP = 1024;  // queue entries per wave per round
 constexpr int VERIFY_QUICK = 4;    // bitmap reads of verification's first pass
-constexpr int VERIFY_SPLIT = 256;  // candidates per round worth two passes
+constexpr int VERIFY_SPLIT = 256;  // candidates per round worth a quick first pass
+constexpr int LQCAP = 256;         // deferred candidates per wave (two-pass verification)
 // an L step runs in the pull form when PULL_RATIO times its frontier is at
 // least the positions it would verify (JobDev.nopen)
 constexpr unsigned long long PULL_RATIO = 2;
@@ -932,6 +933,10 @@ __global__ __launch_bounds__(256, 5) void k_verify_j(JobDev *const *__restrict__
   __shared__ Queue Q;
   __shared__ uint32_t KT[4][64];  // king_table per wave
   __shared__ Mat Ms;
+  // SPLIT: per wave, the candidates its first passes left open, deferred
+  // across the unit's chunks (word offset in the unit << 6 | bit)
+  __shared__ uint32_t LQ[4][SPLIT ? LQCAP : 1];
+  __shared__ int LQn[4];
   load_tabs(T);
   int cached = -1;
   int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -1007,6 +1012,8 @@ __global__ __launch_bounds__(256, 5) void k_verify_j(JobDev *const *__restrict__
       king_table(T, s, stm, KT[wave], lane);
       wave_sync();
     }
+    if (SPLIT && lane == 0) LQn[wave] = 0;
+    uint16_t code = (uint16_t)((n << 2) | 2);
     // one 64-word chunk per wave per pass (the 4 waves cover 256 words); a
     // chunk without candidates costs one load and one store per lane
     for (uint64_t wbase = u0 + (uint64_t)wave * 64; wbase < uend; wbase += 256) {
@@ -1026,44 +1033,97 @@ __global__ __launch_bounds__(256, 5) void k_verify_j(JobDev *const *__restrict__
         if (x && xrp) x &= ~gload(xrp, a);
         if (x && pull) x &= ~gload(prevL_s, a);
       }
+      // SPLIT: the deferred candidates are finished once half a queue has
+      // gathered, and at the wave's last chunk of the unit
+      bool last = wbase + 256 >= uend;
       uint64_t r = 0;
-      if (__ballot(x != 0)) {
+      wave_sync();
+      bool drain = SPLIT && (last || LQn[wave] >= LQCAP / 2);
+      if (__ballot(x != 0) || drain) {
         Q.res[wave][lane] = x;
         int tot;
         int off = wave_excl_scan(popc64(x), lane, &tot);
-        for (int base = 0; base < tot; base += QCAP) {
+        for (int base = 0; base < tot || base == 0; base += QCAP) {
           fill_round(Q.e[wave], x, off, base, lane);
           wave_sync();
           int cnt = tot - base < QCAP ? tot - base : QCAP;
-          // Two passes: most candidates are refuted by one of their first
-          // moves, a lost one reads every move.  The first pass spends at
-          // most VERIFY_QUICK reads per candidate and queues the open ones
-          // in place (a slot never passes the entry being read); the second
-          // finishes them with the lanes evenly loaded with long work.
-          // A round of fewer than VERIFY_SPLIT candidates takes one pass.
-          int todo = cnt, first = SPLIT && cnt >= VERIFY_SPLIT ? 0 : 1;
-          for (int pass = first; pass < 2; pass++) {
+          bool last_round = base + QCAP >= tot;
+          // Most candidates are refuted by one of their first moves, a lost
+          // one reads every move.  SPLIT: in a round of at least
+          // VERIFY_SPLIT candidates a first pass spends at most
+          // VERIFY_QUICK reads per candidate and defers the open ones to
+          // the wave's queue LQ; a smaller round defers all of its
+          // candidates there unread (in place, pass 1, when the queue is
+          // full); pass 2 finishes the deferred ones of all chunks so far
+          // with every lane busy.  Otherwise one pass.
+          int todo = cnt;
+          bool quick = cnt >= VERIFY_SPLIT;
+          for (int pass = SPLIT ? 0 : 1; pass < (SPLIT ? 3 : 2); pass++) {
+            if (pass == 2) {
+              if (!(drain && last_round)) break;
+              todo = LQn[wave] < LQCAP ? LQn[wave] : LQCAP;
+              // the earlier chunks' L words are stored: complete before the
+              // results are OR-ed into them
+              asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            }
             if (lane == 0) Q.n[wave] = 0;
             wave_sync();
             int budget = pass == 0 ? VERIFY_QUICK : 1 << 30;
             for (int e = lane; e < todo; e += 64) {
-              uint16_t v = Q.e[wave][e];
-              uint64_t idx = (wbase + (v >> 6)) * 64 + (v & 63);
-              int r = idx < M.ii.size
-                          ? verify_pos<!SPLIT>(T, M, stm, s, idx, wins_o, sub_o, KT[wave], budget,
-                                               hidx, hD, hp, xrp != nullptr)
-                          : 0;
+              uint64_t ww;  // the candidate's word
+              int bt;
+              if (pass < 2) {
+                uint16_t v = Q.e[wave][e];
+                ww = wbase + (v >> 6);
+                bt = v & 63;
+              } else {
+                uint32_t v = LQ[wave][e];
+                ww = u0 + (v >> 6);
+                bt = (int)(v & 63);
+              }
+              uint64_t idx = ww * 64 + (uint64_t)bt;
+              int r = idx >= M.ii.size ? 0
+                      : pass == 0 && !quick
+                          ? 2
+                          : verify_pos<!SPLIT>(T, M, stm, s, idx, wins_o, sub_o, KT[wave], budget,
+                                               hidx, hD, hp, xrp != nullptr);
               if (r == 1 && pull && !has_hit(T, M, stm, s, idx, prevW_o, KT[wave])) r = 0;
-              if (r == 0)
-                __hip_atomic_fetch_and(reinterpret_cast<unsigned *>(Q.res[wave]) + (v >> 5),
-                                       ~(1u << (v & 31)), __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_WORKGROUP);
-              else if (r == 2)
-                Q.e[wave][__hip_atomic_fetch_add(&Q.n[wave], 1, __ATOMIC_RELAXED,
-                                                 __HIP_MEMORY_SCOPE_WORKGROUP)] = v;
+              bool here = ww >= wbase && ww < wbase + 64;  // a word of this chunk
+              unsigned *res32 = reinterpret_cast<unsigned *>(Q.res[wave]);
+              int hb = (int)(ww - wbase) * 64 + bt;  // its bit in Q.res when here
+              if (pass < 2 && r != 1) {
+                bool keep = false;  // open: stays a candidate of pass 1
+                if (r == 2 && pass == 0) {
+                  int k = __hip_atomic_fetch_add(&LQn[wave], 1, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_WORKGROUP);
+                  if (k < LQCAP) {
+                    LQ[wave][k] = (uint32_t)((ww - u0) << 6) | (uint32_t)bt;
+                  } else {
+                    Q.e[wave][__hip_atomic_fetch_add(&Q.n[wave], 1, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_WORKGROUP)] =
+                        (uint16_t)((ww - wbase) << 6 | bt);
+                    keep = true;
+                  }
+                }
+                if (!keep)
+                  __hip_atomic_fetch_and(res32 + (hb >> 5), ~(1u << (hb & 31)), __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_WORKGROUP);
+              } else if (pass == 2 && r == 1) {
+                if (here) {  // this chunk's word is stored below, with its result
+                  __hip_atomic_fetch_or(res32 + (hb >> 5), 1u << (hb & 31), __ATOMIC_RELAXED,
+                                        __HIP_MEMORY_SCOPE_WORKGROUP);
+                } else {  // an earlier chunk's word, stored already
+                  uint64_t a2 = phys(M, s) * M.words + ww;
+                  atomicOr(reinterpret_cast<unsigned long long *>(newL + a2),
+                           (unsigned long long)bit(bt));
+                  if (codes) codes[phys(M, s) * M.ii.size + idx] = code;
+                  mysum++;
+                }
+              }
             }
             wave_sync();
-            todo = Q.n[wave];
+            if (pass == 0) todo = Q.n[wave];
+            if (pass == 2 && lane == 0) LQn[wave] = 0;
           }
         }
         r = Q.res[wave][lane];
@@ -1074,7 +1134,6 @@ __global__ __launch_bounds__(256, 5) void k_verify_j(JobDev *const *__restrict__
         if (x0) candw[a] = 0;
         if (codes && r) {
           uint16_t *c = codes + phys(M, s) * M.ii.size + w * 64;
-          uint16_t code = (uint16_t)((n << 2) | 2);
           uint64_t rr = r;
           while (rr) {
             int b = ctz64(rr);
