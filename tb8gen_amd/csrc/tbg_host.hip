@@ -548,10 +548,19 @@ struct CtxLayout {
 constexpr size_t kPart = 256;
 size_t part_up(size_t x) { return (x + kPart - 1) & ~(kPart - 1); }
 
+// Words of one bitmap plane (all slices of a side); a plane of 8 MB or more
+// starts on a 2 MB boundary of its block
+uint64_t plane_words(const Mat &M, uint64_t nsl)
+{
+  uint64_t w = nsl * M.words;
+  constexpr uint64_t kAlignWords = (2u << 20) / 8;
+  return w >= 4 * kAlignWords ? (w + kAlignWords - 1) / kAlignWords * kAlignWords : w;
+}
+
 CtxLayout ctx_layout(const Mat &M, uint64_t nsl, int max_ply, bool with_codes)
 {
   CtxLayout L;
-  uint64_t pw = nsl * M.words;
+  uint64_t pw = plane_words(M, nsl);
   L.arena = part_up(pw * 8 * planes_of(M));
   L.sub = part_up(SUB_KINDS * nsl * (M.ctotal[0] + M.ctotal[1]) * 8 + 64);
   L.codes = with_codes ? part_up(nsl * M.ii.size * 2) : 0;
@@ -627,7 +636,7 @@ int ctx_alloc(tbg_ctx *c, bool with_codes, hipStream_t ext = nullptr)
   Mat &M = c->h;
   c->nsl = M.synth ? 1 : NSL;
   M.nslices_phys = c->nsl;
-  c->pw = c->nsl * M.words;
+  c->pw = plane_words(M, c->nsl);
   if (ext) {
     c->st = ext;
     c->pooled = true;
