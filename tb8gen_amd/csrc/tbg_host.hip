@@ -49,6 +49,8 @@ bool pl_layout(const Mat &M)
 }
 
 uint64_t planes_of(const Mat &) { return kPlanes; }
+// chunk-summary bytes per slice of a plane: one per 64 bitmap words
+uint64_t sum_per_slice(const Mat &M) { return (M.words + 63) / 64; }
 Tabs g_htabs;
 // geometry and scatter given to new contexts and servers (tbg_set_geometry(NULL, ..))
 Geom g_geom = { 256, 4096, { 3, 16, 16, 16 } };
@@ -310,6 +312,7 @@ struct tbg_ctx {
   // and the record block `rec` (material, capture tables, init counts,
   // flags, job record, its pointer, per-ply history)
   uint64_t *arena = nullptr, *subarena = nullptr;
+  uint8_t *psum = nullptr;  // the planes' chunk summaries
   void *rec = nullptr;
   uint64_t *wins[2], *prevW[2], *prevL[2], *newW[2], *newL[2];  // canonical planes
   uint64_t *cand[2], *candW[2];
@@ -480,6 +483,13 @@ void reset_job(tbg_ctx *c)
     for (int k = 0; k < SUB_KINDS; k++) J.sub[s][k] = c->sub[s][k];
   }
   J.hist = c->dhist;
+  J.sum = c->psum;
+  J.arena = c->arena;
+  J.pw = c->pw;
+  J.spw = sum_per_slice(c->h);
+  J.sum_pl = c->nsl * J.spw;
+  J.sparse = 0;
+  J.sum_plies = 0;  // summaries unknown until two plies have written them
   J.n = 1;
   J.mask = 0xf;
   J.more = 0xf;
@@ -538,11 +548,12 @@ void dfree(tbg_ctx *c, void *p)
 
 // Byte layout of a context's two device blocks (every part 256-byte
 // aligned).  State block: the bitmap planes, the capture-exit planes, the
-// codes of each side; record block: material, capture tables, counts (16
-// words), flags (4), job record, its pointer, the per-ply history.
+// planes' chunk summaries, the codes of each side; record block: material,
+// capture tables, counts (16 words), flags (4), job record, its pointer,
+// the per-ply history.
 struct CtxLayout {
-  size_t arena, sub, codes;                  // state block parts (codes: per side)
-  size_t o_sub, o_codes, state;              // offsets, total
+  size_t arena, sub, sum, codes;             // state block parts (codes: per side)
+  size_t o_sub, o_sum, o_codes, state;       // offsets, total
   size_t o_m, o_r, o_cnt, o_flags, o_j, o_jp, o_hist, rec;
 };
 constexpr size_t kPart = 256;
@@ -564,8 +575,11 @@ CtxLayout ctx_layout(const Mat &M, uint64_t nsl, int max_ply, bool with_codes)
   L.arena = part_up(pw * 8 * planes_of(M));
   L.sub = part_up(SUB_KINDS * nsl * (M.ctotal[0] + M.ctotal[1]) * 8 + 64);
   L.codes = with_codes ? part_up(nsl * M.ii.size * 2) : 0;
+  // chunk summaries serve the sparse plies of one-slice (synthetic) jobs
+  L.sum = M.synth && nsl == 1 ? part_up(planes_of(M) * sum_per_slice(M)) : 0;
   L.o_sub = L.arena;
-  L.o_codes = L.o_sub + L.sub;
+  L.o_sum = L.o_sub + L.sub;
+  L.o_codes = L.o_sum + L.sum;
   L.state = L.o_codes + 2 * L.codes;
   L.o_m = 0;
   L.o_r = part_up(sizeof(Mat));
@@ -612,6 +626,7 @@ void carve_state(tbg_ctx *c, void *state, const CtxLayout &L, bool with_codes)
     }
   for (int s = 0; s < 2; s++)
     c->codes[s] = with_codes ? reinterpret_cast<uint16_t *>(b + L.o_codes + s * L.codes) : nullptr;
+  c->psum = L.sum ? reinterpret_cast<uint8_t *>(b + L.o_sum) : nullptr;
 }
 
 void carve_rec(tbg_ctx *c, void *rec, const CtxLayout &L)
@@ -801,6 +816,7 @@ int manual_step(tbg_ctx *c, int kind, int stm, int n, int more, uint64_t *count,
   JobDev &J = c->hj;
   J.n = n;
   J.done = 0;
+  J.sparse = 0;  // a host-driven step reads and writes every chunk
   J.mask = 1u << (kind * 2 + stm);
   J.more = more ? (kind == 0 ? 1u << (2 + stm) : 1u << stm) : 0u;
   for (int i = 0; i < 4; i++) J.cnt[i] = 0;
@@ -2799,6 +2815,8 @@ int ctx_restore_image(tbg_ctx *c, CtxImage &im)
     for (int k = 0; k < SUB_KINDS; k++) J.sub[s][k] = rebase(J.sub[s][k], im.state, n, state);
     J.codes[s] = c->codes[s];
   }
+  J.sum = c->psum;
+  J.arena = c->arena;
   HIPCHK(hipMemcpyAsync(c->dj, &c->hj, sizeof(JobDev), hipMemcpyHostToDevice, c->st));
   return TBG_OK;
 }

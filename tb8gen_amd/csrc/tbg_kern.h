@@ -53,7 +53,25 @@ struct JobDev {
   uint64_t blk_pos, nblk, blk_words;
   int blk_set;                // pieces of sets >= blk_set move within a pred block
   int blk_parts;              // pred units per block (> 1 only for one block per slice)
+  // Chunk summaries: one byte per 64 bitmap words of each of the arena's
+  // planes (nonzero: a set bit may be there), plane after plane in arena
+  // order, spw bytes per slice.  Every ply keeps those of the planes it
+  // writes; in a sparse ply (sparse != 0) the kernels read and write only
+  // the chunks whose summaries are set, and pred sets the summaries of the
+  // candidate chunks it marks.
+  uint8_t *sum;
+  const uint64_t *arena;      // the planes' base; pw words per plane
+  uint64_t pw, spw, sum_pl;   // spw: summary bytes per slice; sum_pl: per plane
+  int sparse;
+  int sum_plies;              // plies since the summaries were last unknown
 };
+
+// the summary bytes of plane p (one of the job's arena planes)
+TBG_HD uint8_t *plane_sum(const JobDev &J, const uint64_t *p)
+{
+  uint64_t k = (uint64_t)(p - J.arena) / J.pw;
+  return J.sum + k * J.sum_pl;
+}
 
 // One (job, side to move) pair of a launch.
 struct Pair {

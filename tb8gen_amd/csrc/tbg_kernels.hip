@@ -114,15 +114,20 @@ __device__ __forceinline__ uint64_t phys(const Mat &M, int s) { return M.synth ?
 // their words of `won` are loaded together, then each target not already
 // won gets its 64-bit atomic OR.  One memory round trip per group instead
 // of one per target.
+// sm (nullable): the plane's chunk summaries (a one-slice plane), set for
+// every chunk marked
 __device__ __forceinline__ void mark4(uint64_t *plane, const uint64_t *__restrict__ won,
-                                      const uint64_t (&g)[4])
+                                      const uint64_t (&g)[4], uint8_t *sm = nullptr)
 {
   uint64_t w[4];
 #pragma unroll
   for (int k = 0; k < 4; k++) w[k] = g[k] != ~0ull ? gload(won, g[k] >> 6) : ~0ull;
 #pragma unroll
   for (int k = 0; k < 4; k++)
-    if (!((w[k] >> (g[k] & 63)) & 1)) atomic_or64(plane + (g[k] >> 6), bit(g[k] & 63));
+    if (!((w[k] >> (g[k] & 63)) & 1)) {
+      atomic_or64(plane + (g[k] >> 6), bit(g[k] & 63));
+      if (sm) sm[g[k] >> 12] = 1;
+    }
 }
 
 // mark_king_unmoves (src/tb8gen.c:246-262)
@@ -135,7 +140,7 @@ __device__ __forceinline__ void mark4(uint64_t *plane, const uint64_t *__restric
 // indexed incrementally (king_moved_idx).
 __device__ __forceinline__ void mark_king_unmoves(const Tabs &T, const Mat &M, int stm, uint64_t occ, uint64_t p,
                                   int s, uint64_t *cand, const uint64_t *__restrict__ won,
-                                  uint64_t idx)
+                                  uint64_t idx, uint8_t *sm = nullptr)
 {
   int from = sq_get(p, stm);
   bool inc = M.all_single && idx != ~0ull;
@@ -170,7 +175,7 @@ __device__ __forceinline__ void mark_king_unmoves(const Tabs &T, const Mat &M, i
         }
       }
     }
-    mark4(cand, won, g);
+    mark4(cand, won, g, sm);
   }
 }
 
@@ -232,7 +237,9 @@ __device__ __forceinline__ void seg_or_step(uint32_t key, uint64_t &val)
 // in wins(stm) are not filtered here: the verification and the W sweep mask
 // them, and a read of `won` before each combined atomic costs more than the
 // atomics it saves.  Call with the whole wave.
-__device__ __forceinline__ void wave_mark(uint64_t *plane, uint64_t g, int lane)
+// sm (nullable): the plane's chunk summaries, set for every chunk marked
+__device__ __forceinline__ void wave_mark(uint64_t *plane, uint64_t g, int lane,
+                                          uint8_t *sm = nullptr)
 {
   uint32_t key = g == ~0ull ? ~0u : (uint32_t)(g >> 6);
   uint64_t val = g == ~0ull ? 0 : bit(g & 63);
@@ -249,7 +256,10 @@ __device__ __forceinline__ void wave_mark(uint64_t *plane, uint64_t g, int lane)
     seg_or_step<0x143, 0xc>(key, val);  // lane 31 into rows 2, 3
   }
   bool last = ((heads >> 1 | 1ull << 63) >> lane) & 1;
-  if (key != ~0u && last) atomic_or64(plane + key, val);
+  if (key != ~0u && last) {
+    atomic_or64(plane + key, val);
+    if (sm) sm[key >> 6] = 1;
+  }
 }
 
 // mark_king_unmoves for the whole wave with combining.  Every lane of a
@@ -261,7 +271,7 @@ __device__ __forceinline__ void wave_mark(uint64_t *plane, uint64_t g, int lane)
 // position.
 __device__ void king_unmoves_wave(const Tabs &T, const Mat &M, int stm, bool ok, uint64_t occ,
                                   uint64_t p, int s, uint64_t *cand, int lane,
-                                  uint64_t idx)
+                                  uint64_t idx, uint8_t *sm = nullptr)
 {
   int from = uni(sq_get(p, stm)), oks = uni(sq_get(p, stm ^ 1));
   uint64_t targets = uni64(T.king[from] & ~T.king[oks]);
@@ -298,8 +308,8 @@ __device__ void king_unmoves_wave(const Tabs &T, const Mat &M, int stm, bool ok,
       }
     }
     uint64_t *plane = cand + phys(M, s2) * M.words;
-    wave_mark(plane, g, lane);
-    if (dual) wave_mark(plane, gd, lane);
+    wave_mark(plane, g, lane, sm);
+    if (dual) wave_mark(plane, gd, lane, sm);
   }
 }
 
@@ -323,6 +333,8 @@ __device__ __forceinline__ void mark_unmoves(const Tabs &T, const Mat &M, int j,
 constexpr int QCAP = 1024;  // queue entries per wave per round
 constexpr int VERIFY_QUICK = 4;    // bitmap reads of verification's first pass
 constexpr int VERIFY_SPLIT = 256;  // candidates per round worth a quick first pass
+// a ply is sparse when its frontier times this is below the chunks of a plane
+constexpr unsigned long long SPARSE_UNMOVES = 64;
 constexpr int LQCAP = 256;         // deferred candidates per wave (two-pass verification)
 // an L step runs in the pull form when PULL_RATIO times its frontier is at
 // least the positions it would verify (JobDev.nopen)
@@ -954,6 +966,11 @@ __global__ __launch_bounds__(256, 5) void k_verify_j(JobDev *const *__restrict__
                  *sub_o = nullptr, *xrp = nullptr, *prevW_o = nullptr, *prevL_s = nullptr;
   uint64_t *candw = nullptr, *newL = nullptr;
   uint16_t *codes = nullptr;
+  // chunk summaries of the candidates and of the L plane written (one-slice
+  // jobs; null otherwise); sp: a sparse ply, only chunks summarised as
+  // holding candidates or old L bits are read and written
+  uint8_t *csm = nullptr, *lsm = nullptr;
+  bool sp = false;
   for (uint64_t u = blockIdx.x; u < total; u += gridDim.x) {
     if (u < plo || u >= phi) {  // uniform across the work-group
       int pi = find_pair(P, np, u);
@@ -982,6 +999,10 @@ __global__ __launch_bounds__(256, 5) void k_verify_j(JobDev *const *__restrict__
         prevL_s = uniptr(Jp->prevL[stm]);
         newL = uniptr(Jp->newL[stm]);
         codes = uniptr(Jp->codes[stm]);
+        bool has = uniptr(Jp->sum) != nullptr;
+        csm = has ? uniptr(plane_sum(*Jp, candw)) : nullptr;
+        lsm = has ? uniptr(plane_sum(*Jp, newL)) : nullptr;
+        sp = has && uni(Jp->sparse) != 0;
       }
     }
     if (skip) continue;
@@ -999,11 +1020,22 @@ __global__ __launch_bounds__(256, 5) void k_verify_j(JobDev *const *__restrict__
     // units come with large slices, which only the SPLIT form runs.)
     if (SPLIT && uwe >= VERIFY_BIG_UNIT && !pull) {
       uint64_t sb = phys(M, s) * M.words;
-      auto none = [](uint64_t x, uint64_t) { return x; };
-      uint64_t any = or_words(candr + sb, u0, uend, threadIdx.x, 256, none);
-      if (exits) any |= or_words(extra + sb, u0, uend, threadIdx.x, 256, none);
+      uint64_t any = 0;
+      if (sp) {  // (a sparse ply runs no exits and no pull form)
+        for (uint64_t c = (u0 >> 6) + threadIdx.x; c < ((uend + 63) >> 6); c += 256)
+          any |= (uint64_t)(csm[c] | lsm[c]);
+      } else {
+        auto none = [](uint64_t x, uint64_t) { return x; };
+        any = or_words(candr + sb, u0, uend, threadIdx.x, 256, none);
+        if (exits) any |= or_words(extra + sb, u0, uend, threadIdx.x, 256, none);
+      }
       if (!__syncthreads_or(any != 0)) {
-        for (uint64_t w = u0 + threadIdx.x; w < uend; w += 256) newL[sb + w] = 0;
+        if (!sp) {  // (sparse: the unit's L words and summaries are zero already)
+          for (uint64_t w = u0 + threadIdx.x; w < uend; w += 256) newL[sb + w] = 0;
+          if (lsm)
+            for (uint64_t c = (u0 >> 6) + threadIdx.x; c < ((uend + 63) >> 6); c += 256)
+              lsm[c] = csm[c] = 0;
+        }
         continue;
       }
     }
@@ -1018,7 +1050,9 @@ __global__ __launch_bounds__(256, 5) void k_verify_j(JobDev *const *__restrict__
     // chunk without candidates costs one load and one store per lane
     for (uint64_t wbase = u0 + (uint64_t)wave * 64; wbase < uend; wbase += 256) {
       uint64_t w = wbase + lane;
-      bool live = w < uend;
+      // sparse: a chunk without candidates or old L bits is left alone
+      bool need = !sp || (csm[wbase >> 6] | lsm[wbase >> 6]) != 0;
+      bool live = w < uend && need;
       uint64_t a = phys(M, s) * M.words + w;
       uint64_t x0 = 0, x = 0;
       if (live) {
@@ -1116,6 +1150,7 @@ __global__ __launch_bounds__(256, 5) void k_verify_j(JobDev *const *__restrict__
                   uint64_t a2 = phys(M, s) * M.words + ww;
                   atomicOr(reinterpret_cast<unsigned long long *>(newL + a2),
                            (unsigned long long)bit(bt));
+                  if (lsm) lsm[ww >> 6] = 1;
                   if (codes) codes[phys(M, s) * M.ii.size + idx] = code;
                   mysum++;
                 }
@@ -1140,6 +1175,13 @@ __global__ __launch_bounds__(256, 5) void k_verify_j(JobDev *const *__restrict__
             rr &= rr - 1;
             c[b] = code;
           }
+        }
+      }
+      if (lsm && need) {  // the chunk's summaries: L bits or none; candidates consumed
+        bool nz = __ballot(r != 0) != 0;
+        if (lane == 0) {
+          lsm[wbase >> 6] = nz;
+          csm[wbase >> 6] = 0;
         }
       }
       mysum += popc64(r);
@@ -1177,6 +1219,11 @@ __global__ __launch_bounds__(256) void k_sweep_j(JobDev *const *__restrict__ job
   uint64_t *cand = nullptr, *wins = nullptr, *newW = nullptr;
   const uint64_t *kex = nullptr;
   uint16_t *codes = nullptr;
+  // chunk summaries of the candidates and of the W plane written (one-slice
+  // jobs); sp: a sparse ply, only chunks summarised as holding candidates
+  // or old W bits are read and written
+  uint8_t *csm = nullptr, *wsm = nullptr;
+  bool sp = false;
   for (uint64_t u = blockIdx.x; u < total; u += gridDim.x) {
     if (u < plo || u >= phi) {  // uniform across the work-group
       int pi = find_pair(P, np, u);
@@ -1196,6 +1243,10 @@ __global__ __launch_bounds__(256) void k_sweep_j(JobDev *const *__restrict__ job
         newW = uniptr(Jp->newW[stm]);
         kex = uniptr(Jp->prevW[stm]);
         codes = uniptr(Jp->codes[stm]);
+        bool has = uniptr(Jp->sum) != nullptr;
+        csm = has ? uniptr(plane_sum(*Jp, cand)) : nullptr;
+        wsm = has ? uniptr(plane_sum(*Jp, newW)) : nullptr;
+        sp = has && uni(Jp->sparse) != 0;
       }
     }
     if (skip) continue;
@@ -1205,9 +1256,15 @@ __global__ __launch_bounds__(256) void k_sweep_j(JobDev *const *__restrict__ job
     uint64_t base = phys(M, s) * M.words;
     uint64_t wend = (within + 1) * uw < M.words ? (within + 1) * uw : M.words;
     uint64_t mycnt = 0;
+    // each pass of a wave covers two 64-word chunks: lanes 0-31 the first
     for (uint64_t w = within * uw + threadIdx.x * 2; w < wend; w += 512) {
       uint64_t a = base + w;
-      ulonglong2 c = *reinterpret_cast<const ulonglong2 *>(cand + a);
+      uint64_t ch = w >> 6;
+      // sparse: a chunk without candidates or old W bits is left alone
+      bool need = !sp || (csm[ch] | wsm[ch]) != 0;
+      if (sp && !__ballot(need)) continue;
+      ulonglong2 c = make_ulonglong2(0, 0);
+      if (need) c = *reinterpret_cast<const ulonglong2 *>(cand + a);
       ulonglong2 cc = c;
       if (exits) {
         ulonglong2 e = *reinterpret_cast<const ulonglong2 *>(kex + a);
@@ -1227,8 +1284,15 @@ __global__ __launch_bounds__(256) void k_sweep_j(JobDev *const *__restrict__ job
           *reinterpret_cast<ulonglong2 *>(wins + a) = o2;
         }
       }
-      *reinterpret_cast<ulonglong2 *>(newW + a) = wn;
+      if (need) *reinterpret_cast<ulonglong2 *>(newW + a) = wn;
       if (c.x | c.y) *reinterpret_cast<ulonglong2 *>(cand + a) = make_ulonglong2(0, 0);
+      if (wsm) {  // each half-wave's chunk: W bits or none; candidates consumed
+        uint64_t nzb = __ballot((wn.x | wn.y) != 0);
+        if ((lane & 31) == 0 && need) {
+          wsm[ch] = ((nzb >> (lane & 32)) & 0xffffffffull) != 0;
+          csm[ch] = 0;
+        }
+      }
       mycnt += popc64(wn.x) + popc64(wn.y);
       if (codes && (wn.x | wn.y)) {
         uint16_t code = (uint16_t)((n << 2) | 1);
@@ -1291,12 +1355,17 @@ __device__ void ctrl_one(JobDev &J, unsigned *flags, unsigned *any_edge)
   }
   // W_n of each side: the next ply's L frontier of the other
   unsigned long long wcnt[2] = { (J.mask & 4) ? J.cnt[2] : 0ull, (J.mask & 8) ? J.cnt[3] : 0ull };
+  unsigned long long front = 0;  // the next ply's frontiers, all four
+  for (int i = 0; i < 4; i++) front += ((J.mask >> i) & 1) ? J.cnt[i] : 0ull;
   for (int i = 0; i < 4; i++) J.cnt[i] = 0;
+  // this ply wrote the summaries of its planes
+  J.sum_plies = J.sum_plies < 2 ? J.sum_plies + 1 : 2;
   n++;
   J.more = more;
   if ((n > DRAW_RULE + 1 && !more) || n >= J.max_ply) {
     J.n = n;
     J.mask = 0;
+    J.sparse = 0;
     J.done = 1;
     atomicAdd(&flags[0], 1u);
     return;
@@ -1317,6 +1386,10 @@ __device__ void ctrl_one(JobDev &J, unsigned *flags, unsigned *any_edge)
         pull |= 1u << c;
   }
   J.pull = pull;
+  // a sparse ply: a frontier whose unmoves (about SPARSE_UNMOVES each) can
+  // mark few of the planes' chunks, the summaries of both ping-pong planes
+  // known, no uncapture and no pull-form step
+  J.sparse = J.sum && !forced && !pull && J.sum_plies >= 2 && front * SPARSE_UNMOVES < J.sum_pl;
   if (forced) atomicOr(any_edge, 1u);
   if (J.stop_at > 0 && n >= J.stop_at) {  // a ply boundary the host asked to stop at
     J.done = 2;
