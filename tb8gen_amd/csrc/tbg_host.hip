@@ -288,6 +288,7 @@ struct LaunchSet {
   bool pred_wide = false;  // some job scatters high-set unmoves to device memory
   uint64_t uncap_lds = 0;  // words of the largest whole-slice LDS bitmap of the uncapture
   bool verify_split = false;  // some job's slices take two-pass verification
+  bool sums = false;          // some job keeps chunk summaries (the kernels' SUM form)
   Geom g;
   void release()
   {
@@ -425,8 +426,10 @@ void plan_launchset(LaunchSet &L, const JobDev *const *hjobs, const Mat *const *
   // one block)
   L.uncap_lds = 0;
   L.verify_split = false;
+  L.sums = false;
   for (int i = 0; i < nj; i++) {
     L.verify_split |= mats[i]->ii.size > VERIFY_SPLIT_SIZE;
+    L.sums |= hjobs[i]->sum != nullptr;
     uint64_t nbw = (mats[i]->ii.size + 63) >> 6;
     if ((hjobs[i]->scatter & TBG_SCATTER_BLOCK) && hjobs[i]->nblk == 1 && nbw > L.uncap_lds)
       L.uncap_lds = nbw;
@@ -740,12 +743,10 @@ int launch(tbg_ctx *prof, JobDev *const *jobs, const LaunchSet &L, int kt, int k
   case KT_PRED: {
     Timed t(prof, CAT_PRED);
     uint64_t units = kind == 2 ? 2 * total : total;  // kind 2: both steps
-    if (L.pred_wide)
-      hipLaunchKernelGGL(k_pred_wide_j, dim3(cap_grid(units, (uint64_t)g_cus * L.g.wg[kt])),
-                         dim3(PB_THREADS), L.pred_lds, st, jobs, L.pairs, L.P[kt], L.np, sw0, kind);
-    else
-      hipLaunchKernelGGL(k_pred_j, dim3(cap_grid(units, (uint64_t)g_cus * L.g.wg[kt])),
-                         dim3(PB_THREADS), L.pred_lds, st, jobs, L.pairs, L.P[kt], L.np, sw0, kind);
+    auto k = L.pred_wide ? (L.sums ? k_pred_wide_sum_j : k_pred_wide_j)
+                         : (L.sums ? k_pred_sum_j : k_pred_j);
+    hipLaunchKernelGGL(k, dim3(cap_grid(units, (uint64_t)g_cus * L.g.wg[kt])), dim3(PB_THREADS),
+                       L.pred_lds, st, jobs, L.pairs, L.P[kt], L.np, sw0, kind);
     break;
   }
   case KT_UNCAP: {
@@ -758,17 +759,16 @@ int launch(tbg_ctx *prof, JobDev *const *jobs, const LaunchSet &L, int kt, int k
   }
   case KT_VERIFY: {
     Timed t(prof, CAT_VERIFY);
-    if (L.verify_split)
-      hipLaunchKernelGGL(k_verify_j<true>, dim3(cap_grid(total, (uint64_t)g_cus * L.g.wg[kt])),
-                         dim3(256), 0, st, jobs, L.pairs, L.P[kt], L.np, sw0, (int)L.g.verify_uw);
-    else
-      hipLaunchKernelGGL(k_verify_j<false>, dim3(cap_grid(total, (uint64_t)g_cus * L.g.wg[kt])),
-                         dim3(256), 0, st, jobs, L.pairs, L.P[kt], L.np, sw0, (int)L.g.verify_uw);
+    auto k = L.verify_split ? (L.sums ? k_verify_j<true, true> : k_verify_j<true, false>)
+                            : (L.sums ? k_verify_j<false, true> : k_verify_j<false, false>);
+    hipLaunchKernelGGL(k, dim3(cap_grid(total, (uint64_t)g_cus * L.g.wg[kt])), dim3(256), 0, st,
+                       jobs, L.pairs, L.P[kt], L.np, sw0, (int)L.g.verify_uw);
     break;
   }
   default: {
     Timed t(prof, CAT_SWEEP);
-    hipLaunchKernelGGL(k_sweep_j, dim3(cap_grid(total, (uint64_t)g_cus * L.g.wg[kt])), dim3(256), 0, st, jobs,
+    hipLaunchKernelGGL(L.sums ? k_sweep_j<true> : k_sweep_j<false>,
+                       dim3(cap_grid(total, (uint64_t)g_cus * L.g.wg[kt])), dim3(256), 0, st, jobs,
                        L.pairs, L.P[kt], L.np, sw0, (int)L.g.sweep_uw);
     break;
   }

@@ -133,15 +133,23 @@ TBG_HD uint64_t units_per_slice(const Mat &M, uint64_t nblk, int kt, int side, c
 constexpr int PB_THREADS = 512;
 constexpr int PB_QUEUE_BYTES = 8 * 1024 * 2;
 
+// _sum forms and SUM = true: the kernels that read and write chunk
+// summaries (a launch with some job that keeps them); the others leave
+// them out, registers and all
 __global__ void k_pred_j(JobDev *const *jobs, const Pair *pairs, const uint64_t *P, int np, int sw0,
                          int kind);
+__global__ void k_pred_sum_j(JobDev *const *jobs, const Pair *pairs, const uint64_t *P, int np,
+                             int sw0, int kind);
 __global__ void k_pred_wide_j(JobDev *const *jobs, const Pair *pairs, const uint64_t *P, int np,
                               int sw0, int kind);
+__global__ void k_pred_wide_sum_j(JobDev *const *jobs, const Pair *pairs, const uint64_t *P, int np,
+                                  int sw0, int kind);
 __global__ void k_uncap_j(JobDev *const *jobs, const Pair *pairs, const uint64_t *P, int np, int sw0,
                           int kind, const unsigned *flags, uint64_t lds_words);
-template <bool SPLIT>
+template <bool SPLIT, bool SUM>
 __global__ __launch_bounds__(256, 5) void k_verify_j(JobDev *const *jobs, const Pair *pairs,
                                                    const uint64_t *P, int np, int sw0, int uw);
+template <bool SUM>
 __global__ void k_sweep_j(JobDev *const *jobs, const Pair *pairs, const uint64_t *P, int np, int sw0,
                           int uw);
 // flags[0]: jobs done; flags[1]: some job is at ply 1 or 101 (uncapture)

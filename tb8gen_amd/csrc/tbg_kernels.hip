@@ -469,31 +469,63 @@ __device__ __forceinline__ uint64_t or_words(const uint64_t *p, uint64_t w0, uin
 // Two register budgets: k_pred_j (no spills, 4 waves per SIMD) when every
 // unmove but the king's lands in LDS; k_pred_wide_j (80 VGPRs, a few
 // spilled, 6 waves per SIMD) when pieces of high sets scatter to device
-// memory, whose atomics' latency more resident waves hide.
+// memory, whose atomics' latency more resident waves hide.  The _sum forms
+// add the chunk summaries of sparse plies (a launch with a job that keeps
+// them); the others leave that code out.  They are four plain kernels, not
+// instances of a template: the template's instances compiled to slower code
+// (8-piece pred 474 -> 490 ms) from the same body.
 constexpr int PB_WAVES = 8;   // waves per work-group
 constexpr int PB_CAP = QCAP;  // queue entries per wave per round
 
 __global__ __launch_bounds__(512) void k_pred_j(JobDev *const *__restrict__ jobs,
-                                                 const Pair *__restrict__ pairs,
-                                                 const uint64_t *__restrict__ P, int np, int sw0,
-                                                 int kind)
+                                                const Pair *__restrict__ pairs,
+                                                const uint64_t *__restrict__ P, int np, int sw0,
+                                                int kind)
 {
   __shared__ Tabs T;
   __shared__ Mat Ms;
   extern __shared__ uint64_t dyn[];
   constexpr bool kDecode1 = true;  // registers to spare at 4 waves/SIMD
+  constexpr bool SUM = false;
+#include "tbg_pred.inc"
+}
+
+__global__ __launch_bounds__(512) void k_pred_sum_j(JobDev *const *__restrict__ jobs,
+                                                    const Pair *__restrict__ pairs,
+                                                    const uint64_t *__restrict__ P, int np, int sw0,
+                                                    int kind)
+{
+  __shared__ Tabs T;
+  __shared__ Mat Ms;
+  extern __shared__ uint64_t dyn[];
+  constexpr bool kDecode1 = true;  // registers to spare at 4 waves/SIMD
+  constexpr bool SUM = true;
 #include "tbg_pred.inc"
 }
 
 __global__ __launch_bounds__(512, 6) void k_pred_wide_j(JobDev *const *__restrict__ jobs,
                                                         const Pair *__restrict__ pairs,
-                                                        const uint64_t *__restrict__ P, int np,
-                                                        int sw0, int kind)
+                                                        const uint64_t *__restrict__ P, int np, int sw0,
+                                                        int kind)
 {
   __shared__ Tabs T;
   __shared__ Mat Ms;
   extern __shared__ uint64_t dyn[];
   constexpr bool kDecode1 = false;  // stepped decodes instead
+  constexpr bool SUM = false;
+#include "tbg_pred.inc"
+}
+
+__global__ __launch_bounds__(512, 6) void k_pred_wide_sum_j(JobDev *const *__restrict__ jobs,
+                                                            const Pair *__restrict__ pairs,
+                                                            const uint64_t *__restrict__ P, int np, int sw0,
+                                                            int kind)
+{
+  __shared__ Tabs T;
+  __shared__ Mat Ms;
+  extern __shared__ uint64_t dyn[];
+  constexpr bool kDecode1 = false;  // stepped decodes instead
+  constexpr bool SUM = true;
 #include "tbg_pred.inc"
 }
 
@@ -935,7 +967,7 @@ __device__ bool has_hit(const Tabs &T, const Mat &M, int stm, int s, uint64_t id
 // L_{n-1}, and one that passes the verification must also reach the
 // frontier (has_hit).  The same L_n: a position is a predecessor of the
 // frontier exactly when one of its quiet moves reaches it.
-template <bool SPLIT>
+template <bool SPLIT, bool SUM>
 __global__ __launch_bounds__(256, 5) void k_verify_j(JobDev *const *__restrict__ jobs,
                                                    const Pair *__restrict__ pairs,
                                                    const uint64_t *__restrict__ P, int np, int sw0,
@@ -999,7 +1031,7 @@ __global__ __launch_bounds__(256, 5) void k_verify_j(JobDev *const *__restrict__
         prevL_s = uniptr(Jp->prevL[stm]);
         newL = uniptr(Jp->newL[stm]);
         codes = uniptr(Jp->codes[stm]);
-        bool has = uniptr(Jp->sum) != nullptr;
+        bool has = SUM && uniptr(Jp->sum) != nullptr;
         csm = has ? uniptr(plane_sum(*Jp, candw)) : nullptr;
         lsm = has ? uniptr(plane_sum(*Jp, newL)) : nullptr;
         sp = has && uni(Jp->sparse) != 0;
@@ -1191,16 +1223,21 @@ __global__ __launch_bounds__(256, 5) void k_verify_j(JobDev *const *__restrict__
   }
 }
 
-template __global__ void k_verify_j<false>(JobDev *const *, const Pair *, const uint64_t *, int, int,
-                                           int);
-template __global__ void k_verify_j<true>(JobDev *const *, const Pair *, const uint64_t *, int, int,
-                                          int);
+template __global__ void k_verify_j<false, false>(JobDev *const *, const Pair *, const uint64_t *,
+                                                  int, int, int);
+template __global__ void k_verify_j<true, false>(JobDev *const *, const Pair *, const uint64_t *,
+                                                 int, int, int);
+template __global__ void k_verify_j<false, true>(JobDev *const *, const Pair *, const uint64_t *,
+                                                 int, int, int);
+template __global__ void k_verify_j<true, true>(JobDev *const *, const Pair *, const uint64_t *,
+                                                int, int, int);
 
 // calc_W_n tail (src/tb8gen.c:1072-1077) fused, over every (job, side):
 // W = C & ~wins; wins |= W; count; codes; C cleared.  uw words per unit
 // (a multiple of 512), two per lane per pass with 16-byte accesses.  A step that does not run
 // writes an empty newW.  A synthetic job at n == 1 adds the king-move exit
 // candidates computed by k_init into prevW[stm].
+template <bool SUM>
 __global__ __launch_bounds__(256) void k_sweep_j(JobDev *const *__restrict__ jobs,
                                                   const Pair *__restrict__ pairs,
                                                   const uint64_t *__restrict__ P, int np, int sw0,
@@ -1243,7 +1280,7 @@ __global__ __launch_bounds__(256) void k_sweep_j(JobDev *const *__restrict__ job
         newW = uniptr(Jp->newW[stm]);
         kex = uniptr(Jp->prevW[stm]);
         codes = uniptr(Jp->codes[stm]);
-        bool has = uniptr(Jp->sum) != nullptr;
+        bool has = SUM && uniptr(Jp->sum) != nullptr;
         csm = has ? uniptr(plane_sum(*Jp, cand)) : nullptr;
         wsm = has ? uniptr(plane_sum(*Jp, newW)) : nullptr;
         sp = has && uni(Jp->sparse) != 0;
@@ -1311,6 +1348,11 @@ __global__ __launch_bounds__(256) void k_sweep_j(JobDev *const *__restrict__ job
     if (lane == 0 && mycnt) atomicAdd(&Jp->cnt[2 + stm], (unsigned long long)mycnt);
   }
 }
+
+template __global__ void k_sweep_j<false>(JobDev *const *, const Pair *, const uint64_t *, int, int,
+                                          int);
+template __global__ void k_sweep_j<true>(JobDev *const *, const Pair *, const uint64_t *, int, int,
+                                         int);
 
 // End of a ply for every job (one lane per job): record the counts, derive
 // the next ply's flags exactly as main() does (src/tb8gen.c:1416-1423: an
