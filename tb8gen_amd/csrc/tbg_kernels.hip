@@ -1842,8 +1842,9 @@ __global__ __launch_bounds__(256, 4) void k_init_pl(const Mat *__restrict__ Mp, 
     if (T.king[wk] & bit(bk)) WI = BI = ~0ull;
     // king moves of each side that stay legal (own piece not on the
     // target, target not attacked with the king lifted), as masks over t,
-    // OR-ed into the escapes for the mate test
-    uint64_t esc[2];
+    // OR-ed into the escapes for the mate test (named registers, not an
+    // array indexed by the side, which would live in scratch memory)
+    uint64_t esc0 = 0, esc1 = 0;
     for (int c = 0; c < 2; c++) {
       int o = c ^ 1, k = sq_get(p, c);
       uint64_t own = 0;
@@ -1861,7 +1862,8 @@ __global__ __launch_bounds__(256, 4) void k_init_pl(const Mat *__restrict__ Mp, 
         if (cx == c) ok &= ~bit(y);
         e |= ok;
       }
-      esc[c] = e;
+      if (c == WHITE) esc0 = e;
+      else esc1 = e;
     }
     // mates: in check, not illegal, no king move.  The other moves are
     // tried only for those t, by the full test, with the wave's candidates
@@ -1879,7 +1881,7 @@ __global__ __launch_bounds__(256, 4) void k_init_pl(const Mat *__restrict__ Mp, 
       qm[lane] = 0;
       qm[64 + lane] = 0;
       for (int c = 0; c < 2; c++) {
-        uint64_t cand = valid ? (c == WHITE ? BI & ~WI : WI & ~BI) & ~esc[c] & F : 0;
+        uint64_t cand = valid ? (c == WHITE ? BI & ~WI & ~esc0 : WI & ~BI & ~esc1) & F : 0;
         int tot;
         int off = wave_excl_scan(popc64(cand), lane, &tot);
         for (int base = 0; base < tot; base += QCAP) {
@@ -1904,13 +1906,14 @@ __global__ __launch_bounds__(256, 4) void k_init_pl(const Mat *__restrict__ Mp, 
     // synthetic exits: each of the king's moves out of the slice, its
     // index a per-placement base plus the rank of the moved position's last
     // square; the candidates collect over squares t
-    uint64_t xs[4] = { 0, 0, 0, 0 };  // W candidates W, B; L candidates W, B
-    uint64_t xr0 = 0, xr1 = 0;        // exit refutations W, B
+    // per side: W candidates, L candidates, exit refutations (named
+    // registers: an array indexed by the side would live in scratch memory)
+    uint64_t xsW0 = 0, xsW1 = 0, xsL0 = 0, xsL1 = 0, xr0 = 0, xr1 = 0;
     if (M.synth && valid) {
       for (int c = 0; c < 2; c++) {
         int o = c ^ 1, k = sq_get(p, c);
         uint64_t ill = c == WHITE ? WI : BI;
-        uint64_t own = 0, xrc = 0;
+        uint64_t own = 0, xrc = 0, xsw = 0, xsl = 0;
         for (int i = 0; i < MAXP - 1; i++) {
           if (i >= M.nk[c]) break;
           int j = M.pcs[c][i];
@@ -1986,17 +1989,19 @@ __global__ __launch_bounds__(256, 4) void k_init_pl(const Mat *__restrict__ Mp, 
           }
           xw |= hw;
           xl |= hl;
-          if (c == WHITE) {
-            xs[0] |= xw;
-            xs[2] |= xl;
-          } else {
-            xs[1] |= xw;
-            xs[3] |= xl;
-          }
+          xsw |= xw;
+          xsl |= xl;
           xrc |= legal & ~xl;  // a legal exit not into a win of the opponent
         }
-        if (c == WHITE) xr0 = xrc;
-        else xr1 = xrc;
+        if (c == WHITE) {
+          xsW0 = xsw;
+          xsL0 = xsl;
+          xr0 = xrc;
+        } else {
+          xsW1 = xsw;
+          xsL1 = xsl;
+          xr1 = xrc;
+        }
       }
     }
     // rank-domain masks (bit r = position P R + r): the squares' bits
@@ -2038,7 +2043,10 @@ __global__ __launch_bounds__(256, 4) void k_init_pl(const Mat *__restrict__ Mp, 
       rmw = compress(MW);
       rmb = compress(MB);
       if (M.synth) {
-        for (int k = 0; k < 4; k++) rx[k] = compress(xs[k]);
+        rx[0] = compress(xsW0);
+        rx[1] = compress(xsW1);
+        rx[2] = compress(xsL0);
+        rx[3] = compress(xsL1);
         rr[0] = compress(xr0);
         rr[1] = compress(xr1);
       }
