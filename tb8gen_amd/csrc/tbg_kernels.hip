@@ -475,7 +475,23 @@ __device__ __forceinline__ uint64_t or_words(const uint64_t *p, uint64_t w0, uin
 // instances of a template: the template's instances compiled to slower code
 // (8-piece pred 474 -> 490 ms) from the same body.
 constexpr int PB_WAVES = 8;   // waves per work-group
-constexpr int PB_CAP = QCAP;  // queue entries per wave per round
+constexpr int PB_CAP = PB_QUEUE_BYTES / 4 / PB_WAVES;  // 32-bit queue entries per wave
+constexpr int PB_ROOM = PB_CAP - 64;  // a chunk's entries per fill (64 may be waiting)
+
+// Writes the wave's set bits of rank [base, base + PB_ROOM) (rank = position
+// in the wave's bit order: lane, then bit) to q[rank - base], each as the
+// word's offset in the unit << 6 | the bit.
+__device__ __forceinline__ void fill_pred(uint32_t *q, uint64_t x, int off, int base,
+                                          uint32_t word)
+{
+  int k = off;
+  while (x && k < base + PB_ROOM) {
+    int bt = ctz64(x);
+    x &= x - 1;
+    if (k >= base) q[k - base] = word << 6 | (uint32_t)bt;
+    k++;
+  }
+}
 
 __global__ __launch_bounds__(512) void k_pred_j(JobDev *const *__restrict__ jobs,
                                                 const Pair *__restrict__ pairs,
