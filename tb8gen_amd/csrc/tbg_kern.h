@@ -147,7 +147,7 @@ __global__ void k_pred_wide_sum_j(JobDev *const *jobs, const Pair *pairs, const 
 __global__ void k_uncap_j(JobDev *const *jobs, const Pair *pairs, const uint64_t *P, int np, int sw0,
                           int kind, const unsigned *flags, uint64_t lds_words);
 template <bool SPLIT, bool SUM>
-__global__ __launch_bounds__(256, 5) void k_verify_j(JobDev *const *jobs, const Pair *pairs,
+__global__ __launch_bounds__(256, SPLIT ? 4 : 5) void k_verify_j(JobDev *const *jobs, const Pair *pairs,
                                                    const uint64_t *P, int np, int sw0, int uw);
 template <bool SUM>
 __global__ void k_sweep_j(JobDev *const *jobs, const Pair *pairs, const uint64_t *P, int np, int sw0,

@@ -972,9 +972,12 @@ __device__ bool has_hit(const Tabs &T, const Mat &M, int stm, int s, uint64_t id
 // a decode.  A step that does not run this ply writes an empty newL.  For a
 // synthetic job at n == 1 the king-move exit candidates (computed by k_init
 // into newW[stm]) join the candidates.
-// Verification waits on chains of random bitmap reads: 5 waves per SIMD
-// (at most 96 VGPRs, a few spilled) beat 4 waves without spills and 6 with
-// more spills.
+// Verification waits on chains of random bitmap reads.  The one-pass form
+// runs at 5 waves per SIMD (at most 96 VGPRs, a few spilled), which beats 4
+// waves without spills and 6 with more spills on small slices.  The
+// two-pass form runs at 4 without spills: its deferred queue keeps every
+// lane at work on the dense pull-form plies of large slices, where the
+// spill traffic of 5 waves went to HBM (8-piece verify 490 -> 382 ms).
 // SPLIT: the two-pass form for slices larger than VERIFY_SPLIT_SIZE (one
 // pass otherwise, compiled without the second pass's bookkeeping).
 // Pull form (the job's pull bit of stm, set by k_ctrl for a dense frontier
@@ -984,7 +987,7 @@ __device__ bool has_hit(const Tabs &T, const Mat &M, int stm, int s, uint64_t id
 // frontier (has_hit).  The same L_n: a position is a predecessor of the
 // frontier exactly when one of its quiet moves reaches it.
 template <bool SPLIT, bool SUM>
-__global__ __launch_bounds__(256, 5) void k_verify_j(JobDev *const *__restrict__ jobs,
+__global__ __launch_bounds__(256, SPLIT ? 4 : 5) void k_verify_j(JobDev *const *__restrict__ jobs,
                                                    const Pair *__restrict__ pairs,
                                                    const uint64_t *__restrict__ P, int np, int sw0,
                                                    int uw)
