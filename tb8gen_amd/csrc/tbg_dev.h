@@ -144,6 +144,48 @@ TBG_HD bool flip_test(int wk, int bk)
   int s1 = wk ^ m, s2 = bk ^ m;
   return !((A1D1D4 >> s1) & 1) || (((A1D4 >> s1) & 1) && !((LOWER >> s2) & 1));
 }
+// The square t of a position normalised by (m, fl): t ^ m, then for fl the
+// a1-h8 flip (file and rank exchanged)
+TBG_HD int sym_sq(int t, int m, bool fl)
+{
+  int u = t ^ m;
+  return fl ? ((u & 7) << 3) | (u >> 3) : u;
+}
+// A square mask under t -> t ^ m for m in {0, 7, 56, 63} (mirror_mask): a
+// file mirror reverses the bits of each byte, a rank mirror the byte order
+TBG_HD uint64_t mirror_bb(uint64_t x, int m)
+{
+  uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
+  if (m & 7) {
+    lo = __builtin_bswap32(__builtin_bitreverse32(lo));
+    hi = __builtin_bswap32(__builtin_bitreverse32(hi));
+  }
+  if (m & 56) {
+    uint32_t t = __builtin_bswap32(lo);
+    lo = __builtin_bswap32(hi);
+    hi = t;
+  }
+  return lo | (uint64_t)hi << 32;
+}
+// A square mask flipped about the a1-h8 diagonal (three delta swaps: 4x4
+// blocks, 2x2 blocks, single squares)
+TBG_HD uint64_t transpose_bb(uint64_t x)
+{
+  uint64_t t = 0x0f0f0f0f00000000ull & (x ^ (x << 28));
+  x ^= t ^ (t >> 28);
+  t = 0x3333000033330000ull & (x ^ (x << 14));
+  x ^= t ^ (t >> 14);
+  t = 0x5500550055005500ull & (x ^ (x << 7));
+  x ^= t ^ (t >> 7);
+  return x;
+}
+// the masks of sym_sq's images of a mask's squares, and back
+TBG_HD uint64_t sym_bb(uint64_t x, int m, bool fl)
+{
+  x = mirror_bb(x, m);
+  return fl ? transpose_bb(x) : x;
+}
+TBG_HD uint64_t sym_bb_inv(uint64_t x, int m, bool fl) { return mirror_bb(fl ? transpose_bb(x) : x, m); }
 // FlipDiag of every byte
 TBG_HD uint64_t flip_diag_bytes(uint64_t p)
 {

@@ -2550,6 +2550,21 @@ extern "C" int tbg_host_selftest(char *msg, int len)
       if (synth_value_k(key, idx) != want) ST_FAIL("synth_value_k");
     }
   }
+  // square masks under the slice symmetries, against sym_sq per square
+  for (int m : { 0, 7, 56, 63 })
+    for (int fl = 0; fl < 2; fl++) {
+      for (int t = 0; t < 64; t++) {
+        int u = sym_sq(t, m, fl != 0);
+        if (sym_bb(1ull << t, m, fl != 0) != 1ull << u) ST_FAIL("sym_bb m %d fl %d t %d", m, fl, t);
+        if (sym_bb_inv(1ull << u, m, fl != 0) != 1ull << t) ST_FAIL("sym_bb_inv m %d fl %d", m, fl);
+      }
+      for (int it = 0; it < 1000; it++) {
+        uint64_t x = rnd(), y = 0;
+        for (int t = 0; t < 64; t++)
+          if ((x >> t) & 1) y |= 1ull << sym_sq(t, m, fl != 0);
+        if (sym_bb(x, m, fl != 0) != y || sym_bb_inv(y, m, fl != 0) != x) ST_FAIL("sym_bb mask");
+      }
+    }
   for (int it = 0; it < 200000; it++) {
     uint64_t d = 1 + rnd() % 0xffffffffull, n = rnd() % (1ull << 40), rem;
     uint64_t q = div_fix(n, d, 1.0 / (double)d, &rem);

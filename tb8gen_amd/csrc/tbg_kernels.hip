@@ -1797,11 +1797,6 @@ __device__ __forceinline__ uint64_t partial_idx(const Idx &ii, uint64_t q, int L
 
 // the square t moved by a king exit's symmetry (mirror mask m, then the
 // a1-h8 flip)
-__device__ __forceinline__ int sym_sq(int t, int m, bool fl)
-{
-  int u = t ^ m;
-  return fl ? ((u & 7) << 3) | (u >> 3) : u;
-}
 
 // wins W/B, mates W/B, exit W candidates W/B, exit L candidates W/B, exit
 // refutations W/B
@@ -1972,29 +1967,38 @@ __global__ __launch_bounds__(256, 4) void k_init_pl(const Mat *__restrict__ Mp, 
           }
           // the hash's high half decides each value; where it ties a
           // threshold's (about once in 2^31) the whole exit is evaluated
-          // again with full hashes, off the common path
-          // Each 32-square half of the board in a loop of its own: the
-          // square masks and the lowest set bit are then 32-bit values.
+          // again with full hashes, off the common path.
+          // The loop runs over the squares u of the moved position's frame
+          // (the images of go's squares), whose rank among the free squares
+          // gives the index, in 32-square halves (32-bit masks, the lowest
+          // bit extracted in one step); the results are mapped back to t
+          // once per exit.
+          uint64_t gu = sym_bb(go, m, fl);
+          uint32_t ob0 = (uint32_t)ob, ob1 = (uint32_t)(ob >> 32);
           uint32_t hwh[2] = { 0, 0 }, hlh[2] = { 0, 0 };
-          bool tie = false;
+          uint32_t tmin = ~0u;  // 0 once a high half ties a threshold's
 #pragma unroll
           for (int h = 0; h < 2; h++) {
             uint32_t w32 = 0, l32 = 0;
-            for (uint32_t g2 = (uint32_t)(go >> (32 * h)); g2; g2 &= g2 - 1) {
-              int t = 32 * h + __builtin_ctz(g2);
-              int u = sym_sq(t, m, fl);
-              uint64_t idx = base + (uint64_t)(uint32_t)(u - popc64(ob & (bit(u) - 1)));
-              uint64_t z;
-              uint32_t hh = synth_hi(key, idx, &z);
+            uint32_t obh = h ? ob1 : ob0;
+            // rank of the half's first square: 32 less the pieces below it
+            uint32_t rb = h ? 32u - (uint32_t)__builtin_popcount(ob0) : 0u;
+            for (uint32_t g2 = (uint32_t)(gu >> (32 * h)); g2; g2 &= g2 - 1) {
               uint32_t bt = g2 & (0u - g2);
-              l32 |= hh < SYN_TW_HI ? bt : 0u;                          // value 2
-              w32 |= hh - SYN_TW_HI < SYN_TL_HI - SYN_TW_HI ? bt : 0u;  // value -2
-              tie |= hh == SYN_TW_HI || hh == SYN_TL_HI;
+              uint32_t r = rb + (uint32_t)__builtin_ctz(g2) - (uint32_t)__builtin_popcount(obh & (bt - 1));
+              uint64_t z;
+              uint32_t hh = synth_hi(key, base + r, &z);
+              l32 |= hh < SYN_TW_HI ? bt : 0u;  // value 2
+              uint32_t dw = hh - SYN_TW_HI;
+              w32 |= dw < SYN_TL_HI - SYN_TW_HI ? bt : 0u;  // value -2
+              tmin = min(tmin, min(dw, hh ^ SYN_TL_HI));
             }
             hwh[h] = w32;
             hlh[h] = l32;
           }
-          uint64_t hw = hwh[0] | (uint64_t)hwh[1] << 32, hl = hlh[0] | (uint64_t)hlh[1] << 32;
+          bool tie = tmin == 0;
+          uint64_t hw = sym_bb_inv(hwh[0] | (uint64_t)hwh[1] << 32, m, fl);
+          uint64_t hl = sym_bb_inv(hlh[0] | (uint64_t)hlh[1] << 32, m, fl);
           if (__ballot(tie)) {
             hw = hl = 0;
             for (uint64_t g2 = go; g2; g2 &= g2 - 1) {
