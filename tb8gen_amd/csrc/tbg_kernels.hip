@@ -753,6 +753,16 @@ __device__ __forceinline__ uint64_t king_norm(uint32_t e, uint64_t q0)
   return (e >> 15) & 1 ? flip_diag_bytes(q) : q;
 }
 
+// squares strictly between a and b on a shared rank, file or diagonal (0
+// when a and b share none)
+__device__ __forceinline__ uint64_t between_sq(int a, int b)
+{
+  int dr = (b >> 3) - (a >> 3), df = (b & 7) - (a & 7);
+  if (dr == 0 || df == 0) return rook_att(a, bit(b)) & rook_att(b, bit(a));
+  if (dr == df || dr == -df) return bishop_att(a, bit(b)) & bishop_att(b, bit(a));
+  return 0;
+}
+
 // check_successors_worker + check_moves + check_king_moves
 // (src/tb8gen.c:529-650): is candidate idx of slice s a loss for stm?
 // Candidates come from cand & ~wins(stm), and wins(stm) holds every position
@@ -789,10 +799,25 @@ __device__ __forceinline__ int verify_pos(const Tabs &T, const Mat &M, int stm, 
   hidx = idx;
   hp = p;
   int ks = sq_get(p, stm), oks = sq_get(p, o);
-  uint64_t oppocc = 0;
+  uint64_t oppocc = 0, orq = 0, obq = 0, on = 0;  // o's pieces; rooks and queens, ...
   for (int t = 0; t < MAXP - 1; t++) {
     if (t >= M.nk[o]) break;
-    oppocc |= bit(sq_get(p, M.pcs[o][t]));
+    int jo = M.pcs[o][t], pt = M.pt[jo] & 7;
+    uint64_t b = bit(sq_get(p, jo));
+    oppocc |= b;
+    orq |= pt == ROOK || pt == QUEEN ? b : 0;
+    obq |= pt == BISHOP || pt == QUEEN ? b : 0;
+    on |= pt == KNIGHT ? b : 0;
+  }
+  // stm in check: a quiet move of a piece other than the king is legal only
+  // when it blocks a single slider's check; every other one leads to a
+  // position illegal for o to move, which wins(o) holds from init on, so its
+  // bit is known to be set and is not read.  (Large slices only: on small
+  // ones the test's registers cost the one-pass form more than it saves.)
+  uint64_t quiet_ok = ~0ull;
+  if (!SMALL) {
+    uint64_t chk = (rook_att(ks, occ) & orq) | (bishop_att(ks, occ) & obq) | (T.knight[ks] & on);
+    quiet_ok = !chk ? ~0ull : chk & (chk - 1) ? 0 : between_sq(ctz64(chk), ks);
   }
   const uint64_t *wins_s = wins_o + phys(M, s) * M.words;
   const uint64_t *sub_s = sub_o + phys(M, s) * M.ctotal[o];
@@ -813,7 +838,7 @@ __device__ __forceinline__ int verify_pos(const Tabs &T, const Mat &M, int stm, 
       uint64_t ci = M.all_single ? capt_idx_1(M.capt[l], q, l) : sq_to_idx(T, M.capt[l], q);
       if (!bm_test(sub_s + M.coff[l], ci)) return 0;
     }
-    uint64_t qm = a & ~occ;
+    uint64_t qm = a & ~occ & quiet_ok;
     // quiet moves four at a time: the four bitmap reads are in flight
     // together instead of one round trip after another
     uint64_t ob = 0;
@@ -1723,15 +1748,6 @@ __global__ __launch_bounds__(256, 6) void k_init(const Mat *__restrict__ Mp, uin
 // placement, and each synthetic exit's index is a per-placement base plus
 // the rank of the moved position's last square.
 
-// squares strictly between a and b on a shared rank, file or diagonal (0
-// when a and b share none)
-__device__ __forceinline__ uint64_t between_sq(int a, int b)
-{
-  int dr = (b >> 3) - (a >> 3), df = (b & 7) - (a & 7);
-  if (dr == 0 || df == 0) return rook_att(a, bit(b)) & rook_att(b, bit(a));
-  if (dr == df || dr == -df) return bishop_att(a, bit(b)) & bishop_att(b, bit(a));
-  return 0;
-}
 
 // The squares t of the last set's piece (slot jx) at which square y is
 // attacked by colour c's other non-king pieces over occupancy occ (which
