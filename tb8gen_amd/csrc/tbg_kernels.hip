@@ -1008,8 +1008,8 @@ __device__ bool has_hit(const Tabs &T, const Mat &M, int stm, int s, uint64_t id
 // Pull form (the job's pull bit of stm, set by k_ctrl for a dense frontier
 // when the static refutations are in place): no predecessor pass ran; the
 // candidates are every position not won, not statically refuted and not in
-// L_{n-1}, and one that passes the verification must also reach the
-// frontier (has_hit).  The same L_n: a position is a predecessor of the
+// L_{n-1} or L_{n-2}, and one that passes the verification must also reach
+// the frontier (has_hit).  The same L_n: a position is a predecessor of the
 // frontier exactly when one of its quiet moves reaches it.
 template <bool SPLIT, bool SUM>
 __global__ __launch_bounds__(256, SPLIT ? 4 : 5) void k_verify_j(JobDev *const *__restrict__ jobs,
@@ -1141,7 +1141,11 @@ __global__ __launch_bounds__(256, SPLIT ? 4 : 5) void k_verify_j(JobDev *const *
         if (exits) c |= gload(extra, a);
         x = run && c ? c & ~gload(wins_s, a) : 0;
         if (x && xrp) x &= ~gload(xrp, a);
-        if (x && pull) x &= ~gload(prevL_s, a);
+        // the pull form also drops L_{n-1} and, still in the plane this
+        // step overwrites, L_{n-2} (at n = 2 the mates): an earlier loss
+        // has no move into W_{n-1}(opp), so it never reaches the frontier,
+        // and every move of a mate leads to wins(opp)
+        if (x && pull) x &= ~gload(prevL_s, a) & ~gload(newL, a);
       }
       // SPLIT: the deferred candidates are finished once half a queue has
       // gathered, and at the wave's last chunk of the unit
