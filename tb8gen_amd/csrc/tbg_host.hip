@@ -52,6 +52,21 @@ uint64_t planes_of(const Mat &) { return kPlanes; }
 // chunk-summary bytes per slice of a plane: one per 64 bitmap words
 uint64_t sum_per_slice(const Mat &M) { return (M.words + 63) / 64; }
 Tabs g_htabs;
+
+// a synthetic job on the placement layout whose king moves all leave the
+// slice: k_xr_pl also works out the ply-1 W step's capture candidates (the
+// king's captures are exits, which k_init_pl covers), and that step runs no
+// uncapture.  (A king move that the symmetries map back into the slice is
+// a move within the table, and its captures are the uncapture's.)
+bool w1_in_xr(const Mat &M)
+{
+  if (!M.synth || !pl_layout(M)) return false;
+  const Tabs &T = g_htabs;
+  int kq[2] = { T.kksq[M.synth_slice][0], T.kksq[M.synth_slice][1] };
+  for (int stm = 0; stm < 2; stm++)
+    if (popc64(T.king[kq[stm]] & ~T.king[kq[stm ^ 1]]) != M.nexit[stm]) return false;
+  return true;
+}
 // geometry and scatter given to new contexts and servers (tbg_set_geometry(NULL, ..))
 Geom g_geom = { 256, 4096, { 3, 16, 16, 16 } };
 int g_scatter = TBG_SCATTER_BLOCK | TBG_SCATTER_COMBINE;
@@ -493,6 +508,7 @@ void reset_job(tbg_ctx *c)
   J.sum_pl = c->nsl * J.spw;
   J.sparse = 0;
   J.sum_plies = 0;  // summaries unknown until two plies have written them
+  J.w1cap = c->xr_ok && w1_in_xr(c->h);
   J.n = 1;
   J.mask = 0xf;
   J.more = 0xf;
@@ -900,10 +916,12 @@ int launch_init(tbg_ctx *c)
     }
     if (c->resolved) {
       Timed t(c, CAT_XR);
+      bool w1 = w1_in_xr(M);
       if (pl)
         hipLaunchKernelGGL(k_xr_pl, grid, dim3(256), 0, c->st, c->dM, c->sub[0][TBG_SUB_WIN],
-                           c->sub[1][TBG_SUB_WIN], c->wins[0], c->wins[1], c->xr[0], c->xr[1],
-                           s0, ns, c->dcnt + 4);
+                           c->sub[1][TBG_SUB_WIN], w1 ? c->sub[0][TBG_SUB_LOSS] : nullptr,
+                           w1 ? c->sub[1][TBG_SUB_LOSS] : nullptr, c->wins[0], c->wins[1],
+                           c->xr[0], c->xr[1], c->candW[0], c->candW[1], s0, ns, c->dcnt + 4);
       else
         hipLaunchKernelGGL(k_xr, grid, dim3(256), 0, c->st, c->dM, c->sub[0][TBG_SUB_WIN],
                            c->sub[1][TBG_SUB_WIN], c->wins[0], c->wins[1], c->xr[0], c->xr[1],
@@ -1457,6 +1475,10 @@ int tbg_resolve_captures(tbg_ctx *c, tbg_ctx *const *subs, int nsubs)
     c->xr_ok = false;
     c->hj.xr[0] = c->hj.xr[1] = nullptr;
     c->hj.nopen = nullptr;
+    if (c->hj.w1cap) {  // the ply-1 W capture candidates too: the uncapture makes them again
+      c->hj.w1cap = 0;
+      for (int s = 0; s < 2; s++) HIPCHK(hipMemsetAsync(c->candW[s], 0, c->pw * 8, c->st));
+    }
     if (c->inited) TRY(upload_job(c));
   }
   return TBG_OK;

@@ -64,6 +64,9 @@ struct JobDev {
   uint64_t pw, spw, sum_pl;   // spw: summary bytes per slice; sum_pl: per plane
   int sparse;
   int sum_plies;              // plies since the summaries were last unknown
+  // the ply-1 W step's capture candidates are in candW already (k_xr_pl of a
+  // synthetic job worked them out), so that step runs no uncapture
+  int w1cap;
 };
 
 // the summary bytes of plane p (one of the job's arena planes)
@@ -175,10 +178,14 @@ __global__ void k_init_pl(const Mat *Mp, uint64_t *winsW, uint64_t *winsB, uint6
                           uint64_t *xrB, int s0, int ns, unsigned long long *cnt);
 // the capture part of the static refutations, OR-ed into xrW / xrB (which
 // k_init_pl cleared or filled with the exit part), from the capture-exit
-// planes sub[colour][TBG_SUB_WIN]; same launch shape as k_init_pl
+// planes sub[colour][TBG_SUB_WIN]; same launch shape as k_init_pl.  With
+// sublossW/B (else null) also the ply-1 W step's capture candidates, OR-ed
+// into candWW / candWB
 __global__ void k_xr_pl(const Mat *Mp, const uint64_t *subwinW, const uint64_t *subwinB,
+                        const uint64_t *sublossW, const uint64_t *sublossB,
                         const uint64_t *winsW, const uint64_t *winsB, uint64_t *xrW,
-                        uint64_t *xrB, int s0, int ns, unsigned long long *nopen);
+                        uint64_t *xrB, uint64_t *candWW, uint64_t *candWB, int s0, int ns,
+                        unsigned long long *nopen);
 __global__ void k_resolve(const Mat *Mp, const SubRef *Rp, int c, int k, uint64_t *loss,
                           uint64_t *bloss, uint64_t *cwin, uint64_t *win, uint64_t *lwin, int s0,
                           int ns);

@@ -586,7 +586,7 @@ __global__ __launch_bounds__(256, 5) void k_uncap_j(JobDev *const *__restrict__ 
       o = stm ^ 1;
       n = uni(Jp->n);
       en = !uni(Jp->done) && ((uni(Jp->mask) >> (kind * 2 + stm)) & 1) &&
-           (n == 1 || n == DRAW_RULE + 1);
+           (n == 1 || n == DRAW_RULE + 1) && !(kind == 1 && n == 1 && uni(Jp->w1cap));
       if (en) {
         cache_mat(Ms, Jp->M, pr.job, &cached);
         scatter = uni(Jp->scatter);
@@ -2297,17 +2297,25 @@ __device__ __forceinline__ uint64_t load_bits(const uint64_t *p, uint64_t pos, i
 // King captures are the verification's (real tables) or the exits' (k_init_pl).
 // nopen[c] counts the positions, c to move, neither in wins (illegal) nor
 // refuted.
+// With sublossW/B (a synthetic job, whose king captures are exits), the same
+// captures' results read from sub[opp][TBG_SUB_LOSS] also give the ply-1 W
+// step's capture candidates (predecessors_sub from the loss plane,
+// src/tb8gen.c:1046-1048, worked out forwards), OR-ed into candWW / candWB
+// in place of that step's uncapture.
 __global__ __launch_bounds__(256, 4) void k_xr_pl(const Mat *__restrict__ Mp,
                                                   const uint64_t *__restrict__ subwinW,
                                                   const uint64_t *__restrict__ subwinB,
+                                                  const uint64_t *__restrict__ sublossW,
+                                                  const uint64_t *__restrict__ sublossB,
                                                   const uint64_t *__restrict__ winsW,
                                                   const uint64_t *__restrict__ winsB,
-                                                  uint64_t *xrW, uint64_t *xrB, int s0, int ns,
+                                                  uint64_t *xrW, uint64_t *xrB, uint64_t *candWW,
+                                                  uint64_t *candWB, int s0, int ns,
                                                   unsigned long long *nopen)
 {
   __shared__ Tabs T;
   __shared__ Mat Ms;
-  __shared__ uint64_t buf[4][2][IB_WORDS];
+  __shared__ uint64_t buf[4][4][IB_WORDS];
   load_tabs(T);
   {
     const uint32_t *a = reinterpret_cast<const uint32_t *>(Mp);
@@ -2336,10 +2344,15 @@ __global__ __launch_bounds__(256, 4) void k_xr_pl(const Mat *__restrict__ Mp,
     const uint64_t F = ~occF;
     uint64_t mv[6];
     rank_masks(F, mv);
-    uint64_t rr[2] = { 0, 0 };
+    // per side (named registers, not side-indexed arrays, which would live
+    // in scratch memory): refuted; W1 capture candidates
+    uint64_t rr0 = 0, rr1 = 0, rw0 = 0, rw1 = 0;
+    const bool w1 = sublossW != nullptr;  // uniform
     for (int c = 0; c < 2 && valid; c++) {
       int o = c ^ 1;
       const uint64_t *subS = (o == WHITE ? subwinW : subwinB) + phys(M, s) * M.ctotal[o];
+      const uint64_t *subL = w1 ? (o == WHITE ? sublossW : sublossB) + phys(M, s) * M.ctotal[o]
+                                : nullptr;
       uint64_t opp = 0;
       for (int i = 0; i < MAXP - 1; i++) {
         if (i >= M.nk[o]) break;
@@ -2347,6 +2360,7 @@ __global__ __launch_bounds__(256, 4) void k_xr_pl(const Mat *__restrict__ Mp,
         if (j != jx) opp |= bit(sq_get(p, j));
       }
       uint64_t rk = 0, rsq = 0;  // refuted: rank domain, square domain
+      uint64_t wk = 0, wsq = 0;  // a capture into a loss of o: the same
       for (int i = 0; i < MAXP - 1; i++) {
         if (i >= M.nk[c]) break;
         int j = M.pcs[c][i];
@@ -2367,6 +2381,10 @@ __global__ __launch_bounds__(256, 4) void k_xr_pl(const Mat *__restrict__ Mp,
           uint64_t btw = between_sq(sj, d) & F;
           uint64_t ok = btw ? fullR & ~compress_f(btw, F, mv) : fullR;
           rk |= ok & ~segF;
+          if (w1) {
+            uint64_t sl = load_bits(subL + M.coff[l], base, R + 1);
+            wk |= ok & ((sl & lowm) | ((sl >> 1) & ~lowm));
+          }
         }
         if (cx == o) {  // the last set's piece taken on d, t = d
           uint64_t dd = a0 & F;
@@ -2376,6 +2394,7 @@ __global__ __launch_bounds__(256, 4) void k_xr_pl(const Mat *__restrict__ Mp,
             uint64_t ob;
             uint64_t ci = partial_idx(M.capt[L], sq_set(p, j, d), L, -1, &ob);
             if (!bm_test(subS + M.coff[L], ci)) rsq |= bit(d);
+            if (w1 && bm_test(subL + M.coff[L], ci)) wsq |= bit(d);
           }
         }
       }
@@ -2391,19 +2410,33 @@ __global__ __launch_bounds__(256, 4) void k_xr_pl(const Mat *__restrict__ Mp,
           uint64_t ci = partial_idx(M.capt[l], sq_set(p, jx, d), L, l, &ob);
           ci += (uint64_t)(uint32_t)(d - popc64(ob & (bit(d) - 1)));
           if (!bm_test(subS + M.coff[l], ci)) rsq |= from;
+          if (w1 && bm_test(subL + M.coff[l], ci)) wsq |= from;
         }
       }
-      rr[c] = rk | compress_f(rsq, F, mv);
+      uint64_t rrc = rk | compress_f(rsq, F, mv);
+      uint64_t rwc = w1 ? wk | compress_f(wsq, F, mv) : 0;
+      if (c == WHITE) {
+        rr0 = rrc;
+        rw0 = rwc;
+      } else {
+        rr1 = rrc;
+        rw1 = rwc;
+      }
     }
     // the chunk's R words of each plane: read, OR-ed, written back once
     uint64_t wbase = phys(M, s) * M.words + g * (uint64_t)R;
     uint64_t nwc = M.words - g * (uint64_t)R < (uint64_t)R ? M.words - g * (uint64_t)R : (uint64_t)R;
     B[0][lane] = (uint64_t)lane < nwc ? xrW[wbase + lane] : 0;
     B[1][lane] = (uint64_t)lane < nwc ? xrB[wbase + lane] : 0;
+    if (w1) {
+      B[2][lane] = (uint64_t)lane < nwc ? candWW[wbase + lane] : 0;
+      B[3][lane] = (uint64_t)lane < nwc ? candWB[wbase + lane] : 0;
+    }
     wave_sync();
     int off = lane * R, w0 = off >> 6, sh = off & 63;
-    for (int k = 0; k < 2; k++) {
-      uint64_t v = rr[k];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      uint64_t v = k == 0 ? rr0 : k == 1 ? rr1 : k == 2 ? rw0 : rw1;
       if (!v) continue;
       atomicOr(reinterpret_cast<unsigned long long *>(&B[k][w0]), (unsigned long long)(v << sh));
       if (sh && sh + R > 64)
@@ -2414,6 +2447,10 @@ __global__ __launch_bounds__(256, 4) void k_xr_pl(const Mat *__restrict__ Mp,
     if ((uint64_t)lane < nwc) {
       xrW[wbase + lane] = B[0][lane];
       xrB[wbase + lane] = B[1][lane];
+      if (w1) {
+        candWW[wbase + lane] = B[2][lane];
+        candWB[wbase + lane] = B[3][lane];
+      }
       uint64_t b0 = (g * (uint64_t)R + lane) * 64;  // the word's first index in the slice
       uint64_t vm = b0 >= M.ii.size ? 0 : M.ii.size - b0 >= 64 ? ~0ull : bit(M.ii.size - b0) - 1;
       open[0] += popc64(vm & ~B[0][lane] & ~winsW[wbase + lane]);
