@@ -964,11 +964,19 @@ int launch_resolve(tbg_ctx *c)
   int s0 = M.synth ? M.synth_slice : 0, ns = M.synth ? 1 : NSL;
   for (int k = 0; k < M.numsets; k++) {
     int col = M.setcol[k];
-    uint64_t waves = (uint64_t)ns * M.cwords[k];
+    uint64_t words = (uint64_t)ns * M.cwords[k];
     Timed t(c, CAT_RESOLVE);
-    hipLaunchKernelGGL(k_resolve, dim3(cap_grid((waves + 3) / 4, (uint64_t)g_cus * 16)), dim3(256), 0,
-                       c->st, c->dM, c->dR + k, col, k, c->sub[col][0], c->sub[col][1],
-                       c->sub[col][2], c->sub[col][3], c->sub[col][SUB_LWIN], s0, ns);
+    // a word per lane when the words fill the chip four threads deep
+    if (M.all_single && words >= (uint64_t)g_cus * 1024)
+      hipLaunchKernelGGL(k_resolve_w, dim3(cap_grid((words + 255) / 256, (uint64_t)g_cus * 16)),
+                         dim3(256), 0, c->st, c->dM, c->dR + k, col, k, c->sub[col][0],
+                         c->sub[col][1], c->sub[col][2], c->sub[col][3], c->sub[col][SUB_LWIN], s0,
+                         ns);
+    else  // a word per wave
+      hipLaunchKernelGGL(k_resolve, dim3(cap_grid((words + 3) / 4, (uint64_t)g_cus * 16)),
+                         dim3(256), 0, c->st, c->dM, c->dR + k, col, k, c->sub[col][0],
+                         c->sub[col][1], c->sub[col][2], c->sub[col][3], c->sub[col][SUB_LWIN], s0,
+                         ns);
     HIPCHK(hipGetLastError());
   }
   return TBG_OK;

@@ -189,6 +189,11 @@ __global__ void k_xr_pl(const Mat *Mp, const uint64_t *subwinW, const uint64_t *
 __global__ void k_resolve(const Mat *Mp, const SubRef *Rp, int c, int k, uint64_t *loss,
                           uint64_t *bloss, uint64_t *cwin, uint64_t *win, uint64_t *lwin, int s0,
                           int ns);
+// the same for all-single materials, one capture-space word (64 consecutive
+// positions) per lane
+__global__ void k_resolve_w(const Mat *Mp, const SubRef *Rp, int c, int k, uint64_t *loss,
+                            uint64_t *bloss, uint64_t *cwin, uint64_t *win, uint64_t *lwin, int s0,
+                            int ns);
 
 // bitmap of the positions whose code is `code`, one bit per position, for
 // `nslices` slices of `words` words (the reference's W_n / L_n files)

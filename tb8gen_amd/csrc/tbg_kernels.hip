@@ -2521,6 +2521,87 @@ __global__ __launch_bounds__(256) void k_resolve(const Mat *__restrict__ Mp,
   }
 }
 
+// k_resolve for a material whose sets are all single, one capture-space
+// word per lane: its 64 positions are consecutive indices, so the first is
+// decoded in full and each next one is the last set's piece moved to the
+// next free square (the last digit plus one); when there is none the digit
+// carries and the index is decoded in full again, about once per 60
+// positions.  The same classification per position as k_resolve.
+__global__ __launch_bounds__(256) void k_resolve_w(const Mat *__restrict__ Mp,
+                                                    const SubRef *__restrict__ Rp, int c, int k,
+                                                    uint64_t *loss, uint64_t *bloss,
+                                                    uint64_t *cwin, uint64_t *win,
+                                                    uint64_t *lwin, int s0, int ns)
+{
+  __shared__ Tabs T;
+  load_tabs(T);
+  const Mat &M = *Mp;
+  const SubRef &R = *Rp;
+  const Idx &ci = M.capt[k];
+  int m = M.ii.last[k];
+  // the last set with a piece in this capture space, and that piece's slot
+  int jl = 0;
+  for (int j = 0; j < MAXS; j++) {
+    if (j >= ci.numsets) break;
+    if (ci.mult[j] > 0) jl = ci.last[j];
+  }
+  uint64_t cw = M.cwords[k];
+  uint64_t nw = (uint64_t)ns * cw;
+  uint64_t key = synth_key(M.seedmix, c, 0, k);
+  for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < nw;
+       g += (uint64_t)gridDim.x * blockDim.x) {
+    int s = s0 + (int)(g / cw);
+    uint64_t w = g % cw;
+    uint64_t b0 = 0, b1 = 0, b2 = 0, b3 = 0, b4 = 0;
+    uint64_t p = 0, occ = 0;
+    bool have = false;
+    for (int i = 0; i < 64; i++) {
+      uint64_t idx = w * 64 + (uint64_t)i;
+      if (idx >= ci.size) break;
+      int t = sq_get(p, jl);
+      uint64_t f = have ? ~occ & (~1ull << t) : 0;  // free squares above the last piece's
+      if (f) {
+        int t2 = ctz64(f);
+        p = sq_set(p, jl, t2);
+        occ = (occ ^ bit(t)) | bit(t2);
+      } else {
+        p = kings_of(T, s);
+        occ = idx_to_sq(T, ci, idx, &p);
+        have = true;
+      }
+      int v = 0;
+      bool ill = attacked_by(T, M, p, occ, sq_get(p, c ^ 1), c, m);
+      if (!ill) {
+        if (M.synth) {  // synth_hash(M.seedmix, c, 0, k, idx), decided from its high half
+          v = synth_value_k(key, idx);
+        } else if (R.present) {
+          uint64_t tsq = 0;
+          for (int j = 0; j < MAXP; j++) {
+            if (j >= R.num) break;
+            tsq = sq_set(tsq, j, sq_get(p, sq_get(R.map, j)) ^ (R.flip ? 0x38 : 0));
+          }
+          uint64_t q;
+          int s2 = normalize(T, tsq, &q);
+          uint64_t i2 = sq_to_idx(T, R.ii, q);
+          v = code_wdl(R.codes[c ^ R.flip][(uint64_t)s2 * R.ii.size + i2]);
+        }
+      }
+      uint64_t b = bit(i);
+      b0 |= !ill && v == -2 ? b : 0;
+      b1 |= !ill && v == -1 ? b : 0;
+      b2 |= ill || v >= 1 ? b : 0;
+      b3 |= ill || v == 2 ? b : 0;
+      b4 |= !ill && v == 2 ? b : 0;
+    }
+    uint64_t a = phys(M, s) * M.ctotal[c] + M.coff[k] + w;
+    loss[a] = b0;
+    bloss[a] = b1;
+    cwin[a] = b2;
+    win[a] = b3;
+    lwin[a] = b4;
+  }
+}
+
 // ------------------------------------------------------- Syzygy WDL files
 // r-th free square (zero bit of occ) among the squares of `among`
 __device__ __forceinline__ int nth_free(uint64_t occ, uint64_t among, int r)
