@@ -1581,10 +1581,10 @@ __device__ void synth_exits_pos(const Tabs &T, const Mat &M, uint64_t p, uint64_
 // the same answer, found with less work.  uo: the opponent's piece attacks
 // with c's king lifted (a slider's ray passes through the square the king
 // leaves), so a king move's legality, capture or not, is a bit test.  Only
-// if no king move is legal are the other pieces' moves tried, each with a
-// full attack test
-// (pins): under a double check none can be legal, under a single check only
-// moves onto the checker or between it and the king can be.
+// if no king move is legal are the other pieces' moves tried: under a
+// double check none can be legal, under a single check exactly the moves of
+// unpinned pieces onto the checker or between it and the king are, and
+// without a check each move gets a full attack test (pins).
 __device__ bool has_legal(const Tabs &T, const Mat &M, uint64_t p, uint64_t occ, int c, uint64_t uo)
 {
   int o = c ^ 1;
@@ -1623,6 +1623,30 @@ __device__ bool has_legal(const Tabs &T, const Mat &M, uint64_t p, uint64_t occ,
   }
   if (nchk > 1) return false;
   block &= ~bit(ok);  // the enemy king is never captured
+  if (nchk == 1) {
+    // A pinned piece can not answer the check (its pin line and the check's
+    // meet only at the king), and any other piece's move onto the checker
+    // or between it and the king does (it opens no line onto the king): one
+    // attack test per piece instead of a full attack test per move.
+    uint64_t pinned = 0;
+    for (int t = 0; t < MAXP - 1; t++) {
+      if (t >= M.nk[o]) break;
+      int j = M.pcs[o][t];
+      int s = sq_get(p, j), pt = M.pt[j] & 7;
+      bool line = (s >> 3) == (k >> 3) || (s & 7) == (k & 7);
+      if (!(line ? pt == ROOK || pt == QUEEN : pt == BISHOP || pt == QUEEN)) continue;
+      uint64_t on = between_sq(s, k) & occ;  // empty when not aligned
+      if (on && !(on & (on - 1)) && (on & own)) pinned |= on;
+    }
+    for (int t = 0; t < MAXP - 1; t++) {
+      if (t >= M.nk[c]) break;
+      int j = M.pcs[c][t];
+      int from = sq_get(p, j);
+      if (pinned & bit(from)) continue;
+      if (piece_att(T, M.pt[j], from, occ) & ~own & block) return true;
+    }
+    return false;
+  }
   for (int t = 0; t < MAXP - 1; t++) {
     if (t >= M.nk[c]) break;
     int j = M.pcs[c][t];
