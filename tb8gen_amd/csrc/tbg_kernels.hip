@@ -1785,18 +1785,28 @@ __global__ __launch_bounds__(256, 6) void k_init(const Mat *__restrict__ Mp, uin
 __device__ __forceinline__ uint64_t fixed_attacks_on(const Tabs &T, const Mat &M, uint64_t p,
                                                      uint64_t occ, int y, int c, int jx)
 {
-  uint64_t inter = ~0ull;
-  bool any = false;
+  // the attackers found from y (a slider attacks y exactly when y's rays of
+  // its kind reach it): two ray sets instead of one per piece
+  uint64_t rq = 0, bq = 0, nn = 0;  // rooks and queens, bishops and queens, knights
   for (int i = 0; i < MAXP - 1; i++) {
     if (i >= M.nk[c]) break;
     int j = M.pcs[c][i];
     if (j == jx) continue;
-    int s = sq_get(p, j);
-    if (!(piece_att(T, M.pt[j], s, occ) & bit(y))) continue;
-    any = true;
-    inter &= (M.pt[j] & 7) == KNIGHT ? 0ull : between_sq(s, y);
+    int pt = M.pt[j] & 7;
+    uint64_t b = bit(sq_get(p, j));
+    rq |= pt == ROOK || pt == QUEEN ? b : 0;
+    bq |= pt == BISHOP || pt == QUEEN ? b : 0;
+    nn |= pt == KNIGHT ? b : 0;
   }
-  return any ? ~inter : 0ull;
+  uint64_t atk = T.knight[y] & nn;
+  if (rq) atk |= rook_att(y, occ) & rq;
+  if (bq) atk |= bishop_att(y, occ) & bq;
+  uint64_t inter = ~0ull;
+  for (uint64_t b = atk; b; b &= b - 1) {
+    int s = ctz64(b);
+    inter &= (nn >> s) & 1 ? 0ull : between_sq(s, y);
+  }
+  return atk ? ~inter : 0ull;
 }
 
 // The squares t from which the last set's piece (type pt, colour c) would
