@@ -266,7 +266,10 @@ def run_served(args, tbg):
     arrivals of 70% of the batched capacity, and once more under a budget
     that forces preemption to pinned host memory.  The capacity comparison
     is repeated on a mixed stream of 5- and 6-piece jobs (--mixed-pieces),
-    the stream the batched-over-plain target is stated for."""
+    the stream the batched-over-plain target is stated for.  The server runs
+    with --max-active slice slots (32: faster than more, see
+    capacity_resident, the capacity run at --residency-slots, where the
+    budget alone bounds admission and most of the job state is resident)."""
     import numpy as np
     pieces = [int(x) for x in args.served_pieces.split(",")]
     rng = np.random.default_rng(args.seed)
@@ -301,6 +304,16 @@ def run_served(args, tbg):
         st, _, lat = serve([0.0] * len(jobs), form, args.max_active, budget)
         cap[name] = dict(summary(st, lat), jobs_per_s=len(jobs) / (st.wall_ms / 1000.0))
         print(f"capacity {name}: {cap[name]}", file=sys.stderr, flush=True)
+    # the same jobs with enough slots that admission is bound by the budget:
+    # most of the job state resident at once (fewer slots run faster: the
+    # set-ups of hundreds of admitted slices share the device with the batch)
+    resident = None
+    if args.residency_slots:
+        st, _, lat = serve([0.0] * len(jobs), tbg.FORM_BATCHED, args.residency_slots, budget)
+        resident = dict(summary(st, lat), jobs_per_s=len(jobs) / (st.wall_ms / 1000.0),
+                        slots=args.residency_slots)
+        print(f"capacity batched, {args.residency_slots} slots: {resident}", file=sys.stderr,
+              flush=True)
     mixed = {}
     if args.mixed_pieces:
         mp = [int(x) for x in args.mixed_pieces.split(",")]
@@ -338,6 +351,7 @@ def run_served(args, tbg):
                    "seed": args.seed, "lambda_jobs_per_s": lam, "budget_gb": args.budget_gb,
                    "pieces": pieces, "jobs": len(jobs), "job_state_gb": total / 2**30},
         "capacity": cap,
+        "capacity_resident": resident,
         "batched_over_plain": cap["batched"]["value"] / cap["plain"]["value"],
         "latency_ms": {"batched_p50": b["p50_ms"], "batched_p99": b["p99_ms"],
                        "plain_p50": p["p50_ms"], "plain_p99": p["p99_ms"]},
@@ -572,7 +586,11 @@ def main():
     ap.add_argument("--slice", type=int, default=170)
     ap.add_argument("--jobs", type=int, default=1000, help="served: number of jobs, at most")
     ap.add_argument("--budget-gb", type=float, default=64.0, help="served: HBM admission budget")
-    ap.add_argument("--max-active", type=int, default=1024, help="served: slice slots")
+    ap.add_argument("--max-active", type=int, default=32,
+                    help="served: slice slots (also a capacity run at --residency-slots)")
+    ap.add_argument("--residency-slots", type=int, default=1024,
+                    help="served: slots of the extra capacity run that holds most job state "
+                         "resident (0: none)")
     ap.add_argument("--max-slices", type=int, default=8, help="served: king slices per job, at most")
     ap.add_argument("--fill", type=float, default=0.7,
                     help="served: job state drawn up to this fraction of the budget")

@@ -595,7 +595,7 @@ CtxLayout ctx_layout(const Mat &M, uint64_t nsl, int max_ply, bool with_codes)
   L.sub = part_up(SUB_KINDS * nsl * (M.ctotal[0] + M.ctotal[1]) * 8 + 64);
   L.codes = with_codes ? part_up(nsl * M.ii.size * 2) : 0;
   // chunk summaries serve the sparse plies of one-slice (synthetic) jobs
-  L.sum = M.synth && nsl == 1 ? part_up(planes_of(M) * sum_per_slice(M)) : 0;
+  L.sum = M.synth && nsl == 1 && M.words >= (1u << 20) ? part_up(planes_of(M) * sum_per_slice(M)) : 0;
   L.o_sub = L.arena;
   L.o_sum = L.o_sub + L.sub;
   L.o_codes = L.o_sum + L.sum;
