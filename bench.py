@@ -305,8 +305,9 @@ def run_served(args, tbg):
         cap[name] = dict(summary(st, lat), jobs_per_s=len(jobs) / (st.wall_ms / 1000.0))
         print(f"capacity {name}: {cap[name]}", file=sys.stderr, flush=True)
     # the same jobs with enough slots that admission is bound by the budget:
-    # most of the job state resident at once (fewer slots run faster: the
-    # set-ups of hundreds of admitted slices share the device with the batch)
+    # most of the job state resident at once (slower than a few dozen slots,
+    # also when the running batch alone is capped at that: residency itself
+    # costs here)
     resident = None
     if args.residency_slots:
         st, _, lat = serve([0.0] * len(jobs), tbg.FORM_BATCHED, args.residency_slots, budget)
