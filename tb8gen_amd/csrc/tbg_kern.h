@@ -34,7 +34,9 @@ struct JobDev {
   uint64_t *sub[2][SUB_KINDS];
   // static refutations of the L step (k_init_pl + k_xr_pl): a position with a
   // capture or synthetic exit whose result is not a win of the opponent
-  // (within the 50-move rule); nullptr: not worked out, verify every move
+  // (within the 50-move rule), and for a synthetic job on the placement
+  // layout the mates, lost already and never an L_n candidate; nullptr: not
+  // worked out, verify every move
   uint64_t *xr[2];
   // per side: positions neither won (illegal) at init nor statically refuted,
   // what a pull-form L step verifies at most (k_xr_pl counts them)

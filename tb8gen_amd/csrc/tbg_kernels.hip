@@ -1864,7 +1864,7 @@ constexpr int IB_WORDS = 64;  // words per wave of one plane (R <= 62 of them us
 // them in LDS and stores each word once.  Same outputs as k_init, and for
 // a synthetic job with xrW/xrB the exit refutations: positions with a legal
 // king exit whose value is not a win for the side to move after it (such a
-// position is never lost, whatever the ply).
+// position is never lost, whatever the ply), and the mates.
 __global__ __launch_bounds__(256, 4) void k_init_pl(const Mat *__restrict__ Mp, uint64_t *winsW,
                                                  uint64_t *winsB, uint64_t *mateW, uint64_t *mateB,
                                                  uint16_t *codesW, uint16_t *codesB,
@@ -2124,8 +2124,12 @@ __global__ __launch_bounds__(256, 4) void k_init_pl(const Mat *__restrict__ Mp, 
         rx[1] = compress(xsW1);
         rx[2] = compress(xsL0);
         rx[3] = compress(xsL1);
-        rr[0] = compress(xr0);
-        rr[1] = compress(xr1);
+        // the mates join the refutations: lost already, they are never an
+        // L_n candidate (a push-form step never makes them one, having no
+        // legal move into its frontier), and the pull form then leaves
+        // them out at every ply
+        rr[0] = compress(xr0 | MW);
+        rr[1] = compress(xr1 | MB);
       }
     }
     // assemble the chunk's words in LDS (a lane's R bits start at bit
