@@ -2426,11 +2426,24 @@ __global__ __launch_bounds__(256, 4) void k_xr_pl(const Mat *__restrict__ Mp,
         }
         if (cx == o) {  // the last set's piece taken on d, t = d
           uint64_t dd = a0 & F;
+          // the capture-space index with j still on sj, then each d's from it:
+          // j's rank change and the later sets' pieces it passes (no piece of
+          // the last set in this space)
+          uint64_t ob, c0 = 0, occb = 0;
+          uint32_t ra = 0;
+          Later LS = {};
+          if (dd) {
+            int kj = j - 2;  // j's set (all single)
+            c0 = partial_idx(M.capt[L], p, L, -1, &ob);
+            LS = later_sets(M.capt[L], p, kj);
+            LS.n = L - 1 - kj;
+            occb = occ_before(p, kj);
+            ra = (uint32_t)(sj - popc64(occb & (bit(sj) - 1)));
+          }
           while (dd) {
             int d = ctz64(dd);
             dd &= dd - 1;
-            uint64_t ob;
-            uint64_t ci = partial_idx(M.capt[L], sq_set(p, j, d), L, -1, &ob);
+            uint64_t ci = moved_idx_l(LS, c0, sj, d, occb, ra);
             if (!bm_test(subS + M.coff[L], ci)) rsq |= bit(d);
             if (w1 && bm_test(subL + M.coff[L], ci)) wsq |= bit(d);
           }
