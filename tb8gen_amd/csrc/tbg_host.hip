@@ -1485,7 +1485,10 @@ int tbg_resolve_captures(tbg_ctx *c, tbg_ctx *const *subs, int nsubs)
     c->hj.nopen = nullptr;
     if (c->hj.w1cap) {  // the ply-1 W capture candidates too: the uncapture makes them again
       c->hj.w1cap = 0;
-      for (int s = 0; s < 2; s++) HIPCHK(hipMemsetAsync(c->candW[s], 0, c->pw * 8, c->st));
+      // still unconsumed unless a generation has finished since (its sweeps
+      // leave the candidate planes clear)
+      if (!c->generated)
+        for (int s = 0; s < 2; s++) HIPCHK(hipMemsetAsync(c->candW[s], 0, c->pw * 8, c->st));
     }
     if (c->inited) TRY(upload_job(c));
   }
