@@ -315,9 +315,12 @@ __device__ void king_unmoves_wave(const Tabs &T, const Mat &M, int stm, bool ok,
 
 // mark_unmoves (src/tb8gen.c:264-276) of the non-king piece in slot j; the
 // unmoved position is q with slot j replaced.
+// xb (nullable): a second plane of targets to leave out, tested only for
+// those not in wb
 __device__ __forceinline__ void mark_unmoves(const Tabs &T, const Mat &M, int j, uint64_t occ,
                                              uint64_t q, uint64_t *base,
-                                             const uint64_t *__restrict__ wb)
+                                             const uint64_t *__restrict__ wb,
+                                             const uint64_t *__restrict__ xb = nullptr)
 {
   uint64_t b = piece_att(T, M.pt[j], sq_get(q, j), occ) & ~occ;
   while (b) {
@@ -325,7 +328,9 @@ __device__ __forceinline__ void mark_unmoves(const Tabs &T, const Mat &M, int j,
     b &= b - 1;
     uint64_t q2 = sq_set(q, j, t);
     uint64_t i2 = M.all_single ? sq_to_idx_1(M.ii, q2) : sq_to_idx(T, M.ii, q2);
-    if (!bm_test(wb, i2)) atomic_or64(base + (i2 >> 6), bit(i2 & 63));
+    bool skip = bm_test(wb, i2);
+    if (!skip && xb) skip = bm_test(xb, i2);
+    if (!skip) atomic_or64(base + (i2 >> 6), bit(i2 & 63));
   }
 }
 
@@ -634,6 +639,11 @@ __global__ __launch_bounds__(256, 5) void k_uncap_j(JobDev *const *__restrict__ 
     const uint64_t *won = J.wins[stm];
     uint64_t *own = cand + phys(M, s) * M.words;
     const uint64_t *ownw = won + phys(M, s) * M.words;
+    // the L step's candidates up to ply 100 skip the statically refuted
+    // positions too (the verification would drop them): at ply 1 nearly all
+    // legal ones, so most device atomics are saved for one more read
+    const uint64_t *oxr =
+        kind == 0 && n <= DRAW_RULE && J.xr[stm] ? J.xr[stm] + phys(M, s) * M.words : nullptr;
     uint64_t nbw = (M.ii.size + 63) >> 6;
     bool lds = (scatter & TBG_SCATTER_BLOCK) && nbw <= lds_words;  // uniform
     uint64_t *B = dyn;
@@ -672,7 +682,7 @@ __global__ __launch_bounds__(256, 5) void k_uncap_j(JobDev *const *__restrict__ 
                                     __HIP_MEMORY_SCOPE_WORKGROUP);
             }
           } else {
-            mark_unmoves(T, M, j, occ, q, own, ownw);
+            mark_unmoves(T, M, j, occ, q, own, ownw, oxr);
           }
         }
       }
