@@ -1021,6 +1021,15 @@ __device__ bool has_hit(const Tabs &T, const Mat &M, int stm, int s, uint64_t id
 // L_{n-1} or L_{n-2}, and one that passes the verification must also reach
 // the frontier (has_hit).  The same L_n: a position is a predecessor of the
 // frontier exactly when one of its quiet moves reaches it.
+// A job's counter is one address for the whole grid: each wave adds its
+// count once per (job, side) pair it worked on, not once per unit.
+__device__ __forceinline__ void flush_count(unsigned long long *ctr, uint64_t &v, int lane)
+{
+  uint64_t t = __ballot(v != 0) ? wave_sum64(v) : 0;
+  if (lane == 0 && t) atomicAdd(ctr, (unsigned long long)t);
+  v = 0;
+}
+
 template <bool SPLIT, bool SUM>
 __global__ __launch_bounds__(256, SPLIT ? 4 : 5) void k_verify_j(JobDev *const *__restrict__ jobs,
                                                    const Pair *__restrict__ pairs,
@@ -1057,8 +1066,11 @@ __global__ __launch_bounds__(256, SPLIT ? 4 : 5) void k_verify_j(JobDev *const *
   // holding candidates or old L bits are read and written
   uint8_t *csm = nullptr, *lsm = nullptr;
   bool sp = false;
+  uint64_t mysum = 0;  // L positions found for the current pair
+  unsigned long long *ctr = nullptr;
   for (uint64_t u = blockIdx.x; u < total; u += gridDim.x) {
     if (u < plo || u >= phi) {  // uniform across the work-group
+      if (ctr) flush_count(ctr, mysum, lane);
       int pi = find_pair(P, np, u);
       plo = P[pi];
       phi = P[pi + 1];
@@ -1090,6 +1102,7 @@ __global__ __launch_bounds__(256, SPLIT ? 4 : 5) void k_verify_j(JobDev *const *
         lsm = has ? uniptr(plane_sum(*Jp, newL)) : nullptr;
         sp = has && uni(Jp->sparse) != 0;
       }
+      ctr = skip ? nullptr : &Jp->cnt[stm];
     }
     if (skip) continue;
     const Mat &M = Ms;
@@ -1098,7 +1111,6 @@ __global__ __launch_bounds__(256, SPLIT ? 4 : 5) void k_verify_j(JobDev *const *
     int s = unit_slice(M, sw0, u - plo, (M.words + uwe - 1) / uwe, &within);
     uint64_t u0 = within * (uint64_t)uwe;
     uint64_t uend = u0 + uwe < M.words ? u0 + uwe : M.words;
-    uint64_t mysum = 0;
     uint64_t hidx = ~0ull, hD = 0, hp = 0;  // the lane's last decoded candidate of the unit
     // A large unit without candidates (and exits) only has its L words
     // cleared.  Every thread's candidate loads go out together; a unit with
@@ -1276,9 +1288,8 @@ __global__ __launch_bounds__(256, SPLIT ? 4 : 5) void k_verify_j(JobDev *const *
       }
       mysum += popc64(r);
     }
-    uint64_t sum = __ballot(mysum != 0) ? wave_sum64(mysum) : 0;
-    if (lane == 0 && sum) atomicAdd(&Jp->cnt[stm], (unsigned long long)sum);
   }
+  if (ctr) flush_count(ctr, mysum, lane);
 }
 
 template __global__ void k_verify_j<false, false>(JobDev *const *, const Pair *, const uint64_t *,
@@ -1319,8 +1330,11 @@ __global__ __launch_bounds__(256) void k_sweep_j(JobDev *const *__restrict__ job
   // or old W bits are read and written
   uint8_t *csm = nullptr, *wsm = nullptr;
   bool sp = false;
+  uint64_t mycnt = 0;  // W positions found for the current pair
+  unsigned long long *ctr = nullptr;
   for (uint64_t u = blockIdx.x; u < total; u += gridDim.x) {
     if (u < plo || u >= phi) {  // uniform across the work-group
+      if (ctr) flush_count(ctr, mycnt, lane);
       int pi = find_pair(P, np, u);
       plo = P[pi];
       phi = P[pi + 1];
@@ -1343,6 +1357,7 @@ __global__ __launch_bounds__(256) void k_sweep_j(JobDev *const *__restrict__ job
         wsm = has ? uniptr(plane_sum(*Jp, newW)) : nullptr;
         sp = has && uni(Jp->sparse) != 0;
       }
+      ctr = skip ? nullptr : &Jp->cnt[2 + stm];
     }
     if (skip) continue;
     const Mat &M = Ms;
@@ -1350,61 +1365,75 @@ __global__ __launch_bounds__(256) void k_sweep_j(JobDev *const *__restrict__ job
     int s = unit_slice(M, sw0, u - plo, (M.words + uw - 1) / uw, &within);
     uint64_t base = phys(M, s) * M.words;
     uint64_t wend = (within + 1) * uw < M.words ? (within + 1) * uw : M.words;
-    uint64_t mycnt = 0;
-    // each pass of a wave covers two 64-word chunks: lanes 0-31 the first
-    for (uint64_t w = within * uw + threadIdx.x * 2; w < wend; w += 512) {
-      uint64_t a = base + w;
-      uint64_t ch = w >> 6;
-      // sparse: a chunk without candidates or old W bits is left alone
-      bool need = !sp || (csm[ch] | wsm[ch]) != 0;
-      if (sp && !__ballot(need)) continue;
-      ulonglong2 c = make_ulonglong2(0, 0);
-      if (need) c = *reinterpret_cast<const ulonglong2 *>(cand + a);
-      ulonglong2 cc = c;
-      if (exits) {
-        ulonglong2 e = *reinterpret_cast<const ulonglong2 *>(kex + a);
-        cc.x |= e.x;
-        cc.y |= e.y;
-      }
-      if (!run) cc = make_ulonglong2(0, 0);
-      ulonglong2 wn = make_ulonglong2(0, 0);
-      if (cc.x | cc.y) {
-        ulonglong2 wi = *reinterpret_cast<const ulonglong2 *>(wins + a);
-        wn.x = cc.x & ~wi.x;
-        wn.y = cc.y & ~wi.y;
-        if (wn.x | wn.y) {
-          ulonglong2 o2;
-          o2.x = wi.x | wn.x;
-          o2.y = wi.y | wn.y;
-          *reinterpret_cast<ulonglong2 *>(wins + a) = o2;
+    // each pass of a wave covers two 64-word chunks: lanes 0-31 the first;
+    // two passes at a time, so that both passes' loads are in flight together
+    for (uint64_t w0 = within * uw + threadIdx.x * 2; w0 < wend; w0 += 1024) {
+      ulonglong2 c[2], cc[2], wi[2];
+      bool in[2], need[2];
+#pragma unroll
+      for (int k = 0; k < 2; k++) {
+        uint64_t w = w0 + k * 512;
+        in[k] = w < wend;
+        // sparse: a chunk without candidates or old W bits is left alone
+        need[k] = in[k] && (!sp || (csm[w >> 6] | wsm[w >> 6]) != 0);
+        c[k] = make_ulonglong2(0, 0);
+        if (need[k]) c[k] = *reinterpret_cast<const ulonglong2 *>(cand + base + w);
+        cc[k] = c[k];
+        if (exits && in[k]) {
+          ulonglong2 e = *reinterpret_cast<const ulonglong2 *>(kex + base + w);
+          cc[k].x |= e.x;
+          cc[k].y |= e.y;
         }
+        if (!run) cc[k] = make_ulonglong2(0, 0);
       }
-      if (need) *reinterpret_cast<ulonglong2 *>(newW + a) = wn;
-      if (c.x | c.y) *reinterpret_cast<ulonglong2 *>(cand + a) = make_ulonglong2(0, 0);
-      if (wsm) {  // each half-wave's chunk: W bits or none; candidates consumed
-        uint64_t nzb = __ballot((wn.x | wn.y) != 0);
-        if ((lane & 31) == 0 && need) {
-          wsm[ch] = ((nzb >> (lane & 32)) & 0xffffffffull) != 0;
-          csm[ch] = 0;
+#pragma unroll
+      for (int k = 0; k < 2; k++) {
+        wi[k] = make_ulonglong2(0, 0);
+        if (cc[k].x | cc[k].y) wi[k] = *reinterpret_cast<const ulonglong2 *>(wins + base + w0 + k * 512);
+      }
+#pragma unroll
+      for (int k = 0; k < 2; k++) {
+        uint64_t w = w0 + k * 512;
+        if (sp && !__ballot(need[k])) continue;
+        uint64_t a = base + w;
+        uint64_t ch = w >> 6;
+        ulonglong2 wn = make_ulonglong2(0, 0);
+        if (cc[k].x | cc[k].y) {
+          wn.x = cc[k].x & ~wi[k].x;
+          wn.y = cc[k].y & ~wi[k].y;
+          if (wn.x | wn.y) {
+            ulonglong2 o2;
+            o2.x = wi[k].x | wn.x;
+            o2.y = wi[k].y | wn.y;
+            *reinterpret_cast<ulonglong2 *>(wins + a) = o2;
+          }
         }
-      }
-      mycnt += popc64(wn.x) + popc64(wn.y);
-      if (codes && (wn.x | wn.y)) {
-        uint16_t code = (uint16_t)((n << 2) | 1);
-        for (int h = 0; h < 2; h++) {
-          uint64_t r = h ? wn.y : wn.x;
-          uint16_t *cp = codes + phys(M, s) * M.ii.size + (w + h) * 64;
-          while (r) {
-            int b = ctz64(r);
-            r &= r - 1;
-            cp[b] = code;
+        if (need[k]) *reinterpret_cast<ulonglong2 *>(newW + a) = wn;
+        if (c[k].x | c[k].y) *reinterpret_cast<ulonglong2 *>(cand + a) = make_ulonglong2(0, 0);
+        if (wsm) {  // each half-wave's chunk: W bits or none; candidates consumed
+          uint64_t nzb = __ballot((wn.x | wn.y) != 0);
+          if ((lane & 31) == 0 && need[k]) {
+            wsm[ch] = ((nzb >> (lane & 32)) & 0xffffffffull) != 0;
+            csm[ch] = 0;
+          }
+        }
+        mycnt += popc64(wn.x) + popc64(wn.y);
+        if (codes && (wn.x | wn.y)) {
+          uint16_t code = (uint16_t)((n << 2) | 1);
+          for (int h = 0; h < 2; h++) {
+            uint64_t r = h ? wn.y : wn.x;
+            uint16_t *cp = codes + phys(M, s) * M.ii.size + (w + h) * 64;
+            while (r) {
+              int b = ctz64(r);
+              r &= r - 1;
+              cp[b] = code;
+            }
           }
         }
       }
     }
-    mycnt = __ballot(mycnt != 0) ? wave_sum64(mycnt) : 0;
-    if (lane == 0 && mycnt) atomicAdd(&Jp->cnt[2 + stm], (unsigned long long)mycnt);
   }
+  if (ctr) flush_count(ctr, mycnt, lane);
 }
 
 template __global__ void k_sweep_j<false>(JobDev *const *, const Pair *, const uint64_t *, int, int,
