@@ -1365,6 +1365,12 @@ __global__ __launch_bounds__(256) void k_sweep_j(JobDev *const *__restrict__ job
     int s = unit_slice(M, sw0, u - plo, (M.words + uw - 1) / uw, &within);
     uint64_t base = phys(M, s) * M.words;
     uint64_t wend = (within + 1) * uw < M.words ? (within + 1) * uw : M.words;
+    if (sp) {  // a unit whose chunks hold no candidates and no old W bits is left alone
+      bool any = false;
+      for (uint64_t ch = within * uw / 64 + threadIdx.x; ch < (wend + 63) / 64; ch += 256)
+        any |= (csm[ch] | wsm[ch]) != 0;
+      if (!__syncthreads_or(any)) continue;
+    }
     // each pass of a wave covers two 64-word chunks: lanes 0-31 the first;
     // two passes at a time, so that both passes' loads are in flight together
     for (uint64_t w0 = within * uw + threadIdx.x * 2; w0 < wend; w0 += 1024) {
